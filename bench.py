@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark: incoherent closest-hit Mrays/s on the 1M-triangle trace scene.
+
+Workload (one "step" = one rb_trace launch over one batch):
+  scene   1,000,000 triangles = 750 icospheres (subdivision 3) + 20,000 thin
+          1:50 quads in [-10,10]^3, seed 0, SBVH with the reference defaults
+  rays    2,097,152 incoherent diffuse rays: origins on the hits of a
+          2048x1024 pinhole view, cosine-distributed about the geometric
+          normal, tmin = 1e-3 t  (the headline set); the 2,097,152 coherent
+          primaries and the mixed 4,194,304 set are reported beside it
+  output  (triangle id, t) per ray, closest hit
+Inputs are resident in HBM before timing starts; HIP events on the trace
+stream bracket exactly `steps` launches.  One JSON line goes to stdout.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "incoherent closest-hit Mrays/s @1M tris; batched vs plain; p50/p99 latency"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def time_launches(rb, tracer, d_rays, n, d_hits, any_hit, steps, warmup, stream):
+    for _ in range(warmup):
+        tracer.trace_device(d_rays, n, d_hits, any_hit=any_hit, stream=stream)
+    stream.synchronize()
+    e0, e1 = rb.Event(), rb.Event()
+    e0.record(stream)
+    for _ in range(steps):
+        tracer.trace_device(d_rays, n, d_hits, any_hit=any_hit, stream=stream)
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_ms(e1) / steps
+
+
+def read_traffic(path):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    try:
+        d = {}
+        for line in open(path):
+            if "=" in line and not line.startswith("#"):
+                k, v = line.split("=", 1)
+                d[k.strip()] = v.strip()
+        return float(d["traffic_bytes_per_launch"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
+    ap.add_argument("--extras", type=int, default=1, help="also time coherent / any-hit / mixed sets")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU design is described in DESIGN.md, no figure at several GPUs is produced)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import renderbox2_amd as rb
+    from renderbox2_amd import _lib as L
+
+    rb.device_init(0)
+    t0 = time.time()
+    scene = rb.scenes.trace_1m(seed=0)
+    bvh = rb.scenes.build(scene)
+    st = bvh.stats()
+    build_s = time.time() - t0
+    log(f"scene {scene.get_num_triangles()} tris, SBVH {st['n_inner']} inner / {st['n_tri_refs']} refs "
+        f"in {build_s:.1f}s")
+    tracer = rb.CudaTracer(bvh, scene)
+    cam = rb.scenes.trace_camera(2048, 1024)
+    n = cam.get_total_pixels()
+    stream = rb.Stream()
+    d_prim = rb.DeviceBuffer(n * 32)
+    d_diff = rb.DeviceBuffer(n * 32)
+    d_mixed = rb.DeviceBuffer(2 * n * 32)
+    d_hits = rb.DeviceBuffer(2 * n * 8)
+    cam.generate_rays_device(d_prim, spp=1, seed=0, jitter=True, stream=stream)
+    tracer.trace_device(d_prim, n, d_hits, stream=stream)
+    rb.raygen_diffuse_device(tracer, d_prim, d_hits, n, n, 7, d_diff, stream=stream)
+    L.check(L.lib().rb_memcpy_dtod(d_mixed.ptr, d_prim.ptr, n * 32, stream.handle))
+    L.check(L.lib().rb_memcpy_dtod(d_mixed.ptr + n * 32, d_diff.ptr, n * 32, stream.handle))
+    stream.synchronize()
+    cfg = rb.get_trace_config()
+    log("trace config", cfg)
+
+    ms = time_launches(rb, tracer, d_diff, n, d_hits, False, args.steps, args.warmup, stream)
+    value = n / (ms * 1e-3) / 1e6
+    log(f"incoherent closest: {ms:.3f} ms/launch, {value:.1f} Mrays/s")
+    hits = d_hits.download(rb.HIT_DTYPE, n)
+    hit_frac = float((hits["id"] >= 0).mean())
+
+    extra = {}
+    if args.extras:
+        for name, buf, cnt, anyh in (("coherent_closest", d_prim, n, False), ("incoherent_any", d_diff, n, True),
+                                     ("mixed_closest", d_mixed, 2 * n, False)):
+            m = time_launches(rb, tracer, buf, cnt, d_hits, anyh, max(3, args.steps // 2), 1, stream)
+            extra[name + "_mrays"] = round(cnt / (m * 1e-3) / 1e6, 1)
+            log(f"{name}: {m:.3f} ms/launch, {extra[name + '_mrays']} Mrays/s")
+
+    # device-side traversal counters (this kernel's own visit order)
+    ids, _, cnt = tracer.trace_counts(d_diff.download(np.float32, n * 8).reshape(n, 8)[: 1 << 18])
+    gpu_bpr = float((40 + 64 * cnt[:, 0] + 48 * cnt[:, 1] + 16 * cnt[:, 2] + 4 * (ids >= 0)).mean())
+
+    # CPU baseline (the oracle's restatement of the reference traversal) and
+    # the algorithmic bytes per ray it counts on the reference visit order.
+    cpu = None
+    bytes_per_ray = None
+    if args.cpu_seconds > 0:
+        from oracle import oracle
+        cores = min(os.cpu_count() or 1, 16)
+        rays = d_diff.download(np.float32, n * 8).reshape(n, 8)
+        cn, cw, cx = bvh.export(L.RB_LAYOUT_REF_COMPACT2)
+        cn, cw = cn.reshape(-1, 4), cw.reshape(-1, 4)
+        probe = rays[: 1 << 15]
+        t1 = time.time()
+        oracle.trace_compact2(cn, cw, cx, probe, nthreads=cores)
+        per_ray = (time.time() - t1) / len(probe)
+        size = int(min(n, max(1 << 16, args.cpu_seconds / 3.0 / max(per_ray, 1e-9))))
+        sample = rays[:size]
+        times = []
+        for _ in range(3):
+            t1 = time.time()
+            oid, ot, ocnt = oracle.trace_compact2(cn, cw, cx, sample, counts=True, nthreads=cores)
+            times.append(time.time() - t1)
+        tcpu = sorted(times)[1]
+        bytes_per_ray = float(oracle.bytes_per_ray(ocnt, oid >= 0).mean())
+        same = bool(np.array_equal(oid, hits["id"][:size]) and
+                    np.array_equal(ot.view(np.uint32), hits["t"][:size].view(np.uint32)))
+        cpu = {"value": round(size / tcpu / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
+               "sample": f"first {size} of the {n} incoherent rays, closest hit, median of 3 "
+                         f"(OpenMP, {cpu_model()})",
+               "matches_gpu": same}
+        log(f"cpu baseline: {cpu['value']} Mrays/s on {cores} cores, B/ray {bytes_per_ray:.0f}, match={same}")
+    if bytes_per_ray is None:
+        bytes_per_ray = gpu_bpr
+
+    achieved = bytes_per_ray * n / (ms * 1e-3) / 1e9
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "trace_incoherent_1m_pmc.txt"))
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "trace_1m incoherent closest-hit: 1,000,000 tris (750 icospheres s3 + "
+                               "20,000 thin quads), 2,097,152 diffuse rays from 2048x1024 primary hits",
+                   "tris": scene.get_num_triangles(), "rays": n, "hit_fraction": round(hit_frac, 4),
+                   "bvh_inner_nodes": st["n_inner"], "bvh_tri_refs": st["n_tri_refs"],
+                   "bvh_build_s": round(build_s, 1), "trace_config": cfg},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_ray": round(bytes_per_ray, 1), "gpu_visit_bytes_per_ray": round(gpu_bpr, 1)},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

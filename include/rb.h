@@ -1,0 +1,279 @@
+/*
+ * rb.h -- C-ABI of the MI355X ray-tracing core (librb.so).
+ *
+ * The drop-in boundary for renderbox2's hot path.  Plain pointers and sizes;
+ * device pointers are HIP device allocations; `stream` arguments are
+ * hipStream_t values passed as void* (NULL = the library's default stream).
+ * Every call returns an rb_status; rb_last_error() gives the message of the
+ * last failing call on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to renderbox2/ in the
+ * reference tree) are cited on each declaration.
+ */
+#ifndef RB_H
+#define RB_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status ------------------------------------------------------------ */
+
+/* Replaces checkCuda's print-and-assert (src/core/defs.h:55-63) and the
+ * constant 0.0f of CudaTracer::trace (src/accelerators/sbvh/cudatracer.cu:182). */
+typedef enum rb_status {
+    RB_OK = 0,
+    RB_ERR_INVALID = 1,     /* bad argument / handle / size */
+    RB_ERR_OOM = 2,         /* device or host allocation failed, or over budget */
+    RB_ERR_DEVICE = 3,      /* HIP runtime error */
+    RB_ERR_BUSY = 4,        /* resource in use (e.g. scene with queued requests) */
+    RB_ERR_UNSUPPORTED = 5, /* feature not built / not available */
+    RB_ERR_NOT_READY = 6    /* asynchronous result not complete yet */
+} rb_status;
+
+size_t rb_last_error(char* buf, size_t n);
+const char* rb_status_string(rb_status s);
+const char* rb_version(void);
+
+/* ---- records ----------------------------------------------------------- */
+
+/* Ray, src/core/primitives.h:145-208: float4(o.xyz, tmin), float4(d.xyz, tmax). */
+typedef struct rb_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} rb_ray;
+
+/* Lean trace result: the (triangle id, t) pair the reference's
+ * STORE_RESULT writes (src/accelerators/sbvh/cudatracerkernels.h:180).
+ * id = original triangle index, -1 on a miss (t = tmax then). */
+typedef struct rb_hit {
+    int32_t id;
+    float t;
+} rb_hit;
+
+/* Intersection record, the fields of IntersectionBuffer
+ * (src/core/intersection.h:63-80) that the tracer writes, plus t.  SoA,
+ * device pointers, each of n entries; NULL fields are skipped. */
+typedef struct rb_isect_soa {
+    float* position[3];
+    float* geometric_normal[3];
+    float* shading_normal[3];
+    float* incoming_direction[3];
+    float* uv;                /* 2 floats per entry */
+    uint32_t* primitive_id;
+    uint32_t* material_id;
+    uint8_t* front;
+    uint8_t* intersected;
+    float* epsilon;
+    float* t;
+} rb_isect_soa;
+
+/* ---- device / memory / streams ------------------------------------------ */
+
+/* MemoryAllocator (src/memory/memoryallocator.h:82-86) equivalent plus the
+ * HBM budget that admission control reads. */
+rb_status rb_device_init(int device);
+rb_status rb_device_name(char* buf, size_t n, uint32_t* num_cus);
+rb_status rb_malloc(void** ptr, size_t bytes);
+rb_status rb_free(void* ptr);
+rb_status rb_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream);
+rb_status rb_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
+rb_status rb_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
+rb_status rb_memset(void* dst, int value, size_t bytes, void* stream);
+rb_status rb_mem_info(size_t* free_bytes, size_t* total_bytes, size_t* library_bytes);
+rb_status rb_stream_create(void** stream);
+rb_status rb_stream_destroy(void* stream);
+rb_status rb_stream_synchronize(void* stream);
+rb_status rb_event_create(void** ev);
+rb_status rb_event_destroy(void* ev);
+rb_status rb_event_record(void* ev, void* stream);
+rb_status rb_event_elapsed_ms(void* start, void* stop, float* ms);
+rb_status rb_event_synchronize(void* ev);
+
+/* ---- host scene (Scene, src/core/scene.h:55-199) --------------------------- */
+
+typedef struct rb_scene_s* rb_scene_t;
+
+/* Material: the Lambertian / mirror / diffuse-emitter subset of
+ * Material + *BsdfParams (src/core/material.h, src/core/params.h). */
+enum { RB_MAT_LAMBERT = 0, RB_MAT_MIRROR = 1, RB_MAT_EMITTER = 2 };
+typedef struct rb_material {
+    uint32_t kind;
+    float color[3]; /* albedo, mirror reflectance or emitted radiance */
+} rb_material;
+
+rb_status rb_scene_create(rb_scene_t* out);
+rb_status rb_scene_destroy(rb_scene_t s);
+/* Scene::add_material (src/core/scene.h:133) */
+rb_status rb_scene_add_material(rb_scene_t s, const rb_material* m, uint32_t* id);
+/* Scene::add_mesh (src/core/scene.cpp:83-123): triangles index the mesh's
+ * own vertices; the scene offsets them.  Emissive meshes register each
+ * triangle as a DiffuseAreaLight (src/core/scene.cpp:100-118). */
+rb_status rb_scene_add_mesh(rb_scene_t s, const float* verts, uint32_t nv, const uint32_t* idx,
+                            uint32_t nt, uint32_t material_id);
+rb_status rb_scene_counts(rb_scene_t s, uint32_t* nv, uint32_t* nt, uint32_t* nlights,
+                          uint32_t* nmaterials);
+rb_status rb_scene_geometry(rb_scene_t s, float* verts, uint32_t* idx, uint32_t* tri_material,
+                            uint32_t* light_tris);
+rb_status rb_scene_materials(rb_scene_t s, rb_material* out);
+rb_status rb_scene_bounds(rb_scene_t s, float lo[3], float hi[3]);
+
+/* Seeded synthetic scenes (the configurations the benchmarks run).  No
+ * reference counterpart: the reference reads OBJ/XML scenes
+ * (src/io/customfilereader.cpp:47-309). */
+enum {
+    RB_GEN_ICOSPHERES = 0, /* count spheres, subdivision `subdiv`, radius U[r0,r1], centres U[c0,c1]^3 */
+    RB_GEN_THIN_QUADS = 1, /* count quads, aspect 1:50, random orientation, centres U[c0,c1]^3 */
+    RB_GEN_ROOM = 2,       /* closed box [c0,c1]^3 of 2048 triangles facing inward */
+    RB_GEN_EMITTER_QUADS = 3 /* count axis-aligned emissive quads of area `r0` near the ceiling */
+};
+typedef struct rb_gen_params {
+    uint32_t kind;
+    uint32_t count;
+    uint32_t subdiv;
+    uint64_t seed;
+    float r0, r1;   /* radius range, quad length range, or emitter area */
+    float c0, c1;   /* placement range */
+    int32_t material_id;       /* >= 0: every triangle gets it */
+    uint32_t mix_materials;    /* 1: per-object 70% Lambert U[0.2,0.8] / 30% mirror */
+    float emit[3];             /* emitter radiance for RB_GEN_EMITTER_QUADS */
+    uint32_t subdiv_hi;        /* >subdiv: a seeded share `hi_count` of spheres uses it */
+    uint32_t hi_count;
+} rb_gen_params;
+rb_status rb_scene_generate(rb_scene_t s, const rb_gen_params* p);
+
+/* ---- SBVH (BVH / SplitBVHBuilder / CudaBvh) --------------------------------- */
+
+typedef struct rb_host_bvh_s* rb_host_bvh_t;
+typedef struct rb_bvh_s* rb_bvh_t;
+
+/* Platform + BVH::BuildParams (src/accelerators/sbvh/platform.h:78-135,
+ * src/accelerators/sbvh/bvh.h:100-113). Defaults: min_leaf 1,
+ * max_leaf 0x7FFFFFF, split_alpha 1e-5, node/tri cost 1/1. threads 0 = all. */
+typedef struct rb_build_params {
+    uint32_t min_leaf;
+    uint32_t max_leaf;
+    float split_alpha;
+    float node_cost;
+    float tri_cost;
+    uint32_t threads;
+} rb_build_params;
+void rb_build_params_default(rb_build_params* p);
+
+/* BVH::Stats (src/accelerators/sbvh/bvh.h:78-98). */
+typedef struct rb_bvh_stats {
+    float sah;
+    uint32_t n_inner, n_leaf, n_tri_refs, n_child_nodes, max_depth;
+    uint64_t duplicates;
+    double build_ms;
+} rb_bvh_stats;
+
+/* BVH::BVH(Scene*, Platform, BuildParams) (src/accelerators/sbvh/bvh.h:115,
+ * src/accelerators/sbvh/splitbvhbuilder.cpp:177-214). */
+rb_status rb_bvh_build_host(rb_scene_t s, const rb_build_params* p, rb_host_bvh_t* out);
+rb_status rb_bvh_build_arrays(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                              const rb_build_params* p, rb_host_bvh_t* out);
+rb_status rb_bvh_host_stats(rb_host_bvh_t b, rb_bvh_stats* out);
+rb_status rb_bvh_host_release(rb_host_bvh_t b);
+
+/* CudaBvh(const BVH&, BVHLayout) + get_*_buffer
+ * (src/accelerators/sbvh/cudabvh.h:80-97).  REF_COMPACT2 is byte-compatible
+ * with the reference's Compact2 (uint4 rows / uint4 Woop rows with
+ * terminators / uint32 index); MI355X is this library's traversal layout.
+ * Call with NULL buffers to get the sizes in bytes. */
+enum { RB_LAYOUT_REF_COMPACT2 = 1, RB_LAYOUT_MI355X = 2 };
+rb_status rb_bvh_export(rb_host_bvh_t b, uint32_t layout, void* nodes, size_t* nodes_bytes,
+                        void* tris, size_t* tris_bytes, void* index, size_t* index_bytes);
+
+/* CudaBvh::serialize / CudaBvh(filename) (src/accelerators/sbvh/cudabvh.cpp:108-214):
+ * the reference's text format, read and written for compatibility. */
+rb_status rb_bvh_save_text(rb_host_bvh_t b, const char* path);
+rb_status rb_bvh_load_text_compact2(const char* path, void* nodes, size_t* nodes_bytes,
+                                    void* tris, size_t* tris_bytes, void* index,
+                                    size_t* index_bytes);
+
+/* CudaTracer(const CudaBvh&, MemoryAllocator&) (src/accelerators/sbvh/cudatracer.cu:43-127)
+ * plus Scene::gpu_prepare (src/core/scene.cpp:166-301): uploads the MI355X
+ * layout and the scene's shading data (vertices, materials, lights).
+ * `scene` may be NULL for trace-only use. */
+rb_status rb_bvh_upload(rb_host_bvh_t b, rb_scene_t scene, rb_bvh_t* out);
+rb_status rb_bvh_release(rb_bvh_t b);
+rb_status rb_bvh_device_bytes(rb_bvh_t b, size_t* bytes);
+
+/* ---- trace ---------------------------------------------------------------- */
+
+enum {
+    RB_CLOSEST = 0,
+    RB_ANY = 1,       /* terminate on the first hit (occlusion) */
+    RB_MASKED = 2     /* mask[i] == 0 skips ray i (writes a miss) */
+};
+
+/* CudaTracer::trace(rb, isects, scene, stencil, any_hit)
+ * (src/accelerators/sbvh/cudatracer.h:87, cudatracer.cu:134-183), lean
+ * output form.  rays/hits/mask are device pointers; asynchronous on
+ * `stream`.  n == 0 is a no-op. */
+rb_status rb_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
+                   const uint8_t* mask, rb_hit* hits, void* stream);
+
+/* The same call writing the reference IntersectionBuffer fields (+ t), the
+ * result-write block of kernel_kepler_trace (kepler_dynamic_fetch.h:312-360).
+ * Needs a bvh uploaded with its scene. `hits` may be NULL (scratch used). */
+rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
+                        const uint8_t* mask, rb_hit* hits, const rb_isect_soa* out,
+                        void* stream);
+
+/* Traversal tuning: the knobs the autotuner searches (the reference
+ * hard-codes STACK_SIZE 64 / DYNAMIC_FETCH_THRESHOLD 20 / 720 warps of
+ * (32,4), kepler_dynamic_fetch.h:62-63 and cudatracer.cu:170-171). */
+typedef struct rb_trace_config {
+    uint32_t block;          /* threads per block: 64, 128, 256 */
+    uint32_t lds_stack;      /* stack entries per lane kept in LDS: 8..24 */
+    uint32_t chunk;          /* rays claimed per work-counter atomic (multiple of 64) */
+    uint32_t fetch_threshold;/* refill when fewer active lanes than this (0..64) */
+    uint32_t waves_per_cu;   /* persistent grid size = CUs x this / (block/64) */
+} rb_trace_config;
+rb_status rb_trace_get_config(rb_trace_config* c);
+rb_status rb_trace_set_config(const rb_trace_config* c);
+/* Per-ray traversal counters of the device kernel (inner nodes, triangles,
+ * leaves) for traffic accounting; counts is device uint32[3*n]. */
+rb_status rb_trace_counts(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
+                          rb_hit* hits, uint32_t* counts, void* stream);
+
+/* ---- ray generation ---------------------------------------------------- */
+
+/* PerspectiveCamera (src/core/camera.h:48-71, camera.cu:30-98). */
+typedef struct rb_camera {
+    float position[3];
+    float lookat[3];
+    float up[3];
+    float hfov_deg;
+    uint32_t width, height;
+} rb_camera;
+
+/* The two matrices PerspectiveCamera::generate_ray applies
+ * (t_raster_to_camera, t_camera_to_world; src/core/camera.cu:30-57,91-98),
+ * row-major 4x4, computed on the host with the reference's transform chain. */
+rb_status rb_camera_matrices(const rb_camera* cam, float raster_to_camera[16], float camera_to_world[16]);
+
+/* kernel_generate_perspective_camera_samples (src/renderers/samplerrenderer.cu:48-96):
+ * spp rays per pixel of the tile [x0,x0+w) x [y0,y0+h), jittered with the
+ * Philox stream (seed, pixel, sample); jitter = 0 puts rays at pixel centres.
+ * rays: device rb_ray[w*h*spp], ordered (pixel-major, sample-minor). */
+rb_status rb_raygen_camera(const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                           uint32_t spp, uint64_t seed, int jitter, rb_ray* rays, void* stream);
+
+/* Incoherent diffuse rays from primary hits: ray i starts at the hit of
+ * primary i (or a Philox-redrawn hit primary when i missed), with a
+ * cosine-distributed direction about the geometric normal facing the
+ * incoming ray and tmin = 1e-3 t.  Needs the bvh's scene. */
+rb_status rb_raygen_diffuse(rb_bvh_t bvh, const rb_ray* primaries, const rb_hit* hits,
+                            uint32_t n_primary, uint32_t n_out, uint64_t seed, rb_ray* out,
+                            void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RB_H */

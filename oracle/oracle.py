@@ -1,0 +1,158 @@
+"""ctypes front-end of the oracle (test infrastructure only).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+The product package (renderbox2_amd) never imports this module.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liborc.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle not built: run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(path)
+        P = C.c_void_p
+        L.orc_invert4x4.argtypes = [P, P]
+        L.orc_woopify.argtypes = [P, P, P, P]
+        L.orc_bvh_build.restype = P
+        L.orc_bvh_build.argtypes = [P, C.c_uint32, P, C.c_uint32, C.c_uint32, C.c_uint32,
+                                    C.c_float, C.c_float, C.c_float]
+        L.orc_bvh_free.argtypes = [P]
+        L.orc_bvh_stats.argtypes = [P, P, P]
+        L.orc_bvh_nodes.restype = C.c_uint32
+        L.orc_bvh_nodes.argtypes = [P, P, P]
+        L.orc_bvh_tri_indices.restype = C.c_uint32
+        L.orc_bvh_tri_indices.argtypes = [P, P]
+        L.orc_bvh_compact2.restype = C.c_int
+        L.orc_bvh_compact2.argtypes = [P, P, P, P, P, P, P]
+        L.orc_trace_compact2.argtypes = [P, P, P, P, C.c_uint32, C.c_int, P, P, P, C.c_int]
+        L.orc_bruteforce_f64.argtypes = [P, P, C.c_uint32, P, C.c_uint32, P, P, P, P, C.c_int]
+        L.orc_philox4x32_10.argtypes = [P, P, P]
+        _LIB = L
+    return _LIB
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def invert4x4(m):
+    m = np.ascontiguousarray(m, dtype=np.float32).reshape(16)
+    out = np.empty(16, np.float32)
+    lib().orc_invert4x4(_p(m), _p(out))
+    return out.reshape(4, 4)
+
+
+def woopify(v0, v1, v2):
+    a = np.ascontiguousarray(v0, np.float32)
+    b = np.ascontiguousarray(v1, np.float32)
+    c = np.ascontiguousarray(v2, np.float32)
+    out = np.empty(12, np.float32)
+    lib().orc_woopify(_p(a), _p(b), _p(c), _p(out))
+    return out.reshape(3, 4)
+
+
+class Bvh:
+    """SplitBVHBuilder restated (sequential)."""
+
+    def __init__(self, verts, idx, min_leaf=1, max_leaf=0x7FFFFFF, split_alpha=1.0e-5,
+                 node_cost=1.0, tri_cost=1.0):
+        self.verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        self.idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        h = lib().orc_bvh_build(_p(self.verts), len(self.verts), _p(self.idx), len(self.idx),
+                                min_leaf, max_leaf, split_alpha, node_cost, tri_cost)
+        if not h:
+            raise MemoryError("oracle SBVH build failed")
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_bvh_free(self._h)
+            self._h = None
+
+    def stats(self):
+        st = np.zeros(6, np.uint64)
+        sah = np.zeros(1, np.float32)
+        lib().orc_bvh_stats(self._h, _p(st), _p(sah))
+        keys = ["n_inner", "n_leaf", "n_tris", "n_child_nodes", "duplicates", "max_depth"]
+        d = {k: int(v) for k, v in zip(keys, st)}
+        d["sah"] = float(sah[0])
+        return d
+
+    def nodes(self):
+        n = lib().orc_bvh_nodes(self._h, None, None)
+        info = np.zeros((n, 5), np.int32)
+        bnd = np.zeros((n, 6), np.float32)
+        k = lib().orc_bvh_nodes(self._h, _p(info), _p(bnd))
+        return info[:k], bnd[:k]
+
+    def tri_indices(self):
+        n = lib().orc_bvh_tri_indices(self._h, None)
+        out = np.zeros(n, np.uint32)
+        lib().orc_bvh_tri_indices(self._h, _p(out))
+        return out
+
+    def compact2(self):
+        """(nodes uint32[R,4], woop uint32[W,4], tri_index uint32[I])"""
+        nr = C.c_size_t()
+        wr = C.c_size_t()
+        ir = C.c_size_t()
+        rc = lib().orc_bvh_compact2(self._h, None, C.byref(nr), None, C.byref(wr), None, C.byref(ir))
+        if rc != 0:
+            raise ValueError("compact2 layout needs an inner root")
+        nodes = np.zeros((nr.value, 4), np.uint32)
+        woop = np.zeros((wr.value, 4), np.uint32)
+        tidx = np.zeros(ir.value, np.uint32)
+        lib().orc_bvh_compact2(self._h, _p(nodes), C.byref(nr), _p(woop), C.byref(wr), _p(tidx), C.byref(ir))
+        return nodes, woop, tidx
+
+
+def trace_compact2(nodes, woop, tidx, rays, any_hit=False, counts=False, nthreads=0):
+    nodes = np.ascontiguousarray(nodes, np.uint32)
+    woop = np.ascontiguousarray(woop, np.uint32)
+    tidx = np.ascontiguousarray(tidx, np.uint32)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(rays)
+    ids = np.empty(n, np.int32)
+    ts = np.empty(n, np.float32)
+    cnt = np.empty((n, 3), np.uint32) if counts else None
+    lib().orc_trace_compact2(_p(nodes), _p(woop), _p(tidx), _p(rays), n, int(any_hit),
+                             _p(ids), _p(ts), _p(cnt) if counts else None, nthreads)
+    return (ids, ts, cnt) if counts else (ids, ts)
+
+
+def bruteforce_f64(verts, idx, rays, nthreads=0):
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(rays)
+    ids = np.empty(n, np.int32)
+    t = np.empty(n, np.float64)
+    margin = np.empty(n, np.float64)
+    gap = np.empty(n, np.float64)
+    lib().orc_bruteforce_f64(_p(verts), _p(idx), len(idx), _p(rays), n, _p(ids), _p(t),
+                             _p(margin), _p(gap), nthreads)
+    return ids, t, margin, gap
+
+
+def philox(ctr, key):
+    c = np.ascontiguousarray(ctr, np.uint32)
+    k = np.ascontiguousarray(key, np.uint32)
+    out = np.empty(4, np.uint32)
+    lib().orc_philox4x32_10(_p(c), _p(k), _p(out))
+    return out
+
+
+def bytes_per_ray(counts, hit):
+    """Algorithmic bytes of one ray on the reference traversal:
+    32 (ray) + 8 (id,t out) + 64*inner + 48*tris + 16*leaf terminators + 4*[hit]."""
+    counts = np.asarray(counts, np.float64)
+    return 40.0 + 64.0 * counts[:, 0] + 48.0 * counts[:, 1] + 16.0 * counts[:, 2] + 4.0 * np.asarray(hit, np.float64)

@@ -1,0 +1,1049 @@
+/*
+ * rb_oracle.c -- CPU restatement of the renderbox2 ray-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg; never by the
+ * product library.  Build: oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp).
+ *
+ * Reference paths below are relative to renderbox2/ in the reference tree.
+ */
+#define _GNU_SOURCE
+#include "rb_oracle.h"
+
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* float3 helpers: the component-wise forms of the CUDA-Samples helper_math  */
+/* operators the reference uses (normalize = v * (1/sqrt(dot(v,v))) on host). */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { float x, y, z; } f3;
+
+static inline f3 mk3(float x, float y, float z) { f3 r = {x, y, z}; return r; }
+static inline f3 sub3(f3 a, f3 b) { return mk3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline f3 add3(f3 a, f3 b) { return mk3(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline f3 mul3s(f3 a, float s) { return mk3(a.x * s, a.y * s, a.z * s); }
+static inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline f3 cross3(f3 a, f3 b)
+{
+    return mk3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline f3 normalize3(f3 v) { return mul3s(v, 1.0f / sqrtf(dot3(v, v))); }
+static inline f3 min3v(f3 a, f3 b) { return mk3(fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)); }
+static inline f3 max3v(f3 a, f3 b) { return mk3(fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)); }
+static inline float get3(f3 v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+static inline void set3(f3* v, int d, float s)
+{
+    if (d == 0) v->x = s; else if (d == 1) v->y = s; else v->z = s;
+}
+
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+
+/* (int)f as x86 cvttss2si computes it (INT_MIN for NaN / out of range); the
+ * reference's spatial binning relies on that conversion
+ * (src/accelerators/sbvh/splitbvhbuilder.cpp:445-446). */
+static inline int trunc_int(float f)
+{
+    if (!(f > -2147483648.0f && f < 2147483648.0f)) return INT_MIN;
+    return (int)f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* AABB, src/core/primitives.h:52-138                                        */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { f3 lo, hi; } aabb;
+
+static inline aabb aabb_empty(void)
+{
+    aabb b; b.lo = mk3(FLT_MAX, FLT_MAX, FLT_MAX); b.hi = mk3(-FLT_MAX, -FLT_MAX, -FLT_MAX);
+    return b;
+}
+static inline int aabb_valid(const aabb* b)
+{
+    return b->lo.x <= b->hi.x && b->lo.y <= b->hi.y && b->lo.z <= b->hi.z;
+}
+static inline void aabb_insert(aabb* b, f3 p) { b->lo = min3v(b->lo, p); b->hi = max3v(b->hi, p); }
+static inline void aabb_grow(aabb* b, const aabb* o) { aabb_insert(b, o->lo); aabb_insert(b, o->hi); }
+static inline void aabb_intersect(aabb* b, const aabb* o) { b->lo = max3v(b->lo, o->lo); b->hi = min3v(b->hi, o->hi); }
+static inline float aabb_area(const aabb* b)
+{
+    if (!aabb_valid(b)) return 0.0f;
+    f3 d = sub3(b->hi, b->lo);
+    return (d.x * d.y + d.y * d.z + d.z * d.x) * 2.0f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Matrix4x4::invert, src/core/transform.h:205-333                           */
+/* ------------------------------------------------------------------------ */
+
+void orc_invert4x4(const float* M, float* out)
+{
+    float inv[16], det;
+#define E(i) M[i]
+    inv[0] = E(5) * E(10) * E(15) - E(5) * E(11) * E(14) - E(9) * E(6) * E(15) + E(9) * E(7) * E(14) + E(13) * E(6) * E(11) - E(13) * E(7) * E(10);
+    inv[4] = -E(4) * E(10) * E(15) + E(4) * E(11) * E(14) + E(8) * E(6) * E(15) - E(8) * E(7) * E(14) - E(12) * E(6) * E(11) + E(12) * E(7) * E(10);
+    inv[8] = E(4) * E(9) * E(15) - E(4) * E(11) * E(13) - E(8) * E(5) * E(15) + E(8) * E(7) * E(13) + E(12) * E(5) * E(11) - E(12) * E(7) * E(9);
+    inv[12] = -E(4) * E(9) * E(14) + E(4) * E(10) * E(13) + E(8) * E(5) * E(14) - E(8) * E(6) * E(13) - E(12) * E(5) * E(10) + E(12) * E(6) * E(9);
+    inv[1] = -E(1) * E(10) * E(15) + E(1) * E(11) * E(14) + E(9) * E(2) * E(15) - E(9) * E(3) * E(14) - E(13) * E(2) * E(11) + E(13) * E(3) * E(10);
+    inv[5] = E(0) * E(10) * E(15) - E(0) * E(11) * E(14) - E(8) * E(2) * E(15) + E(8) * E(3) * E(14) + E(12) * E(2) * E(11) - E(12) * E(3) * E(10);
+    inv[9] = -E(0) * E(9) * E(15) + E(0) * E(11) * E(13) + E(8) * E(1) * E(15) - E(8) * E(3) * E(13) - E(12) * E(1) * E(11) + E(12) * E(3) * E(9);
+    inv[13] = E(0) * E(9) * E(14) - E(0) * E(10) * E(13) - E(8) * E(1) * E(14) + E(8) * E(2) * E(13) + E(12) * E(1) * E(10) - E(12) * E(2) * E(9);
+    inv[2] = E(1) * E(6) * E(15) - E(1) * E(7) * E(14) - E(5) * E(2) * E(15) + E(5) * E(3) * E(14) + E(13) * E(2) * E(7) - E(13) * E(3) * E(6);
+    inv[6] = -E(0) * E(6) * E(15) + E(0) * E(7) * E(14) + E(4) * E(2) * E(15) - E(4) * E(3) * E(14) - E(12) * E(2) * E(7) + E(12) * E(3) * E(6);
+    inv[10] = E(0) * E(5) * E(15) - E(0) * E(7) * E(13) - E(4) * E(1) * E(15) + E(4) * E(3) * E(13) + E(12) * E(1) * E(7) - E(12) * E(3) * E(5);
+    inv[14] = -E(0) * E(5) * E(14) + E(0) * E(6) * E(13) + E(4) * E(1) * E(14) - E(4) * E(2) * E(13) - E(12) * E(1) * E(6) + E(12) * E(2) * E(5);
+    inv[3] = -E(1) * E(6) * E(11) + E(1) * E(7) * E(10) + E(5) * E(2) * E(11) - E(5) * E(3) * E(10) - E(9) * E(2) * E(7) + E(9) * E(3) * E(6);
+    inv[7] = E(0) * E(6) * E(11) - E(0) * E(7) * E(10) - E(4) * E(2) * E(11) + E(4) * E(3) * E(10) + E(8) * E(2) * E(7) - E(8) * E(3) * E(6);
+    inv[11] = -E(0) * E(5) * E(11) + E(0) * E(7) * E(9) + E(4) * E(1) * E(11) - E(4) * E(3) * E(9) - E(8) * E(1) * E(7) + E(8) * E(3) * E(5);
+    inv[15] = E(0) * E(5) * E(10) - E(0) * E(6) * E(9) - E(4) * E(1) * E(10) + E(4) * E(2) * E(9) + E(8) * E(1) * E(6) - E(8) * E(2) * E(5);
+    det = E(0) * inv[0] + E(1) * inv[4] + E(2) * inv[8] + E(3) * inv[12];
+#undef E
+    det = 1.0f / det;
+    for (int i = 0; i < 16; i++) out[i] = inv[i] * det;
+}
+
+/* CudaBvh::woopify_tri, src/accelerators/sbvh/cudabvh.cpp:303-322: the
+ * columns of M are (v0-v2, v1-v2, normalize(cross(v0-v2, v1-v2)), v2) with
+ * w = (0,0,0,1); the rows kept are row 2 of M^-1 (with its w negated), row 0
+ * and row 1. */
+void orc_woopify(const float* a, const float* b, const float* c, float* out)
+{
+    f3 v0 = mk3(a[0], a[1], a[2]), v1 = mk3(b[0], b[1], b[2]), v2 = mk3(c[0], c[1], c[2]);
+    f3 c0 = sub3(v0, v2), c1 = sub3(v1, v2);
+    f3 c2 = normalize3(cross3(sub3(v0, v2), sub3(v1, v2)));
+    float m[16] = {
+        c0.x, c1.x, c2.x, v2.x,
+        c0.y, c1.y, c2.y, v2.y,
+        c0.z, c1.z, c2.z, v2.z,
+        0.0f, 0.0f, 0.0f, 1.0f,
+    };
+    float inv[16];
+    orc_invert4x4(m, inv);
+    out[0] = inv[8]; out[1] = inv[9]; out[2] = inv[10]; out[3] = -inv[11];
+    out[4] = inv[0]; out[5] = inv[1]; out[6] = inv[2]; out[7] = inv[3];
+    out[8] = inv[4]; out[9] = inv[5]; out[10] = inv[6]; out[11] = inv[7];
+}
+
+/* ------------------------------------------------------------------------ */
+/* SBVH builder: SplitBVHBuilder, src/accelerators/sbvh/splitbvhbuilder.cpp  */
+/* ------------------------------------------------------------------------ */
+
+enum { MAX_DEPTH = 64, MAX_SPATIAL_DEPTH = 48, NUM_BINS = 128 };
+
+typedef struct { uint32_t tri; aabb b; } ref_t;
+
+typedef struct {
+    int leaf;
+    aabb b;
+    int32_t child[2];
+    uint32_t lo, hi;
+} node_t;
+
+struct orc_bvh {
+    node_t* nodes; uint32_t n_nodes, cap_nodes;
+    uint32_t* tris; uint32_t n_tris, cap_tris;
+    int32_t root;
+    uint64_t duplicates;
+    uint32_t max_depth;
+    float node_cost, tri_cost;
+    /* copies of the scene for the layout stage */
+    const float* verts; const uint32_t* idx;
+};
+
+typedef struct {
+    orc_bvh* bvh;
+    const float* verts; const uint32_t* idx;
+    ref_t* st; size_t n, cap;          /* m_ref_stack */
+    aabb* right_bounds;                /* m_right_bounds */
+    float min_overlap;
+    uint32_t min_leaf, max_leaf;
+    int sort_dim;
+    int oom;
+    aabb bin_b[3][NUM_BINS];
+    uint32_t bin_enter[3][NUM_BINS], bin_exit[3][NUM_BINS];
+} builder_t;
+
+/* Platform::get_triangle_cost / get_node_cost with batch size 1,
+ * src/accelerators/sbvh/platform.h:102-110. */
+static inline float tri_cost(const builder_t* B, uint32_t n) { return (float)n * B->bvh->tri_cost; }
+static inline float node_cost(const builder_t* B, uint32_t n) { return (float)n * B->bvh->node_cost; }
+
+static f3 vtx(const builder_t* B, uint32_t i) { return mk3(B->verts[3 * i], B->verts[3 * i + 1], B->verts[3 * i + 2]); }
+
+static int push_ref(builder_t* B, ref_t r)
+{
+    if (B->n == B->cap) {
+        size_t nc = B->cap ? B->cap * 2 : 1024;
+        ref_t* p = (ref_t*)realloc(B->st, nc * sizeof(ref_t));
+        if (!p) { B->oom = 1; return 0; }
+        B->st = p; B->cap = nc;
+    }
+    B->st[B->n++] = r;
+    return 1;
+}
+
+static int32_t new_node(builder_t* B)
+{
+    orc_bvh* T = B->bvh;
+    if (T->n_nodes == T->cap_nodes) {
+        uint32_t nc = T->cap_nodes ? T->cap_nodes * 2 : 1024;
+        node_t* p = (node_t*)realloc(T->nodes, (size_t)nc * sizeof(node_t));
+        if (!p) { B->oom = 1; return -1; }
+        T->nodes = p; T->cap_nodes = nc;
+    }
+    return (int32_t)T->n_nodes++;
+}
+
+/* sort_compare, splitbvhbuilder.cpp:273-282: centroid*2 along sort_dim, ties
+ * by triangle index.  A total order inside one node, so any sort gives the
+ * reference's bespoke quicksort result. */
+static int cmp_ref(const void* pa, const void* pb, void* ctx)
+{
+    int dim = *(const int*)ctx;
+    const ref_t* a = (const ref_t*)pa;
+    const ref_t* b = (const ref_t*)pb;
+    float ca = get3(a->b.lo, dim) + get3(a->b.hi, dim);
+    float cb = get3(b->b.lo, dim) + get3(b->b.hi, dim);
+    if (ca < cb) return -1;
+    if (cb < ca) return 1;
+    if (a->tri < b->tri) return -1;
+    if (a->tri > b->tri) return 1;
+    return 0;
+}
+
+static void sort_range(builder_t* B, size_t start, size_t end, int dim)
+{
+    if (end - start < 2) return;
+    qsort_r(B->st + start, end - start, sizeof(ref_t), cmp_ref, &dim);
+}
+
+typedef struct { float sah; int dim; uint32_t num_left; aabb lb, rb; } obj_split;
+typedef struct { float sah; int dim; float pos; } sp_split;
+
+/* find_object_split, splitbvhbuilder.cpp:369-408 */
+static obj_split find_object_split(builder_t* B, uint32_t num, float nodeSAH)
+{
+    obj_split s; s.sah = FLT_MAX; s.dim = 0; s.num_left = 0;
+    s.lb = aabb_empty(); s.rb = aabb_empty();
+    size_t base = B->n - num;
+    float bestTie = FLT_MAX;
+    for (int dim = 0; dim < 3; dim++) {
+        B->sort_dim = dim;
+        sort_range(B, base, B->n, dim);
+        const ref_t* r = B->st + base;
+        aabb rb = aabb_empty();
+        for (uint32_t i = num - 1; i > 0; i--) {
+            aabb_grow(&rb, &r[i].b);
+            B->right_bounds[i - 1] = rb;
+        }
+        aabb lb = aabb_empty();
+        for (uint32_t i = 1; i < num; i++) {
+            aabb_grow(&lb, &r[i - 1].b);
+            float sah = nodeSAH + aabb_area(&lb) * tri_cost(B, i) + aabb_area(&B->right_bounds[i - 1]) * tri_cost(B, num - i);
+            float fi = (float)i, fr = (float)(num - i);
+            float tie = fi * fi + fr * fr;
+            if (sah < s.sah || (sah == s.sah && tie < bestTie)) {
+                s.sah = sah; s.dim = dim; s.num_left = i;
+                s.lb = lb; s.rb = B->right_bounds[i - 1];
+                bestTie = tie;
+            }
+        }
+    }
+    return s;
+}
+
+/* split_reference, splitbvhbuilder.cpp:586-627 */
+static void split_reference(const builder_t* B, ref_t* L, ref_t* R, const ref_t* ref, int dim, float pos)
+{
+    L->tri = R->tri = ref->tri;
+    L->b = aabb_empty(); R->b = aabb_empty();
+    const uint32_t* ind = B->idx + 3 * (size_t)ref->tri;
+    f3 v1 = vtx(B, ind[2]);
+    for (int i = 0; i < 3; i++) {
+        f3 v0 = v1;
+        v1 = vtx(B, ind[i]);
+        float v0p = get3(v0, dim), v1p = get3(v1, dim);
+        if (v0p <= pos) aabb_insert(&L->b, v0);
+        if (v0p >= pos) aabb_insert(&R->b, v0);
+        if ((v0p < pos && v1p > pos) || (v0p > pos && v1p < pos)) {
+            float t = (pos - v0p) / (v1p - v0p);
+            t = fmaxf(0.0f, fminf(t, 1.0f));
+            f3 p = add3(v0, mul3s(sub3(v1, v0), t));
+            aabb_insert(&L->b, p);
+            aabb_insert(&R->b, p);
+        }
+    }
+    set3(&L->b.hi, dim, pos);
+    set3(&R->b.lo, dim, pos);
+    aabb_intersect(&L->b, &ref->b);
+    aabb_intersect(&R->b, &ref->b);
+}
+
+/* find_spatial_split, splitbvhbuilder.cpp:420-497 */
+static sp_split find_spatial_split(builder_t* B, const aabb* nb, uint32_t num, float nodeSAH)
+{
+    f3 origin = nb->lo;
+    f3 binSize = mul3s(sub3(nb->hi, origin), 1.0f / (float)NUM_BINS);
+    f3 inv = mk3(1.0f / binSize.x, 1.0f / binSize.y, 1.0f / binSize.z);
+    for (int d = 0; d < 3; d++)
+        for (int i = 0; i < NUM_BINS; i++) {
+            B->bin_b[d][i] = aabb_empty(); B->bin_enter[d][i] = 0; B->bin_exit[d][i] = 0;
+        }
+    for (size_t ri = B->n - num; ri < B->n; ri++) {
+        const ref_t* ref = &B->st[ri];
+        f3 rmin = mk3((ref->b.lo.x - origin.x) * inv.x, (ref->b.lo.y - origin.y) * inv.y, (ref->b.lo.z - origin.z) * inv.z);
+        f3 rmax = mk3((ref->b.hi.x - origin.x) * inv.x, (ref->b.hi.y - origin.y) * inv.y, (ref->b.hi.z - origin.z) * inv.z);
+        int first[3], last[3];
+        for (int d = 0; d < 3; d++) {
+            int f = trunc_int(get3(rmin, d));
+            f = f < 0 ? 0 : (f > NUM_BINS - 1 ? NUM_BINS - 1 : f);
+            int l = trunc_int(get3(rmax, d));
+            l = l > NUM_BINS - 1 ? NUM_BINS - 1 : l;
+            l = l < f ? f : l;
+            first[d] = f; last[d] = l;
+        }
+        for (int d = 0; d < 3; d++) {
+            ref_t cur = *ref;
+            for (int i = first[d]; i < last[d]; i++) {
+                ref_t L, R;
+                split_reference(B, &L, &R, &cur, d, get3(origin, d) + get3(binSize, d) * (float)(i + 1));
+                aabb_grow(&B->bin_b[d][i], &L.b);
+                cur = R;
+            }
+            aabb_grow(&B->bin_b[d][last[d]], &cur.b);
+            B->bin_enter[d][first[d]]++;
+            B->bin_exit[d][last[d]]++;
+        }
+    }
+    sp_split s; s.sah = FLT_MAX; s.dim = 0; s.pos = 0.0f;
+    for (int d = 0; d < 3; d++) {
+        aabb rb = aabb_empty();
+        for (int i = NUM_BINS - 1; i > 0; i--) {
+            aabb_grow(&rb, &B->bin_b[d][i]);
+            B->right_bounds[i - 1] = rb;
+        }
+        aabb lb = aabb_empty();
+        uint32_t ln = 0, rn = num;
+        for (int i = 1; i < NUM_BINS; i++) {
+            aabb_grow(&lb, &B->bin_b[d][i - 1]);
+            ln += B->bin_enter[d][i - 1];
+            rn -= B->bin_exit[d][i - 1];
+            float sah = nodeSAH + aabb_area(&lb) * tri_cost(B, ln) + aabb_area(&B->right_bounds[i - 1]) * tri_cost(B, rn);
+            if (sah < s.sah) {
+                s.sah = sah; s.dim = d;
+                s.pos = get3(origin, d) + get3(binSize, d) * (float)i;
+            }
+        }
+    }
+    return s;
+}
+
+/* perform_spatial_split, splitbvhbuilder.cpp:499-584 */
+static void perform_spatial_split(builder_t* B, uint32_t* lnum, aabb* lb, uint32_t* rnum, aabb* rb,
+                                  uint32_t num, const sp_split* sp)
+{
+    size_t leftStart = B->n - num, leftEnd = leftStart, rightStart = B->n;
+    *lb = aabb_empty(); *rb = aabb_empty();
+    for (size_t i = leftEnd; i < rightStart; i++) {
+        if (get3(B->st[i].b.hi, sp->dim) <= sp->pos) {
+            aabb_grow(lb, &B->st[i].b);
+            ref_t t = B->st[i]; B->st[i] = B->st[leftEnd]; B->st[leftEnd] = t; leftEnd++;
+        } else if (get3(B->st[i].b.lo, sp->dim) >= sp->pos) {
+            aabb_grow(rb, &B->st[i].b);
+            rightStart--;
+            ref_t t = B->st[i]; B->st[i] = B->st[rightStart]; B->st[rightStart] = t;
+            i--;
+        }
+    }
+    while (leftEnd < rightStart) {
+        ref_t L, R;
+        split_reference(B, &L, &R, &B->st[leftEnd], sp->dim, sp->pos);
+        aabb lub = *lb, rub = *rb, ldb = *lb, rdb = *rb;
+        aabb_grow(&lub, &B->st[leftEnd].b);
+        aabb_grow(&rub, &B->st[leftEnd].b);
+        aabb_grow(&ldb, &L.b);
+        aabb_grow(&rdb, &R.b);
+        float lac = tri_cost(B, (uint32_t)(leftEnd - leftStart));
+        float rac = tri_cost(B, (uint32_t)(B->n - rightStart));
+        float lbc = tri_cost(B, (uint32_t)(leftEnd - leftStart + 1));
+        float rbc = tri_cost(B, (uint32_t)(B->n - rightStart + 1));
+        float usl = aabb_area(&lub) * lbc + aabb_area(rb) * rac;
+        float usr = aabb_area(lb) * lac + aabb_area(&rub) * rbc;
+        float dup = aabb_area(&ldb) * lbc + aabb_area(&rdb) * rbc;
+        float m = fminf(usl, fminf(usr, dup));
+        if (m == usl) {
+            *lb = lub; leftEnd++;
+        } else if (m == usr) {
+            *rb = rub;
+            rightStart--;
+            ref_t t = B->st[leftEnd]; B->st[leftEnd] = B->st[rightStart]; B->st[rightStart] = t;
+        } else {
+            *lb = ldb; *rb = rdb;
+            B->st[leftEnd++] = L;
+            if (!push_ref(B, R)) return;
+        }
+    }
+    *lnum = (uint32_t)(leftEnd - leftStart);
+    *rnum = (uint32_t)(B->n - rightStart);
+}
+
+/* create_leaf, splitbvhbuilder.cpp:359-367: refs are popped from the back. */
+static int32_t create_leaf(builder_t* B, uint32_t num, const aabb* b)
+{
+    orc_bvh* T = B->bvh;
+    int32_t id = new_node(B);
+    if (id < 0) return -1;
+    if (T->n_tris + num > T->cap_tris) {
+        uint32_t nc = T->cap_tris ? T->cap_tris : 1024;
+        while (nc < T->n_tris + num) nc *= 2;
+        uint32_t* p = (uint32_t*)realloc(T->tris, (size_t)nc * 4);
+        if (!p) { B->oom = 1; return -1; }
+        T->tris = p; T->cap_tris = nc;
+    }
+    for (uint32_t i = 0; i < num; i++) T->tris[T->n_tris++] = B->st[--B->n].tri;
+    node_t* nd = &T->nodes[id];
+    nd->leaf = 1; nd->b = *b; nd->child[0] = nd->child[1] = -1;
+    nd->hi = T->n_tris; nd->lo = T->n_tris - num;
+    return id;
+}
+
+/* build_node, splitbvhbuilder.cpp:290-357 */
+static int32_t build_node(builder_t* B, uint32_t num, aabb bounds, int level)
+{
+    if (B->oom) return -1;
+    if ((uint32_t)level > B->bvh->max_depth) B->bvh->max_depth = (uint32_t)level;
+    /* Remove degenerates (:300-316). */
+    {
+        long firstRef = (long)B->n - (long)num;
+        for (long i = (long)B->n - 1; i >= firstRef; i--) {
+            f3 sz = sub3(B->st[i].b.hi, B->st[i].b.lo);
+            float mn = sz.x; if (sz.y < mn) mn = sz.y; if (sz.z < mn) mn = sz.z;
+            float mx = sz.x; if (sz.y > mx) mx = sz.y; if (sz.z > mx) mx = sz.z;
+            float sum = sz.x + sz.y + sz.z;
+            if (mn < 0.0f || sum == mx) {
+                ref_t back = B->st[B->n - 1];
+                B->n--;
+                if ((size_t)i < B->n) B->st[i] = back;
+            }
+        }
+        num = (uint32_t)((long)B->n - firstRef);
+    }
+    if (num <= B->min_leaf || level >= MAX_DEPTH) return create_leaf(B, num, &bounds);
+
+    float area = aabb_area(&bounds);
+    float leafSAH = area * tri_cost(B, num);
+    float nodeSAH = area * node_cost(B, 2);
+    obj_split obj = find_object_split(B, num, nodeSAH);
+
+    sp_split sp; sp.sah = FLT_MAX; sp.dim = 0; sp.pos = 0.0f;
+    if (level < MAX_SPATIAL_DEPTH) {
+        aabb ov = obj.lb;
+        aabb_intersect(&ov, &obj.rb);
+        if (aabb_area(&ov) >= B->min_overlap) sp = find_spatial_split(B, &bounds, num, nodeSAH);
+    }
+
+    float minSAH = leafSAH;
+    { float m2 = obj.sah < sp.sah ? obj.sah : sp.sah; if (m2 < minSAH) minSAH = m2; }
+    if (minSAH == leafSAH && num <= B->max_leaf) return create_leaf(B, num, &bounds);
+
+    uint32_t ln = 0, rn = 0;
+    aabb lb = aabb_empty(), rb = aabb_empty();
+    if (minSAH == sp.sah) perform_spatial_split(B, &ln, &lb, &rn, &rb, num, &sp);
+    if (B->oom) return -1;
+    if (!ln || !rn) {
+        /* perform_object_split, :409-418 */
+        sort_range(B, B->n - num, B->n, obj.dim);
+        ln = obj.num_left; lb = obj.lb;
+        rn = num - obj.num_left; rb = obj.rb;
+    }
+    B->bvh->duplicates += (uint64_t)(ln + rn - num);
+    int32_t right = build_node(B, rn, rb, level + 1);
+    int32_t left = build_node(B, ln, lb, level + 1);
+    if (right < 0 || left < 0) return -1;
+    int32_t id = new_node(B);
+    if (id < 0) return -1;
+    node_t* nd = &B->bvh->nodes[id];
+    nd->leaf = 0; nd->b = bounds; nd->child[0] = left; nd->child[1] = right; nd->lo = nd->hi = 0;
+    return id;
+}
+
+orc_bvh* orc_bvh_build(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                       uint32_t min_leaf, uint32_t max_leaf, float split_alpha,
+                       float ncost, float tcost)
+{
+    (void)nv;
+    orc_bvh* T = (orc_bvh*)calloc(1, sizeof(orc_bvh));
+    builder_t* B = (builder_t*)calloc(1, sizeof(builder_t));
+    if (!T || !B) { free(T); free(B); return NULL; }
+    T->node_cost = ncost; T->tri_cost = tcost;
+    T->verts = verts; T->idx = idx;
+    B->bvh = T; B->verts = verts; B->idx = idx;
+    B->min_leaf = min_leaf; B->max_leaf = max_leaf;
+    /* run(), :234-271 */
+    aabb root = aabb_empty();
+    for (uint32_t i = 0; i < nt; i++) {
+        ref_t r; r.tri = i; r.b = aabb_empty();
+        aabb_insert(&r.b, vtx(B, idx[3 * (size_t)i]));
+        aabb_insert(&r.b, vtx(B, idx[3 * (size_t)i + 1]));
+        aabb_insert(&r.b, vtx(B, idx[3 * (size_t)i + 2]));
+        if (!push_ref(B, r)) break;
+        aabb_grow(&root, &r.b);
+    }
+    B->min_overlap = aabb_area(&root) * split_alpha;
+    size_t rbn = (nt > NUM_BINS ? nt : NUM_BINS) - 1;
+    B->right_bounds = (aabb*)malloc(rbn * sizeof(aabb));
+    if (!B->right_bounds) B->oom = 1;
+    if (!B->oom) T->root = build_node(B, nt, root, 0);
+    int oom = B->oom;
+    free(B->st); free(B->right_bounds); free(B);
+    if (oom || T->root < 0) { orc_bvh_free(T); return NULL; }
+    return T;
+}
+
+void orc_bvh_free(orc_bvh* T)
+{
+    if (!T) return;
+    free(T->nodes); free(T->tris); free(T);
+}
+
+/* get_subtree_size / compute_subtree_probabilities,
+ * src/accelerators/sbvh/bvnode.cpp:64-100. */
+static void stats_rec(const orc_bvh* T, int32_t id, float prob, float* sah, uint64_t st[6])
+{
+    const node_t* n = &T->nodes[id];
+    uint32_t nchild = n->leaf ? 0 : 2;
+    uint32_t ntri = n->leaf ? n->hi - n->lo : 0;
+    *sah += prob * ((float)nchild * T->node_cost + (float)ntri * T->tri_cost);
+    if (n->leaf) { st[1]++; st[2] += ntri; return; }
+    st[0]++; st[3] += 2;
+    for (int i = 0; i < 2; i++) {
+        const node_t* c = &T->nodes[n->child[i]];
+        float cp = 0.0f;
+        if (prob > 0.0f) cp = prob * aabb_area(&c->b) / aabb_area(&n->b);
+        stats_rec(T, n->child[i], cp, sah, st);
+    }
+}
+
+void orc_bvh_stats(const orc_bvh* T, uint64_t st[6], float* sah)
+{
+    memset(st, 0, 6 * sizeof(uint64_t));
+    float s = 0.0f;
+    stats_rec(T, T->root, 1.0f, &s, st);
+    st[4] = T->duplicates;
+    st[5] = T->max_depth;
+    if (sah) *sah = s;
+}
+
+static uint32_t dump_rec(const orc_bvh* T, int32_t id, int32_t* info, float* bnd, uint32_t* k)
+{
+    uint32_t me = (*k)++;
+    const node_t* n = &T->nodes[id];
+    if (info) {
+        info[5 * me + 0] = n->leaf;
+        info[5 * me + 1] = -1; info[5 * me + 2] = -1;
+        info[5 * me + 3] = (int32_t)n->lo; info[5 * me + 4] = (int32_t)n->hi;
+    }
+    if (bnd) {
+        bnd[6 * me + 0] = n->b.lo.x; bnd[6 * me + 1] = n->b.lo.y; bnd[6 * me + 2] = n->b.lo.z;
+        bnd[6 * me + 3] = n->b.hi.x; bnd[6 * me + 4] = n->b.hi.y; bnd[6 * me + 5] = n->b.hi.z;
+    }
+    if (!n->leaf) {
+        uint32_t c0 = dump_rec(T, n->child[0], info, bnd, k);
+        uint32_t c1 = dump_rec(T, n->child[1], info, bnd, k);
+        if (info) { info[5 * me + 1] = (int32_t)c0; info[5 * me + 2] = (int32_t)c1; }
+    }
+    return me;
+}
+
+uint32_t orc_bvh_nodes(const orc_bvh* T, int32_t* info5, float* bounds6)
+{
+    uint32_t k = 0;
+    if (!info5 && !bounds6) return T->n_nodes;
+    dump_rec(T, T->root, info5, bounds6, &k);
+    return k;
+}
+
+uint32_t orc_bvh_tri_indices(const orc_bvh* T, uint32_t* out)
+{
+    if (out) memcpy(out, T->tris, (size_t)T->n_tris * 4);
+    return T->n_tris;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Compact2 layout, CudaBvh::create_compact(bvh, 16),                        */
+/* src/accelerators/sbvh/cudabvh.cpp:216-296                                 */
+/* ------------------------------------------------------------------------ */
+
+int orc_bvh_compact2(const orc_bvh* T, uint32_t* nodes, size_t* n_node_rows,
+                     uint32_t* woop, size_t* n_woop_rows, uint32_t* tri_index, size_t* n_index)
+{
+    /* A root leaf has no Compact2 form (the reference asserts, :237). */
+    if (T->nodes[T->root].leaf) return -1;
+    size_t cap_stack = 256, sp = 0;
+    struct ent { int32_t node; size_t row; } *stack = malloc(cap_stack * sizeof(*stack));
+    if (!stack) return -2;
+    size_t nrows = 4, wrows = 0, irows = 0;
+    stack[sp].node = T->root; stack[sp].row = 0; sp++;
+    while (sp) {
+        struct ent e = stack[--sp];
+        const node_t* n = &T->nodes[e.node];
+        const aabb* cb[2];
+        int32_t cidx[2];
+        for (int i = 0; i < 2; i++) {
+            const node_t* c = &T->nodes[n->child[i]];
+            cb[i] = &c->b;
+            if (!c->leaf) {
+                cidx[i] = (int32_t)(nrows * 16 / 16);
+                if (sp == cap_stack) {
+                    cap_stack *= 2;
+                    void* p = realloc(stack, cap_stack * sizeof(*stack));
+                    if (!p) { free(stack); return -2; }
+                    stack = p;
+                }
+                stack[sp].node = n->child[i]; stack[sp].row = nrows; sp++;
+                nrows += 4;
+                continue;
+            }
+            cidx[i] = ~(int32_t)wrows;
+            for (uint32_t j = c->lo; j < c->hi; j++) {
+                uint32_t tri = T->tris[j];
+                const uint32_t* ind = T->idx + 3 * (size_t)tri;
+                float w[12];
+                orc_woopify(T->verts + 3 * (size_t)ind[0], T->verts + 3 * (size_t)ind[1], T->verts + 3 * (size_t)ind[2], w);
+                if (w[0] == 0.0f) w[0] = 0.0f;
+                if (woop) for (int k = 0; k < 12; k++) woop[wrows * 4 + k] = f2u(w[k]);
+                if (tri_index) { tri_index[irows] = tri; tri_index[irows + 1] = 0; tri_index[irows + 2] = 0; }
+                wrows += 3; irows += 3;
+            }
+            if (woop) { woop[wrows * 4 + 0] = woop[wrows * 4 + 1] = woop[wrows * 4 + 2] = woop[wrows * 4 + 3] = 0x80000000u; }
+            if (tri_index) tri_index[irows] = 0;
+            wrows += 1; irows += 1;
+        }
+        if (nodes) {
+            uint32_t* d = nodes + e.row * 4;
+            d[0] = f2u(cb[0]->lo.x); d[1] = f2u(cb[0]->hi.x); d[2] = f2u(cb[0]->lo.y); d[3] = f2u(cb[0]->hi.y);
+            d[4] = f2u(cb[1]->lo.x); d[5] = f2u(cb[1]->hi.x); d[6] = f2u(cb[1]->lo.y); d[7] = f2u(cb[1]->hi.y);
+            d[8] = f2u(cb[0]->lo.z); d[9] = f2u(cb[0]->hi.z); d[10] = f2u(cb[1]->lo.z); d[11] = f2u(cb[1]->hi.z);
+            d[12] = (uint32_t)cidx[0]; d[13] = (uint32_t)cidx[1]; d[14] = 0; d[15] = 0;
+        }
+    }
+    free(stack);
+    if (n_node_rows) *n_node_rows = nrows;
+    if (n_woop_rows) *n_woop_rows = wrows;
+    if (n_index) *n_index = irows;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Traversal: kernel_kepler_trace, kepler_dynamic_fetch.h:67-363             */
+/* ------------------------------------------------------------------------ */
+
+#define SENTINEL 0x76543210
+#define ORC_STACK 96
+
+typedef struct { float x, y, z, w; } f4;
+static inline f4 row(const uint32_t* a, int32_t r)
+{
+    f4 v; const uint32_t* p = a + 4 * (size_t)r;
+    v.x = u2f(p[0]); v.y = u2f(p[1]); v.z = u2f(p[2]); v.w = u2f(p[3]);
+    return v;
+}
+
+static void trace_one(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx,
+                      const float* ray, int any_hit, int32_t* out_id, float* out_t, uint32_t* cnt)
+{
+    int32_t stack[ORC_STACK];
+    int sp = 0;
+    stack[0] = SENTINEL;
+    const float ox = ray[0], oy = ray[1], oz = ray[2], tmin = ray[3];
+    const float dx = ray[4], dy = ray[5], dz = ray[6];
+    float hitT = ray[7];
+    const float ooeps = 0x1p-80f;
+    const float idx_ = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
+    const float idy_ = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
+    const float idz_ = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
+    const float oodx = ox * idx_, oody = oy * idy_, oodz = oz * idz_;
+    int32_t leafAddr = 0, nodeAddr = 0, hitIndex = -1;
+    uint32_t hitId = 0xFFFFFFFFu;
+    uint32_t n_inner = 0, n_tri = 0, n_leaf = 0;
+
+    while (nodeAddr != SENTINEL) {
+        while ((uint32_t)nodeAddr < (uint32_t)SENTINEL) {
+            n_inner++;
+            const f4 n0 = row(nodes, nodeAddr + 0), n1 = row(nodes, nodeAddr + 1);
+            const f4 nz = row(nodes, nodeAddr + 2);
+            const uint32_t* c = nodes + 4 * (size_t)(nodeAddr + 3);
+            int32_t cx = (int32_t)c[0], cy = (int32_t)c[1];
+            /* slab tests, :173-188, with n*idir - ood written as one fma */
+            const float c0lox = fmaf(n0.x, idx_, -oodx), c0hix = fmaf(n0.y, idx_, -oodx);
+            const float c0loy = fmaf(n0.z, idy_, -oody), c0hiy = fmaf(n0.w, idy_, -oody);
+            const float c0loz = fmaf(nz.x, idz_, -oodz), c0hiz = fmaf(nz.y, idz_, -oodz);
+            const float c1loz = fmaf(nz.z, idz_, -oodz), c1hiz = fmaf(nz.w, idz_, -oodz);
+            const float c1lox = fmaf(n1.x, idx_, -oodx), c1hix = fmaf(n1.y, idx_, -oodx);
+            const float c1loy = fmaf(n1.z, idy_, -oody), c1hiy = fmaf(n1.w, idy_, -oody);
+            /* spanBeginKepler / spanEndKepler, cudatracerkernels.h:255-256 */
+            const float c0min = fmaxf(fmaxf(fminf(c0lox, c0hix), fminf(c0loy, c0hiy)), fmaxf(fminf(c0loz, c0hiz), tmin));
+            const float c0max = fminf(fminf(fmaxf(c0lox, c0hix), fmaxf(c0loy, c0hiy)), fminf(fmaxf(c0loz, c0hiz), hitT));
+            const float c1min = fmaxf(fmaxf(fminf(c1lox, c1hix), fminf(c1loy, c1hiy)), fmaxf(fminf(c1loz, c1hiz), tmin));
+            const float c1max = fminf(fminf(fmaxf(c1lox, c1hix), fmaxf(c1loy, c1hiy)), fminf(fmaxf(c1loz, c1hiz), hitT));
+            const int swp = c1min < c0min;
+            const int t0 = c0max >= c0min, t1 = c1max >= c1min;
+            if (!t0 && !t1) {
+                nodeAddr = stack[sp--];
+            } else {
+                nodeAddr = t0 ? cx : cy;
+                if (t0 && t1) {
+                    if (swp) { int32_t tmp = nodeAddr; nodeAddr = cy; cy = tmp; }
+                    stack[++sp] = cy;
+                }
+            }
+            if (nodeAddr < 0 && leafAddr >= 0) {
+                leafAddr = nodeAddr;
+                nodeAddr = stack[sp--];
+            }
+            if (!(leafAddr >= 0)) break;
+        }
+        while (leafAddr < 0) {
+            for (int32_t triAddr = ~leafAddr;; triAddr += 3) {
+                const f4 v00 = row(woop, triAddr + 0);
+                if (f2u(v00.x) == 0x80000000u) { n_leaf++; break; }
+                const f4 v11 = row(woop, triAddr + 1), v22 = row(woop, triAddr + 2);
+                n_tri++;
+                /* Woop test, :260-282, dot products written as fma chains */
+                const float Oz = fmaf(-oz, v00.z, fmaf(-oy, v00.y, fmaf(-ox, v00.x, v00.w)));
+                const float invDz = 1.0f / fmaf(dz, v00.z, fmaf(dy, v00.y, dx * v00.x));
+                const float t = Oz * invDz;
+                if (t > tmin && (t < hitT || (hitIndex != -1 && t == hitT))) {
+                    const float Ox = fmaf(oz, v11.z, fmaf(oy, v11.y, fmaf(ox, v11.x, v11.w)));
+                    const float Dx = fmaf(dz, v11.z, fmaf(dy, v11.y, dx * v11.x));
+                    const float u = fmaf(t, Dx, Ox);
+                    if (u >= 0.0f) {
+                        const float Oy = fmaf(oz, v22.z, fmaf(oy, v22.y, fmaf(ox, v22.x, v22.w)));
+                        const float Dy = fmaf(dz, v22.z, fmaf(dy, v22.y, dx * v22.x));
+                        const float v = fmaf(t, Dy, Oy);
+                        if (v >= 0.0f && u + v <= 1.0f) {
+                            const uint32_t id = tidx[triAddr];
+                            if (t < hitT || id < hitId) {
+                                hitT = t; hitIndex = triAddr; hitId = id;
+                                if (any_hit) { nodeAddr = SENTINEL; break; }
+                            }
+                        }
+                    }
+                }
+            }
+            leafAddr = nodeAddr;
+            if (nodeAddr < 0) nodeAddr = stack[sp--];
+        }
+    }
+    *out_id = hitIndex == -1 ? -1 : (int32_t)hitId;
+    *out_t = hitT;
+    if (cnt) { cnt[0] = n_inner; cnt[1] = n_tri; cnt[2] = n_leaf; }
+}
+
+void orc_trace_compact2(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tri_index,
+                        const float* rays, uint32_t n, int any_hit,
+                        int32_t* out_id, float* out_t, uint32_t* counts, int nthreads)
+{
+#ifdef _OPENMP
+    int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+#else
+    (void)nthreads;
+#endif
+    for (long i = 0; i < (long)n; i++)
+        trace_one(nodes, woop, tri_index, rays + 8 * (size_t)i, any_hit, out_id + i, out_t + i,
+                  counts ? counts + 3 * (size_t)i : NULL);
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 brute force                                                          */
+/* ------------------------------------------------------------------------ */
+
+void orc_bruteforce_f64(const float* verts, const uint32_t* idx, uint32_t nt,
+                        const float* rays, uint32_t n, int32_t* out_id, double* out_t64,
+                        double* out_margin, double* out_gap, int nthreads)
+{
+#ifdef _OPENMP
+    int nth = nthreads > 0 ? nthreads : omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nth)
+#else
+    (void)nthreads;
+#endif
+    for (long r = 0; r < (long)n; r++) {
+        const float* R = rays + 8 * (size_t)r;
+        double o[3] = {R[0], R[1], R[2]}, d[3] = {R[4], R[5], R[6]};
+        double tmin = R[3], tmax = R[7];
+        double best = INFINITY, second = INFINITY, margin = 0.0;
+        int32_t bid = -1;
+        for (uint32_t k = 0; k < nt; k++) {
+            const float* a = verts + 3 * (size_t)idx[3 * (size_t)k];
+            const float* b = verts + 3 * (size_t)idx[3 * (size_t)k + 1];
+            const float* c = verts + 3 * (size_t)idx[3 * (size_t)k + 2];
+            double e1[3] = {(double)b[0] - a[0], (double)b[1] - a[1], (double)b[2] - a[2]};
+            double e2[3] = {(double)c[0] - a[0], (double)c[1] - a[1], (double)c[2] - a[2]};
+            double p[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+            double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+            if (det == 0.0) continue;
+            double id = 1.0 / det;
+            double tv[3] = {o[0] - a[0], o[1] - a[1], o[2] - a[2]};
+            double u = (tv[0] * p[0] + tv[1] * p[1] + tv[2] * p[2]) * id;
+            if (u < 0.0 || u > 1.0) continue;
+            double q[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+            double v = (d[0] * q[0] + d[1] * q[1] + d[2] * q[2]) * id;
+            if (v < 0.0 || u + v > 1.0) continue;
+            double t = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * id;
+            if (!(t > tmin && t < tmax)) continue;
+            double m = u < v ? u : v;
+            if (1.0 - u - v < m) m = 1.0 - u - v;
+            if (t < best || (t == best && (int32_t)k < bid)) {
+                if (t < best) second = best;
+                best = t; bid = (int32_t)k; margin = m;
+            } else if (t < second) {
+                second = t;
+            }
+        }
+        out_id[r] = bid;
+        out_t64[r] = bid < 0 ? (double)tmax : best;
+        if (out_margin) out_margin[r] = bid < 0 ? 1.0 : margin;
+        if (out_gap) out_gap[r] = bid < 0 ? INFINITY : (second - best) / (best > 1e-30 ? best : 1e-30);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers: as   */
+/* easy as 1, 2, 3", SC'11): multipliers 0xD2511F53 / 0xCD9E8D57, Weyl key   */
+/* increments 0x9E3779B9 / 0xBB67AE85, ten rounds.                           */
+/* ------------------------------------------------------------------------ */
+
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; r++) {
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sampling (shared conventions of the passes).  The reference draws from    */
+/* Thrust default_random_engine / cuRAND (src/core/montecarlo.cu, integrators)*/
+/* whose streams cannot be reproduced here; both sides use Philox4x32-10     */
+/* keyed (seed, stream tag) with counters (index, sample, bounce, dim).      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { uint32_t k0, k1; } orc_rng;
+
+static orc_rng rng_make(uint64_t seed, uint32_t tag)
+{
+    orc_rng r; r.k0 = (uint32_t)seed; r.k1 = (uint32_t)(seed >> 32) ^ (tag * 0x85EBCA6Bu);
+    return r;
+}
+
+static void rng_draw(orc_rng r, uint32_t a, uint32_t b, uint32_t c, uint32_t d, uint32_t out[4])
+{
+    uint32_t ctr[4] = {a, b, c, d}, key[2] = {r.k0, r.k1};
+    orc_philox4x32_10(ctr, key, out);
+}
+
+static inline float u01(uint32_t x) { return (float)(x >> 8) * 5.9604644775390625e-08f; }
+
+/* sin/cos of 2*pi*u: quarter-turn reduction + degree-9/10 Taylor series. */
+void orc_sincos_2pi(float u, float* s, float* c)
+{
+    const float a = u * 4.0f;
+    const int k = (int)(a + 0.5f);
+    const float r = a - (float)k;
+    const float x = r * 1.57079632679489662f;
+    const float x2 = x * x;
+    float ps = fmaf(x2, 2.75573192e-06f, -1.98412698e-04f);
+    ps = fmaf(x2, ps, 8.33333333e-03f);
+    ps = fmaf(x2, ps, -1.66666667e-01f);
+    const float sn = fmaf(x * x2, ps, x);
+    float pc = fmaf(x2, -2.75573192e-07f, 2.48015873e-05f);
+    pc = fmaf(x2, pc, -1.38888889e-03f);
+    pc = fmaf(x2, pc, 4.16666667e-02f);
+    pc = fmaf(x2, pc, -0.5f);
+    const float cs = fmaf(x2, pc, 1.0f);
+    switch (k & 3) {
+    case 0: *s = sn; *c = cs; break;
+    case 1: *s = cs; *c = -sn; break;
+    case 2: *s = -sn; *c = -cs; break;
+    default: *s = -cs; *c = sn; break;
+    }
+}
+
+/* Duff et al. 2017 orthonormal basis + Malley cosine-weighted direction. */
+static f3 cosine_dir(f3 n, float u1, float u2)
+{
+    float s, c;
+    orc_sincos_2pi(u2, &s, &c);
+    const float r = sqrtf(u1);
+    const float x = r * c, y = r * s;
+    const float zz = 1.0f - u1;
+    const float z = sqrtf(zz > 0.0f ? zz : 0.0f);
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    f3 b1 = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f3 b2 = mk3(b, sign + n.y * n.y * a, -n.y);
+    return add3(add3(mul3s(b1, x), mul3s(b2, y)), mul3s(n, z));
+}
+
+void orc_cosine_dir(const float* n, float u1, float u2, float* out)
+{
+    f3 d = cosine_dir(mk3(n[0], n[1], n[2]), u1, u2);
+    out[0] = d.x; out[1] = d.y; out[2] = d.z;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Camera: PerspectiveCamera ctor + generate_ray, src/core/camera.cu:30-98,  */
+/* transforms of src/core/transform.cu:28-126 and transform.h:168-197.       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { float m[16]; } m4;
+typedef struct { m4 m, inv; } xf4;
+
+static m4 m4_ident(void) { m4 r; memset(&r, 0, sizeof(r)); r.m[0] = r.m[5] = r.m[10] = r.m[15] = 1.0f; return r; }
+static m4 m4_mul(const m4* a, const m4* b)
+{
+    m4 r;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            r.m[4 * i + j] = a->m[4 * i] * b->m[j] + a->m[4 * i + 1] * b->m[4 + j] + a->m[4 * i + 2] * b->m[8 + j] + a->m[4 * i + 3] * b->m[12 + j];
+    return r;
+}
+static m4 m4_inv(const m4* a) { m4 r; orc_invert4x4(a->m, r.m); return r; }
+static xf4 xf_of(m4 m) { xf4 t; t.m = m; t.inv = m4_inv(&m); return t; }
+static xf4 xf_mul(const xf4* a, const xf4* b) { xf4 t; t.m = m4_mul(&a->m, &b->m); t.inv = m4_mul(&b->inv, &a->inv); return t; }
+static xf4 xf_inverse(const xf4* a) { xf4 t; t.m = a->inv; t.inv = a->m; return t; }
+static xf4 xf_scale(float x, float y, float z) { m4 r = m4_ident(); r.m[0] = x; r.m[5] = y; r.m[10] = z; r.m[15] = 1.0f; return xf_of(r); }
+static xf4 xf_translate(float x, float y, float z) { m4 r = m4_ident(); r.m[3] = x; r.m[7] = y; r.m[11] = z; return xf_of(r); }
+
+void orc_camera_matrices(const float* pos, const float* at, const float* up, float hfov,
+                         uint32_t width, uint32_t height, float* r2c, float* c2w)
+{
+    f3 P = mk3(pos[0], pos[1], pos[2]), A = mk3(at[0], at[1], at[2]), U = mk3(up[0], up[1], up[2]);
+    /* lookat, transform.cu:78-107 */
+    f3 dir = normalize3(sub3(A, P));
+    f3 left = normalize3(cross3(normalize3(U), dir));
+    f3 nup = cross3(dir, left);
+    m4 cw;
+    float mm[16] = {left.x, nup.x, dir.x, P.x, left.y, nup.y, dir.y, P.y, left.z, nup.z, dir.z, P.z, 0.0f, 0.0f, 0.0f, 1.0f};
+    memcpy(cw.m, mm, sizeof(mm));
+    xf4 look; look.m = m4_inv(&cw); look.inv = cw;
+    xf4 s = xf_scale(-1.0f, 1.0f, 1.0f);
+    /* camera-space handedness flip (scale * lookat); the reference's
+     * lookat * scale (camera.cu:39) mirrors the world about x = 0 instead */
+    xf4 w2c = xf_mul(&s, &look);
+    xf4 cam2world = xf_inverse(&w2c);
+    /* perspective, transform.cu:114-124 */
+    m4 pp; memset(&pp, 0, sizeof(pp));
+    const float n = 1.0f, f = 1000.0f;
+    pp.m[0] = 1.0f; pp.m[5] = 1.0f; pp.m[10] = f / (f - n); pp.m[11] = -f * n / (f - n); pp.m[14] = 1.0f;
+    const float to_rad = 3.14159265358979323846f / 180.f;
+    float inv_tan = 1.0f / tanf(to_rad * hfov / 2.0f);
+    xf4 sc = xf_scale(inv_tan, inv_tan, 1.0f), pr = xf_of(pp);
+    xf4 cam2screen = xf_mul(&sc, &pr);
+    xf4 a = xf_scale((float)width, (float)height, 1.0f), b = xf_scale(0.5f, -0.5f, 1.0f), t = xf_translate(1.0f, -1.0f, 0.0f);
+    xf4 ab = xf_mul(&a, &b);
+    xf4 screen2raster = xf_mul(&ab, &t);
+    xf4 raster2screen = xf_inverse(&screen2raster);
+    xf4 screen2cam = xf_inverse(&cam2screen);
+    xf4 raster2cam = xf_mul(&screen2cam, &raster2screen);
+    memcpy(r2c, raster2cam.m.m, 64);
+    memcpy(c2w, cam2world.m.m, 64);
+}
+
+static f3 xform_point(const float* m, float x, float y, float z)
+{
+    float r[4];
+    for (int i = 0; i < 4; i++) r[i] = m[4 * i] * x + m[4 * i + 1] * y + m[4 * i + 2] * z + m[4 * i + 3] * 1.0f;
+    return mk3(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
+}
+
+static f3 xform_vector(const float* m, f3 v)
+{
+    float r[3];
+    for (int i = 0; i < 3; i++) r[i] = m[4 * i] * v.x + m[4 * i + 1] * v.y + m[4 * i + 2] * v.z + m[4 * i + 3] * 0.0f;
+    return mk3(r[0], r[1], r[2]);
+}
+
+/* kernel_generate_perspective_camera_samples, src/renderers/samplerrenderer.cu:48-96:
+ * ray (pixel-major, sample-minor) = generate_ray(px + jx, py + jy). */
+void orc_raygen_camera(const float* r2c, const float* c2w, uint32_t width, uint32_t x0, uint32_t y0,
+                       uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, int jitter, float* rays)
+{
+    orc_rng rng = rng_make(seed, 1u);
+    for (uint64_t i = 0; i < (uint64_t)w * h * spp; i++) {
+        uint32_t s = (uint32_t)(i % spp), p = (uint32_t)(i / spp);
+        uint32_t px = x0 + p % w, py = y0 + p / w;
+        float jx = 0.5f, jy = 0.5f;
+        if (jitter) {
+            uint32_t r[4];
+            rng_draw(rng, py * width + px, s, 0, 0, r);
+            jx = u01(r[0]); jy = u01(r[1]);
+        }
+        f3 pc = xform_point(r2c, (float)px + jx, (float)py + jy, 0.0f);
+        f3 d = normalize3(pc);
+        f3 o = xform_point(c2w, 0.0f, 0.0f, 0.0f);
+        f3 dw = xform_vector(c2w, d);
+        float* R = rays + 8 * i;
+        R[0] = o.x; R[1] = o.y; R[2] = o.z; R[3] = 0.0f;
+        R[4] = dw.x; R[5] = dw.y; R[6] = dw.z; R[7] = FLT_MAX;
+    }
+}
+
+/* Incoherent diffuse rays from primary hits (the trace benchmark's second
+ * set): origin = hit point of primary i (a Philox-redrawn primary when i
+ * missed), direction cosine-distributed about the geometric normal turned
+ * against the incoming ray, tmin = 1e-3 t (the epsilon convention of
+ * kepler_dynamic_fetch.h:356). */
+void orc_raygen_diffuse(const float* verts, const uint32_t* idx, const float* prim, const int32_t* hit_id,
+                        const float* hit_t, uint32_t np, uint32_t nout, uint64_t seed, float* out)
+{
+    orc_rng rng = rng_make(seed, 3u), redraw = rng_make(seed, 4u);
+    for (uint32_t i = 0; i < nout; i++) {
+        uint32_t j = i % np;
+        for (uint32_t att = 0; hit_id[j] < 0 && att < 64; att++) {
+            uint32_t r[4];
+            rng_draw(redraw, i, att, 0, 0, r);
+            j = r[0] % np;
+        }
+        float* R = out + 8 * (size_t)i;
+        if (hit_id[j] < 0) {
+            R[0] = R[1] = R[2] = R[3] = 0.0f; R[4] = 0.0f; R[5] = 0.0f; R[6] = 1.0f; R[7] = 0.0f;
+            continue;
+        }
+        const float* o = prim + 8 * (size_t)j;
+        const float t = hit_t[j];
+        f3 p = add3(mk3(o[0], o[1], o[2]), mul3s(mk3(o[4], o[5], o[6]), t));
+        const uint32_t* ind = idx + 3 * (size_t)hit_id[j];
+        f3 v0 = mk3(verts[3 * ind[0]], verts[3 * ind[0] + 1], verts[3 * ind[0] + 2]);
+        f3 v1 = mk3(verts[3 * ind[1]], verts[3 * ind[1] + 1], verts[3 * ind[1] + 2]);
+        f3 v2 = mk3(verts[3 * ind[2]], verts[3 * ind[2] + 1], verts[3 * ind[2] + 2]);
+        f3 n = normalize3(cross3(sub3(v1, v0), sub3(v2, v0)));
+        if (dot3(n, mk3(o[4], o[5], o[6])) > 0.0f) n = mk3(-n.x, -n.y, -n.z);
+        uint32_t r[4];
+        rng_draw(rng, i, 0, 0, 0, r);
+        f3 d = cosine_dir(n, u01(r[0]), u01(r[1]));
+        R[0] = p.x; R[1] = p.y; R[2] = p.z; R[3] = 1.0e-3f * t;
+        R[4] = d.x; R[5] = d.y; R[6] = d.z; R[7] = FLT_MAX;
+    }
+}

@@ -1,0 +1,119 @@
+/*
+ * rb_oracle.h -- CPU restatement of the renderbox2 ray-tracing hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * and nowhere else.  The product library (renderbox2_amd/) never links,
+ * loads or calls it.
+ *
+ * Every function restates the reference algorithm named in its comment
+ * (paths relative to renderbox2/ in the reference tree).  Arithmetic is plain
+ * IEEE-754 binary32; the file is compiled with -ffp-contract=off so that every
+ * multiply and add rounds on its own unless an fmaf() is written out.
+ *
+ * The pins of each function (closed forms, FP64 brute force, invariants,
+ * known-answer vectors) live in tests/test_oracle_*.py.
+ */
+#ifndef RB_ORACLE_H
+#define RB_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- math ------------------------------------------------------------- */
+
+/* Matrix4x4::invert, src/core/transform.h:205-333 (cofactor expansion). */
+void orc_invert4x4(const float* m, float* out);
+
+/* CudaBvh::woopify_tri, src/accelerators/sbvh/cudabvh.cpp:303-322.
+ * out = 3 rows of float4 (12 floats). */
+void orc_woopify(const float* v0, const float* v1, const float* v2, float* out);
+
+/* ---- SBVH build ------------------------------------------------------- */
+
+typedef struct orc_bvh orc_bvh;
+
+/* SplitBVHBuilder::run/build_node, src/accelerators/sbvh/splitbvhbuilder.cpp:234-627
+ * with Platform defaults (src/accelerators/sbvh/platform.h:81-90) except the
+ * leaf preferences, and BuildParams::m_split_alpha (src/accelerators/sbvh/bvh.h:110).
+ * Returns NULL on allocation failure. */
+orc_bvh* orc_bvh_build(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                       uint32_t min_leaf, uint32_t max_leaf, float split_alpha,
+                       float node_cost, float tri_cost);
+void orc_bvh_free(orc_bvh* b);
+
+/* BVH::Stats (src/accelerators/sbvh/bvh.h:78-98) filled as in
+ * src/accelerators/sbvh/splitbvhbuilder.cpp:200-213 and bvnode.cpp:64-100.
+ * stats: [0]=n_inner [1]=n_leaf [2]=n_tri_refs [3]=n_child_nodes [4]=duplicates
+ *        [5]=max_depth  ; sah written to *sah. */
+void orc_bvh_stats(const orc_bvh* b, uint64_t stats[6], float* sah);
+
+/* Node dump for invariant tests: per node (pre-order from the root)
+ * is_leaf, child0, child1 (node numbers, -1 for leaves), lo, hi (into the
+ * tri-index list), bounds[6].  Returns the node count when buffers are NULL. */
+uint32_t orc_bvh_nodes(const orc_bvh* b, int32_t* info5, float* bounds6);
+uint32_t orc_bvh_tri_indices(const orc_bvh* b, uint32_t* out);
+
+/* CudaBvh::create_compact(bvh, 16) -- the "Compact2" layout,
+ * src/accelerators/sbvh/cudabvh.cpp:216-296.  Sizes are in uint4 rows (nodes,
+ * woop) and uint32 entries (index).  Call once with NULL buffers for sizes. */
+int orc_bvh_compact2(const orc_bvh* b, uint32_t* nodes, size_t* n_node_rows,
+                     uint32_t* woop, size_t* n_woop_rows, uint32_t* tri_index,
+                     size_t* n_index);
+
+/* ---- traversal -------------------------------------------------------- */
+
+/* kernel_kepler_trace, src/accelerators/sbvh/kepler_dynamic_fetch.h:67-363,
+ * restated for one ray at a time (the warp vote of :234-243 reduces to the
+ * ray's own predicate).  Ties at equal t go to the lowest triangle id.
+ *   rays:   n x 8 floats (ox,oy,oz,tmin, dx,dy,dz,tmax)
+ *   out_id: original triangle id or -1; out_t: hit t (tmax on a miss)
+ *   counts: optional n x 3 (inner nodes fetched, triangles tested, leaf
+ *           terminators reached).
+ * nthreads <= 0 uses every core (OpenMP). */
+void orc_trace_compact2(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tri_index,
+                        const float* rays, uint32_t n, int any_hit,
+                        int32_t* out_id, float* out_t, uint32_t* counts, int nthreads);
+
+/* FP64 brute force over every triangle (Moller-Trumbore, the textbook form
+ * of intersect_ray_triangle, src/core/primitives.cu:51-70, without its
+ * epsilon cull).  out_t64 is the double t; out_margin is the smallest of
+ * (u, v, 1-u-v) at the hit, scaled by nothing (used to flag grazing hits);
+ * out_gap is the relative distance to the next different triangle's t. */
+void orc_bruteforce_f64(const float* verts, const uint32_t* idx, uint32_t nt,
+                        const float* rays, uint32_t n, int32_t* out_id, double* out_t64,
+                        double* out_margin, double* out_gap, int nthreads);
+
+/* ---- RNG -------------------------------------------------------------- */
+
+/* Philox4x32-10 (Salmon et al., SC'11), the counter-based generator that
+ * replaces Thrust default_random_engine / cuRAND XORWOW. */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- sampling / camera / ray generation -------------------------------- */
+
+/* sin and cos of 2*pi*u by quarter-turn reduction and Taylor series. */
+void orc_sincos_2pi(float u, float* s, float* c);
+/* cosine-weighted direction about unit n (Duff et al. basis, Malley map). */
+void orc_cosine_dir(const float* n, float u1, float u2, float* out);
+
+/* PerspectiveCamera ctor (src/core/camera.cu:30-57): raster->camera and
+ * camera->world matrices, row-major. */
+void orc_camera_matrices(const float* pos, const float* at, const float* up, float hfov,
+                         uint32_t width, uint32_t height, float* r2c, float* c2w);
+/* kernel_generate_perspective_camera_samples (src/renderers/samplerrenderer.cu:48-96)
+ * with generate_ray (src/core/camera.cu:91-98). rays: w*h*spp x 8 floats. */
+void orc_raygen_camera(const float* r2c, const float* c2w, uint32_t width, uint32_t x0, uint32_t y0,
+                       uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, int jitter, float* rays);
+/* Diffuse rays from primary hits (trace benchmark ray set ii). */
+void orc_raygen_diffuse(const float* verts, const uint32_t* idx, const float* prim, const int32_t* hit_id,
+                        const float* hit_t, uint32_t np, uint32_t nout, uint64_t seed, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,11 @@
+"""renderbox2_amd -- MI355X-native ray-tracing core for renderbox2's hot path.
+
+librb.so (host C++ SBVH builder + gfx950 HIP kernels) behind the C-ABI in
+include/rb.h; this package mirrors the reference's Scene / BVH / CudaBvh /
+CudaTracer / PerspectiveCamera interface over it.
+"""
+from ._lib import RbError, lib  # noqa: F401
+from .core import (BVH, FLT_MAX, HIT_DTYPE, RAY_DTYPE, BuildParams, CudaBvh, CudaTracer,  # noqa: F401
+                   DeviceBuffer, Event, PerspectiveCamera, Platform, Scene, Stream, device_init,
+                   get_trace_config, raygen_diffuse_device, set_trace_config)
+from . import scenes  # noqa: F401

@@ -1,0 +1,164 @@
+"""ctypes binding of librb.so (the C-ABI declared in include/rb.h)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librb.so")
+
+P = C.c_void_p
+U32 = C.c_uint32
+U64 = C.c_uint64
+F32 = C.c_float
+I32 = C.c_int32
+SZ = C.c_size_t
+
+RB_OK, RB_ERR_INVALID, RB_ERR_OOM, RB_ERR_DEVICE, RB_ERR_BUSY, RB_ERR_UNSUPPORTED, RB_ERR_NOT_READY = range(7)
+RB_CLOSEST, RB_ANY, RB_MASKED = 0, 1, 2
+RB_LAYOUT_REF_COMPACT2, RB_LAYOUT_MI355X = 1, 2
+RB_MAT_LAMBERT, RB_MAT_MIRROR, RB_MAT_EMITTER = 0, 1, 2
+RB_GEN_ICOSPHERES, RB_GEN_THIN_QUADS, RB_GEN_ROOM, RB_GEN_EMITTER_QUADS = 0, 1, 2, 3
+
+
+class RbError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+_STATUS = {0: "RB_OK", 1: "RB_ERR_INVALID", 2: "RB_ERR_OOM", 3: "RB_ERR_DEVICE", 4: "RB_ERR_BUSY",
+           5: "RB_ERR_UNSUPPORTED", 6: "RB_ERR_NOT_READY"}
+
+
+class rb_material(C.Structure):
+    _fields_ = [("kind", U32), ("color", F32 * 3)]
+
+
+class rb_gen_params(C.Structure):
+    _fields_ = [("kind", U32), ("count", U32), ("subdiv", U32), ("seed", U64),
+                ("r0", F32), ("r1", F32), ("c0", F32), ("c1", F32),
+                ("material_id", I32), ("mix_materials", U32), ("emit", F32 * 3),
+                ("subdiv_hi", U32), ("hi_count", U32)]
+
+
+class rb_build_params(C.Structure):
+    _fields_ = [("min_leaf", U32), ("max_leaf", U32), ("split_alpha", F32), ("node_cost", F32),
+                ("tri_cost", F32), ("threads", U32)]
+
+
+class rb_bvh_stats(C.Structure):
+    _fields_ = [("sah", F32), ("n_inner", U32), ("n_leaf", U32), ("n_tri_refs", U32),
+                ("n_child_nodes", U32), ("max_depth", U32), ("duplicates", U64), ("build_ms", C.c_double)]
+
+
+class rb_trace_config(C.Structure):
+    _fields_ = [("block", U32), ("lds_stack", U32), ("chunk", U32), ("fetch_threshold", U32),
+                ("waves_per_cu", U32)]
+
+
+class rb_camera(C.Structure):
+    _fields_ = [("position", F32 * 3), ("lookat", F32 * 3), ("up", F32 * 3), ("hfov_deg", F32),
+                ("width", U32), ("height", U32)]
+
+
+class rb_isect_soa(C.Structure):
+    _fields_ = [("position", P * 3), ("geometric_normal", P * 3), ("shading_normal", P * 3),
+                ("incoming_direction", P * 3), ("uv", P), ("primitive_id", P), ("material_id", P),
+                ("front", P), ("intersected", P), ("epsilon", P), ("t", P)]
+
+
+# name -> (restype, argtypes); every function returns rb_status unless stated.
+_SIGS = {
+    "rb_last_error": (SZ, [C.c_char_p, SZ]),
+    "rb_status_string": (C.c_char_p, [C.c_int]),
+    "rb_version": (C.c_char_p, []),
+    "rb_device_init": (C.c_int, [C.c_int]),
+    "rb_device_name": (C.c_int, [C.c_char_p, SZ, P]),
+    "rb_malloc": (C.c_int, [P, SZ]),
+    "rb_free": (C.c_int, [P]),
+    "rb_memcpy_htod": (C.c_int, [P, P, SZ, P]),
+    "rb_memcpy_dtoh": (C.c_int, [P, P, SZ, P]),
+    "rb_memcpy_dtod": (C.c_int, [P, P, SZ, P]),
+    "rb_memset": (C.c_int, [P, C.c_int, SZ, P]),
+    "rb_mem_info": (C.c_int, [P, P, P]),
+    "rb_stream_create": (C.c_int, [P]),
+    "rb_stream_destroy": (C.c_int, [P]),
+    "rb_stream_synchronize": (C.c_int, [P]),
+    "rb_event_create": (C.c_int, [P]),
+    "rb_event_destroy": (C.c_int, [P]),
+    "rb_event_record": (C.c_int, [P, P]),
+    "rb_event_elapsed_ms": (C.c_int, [P, P, P]),
+    "rb_event_synchronize": (C.c_int, [P]),
+    "rb_scene_create": (C.c_int, [P]),
+    "rb_scene_destroy": (C.c_int, [P]),
+    "rb_scene_add_material": (C.c_int, [P, P, P]),
+    "rb_scene_add_mesh": (C.c_int, [P, P, U32, P, U32, U32]),
+    "rb_scene_counts": (C.c_int, [P, P, P, P, P]),
+    "rb_scene_geometry": (C.c_int, [P, P, P, P, P]),
+    "rb_scene_materials": (C.c_int, [P, P]),
+    "rb_scene_bounds": (C.c_int, [P, P, P]),
+    "rb_scene_generate": (C.c_int, [P, P]),
+    "rb_build_params_default": (None, [P]),
+    "rb_bvh_build_host": (C.c_int, [P, P, P]),
+    "rb_bvh_build_arrays": (C.c_int, [P, U32, P, U32, P, P]),
+    "rb_bvh_host_stats": (C.c_int, [P, P]),
+    "rb_bvh_host_release": (C.c_int, [P]),
+    "rb_bvh_export": (C.c_int, [P, U32, P, P, P, P, P, P]),
+    "rb_bvh_save_text": (C.c_int, [P, C.c_char_p]),
+    "rb_bvh_load_text_compact2": (C.c_int, [C.c_char_p, P, P, P, P, P, P]),
+    "rb_bvh_upload": (C.c_int, [P, P, P]),
+    "rb_bvh_release": (C.c_int, [P]),
+    "rb_bvh_device_bytes": (C.c_int, [P, P]),
+    "rb_trace": (C.c_int, [P, P, U32, U32, P, P, P]),
+    "rb_trace_fill": (C.c_int, [P, P, U32, U32, P, P, P, P]),
+    "rb_trace_get_config": (C.c_int, [P]),
+    "rb_trace_set_config": (C.c_int, [P]),
+    "rb_trace_counts": (C.c_int, [P, P, U32, U32, P, P, P]),
+    "rb_camera_matrices": (C.c_int, [P, P, P]),
+    "rb_raygen_camera": (C.c_int, [P, U32, U32, U32, U32, U32, U64, C.c_int, P, P]),
+    "rb_raygen_diffuse": (C.c_int, [P, P, P, U32, U32, U64, P, P]),
+}
+
+_L = None
+
+
+def lib():
+    """Load librb.so.  Raises if it has not been built: there is no fallback."""
+    global _L
+    if _L is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                               "(make -C renderbox2_amd/csrc)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name, None)
+            if f is None:
+                continue
+            f.restype = res
+            f.argtypes = args
+        _L = L
+    return _L
+
+
+def last_error():
+    buf = C.create_string_buffer(2048)
+    lib().rb_last_error(buf, len(buf))
+    return buf.value.decode(errors="replace")
+
+
+def check(status):
+    if status != RB_OK:
+        raise RbError(status, last_error())
+    return status
+
+
+def ptr(a):
+    """Host pointer of a numpy array (or None)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(P)
+
+
+def exported_symbols():
+    return list(_SIGS)

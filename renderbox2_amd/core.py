@@ -1,0 +1,405 @@
+"""Host-side mirror of renderbox2's scene / BVH / tracer interface over the C-ABI.
+
+Class and method names follow the reference (paths relative to renderbox2/):
+  Scene            src/core/scene.h:55-199
+  Platform         src/accelerators/sbvh/platform.h:78-135
+  BVH, BuildParams src/accelerators/sbvh/bvh.h:75-133
+  CudaBvh          src/accelerators/sbvh/cudabvh.h:64-120
+  CudaTracer       src/accelerators/sbvh/cudatracer.h:63-117
+  PerspectiveCamera src/core/camera.h:39-91
+Every computation runs in librb.so (host C++ for the SBVH build, HIP kernels
+for everything per ray); this module only moves arguments.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, lib, ptr
+
+RAY_DTYPE = np.dtype([("ox", "<f4"), ("oy", "<f4"), ("oz", "<f4"), ("tmin", "<f4"),
+                      ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("tmax", "<f4")])
+HIT_DTYPE = np.dtype([("id", "<i4"), ("t", "<f4")])
+FLT_MAX = np.float32(3.4028234663852886e38)
+
+
+def device_init(device=0):
+    check(lib().rb_device_init(device))
+
+
+class DeviceBuffer:
+    """A device allocation (MemoryAllocator::allocate, src/memory/memoryallocator.cpp:54-67)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().rb_malloc(C.byref(p), self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        check(lib().rb_memcpy_htod(b.ptr, ptr(a), a.nbytes, stream))
+        return b
+
+    def upload(self, a, stream=None):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        check(lib().rb_memcpy_htod(self.ptr, ptr(a), a.nbytes, stream))
+
+    def download(self, dtype, count=None, stream=None):
+        dtype = np.dtype(dtype)
+        count = self.nbytes // dtype.itemsize if count is None else count
+        out = np.empty(count, dtype)
+        check(lib().rb_memcpy_dtoh(ptr(out), self.ptr, out.nbytes, stream))
+        return out
+
+    def zero(self, stream=None):
+        check(lib().rb_memset(self.ptr, 0, self.nbytes, stream))
+
+    def free(self):
+        if self.ptr:
+            check(lib().rb_free(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "ptr", None):
+                lib().rb_free(self.ptr)
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = C.c_void_p()
+        check(lib().rb_stream_create(C.byref(s)))
+        self.handle = s.value
+
+    def synchronize(self):
+        check(lib().rb_stream_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().rb_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        check(lib().rb_event_create(C.byref(e)))
+        self.handle = e.value
+
+    def record(self, stream=None):
+        check(lib().rb_event_record(self.handle, stream.handle if isinstance(stream, Stream) else stream))
+
+    def elapsed_ms(self, later):
+        ms = C.c_float()
+        check(lib().rb_event_elapsed_ms(self.handle, later.handle, C.byref(ms)))
+        return ms.value
+
+    def synchronize(self):
+        check(lib().rb_event_synchronize(self.handle))
+
+    def __del__(self):
+        try:
+            if getattr(self, "handle", None):
+                lib().rb_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Scene:
+    """Host scene: vertices, triangles, materials, lights (Scene, src/core/scene.h)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().rb_scene_create(C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().rb_scene_destroy(self._h)
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def add_material(self, kind, color):
+        m = L.rb_material(kind, (C.c_float * 3)(*color))
+        mid = C.c_uint32()
+        check(lib().rb_scene_add_material(self._h, C.byref(m), C.byref(mid)))
+        return mid.value
+
+    # Scene::add_diffuse_bsdf / add_mirror_bsdf / add_diffuse_light_bsdf
+    def add_diffuse_bsdf(self, albedo):
+        return self.add_material(L.RB_MAT_LAMBERT, albedo)
+
+    def add_mirror_bsdf(self, color=(1.0, 1.0, 1.0)):
+        return self.add_material(L.RB_MAT_MIRROR, color)
+
+    def add_diffuse_light_bsdf(self, radiance):
+        return self.add_material(L.RB_MAT_EMITTER, radiance)
+
+    def add_mesh(self, verts, idx, material_id):
+        v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+        i = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+        check(lib().rb_scene_add_mesh(self._h, ptr(v), len(v), ptr(i), len(i), material_id))
+
+    def generate(self, kind, count=0, subdiv=0, seed=0, r=(0.0, 0.0), c=(0.0, 0.0), material_id=-1,
+                 mix_materials=False, emit=(0.0, 0.0, 0.0), subdiv_hi=0, hi_count=0):
+        p = L.rb_gen_params(kind, count, subdiv, seed, r[0], r[1], c[0], c[1], material_id,
+                            int(mix_materials), (C.c_float * 3)(*emit), subdiv_hi, hi_count)
+        check(lib().rb_scene_generate(self._h, C.byref(p)))
+
+    def counts(self):
+        nv, nt, nl, nm = C.c_uint32(), C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().rb_scene_counts(self._h, C.byref(nv), C.byref(nt), C.byref(nl), C.byref(nm)))
+        return nv.value, nt.value, nl.value, nm.value
+
+    def get_num_triangles(self):
+        return self.counts()[1]
+
+    def get_num_lights(self):
+        return self.counts()[2]
+
+    def geometry(self):
+        """(verts f32[nv,3], idx u32[nt,3], tri_material u32[nt], light_tris u32[nl])"""
+        nv, nt, nl, _ = self.counts()
+        v = np.empty((nv, 3), np.float32)
+        i = np.empty((nt, 3), np.uint32)
+        m = np.empty(nt, np.uint32)
+        lt = np.empty(nl, np.uint32)
+        check(lib().rb_scene_geometry(self._h, ptr(v), ptr(i), ptr(m), ptr(lt)))
+        return v, i, m, lt
+
+    def materials(self):
+        nm = self.counts()[3]
+        arr = (L.rb_material * nm)()
+        if nm:
+            check(lib().rb_scene_materials(self._h, arr))
+        return [(m.kind, tuple(m.color)) for m in arr]
+
+    def get_scene_aabb(self):
+        lo = np.empty(3, np.float32)
+        hi = np.empty(3, np.float32)
+        check(lib().rb_scene_bounds(self._h, ptr(lo), ptr(hi)))
+        return lo, hi
+
+
+class Platform:
+    """SAH costs and leaf preferences (Platform, src/accelerators/sbvh/platform.h:78-135)."""
+
+    def __init__(self, node_cost=1.0, tri_cost=1.0):
+        self.node_cost = node_cost
+        self.tri_cost = tri_cost
+        self.min_leaf = 1
+        self.max_leaf = 0x7FFFFFF
+
+    def set_leaf_preferences(self, min_size, max_size):
+        self.min_leaf, self.max_leaf = min_size, max_size
+
+
+class BuildParams:
+    def __init__(self, split_alpha=1.0e-5, threads=0):
+        self.split_alpha = split_alpha
+        self.threads = threads
+
+
+class BVH:
+    """SBVH built on the host (BVH::BVH, src/accelerators/sbvh/splitbvhbuilder.cpp:177-214)."""
+
+    def __init__(self, scene_or_arrays, platform=None, params=None):
+        platform = platform or Platform()
+        params = params or BuildParams()
+        bp = L.rb_build_params(platform.min_leaf, platform.max_leaf, params.split_alpha,
+                               platform.node_cost, platform.tri_cost, params.threads)
+        h = C.c_void_p()
+        if isinstance(scene_or_arrays, Scene):
+            check(lib().rb_bvh_build_host(scene_or_arrays.handle, C.byref(bp), C.byref(h)))
+        else:
+            v, i = scene_or_arrays
+            v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+            i = np.ascontiguousarray(i, np.uint32).reshape(-1, 3)
+            check(lib().rb_bvh_build_arrays(ptr(v), len(v), ptr(i), len(i), C.byref(bp), C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().rb_bvh_host_release(self._h)
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def stats(self):
+        s = L.rb_bvh_stats()
+        check(lib().rb_bvh_host_stats(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in L.rb_bvh_stats._fields_}
+
+    def export(self, layout):
+        nb, tb, ib = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        check(lib().rb_bvh_export(self._h, layout, None, C.byref(nb), None, C.byref(tb), None, C.byref(ib)))
+        n = np.empty(nb.value // 4, np.uint32)
+        t = np.empty(tb.value // 4, np.uint32)
+        i = np.empty(ib.value // 4, np.uint32)
+        check(lib().rb_bvh_export(self._h, layout, ptr(n), C.byref(nb), ptr(t), C.byref(tb), ptr(i), C.byref(ib)))
+        return n, t, i
+
+    def save_text(self, path):
+        check(lib().rb_bvh_save_text(self._h, path.encode()))
+
+
+class CudaBvh:
+    """Layout export (CudaBvh(const BVH&, BVHLayout), src/accelerators/sbvh/cudabvh.h:80-97)."""
+
+    def __init__(self, bvh, layout=L.RB_LAYOUT_REF_COMPACT2):
+        self.layout = layout
+        n, t, i = bvh.export(layout)
+        self.nodes = n.reshape(-1, 4)
+        self.tris = t.reshape(-1, 4)
+        self.index = i
+
+    def get_node_buffer(self):
+        return self.nodes
+
+    def get_tri_woop_buffer(self):
+        return self.tris
+
+    def get_tri_index_buffer(self):
+        return self.index
+
+    @staticmethod
+    def load_text(path):
+        nb, tb, ib = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        check(lib().rb_bvh_load_text_compact2(path.encode(), None, C.byref(nb), None, C.byref(tb), None, C.byref(ib)))
+        n = np.empty(nb.value // 4, np.uint32)
+        t = np.empty(tb.value // 4, np.uint32)
+        i = np.empty(ib.value // 4, np.uint32)
+        check(lib().rb_bvh_load_text_compact2(path.encode(), ptr(n), C.byref(nb), ptr(t), C.byref(tb), ptr(i),
+                                              C.byref(ib)))
+        return n.reshape(-1, 4), t.reshape(-1, 4), i
+
+
+class CudaTracer:
+    """Device-resident BVH + scene and the batched trace call
+    (CudaTracer, src/accelerators/sbvh/cudatracer.cu:43-183)."""
+
+    def __init__(self, bvh, scene=None):
+        h = C.c_void_p()
+        check(lib().rb_bvh_upload(bvh.handle, scene.handle if scene is not None else None, C.byref(h)))
+        self._h = h.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_h", None):
+                lib().rb_bvh_release(self._h)
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_bytes(self):
+        b = C.c_size_t()
+        check(lib().rb_bvh_device_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def trace_device(self, d_rays, n, d_hits, any_hit=False, d_mask=None, stream=None):
+        flags = (L.RB_ANY if any_hit else L.RB_CLOSEST) | (L.RB_MASKED if d_mask is not None else 0)
+        check(lib().rb_trace(self._h, _dp(d_rays), n, flags, _dp(d_mask), _dp(d_hits), _sp(stream)))
+
+    def trace(self, rays, any_hit=False, mask=None, stream=None):
+        """Host convenience: upload rays (n x 8 float32 or RAY_DTYPE), trace, download (ids, t)."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        if n == 0:
+            return np.empty(0, np.int32), np.empty(0, np.float32)
+        dr = DeviceBuffer.from_array(rays, _sp(stream))
+        dh = DeviceBuffer(n * 8)
+        dm = DeviceBuffer.from_array(np.ascontiguousarray(mask, np.uint8), _sp(stream)) if mask is not None else None
+        self.trace_device(dr, n, dh, any_hit, dm, stream)
+        h = dh.download(HIT_DTYPE, n, _sp(stream))
+        return h["id"].copy(), h["t"].copy()
+
+    def trace_counts(self, rays, any_hit=False):
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        dr = DeviceBuffer.from_array(rays)
+        dh = DeviceBuffer(n * 8)
+        dc = DeviceBuffer(n * 12)
+        check(lib().rb_trace_counts(self._h, dr.ptr, n, L.RB_ANY if any_hit else L.RB_CLOSEST, dh.ptr, dc.ptr, None))
+        h = dh.download(HIT_DTYPE, n)
+        c = dc.download(np.uint32, 3 * n).reshape(n, 3)
+        return h["id"].copy(), h["t"].copy(), c
+
+
+def _dp(b):
+    if b is None:
+        return None
+    return b.ptr if isinstance(b, DeviceBuffer) else b
+
+
+def _sp(s):
+    if s is None:
+        return None
+    return s.handle if isinstance(s, Stream) else s
+
+
+def get_trace_config():
+    c = L.rb_trace_config()
+    check(lib().rb_trace_get_config(C.byref(c)))
+    return {k: getattr(c, k) for k, _ in L.rb_trace_config._fields_}
+
+
+def set_trace_config(**kw):
+    cur = get_trace_config()
+    cur.update(kw)
+    c = L.rb_trace_config(**cur)
+    check(lib().rb_trace_set_config(C.byref(c)))
+
+
+class PerspectiveCamera:
+    """Pinhole camera (PerspectiveCamera, src/core/camera.h:48-71)."""
+
+    def __init__(self, position, lookat, up, resolution, horizontal_fov):
+        self.c = L.rb_camera((C.c_float * 3)(*position), (C.c_float * 3)(*lookat), (C.c_float * 3)(*up),
+                             float(horizontal_fov), int(resolution[0]), int(resolution[1]))
+
+    @property
+    def width(self):
+        return self.c.width
+
+    @property
+    def height(self):
+        return self.c.height
+
+    def get_total_pixels(self):
+        return self.c.width * self.c.height
+
+    def matrices(self):
+        a = np.empty(16, np.float32)
+        b = np.empty(16, np.float32)
+        check(lib().rb_camera_matrices(C.byref(self.c), ptr(a), ptr(b)))
+        return a.reshape(4, 4), b.reshape(4, 4)
+
+    def generate_rays_device(self, d_rays, spp=1, seed=0, jitter=True, tile=None, stream=None):
+        x0, y0, w, h = tile if tile else (0, 0, self.c.width, self.c.height)
+        check(lib().rb_raygen_camera(C.byref(self.c), x0, y0, w, h, spp, seed, int(jitter), _dp(d_rays), _sp(stream)))
+        return w * h * spp
+
+
+def raygen_diffuse_device(tracer, d_prim, d_hits, n_primary, n_out, seed, d_out, stream=None):
+    check(lib().rb_raygen_diffuse(tracer.handle, _dp(d_prim), _dp(d_hits), n_primary, n_out, seed, _dp(d_out),
+                                  _sp(stream)))

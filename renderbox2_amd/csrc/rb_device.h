@@ -1,0 +1,53 @@
+// rb_device.h -- device-side structures shared by the HIP sources.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "rb_internal.h"
+
+namespace rb {
+
+// Work-counter slots: each launch of a persistent kernel takes one slot
+// (work counter + exit counter); the last block out resets both to zero, so
+// no memset node is needed between launches or inside captured graphs.
+constexpr uint32_t kCounterSlots = 4096;
+
+// Traversal stack bound: the builder stops at depth 64 and leaves of more than
+// 31 references are split by the exporter, so 96 entries never overflow.
+constexpr int kMaxStack = 96;
+constexpr int kSentinel = 0x76543210;
+
+struct DeviceState {
+    int device = -1;
+    uint32_t num_cus = 0;
+    hipStream_t stream = nullptr;  // library default stream
+    uint32_t* counters = nullptr;  // 2 * kCounterSlots
+    std::atomic<uint32_t> next_slot{0};
+    std::atomic<size_t> library_bytes{0};
+    rb_trace_config cfg{};
+};
+DeviceState& dev();
+rb_status ensure_device();
+uint32_t* take_counter_slot();
+rb_status hip_fail(hipError_t e, const char* what);
+
+// Device allocation tracked against the library's HBM accounting
+// (MemoryAllocator::allocate, src/memory/memoryallocator.cpp:54-67).
+rb_status dalloc(void** p, size_t bytes);
+void dfree(void* p, size_t bytes);
+
+}  // namespace rb
+
+// Device-resident BVH + optional shading data.
+struct rb_bvh_s {
+    float4* nodes = nullptr;    // 4 float4 per inner node (MI355X layout)
+    float4* tris = nullptr;     // 3 float4 Woop rows per triangle reference
+    uint32_t* index = nullptr;  // original triangle id per reference
+    uint32_t n_nodes = 0, n_refs = 0;
+    // shading data (Scene::gpu_prepare, src/core/scene.cpp:166-301)
+    float4* tri_v = nullptr;          // 3 float4 per triangle: v0.xyz|mat, v1.xyz|0, v2.xyz|0
+    rb_material* mats = nullptr;
+    uint32_t* light_tris = nullptr;
+    uint32_t n_tris = 0, n_mats = 0, n_lights = 0;
+    float scene_lo[3] = {0, 0, 0}, scene_hi[3] = {0, 0, 0};
+    size_t bytes = 0;
+};

@@ -1,0 +1,497 @@
+// rb_host_api.cpp -- host half of the C-ABI: errors, scenes, seeded scene
+// generators, SBVH build/export and the reference's text BVH format.
+#include <cstdarg>
+#include <cstdio>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <sstream>
+
+#include "rb_internal.h"
+#include "rb_rng.h"
+
+namespace rb {
+
+static thread_local std::string g_err;
+
+void set_error(const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+rb_status fail(rb_status s, const char* fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return s;
+}
+
+}  // namespace rb
+
+using namespace rb;
+
+#define RB_CHECK_ARG(c, msg)                            \
+    do {                                                \
+        if (!(c)) return fail(RB_ERR_INVALID, "%s", msg); \
+    } while (0)
+
+extern "C" {
+
+size_t rb_last_error(char* buf, size_t n) {
+    if (buf && n) {
+        size_t k = g_err.size() < n - 1 ? g_err.size() : n - 1;
+        std::memcpy(buf, g_err.data(), k);
+        buf[k] = 0;
+    }
+    return g_err.size();
+}
+
+const char* rb_status_string(rb_status s) {
+    switch (s) {
+        case RB_OK: return "RB_OK";
+        case RB_ERR_INVALID: return "RB_ERR_INVALID";
+        case RB_ERR_OOM: return "RB_ERR_OOM";
+        case RB_ERR_DEVICE: return "RB_ERR_DEVICE";
+        case RB_ERR_BUSY: return "RB_ERR_BUSY";
+        case RB_ERR_UNSUPPORTED: return "RB_ERR_UNSUPPORTED";
+        case RB_ERR_NOT_READY: return "RB_ERR_NOT_READY";
+    }
+    return "RB_ERR_?";
+}
+
+const char* rb_version(void) { return "renderbox2_amd 0.1 (gfx950)"; }
+
+// ------------------------------------------------------------------ scene --
+
+rb_status rb_scene_create(rb_scene_t* out) {
+    RB_CHECK_ARG(out, "out is NULL");
+    try {
+        *out = new rb_scene_s();
+    } catch (...) {
+        return fail(RB_ERR_OOM, "scene allocation failed");
+    }
+    return RB_OK;
+}
+
+rb_status rb_scene_destroy(rb_scene_t s) {
+    delete s;
+    return RB_OK;
+}
+
+rb_status rb_scene_add_material(rb_scene_t s, const rb_material* m, uint32_t* id) {
+    RB_CHECK_ARG(s && m, "NULL scene or material");
+    RB_CHECK_ARG(m->kind <= RB_MAT_EMITTER, "unknown material kind");
+    s->s.mats.push_back(*m);
+    if (id) *id = (uint32_t)(s->s.mats.size() - 1);
+    return RB_OK;
+}
+
+rb_status rb_scene_add_mesh(rb_scene_t s, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                            uint32_t material_id) {
+    RB_CHECK_ARG(s, "NULL scene");
+    RB_CHECK_ARG(nt == 0 || (verts && idx), "NULL mesh arrays");
+    RB_CHECK_ARG(material_id < s->s.mats.size(), "material id out of range (add the material first)");
+    for (uint64_t i = 0; i < 3ull * nt; i++)
+        if (idx[i] >= nv) return fail(RB_ERR_INVALID, "mesh triangle %llu indexes vertex %u of %u",
+                                      (unsigned long long)(i / 3), idx[i], nv);
+    Scene& S = s->s;
+    const uint32_t base = S.num_verts();
+    const uint32_t tbase = S.num_tris();
+    try {
+        S.verts.insert(S.verts.end(), verts, verts + 3 * (size_t)nv);
+        for (uint64_t i = 0; i < 3ull * nt; i++) S.idx.push_back(idx[i] + base);
+        S.tri_mat.insert(S.tri_mat.end(), nt, material_id);
+        if (S.mats[material_id].kind == RB_MAT_EMITTER)
+            for (uint32_t t = 0; t < nt; t++) S.light_tris.push_back(tbase + t);
+    } catch (...) {
+        return fail(RB_ERR_OOM, "scene growth failed");
+    }
+    return RB_OK;
+}
+
+rb_status rb_scene_counts(rb_scene_t s, uint32_t* nv, uint32_t* nt, uint32_t* nl, uint32_t* nm) {
+    RB_CHECK_ARG(s, "NULL scene");
+    if (nv) *nv = s->s.num_verts();
+    if (nt) *nt = s->s.num_tris();
+    if (nl) *nl = (uint32_t)s->s.light_tris.size();
+    if (nm) *nm = (uint32_t)s->s.mats.size();
+    return RB_OK;
+}
+
+rb_status rb_scene_geometry(rb_scene_t s, float* verts, uint32_t* idx, uint32_t* tri_mat, uint32_t* lights) {
+    RB_CHECK_ARG(s, "NULL scene");
+    const Scene& S = s->s;
+    if (verts) std::memcpy(verts, S.verts.data(), S.verts.size() * 4);
+    if (idx) std::memcpy(idx, S.idx.data(), S.idx.size() * 4);
+    if (tri_mat) std::memcpy(tri_mat, S.tri_mat.data(), S.tri_mat.size() * 4);
+    if (lights) std::memcpy(lights, S.light_tris.data(), S.light_tris.size() * 4);
+    return RB_OK;
+}
+
+rb_status rb_scene_materials(rb_scene_t s, rb_material* out) {
+    RB_CHECK_ARG(s && out, "NULL argument");
+    std::memcpy(out, s->s.mats.data(), s->s.mats.size() * sizeof(rb_material));
+    return RB_OK;
+}
+
+rb_status rb_scene_bounds(rb_scene_t s, float lo[3], float hi[3]) {
+    RB_CHECK_ARG(s && lo && hi, "NULL argument");
+    Box b;
+    const Scene& S = s->s;
+    for (uint32_t i = 0; i < S.num_verts(); i++) b.insert(f3(S.verts[3 * i], S.verts[3 * i + 1], S.verts[3 * i + 2]));
+    lo[0] = b.lo.x; lo[1] = b.lo.y; lo[2] = b.lo.z;
+    hi[0] = b.hi.x; hi[1] = b.hi.y; hi[2] = b.hi.z;
+    return RB_OK;
+}
+
+}  // extern "C"
+
+// --------------------------------------------------------------- generators --
+
+namespace {
+
+struct Mesh {
+    std::vector<double> v;  // xyz
+    std::vector<uint32_t> t;
+};
+
+// Icosahedron subdivided `s` times, vertices on the unit sphere; 20*4^s
+// triangles wound counter-clockwise seen from outside.
+const Mesh& icosphere(uint32_t s) {
+    static std::mutex mu;
+    static std::map<uint32_t, Mesh> cache;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(s);
+    if (it != cache.end()) return it->second;
+    Mesh m;
+    const double p = (1.0 + std::sqrt(5.0)) / 2.0;
+    const double base[12][3] = {{-1, p, 0}, {1, p, 0}, {-1, -p, 0}, {1, -p, 0}, {0, -1, p}, {0, 1, p},
+                                {0, -1, -p}, {0, 1, -p}, {p, 0, -1}, {p, 0, 1}, {-p, 0, -1}, {-p, 0, 1}};
+    const uint32_t faces[20][3] = {{0, 11, 5}, {0, 5, 1}, {0, 1, 7}, {0, 7, 10}, {0, 10, 11}, {1, 5, 9}, {5, 11, 4},
+                                   {11, 10, 2}, {10, 7, 6}, {7, 1, 8}, {3, 9, 4}, {3, 4, 2}, {3, 2, 6}, {3, 6, 8},
+                                   {3, 8, 9}, {4, 9, 5}, {2, 4, 11}, {6, 2, 10}, {8, 6, 7}, {9, 8, 1}};
+    auto push = [&](double x, double y, double z) {
+        double l = std::sqrt(x * x + y * y + z * z);
+        m.v.push_back(x / l);
+        m.v.push_back(y / l);
+        m.v.push_back(z / l);
+        return (uint32_t)(m.v.size() / 3 - 1);
+    };
+    for (auto& b : base) push(b[0], b[1], b[2]);
+    for (auto& f : faces) m.t.insert(m.t.end(), f, f + 3);
+    for (uint32_t l = 0; l < s; l++) {
+        std::map<uint64_t, uint32_t> mid;
+        auto midpoint = [&](uint32_t a, uint32_t b) {
+            uint64_t key = a < b ? ((uint64_t)a << 32 | b) : ((uint64_t)b << 32 | a);
+            auto f = mid.find(key);
+            if (f != mid.end()) return f->second;
+            uint32_t r = push((m.v[3 * a] + m.v[3 * b]) * 0.5, (m.v[3 * a + 1] + m.v[3 * b + 1]) * 0.5,
+                              (m.v[3 * a + 2] + m.v[3 * b + 2]) * 0.5);
+            mid[key] = r;
+            return r;
+        };
+        std::vector<uint32_t> nt;
+        for (size_t i = 0; i < m.t.size(); i += 3) {
+            uint32_t a = m.t[i], b = m.t[i + 1], c = m.t[i + 2];
+            uint32_t ab = midpoint(a, b), bc = midpoint(b, c), ca = midpoint(c, a);
+            uint32_t add[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+            nt.insert(nt.end(), add, add + 12);
+        }
+        m.t.swap(nt);
+    }
+    return cache.emplace(s, std::move(m)).first->second;
+}
+
+struct Gen {
+    rb::Rng rng;
+    uint32_t ctr = 0;
+    rb::U4 buf{};
+    int left = 0;
+    explicit Gen(uint64_t seed, uint32_t kind) { rng = rb::Rng::make(seed, rb::RNG_SCENE + 16 * kind); }
+    double uni() {  // [0,1) with 24 bits
+        if (left == 0) {
+            buf = rng.draw(ctr++, 0, 0, 0);
+            left = 4;
+        }
+        uint32_t v = left == 4 ? buf.x : left == 3 ? buf.y : left == 2 ? buf.z : buf.w;
+        left--;
+        return (double)(v >> 8) * (1.0 / 16777216.0);
+    }
+    double range(double a, double b) { return a + (b - a) * uni(); }
+};
+
+rb_status add_material_mix(rb_scene_t s, Gen& g, const rb_gen_params* p, uint32_t* mat) {
+    if (p->material_id >= 0) {
+        *mat = (uint32_t)p->material_id;
+        return RB_OK;
+    }
+    rb_material m{};
+    if (p->mix_materials && g.uni() >= 0.7) {
+        m.kind = RB_MAT_MIRROR;
+        m.color[0] = m.color[1] = m.color[2] = 0.9f;
+    } else {
+        m.kind = RB_MAT_LAMBERT;
+        for (int i = 0; i < 3; i++) m.color[i] = (float)g.range(0.2, 0.8);
+    }
+    return rb_scene_add_material(s, &m, mat);
+}
+
+}  // namespace
+
+extern "C" rb_status rb_scene_generate(rb_scene_t s, const rb_gen_params* p) {
+    RB_CHECK_ARG(s && p, "NULL argument");
+    Gen g(p->seed, p->kind);
+    if (p->material_id >= (int32_t)s->s.mats.size()) return fail(RB_ERR_INVALID, "material id out of range");
+    rb_status st;
+    std::vector<float> v;
+    std::vector<uint32_t> t;
+    uint32_t mat = 0;
+    switch (p->kind) {
+        case RB_GEN_ICOSPHERES: {
+            RB_CHECK_ARG(p->subdiv <= 7 && p->subdiv_hi <= 7, "subdivision too deep");
+            // which spheres use the finer subdivision: a seeded choice of hi_count
+            std::vector<uint8_t> hi(p->count, 0);
+            if (p->subdiv_hi > p->subdiv && p->hi_count) {
+                uint32_t k = 0;
+                while (k < p->hi_count && k < p->count) {
+                    uint32_t j = (uint32_t)(g.uni() * p->count);
+                    if (j < p->count && !hi[j]) { hi[j] = 1; k++; }
+                }
+            }
+            for (uint32_t i = 0; i < p->count; i++) {
+                const Mesh& m = icosphere(hi[i] ? p->subdiv_hi : p->subdiv);
+                double r = g.range(p->r0, p->r1);
+                double c[3] = {g.range(p->c0, p->c1), g.range(p->c0, p->c1), g.range(p->c0, p->c1)};
+                if ((st = add_material_mix(s, g, p, &mat)) != RB_OK) return st;
+                v.resize(m.v.size());
+                for (size_t k = 0; k < m.v.size(); k++) v[k] = (float)(c[k % 3] + r * m.v[k]);
+                if ((st = rb_scene_add_mesh(s, v.data(), (uint32_t)(v.size() / 3), m.t.data(),
+                                            (uint32_t)(m.t.size() / 3), mat)) != RB_OK)
+                    return st;
+            }
+            return RB_OK;
+        }
+        case RB_GEN_THIN_QUADS: {
+            for (uint32_t i = 0; i < p->count; i++) {
+                double L = g.range(p->r0, p->r1), W = L / 50.0;
+                double c[3] = {g.range(p->c0, p->c1), g.range(p->c0, p->c1), g.range(p->c0, p->c1)};
+                // uniform rotation (Shoemake's unit quaternion)
+                double u1 = g.uni(), u2 = g.uni(), u3 = g.uni();
+                double a = std::sqrt(1 - u1), b = std::sqrt(u1);
+                double qx = a * std::sin(2 * M_PI * u2), qy = a * std::cos(2 * M_PI * u2);
+                double qz = b * std::sin(2 * M_PI * u3), qw = b * std::cos(2 * M_PI * u3);
+                double R[3][3] = {{1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qz * qw), 2 * (qx * qz + qy * qw)},
+                                  {2 * (qx * qy + qz * qw), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qx * qw)},
+                                  {2 * (qx * qz - qy * qw), 2 * (qy * qz + qx * qw), 1 - 2 * (qx * qx + qy * qy)}};
+                const double corner[4][2] = {{-0.5, -0.5}, {0.5, -0.5}, {0.5, 0.5}, {-0.5, 0.5}};
+                v.resize(12);
+                for (int k = 0; k < 4; k++) {
+                    double lx = corner[k][0] * L, ly = corner[k][1] * W;
+                    for (int d = 0; d < 3; d++) v[3 * k + d] = (float)(c[d] + R[d][0] * lx + R[d][1] * ly);
+                }
+                const uint32_t qt[6] = {0, 1, 2, 0, 2, 3};
+                if ((st = add_material_mix(s, g, p, &mat)) != RB_OK) return st;
+                if ((st = rb_scene_add_mesh(s, v.data(), 4, qt, 2, mat)) != RB_OK) return st;
+            }
+            return RB_OK;
+        }
+        case RB_GEN_ROOM: {
+            // closed box [c0,c1]^3, normals pointing inward; floor and
+            // ceiling 16x16 quads, walls 16x8 quads: 2048 triangles.
+            const double lo = p->c0, hi = p->c1;
+            struct Face { int axis; double at; int sgn; int nu, nv; };
+            const Face faces[6] = {{1, lo, +1, 16, 16}, {1, hi, -1, 16, 16}, {0, lo, +1, 16, 8},
+                                   {0, hi, -1, 16, 8},  {2, lo, +1, 16, 8},  {2, hi, -1, 16, 8}};
+            if ((st = add_material_mix(s, g, p, &mat)) != RB_OK) return st;
+            for (const Face& f : faces) {
+                int ua = (f.axis + 1) % 3, va = (f.axis + 2) % 3;
+                v.clear();
+                t.clear();
+                for (int j = 0; j <= f.nv; j++)
+                    for (int i = 0; i <= f.nu; i++) {
+                        double x[3];
+                        x[f.axis] = f.at;
+                        x[ua] = lo + (hi - lo) * i / f.nu;
+                        x[va] = lo + (hi - lo) * j / f.nv;
+                        for (int d = 0; d < 3; d++) v.push_back((float)x[d]);
+                    }
+                for (int j = 0; j < f.nv; j++)
+                    for (int i = 0; i < f.nu; i++) {
+                        uint32_t a = j * (f.nu + 1) + i, b = a + 1, c = a + f.nu + 1, d = c + 1;
+                        // (ua x va) = +axis; flip winding for a negative inward normal
+                        if (f.sgn > 0) {
+                            uint32_t add[6] = {a, b, d, a, d, c};
+                            t.insert(t.end(), add, add + 6);
+                        } else {
+                            uint32_t add[6] = {a, d, b, a, c, d};
+                            t.insert(t.end(), add, add + 6);
+                        }
+                    }
+                if ((st = rb_scene_add_mesh(s, v.data(), (uint32_t)(v.size() / 3), t.data(), (uint32_t)(t.size() / 3),
+                                            mat)) != RB_OK)
+                    return st;
+            }
+            return RB_OK;
+        }
+        case RB_GEN_EMITTER_QUADS: {
+            rb_material m{};
+            m.kind = RB_MAT_EMITTER;
+            for (int i = 0; i < 3; i++) m.color[i] = p->emit[i];
+            if ((st = rb_scene_add_material(s, &m, &mat)) != RB_OK) return st;
+            const double side = std::sqrt((double)p->r0);
+            for (uint32_t i = 0; i < p->count; i++) {
+                double cx = g.range(p->c0 + side, p->c1 - side), cz = g.range(p->c0 + side, p->c1 - side);
+                double y = p->c1 - 0.01;
+                float q[12] = {(float)(cx - side / 2), (float)y, (float)(cz - side / 2),
+                               (float)(cx + side / 2), (float)y, (float)(cz - side / 2),
+                               (float)(cx + side / 2), (float)y, (float)(cz + side / 2),
+                               (float)(cx - side / 2), (float)y, (float)(cz + side / 2)};
+                // facing down (-y): (v1-v0) x (v2-v0) must point to -y
+                const uint32_t qt[6] = {0, 2, 1, 0, 3, 2};
+                if ((st = rb_scene_add_mesh(s, q, 4, qt, 2, mat)) != RB_OK) return st;
+            }
+            return RB_OK;
+        }
+    }
+    return fail(RB_ERR_INVALID, "unknown generator kind %u", p->kind);
+}
+
+// -------------------------------------------------------------------- BVH --
+
+extern "C" {
+
+void rb_build_params_default(rb_build_params* p) {
+    if (!p) return;
+    p->min_leaf = 1;
+    p->max_leaf = 0x7FFFFFF;
+    p->split_alpha = 1.0e-5f;
+    p->node_cost = 1.0f;
+    p->tri_cost = 1.0f;
+    p->threads = 0;
+}
+
+rb_status rb_bvh_build_arrays(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                              const rb_build_params* p, rb_host_bvh_t* out) {
+    RB_CHECK_ARG(out, "out is NULL");
+    RB_CHECK_ARG(nt > 0 && verts && idx, "empty geometry");
+    rb_build_params d;
+    rb_build_params_default(&d);
+    if (!p) p = &d;
+    RB_CHECK_ARG(p->max_leaf >= 1 && p->min_leaf <= p->max_leaf, "bad leaf preferences");
+    rb_status st = RB_OK;
+    HostBvh* h = nullptr;
+    try {
+        h = build_sbvh(verts, nv, idx, nt, *p, &st);
+    } catch (...) {
+        return fail(RB_ERR_OOM, "SBVH build failed (host allocation)");
+    }
+    if (!h) return st;
+    *out = new rb_host_bvh_s{h};
+    return RB_OK;
+}
+
+rb_status rb_bvh_build_host(rb_scene_t s, const rb_build_params* p, rb_host_bvh_t* out) {
+    RB_CHECK_ARG(s, "NULL scene");
+    return rb_bvh_build_arrays(s->s.verts.data(), s->s.num_verts(), s->s.idx.data(), s->s.num_tris(), p, out);
+}
+
+rb_status rb_bvh_host_stats(rb_host_bvh_t b, rb_bvh_stats* out) {
+    RB_CHECK_ARG(b && out, "NULL argument");
+    *out = b->b->stats;
+    return RB_OK;
+}
+
+rb_status rb_bvh_host_release(rb_host_bvh_t b) {
+    if (b) {
+        delete b->b;
+        delete b;
+    }
+    return RB_OK;
+}
+
+static rb_status copy_out(const void* src, size_t bytes, void* dst, size_t* cap) {
+    if (!cap) return RB_OK;
+    if (dst) {
+        if (*cap < bytes) return fail(RB_ERR_INVALID, "output buffer too small (%zu < %zu bytes)", *cap, bytes);
+        std::memcpy(dst, src, bytes);
+    }
+    *cap = bytes;
+    return RB_OK;
+}
+
+rb_status rb_bvh_export(rb_host_bvh_t b, uint32_t layout, void* nodes, size_t* nb, void* tris, size_t* tb,
+                        void* index, size_t* ib) {
+    RB_CHECK_ARG(b, "NULL bvh");
+    try {
+        if (layout == RB_LAYOUT_REF_COMPACT2) {
+            std::vector<uint32_t> n, w, x;
+            export_compact2(*b->b, n, w, x);
+            rb_status st;
+            if ((st = copy_out(n.data(), n.size() * 4, nodes, nb)) != RB_OK) return st;
+            if ((st = copy_out(w.data(), w.size() * 4, tris, tb)) != RB_OK) return st;
+            return copy_out(x.data(), x.size() * 4, index, ib);
+        }
+        if (layout == RB_LAYOUT_MI355X) {
+            Mi355Layout L;
+            export_mi355x(*b->b, L);
+            rb_status st;
+            if ((st = copy_out(L.nodes.data(), L.nodes.size() * 4, nodes, nb)) != RB_OK) return st;
+            if ((st = copy_out(L.tris.data(), L.tris.size() * 4, tris, tb)) != RB_OK) return st;
+            return copy_out(L.index.data(), L.index.size() * 4, index, ib);
+        }
+    } catch (...) {
+        return fail(RB_ERR_OOM, "layout export failed (host allocation)");
+    }
+    return fail(RB_ERR_INVALID, "unknown layout %u", layout);
+}
+
+// CudaBvh::serialize (cudabvh.cpp:176-214): magic, three sizes, layout
+// (1 = BVHLayout_Compact2), decimal uint4 rows, decimal indices, EOF marker.
+rb_status rb_bvh_save_text(rb_host_bvh_t b, const char* path) {
+    RB_CHECK_ARG(b && path, "NULL argument");
+    std::vector<uint32_t> n, w, x;
+    export_compact2(*b->b, n, w, x);
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return fail(RB_ERR_INVALID, "cannot open %s", path);
+    out << 0x76543210u << "\n" << n.size() / 4 << "\n" << w.size() / 4 << "\n" << x.size() << "\n" << 1u << "\n";
+    for (size_t i = 0; i < n.size(); i += 4) out << n[i] << " " << n[i + 1] << " " << n[i + 2] << " " << n[i + 3] << "\n";
+    for (size_t i = 0; i < w.size(); i += 4) out << w[i] << " " << w[i + 1] << " " << w[i + 2] << " " << w[i + 3] << "\n";
+    for (size_t i = 0; i < x.size(); i++) out << x[i] << "\n";
+    out << 0x1234567u << "\n";
+    return out ? RB_OK : fail(RB_ERR_INVALID, "write to %s failed", path);
+}
+
+// CudaBvh(filename, ok) (cudabvh.cpp:108-165), with the layout check of
+// Application::prepare (src/core/application.cpp:154-161).
+rb_status rb_bvh_load_text_compact2(const char* path, void* nodes, size_t* nb, void* tris, size_t* tb, void* index,
+                                    size_t* ib) {
+    RB_CHECK_ARG(path, "NULL path");
+    std::ifstream in(path);
+    if (!in) return fail(RB_ERR_INVALID, "cannot open %s", path);
+    uint32_t magic = 0, layout = 0, eof = 0;
+    size_t nn = 0, nw = 0, ni = 0;
+    in >> magic;
+    if (magic != 0x76543210u) return fail(RB_ERR_INVALID, "%s: not a BVH file (magic %u)", path, magic);
+    in >> nn >> nw >> ni >> layout;
+    if (!in || layout != 1) return fail(RB_ERR_INVALID, "%s: layout %u is not Compact2", path, layout);
+    if (nn > (1ull << 32) || nw > (1ull << 32) || ni > (1ull << 32)) return fail(RB_ERR_INVALID, "%s: sizes", path);
+    std::vector<uint32_t> n(nn * 4), w(nw * 4), x(ni);
+    for (auto& e : n) in >> e;
+    for (auto& e : w) in >> e;
+    for (auto& e : x) in >> e;
+    in >> eof;
+    if (!in || eof != 0x1234567u) return fail(RB_ERR_INVALID, "%s: truncated or missing EOF marker", path);
+    rb_status st;
+    if ((st = copy_out(n.data(), n.size() * 4, nodes, nb)) != RB_OK) return st;
+    if ((st = copy_out(w.data(), w.size() * 4, tris, tb)) != RB_OK) return st;
+    return copy_out(x.data(), x.size() * 4, index, ib);
+}
+
+}  // extern "C"

@@ -1,0 +1,111 @@
+// rb_internal.h -- host-side structures shared by the librb.so sources.
+#pragma once
+
+#include <atomic>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/rb.h"
+
+namespace rb {
+
+// ---------------------------------------------------------------- errors --
+
+void set_error(const char* fmt, ...);
+rb_status fail(rb_status s, const char* fmt, ...);
+
+// ------------------------------------------------------------ host math --
+// Component-wise float3 operators with the rounding of the reference's
+// host build (helper_math.h semantics; this file is compiled with
+// -ffp-contract=off so every product and sum rounds separately).
+
+struct F3 {
+    float x, y, z;
+};
+inline F3 f3(float x, float y, float z) { return F3{x, y, z}; }
+inline F3 operator-(F3 a, F3 b) { return f3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline F3 operator+(F3 a, F3 b) { return f3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline F3 operator*(F3 a, float s) { return f3(a.x * s, a.y * s, a.z * s); }
+inline float dot(F3 a, F3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline F3 cross(F3 a, F3 b) { return f3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x); }
+inline F3 normalize(F3 v) { return v * (1.0f / std::sqrt(dot(v, v))); }
+inline F3 vmin(F3 a, F3 b) { return f3(std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)); }
+inline F3 vmax(F3 a, F3 b) { return f3(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)); }
+inline float get(const F3& v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+inline float& getr(F3& v, int d) { return d == 0 ? v.x : (d == 1 ? v.y : v.z); }
+
+inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+
+struct Box {
+    F3 lo{FLT_MAX, FLT_MAX, FLT_MAX};
+    F3 hi{-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    bool valid() const { return lo.x <= hi.x && lo.y <= hi.y && lo.z <= hi.z; }
+    void insert(F3 p) { lo = vmin(lo, p); hi = vmax(hi, p); }
+    void grow(const Box& b) { insert(b.lo); insert(b.hi); }
+    void intersect(const Box& b) { lo = vmax(lo, b.lo); hi = vmin(hi, b.hi); }
+    float area() const {
+        if (!valid()) return 0.0f;
+        F3 d = hi - lo;
+        return (d.x * d.y + d.y * d.z + d.z * d.x) * 2.0f;
+    }
+};
+
+// ---------------------------------------------------------------- scene --
+
+struct Scene {
+    std::vector<float> verts;          // xyz per vertex
+    std::vector<uint32_t> idx;         // 3 per triangle (scene-global vertex ids)
+    std::vector<uint32_t> tri_mat;     // material per triangle
+    std::vector<rb_material> mats;
+    std::vector<uint32_t> light_tris;  // triangle ids of emissive triangles
+    uint32_t num_tris() const { return (uint32_t)(idx.size() / 3); }
+    uint32_t num_verts() const { return (uint32_t)(verts.size() / 3); }
+};
+
+// ------------------------------------------------------------ host BVH --
+
+struct BNode {
+    Box b;
+    BNode* c[2] = {nullptr, nullptr};
+    uint32_t* tris = nullptr;  // leaf triangle ids, in Compact2 order
+    uint32_t ntris = 0;
+    bool leaf = false;
+};
+
+struct HostBvh {
+    BNode* root = nullptr;
+    std::vector<std::unique_ptr<std::vector<BNode>>> node_blocks;
+    std::vector<std::unique_ptr<std::vector<uint32_t>>> tri_blocks;
+    std::vector<float> verts;     // copy of the geometry the tree was built on
+    std::vector<uint32_t> idx;
+    rb_build_params params{};
+    rb_bvh_stats stats{};
+};
+
+HostBvh* build_sbvh(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                    const rb_build_params& p, rb_status* st);
+void woopify(const float* v0, const float* v1, const float* v2, float out[12]);
+
+// MI355X layout arrays (host copies).
+struct Mi355Layout {
+    std::vector<uint32_t> nodes;  // 16 words per node
+    std::vector<float> tris;      // 12 floats per triangle reference
+    std::vector<uint32_t> index;  // 1 per triangle reference
+};
+rb_status export_mi355x(const HostBvh& b, Mi355Layout& out);
+rb_status export_compact2(const HostBvh& b, std::vector<uint32_t>& nodes, std::vector<uint32_t>& woop,
+                          std::vector<uint32_t>& index);
+
+}  // namespace rb
+
+struct rb_scene_s {
+    rb::Scene s;
+};
+struct rb_host_bvh_s {
+    rb::HostBvh* b;
+};

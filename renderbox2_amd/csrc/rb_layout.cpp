@@ -1,0 +1,218 @@
+// rb_layout.cpp -- GPU node layouts of the host SBVH.
+//
+//  * Compact2: byte-compatible with CudaBvh::create_compact(bvh, 16)
+//    (renderbox2 src/accelerators/sbvh/cudabvh.cpp:216-296): kept so that
+//    BVH files and byte streams can be compared with the reference format.
+//  * MI355X: the traversal layout of this library (see export_mi355x).
+#include <cstdio>
+#include <fstream>
+
+#include "rb_internal.h"
+
+namespace rb {
+
+// Matrix4x4::invert (src/core/transform.h:205-333): cofactor expansion,
+// evaluated term by term from left to right.
+void invert4x4_ref(const float* m, float* out) {
+    float a[16];
+    a[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+    a[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+    a[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+    a[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+    a[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+    a[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+    a[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+    a[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+    a[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+    a[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+    a[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+    a[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+    a[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+    a[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+    a[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+    a[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+    float det = m[0] * a[0] + m[1] * a[4] + m[2] * a[8] + m[3] * a[12];
+    det = 1.0f / det;
+    for (int i = 0; i < 16; i++) out[i] = a[i] * det;
+}
+
+// CudaBvh::woopify_tri (src/accelerators/sbvh/cudabvh.cpp:303-322): the
+// unit-triangle transform, rows (M^-1 row 2 with w negated, row 0, row 1).
+// A leading -0 is stored as +0 so it cannot alias the Compact2 terminator.
+void woopify(const float* a, const float* b, const float* c, float out[12]) {
+    F3 v0 = f3(a[0], a[1], a[2]), v1 = f3(b[0], b[1], b[2]), v2 = f3(c[0], c[1], c[2]);
+    F3 e0 = v0 - v2, e1 = v1 - v2;
+    F3 n = normalize(cross(v0 - v2, v1 - v2));
+    const float m[16] = {e0.x, e1.x, n.x, v2.x, e0.y, e1.y, n.y, v2.y, e0.z, e1.z, n.z, v2.z, 0.0f, 0.0f, 0.0f, 1.0f};
+    float inv[16];
+    invert4x4_ref(m, inv);
+    out[0] = inv[8];
+    out[1] = inv[9];
+    out[2] = inv[10];
+    out[3] = -inv[11];
+    for (int i = 0; i < 4; i++) out[4 + i] = inv[i];
+    for (int i = 0; i < 4; i++) out[8 + i] = inv[4 + i];
+    if (out[0] == 0.0f) out[0] = 0.0f;
+}
+
+// A root leaf is wrapped in an inner node whose second child is an empty
+// leaf with an inverted box (never entered); the reference asserts instead
+// (cudabvh.cpp:237).
+struct RootView {
+    BNode wrap;
+    BNode empty;
+    const BNode* root;
+    explicit RootView(const BNode* r) {
+        root = r;
+        if (r->leaf) {
+            empty.leaf = true;
+            wrap.leaf = false;
+            wrap.b = r->b;
+            wrap.c[0] = const_cast<BNode*>(r);
+            wrap.c[1] = &empty;
+            root = &wrap;
+        }
+    }
+};
+
+rb_status export_compact2(const HostBvh& H, std::vector<uint32_t>& nodes, std::vector<uint32_t>& woop,
+                          std::vector<uint32_t>& index) {
+    RootView rv(H.root);
+    nodes.assign(16, 0);
+    woop.clear();
+    index.clear();
+    struct Ent {
+        const BNode* n;
+        size_t row;
+    };
+    std::vector<Ent> stack;
+    stack.push_back({rv.root, 0});
+    while (!stack.empty()) {
+        Ent e = stack.back();
+        stack.pop_back();
+        const Box* cb[2];
+        int32_t cidx[2];
+        for (int i = 0; i < 2; i++) {
+            const BNode* ch = e.n->c[i];
+            cb[i] = &ch->b;
+            if (!ch->leaf) {
+                size_t row = nodes.size() / 4;
+                cidx[i] = (int32_t)row;  // byte offset / 16
+                stack.push_back({ch, row});
+                nodes.resize(nodes.size() + 16, 0);
+                continue;
+            }
+            cidx[i] = ~(int32_t)(woop.size() / 4);
+            for (uint32_t j = 0; j < ch->ntris; j++) {
+                const uint32_t t = ch->tris[j];
+                const uint32_t* ind = &H.idx[3 * (size_t)t];
+                float w[12];
+                woopify(&H.verts[3 * (size_t)ind[0]], &H.verts[3 * (size_t)ind[1]], &H.verts[3 * (size_t)ind[2]], w);
+                for (int k = 0; k < 12; k++) woop.push_back(fbits(w[k]));
+                index.push_back(t);
+                index.push_back(0);
+                index.push_back(0);
+            }
+            for (int k = 0; k < 4; k++) woop.push_back(0x80000000u);
+            index.push_back(0);
+        }
+        uint32_t* d = &nodes[e.row * 4];
+        d[0] = fbits(cb[0]->lo.x); d[1] = fbits(cb[0]->hi.x); d[2] = fbits(cb[0]->lo.y); d[3] = fbits(cb[0]->hi.y);
+        d[4] = fbits(cb[1]->lo.x); d[5] = fbits(cb[1]->hi.x); d[6] = fbits(cb[1]->lo.y); d[7] = fbits(cb[1]->hi.y);
+        d[8] = fbits(cb[0]->lo.z); d[9] = fbits(cb[0]->hi.z); d[10] = fbits(cb[1]->lo.z); d[11] = fbits(cb[1]->hi.z);
+        d[12] = (uint32_t)cidx[0];
+        d[13] = (uint32_t)cidx[1];
+        d[14] = 0;
+        d[15] = 0;
+    }
+    return RB_OK;
+}
+
+// MI355X layout.  One 64-byte record per inner node (64-byte aligned, so a
+// node is four 16-byte loads of one half of a 128-byte cache line):
+//   word 0-3   c0.lo.x c0.hi.x c0.lo.y c0.hi.y
+//   word 4-7   c1.lo.x c1.hi.x c1.lo.y c1.hi.y
+//   word 8-11  c0.lo.z c0.hi.z c1.lo.z c1.hi.z
+//   word 12-15 link0 link1 count0 count1
+// link >= 0: inner child record index.  link < 0: leaf, link =
+// ~(first_reference << 5 | count) with count <= 31, so a leaf is fully
+// described by its 32-bit link (no terminator rows, nothing to fetch before
+// the triangles); words 14/15 repeat the counts for readers.  Leaves of more
+// than 31 references are split into a balanced subtree of inner records
+// that repeat the leaf's box (results are unchanged: every reference is
+// still tested).  Triangles: 3 float4 Woop rows (48 bytes) per reference,
+// leaf-contiguous; index: original triangle id per reference.  Records are
+// allocated in the Compact2 order (both inner children of a node side by side).
+static const BNode* split_big_leaf(const BNode* leaf, std::vector<std::unique_ptr<BNode>>& pool) {
+    if (!leaf->leaf || leaf->ntris <= 31) return leaf;
+    uint32_t h = leaf->ntris / 2;
+    pool.emplace_back(new BNode());
+    BNode* a = pool.back().get();
+    a->leaf = true; a->b = leaf->b; a->tris = leaf->tris; a->ntris = h;
+    pool.emplace_back(new BNode());
+    BNode* b = pool.back().get();
+    b->leaf = true; b->b = leaf->b; b->tris = leaf->tris + h; b->ntris = leaf->ntris - h;
+    pool.emplace_back(new BNode());
+    BNode* in = pool.back().get();
+    in->leaf = false; in->b = leaf->b;
+    in->c[0] = const_cast<BNode*>(split_big_leaf(a, pool));
+    in->c[1] = const_cast<BNode*>(split_big_leaf(b, pool));
+    return in;
+}
+
+rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
+    RootView rv(H.root);
+    std::vector<std::unique_ptr<BNode>> pool;
+    L.nodes.assign(16, 0);
+    L.tris.clear();
+    L.index.clear();
+    struct Ent {
+        const BNode* n;
+        size_t rec;
+    };
+    std::vector<Ent> stack;
+    stack.push_back({rv.root, 0});
+    while (!stack.empty()) {
+        Ent e = stack.back();
+        stack.pop_back();
+        const Box* cb[2];
+        int32_t link[2];
+        uint32_t cnt[2];
+        for (int i = 0; i < 2; i++) {
+            const BNode* ch = split_big_leaf(e.n->c[i], pool);
+            cb[i] = &ch->b;
+            if (!ch->leaf) {
+                size_t rec = L.nodes.size() / 16;
+                if (rec >= 0x7FFFFFFFu) return fail(RB_ERR_UNSUPPORTED, "too many BVH nodes");
+                link[i] = (int32_t)rec;
+                cnt[i] = 0;
+                stack.push_back({ch, rec});
+                L.nodes.resize(L.nodes.size() + 16, 0);
+                continue;
+            }
+            if (L.index.size() >= (1u << 26))
+                return fail(RB_ERR_UNSUPPORTED, "more than 2^26 triangle references");
+            link[i] = ~(int32_t)(((uint32_t)L.index.size() << 5) | ch->ntris);
+            cnt[i] = ch->ntris;
+            for (uint32_t j = 0; j < ch->ntris; j++) {
+                const uint32_t t = ch->tris[j];
+                const uint32_t* ind = &H.idx[3 * (size_t)t];
+                float w[12];
+                woopify(&H.verts[3 * (size_t)ind[0]], &H.verts[3 * (size_t)ind[1]], &H.verts[3 * (size_t)ind[2]], w);
+                L.tris.insert(L.tris.end(), w, w + 12);
+                L.index.push_back(t);
+            }
+        }
+        uint32_t* d = &L.nodes[e.rec * 16];
+        d[0] = fbits(cb[0]->lo.x); d[1] = fbits(cb[0]->hi.x); d[2] = fbits(cb[0]->lo.y); d[3] = fbits(cb[0]->hi.y);
+        d[4] = fbits(cb[1]->lo.x); d[5] = fbits(cb[1]->hi.x); d[6] = fbits(cb[1]->lo.y); d[7] = fbits(cb[1]->hi.y);
+        d[8] = fbits(cb[0]->lo.z); d[9] = fbits(cb[0]->hi.z); d[10] = fbits(cb[1]->lo.z); d[11] = fbits(cb[1]->hi.z);
+        d[12] = (uint32_t)link[0];
+        d[13] = (uint32_t)link[1];
+        d[14] = cnt[0];
+        d[15] = cnt[1];
+    }
+    return RB_OK;
+}
+
+}  // namespace rb

@@ -1,0 +1,207 @@
+// rb_raygen.hip -- ray generation and the intersection-fill pass.
+//  * camera rays: kernel_generate_perspective_camera_samples
+//    (renderbox2 src/renderers/samplerrenderer.cu:48-96) + generate_ray
+//    (src/core/camera.cu:91-98), Philox-keyed per (pixel, sample);
+//  * incoherent diffuse rays from primary hits (the trace benchmark's
+//    second ray set);
+//  * intersection fill: the result-write block of kernel_kepler_trace
+//    (src/accelerators/sbvh/kepler_dynamic_fetch.h:312-360) as its own pass.
+#include "rb_device.h"
+#include "rb_sample.h"
+
+namespace rb {
+
+rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
+                       rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg);
+
+struct CamArgs {
+    float r2c[16];
+    float c2w[16];
+};
+
+// Matrix4x4::operator()(float3, point) (src/core/transform.h:182-197):
+// float4 dot products summed left to right, then divide by w for points.
+__device__ __forceinline__ V3 xform_point(const float* m, float x, float y, float z) {
+    float r[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) r[i] = m[4 * i] * x + m[4 * i + 1] * y + m[4 * i + 2] * z + m[4 * i + 3] * 1.0f;
+    return v3(r[0] / r[3], r[1] / r[3], r[2] / r[3]);
+}
+__device__ __forceinline__ V3 xform_vector(const float* m, V3 v) {
+    float r[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) r[i] = m[4 * i] * v.x + m[4 * i + 1] * v.y + m[4 * i + 2] * v.z + m[4 * i + 3] * 0.0f;
+    return v3(r[0], r[1], r[2]);
+}
+
+__device__ __forceinline__ void camera_ray(const CamArgs& c, float px, float py, float4* out) {
+    const V3 pc = xform_point(c.r2c, px, py, 0.0f);
+    const V3 d = vnorm(pc);
+    const V3 o = xform_point(c.c2w, 0.0f, 0.0f, 0.0f);
+    const V3 dw = xform_vector(c.c2w, d);
+    out[0] = make_float4(o.x, o.y, o.z, 0.0f);
+    out[1] = make_float4(dw.x, dw.y, dw.z, 3.402823466e+38f);
+}
+
+__global__ void k_raygen_camera(CamArgs c, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                                uint32_t spp, Rng rng, int jitter, float4* __restrict__ rays) {
+    const uint64_t total = (uint64_t)w * h * spp;
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = (uint32_t)(i % spp);
+        const uint32_t p = (uint32_t)(i / spp);
+        const uint32_t px = x0 + p % w, py = y0 + p / w;
+        float jx = 0.5f, jy = 0.5f;
+        if (jitter) {
+            const U4 r = rng.draw(py * width + px, s, 0, 0);
+            jx = u01(r.x);
+            jy = u01(r.y);
+        }
+        camera_ray(c, (float)px + jx, (float)py + jy, rays + 2 * i);
+    }
+}
+
+__global__ void k_raygen_diffuse(const float4* __restrict__ prim, const int2* __restrict__ hits, uint32_t np,
+                                 uint32_t nout, const float4* __restrict__ tri_v, Rng rng, Rng redraw,
+                                 float4* __restrict__ out) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nout; i += gridDim.x * blockDim.x) {
+        uint32_t j = i % np;
+        int2 h = hits[j];
+        for (uint32_t att = 0; h.x < 0 && att < 64; att++) {
+            j = redraw.draw(i, att, 0, 0).x % np;
+            h = hits[j];
+        }
+        if (h.x < 0) {  // nothing was hit anywhere near: an empty ray
+            out[2 * (size_t)i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            out[2 * (size_t)i + 1] = make_float4(0.f, 0.f, 1.f, 0.f);
+            continue;
+        }
+        const float4 o = prim[2 * (size_t)j], d = prim[2 * (size_t)j + 1];
+        const float t = __int_as_float(h.y);
+        const V3 p = vadd(v3(o.x, o.y, o.z), vmul(v3(d.x, d.y, d.z), t));
+        const float4 a = tri_v[3 * (size_t)h.x], b = tri_v[3 * (size_t)h.x + 1], c = tri_v[3 * (size_t)h.x + 2];
+        V3 n = tri_normal(v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z));
+        if (vdot(n, v3(d.x, d.y, d.z)) > 0.0f) n = vneg(n);
+        const U4 r = rng.draw(i, 0, 0, 0);
+        const V3 dir = cosine_hemisphere(n, u01(r.x), u01(r.y));
+        out[2 * (size_t)i] = make_float4(p.x, p.y, p.z, 1.0e-3f * t);
+        out[2 * (size_t)i + 1] = make_float4(dir.x, dir.y, dir.z, 3.402823466e+38f);
+    }
+}
+
+__global__ void k_fill(const float4* __restrict__ rays, const int2* __restrict__ hits, uint32_t n,
+                       const float4* __restrict__ tri_v, rb_isect_soa o) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 h = hits[i];
+        const bool hit = h.x >= 0;
+        if (o.intersected) o.intersected[i] = hit ? 1 : 0;
+        if (o.t) o.t[i] = __int_as_float(h.y);
+        if (!hit) continue;
+        const float4 ro = rays[2 * (size_t)i], rd = rays[2 * (size_t)i + 1];
+        const float t = __int_as_float(h.y);
+        const float4 a = tri_v[3 * (size_t)h.x], b = tri_v[3 * (size_t)h.x + 1], c = tri_v[3 * (size_t)h.x + 2];
+        const V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+        const V3 d = v3(rd.x, rd.y, rd.z);
+        const V3 p = vadd(v3(ro.x, ro.y, ro.z), vmul(d, t));
+        const V3 ng = tri_normal(v0, v1, v2);
+        if (o.primitive_id) o.primitive_id[i] = (uint32_t)h.x;
+        if (o.material_id) o.material_id[i] = __float_as_uint(a.w);
+        for (int k = 0; k < 3; k++) {
+            const float pv = k == 0 ? p.x : (k == 1 ? p.y : p.z);
+            const float nv = k == 0 ? ng.x : (k == 1 ? ng.y : ng.z);
+            const float dv = k == 0 ? d.x : (k == 1 ? d.y : d.z);
+            if (o.position[k]) o.position[k][i] = pv;
+            if (o.geometric_normal[k]) o.geometric_normal[k][i] = nv;
+            if (o.shading_normal[k]) o.shading_normal[k][i] = nv;
+            if (o.incoming_direction[k]) o.incoming_direction[k][i] = -dv;
+        }
+        if (o.front) o.front[i] = vdot(ng, d) < 0.0f ? 1 : 0;
+        if (o.epsilon) o.epsilon[i] = 1.0e-3f * t;
+        if (o.uv) {
+            // barycentrics of p: weights of v0 and v1 (the Woop u, v convention)
+            const V3 e0 = vsub(v0, v2), e1 = vsub(v1, v2), q = vsub(p, v2);
+            const float d00 = vdot(e0, e0), d01 = vdot(e0, e1), d11 = vdot(e1, e1);
+            const float d20 = vdot(q, e0), d21 = vdot(q, e1);
+            const float den = d00 * d11 - d01 * d01;
+            const float u = den != 0.0f ? (d11 * d20 - d01 * d21) / den : 0.0f;
+            const float v = den != 0.0f ? (d00 * d21 - d01 * d20) / den : 0.0f;
+            o.uv[2 * (size_t)i] = u;
+            o.uv[2 * (size_t)i + 1] = v;
+        }
+    }
+}
+
+static uint32_t grid_for(uint64_t n, uint32_t block) {
+    uint64_t g = (n + block - 1) / block;
+    const uint64_t cap = (uint64_t)dev().num_cus * 16;
+    if (g > cap) g = cap;
+    return (uint32_t)(g ? g : 1);
+}
+
+}  // namespace rb
+
+using namespace rb;
+
+extern "C" {
+
+rb_status rb_raygen_camera(const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp,
+                           uint64_t seed, int jitter, rb_ray* rays, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if (!cam || !rays) return fail(RB_ERR_INVALID, "NULL camera or rays");
+    if (spp == 0 || w == 0 || h == 0) return RB_OK;
+    if (x0 + w > cam->width || y0 + h > cam->height) return fail(RB_ERR_INVALID, "tile outside the film");
+    CamArgs c;
+    if ((st = rb_camera_matrices(cam, c.r2c, c.c2w)) != RB_OK) return st;
+    const uint64_t total = (uint64_t)w * h * spp;
+    hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
+    hipLaunchKernelGGL(k_raygen_camera, dim3(grid_for(total, 256)), dim3(256), 0, s, c, cam->width, x0, y0, w, h, spp,
+                       Rng::make(seed, RNG_CAMERA), jitter, reinterpret_cast<float4*>(rays));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RB_OK : hip_fail(e, "raygen launch");
+}
+
+rb_status rb_raygen_diffuse(rb_bvh_t bvh, const rb_ray* prim, const rb_hit* hits, uint32_t np, uint32_t nout,
+                            uint64_t seed, rb_ray* out, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if (!bvh || !prim || !hits || !out) return fail(RB_ERR_INVALID, "NULL argument");
+    if (!bvh->tri_v) return fail(RB_ERR_INVALID, "bvh was uploaded without its scene");
+    if (nout == 0) return RB_OK;
+    if (np == 0) return fail(RB_ERR_INVALID, "no primary rays");
+    hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
+    hipLaunchKernelGGL(k_raygen_diffuse, dim3(grid_for(nout, 256)), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(prim), reinterpret_cast<const int2*>(hits), np, nout, bvh->tri_v,
+                       Rng::make(seed, RNG_DIFFUSE), Rng::make(seed, RNG_REDRAW), reinterpret_cast<float4*>(out));
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RB_OK : hip_fail(e, "diffuse raygen launch");
+}
+
+rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
+                        rb_hit* hits, const rb_isect_soa* out, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if (!bvh || !out) return fail(RB_ERR_INVALID, "NULL argument");
+    if (!bvh->tri_v) return fail(RB_ERR_INVALID, "bvh was uploaded without its scene");
+    if (n == 0) return RB_OK;
+    hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
+    rb_hit* tmp = nullptr;
+    if (!hits) {
+        if ((st = dalloc((void**)&tmp, (size_t)n * sizeof(rb_hit))) != RB_OK) return st;
+        hits = tmp;
+    }
+    st = launch_trace(bvh, rays, n, flags, mask, hits, nullptr, s, dev().cfg);
+    if (st == RB_OK) {
+        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, reinterpret_cast<const float4*>(rays),
+                           reinterpret_cast<const int2*>(hits), n, bvh->tri_v, *out);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) st = hip_fail(e, "fill launch");
+    }
+    if (tmp) {
+        hipStreamSynchronize(s);
+        dfree(tmp, (size_t)n * sizeof(rb_hit));
+    }
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+// rb_trace.hip -- persistent while-while BVH traversal for gfx950 (wave64).
+//
+// The Aila-Laine persistent "while-while" kernel of renderbox2
+// (src/accelerators/sbvh/kepler_dynamic_fetch.h:67-363) redesigned for CDNA4:
+//  * work fetch: one ballot per wave, one atomic per `chunk` rays from the
+//    wave's leader lane, rays handed to terminated lanes by ballot rank
+//    (the reference does one atomic per warp refill, :99-153);
+//  * traversal stack: entry 0..S-1 of each lane in LDS (lane-interleaved,
+//    conflict-free), deeper entries in per-lane private memory (the
+//    reference keeps all 64 entries in local memory, :62-71);
+//  * nodes: the MI355X layout (4 x 16-byte loads per node, leaf size in the
+//    32-bit link, no terminator rows);
+//  * ties at equal t go to the lowest original triangle id; the reference
+//    keeps the first one found (:264);
+//  * the work counter lives in a per-launch slot that the last block resets
+//    (the reference copies a zero into a global symbol before each launch,
+//    cudatracer.cu:164-165), so concurrent launches and graph replays work.
+// Arithmetic (fp32, explicit fmaf, exact division) matches the oracle's
+// restatement bit for bit.
+#include "rb_device.h"
+
+namespace rb {
+
+struct TraceArgs {
+    const float4* __restrict__ nodes;
+    const float4* __restrict__ tris;
+    const uint32_t* __restrict__ index;
+    const float4* __restrict__ rays;
+    const uint8_t* __restrict__ mask;
+    int2* __restrict__ hits;
+    uint32_t* __restrict__ counts;
+    uint32_t* counter;
+    uint32_t n;
+    uint32_t chunk;
+    uint32_t threshold;
+};
+
+template <bool ANY, bool COUNT, int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+    __shared__ int lstack[S * BLOCK];
+    int spill[kMaxStack - S];
+    const int tid = threadIdx.x;
+    const uint32_t lane = __lane_id();
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+
+#define RB_PUSH(v)                                            \
+    do {                                                      \
+        if (sp + 1 < kMaxStack) {                             \
+            ++sp;                                             \
+            if (sp < S)                                       \
+                lstack[sp * BLOCK + tid] = (v);               \
+            else                                              \
+                spill[sp - S] = (v);                          \
+        }                                                     \
+    } while (0)
+#define RB_POP(dst)                                                        \
+    do {                                                                   \
+        (dst) = sp < S ? lstack[sp * BLOCK + tid] : spill[sp - S];         \
+        --sp;                                                              \
+    } while (0)
+
+    lstack[tid] = kSentinel;
+    uint32_t pool_next = 0, pool_end = 0;
+    int nodeAddr = kSentinel, leafAddr = 0, sp = 0, hitIndex = -1;
+    uint32_t rayidx = 0;
+    float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, hitT = 0.f;
+    float idx_ = 0.f, idy_ = 0.f, idz_ = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
+    uint32_t c_inner = 0, c_tri = 0, c_leaf = 0;
+
+    while (true) {
+        // ---- fetch rays for terminated lanes (wave-uniform control) ----
+        const bool term = nodeAddr == kSentinel;
+        const uint64_t m = __ballot(term);
+        if (m) {
+            const uint32_t nterm = __popcll(m);
+            const uint32_t rank = __popcll(m & lt_mask);
+            const uint32_t avail = pool_end - pool_next;
+            uint32_t mine;
+            if (avail >= nterm) {
+                mine = pool_next + rank;
+                pool_next += nterm;
+            } else {
+                const int leader = __builtin_ctzll(m);
+                uint32_t base = 0;
+                if ((int)lane == leader) base = atomicAdd(a.counter, a.chunk);
+                base = __shfl(base, leader);
+                mine = rank < avail ? pool_next + rank : base + (rank - avail);
+                pool_next = base + (nterm - avail);
+                pool_end = base + a.chunk;
+            }
+            if (term) {
+                if (mine >= a.n) break;
+                rayidx = mine;
+                const float4 o = a.rays[2 * (size_t)rayidx];
+                const float4 d = a.rays[2 * (size_t)rayidx + 1];
+                ox = o.x; oy = o.y; oz = o.z; tmin = o.w;
+                dx = d.x; dy = d.y; dz = d.z; hitT = d.w;
+                const float ooeps = 0x1p-80f;
+                idx_ = 1.0f / (fabsf(dx) > ooeps ? dx : copysignf(ooeps, dx));
+                idy_ = 1.0f / (fabsf(dy) > ooeps ? dy : copysignf(ooeps, dy));
+                idz_ = 1.0f / (fabsf(dz) > ooeps ? dz : copysignf(ooeps, dz));
+                oodx = ox * idx_;
+                oody = oy * idy_;
+                oodz = oz * idz_;
+                sp = 0;
+                leafAddr = 0;
+                hitIndex = -1;
+                nodeAddr = (a.mask && a.mask[rayidx] == 0) ? kSentinel : 0;
+                if (COUNT) { c_inner = 0; c_tri = 0; c_leaf = 0; }
+            }
+        }
+
+        // ---- traversal ----
+        while (nodeAddr != kSentinel) {
+            while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
+                const float4* np = a.nodes + 4 * (size_t)nodeAddr;
+                const float4 n0 = np[0];
+                const float4 n1 = np[1];
+                const float4 nz = np[2];
+                const int4 lk = *reinterpret_cast<const int4*>(np + 3);
+                if (COUNT) c_inner++;
+                const float c0lox = fmaf(n0.x, idx_, -oodx), c0hix = fmaf(n0.y, idx_, -oodx);
+                const float c0loy = fmaf(n0.z, idy_, -oody), c0hiy = fmaf(n0.w, idy_, -oody);
+                const float c0loz = fmaf(nz.x, idz_, -oodz), c0hiz = fmaf(nz.y, idz_, -oodz);
+                const float c1loz = fmaf(nz.z, idz_, -oodz), c1hiz = fmaf(nz.w, idz_, -oodz);
+                const float c1lox = fmaf(n1.x, idx_, -oodx), c1hix = fmaf(n1.y, idx_, -oodx);
+                const float c1loy = fmaf(n1.z, idy_, -oody), c1hiy = fmaf(n1.w, idy_, -oody);
+                const float c0min = fmaxf(fmaxf(fminf(c0lox, c0hix), fminf(c0loy, c0hiy)), fmaxf(fminf(c0loz, c0hiz), tmin));
+                const float c0max = fminf(fminf(fmaxf(c0lox, c0hix), fmaxf(c0loy, c0hiy)), fminf(fmaxf(c0loz, c0hiz), hitT));
+                const float c1min = fmaxf(fmaxf(fminf(c1lox, c1hix), fminf(c1loy, c1hiy)), fmaxf(fminf(c1loz, c1hiz), tmin));
+                const float c1max = fminf(fminf(fmaxf(c1lox, c1hix), fmaxf(c1loy, c1hiy)), fminf(fmaxf(c1loz, c1hiz), hitT));
+                const bool swp = c1min < c0min;
+                const bool t0 = c0max >= c0min;
+                const bool t1 = c1max >= c1min;
+                int far = lk.y;
+                if (!t0 && !t1) {
+                    RB_POP(nodeAddr);
+                } else {
+                    nodeAddr = t0 ? lk.x : lk.y;
+                    if (t0 && t1) {
+                        if (swp) {
+                            const int tmp = nodeAddr;
+                            nodeAddr = far;
+                            far = tmp;
+                        }
+                        RB_PUSH(far);
+                    }
+                }
+                if (nodeAddr < 0 && leafAddr >= 0) {
+                    leafAddr = nodeAddr;
+                    RB_POP(nodeAddr);
+                }
+                if (!__any(leafAddr >= 0)) break;
+            }
+
+            while (leafAddr < 0) {
+                const uint32_t l = ~(uint32_t)leafAddr;
+                const uint32_t first = l >> 5;
+                const uint32_t cnt = l & 31u;
+                if (COUNT) c_leaf++;
+                for (uint32_t k = 0; k < cnt; k++) {
+                    const float4* w = a.tris + 3 * (size_t)(first + k);
+                    const float4 v0 = w[0];
+                    const float4 v1 = w[1];
+                    const float4 v2 = w[2];
+                    if (COUNT) c_tri++;
+                    const float Oz = fmaf(-oz, v0.z, fmaf(-oy, v0.y, fmaf(-ox, v0.x, v0.w)));
+                    const float invDz = 1.0f / fmaf(dz, v0.z, fmaf(dy, v0.y, dx * v0.x));
+                    const float t = Oz * invDz;
+                    if (t > tmin && (t < hitT || (hitIndex != -1 && t == hitT))) {
+                        const float Ox = fmaf(oz, v1.z, fmaf(oy, v1.y, fmaf(ox, v1.x, v1.w)));
+                        const float Dx = fmaf(dz, v1.z, fmaf(dy, v1.y, dx * v1.x));
+                        const float u = fmaf(t, Dx, Ox);
+                        if (u >= 0.0f) {
+                            const float Oy = fmaf(oz, v2.z, fmaf(oy, v2.y, fmaf(ox, v2.x, v2.w)));
+                            const float Dy = fmaf(dz, v2.z, fmaf(dy, v2.y, dx * v2.x));
+                            const float v = fmaf(t, Dy, Oy);
+                            if (v >= 0.0f && u + v <= 1.0f) {
+                                bool take = t < hitT;
+                                if (!take) take = a.index[first + k] < a.index[hitIndex];
+                                if (take) {
+                                    hitT = t;
+                                    hitIndex = (int)(first + k);
+                                    if (ANY) {
+                                        nodeAddr = kSentinel;
+                                        break;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                leafAddr = nodeAddr;
+                if (nodeAddr < 0) RB_POP(nodeAddr);
+            }
+
+            // Dynamic fetch: refill when too few lanes are still busy.
+            if (__popcll(__ballot(1)) < a.threshold) break;
+        }
+
+        if (nodeAddr == kSentinel) {
+            int2 r;
+            r.x = hitIndex < 0 ? -1 : (int)a.index[hitIndex];
+            r.y = __float_as_int(hitT);
+            a.hits[rayidx] = r;
+            if (COUNT) {
+                a.counts[3 * (size_t)rayidx + 0] = c_inner;
+                a.counts[3 * (size_t)rayidx + 1] = c_tri;
+                a.counts[3 * (size_t)rayidx + 2] = c_leaf;
+            }
+        }
+    }
+#undef RB_PUSH
+#undef RB_POP
+
+    __syncthreads();
+    if (tid == 0) {
+        __threadfence();
+        if (atomicAdd(a.counter + 1, 1u) == gridDim.x - 1) {
+            atomicExch(a.counter + 0, 0u);
+            atomicExch(a.counter + 1, 0u);
+        }
+    }
+}
+
+// ---------------------------------------------------------------- launch --
+
+typedef void (*TraceKernel)(TraceArgs);
+
+template <bool ANY, bool COUNT>
+static TraceKernel pick(uint32_t S, uint32_t B) {
+#define RB_K(s, b) \
+    if (S == s && B == b) return k_trace<ANY, COUNT, s, b>;
+    RB_K(8, 64) RB_K(8, 128) RB_K(8, 256)
+    RB_K(12, 64) RB_K(12, 128) RB_K(12, 256)
+    RB_K(16, 64) RB_K(16, 128) RB_K(16, 256)
+    RB_K(20, 64) RB_K(20, 128) RB_K(20, 256)
+    RB_K(24, 64) RB_K(24, 128) RB_K(24, 256)
+#undef RB_K
+    return nullptr;
+}
+
+rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
+                       rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg) {
+    if (n == 0) return RB_OK;
+    if (!bvh || !rays || !hits) return fail(RB_ERR_INVALID, "NULL bvh, rays or hits");
+    if (n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "too many rays in one call (%u)", n);
+    if ((flags & RB_MASKED) && !mask) return fail(RB_ERR_INVALID, "RB_MASKED without a mask");
+    const bool any = (flags & RB_ANY) != 0;
+    TraceKernel k = counts ? (any ? pick<true, true>(cfg.lds_stack, cfg.block) : pick<false, true>(cfg.lds_stack, cfg.block))
+                           : (any ? pick<true, false>(cfg.lds_stack, cfg.block) : pick<false, false>(cfg.lds_stack, cfg.block));
+    if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
+    TraceArgs a;
+    a.nodes = bvh->nodes;
+    a.tris = bvh->tris;
+    a.index = bvh->index;
+    a.rays = reinterpret_cast<const float4*>(rays);
+    a.mask = (flags & RB_MASKED) ? mask : nullptr;
+    a.hits = reinterpret_cast<int2*>(hits);
+    a.counts = counts;
+    a.counter = take_counter_slot();
+    a.n = n;
+    a.chunk = cfg.chunk;
+    a.threshold = cfg.fetch_threshold;
+    const uint32_t waves = dev().num_cus * cfg.waves_per_cu;
+    uint32_t blocks = waves * 64 / cfg.block;
+    const uint32_t need = (uint32_t)(((uint64_t)n + cfg.block - 1) / cfg.block);
+    if (blocks > need) blocks = need;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(cfg.block), 0, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "trace launch");
+    return RB_OK;
+}
+
+static rb_status check_cfg(const rb_trace_config& c) {
+    if (c.block != 64 && c.block != 128 && c.block != 256) return fail(RB_ERR_INVALID, "block must be 64/128/256");
+    if (c.lds_stack < 8 || c.lds_stack > 24 || c.lds_stack % 4) return fail(RB_ERR_INVALID, "lds_stack must be 8..24, step 4");
+    if (c.chunk < 64 || c.chunk % 64 || c.chunk > 65536) return fail(RB_ERR_INVALID, "chunk must be a multiple of 64 up to 65536");
+    if (c.fetch_threshold > 64) return fail(RB_ERR_INVALID, "fetch_threshold must be <= 64");
+    if (c.waves_per_cu < 1 || c.waves_per_cu > 64) return fail(RB_ERR_INVALID, "waves_per_cu must be 1..64");
+    return RB_OK;
+}
+
+}  // namespace rb
+
+using namespace rb;
+
+extern "C" {
+
+rb_status rb_trace_get_config(rb_trace_config* c) {
+    if (!c) return fail(RB_ERR_INVALID, "NULL config");
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    *c = dev().cfg;
+    return RB_OK;
+}
+
+rb_status rb_trace_set_config(const rb_trace_config* c) {
+    if (!c) return fail(RB_ERR_INVALID, "NULL config");
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if ((st = check_cfg(*c)) != RB_OK) return st;
+    dev().cfg = *c;
+    return RB_OK;
+}
+
+rb_status rb_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask, rb_hit* hits,
+                   void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    return launch_trace(bvh, rays, n, flags, mask, hits, nullptr, stream ? (hipStream_t)stream : dev().stream,
+                        dev().cfg);
+}
+
+rb_status rb_trace_counts(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits,
+                          uint32_t* counts, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if (!counts) return fail(RB_ERR_INVALID, "NULL counts");
+    return launch_trace(bvh, rays, n, flags & ~RB_MASKED, nullptr, hits, counts,
+                        stream ? (hipStream_t)stream : dev().stream, dev().cfg);
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+"""GPU parity of the trace path: librb.so's HIP kernels against the oracle.
+
+Closest hit: triangle id and t bit-identical to the oracle's restatement of
+the reference traversal on the same SBVH (ties to the lowest id).  Any hit:
+hit/miss identical, and the reported (id, t) is a real hit of that ray.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from conftest import random_rays, random_triangles
+
+pytestmark = pytest.mark.gpu
+
+CONFIGS = [
+    dict(block=128, lds_stack=16, chunk=64, fetch_threshold=24, waves_per_cu=16),
+    dict(block=64, lds_stack=8, chunk=256, fetch_threshold=0, waves_per_cu=8),
+    dict(block=256, lds_stack=24, chunk=64, fetch_threshold=48, waves_per_cu=32),
+]
+
+
+def _compact2(rbm, bvh):
+    from renderbox2_amd import _lib as L
+    n, w, x = bvh.export(L.RB_LAYOUT_REF_COMPACT2)
+    return n.reshape(-1, 4), w.reshape(-1, 4), x
+
+
+@pytest.fixture(scope="module")
+def small(gpu):
+    v, idx = random_triangles(3000, seed=50)
+    b = gpu.BVH((v, idx))
+    return v, idx, b, gpu.CudaTracer(b), _compact2(gpu, b)
+
+
+@pytest.fixture(scope="module")
+def room(gpu):
+    s = gpu.scenes.ao_room()
+    b = gpu.scenes.build(s, max_leaf=4)
+    return s, b, gpu.CudaTracer(b, s), _compact2(gpu, b)
+
+
+def _check_any(orc, c2, rays, ids, ts, aid, at):
+    assert np.array_equal(aid >= 0, ids >= 0)
+    h = aid >= 0
+    assert np.all((at[h] > rays[h, 3]) & (at[h] < rays[h, 7]) & (at[h] >= ts[h]))
+    # the any-hit (id, t) is a real hit: the closest hit in [t-, t] is at exactly t
+    r = rays[h].copy()
+    r[:, 3] = np.nextafter(at[h], np.float32(-np.inf))
+    r[:, 7] = np.nextafter(at[h], np.float32(np.inf))
+    i2, t2 = orc.trace_compact2(*c2, r)
+    assert np.array_equal(t2, at[h])
+    assert np.all(i2 <= aid[h])
+
+
+@pytest.mark.parametrize("cfg", range(len(CONFIGS)))
+def test_trace_small_bit_exact(gpu, orc, small, cfg):
+    v, idx, b, tr, c2 = small
+    gpu.set_trace_config(**CONFIGS[cfg])
+    try:
+        rays = random_rays(50000, seed=60 + cfg)
+        ids, ts = tr.trace(rays)
+        oid, ot = orc.trace_compact2(*c2, rays)
+        assert (ids >= 0).sum() > 10000
+        assert np.array_equal(ids, oid)
+        assert np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
+        aid, at = tr.trace(rays, any_hit=True)
+        _check_any(orc, c2, rays, ids, ts, aid, at)
+    finally:
+        gpu.set_trace_config(**CONFIGS[0])
+
+
+def test_trace_room_camera_and_diffuse_rays(gpu, orc, room):
+    s, b, tr, c2 = room
+    cam = gpu.scenes.ao_camera(256, 256)
+    n = cam.get_total_pixels()
+    d_prim = gpu.DeviceBuffer(n * 32)
+    cam.generate_rays_device(d_prim, spp=1, seed=3, jitter=True)
+    prim = d_prim.download(np.float32, n * 8).reshape(n, 8)
+    # camera rays equal the oracle's restatement bit for bit
+    lib = orc.lib()
+    P = C.c_void_p
+    lib.orc_raygen_camera.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint64, C.c_int, P]
+    r2c, c2w = cam.matrices()
+    r2c = np.ascontiguousarray(r2c)
+    c2w = np.ascontiguousarray(c2w)
+    orays = np.empty((n, 8), np.float32)
+    lib.orc_raygen_camera(r2c.ctypes.data_as(P), c2w.ctypes.data_as(P), 256, 0, 0, 256, 256, 1, 3, 1,
+                          orays.ctypes.data_as(P))
+    assert np.array_equal(prim.view(np.uint32), orays.view(np.uint32))
+    d_hits = gpu.DeviceBuffer(n * 8)
+    tr.trace_device(d_prim, n, d_hits)
+    hits = d_hits.download(gpu.HIT_DTYPE, n)
+    oid, ot = orc.trace_compact2(*c2, prim)
+    assert np.array_equal(hits["id"], oid) and np.array_equal(hits["t"].view(np.uint32), ot.view(np.uint32))
+    assert (oid >= 0).all()  # closed room
+    # diffuse rays from the hits
+    nd = 2 * n
+    d_diff = gpu.DeviceBuffer(nd * 32)
+    gpu.raygen_diffuse_device(tr, d_prim, d_hits, n, nd, 11, d_diff)
+    diff = d_diff.download(np.float32, nd * 8).reshape(nd, 8)
+    v, idx, _, _ = s.geometry()
+    lib.orc_raygen_diffuse.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint64, P]
+    od = np.empty((nd, 8), np.float32)
+    lib.orc_raygen_diffuse(v.ctypes.data_as(P), idx.ctypes.data_as(P), prim.ctypes.data_as(P),
+                           oid.ctypes.data_as(P), ot.ctypes.data_as(P), n, nd, 11, od.ctypes.data_as(P))
+    assert np.array_equal(diff.view(np.uint32), od.view(np.uint32))
+    ids, ts = tr.trace(diff)
+    oid2, ot2 = orc.trace_compact2(*c2, diff)
+    assert np.array_equal(ids, oid2) and np.array_equal(ts.view(np.uint32), ot2.view(np.uint32))
+    aid, at = tr.trace(diff, any_hit=True)
+    _check_any(orc, c2, diff, ids, ts, aid, at)
+
+
+def test_masked_degenerate_and_empty_batches(gpu, orc, small):
+    v, idx, b, tr, c2 = small
+    rays = random_rays(4096, seed=70)
+    rays[0, 4:7] = 0.0                       # zero direction
+    rays[1, 4:7] = np.nan                    # NaN direction
+    rays[2, 3], rays[2, 7] = 5.0, 1.0        # tmin > tmax
+    rays[3, 4:7] = [0.0, 0.0, 1.0]           # axis-aligned
+    rays[4, 0:3] = np.nan                    # NaN origin
+    mask = (np.arange(4096) % 3 != 0).astype(np.uint8)
+    ids, ts = tr.trace(rays, mask=mask)
+    oid, ot = orc.trace_compact2(*c2, rays)
+    m = mask.astype(bool)
+    assert np.array_equal(ids[m], oid[m]) and np.array_equal(ts[m].view(np.uint32), ot[m].view(np.uint32))
+    assert np.all(ids[~m] == -1) and np.array_equal(ts[~m], rays[~m, 7])
+    assert ids[1] == -1 and ids[2] == -1 and ids[4] == -1
+    # empty and single-ray batches
+    e_ids, e_ts = tr.trace(rays[:0])
+    assert e_ids.size == 0
+    one_id, one_t = tr.trace(rays[5:6])
+    assert one_id[0] == oid[5] and one_t[0] == ot[5]
+
+
+def test_trace_is_deterministic_and_reentrant(gpu, small):
+    v, idx, b, tr, c2 = small
+    rays = random_rays(100000, seed=80)
+    dr = gpu.DeviceBuffer.from_array(rays)
+    outs = []
+    streams = [gpu.Stream(), gpu.Stream()]
+    bufs = [gpu.DeviceBuffer(len(rays) * 8) for _ in range(4)]
+    for k in range(4):   # concurrent launches on two streams share nothing
+        tr.trace_device(dr, len(rays), bufs[k], stream=streams[k % 2])
+    for s in streams:
+        s.synchronize()
+    for k in range(4):
+        outs.append(bufs[k].download(np.uint32, 2 * len(rays)))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
+
+
+def test_trace_fill_matches_hits(gpu, room):
+    s, b, tr, c2 = room
+    from renderbox2_amd import _lib as L
+    rays = random_rays(20000, seed=90, box=4.0)
+    n = len(rays)
+    dr = gpu.DeviceBuffer.from_array(rays)
+    dh = gpu.DeviceBuffer(n * 8)
+    bufs = {k: gpu.DeviceBuffer(n * 4) for k in ("px", "py", "pz", "nx", "ny", "nz", "pid", "mid", "eps", "t")}
+    inter = gpu.DeviceBuffer(n)
+    soa = L.rb_isect_soa()
+    soa.position[0], soa.position[1], soa.position[2] = bufs["px"].ptr, bufs["py"].ptr, bufs["pz"].ptr
+    soa.geometric_normal[0], soa.geometric_normal[1], soa.geometric_normal[2] = bufs["nx"].ptr, bufs["ny"].ptr, bufs["nz"].ptr
+    soa.primitive_id, soa.material_id, soa.epsilon, soa.t = bufs["pid"].ptr, bufs["mid"].ptr, bufs["eps"].ptr, bufs["t"].ptr
+    soa.intersected = inter.ptr
+    L.check(L.lib().rb_trace_fill(tr.handle, dr.ptr, n, 0, None, dh.ptr, C.byref(soa), None))
+    hits = dh.download(gpu.HIT_DTYPE, n)
+    got = {k: bufs[k].download(np.float32 if k not in ("pid", "mid") else np.uint32, n) for k in bufs}
+    it = inter.download(np.uint8, n)
+    h = hits["id"] >= 0
+    assert np.array_equal(it.astype(bool), h)
+    assert np.array_equal(got["pid"][h], hits["id"][h].astype(np.uint32))
+    assert np.array_equal(got["t"], hits["t"])
+    p = rays[h, :3] + rays[h, 4:7] * hits["t"][h, None]
+    assert np.allclose(np.stack([got["px"][h], got["py"][h], got["pz"][h]], 1), p, atol=1e-4)
+    v, idx, _, _ = s.geometry()
+    tri = v[idx[hits["id"][h]]]
+    ng = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    ng /= np.linalg.norm(ng, axis=1, keepdims=True)
+    assert np.allclose(np.stack([got["nx"][h], got["ny"][h], got["nz"][h]], 1), ng, atol=1e-5)
+    assert np.allclose(got["eps"][h], 1e-3 * hits["t"][h])
+
+
+def test_million_triangles_properties(gpu, orc):
+    """Full-size scene (1M triangles): a 64k-ray sample bit-exact with the oracle,
+    any-hit <=> closest-hit on all 1M rays, and run-to-run determinism."""
+    s = gpu.scenes.trace_1m()
+    b = gpu.scenes.build(s)
+    tr = gpu.CudaTracer(b, s)
+    cam = gpu.scenes.trace_camera(1024, 512)
+    n = cam.get_total_pixels()
+    d_prim = gpu.DeviceBuffer(n * 32)
+    cam.generate_rays_device(d_prim, seed=1)
+    d_hits = gpu.DeviceBuffer(n * 8)
+    tr.trace_device(d_prim, n, d_hits)
+    d_diff = gpu.DeviceBuffer(2 * n * 32)
+    gpu.raygen_diffuse_device(tr, d_prim, d_hits, n, 2 * n, 5, d_diff)
+    diff = d_diff.download(np.float32, 2 * n * 8).reshape(-1, 8)
+    ids, ts = tr.trace(diff)
+    ids2, ts2 = tr.trace(diff)
+    assert np.array_equal(ids, ids2) and np.array_equal(ts, ts2)
+    aid, _ = tr.trace(diff, any_hit=True)
+    assert np.array_equal(aid >= 0, ids >= 0)
+    c2 = _compact2(gpu, b)
+    sel = np.random.default_rng(0).choice(len(diff), 65536, replace=False)
+    oid, ot = orc.trace_compact2(*c2, diff[sel])
+    assert np.array_equal(ids[sel], oid) and np.array_equal(ts[sel].view(np.uint32), ot.view(np.uint32))
