@@ -143,6 +143,20 @@ def bruteforce_f64(verts, idx, rays, nthreads=0):
     return ids, t, margin, gap
 
 
+def woop_hits(verts, idx, ids, rays):
+    """Per ray: (t, hit) of the Woop test against triangle ids[i]."""
+    verts = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
+    idx = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
+    ids = np.ascontiguousarray(ids, np.int32)
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    t = np.empty(len(rays), np.float32)
+    hit = np.empty(len(rays), np.uint8)
+    L = lib()
+    L.orc_woop_hits.argtypes = [C.c_void_p] * 4 + [C.c_uint32, C.c_void_p, C.c_void_p]
+    L.orc_woop_hits(_p(verts), _p(idx), _p(ids), _p(rays), len(rays), _p(t), _p(hit))
+    return t, hit.astype(bool)
+
+
 def philox(ctr, key):
     c = np.ascontiguousarray(ctr, np.uint32)
     k = np.ascontiguousarray(key, np.uint32)

@@ -1047,3 +1047,34 @@ void orc_raygen_diffuse(const float* verts, const uint32_t* idx, const float* pr
         R[4] = d.x; R[5] = d.y; R[6] = d.z; R[7] = FLT_MAX;
     }
 }
+
+/* The Woop ray/triangle test of kepler_dynamic_fetch.h:260-282 on one
+ * triangle (Woop rows from orc_woopify), with the traversal's arithmetic.
+ * Returns 1 on a hit with tmin < t < tmax, u, v >= 0, u + v <= 1. */
+int orc_woop_hit(const float* v0, const float* v1, const float* v2, const float* ray, float* t_out)
+{
+    float w[12];
+    orc_woopify(v0, v1, v2, w);
+    if (w[0] == 0.0f) w[0] = 0.0f;
+    const float ox = ray[0], oy = ray[1], oz = ray[2], tmin = ray[3], dx = ray[4], dy = ray[5], dz = ray[6];
+    const float Oz = fmaf(-oz, w[2], fmaf(-oy, w[1], fmaf(-ox, w[0], w[3])));
+    const float invDz = 1.0f / fmaf(dz, w[2], fmaf(dy, w[1], dx * w[0]));
+    const float t = Oz * invDz;
+    *t_out = t;
+    if (!(t > tmin && t < ray[7])) return 0;
+    const float u = fmaf(t, fmaf(dz, w[6], fmaf(dy, w[5], dx * w[4])), fmaf(oz, w[6], fmaf(oy, w[5], fmaf(ox, w[4], w[7]))));
+    if (!(u >= 0.0f)) return 0;
+    const float v = fmaf(t, fmaf(dz, w[10], fmaf(dy, w[9], dx * w[8])), fmaf(oz, w[10], fmaf(oy, w[9], fmaf(ox, w[8], w[11]))));
+    return v >= 0.0f && u + v <= 1.0f;
+}
+
+void orc_woop_hits(const float* verts, const uint32_t* idx, const int32_t* ids, const float* rays, uint32_t n,
+                   float* out_t, uint8_t* out_hit)
+{
+    for (uint32_t i = 0; i < n; i++) {
+        if (ids[i] < 0) { out_hit[i] = 0; out_t[i] = 0.0f; continue; }
+        const uint32_t* ind = idx + 3 * (size_t)ids[i];
+        out_hit[i] = (uint8_t)orc_woop_hit(verts + 3 * (size_t)ind[0], verts + 3 * (size_t)ind[1],
+                                           verts + 3 * (size_t)ind[2], rays + 8 * (size_t)i, out_t + i);
+    }
+}

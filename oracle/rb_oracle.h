@@ -88,6 +88,13 @@ void orc_bruteforce_f64(const float* verts, const uint32_t* idx, uint32_t nt,
                         const float* rays, uint32_t n, int32_t* out_id, double* out_t64,
                         double* out_margin, double* out_gap, int nthreads);
 
+/* The Woop test of kepler_dynamic_fetch.h:260-282 on a single triangle.
+ * orc_woop_hits: for ray i and triangle ids[i] (skip if < 0) write the t and
+ * whether it is a hit inside (tmin, tmax) with u, v >= 0, u + v <= 1. */
+int orc_woop_hit(const float* v0, const float* v1, const float* v2, const float* ray, float* t_out);
+void orc_woop_hits(const float* verts, const uint32_t* idx, const int32_t* ids, const float* rays, uint32_t n,
+                   float* out_t, uint8_t* out_hit);
+
 /* ---- RNG -------------------------------------------------------------- */
 
 /* Philox4x32-10 (Salmon et al., SC'11), the counter-based generator that

@@ -40,17 +40,16 @@ def room(gpu):
     return s, b, gpu.CudaTracer(b, s), _compact2(gpu, b)
 
 
-def _check_any(orc, c2, rays, ids, ts, aid, at):
+def _check_any(orc, geom, rays, ids, ts, aid, at):
     assert np.array_equal(aid >= 0, ids >= 0)
     h = aid >= 0
     assert np.all((at[h] > rays[h, 3]) & (at[h] < rays[h, 7]) & (at[h] >= ts[h]))
-    # the any-hit (id, t) is a real hit: the closest hit in [t-, t] is at exactly t
-    r = rays[h].copy()
-    r[:, 3] = np.nextafter(at[h], np.float32(-np.inf))
-    r[:, 7] = np.nextafter(at[h], np.float32(np.inf))
-    i2, t2 = orc.trace_compact2(*c2, r)
-    assert np.array_equal(t2, at[h])
-    assert np.all(i2 <= aid[h])
+    # the any-hit (id, t) is a real hit: the Woop test of triangle id on that
+    # ray gives exactly t, inside the triangle
+    v, idx = geom
+    t1, ok = orc.woop_hits(v, idx, aid[h], rays[h])
+    assert ok.all()
+    assert np.array_equal(t1.view(np.uint32), at[h].view(np.uint32))
 
 
 @pytest.mark.parametrize("cfg", range(len(CONFIGS)))
@@ -65,7 +64,7 @@ def test_trace_small_bit_exact(gpu, orc, small, cfg):
         assert np.array_equal(ids, oid)
         assert np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
         aid, at = tr.trace(rays, any_hit=True)
-        _check_any(orc, c2, rays, ids, ts, aid, at)
+        _check_any(orc, (v, idx), rays, ids, ts, aid, at)
     finally:
         gpu.set_trace_config(**CONFIGS[0])
 
@@ -110,7 +109,7 @@ def test_trace_room_camera_and_diffuse_rays(gpu, orc, room):
     oid2, ot2 = orc.trace_compact2(*c2, diff)
     assert np.array_equal(ids, oid2) and np.array_equal(ts.view(np.uint32), ot2.view(np.uint32))
     aid, at = tr.trace(diff, any_hit=True)
-    _check_any(orc, c2, diff, ids, ts, aid, at)
+    _check_any(orc, (v, idx), diff, ids, ts, aid, at)
 
 
 def test_masked_degenerate_and_empty_batches(gpu, orc, small):

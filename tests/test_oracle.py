@@ -313,3 +313,16 @@ def test_traversal_counts_are_consistent(orc):
     assert np.all(cnt[:, 1] >= 0) and np.all(cnt[:, 2] <= cnt[:, 0] * 2)
     bpr = orc.bytes_per_ray(cnt, ids >= 0)
     assert np.all(bpr == 40 + 64 * cnt[:, 0] + 48 * cnt[:, 1] + 16 * cnt[:, 2] + 4 * (ids >= 0))
+
+
+def test_single_triangle_woop_test_agrees_with_traversal(orc):
+    v, idx = random_triangles(1500, seed=42)
+    b = orc.Bvh(v, idx)
+    rays = random_rays(8000, seed=43)
+    ids, ts = orc.trace_compact2(*b.compact2(), rays)
+    h = ids >= 0
+    t, ok = orc.woop_hits(v, idx, ids[h], rays[h])
+    assert ok.all() and np.array_equal(t.view(np.uint32), ts[h].view(np.uint32))
+    # a wrong triangle id is (almost always) not a hit
+    t2, ok2 = orc.woop_hits(v, idx, (ids[h] + 1) % len(idx), rays[h])
+    assert ok2.mean() < 0.05
