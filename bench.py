@@ -72,6 +72,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="also time coherent / any-hit / mixed sets")
+    ap.add_argument("--autotune", type=int, default=1, help="pick the traversal config on this batch first")
+    ap.add_argument("--config", default="", help="block,lds_stack,chunk,fetch_threshold,waves_per_cu")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="launch one copy kernel of this size first (traffic counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -106,8 +110,21 @@ def main():
     L.check(L.lib().rb_memcpy_dtod(d_mixed.ptr, d_prim.ptr, n * 32, stream.handle))
     L.check(L.lib().rb_memcpy_dtod(d_mixed.ptr + n * 32, d_diff.ptr, n * 32, stream.handle))
     stream.synchronize()
+    if args.copy_probe_mib:
+        nb = args.copy_probe_mib << 20
+        src, dst = rb.DeviceBuffer(nb), rb.DeviceBuffer(nb)
+        src.zero(stream.handle)
+        rb.copy_kernel(dst, src, nb, stream)
+        stream.synchronize()
+        del src, dst
+    tuned_ms = None
+    if args.config:
+        keys = ["block", "lds_stack", "chunk", "fetch_threshold", "waves_per_cu"]
+        rb.set_trace_config(**dict(zip(keys, (int(x) for x in args.config.split(",")))))
+    elif args.autotune:
+        _, tuned_ms = rb.autotune(tracer, d_diff, n, d_hits, reps=3, stream=stream)
     cfg = rb.get_trace_config()
-    log("trace config", cfg)
+    log("trace config", cfg, "autotuned %.3f ms" % tuned_ms if tuned_ms else "")
 
     ms = time_launches(rb, tracer, d_diff, n, d_hits, False, args.steps, args.warmup, stream)
     value = n / (ms * 1e-3) / 1e6

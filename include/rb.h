@@ -83,6 +83,10 @@ rb_status rb_memcpy_htod(void* dst, const void* src, size_t bytes, void* stream)
 rb_status rb_memcpy_dtoh(void* dst, const void* src, size_t bytes, void* stream);
 rb_status rb_memcpy_dtod(void* dst, const void* src, size_t bytes, void* stream);
 rb_status rb_memset(void* dst, int value, size_t bytes, void* stream);
+/* Device-to-device copy as a plain streaming kernel (16-byte loads/stores):
+ * usable inside captured graphs, and the known-size reference kernel for
+ * HBM traffic counters. bytes must be a multiple of 16 and pointers 16-aligned. */
+rb_status rb_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 rb_status rb_mem_info(size_t* free_bytes, size_t* total_bytes, size_t* library_bytes);
 rb_status rb_stream_create(void** stream);
 rb_status rb_stream_destroy(void* stream);
@@ -237,6 +241,12 @@ typedef struct rb_trace_config {
 } rb_trace_config;
 rb_status rb_trace_get_config(rb_trace_config* c);
 rb_status rb_trace_set_config(const rb_trace_config* c);
+/* Autotuner (no reference counterpart; the reference hard-codes the launch
+ * shape): times candidate configurations on the caller's device rays by
+ * coordinate descent over block, lds_stack, chunk, fetch_threshold and
+ * waves_per_cu, installs the fastest and returns it with its time per launch. */
+rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits,
+                            uint32_t reps, rb_trace_config* best, float* best_ms, void* stream);
 /* Per-ray traversal counters of the device kernel (inner nodes, triangles,
  * leaves) for traffic accounting; counts is device uint32[3*n]. */
 rb_status rb_trace_counts(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,

@@ -370,6 +370,19 @@ def set_trace_config(**kw):
     check(lib().rb_trace_set_config(C.byref(c)))
 
 
+def autotune(tracer, d_rays, n, d_hits, any_hit=False, reps=3, stream=None):
+    """Pick and install the fastest traversal configuration for this batch."""
+    c = L.rb_trace_config()
+    ms = C.c_float()
+    check(lib().rb_trace_autotune(tracer.handle, _dp(d_rays), n, L.RB_ANY if any_hit else L.RB_CLOSEST,
+                                  _dp(d_hits), reps, C.byref(c), C.byref(ms), _sp(stream)))
+    return {k: getattr(c, k) for k, _ in L.rb_trace_config._fields_}, ms.value
+
+
+def copy_kernel(d_dst, d_src, nbytes, stream=None):
+    check(lib().rb_copy_kernel(_dp(d_dst), _dp(d_src), nbytes, _sp(stream)))
+
+
 class PerspectiveCamera:
     """Pinhole camera (PerspectiveCamera, src/core/camera.h:48-71)."""
 

@@ -164,6 +164,31 @@ rb_status rb_memset(void* dst, int value, size_t bytes, void* stream) {
     return RB_OK;
 }
 
+}  // extern "C"
+
+namespace rb {
+__global__ void k_copy16(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n16; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+}  // namespace rb
+
+extern "C" {
+
+rb_status rb_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
+    RB_ENSURE();
+    if (bytes == 0) return RB_OK;
+    if (!dst || !src || (bytes & 15) || ((uintptr_t)dst & 15) || ((uintptr_t)src & 15))
+        return fail(RB_ERR_INVALID, "rb_copy_kernel needs 16-byte aligned pointers and sizes");
+    const size_t n16 = bytes / 16;
+    size_t grid = (n16 + 255) / 256;
+    const size_t cap = (size_t)dev().num_cus * 32;
+    if (grid > cap) grid = cap;
+    hipLaunchKernelGGL(k_copy16, dim3((uint32_t)grid), dim3(256), 0, S(stream), (uint4*)dst, (const uint4*)src, n16);
+    RB_HIP(hipGetLastError(), "copy kernel launch");
+    return RB_OK;
+}
+
 rb_status rb_mem_info(size_t* free_bytes, size_t* total_bytes, size_t* library_bytes) {
     RB_ENSURE();
     size_t f = 0, t = 0;

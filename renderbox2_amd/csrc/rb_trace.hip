@@ -313,6 +313,75 @@ rb_status rb_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
                         dev().cfg);
 }
 
+rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits,
+                            uint32_t reps, rb_trace_config* best, float* best_ms, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if (!bvh || !rays || !hits || n == 0) return fail(RB_ERR_INVALID, "autotune needs a bvh and a non-empty ray batch");
+    if (flags & RB_MASKED) return fail(RB_ERR_INVALID, "autotune takes unmasked batches");
+    if (reps == 0) reps = 3;
+    hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(RB_ERR_DEVICE, "hipEventCreate failed");
+    auto measure = [&](const rb_trace_config& c, float* ms) -> rb_status {
+        rb_status r = launch_trace(bvh, rays, n, flags, nullptr, hits, nullptr, s, c);  // warm-up
+        if (r != RB_OK) return r;
+        hipEventRecord(e0, s);
+        for (uint32_t i = 0; i < reps; i++)
+            if ((r = launch_trace(bvh, rays, n, flags, nullptr, hits, nullptr, s, c)) != RB_OK) return r;
+        hipEventRecord(e1, s);
+        hipError_t e = hipEventSynchronize(e1);
+        if (e != hipSuccess) return hip_fail(e, "autotune sync");
+        hipEventElapsedTime(ms, e0, e1);
+        *ms /= (float)reps;
+        return RB_OK;
+    };
+    rb_trace_config cur = dev().cfg;
+    float cur_ms = 0.f;
+    if ((st = measure(cur, &cur_ms)) != RB_OK) goto done;
+    {
+        const uint32_t blocks[] = {64, 128, 256};
+        const uint32_t stacks[] = {8, 12, 16, 20, 24};
+        const uint32_t chunks[] = {64, 128, 256, 1024};
+        const uint32_t thresholds[] = {0, 8, 16, 24, 32, 40, 48};
+        const uint32_t waves[] = {8, 12, 16, 24, 32};
+        for (int pass = 0; pass < 2; pass++) {
+            for (int dim = 0; dim < 5; dim++) {
+                const uint32_t* vals;
+                int nv;
+                switch (dim) {
+                    case 0: vals = blocks; nv = 3; break;
+                    case 1: vals = stacks; nv = 5; break;
+                    case 2: vals = chunks; nv = 4; break;
+                    case 3: vals = thresholds; nv = 7; break;
+                    default: vals = waves; nv = 5; break;
+                }
+                for (int k = 0; k < nv; k++) {
+                    rb_trace_config c = cur;
+                    uint32_t* f = dim == 0 ? &c.block : dim == 1 ? &c.lds_stack : dim == 2 ? &c.chunk
+                                : dim == 3 ? &c.fetch_threshold : &c.waves_per_cu;
+                    if (*f == vals[k]) continue;
+                    *f = vals[k];
+                    float ms = 0.f;
+                    if ((st = measure(c, &ms)) != RB_OK) goto done;
+                    if (ms < cur_ms * 0.99f) {
+                        cur = c;
+                        cur_ms = ms;
+                    }
+                }
+            }
+        }
+    }
+    dev().cfg = cur;
+    if (best) *best = cur;
+    if (best_ms) *best_ms = cur_ms;
+done:
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return st;
+}
+
 rb_status rb_trace_counts(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits,
                           uint32_t* counts, void* stream) {
     rb_status st = ensure_device();
