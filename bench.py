@@ -65,6 +65,38 @@ def read_traffic(path):
         return None
 
 
+def bench_render(rb, kind, scene, bvh_max_leaf, cam, frames, **params):
+    """Time whole frames (raygen + integrator + film) with HIP events; rays
+    traced per frame come from the device ray counter."""
+    import ctypes as C
+
+    import numpy as np
+    t0 = time.time()
+    bvh = rb.scenes.build(scene, max_leaf=bvh_max_leaf)
+    build_s = time.time() - t0
+    tracer = rb.CudaTracer(bvh, scene)
+    stream = rb.Stream()
+    counter = rb.DeviceBuffer(8)
+    counter.zero()
+    r = rb.SamplerRenderer(tracer, cam, kind, **params)
+    r.params.ray_counter = C.c_void_p(counter.ptr)
+    out = rb.DeviceBuffer(cam.width * cam.height * 12)
+    r.render_device(out, stream=stream.handle)
+    stream.synchronize()
+    rays = int(counter.download(np.uint64, 1)[0])
+    r.params.ray_counter = None
+    e0, e1 = rb.Event(), rb.Event()
+    e0.record(stream)
+    for _ in range(frames):
+        r.render_device(out, stream=stream.handle)
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1) / frames
+    img = out.download(np.float32).reshape(cam.height, cam.width, 3)
+    return {"frame_ms": round(ms, 3), "rays_per_frame": rays, "mrays": round(rays / (ms * 1e-3) / 1e6, 1),
+            "tris": scene.get_num_triangles(), "bvh_build_s": round(build_s, 1), "mean": round(float(img.mean()), 5)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +108,7 @@ def main():
     ap.add_argument("--config", default="", help="block,lds_stack,chunk,fetch_threshold,waves_per_cu")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one copy kernel of this size first (traffic counter calibration)")
+    ap.add_argument("--render", type=int, default=1, help="also time the AO and path-traced frames")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -176,6 +209,17 @@ def main():
         log(f"cpu baseline: {cpu['value']} Mrays/s on {cores} cores, B/ray {bytes_per_ray:.0f}, match={same}")
     if bytes_per_ray is None:
         bytes_per_ray = gpu_bpr
+
+    if args.render:
+        del tracer, d_prim, d_diff, d_mixed, d_hits
+        ao = bench_render(rb, "ao", rb.scenes.ao_room(seed=0), 4, rb.scenes.ao_camera(512, 512), 10,
+                          spp=1, seed=0, ao_samples=8, ao_radius=1.0)
+        log("ao frame (config 1: 78,848 tris, 512x512, 1 spp, 8 AO rays)", ao)
+        extra["ao_512"] = ao
+        pt = bench_render(rb, "pt", rb.scenes.pt_room(seed=0), None, rb.scenes.pt_camera(1024, 1024), 2,
+                          spp=8, seed=0, max_depth=8, rr_start=3, rr_prob=0.5)
+        log("pt frame (config 3: 2,000,136 tris, 1024x1024, 8 spp, depth 8)", pt)
+        extra["pt_1024_8spp"] = pt
 
     achieved = bytes_per_ray * n / (ms * 1e-3) / 1e9
     traffic = read_traffic(os.path.join(ROOT, "profiles", "trace_incoherent_1m_pmc.txt"))

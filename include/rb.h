@@ -283,6 +283,42 @@ rb_status rb_raygen_diffuse(rb_bvh_t bvh, const rb_ray* primaries, const rb_hit*
                             uint32_t n_primary, uint32_t n_out, uint64_t seed, rb_ray* out,
                             void* stream);
 
+/* ---- render passes ------------------------------------------------------- */
+
+/* Parameters of AmbientOcclusionIntegratorParams / PathIntegratorParams /
+ * BdptLvcIntegratorParams (src/integrators/integrator_*.h, defaults in the
+ * integrator XML files) plus the sampler's spp and the RNG seed. */
+typedef struct rb_render_params {
+    uint32_t spp;
+    uint64_t seed;
+    uint32_t ao_samples;   /* AO rays per primary hit */
+    float ao_radius;       /* AO ray tmax */
+    uint32_t max_depth;    /* last bounce index that is shaded */
+    uint32_t rr_start;     /* Russian roulette once bounce > rr_start */
+    float rr_prob;         /* roulette continue probability cap */
+    uint32_t light_paths;  /* BDPT: light sub-paths traced per frame */
+    uint32_t lvc_capacity; /* BDPT: light-vertex cache size */
+    uint32_t connections;  /* BDPT: LVC connections per camera vertex */
+    uint32_t light_max_depth; /* BDPT: light sub-path depth */
+    unsigned long long* ray_counter; /* optional device counter: += rays traced by the call */
+} rb_render_params;
+void rb_render_params_default(rb_render_params* p);
+
+/* One render of the tile [x0,x0+w) x [y0,y0+h): camera rays (spp per pixel),
+ * the integrator, and the per-pixel box-filtered average into out_rgb
+ * (device float[w*h*3], row-major).  Replaces SamplerRenderer::render
+ * (src/renderers/samplerrenderer.cpp:58-107) + Integrator::render
+ * (src/core/integrator.h:59) + kernel_update_film
+ * (src/renderers/samplerrenderer.cu:103-169).  Needs a bvh uploaded with its
+ * scene.  Asynchronous on `stream`; work buffers are stream-ordered allocations. */
+rb_status rb_render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                       const rb_render_params* p, float* out_rgb, void* stream);
+/* Wavefront path tracer (IntegratorPath, src/integrators/integrator_path.cu:43-283):
+ * per bounce shade -> any-hit shadow rays + closest-hit MIS and bounce rays
+ * -> connect -> compaction of live paths; queue sizes stay on the device. */
+rb_status rb_render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                       const rb_render_params* p, float* out_rgb, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -165,6 +165,58 @@ def philox(ctr, key):
     return out
 
 
+def camera_matrices(pos, at, up, hfov, width, height):
+    P = C.c_void_p
+    L = lib()
+    L.orc_camera_matrices.argtypes = [P, P, P, C.c_float, C.c_uint32, C.c_uint32, P, P]
+    a = [np.ascontiguousarray(x, np.float32) for x in (pos, at, up)]
+    r2c = np.empty(16, np.float32)
+    c2w = np.empty(16, np.float32)
+    L.orc_camera_matrices(_p(a[0]), _p(a[1]), _p(a[2]), hfov, width, height, _p(r2c), _p(c2w))
+    return r2c, c2w
+
+
+def render_ao(compact2, verts, idx, cam, tile, spp, seed, nao, radius):
+    """cam = (pos, at, up, hfov, width, height); tile = (x0, y0, w, h) -> float32[h, w, 3]"""
+    nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
+    verts = np.ascontiguousarray(verts, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    r2c, c2w = camera_matrices(*cam)
+    x0, y0, w, h = tile
+    out = np.empty((h, w, 3), np.float32)
+    L = lib()
+    P = C.c_void_p
+    U = C.c_uint32
+    L.orc_render_ao.argtypes = [P] * 7 + [U] * 7 + [C.c_uint64, U, C.c_float, P]
+    L.orc_render_ao(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(r2c), _p(c2w), cam[4], cam[5], x0, y0,
+                    w, h, spp, seed, nao, radius, _p(out))
+    return out
+
+
+def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob):
+    """materials: list of (kind, (r, g, b)); returns float32[h, w, 3]."""
+    nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
+    verts = np.ascontiguousarray(verts, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    tri_mat = np.ascontiguousarray(tri_mat, np.uint32)
+    lights = np.ascontiguousarray(lights, np.uint32)
+    m4 = np.zeros((max(len(materials), 1), 4), np.float32)
+    for i, (k, col) in enumerate(materials):
+        m4[i, 0] = np.array([k], np.uint32).view(np.float32)[0]
+        m4[i, 1:] = col
+    r2c, c2w = camera_matrices(*cam)
+    x0, y0, w, h = tile
+    out = np.empty((h, w, 3), np.float32)
+    L = lib()
+    P = C.c_void_p
+    U = C.c_uint32
+    L.orc_render_pt.argtypes = [P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, P]
+    L.orc_render_pt(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m4), len(materials),
+                    _p(lights), len(lights), _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, max_depth, rr_start,
+                    rr_prob, _p(out))
+    return out
+
+
 def bytes_per_ray(counts, hit):
     """Algorithmic bytes of one ray on the reference traversal:
     32 (ray) + 8 (id,t out) + 64*inner + 48*tris + 16*leaf terminators + 4*[hit]."""

@@ -587,20 +587,28 @@ uint32_t orc_bvh_tri_indices(const orc_bvh* T, uint32_t* out)
 int orc_bvh_compact2(const orc_bvh* T, uint32_t* nodes, size_t* n_node_rows,
                      uint32_t* woop, size_t* n_woop_rows, uint32_t* tri_index, size_t* n_index)
 {
-    /* A root leaf has no Compact2 form (the reference asserts, :237). */
-    if (T->nodes[T->root].leaf) return -1;
+    /* A root leaf has no Compact2 form (the reference asserts, :237); it is
+     * wrapped in an inner node whose second child is an empty leaf with an
+     * inverted (never entered) box. */
+    node_t wrap, empty;
+    const int root_leaf = T->nodes[T->root].leaf;
+    if (root_leaf) {
+        empty.leaf = 1; empty.b = aabb_empty(); empty.child[0] = empty.child[1] = -1; empty.lo = empty.hi = 0;
+        wrap.leaf = 0; wrap.b = T->nodes[T->root].b; wrap.child[0] = T->root; wrap.child[1] = -2;
+        wrap.lo = wrap.hi = 0;
+    }
     size_t cap_stack = 256, sp = 0;
     struct ent { int32_t node; size_t row; } *stack = malloc(cap_stack * sizeof(*stack));
     if (!stack) return -2;
     size_t nrows = 4, wrows = 0, irows = 0;
-    stack[sp].node = T->root; stack[sp].row = 0; sp++;
+    stack[sp].node = root_leaf ? -3 : T->root; stack[sp].row = 0; sp++;
     while (sp) {
         struct ent e = stack[--sp];
-        const node_t* n = &T->nodes[e.node];
+        const node_t* n = e.node == -3 ? &wrap : &T->nodes[e.node];
         const aabb* cb[2];
         int32_t cidx[2];
         for (int i = 0; i < 2; i++) {
-            const node_t* c = &T->nodes[n->child[i]];
+            const node_t* c = n->child[i] == -2 ? &empty : &T->nodes[n->child[i]];
             cb[i] = &c->b;
             if (!c->leaf) {
                 cidx[i] = (int32_t)(nrows * 16 / 16);
@@ -1077,4 +1085,282 @@ void orc_woop_hits(const float* verts, const uint32_t* idx, const int32_t* ids, 
         out_hit[i] = (uint8_t)orc_woop_hit(verts + 3 * (size_t)ind[0], verts + 3 * (size_t)ind[1],
                                            verts + 3 * (size_t)ind[2], rays + 8 * (size_t)i, out_t + i);
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Render passes: AO (src/integrators/integrator_ao.cu:42-85), path tracer   */
+/* (src/integrators/integrator_path.cu:43-261), film average                 */
+/* (src/renderers/samplerrenderer.cu:103-169 with the box filter).           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { uint32_t kind; float color[3]; } orc_material;
+
+typedef struct {
+    const uint32_t *nodes, *woop, *tidx;      /* Compact2 */
+    const float* verts; const uint32_t* idx;  /* scene */
+    const uint32_t* tri_mat; const orc_material* mats;
+    const uint32_t* lights; uint32_t nl;
+} orc_scene;
+
+static void tri_verts(const orc_scene* S, uint32_t t, f3* a, f3* b, f3* c)
+{
+    const uint32_t* ind = S->idx + 3 * (size_t)t;
+    *a = mk3(S->verts[3 * (size_t)ind[0]], S->verts[3 * (size_t)ind[0] + 1], S->verts[3 * (size_t)ind[0] + 2]);
+    *b = mk3(S->verts[3 * (size_t)ind[1]], S->verts[3 * (size_t)ind[1] + 1], S->verts[3 * (size_t)ind[1] + 2]);
+    *c = mk3(S->verts[3 * (size_t)ind[2]], S->verts[3 * (size_t)ind[2] + 1], S->verts[3 * (size_t)ind[2] + 2]);
+}
+
+static f3 tri_norm(f3 a, f3 b, f3 c) { return normalize3(cross3(sub3(b, a), sub3(c, a))); }
+
+static void trace_ray8(const orc_scene* S, const float* r, int any, int32_t* id, float* t)
+{
+    trace_one(S->nodes, S->woop, S->tidx, r, any, id, t, NULL);
+}
+
+static uint32_t sample_gid(uint32_t width, uint32_t x0, uint32_t y0, uint32_t w, uint32_t spp, uint32_t i)
+{
+    uint32_t p = i / spp, s = i % spp;
+    return ((y0 + p / w) * width + (x0 + p % w)) * spp + s;
+}
+
+/* AO: per sample, nao cosine rays from the hit (normal turned towards the
+ * viewer), tmin = 1e-3 t, tmax = radius; value = unoccluded / nao. */
+void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                   const uint32_t* idx, const float* r2c, const float* c2w, uint32_t width, uint32_t height,
+                   uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, uint32_t nao,
+                   float radius, float* out_rgb)
+{
+    (void)height;
+    orc_scene S = {nodes, woop, tidx, verts, idx, NULL, NULL, NULL, 0};
+    uint32_t n = w * h * spp;
+    float* rays = (float*)malloc((size_t)n * 32);
+    orc_raygen_camera(r2c, c2w, width, x0, y0, w, h, spp, seed, 1, rays);
+    orc_rng rng = rng_make(seed, 2u);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long px = 0; px < (long)(w * h); px++) {
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < spp; s++) {
+            uint32_t i = (uint32_t)px * spp + s;
+            const float* R = rays + 8 * (size_t)i;
+            int32_t id; float t;
+            trace_ray8(&S, R, 0, &id, &t);
+            float c = 0.0f;
+            if (id >= 0) {
+                f3 d = mk3(R[4], R[5], R[6]);
+                f3 p = add3(mk3(R[0], R[1], R[2]), mul3s(d, t));
+                f3 a, b, cc;
+                tri_verts(&S, (uint32_t)id, &a, &b, &cc);
+                f3 nn = tri_norm(a, b, cc);
+                if (dot3(nn, d) > 0.0f) nn = mk3(-nn.x, -nn.y, -nn.z);
+                uint32_t open = 0;
+                uint32_t gid = sample_gid(width, x0, y0, w, spp, i);
+                for (uint32_t k = 0; k < nao; k++) {
+                    uint32_t r[4];
+                    rng_draw(rng, gid, k, 0, 0, r);
+                    f3 dir = cosine_dir(nn, u01(r[0]), u01(r[1]));
+                    float ar[8] = {p.x, p.y, p.z, 1.0e-3f * t, dir.x, dir.y, dir.z, radius};
+                    int32_t aid; float at;
+                    trace_ray8(&S, ar, 1, &aid, &at);
+                    open += aid < 0 ? 1u : 0u;
+                }
+                c = (float)open / (float)nao;
+            }
+            sum += c;
+        }
+        float v = sum / (float)spp;
+        out_rgb[3 * px] = v; out_rgb[3 * px + 1] = v; out_rgb[3 * px + 2] = v;
+    }
+    free(rays);
+}
+
+static float light_pdf_mt(f3 p, f3 d, f3 v0, f3 v1, f3 v2, f3 ln, float area)
+{
+    f3 e1 = sub3(v1, v0), e2 = sub3(v2, v0);
+    f3 pv = cross3(d, e2);
+    float det = dot3(e1, pv);
+    if (det < 1.0e-6f && det > -1.0e-6f) return 0.0f;
+    float inv = 1.0f / det;
+    f3 tv = sub3(p, v0);
+    float u = dot3(tv, pv) * inv;
+    if (u < 0.0f || u > 1.0f) return 0.0f;
+    f3 qv = cross3(tv, e1);
+    float v = dot3(d, qv) * inv;
+    if (v < 0.0f || u + v > 1.0f) return 0.0f;
+    float t = dot3(e2, qv) * inv;
+    if (!(t > 0.0f)) return 0.0f;
+    f3 hp = add3(p, mul3s(d, t));
+    f3 dd = sub3(hp, p);
+    float r2 = dot3(dd, dd);
+    float cth = fabsf(dot3(ln, mk3(-d.x, -d.y, -d.z)));
+    return r2 / (area * cth);
+}
+
+static int black3(f3 c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
+
+static f3 lambert_f3(f3 wo, f3 wi, f3 ng, const orc_material* m)
+{
+    if (!(dot3(wo, ng) * dot3(wi, ng) > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
+    const float k = 0.318309886183790672f;
+    return mk3(m->color[0] * k, m->color[1] * k, m->color[2] * k);
+}
+
+/* One path of the path tracer, integrator_path.cu:52-254 with the fixes
+ * listed in the design notes (roulette divides by the continue probability,
+ * a fresh random number per bounce and dimension, true triangle areas). */
+static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng, uint32_t max_depth,
+                  uint32_t rr_start, float rr_prob)
+{
+    f3 L = mk3(0.0f, 0.0f, 0.0f), T = mk3(1.0f, 1.0f, 1.0f);
+    float ray[8];
+    memcpy(ray, R0, 32);
+    int32_t id; float t;
+    trace_ray8(S, ray, 0, &id, &t);
+    if (id < 0) return L;
+    int spec = 0;
+    for (uint32_t bounce = 0;; ) {
+        f3 d = mk3(ray[4], ray[5], ray[6]);
+        f3 p = add3(mk3(ray[0], ray[1], ray[2]), mul3s(d, t));
+        float eps = 1.0e-3f * t;
+        f3 v0, v1, v2;
+        tri_verts(S, (uint32_t)id, &v0, &v1, &v2);
+        const orc_material* m = &S->mats[S->tri_mat[id]];
+        f3 ng = tri_norm(v0, v1, v2);
+        f3 wo = mk3(-d.x, -d.y, -d.z);
+        f3 nf = dot3(ng, wo) >= 0.0f ? ng : mk3(-ng.x, -ng.y, -ng.z);
+        if (m->kind == 2u) {
+            if (bounce == 0 || spec) {
+                f3 le = dot3(ng, wo) > 0.0f ? mk3(m->color[0], m->color[1], m->color[2]) : mk3(0.0f, 0.0f, 0.0f);
+                L = mk3(L.x + T.x * le.x, L.y + T.y * le.y, L.z + T.z * le.z);
+            }
+            break;
+        }
+        uint32_t r0[4], r1[4];
+        rng_draw(rng, gid, bounce, 0, 0, r0);
+        rng_draw(rng, gid, bounce, 1, 0, r1);
+        int lambert = m->kind == 0u;
+        f3 Ld = mk3(0.0f, 0.0f, 0.0f);
+        if (S->nl > 0) {
+            uint32_t lid = (uint32_t)(u01(r0[0]) * (float)S->nl);
+            if (lid > S->nl - 1) lid = S->nl - 1;
+            uint32_t lt = S->lights[lid];
+            f3 l0, l1, l2;
+            tri_verts(S, lt, &l0, &l1, &l2);
+            const orc_material* lm = &S->mats[S->tri_mat[lt]];
+            f3 emit = mk3(lm->color[0], lm->color[1], lm->color[2]);
+            f3 ln = tri_norm(l0, l1, l2);
+            float area = 0.5f * sqrtf(dot3(cross3(sub3(l1, l0), sub3(l2, l0)), cross3(sub3(l1, l0), sub3(l2, l0))));
+            /* light sample */
+            /* uniform_sample_triangle, src/core/montecarlo.cu:157-164 */
+            float su1 = sqrtf(u01(r0[1]));
+            float uu = 1.0f - su1, vv = u01(r0[2]) * su1;
+            f3 pl = add3(add3(mul3s(l0, uu), mul3s(l1, vv)), mul3s(l2, 1.0f - uu - vv));
+            f3 wi = normalize3(sub3(pl, p));
+            float lpdf = light_pdf_mt(p, wi, l0, l1, l2, ln, area);
+            f3 Li = dot3(ln, mk3(-wi.x, -wi.y, -wi.z)) > 0.0f ? emit : mk3(0.0f, 0.0f, 0.0f);
+            if (lambert && !black3(Li) && lpdf > 0.0f) {
+                f3 f = lambert_f3(wo, wi, ng, m);
+                if (!black3(f)) {
+                    float bpdf = fabsf(dot3(wi, nf)) * 0.318309886183790672f;
+                    float w = lpdf / (lpdf + bpdf);
+                    float s = fabsf(dot3(wi, ng)) * w;
+                    f3 ca = mk3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
+                    f3 seg = sub3(pl, p);
+                    float dist = sqrtf(dot3(seg, seg));
+                    f3 sd = normalize3(seg);
+                    float sr[8] = {p.x, p.y, p.z, eps, sd.x, sd.y, sd.z, dist * (1.0f - 1.0e-3f)};
+                    int32_t sid; float st;
+                    trace_ray8(S, sr, 1, &sid, &st);
+                    if (sid < 0) Ld = add3(Ld, ca);
+                }
+            }
+            if (lambert) {
+                f3 wi2 = cosine_dir(nf, u01(r0[3]), u01(r1[0]));
+                f3 f2 = lambert_f3(wo, wi2, ng, m);
+                float pdf2 = fabsf(dot3(wi2, nf)) * 0.318309886183790672f;
+                if (!black3(f2) && pdf2 > 0.0f) {
+                    float lpdf2 = light_pdf_mt(p, wi2, l0, l1, l2, ln, area);
+                    if (lpdf2 > 0.0f) {
+                        float w2 = pdf2 / (pdf2 + lpdf2);
+                        f3 Li2 = dot3(ln, mk3(-wi2.x, -wi2.y, -wi2.z)) > 0.0f ? emit : mk3(0.0f, 0.0f, 0.0f);
+                        float s2 = fabsf(dot3(wi2, ng)) * w2;
+                        f3 cb = mk3(f2.x * Li2.x * s2 / pdf2, f2.y * Li2.y * s2 / pdf2, f2.z * Li2.z * s2 / pdf2);
+                        float mr[8] = {p.x, p.y, p.z, eps, wi2.x, wi2.y, wi2.z, FLT_MAX};
+                        int32_t mid2; float mt;
+                        trace_ray8(S, mr, 0, &mid2, &mt);
+                        if (mid2 == (int32_t)lt) Ld = add3(Ld, cb);
+                    }
+                }
+            }
+        }
+        Ld = mul3s(Ld, (float)S->nl);
+        L = mk3(L.x + T.x * Ld.x, L.y + T.y * Ld.y, L.z + T.z * Ld.z);
+        /* next bounce */
+        f3 wi3, f3v;
+        float pdf3;
+        int spec3;
+        if (lambert) {
+            wi3 = cosine_dir(nf, u01(r1[1]), u01(r1[2]));
+            f3v = lambert_f3(wo, wi3, ng, m);
+            pdf3 = fabsf(dot3(wi3, nf)) * 0.318309886183790672f;
+            spec3 = 0;
+        } else {
+            float c = 2.0f * dot3(nf, wo);
+            wi3 = sub3(mul3s(nf, c), wo);
+            float ac = fabsf(dot3(wi3, nf));
+            f3v = mk3(m->color[0] / ac, m->color[1] / ac, m->color[2] / ac);
+            pdf3 = 1.0f;
+            spec3 = 1;
+        }
+        if (black3(f3v) || !(pdf3 > 0.0f)) break;
+        float s3 = fabsf(dot3(wi3, ng));
+        f3 Tn = mk3(T.x * (f3v.x * s3 / pdf3), T.y * (f3v.y * s3 / pdf3), T.z * (f3v.z * s3 / pdf3));
+        int alive = 1;
+        if (bounce > rr_start) {
+            float lum = 0.212671f * Tn.x + 0.715160f * Tn.y + 0.072169f * Tn.z;
+            float q = rr_prob < lum ? rr_prob : lum;
+            if (u01(r1[3]) > q) alive = 0;
+            else Tn = mk3(Tn.x / q, Tn.y / q, Tn.z / q);
+        }
+        if (!alive || bounce == max_depth) break;
+        float nr[8] = {p.x, p.y, p.z, eps, wi3.x, wi3.y, wi3.z, FLT_MAX};
+        trace_ray8(S, nr, 0, &id, &t);
+        if (id < 0) break;
+        memcpy(ray, nr, 32);
+        T = Tn;
+        spec = spec3;
+        bounce++;
+    }
+    return L;
+}
+
+void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                   const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
+                   const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
+                   uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
+                   uint32_t max_depth, uint32_t rr_start, float rr_prob, float* out_rgb)
+{
+    orc_material* mats = (orc_material*)malloc(sizeof(orc_material) * (nmats ? nmats : 1));
+    for (uint32_t i = 0; i < nmats; i++) {
+        uint32_t k; memcpy(&k, &mats4[4 * i], 4);
+        mats[i].kind = k;
+        mats[i].color[0] = mats4[4 * i + 1]; mats[i].color[1] = mats4[4 * i + 2]; mats[i].color[2] = mats4[4 * i + 3];
+    }
+    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
+    uint32_t n = w * h * spp;
+    float* rays = (float*)malloc((size_t)n * 32);
+    orc_raygen_camera(r2c, c2w, width, x0, y0, w, h, spp, seed, 1, rays);
+    orc_rng rng = rng_make(seed, 5u);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long px = 0; px < (long)(w * h); px++) {
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        for (uint32_t s = 0; s < spp; s++) {
+            uint32_t i = (uint32_t)px * spp + s;
+            f3 L = pt_path(&S, rays + 8 * (size_t)i, sample_gid(width, x0, y0, w, spp, i), rng, max_depth, rr_start, rr_prob);
+            r += L.x; g += L.y; b += L.z;
+        }
+        float inv = (float)spp;
+        out_rgb[3 * px] = r / inv; out_rgb[3 * px + 1] = g / inv; out_rgb[3 * px + 2] = b / inv;
+    }
+    free(rays);
+    free(mats);
 }

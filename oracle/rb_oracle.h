@@ -120,6 +120,23 @@ void orc_raygen_camera(const float* r2c, const float* c2w, uint32_t width, uint3
 void orc_raygen_diffuse(const float* verts, const uint32_t* idx, const float* prim, const int32_t* hit_id,
                         const float* hit_t, uint32_t np, uint32_t nout, uint64_t seed, float* out);
 
+
+/* ---- render passes ------------------------------------------------------ */
+
+/* AO (src/integrators/integrator_ao.cu:42-85) over the tile, box-filtered
+ * average of spp samples per pixel into out_rgb[w*h*3]. */
+void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                   const uint32_t* idx, const float* r2c, const float* c2w, uint32_t width, uint32_t height,
+                   uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, uint32_t nao,
+                   float radius, float* out_rgb);
+/* Path tracer (src/integrators/integrator_path.cu:43-261): NEE with MIS,
+ * Lambert / mirror / emitter materials (mats4: kind bits + rgb per material). */
+void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                   const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
+                   const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
+                   uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
+                   uint32_t max_depth, uint32_t rr_start, float rr_prob, float* out_rgb);
+
 #ifdef __cplusplus
 }
 #endif

@@ -120,7 +120,16 @@ _SIGS = {
     "rb_camera_matrices": (C.c_int, [P, P, P]),
     "rb_raygen_camera": (C.c_int, [P, U32, U32, U32, U32, U32, U64, C.c_int, P, P]),
     "rb_raygen_diffuse": (C.c_int, [P, P, P, U32, U32, U64, P, P]),
+    "rb_render_params_default": (None, [P]),
+    "rb_render_ao": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
+    "rb_render_pt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
 }
+
+
+class rb_render_params(C.Structure):
+    _fields_ = [("spp", U32), ("seed", U64), ("ao_samples", U32), ("ao_radius", F32), ("max_depth", U32),
+                ("rr_start", U32), ("rr_prob", F32), ("light_paths", U32), ("lvc_capacity", U32),
+                ("connections", U32), ("light_max_depth", U32), ("ray_counter", P)]
 
 _L = None
 

@@ -413,6 +413,48 @@ class PerspectiveCamera:
         return w * h * spp
 
 
+def render_params(**kw):
+    """rb_render_params with the library defaults, overridden by keywords."""
+    p = L.rb_render_params()
+    lib().rb_render_params_default(C.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class SamplerRenderer:
+    """Camera rays + integrator + film for one tile (SamplerRenderer::render,
+    src/renderers/samplerrenderer.cpp:58-107; Integrator::render, src/core/integrator.h:59)."""
+
+    INTEGRATORS = {"ao": "rb_render_ao", "pt": "rb_render_pt"}
+
+    def __init__(self, tracer, camera, integrator="ao", **params):
+        if integrator not in self.INTEGRATORS:
+            raise ValueError(f"unknown integrator {integrator!r}")
+        self.tracer, self.camera, self.integrator = tracer, camera, integrator
+        self.params = render_params(**params)
+
+    def render_device(self, d_out, tile=None, stream=None):
+        x0, y0, w, h = tile if tile else (0, 0, self.camera.width, self.camera.height)
+        fn = getattr(lib(), self.INTEGRATORS[self.integrator])
+        check(fn(self.tracer.handle, C.byref(self.camera.c), x0, y0, w, h, C.byref(self.params), _dp(d_out),
+                 _sp(stream)))
+        return w, h
+
+    def render(self, tile=None, stream=None):
+        """Host convenience: returns float32[h, w, 3]."""
+        x0, y0, w, h = tile if tile else (0, 0, self.camera.width, self.camera.height)
+        d = DeviceBuffer(w * h * 12)
+        self.render_device(d, (x0, y0, w, h), stream)
+        if stream is not None:
+            _sync(stream)
+        return d.download(np.float32, w * h * 3).reshape(h, w, 3)
+
+
+def _sync(stream):
+    check(lib().rb_stream_synchronize(_sp(stream)))
+
+
 def raygen_diffuse_device(tracer, d_prim, d_hits, n_primary, n_out, seed, d_out, stream=None):
     check(lib().rb_raygen_diffuse(tracer.handle, _dp(d_prim), _dp(d_hits), n_primary, n_out, seed, _dp(d_out),
                                   _sp(stream)))

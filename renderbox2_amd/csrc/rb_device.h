@@ -30,6 +30,13 @@ rb_status ensure_device();
 uint32_t* take_counter_slot();
 rb_status hip_fail(hipError_t e, const char* what);
 
+// Persistent traversal launch (rb_trace.hip).  n_dev, when set, holds the ray
+// count in device memory (queues sized by earlier kernels); n is then an
+// upper bound that only sizes the grid.
+rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
+                       rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg,
+                       const uint32_t* n_dev = nullptr);
+
 // Device allocation tracked against the library's HBM accounting
 // (MemoryAllocator::allocate, src/memory/memoryallocator.cpp:54-67).
 rb_status dalloc(void** p, size_t bytes);

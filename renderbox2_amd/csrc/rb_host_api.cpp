@@ -353,8 +353,8 @@ extern "C" rb_status rb_scene_generate(rb_scene_t s, const rb_gen_params* p) {
                                (float)(cx + side / 2), (float)y, (float)(cz - side / 2),
                                (float)(cx + side / 2), (float)y, (float)(cz + side / 2),
                                (float)(cx - side / 2), (float)y, (float)(cz + side / 2)};
-                // facing down (-y): (v1-v0) x (v2-v0) must point to -y
-                const uint32_t qt[6] = {0, 2, 1, 0, 3, 2};
+                // facing down (-y): (v1-v0) x (v2-v0) = (s,0,0) x (s,0,s) = -s^2 y
+                const uint32_t qt[6] = {0, 1, 2, 0, 2, 3};
                 if ((st = rb_scene_add_mesh(s, q, 4, qt, 2, mat)) != RB_OK) return st;
             }
             return RB_OK;

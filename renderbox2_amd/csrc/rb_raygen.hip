@@ -11,9 +11,6 @@
 
 namespace rb {
 
-rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
-                       rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg);
-
 struct CamArgs {
     float r2c[16];
     float c2w[16];

@@ -1,0 +1,551 @@
+// rb_render.hip -- AO and path-tracing passes around the persistent tracer.
+//
+// Wavefront forms of the reference megakernels:
+//  * AO: kernel_ambient_occlusion (src/integrators/integrator_ao.cu:42-85)
+//    becomes spawn (n rays per primary hit) -> one any-hit trace launch ->
+//    resolve;
+//  * PT: kernel_pathtracer_single_monolithic (src/integrators/integrator_path.cu:43-261)
+//    becomes, per bounce: shade (emission, light sample, MIS BSDF sample,
+//    next-bounce sample, roulette) -> any-hit trace of the shadow rays and one
+//    closest-hit trace of the MIS + bounce rays -> connect (direct light,
+//    next hit) -> compaction of the live paths into the next queue.  Queue
+//    lengths live in device memory, so the bounce loop needs no host
+//    synchronisation (the reference counts live paths with thrust on the
+//    host every bounce, integrator_bdpt_lvc_mk.cu:529-544);
+//  * film: per-pixel box-filtered average of the spp samples in sample order
+//    (kernel_update_film, src/renderers/samplerrenderer.cu:103-169, without
+//    float atomics, so images are bit-reproducible).
+#include "rb_shade.h"
+
+namespace rb {
+
+struct TileArgs {
+    uint32_t width, x0, y0, w, spp;
+};
+
+// Global sample id of primary ray i of a tile: (image pixel) * spp + sample.
+__device__ __forceinline__ uint32_t sample_gid(const TileArgs& T, uint32_t i) {
+    const uint32_t p = i / T.spp, s = i % T.spp;
+    return ((T.y0 + p / T.w) * T.width + (T.x0 + p % T.w)) * T.spp + s;
+}
+
+// ---------------------------------------------------------------- AO ----
+
+__global__ void k_ao_spawn(const float4* __restrict__ prim, const int2* __restrict__ hits, uint32_t n, TileArgs T,
+                           SceneDev S, uint32_t nao, float radius, Rng rng, float4* __restrict__ rays,
+                           uint8_t* __restrict__ mask, unsigned long long* ray_counter) {
+    const uint64_t total = (uint64_t)n * nao;
+    for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < total; j += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t i = (uint32_t)(j / nao), k = (uint32_t)(j % nao);
+        const int2 h = hits[i];
+        if (ray_counter && k == 0 && h.x >= 0) atomicAdd(ray_counter, (unsigned long long)nao);
+        if (h.x < 0) {
+            mask[j] = 0;
+            rays[2 * j] = make_float4(0.f, 0.f, 0.f, 0.f);
+            rays[2 * j + 1] = make_float4(0.f, 0.f, 1.f, 0.f);
+            continue;
+        }
+        const float4 o = prim[2 * (size_t)i], d = prim[2 * (size_t)i + 1];
+        const float t = __int_as_float(h.y);
+        const V3 dv = v3(d.x, d.y, d.z);
+        const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
+        V3 v0, v1, v2;
+        tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, nullptr);
+        V3 n = tri_normal(v0, v1, v2);
+        if (vdot(n, dv) > 0.0f) n = vneg(n);
+        const U4 r = rng.draw(sample_gid(T, i), k, 0, 0);
+        const V3 dir = cosine_hemisphere(n, u01(r.x), u01(r.y));
+        rays[2 * j] = make_float4(p.x, p.y, p.z, 1.0e-3f * t);
+        rays[2 * j + 1] = make_float4(dir.x, dir.y, dir.z, radius);
+        mask[j] = 1;
+    }
+}
+
+// Per pixel: mean over samples of (unoccluded AO rays / nao); misses give 0.
+__global__ void k_ao_resolve(const int2* __restrict__ prim_hits, const int2* __restrict__ ao_hits, uint32_t npix,
+                             uint32_t spp, uint32_t nao, float* __restrict__ out) {
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+        float sum = 0.0f;
+        for (uint32_t s = 0; s < spp; s++) {
+            const uint32_t i = px * spp + s;
+            float c = 0.0f;
+            if (prim_hits[i].x >= 0) {
+                uint32_t open = 0;
+                for (uint32_t k = 0; k < nao; k++) open += ao_hits[(size_t)i * nao + k].x < 0 ? 1u : 0u;
+                c = (float)open / (float)nao;
+            }
+            sum += c;
+        }
+        const float v = sum / (float)spp;
+        out[3 * (size_t)px] = v;
+        out[3 * (size_t)px + 1] = v;
+        out[3 * (size_t)px + 2] = v;
+    }
+}
+
+// ---------------------------------------------------------------- PT ----
+
+struct PathBufs {
+    float4* T;       // throughput (w unused)
+    float4* Tn;      // throughput after this bounce's sample
+    float4* L;       // radiance
+    float4* ray;     // 2 per path: current ray
+    int2* hit;       // current hit
+    float4* ca;      // light-sample contribution if unoccluded; w = n/a
+    float4* cb;      // MIS contribution if the MIS ray hits light `lprim`
+    int4* slot;      // x: shadow slot, y: MIS slot, z: bounce slot, w: light triangle
+    uint32_t* info;  // bounce | specular << 8 | next specular << 9 | emitter-done << 10
+    uint32_t* q[2];  // path queues
+    uint32_t* cnt;   // [0],[1] queue lengths, [2] shadow rays, [3] closest rays
+    float4* ra;      // shadow rays
+    int2* ha;
+    float4* rbc;     // MIS + bounce rays
+    int2* hbc;
+};
+
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
+    // one atomic per wave: lanes that want a slot get consecutive ones
+    const uint64_t m = __ballot(want);
+    uint32_t base = 0;
+    if (m) {
+        const int leader = __builtin_ctzll(m);
+        if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+    }
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return want ? base + (uint32_t)__popcll(m & lt) : 0xFFFFFFFFu;
+}
+
+__global__ void k_pt_init(const int2* __restrict__ hits, const float4* __restrict__ prim, uint32_t n, PathBufs P) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 h = hits[i];
+        P.T[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+        P.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        P.ray[2 * (size_t)i] = prim[2 * (size_t)i];
+        P.ray[2 * (size_t)i + 1] = prim[2 * (size_t)i + 1];
+        P.hit[i] = h;
+        P.info[i] = 0;
+        const uint32_t q = wave_append(&P.cnt[0], h.x >= 0);
+        if (h.x >= 0) P.q[0][q] = i;
+    }
+}
+
+__global__ void k_reset(uint32_t* c, uint32_t a, uint32_t b, uint32_t d) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        c[a] = 0;
+        c[b] = 0;
+        c[d] = 0;
+    }
+}
+
+__global__ void k_add64(unsigned long long* total, unsigned long long v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(total, v);
+}
+
+// Adds the rays of this bounce (shadow + closest queues) to the caller's counter.
+__global__ void k_count_rays(const uint32_t* c, unsigned long long* total, unsigned long long extra) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && total)
+        atomicAdd(total, (unsigned long long)c[2] + (unsigned long long)c[3] + extra);
+}
+
+__global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng rng, uint32_t max_depth,
+                           uint32_t rr_start, float rr_prob) {
+    const uint32_t count = P.cnt[qi];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t end = (count + 63u) & ~63u;  // whole waves iterate together (wave_append)
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
+        const bool valid = q < count;
+        const uint32_t pid = valid ? P.q[qi][q] : 0u;
+        bool wantA = false, wantB = false, wantC = false;
+        float4 ra0, ra1, rb1, rc1;
+        V3 p = v3(0.f, 0.f, 0.f);
+        float eps = 0.0f;
+        if (valid) {
+            const float4 o = P.ray[2 * (size_t)pid], d = P.ray[2 * (size_t)pid + 1];
+            const int2 h = P.hit[pid];
+            const uint32_t info = P.info[pid];
+            const uint32_t bounce = info & 255u;
+            const bool spec = (info >> 8) & 1u;
+            const float t = __int_as_float(h.y);
+            const V3 dv = v3(d.x, d.y, d.z);
+            p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
+            eps = 1.0e-3f * t;
+            V3 v0, v1, v2;
+            uint32_t mid;
+            tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid);
+            const rb_material m = S.mats[mid];
+            const V3 ng = tri_normal(v0, v1, v2);
+            const V3 wo = vneg(dv);
+            const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
+            const float4 Tp = P.T[pid];
+            const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
+            int4 slot = make_int4(-1, -1, -1, -1);
+            if (m.kind == RB_MAT_EMITTER) {
+                // emission seen directly or through mirrors (integrator_path.cu:94-109), then stop
+                uint32_t ninfo = info | (1u << 10);
+                if (bounce == 0 || spec) {
+                    const V3 le = vdot(ng, wo) > 0.0f ? v3(m.color[0], m.color[1], m.color[2]) : v3(0.f, 0.f, 0.f);
+                    const float4 L = P.L[pid];
+                    P.L[pid] = make_float4(L.x + Tv.x * le.x, L.y + Tv.y * le.y, L.z + Tv.z * le.z, 0.f);
+                }
+                P.info[pid] = ninfo;
+                P.slot[pid] = slot;
+            } else {
+                const uint32_t gid = sample_gid(T, pid);
+                const U4 r0 = rng.draw(gid, bounce, 0, 0);
+                const U4 r1 = rng.draw(gid, bounce, 1, 0);
+                V3 ca = v3(0.f, 0.f, 0.f), cb = v3(0.f, 0.f, 0.f);
+                const bool lambert = m.kind == RB_MAT_LAMBERT;
+                if (S.n_lights > 0) {
+                    uint32_t lid = (uint32_t)(u01(r0.x) * (float)S.n_lights);
+                    if (lid > S.n_lights - 1) lid = S.n_lights - 1;
+                    const uint32_t lt = S.light_tris[lid];
+                    V3 l0, l1, l2;
+                    uint32_t lmid;
+                    tri_fetch(S, lt, &l0, &l1, &l2, &lmid);
+                    const rb_material lm = S.mats[lmid];
+                    const V3 emit = v3(lm.color[0], lm.color[1], lm.color[2]);
+                    const V3 ln = tri_normal(l0, l1, l2);
+                    const float area = tri_area(l0, l1, l2);
+                    slot.w = (int)lt;
+                    // light sample (DiffuseAreaLight::sample_L, src/core/lights.cu:56-65)
+                    const V3 pl = sample_triangle(l0, l1, l2, u01(r0.y), u01(r0.z));
+                    const V3 wi = vnorm(vsub(pl, p));
+                    const float lpdf = light_pdf(p, wi, l0, l1, l2, ln, area);
+                    const V3 Li = vdot(ln, vneg(wi)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
+                    if (lambert && !is_black(Li) && lpdf > 0.0f) {
+                        const V3 f = lambert_f(wo, wi, ng, m);
+                        if (!is_black(f)) {
+                            const float bpdf = fabsf(vdot(wi, nf)) * 0.318309886183790672f;
+                            const float w = lpdf / (lpdf + bpdf);
+                            const float s = fabsf(vdot(wi, ng)) * w;
+                            ca = v3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
+                            const V3 seg = vsub(pl, p);
+                            const float dist = sqrtf(vdot(seg, seg));
+                            const V3 sd = vnorm(seg);
+                            ra0 = make_float4(p.x, p.y, p.z, eps);
+                            ra1 = make_float4(sd.x, sd.y, sd.z, dist * (1.0f - 1.0e-3f));
+                            wantA = true;
+                        }
+                    }
+                    // BSDF sample with MIS (non-specular components only)
+                    if (lambert) {
+                        const V3 wi2 = cosine_hemisphere(nf, u01(r0.w), u01(r1.x));
+                        const V3 f2 = lambert_f(wo, wi2, ng, m);
+                        const float pdf2 = fabsf(vdot(wi2, nf)) * 0.318309886183790672f;
+                        if (!is_black(f2) && pdf2 > 0.0f) {
+                            const float lpdf2 = light_pdf(p, wi2, l0, l1, l2, ln, area);
+                            if (lpdf2 > 0.0f) {
+                                const float w2 = pdf2 / (pdf2 + lpdf2);
+                                const V3 Li2 = vdot(ln, vneg(wi2)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
+                                const float s2 = fabsf(vdot(wi2, ng)) * w2;
+                                cb = v3(f2.x * Li2.x * s2 / pdf2, f2.y * Li2.y * s2 / pdf2, f2.z * Li2.z * s2 / pdf2);
+                                rb1 = make_float4(wi2.x, wi2.y, wi2.z, 3.402823466e+38f);
+                                wantB = true;
+                            }
+                        }
+                    }
+                }
+                // next bounce (material_sample_f with every component)
+                V3 wi3, f3;
+                float pdf3;
+                bool spec3;
+                if (lambert) {
+                    wi3 = cosine_hemisphere(nf, u01(r1.y), u01(r1.z));
+                    f3 = lambert_f(wo, wi3, ng, m);
+                    pdf3 = fabsf(vdot(wi3, nf)) * 0.318309886183790672f;
+                    spec3 = false;
+                } else {  // mirror (sample_mirror, src/core/reflection.cu:76-92)
+                    const float c = 2.0f * vdot(nf, wo);
+                    wi3 = vsub(vmul(nf, c), wo);
+                    const float ac = fabsf(vdot(wi3, nf));
+                    f3 = v3(m.color[0] / ac, m.color[1] / ac, m.color[2] / ac);
+                    pdf3 = 1.0f;
+                    spec3 = true;
+                }
+                uint32_t ninfo = info & 0x1FFu;
+                if (!is_black(f3) && pdf3 > 0.0f) {
+                    const float s3 = fabsf(vdot(wi3, ng));
+                    V3 Tn = v3(Tv.x * (f3.x * s3 / pdf3), Tv.y * (f3.y * s3 / pdf3), Tv.z * (f3.z * s3 / pdf3));
+                    bool alive = true;
+                    if (bounce > rr_start) {
+                        const float lum = luminance(Tn);
+                        const float q = rr_prob < lum ? rr_prob : lum;
+                        if (u01(r1.w) > q) alive = false;
+                        else Tn = v3(Tn.x / q, Tn.y / q, Tn.z / q);
+                    }
+                    P.Tn[pid] = make_float4(Tn.x, Tn.y, Tn.z, 0.f);
+                    if (alive && bounce != max_depth) {
+                        rc1 = make_float4(wi3.x, wi3.y, wi3.z, 3.402823466e+38f);
+                        wantC = true;
+                        if (spec3) ninfo |= 1u << 9;
+                    }
+                }
+                P.ca[pid] = make_float4(ca.x, ca.y, ca.z, 0.f);
+                P.cb[pid] = make_float4(cb.x, cb.y, cb.z, 0.f);
+                P.info[pid] = ninfo;
+                P.slot[pid] = slot;
+            }
+        }
+        // wave-aggregated appends (every lane of the wave reaches these)
+        const uint32_t sa = wave_append(&P.cnt[2], wantA);
+        const uint32_t sb = wave_append(&P.cnt[3], wantB);
+        const uint32_t sc = wave_append(&P.cnt[3], wantC);
+        if (wantA) {
+            P.ra[2 * (size_t)sa] = ra0;
+            P.ra[2 * (size_t)sa + 1] = ra1;
+        }
+        if (wantB) {
+            P.rbc[2 * (size_t)sb] = make_float4(p.x, p.y, p.z, eps);
+            P.rbc[2 * (size_t)sb + 1] = rb1;
+        }
+        if (wantC) {
+            P.rbc[2 * (size_t)sc] = make_float4(p.x, p.y, p.z, eps);
+            P.rbc[2 * (size_t)sc + 1] = rc1;
+        }
+        if (valid && (wantA || wantB || wantC)) {
+            int4 s = P.slot[pid];
+            s.x = wantA ? (int)sa : -1;
+            s.y = wantB ? (int)sb : -1;
+            s.z = wantC ? (int)sc : -1;
+            P.slot[pid] = s;
+        }
+    }
+}
+
+__global__ void k_pt_connect(PathBufs P, uint32_t qi, float nlights) {
+    const uint32_t count = P.cnt[qi];
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t end = (count + 63u) & ~63u;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
+        const bool valid = q < count;
+        bool next = false;
+        uint32_t pid = 0;
+        if (valid) {
+            pid = P.q[qi][q];
+            const uint32_t info = P.info[pid];
+            if (!(info & (1u << 10))) {
+                const int4 s = P.slot[pid];
+                V3 Ld = v3(0.f, 0.f, 0.f);
+                if (s.x >= 0 && P.ha[s.x].x < 0) {
+                    const float4 c = P.ca[pid];
+                    Ld = vadd(Ld, v3(c.x, c.y, c.z));
+                }
+                if (s.y >= 0 && P.hbc[s.y].x == s.w) {
+                    const float4 c = P.cb[pid];
+                    Ld = vadd(Ld, v3(c.x, c.y, c.z));
+                }
+                Ld = vmul(Ld, nlights);
+                const float4 Tp = P.T[pid];
+                const float4 L = P.L[pid];
+                P.L[pid] = make_float4(L.x + Tp.x * Ld.x, L.y + Tp.y * Ld.y, L.z + Tp.z * Ld.z, 0.f);
+                if (s.z >= 0) {
+                    const int2 h = P.hbc[s.z];
+                    if (h.x >= 0) {
+                        P.ray[2 * (size_t)pid] = P.rbc[2 * (size_t)s.z];
+                        P.ray[2 * (size_t)pid + 1] = P.rbc[2 * (size_t)s.z + 1];
+                        P.hit[pid] = h;
+                        P.T[pid] = P.Tn[pid];
+                        const uint32_t b = (info & 255u) + 1u;
+                        P.info[pid] = b | (((info >> 9) & 1u) << 8);
+                        next = true;
+                    }
+                }
+            }
+        }
+        const uint32_t o = wave_append(&P.cnt[qi ^ 1], next);
+        if (next) P.q[qi ^ 1][o] = pid;
+    }
+}
+
+__global__ void k_film(const float4* __restrict__ L, uint32_t npix, uint32_t spp, float* __restrict__ out) {
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+        float r = 0.f, g = 0.f, b = 0.f;
+        for (uint32_t s = 0; s < spp; s++) {
+            const float4 v = L[(size_t)px * spp + s];
+            r += v.x;
+            g += v.y;
+            b += v.z;
+        }
+        const float inv = (float)spp;
+        out[3 * (size_t)px] = r / inv;
+        out[3 * (size_t)px + 1] = g / inv;
+        out[3 * (size_t)px + 2] = b / inv;
+    }
+}
+
+static uint32_t grid_of(uint64_t n) {
+    uint64_t g = (n + 255) / 256;
+    const uint64_t cap = (uint64_t)dev().num_cus * 16;
+    if (g > cap) g = cap;
+    return (uint32_t)(g ? g : 1);
+}
+
+// Stream-ordered scratch allocations released at the end of a render call.
+struct Scratch {
+    hipStream_t s;
+    std::vector<void*> ptrs;
+    rb_status st = RB_OK;
+    explicit Scratch(hipStream_t st_) : s(st_) {}
+    template <class T>
+    T* get(size_t count) {
+        void* p = nullptr;
+        if (st != RB_OK) return nullptr;
+        hipError_t e = hipMallocAsync(&p, count * sizeof(T) + 16, s);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            st = fail(e == hipErrorOutOfMemory ? RB_ERR_OOM : RB_ERR_DEVICE, "render scratch (%zu bytes): %s",
+                      count * sizeof(T), hipGetErrorString(e));
+            return nullptr;
+        }
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFreeAsync(p, s);
+    }
+};
+
+static rb_status check_render(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                              const rb_render_params* p, float* out) {
+    if (!bvh || !cam || !p || !out) return fail(RB_ERR_INVALID, "NULL argument");
+    if (!bvh->tri_v) return fail(RB_ERR_INVALID, "bvh was uploaded without its scene");
+    if (w == 0 || h == 0 || x0 + w > cam->width || y0 + h > cam->height) return fail(RB_ERR_INVALID, "bad tile");
+    if (p->spp == 0) return fail(RB_ERR_INVALID, "spp must be >= 1");
+    if ((uint64_t)w * h * p->spp > (1ull << 30)) return fail(RB_ERR_INVALID, "tile x spp too large for one call");
+    return RB_OK;
+}
+
+static SceneDev scene_of(rb_bvh_t b) {
+    SceneDev S;
+    S.tri_v = b->tri_v;
+    S.mats = b->mats;
+    S.light_tris = b->light_tris;
+    S.n_lights = b->n_lights;
+    return S;
+}
+
+rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    const rb_render_params* p, float* out, hipStream_t s) {
+    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
+    if (st != RB_OK) return st;
+    if (p->ao_samples == 0) return fail(RB_ERR_INVALID, "ao_samples must be >= 1");
+    const uint32_t n = w * h * p->spp;
+    const uint64_t na = (uint64_t)n * p->ao_samples;
+    if (na > 0x7FFFFFFFull) return fail(RB_ERR_INVALID, "too many AO rays for one call");
+    Scratch M(s);
+    float4* rays = M.get<float4>(2 * (size_t)n);
+    int2* hits = M.get<int2>(n);
+    float4* arays = M.get<float4>(2 * na);
+    uint8_t* amask = M.get<uint8_t>(na);
+    int2* ahits = M.get<int2>(na);
+    if (M.st != RB_OK) return M.st;
+    if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)rays, s)) != RB_OK) return st;
+    if ((st = launch_trace(bvh, (rb_ray*)rays, n, RB_CLOSEST, nullptr, (rb_hit*)hits, nullptr, s, dev().cfg)) != RB_OK)
+        return st;
+    TileArgs T{cam->width, x0, y0, w, p->spp};
+    hipLaunchKernelGGL(k_ao_spawn, dim3(grid_of(na)), dim3(256), 0, s, rays, hits, n, T, scene_of(bvh), p->ao_samples,
+                       p->ao_radius, Rng::make(p->seed, RNG_AO), arays, amask, p->ray_counter);
+    if (p->ray_counter) hipLaunchKernelGGL(k_add64, dim3(1), dim3(64), 0, s, p->ray_counter, (unsigned long long)n);
+    if ((st = launch_trace(bvh, (rb_ray*)arays, (uint32_t)na, RB_ANY | RB_MASKED, amask, (rb_hit*)ahits, nullptr, s,
+                           dev().cfg)) != RB_OK)
+        return st;
+    hipLaunchKernelGGL(k_ao_resolve, dim3(grid_of(w * h)), dim3(256), 0, s, hits, ahits, w * h, p->spp,
+                       p->ao_samples, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RB_OK : hip_fail(e, "AO launch");
+}
+
+rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    const rb_render_params* p, float* out, hipStream_t s) {
+    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
+    if (st != RB_OK) return st;
+    if (p->max_depth > 250) return fail(RB_ERR_INVALID, "max_depth must be <= 250");
+    const uint32_t n = w * h * p->spp;
+    Scratch M(s);
+    float4* prim = M.get<float4>(2 * (size_t)n);
+    int2* phits = M.get<int2>(n);
+    PathBufs P;
+    P.T = M.get<float4>(n);
+    P.Tn = M.get<float4>(n);
+    P.L = M.get<float4>(n);
+    P.ray = M.get<float4>(2 * (size_t)n);
+    P.hit = M.get<int2>(n);
+    P.ca = M.get<float4>(n);
+    P.cb = M.get<float4>(n);
+    P.slot = M.get<int4>(n);
+    P.info = M.get<uint32_t>(n);
+    P.q[0] = M.get<uint32_t>(n);
+    P.q[1] = M.get<uint32_t>(n);
+    P.cnt = M.get<uint32_t>(4);
+    P.ra = M.get<float4>(2 * (size_t)n);
+    P.ha = M.get<int2>(n);
+    P.rbc = M.get<float4>(4 * (size_t)n);
+    P.hbc = M.get<int2>(2 * (size_t)n);
+    if (M.st != RB_OK) return M.st;
+    if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)prim, s)) != RB_OK) return st;
+    if ((st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg)) != RB_OK)
+        return st;
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, 0u, 1u, 2u);
+    hipLaunchKernelGGL(k_pt_init, dim3(grid_of(n)), dim3(256), 0, s, phits, prim, n, P);
+    TileArgs T{cam->width, x0, y0, w, p->spp};
+    const SceneDev S = scene_of(bvh);
+    const Rng rng = Rng::make(p->seed, RNG_PATH);
+    for (uint32_t b = 0, qi = 0; b <= p->max_depth; b++, qi ^= 1) {
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
+        hipLaunchKernelGGL(k_pt_shade, dim3(grid_of(n)), dim3(256), 0, s, P, qi, T, S, rng, p->max_depth, p->rr_start,
+                           p->rr_prob);
+        if ((st = launch_trace(bvh, (rb_ray*)P.ra, n, RB_ANY, nullptr, (rb_hit*)P.ha, nullptr, s, dev().cfg,
+                               P.cnt + 2)) != RB_OK)
+            return st;
+        if ((st = launch_trace(bvh, (rb_ray*)P.rbc, 2 * n, RB_CLOSEST, nullptr, (rb_hit*)P.hbc, nullptr, s, dev().cfg,
+                               P.cnt + 3)) != RB_OK)
+            return st;
+        hipLaunchKernelGGL(k_pt_connect, dim3(grid_of(n)), dim3(256), 0, s, P, qi, (float)bvh->n_lights);
+        if (p->ray_counter)
+            hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt, p->ray_counter,
+                               b == 0 ? (unsigned long long)n : 0ull);
+    }
+    hipLaunchKernelGGL(k_film, dim3(grid_of(w * h)), dim3(256), 0, s, P.L, w * h, p->spp, out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RB_OK : hip_fail(e, "PT launch");
+}
+
+}  // namespace rb
+
+using namespace rb;
+
+extern "C" {
+
+void rb_render_params_default(rb_render_params* p) {
+    if (!p) return;
+    p->spp = 1;
+    p->seed = 0;
+    p->ao_samples = 8;
+    p->ao_radius = 1.0f;
+    p->max_depth = 8;
+    p->rr_start = 3;
+    p->rr_prob = 0.5f;
+    p->light_paths = 1u << 20;
+    p->lvc_capacity = 1u << 20;
+    p->connections = 5;
+    p->light_max_depth = 6;
+    p->ray_counter = nullptr;
+}
+
+rb_status rb_render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                       const rb_render_params* p, float* out, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    return render_ao(bvh, cam, x0, y0, w, h, p, out, stream ? (hipStream_t)stream : dev().stream);
+}
+
+rb_status rb_render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                       const rb_render_params* p, float* out, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    return render_pt(bvh, cam, x0, y0, w, h, p, out, stream ? (hipStream_t)stream : dev().stream);
+}
+
+}  // extern "C"

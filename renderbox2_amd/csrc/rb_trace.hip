@@ -31,6 +31,7 @@ struct TraceArgs {
     uint32_t* __restrict__ counts;
     uint32_t* counter;
     uint32_t n;
+    const uint32_t* n_dev;  // when set, the ray count is read from device memory
     uint32_t chunk;
     uint32_t threshold;
 };
@@ -60,6 +61,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     } while (0)
 
     lstack[tid] = kSentinel;
+    const uint32_t nrays = a.n_dev ? __builtin_nontemporal_load(a.n_dev) : a.n;
     uint32_t pool_next = 0, pool_end = 0;
     int nodeAddr = kSentinel, leafAddr = 0, sp = 0, hitIndex = -1;
     uint32_t rayidx = 0;
@@ -89,7 +91,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
                 pool_end = base + a.chunk;
             }
             if (term) {
-                if (mine >= a.n) break;
+                if (mine >= nrays) break;
                 rayidx = mine;
                 const float4 o = a.rays[2 * (size_t)rayidx];
                 const float4 d = a.rays[2 * (size_t)rayidx + 1];
@@ -241,7 +243,8 @@ static TraceKernel pick(uint32_t S, uint32_t B) {
 }
 
 rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
-                       rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg) {
+                       rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg,
+                       const uint32_t* n_dev) {
     if (n == 0) return RB_OK;
     if (!bvh || !rays || !hits) return fail(RB_ERR_INVALID, "NULL bvh, rays or hits");
     if (n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "too many rays in one call (%u)", n);
@@ -260,6 +263,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.counts = counts;
     a.counter = take_counter_slot();
     a.n = n;
+    a.n_dev = n_dev;
     a.chunk = cfg.chunk;
     a.threshold = cfg.fetch_threshold;
     const uint32_t waves = dev().num_cus * cfg.waves_per_cu;

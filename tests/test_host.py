@@ -194,6 +194,20 @@ def test_generated_scene_sizes_are_exact(rbm):
     assert kinds.count(2) == 1 and kinds.count(1) > 200 and kinds.count(0) > 600
 
 
+def test_room_faces_inward_and_lights_face_down(rbm):
+    s = rbm.scenes.pt_room(spheres_hi=0, spheres_lo=1)
+    v, idx, tm, lt = s.geometry()
+    tri = v[idx]
+    n = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    assert len(lt) == 8 and np.allclose(n[lt], [0, -1, 0], atol=1e-6)
+    area = 0.5 * np.linalg.norm(np.cross(tri[lt, 1] - tri[lt, 0], tri[lt, 2] - tri[lt, 0]), axis=1)
+    assert np.allclose(area.reshape(4, 2).sum(1), 0.25, rtol=1e-5)
+    room = np.arange(2048)
+    # inward: the normal points towards the room centre
+    assert np.all((n[room] * (0 - tri[room].mean(1))).sum(1) > 0)
+
+
 def test_generators_are_deterministic(rbm):
     a = rbm.scenes.trace_scaled(20_000, seed=4).geometry()
     b = rbm.scenes.trace_scaled(20_000, seed=4).geometry()

@@ -1,0 +1,166 @@
+"""Closed-form pins of the oracle's AO and path-tracing passes (CPU)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+
+def _quad(x0, x1, z0, z1, y, up=True):
+    v = np.array([[x0, y, z0], [x1, y, z0], [x1, y, z1], [x0, y, z1]], np.float32)
+    # (v1-v0) x (v2-v0) = +y for this winding when up
+    idx = np.array([[0, 2, 1], [0, 3, 2]] if up else [[0, 1, 2], [0, 2, 3]], np.uint32)
+    return v, idx
+
+
+def _merge(*parts):
+    vs, ids, off = [], [], 0
+    for v, i in parts:
+        vs.append(v)
+        ids.append(i + off)
+        off += len(v)
+    return np.concatenate(vs), np.concatenate(ids)
+
+
+def _primary(orc, c2, cam, spp, seed):
+    r2c, c2w = orc.camera_matrices(*cam)
+    w, h = cam[4], cam[5]
+    lib = orc.lib()
+    P = C.c_void_p
+    lib.orc_raygen_camera.argtypes = [P, P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                      C.c_uint64, C.c_int, P]
+    rays = np.empty((w * h * spp, 8), np.float32)
+    lib.orc_raygen_camera(r2c.ctypes.data_as(P), c2w.ctypes.data_as(P), w, 0, 0, w, h, spp, seed, 1,
+                          rays.ctypes.data_as(P))
+    ids, t = orc.trace_compact2(*c2, rays)
+    return rays, ids, t
+
+
+def test_ao_open_plane_is_one(orc):
+    v, idx = _quad(-50, 50, -50, 50, 0.0)
+    b = orc.Bvh(v, idx)
+    cam = ((0.0, 3.0, -3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, 48, 32)
+    img = orc.render_ao(b.compact2(), v, idx, cam, (0, 0, 48, 32), 1, 3, 8, 1.0)
+    vals = np.unique(img)
+    assert set(vals.tolist()) <= {0.0, 1.0}
+    assert (img == 1.0).mean() > 0.5
+
+
+def test_ao_inside_closed_sphere_is_zero(orc, rbm):
+    from renderbox2_amd import _lib as L
+    s = rbm.Scene()
+    m = s.add_diffuse_bsdf((1, 1, 1))
+    s.generate(L.RB_GEN_ICOSPHERES, count=1, subdiv=2, seed=0, r=(0.5, 0.5), c=(0.0, 0.0), material_id=m)
+    v, idx, _, _ = s.geometry()
+    b = orc.Bvh(v, idx)
+    cam = ((0.0, 0.0, 0.0), (0.0, 0.0, 1.0), (0.0, 1.0, 0.0), 90.0, 32, 32)
+    img = orc.render_ao(b.compact2(), v, idx, cam, (0, 0, 32, 32), 2, 5, 8, 1.5)
+    assert np.all(img == 0.0)
+
+
+def test_ao_floor_wall_wedge_closed_form(orc):
+    # floor y = 0 for x < 0, wall x = 0 facing -x; AO radius r.  From a floor
+    # point at distance d from the wall, cosine-weighted rays of length r
+    # reach the wall iff their x component exceeds a = d / r; for a uniform
+    # disk point that has probability (acos(a) - a sqrt(1 - a^2)) / pi.
+    floor = _quad(-50, 0, -50, 50, 0.0)
+    wv = np.array([[0, 0, -50], [0, 0, 50], [0, 50, 50], [0, 50, -50]], np.float32)
+    wall = (wv, np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
+    v, idx = _merge(floor, wall)
+    b = orc.Bvh(v, idx)
+    c2 = b.compact2()
+    r = 1.0
+    cam = ((-0.75, 4.0, 0.0), (-0.75, 0.0, 0.0), (0.0, 0.0, 1.0), 30.0, 64, 64)
+    spp, nao, seed = 4, 16, 9
+    img = orc.render_ao(c2, v, idx, cam, (0, 0, 64, 64), spp, seed, nao, r)
+    rays, ids, t = _primary(orc, c2, cam, spp, seed)
+    hitp = rays[:, :3] + rays[:, 4:7] * t[:, None]
+    on_floor = (ids >= 0) & (ids < 2) & (np.abs(hitp[:, 1]) < 1e-4)
+    a = np.clip(-hitp[:, 0] / r, 0.0, 1.0)
+    expect = 1.0 - (np.arccos(a) - a * np.sqrt(1 - a * a)) / np.pi
+    # pixels whose spp samples all land on the floor
+    pix_floor = on_floor.reshape(-1, spp).all(1)
+    exp_pix = expect.reshape(-1, spp).mean(1)
+    var_pix = (expect * (1 - expect) / nao).reshape(-1, spp).sum(1) / spp ** 2
+    got = img.reshape(-1, 3)[:, 0]
+    sel = pix_floor
+    assert sel.sum() > 1500
+    diff = got[sel] - exp_pix[sel]
+    sigma = np.sqrt(var_pix[sel].sum()) / sel.sum()
+    assert abs(diff.mean()) < 4 * sigma + 1e-4
+    # and by distance bins
+    dsel = -hitp[:, 0].reshape(-1, spp).mean(1)[sel]
+    for lo, hi in [(0, 0.3), (0.3, 0.6), (0.6, 1.2)]:
+        k = (dsel >= lo) & (dsel < hi)
+        if k.sum() > 50:
+            s = np.sqrt(var_pix[sel][k].sum()) / k.sum()
+            assert abs(diff[k].mean()) < 4 * s + 1e-4
+
+
+def _direct_light_closed_form(p, rho, le, lv0, lv1, lv2, lv3, n=64):
+    # (rho / pi) * Le * integral over the light quad of cos_p cos_l / r^2 dA
+    uu = (np.arange(n) + 0.5) / n
+    a = lv1 - lv0
+    c = lv3 - lv0
+    area = np.linalg.norm(np.cross(a, c))
+    ln = np.cross(a, c) / area
+    pts = lv0 + uu[:, None, None] * a + uu[None, :, None] * c
+    d = pts - p
+    r2 = (d * d).sum(-1)
+    dn = d / np.sqrt(r2)[..., None]
+    cos_p = np.clip(dn[..., 1], 0, None)
+    cos_l = np.clip(-(dn * ln).sum(-1), 0, None)
+    return rho / np.pi * le * (cos_p * cos_l / r2).mean() * area
+
+
+def test_pt_direct_light_closed_form(orc):
+    rho, le = 0.5, 10.0
+    floor = _quad(-20, 20, -20, 20, 0.0)
+    L = 0.5
+    lq = np.array([[-L, 2, -L], [L, 2, -L], [L, 2, L], [-L, 2, L]], np.float32)
+    light = (lq, np.array([[0, 1, 2], [0, 2, 3]], np.uint32))  # (v1-v0)x(v2-v0) = -y: faces down
+    v, idx = _merge(floor, light)
+    tri_mat = np.array([0, 0, 1, 1], np.uint32)
+    mats = [(0, (rho, rho, rho)), (2, (le, le, le))]
+    lights = np.array([2, 3], np.uint32)
+    b = orc.Bvh(v, idx)
+    c2 = b.compact2()
+    cam = ((0.0, 1.0, -3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 60.0, 32, 32)
+    spp, seed = 16, 2
+    img = orc.render_pt(c2, v, idx, tri_mat, mats, lights, cam, (0, 0, 32, 32), spp, seed, 0, 3, 0.5)
+    rays, ids, t = _primary(orc, c2, cam, spp, seed)
+    hitp = rays[:, :3] + rays[:, 4:7] * t[:, None]
+    floor_hit = ((ids == 0) | (ids == 1)).reshape(-1, spp).all(1)
+    exp = np.array([_direct_light_closed_form(hp.astype(np.float64), rho, le, *lq.astype(np.float64))
+                    for hp in hitp])
+    exp_pix = exp.reshape(-1, spp).mean(1)
+    got = img.reshape(-1, 3)[:, 0]
+    sel = floor_hit & (exp_pix > 0.05)
+    assert sel.sum() > 300
+    rel = (got[sel] - exp_pix[sel]) / exp_pix[sel]
+    assert abs(rel.mean()) < 0.02
+    assert np.percentile(np.abs(rel), 90) < 0.25
+    assert np.allclose(img[..., 0], img[..., 1]) and np.allclose(img[..., 0], img[..., 2])
+
+
+def test_pt_sees_emitter_and_its_mirror_image(orc):
+    le, refl = 7.0, 0.8
+    mirror = _quad(-20, 20, -20, 20, 0.0)
+    lq = np.array([[-1, 2, -1], [1, 2, -1], [1, 2, 1], [-1, 2, 1]], np.float32)
+    light = (lq, np.array([[0, 1, 2], [0, 2, 3]], np.uint32))
+    v, idx = _merge(mirror, light)
+    tri_mat = np.array([0, 0, 1, 1], np.uint32)
+    mats = [(1, (refl, refl, refl)), (2, (le, le, le))]
+    b = orc.Bvh(v, idx)
+    # camera between the mirror (y = 0) and the 2x2 light (y = 2) looking
+    # down: the light's image sits 3 below the camera, half-angle atan(1/3)
+    # = 18.4 deg, so a 20 deg field of view sees only that image
+    cam = ((0.0, 1.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 1.0), 20.0, 16, 16)
+    img = orc.render_pt(b.compact2(), v, idx, tri_mat, mats, np.array([2, 3], np.uint32), cam, (0, 0, 16, 16), 1,
+                        0, 3, 3, 0.5)
+    # every pixel: mirror at bounce 0 (no direct light: specular), reflected
+    # ray hits the light from below -> refl * Le (through the specular chain)
+    assert np.allclose(img, refl * le, rtol=1e-5)
+    cam2 = ((0.0, 1.0, 0.0), (0.0, 3.0, 0.0), (0.0, 0.0, 1.0), 40.0, 8, 8)
+    img2 = orc.render_pt(b.compact2(), v, idx, tri_mat, mats, np.array([2, 3], np.uint32), cam2, (0, 0, 8, 8), 1,
+                         0, 3, 3, 0.5)
+    assert np.all(img2 == le)  # the light's emitting face seen directly
