@@ -97,6 +97,76 @@ def bench_render(rb, kind, scene, bvh_max_leaf, cam, frames, **params):
             "tris": scene.get_num_triangles(), "bvh_build_s": round(build_s, 1), "mean": round(float(img.mean()), 5)}
 
 
+BATCH_SCENES = [10_000, 50_000, 200_000, 500_000, 1_000_000, 2_000_000, 3_000_000, 4_000_000]
+
+
+def bench_batched(rb, reps=5, seed=0):
+    """Config 4: 8 resident scenes (trace generator, seeds 0-7), 64 closest-hit
+    batches of log-uniform size in [4096, 2 Mi] rays, half primaries / half
+    diffuse rays of scene (batch % 8).  plain = one rb_trace + stream sync per
+    batch; batched = one rb_trace_batch call + one sync."""
+    import numpy as np
+
+    from renderbox2_amd import _lib as L
+    t0 = time.time()
+    stream = rb.Stream()
+    tracers, pools = [], []
+    cam = rb.scenes.trace_camera(1024, 1024)
+    npool = cam.get_total_pixels()
+    for k, T in enumerate(BATCH_SCENES):
+        s = rb.scenes.trace_scaled(T, seed=k)
+        tr = rb.CudaTracer(rb.scenes.build(s), s)
+        prim, diff, h = rb.DeviceBuffer(npool * 32), rb.DeviceBuffer(npool * 32), rb.DeviceBuffer(npool * 8)
+        cam.generate_rays_device(prim, seed=k, stream=stream)
+        tr.trace_device(prim, npool, h, stream=stream)
+        rb.raygen_diffuse_device(tr, prim, h, npool, npool, 100 + k, diff, stream=stream)
+        stream.synchronize()
+        tracers.append(tr)
+        pools.append((prim, diff))
+        log(f"batched: scene {k} ({T} tris) ready, {time.time() - t0:.1f}s")
+    build_s = time.time() - t0
+    rng = np.random.default_rng(seed)
+    sizes = (4096 * 512.0 ** rng.uniform(0, 1, 64)).astype(np.int64) & ~1
+    batches = []
+    for i, sz in enumerate(sizes):
+        k = i % 8
+        half = int(sz) // 2
+        d = rb.DeviceBuffer(int(sz) * 32)
+        o1, o2 = (int(x) for x in rng.integers(0, npool - half + 1, 2))
+        L.check(L.lib().rb_memcpy_dtod(d.ptr, pools[k][0].ptr + o1 * 32, half * 32, stream.handle))
+        L.check(L.lib().rb_memcpy_dtod(d.ptr + half * 32, pools[k][1].ptr + o2 * 32, half * 32, stream.handle))
+        batches.append((tracers[k], d, int(sz), rb.DeviceBuffer(int(sz) * 8), False))
+    stream.synchronize()
+    total = int(sizes.sum())
+
+    def plain():
+        for tr, d, n, h, _ in batches:
+            tr.trace_device(d, n, h, stream=stream)
+            stream.synchronize()
+
+    def batched():
+        rb.trace_batch(batches, stream=stream)
+        stream.synchronize()
+
+    plain()
+    ref = [h.download(np.uint32, 2 * n) for _, _, n, h, _ in batches]
+    batched()
+    match = all(np.array_equal(r, h.download(np.uint32, 2 * n)) for r, (_, _, n, h, _) in zip(ref, batches))
+    tp, tb = [], []
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        plain()
+        tp.append(time.perf_counter() - t1)
+        t1 = time.perf_counter()
+        batched()
+        tb.append(time.perf_counter() - t1)
+    tp, tb = sorted(tp)[reps // 2], sorted(tb)[reps // 2]
+    return {"plain_mrays": round(total / tp / 1e6, 1), "batched_mrays": round(total / tb / 1e6, 1),
+            "speedup": round(tp / tb, 3), "plain_ms": round(tp * 1e3, 2), "batched_ms": round(tb * 1e3, 2),
+            "rays": total, "batches": len(batches), "scene_tris": BATCH_SCENES, "match": bool(match),
+            "setup_s": round(build_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +179,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one copy kernel of this size first (traffic counter calibration)")
     ap.add_argument("--render", type=int, default=1, help="also time the AO and path-traced frames")
+    ap.add_argument("--batched", type=int, default=1, help="also time batched vs plain (8 scenes, 64 batches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -210,8 +281,13 @@ def main():
     if bytes_per_ray is None:
         bytes_per_ray = gpu_bpr
 
-    if args.render:
+    if args.render or args.batched:
         del tracer, d_prim, d_diff, d_mixed, d_hits
+    if args.batched:
+        bt = bench_batched(rb)
+        log("batched vs plain (config 4)", bt)
+        extra["batched"] = bt
+    if args.render:
         ao = bench_render(rb, "ao", rb.scenes.ao_room(seed=0), 4, rb.scenes.ao_camera(512, 512), 10,
                           spp=1, seed=0, ao_samples=8, ao_radius=1.0)
         log("ao frame (config 1: 78,848 tris, 512x512, 1 spp, 8 AO rays)", ao)

@@ -229,6 +229,21 @@ rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t f
                         const uint8_t* mask, rb_hit* hits, const rb_isect_soa* out,
                         void* stream);
 
+/* Batched trace (no reference counterpart: the reference issues one
+ * synchronous launch per call, cudatracer.cu:134-183).  Ray batches of
+ * uneven size against several resident BVHs go through one persistent
+ * launch per 64 batches; each batch keeps its own rays/hits buffers and
+ * closest/any flag (masks are not supported here).  Results equal those of
+ * one rb_trace call per batch. */
+typedef struct rb_batch_desc {
+    rb_bvh_t bvh;
+    const rb_ray* rays;
+    rb_hit* hits;
+    uint32_t n;
+    uint32_t flags;
+} rb_batch_desc;
+rb_status rb_trace_batch(const rb_batch_desc* batches, uint32_t nb, void* stream);
+
 /* Traversal tuning: the knobs the autotuner searches (the reference
  * hard-codes STACK_SIZE 64 / DYNAMIC_FETCH_THRESHOLD 20 / 720 warps of
  * (32,4), kepler_dynamic_fetch.h:62-63 and cudatracer.cu:170-171). */

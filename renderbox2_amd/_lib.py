@@ -120,10 +120,15 @@ _SIGS = {
     "rb_camera_matrices": (C.c_int, [P, P, P]),
     "rb_raygen_camera": (C.c_int, [P, U32, U32, U32, U32, U32, U64, C.c_int, P, P]),
     "rb_raygen_diffuse": (C.c_int, [P, P, P, U32, U32, U64, P, P]),
+    "rb_trace_batch": (C.c_int, [P, U32, P]),
     "rb_render_params_default": (None, [P]),
     "rb_render_ao": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_pt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
 }
+
+
+class rb_batch_desc(C.Structure):
+    _fields_ = [("bvh", P), ("rays", P), ("hits", P), ("n", U32), ("flags", U32)]
 
 
 class rb_render_params(C.Structure):

@@ -413,6 +413,14 @@ class PerspectiveCamera:
         return w * h * spp
 
 
+def trace_batch(batches, stream=None):
+    """One persistent launch over many (tracer, d_rays, n, d_hits, any_hit) batches."""
+    arr = (L.rb_batch_desc * max(len(batches), 1))()
+    for i, (tr, dr, n, dh, anyh) in enumerate(batches):
+        arr[i] = L.rb_batch_desc(tr.handle, _dp(dr), _dp(dh), n, L.RB_ANY if anyh else L.RB_CLOSEST)
+    check(lib().rb_trace_batch(arr, len(batches), _sp(stream)))
+
+
 def render_params(**kw):
     """rb_render_params with the library defaults, overridden by keywords."""
     p = L.rb_render_params()

@@ -14,7 +14,11 @@
 //    keeps the first one found (:264);
 //  * the work counter lives in a per-launch slot that the last block resets
 //    (the reference copies a zero into a global symbol before each launch,
-//    cudatracer.cu:164-165), so concurrent launches and graph replays work.
+//    cudatracer.cu:164-165), so concurrent launches and graph replays work;
+//  * batched mode: one launch serves up to kMaxBatch ray batches against
+//    different resident BVHs; every lane carries the scene pointers and the
+//    any/closest flag of the batch its current ray belongs to, so a wave can
+//    mix rays of several scenes while it refills.
 // Arithmetic (fp32, explicit fmaf, exact division) matches the oracle's
 // restatement bit for bit.
 #include "rb_device.h"
@@ -36,8 +40,26 @@ struct TraceArgs {
     uint32_t threshold;
 };
 
-template <bool ANY, bool COUNT, int S, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+constexpr uint32_t kMaxBatch = 64;
+
+struct BatchDesc {
+    const float4* nodes;
+    const float4* tris;
+    const uint32_t* index;
+    const float4* rays;
+    int2* hits;
+    uint32_t start;  // first global ray index of the batch
+    uint32_t any;    // 1: terminate on the first hit
+};
+
+struct BatchTable {
+    uint32_t nb;
+    uint32_t total;
+    BatchDesc d[kMaxBatch];
+};
+
+template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI>
+__device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays) {
     __shared__ int lstack[S * BLOCK];
     int spill[kMaxStack - S];
     const int tid = threadIdx.x;
@@ -61,13 +83,18 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     } while (0)
 
     lstack[tid] = kSentinel;
-    const uint32_t nrays = a.n_dev ? __builtin_nontemporal_load(a.n_dev) : a.n;
     uint32_t pool_next = 0, pool_end = 0;
     int nodeAddr = kSentinel, leafAddr = 0, sp = 0, hitIndex = -1;
     uint32_t rayidx = 0;
     float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, hitT = 0.f;
     float idx_ = 0.f, idy_ = 0.f, idz_ = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
     uint32_t c_inner = 0, c_tri = 0, c_leaf = 0;
+    // scene of the lane's current ray (uniform in single-batch mode)
+    const float4* nodes = a.nodes;
+    const float4* tris = a.tris;
+    const uint32_t* index = a.index;
+    int2* hits = a.hits;
+    bool any = ANY_T;
 
     while (true) {
         // ---- fetch rays for terminated lanes (wave-uniform control) ----
@@ -92,9 +119,26 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
             }
             if (term) {
                 if (mine >= nrays) break;
+                const float4* rays = a.rays;
                 rayidx = mine;
-                const float4 o = a.rays[2 * (size_t)rayidx];
-                const float4 d = a.rays[2 * (size_t)rayidx + 1];
+                if (MULTI) {
+                    uint32_t lo = 0, hi = nb;
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (tab[mid].start <= mine) lo = mid;
+                        else hi = mid;
+                    }
+                    const BatchDesc& B = tab[lo];
+                    nodes = B.nodes;
+                    tris = B.tris;
+                    index = B.index;
+                    hits = B.hits;
+                    rays = B.rays;
+                    any = B.any != 0;
+                    rayidx = mine - B.start;
+                }
+                const float4 o = rays[2 * (size_t)rayidx];
+                const float4 d = rays[2 * (size_t)rayidx + 1];
                 ox = o.x; oy = o.y; oz = o.z; tmin = o.w;
                 dx = d.x; dy = d.y; dz = d.z; hitT = d.w;
                 const float ooeps = 0x1p-80f;
@@ -107,7 +151,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
                 sp = 0;
                 leafAddr = 0;
                 hitIndex = -1;
-                nodeAddr = (a.mask && a.mask[rayidx] == 0) ? kSentinel : 0;
+                nodeAddr = (!MULTI && a.mask && a.mask[rayidx] == 0) ? kSentinel : 0;
                 if (COUNT) { c_inner = 0; c_tri = 0; c_leaf = 0; }
             }
         }
@@ -115,7 +159,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
         // ---- traversal ----
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
-                const float4* np = a.nodes + 4 * (size_t)nodeAddr;
+                const float4* np = nodes + 4 * (size_t)nodeAddr;
                 const float4 n0 = np[0];
                 const float4 n1 = np[1];
                 const float4 nz = np[2];
@@ -161,7 +205,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
                 const uint32_t cnt = l & 31u;
                 if (COUNT) c_leaf++;
                 for (uint32_t k = 0; k < cnt; k++) {
-                    const float4* w = a.tris + 3 * (size_t)(first + k);
+                    const float4* w = tris + 3 * (size_t)(first + k);
                     const float4 v0 = w[0];
                     const float4 v1 = w[1];
                     const float4 v2 = w[2];
@@ -179,11 +223,11 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
                             const float v = fmaf(t, Dy, Oy);
                             if (v >= 0.0f && u + v <= 1.0f) {
                                 bool take = t < hitT;
-                                if (!take) take = a.index[first + k] < a.index[hitIndex];
+                                if (!take) take = index[first + k] < index[hitIndex];
                                 if (take) {
                                     hitT = t;
                                     hitIndex = (int)(first + k);
-                                    if (ANY) {
+                                    if (any) {
                                         nodeAddr = kSentinel;
                                         break;
                                     }
@@ -202,9 +246,9 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
 
         if (nodeAddr == kSentinel) {
             int2 r;
-            r.x = hitIndex < 0 ? -1 : (int)a.index[hitIndex];
+            r.x = hitIndex < 0 ? -1 : (int)index[hitIndex];
             r.y = __float_as_int(hitT);
-            a.hits[rayidx] = r;
+            hits[rayidx] = r;
             if (COUNT) {
                 a.counts[3 * (size_t)rayidx + 0] = c_inner;
                 a.counts[3 * (size_t)rayidx + 1] = c_tri;
@@ -225,21 +269,53 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     }
 }
 
+template <bool ANY, bool COUNT, int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
+    const uint32_t nrays = a.n_dev ? __builtin_nontemporal_load(a.n_dev) : a.n;
+    trace_body<ANY, COUNT, S, BLOCK, false>(a, nullptr, 0, nrays);
+}
+
+template <int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a, BatchTable t) {
+    __shared__ BatchDesc tab[kMaxBatch];
+    for (uint32_t i = threadIdx.x; i < t.nb; i += BLOCK) tab[i] = t.d[i];
+    __syncthreads();
+    trace_body<false, false, S, BLOCK, true>(a, tab, t.nb, t.total);
+}
+
 // ---------------------------------------------------------------- launch --
 
 typedef void (*TraceKernel)(TraceArgs);
+typedef void (*BatchKernel)(TraceArgs, BatchTable);
+
+#define RB_FOR_CONFIGS(X)                                          \
+    X(8, 64) X(8, 128) X(8, 256) X(12, 64) X(12, 128) X(12, 256)   \
+    X(16, 64) X(16, 128) X(16, 256) X(20, 64) X(20, 128) X(20, 256) \
+    X(24, 64) X(24, 128) X(24, 256)
 
 template <bool ANY, bool COUNT>
 static TraceKernel pick(uint32_t S, uint32_t B) {
 #define RB_K(s, b) \
     if (S == s && B == b) return k_trace<ANY, COUNT, s, b>;
-    RB_K(8, 64) RB_K(8, 128) RB_K(8, 256)
-    RB_K(12, 64) RB_K(12, 128) RB_K(12, 256)
-    RB_K(16, 64) RB_K(16, 128) RB_K(16, 256)
-    RB_K(20, 64) RB_K(20, 128) RB_K(20, 256)
-    RB_K(24, 64) RB_K(24, 128) RB_K(24, 256)
+    RB_FOR_CONFIGS(RB_K)
 #undef RB_K
     return nullptr;
+}
+
+static BatchKernel pick_batch(uint32_t S, uint32_t B) {
+#define RB_K(s, b) \
+    if (S == s && B == b) return k_trace_batch<s, b>;
+    RB_FOR_CONFIGS(RB_K)
+#undef RB_K
+    return nullptr;
+}
+
+static uint32_t persistent_blocks(const rb_trace_config& cfg, uint64_t n) {
+    const uint32_t waves = dev().num_cus * cfg.waves_per_cu;
+    uint32_t blocks = waves * 64 / cfg.block;
+    const uint64_t need = (n + cfg.block - 1) / cfg.block;
+    if (blocks > need) blocks = (uint32_t)need;
+    return blocks ? blocks : 1;
 }
 
 rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
@@ -266,14 +342,46 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.n_dev = n_dev;
     a.chunk = cfg.chunk;
     a.threshold = cfg.fetch_threshold;
-    const uint32_t waves = dev().num_cus * cfg.waves_per_cu;
-    uint32_t blocks = waves * 64 / cfg.block;
-    const uint32_t need = (uint32_t)(((uint64_t)n + cfg.block - 1) / cfg.block);
-    if (blocks > need) blocks = need;
-    if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(k, dim3(blocks), dim3(cfg.block), 0, stream, a);
+    hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), 0, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "trace launch");
+    return RB_OK;
+}
+
+rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg) {
+    BatchKernel k = pick_batch(cfg.lds_stack, cfg.block);
+    if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
+    for (uint32_t i0 = 0; i0 < nb; i0 += kMaxBatch) {
+        BatchTable t;
+        std::memset(&t, 0, sizeof(t));
+        uint64_t total = 0;
+        for (uint32_t i = i0; i < nb && i < i0 + kMaxBatch; i++) {
+            if (b[i].n == 0) continue;
+            if (!b[i].bvh || !b[i].rays || !b[i].hits) return fail(RB_ERR_INVALID, "batch %u: NULL bvh, rays or hits", i);
+            if (b[i].flags & RB_MASKED) return fail(RB_ERR_INVALID, "batch %u: masks are not supported in batches", i);
+            BatchDesc& d = t.d[t.nb++];
+            d.nodes = b[i].bvh->nodes;
+            d.tris = b[i].bvh->tris;
+            d.index = b[i].bvh->index;
+            d.rays = reinterpret_cast<const float4*>(b[i].rays);
+            d.hits = reinterpret_cast<int2*>(b[i].hits);
+            d.start = (uint32_t)total;
+            d.any = (b[i].flags & RB_ANY) ? 1u : 0u;
+            total += b[i].n;
+            if (total > 0x7FFFFFFFull) return fail(RB_ERR_INVALID, "more than 2^31 rays in one batched call");
+        }
+        if (t.nb == 0) continue;
+        t.total = (uint32_t)total;
+        TraceArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.counter = take_counter_slot();
+        a.n = t.total;
+        a.chunk = cfg.chunk;
+        a.threshold = cfg.fetch_threshold;
+        hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, total)), dim3(cfg.block), 0, stream, a, t);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "batched trace launch");
+    }
     return RB_OK;
 }
 
@@ -315,6 +423,13 @@ rb_status rb_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
     if (st != RB_OK) return st;
     return launch_trace(bvh, rays, n, flags, mask, hits, nullptr, stream ? (hipStream_t)stream : dev().stream,
                         dev().cfg);
+}
+
+rb_status rb_trace_batch(const rb_batch_desc* batches, uint32_t nb, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if (nb && !batches) return fail(RB_ERR_INVALID, "NULL batch table");
+    return launch_trace_batch(batches, nb, stream ? (hipStream_t)stream : dev().stream, dev().cfg);
 }
 
 rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits,

@@ -207,3 +207,37 @@ def test_million_triangles_properties(gpu, orc):
     sel = np.random.default_rng(0).choice(len(diff), 65536, replace=False)
     oid, ot = orc.trace_compact2(*c2, diff[sel])
     assert np.array_equal(ids[sel], oid) and np.array_equal(ts[sel].view(np.uint32), ot.view(np.uint32))
+
+
+def test_batched_trace_equals_per_batch_calls(gpu):
+    """Uneven batches against several resident scenes, closest and any hit,
+    more batches than one launch takes (64), empty batches: one batched call
+    gives exactly the per-batch rb_trace results."""
+    rng = np.random.default_rng(3)
+    tracers = []
+    for k, n in enumerate([800, 3000, 20000]):
+        v, idx = random_triangles(n, seed=100 + k, spread=4.0 + k)
+        tracers.append(gpu.CudaTracer(gpu.BVH((v, idx))))
+    batches, expect = [], []
+    for i in range(70):
+        n = 0 if i % 17 == 5 else int(np.exp(rng.uniform(np.log(64), np.log(40000))))
+        tr = tracers[i % 3]
+        anyh = i % 4 == 3
+        rays = random_rays(max(n, 1), seed=1000 + i, box=7.0)[:n]
+        dr = gpu.DeviceBuffer.from_array(rays) if n else gpu.DeviceBuffer(32)
+        dh = gpu.DeviceBuffer(max(n, 1) * 8)
+        batches.append((tr, dr, n, dh, anyh))
+        if n:
+            ids, ts = tr.trace(rays, any_hit=anyh)
+            expect.append((ids, ts))
+        else:
+            expect.append(None)
+    gpu.trace_batch(batches)
+    for (tr, dr, n, dh, anyh), e in zip(batches, expect):
+        if not n:
+            continue
+        h = dh.download(gpu.HIT_DTYPE, n)
+        if anyh:  # any hit: same hit/miss; (id, t) may differ by visit order
+            assert np.array_equal(h["id"] >= 0, e[0] >= 0)
+        else:
+            assert np.array_equal(h["id"], e[0]) and np.array_equal(h["t"].view(np.uint32), e[1].view(np.uint32))
