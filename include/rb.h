@@ -333,6 +333,79 @@ rb_status rb_render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t
  * -> connect -> compaction of live paths; queue sizes stay on the device. */
 rb_status rb_render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                        const rb_render_params* p, float* out_rgb, void* stream);
+/* BDPT with a light-vertex cache (IntegratorBdptLvc, src/integrators/integrator_bdpt_lvc.cpp:62-74,
+ * integrator_bdpt_lvc_sk.cu:47-514): light pass of light_paths sub-paths
+ * (light_paths = 0: lvc_capacity / light_max_depth of them) of up to
+ * light_max_depth vertices cached at fixed slots, then the camera pass with
+ * direct light and `connections` cache connections per camera vertex. */
+rb_status rb_render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                         const rb_render_params* p, float* out_rgb, void* stream);
+
+/* ---- serving ------------------------------------------------------------ */
+
+/* Continuous-batching request server (no reference counterpart: the
+ * reference renders one configured frame per process run,
+ * src/core/application.cpp:225-239).  Scenes stay resident; requests are
+ * admitted while fewer than max_inflight run and their estimated device
+ * memory fits the budget; each running request owns a HIP stream so their
+ * kernels overlap on the GPU; a scheduler thread inside the library
+ * launches, tracks completion and records latency (submit -> result ready). */
+typedef struct rb_server_s* rb_server_t;
+
+typedef struct rb_server_config {
+    uint32_t max_inflight; /* requests on the device at once (1..32) */
+    uint64_t hbm_budget;   /* bytes of request work memory (0: 60% of free HBM) */
+    uint32_t max_queue;    /* waiting requests before submit says RB_ERR_BUSY (0: 65536) */
+} rb_server_config;
+
+enum { RB_REQ_AO = 0, RB_REQ_PT = 1, RB_REQ_BDPT = 2, RB_REQ_TRACE = 3 };
+enum { RB_REQ_QUEUED = 0, RB_REQ_RUNNING = 1, RB_REQ_DONE = 2, RB_REQ_FAILED = 3 };
+
+typedef struct rb_request {
+    uint32_t kind;           /* RB_REQ_* */
+    uint32_t scene;          /* id from rb_server_add_scene */
+    rb_camera camera;        /* render kinds */
+    uint32_t x0, y0, w, h;   /* render kinds: tile */
+    rb_render_params params; /* render kinds (ray_counter is ignored) */
+    float* out_host;         /* render kinds: optional host image w*h*3 (valid once done) */
+    const rb_ray* rays;      /* RB_REQ_TRACE: device rays / hits, n, RB_CLOSEST or RB_ANY */
+    rb_hit* hits;
+    uint32_t n;
+    uint32_t flags;
+} rb_request;
+
+typedef struct rb_request_status {
+    int32_t state;           /* RB_REQ_* */
+    int32_t result;          /* rb_status of a finished request */
+    double queue_ms;         /* submit -> admitted */
+    double latency_ms;       /* submit -> result ready */
+    uint64_t rays;           /* rays traced for this request */
+} rb_request_status;
+
+typedef struct rb_server_stats {
+    uint64_t submitted, completed, failed, rejected, deferrals;
+    double p50_ms, p99_ms, mean_ms, max_ms;
+    uint64_t rays;
+    double first_submit_to_last_done_s;
+    uint64_t peak_budget_bytes;
+} rb_server_stats;
+
+rb_status rb_server_create(const rb_server_config* cfg, rb_server_t* out);
+rb_status rb_server_destroy(rb_server_t s); /* finishes running work, fails queued requests */
+rb_status rb_server_add_scene(rb_server_t s, rb_bvh_t bvh, uint32_t* scene_id);
+rb_status rb_server_remove_scene(rb_server_t s, uint32_t scene_id); /* RB_ERR_BUSY while referenced */
+/* RB_ERR_OOM: the request alone exceeds the budget; RB_ERR_BUSY: queue full. */
+rb_status rb_server_submit(rb_server_t s, const rb_request* req, uint64_t* request_id);
+rb_status rb_server_poll(rb_server_t s, uint64_t request_id, rb_request_status* st); /* RB_ERR_NOT_READY */
+rb_status rb_server_wait(rb_server_t s, uint64_t request_id, rb_request_status* st);
+rb_status rb_server_drain(rb_server_t s);
+rb_status rb_server_get_stats(rb_server_t s, rb_server_stats* out);
+rb_status rb_server_reset_stats(rb_server_t s);
+/* Estimated device bytes the server reserves for a request. */
+rb_status rb_server_request_bytes(const rb_request* req, uint64_t* bytes);
+/* Failure injection for tests: the next k request admissions fail their
+ * device allocation (the request ends RB_ERR_OOM, others are unaffected). */
+rb_status rb_server_inject_alloc_failures(rb_server_t s, uint32_t k);
 
 #ifdef __cplusplus
 }

@@ -217,6 +217,38 @@ def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, 
     return out
 
 
+def render_bdpt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob,
+                light_paths, light_max_depth, connections):
+    """BDPT-LVC image of the tile; returns float32[h, w, 3]."""
+    nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
+    verts = np.ascontiguousarray(verts, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    tri_mat = np.ascontiguousarray(tri_mat, np.uint32)
+    lights = np.ascontiguousarray(lights, np.uint32)
+    m4 = np.zeros((max(len(materials), 1), 4), np.float32)
+    for i, (k, col) in enumerate(materials):
+        m4[i, 0] = np.array([k], np.uint32).view(np.float32)[0]
+        m4[i, 1:] = col
+    r2c, c2w = camera_matrices(*cam)
+    x0, y0, w, h = tile
+    out = np.empty((h, w, 3), np.float32)
+    L = lib()
+    P = C.c_void_p
+    U = C.c_uint32
+    L.orc_render_bdpt.argtypes = [P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, U, U, U, P]
+    L.orc_render_bdpt(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m4), len(materials),
+                      _p(lights), len(lights), _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, max_depth,
+                      rr_start, rr_prob, light_paths, light_max_depth, connections, _p(out))
+    return out
+
+
+def bdpt_techniques(mask, n, max_depth, lvc_depth):
+    L = lib()
+    L.orc_bdpt_techniques.restype = C.c_uint32
+    L.orc_bdpt_techniques.argtypes = [C.c_uint32] * 4
+    return int(L.orc_bdpt_techniques(mask, n, max_depth, lvc_depth))
+
+
 def bytes_per_ray(counts, hit):
     """Algorithmic bytes of one ray on the reference traversal:
     32 (ray) + 8 (id,t out) + 64*inner + 48*tris + 16*leaf terminators + 4*[hit]."""

@@ -1204,12 +1204,124 @@ static f3 lambert_f3(f3 wo, f3 wi, f3 ng, const orc_material* m)
     return mk3(m->color[0] * k, m->color[1] * k, m->color[2] * k);
 }
 
+/* ------------------------------------------------------------------------ */
+/* BDPT with a light-vertex cache: IntegratorBdptLvc,                        */
+/* src/integrators/integrator_bdpt_lvc_sk.cu:47-435, in the form the design  */
+/* fixes: light vertices at fixed slots (path, depth), a connection picks a  */
+/* path uniformly and one of its n_j vertices uniformly and is scaled by     */
+/* n_j / connections, and every path is weighted by 1 / (number of          */
+/* techniques able to build it at the given depth limits).                   */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t paths, depth, nconn;
+    const uint32_t* count;                   /* vertices stored per light path */
+    const float *pos, *ng, *wo, *alpha;      /* 3 floats per slot j * depth + d */
+    const uint32_t *mid, *mask;              /* material, specular bits of y_1..y_(d+1) */
+    orc_rng crng;
+} orc_lvc;
+
+/* Techniques that build the path z_1..z_n (z_0 camera, z_n on the light,
+ * mask bit i-1 = z_i specular) when camera vertices exist for bounces
+ * 0..max_depth and the cache holds light vertices of depth 0..lvc_depth-1:
+ * the direct-light technique (light sample + MIS BSDF sample at a diffuse
+ * z_(n-1) on bounce n-2, or the emitter hit on bounce n-1 after a specular
+ * z_(n-1) or straight from the camera) and each connection edge
+ * (z_i, z_(i+1)) whose ends are both diffuse, camera end on bounce i-1,
+ * light end stored at depth n-i-2. */
+uint32_t orc_bdpt_techniques(uint32_t mask, uint32_t n, uint32_t max_depth, uint32_t lvc_depth)
+{
+    uint32_t c;
+    if (n <= 1) c = 1;
+    else if ((mask >> (n - 2)) & 1u) c = n - 1 <= max_depth ? 1u : 0u;
+    else c = n - 2 <= max_depth ? 1u : 0u;
+    for (uint32_t i = 1; i + 2 <= n; i++) {
+        if ((mask >> (i - 1)) & 1u) continue;   /* camera end z_i specular */
+        if ((mask >> i) & 1u) continue;         /* light end z_(i+1) specular */
+        if (i - 1 > max_depth) continue;        /* camera sub-path too long */
+        if (n - i - 2 >= lvc_depth) continue;   /* light sub-path not cached */
+        c++;
+    }
+    return c;
+}
+
+static float bdpt_weight(uint32_t mask, uint32_t n, uint32_t max_depth, uint32_t lvc_depth)
+{
+    uint32_t c = orc_bdpt_techniques(mask, n, max_depth, lvc_depth);
+    return c ? 1.0f / (float)c : 0.0f;
+}
+
+/* uniform_sample_hemisphere (src/core/montecarlo.cu:116-124) about unit n. */
+static f3 uniform_dir(f3 n, float u1, float u2)
+{
+    float s, c;
+    orc_sincos_2pi(u2, &s, &c);
+    const float z = u1;
+    const float rr = 1.0f - z * z;
+    const float r = sqrtf(rr > 0.0f ? rr : 0.0f);
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    f3 b1 = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f3 b2 = mk3(b, sign + n.y * n.y * a, -n.y);
+    return add3(add3(mul3s(b1, r * c), mul3s(b2, r * s)), mul3s(n, z));
+}
+
+/* Connections of the diffuse camera vertex p (bounce `bounce`, throughput
+ * T) to the cache (integrator_bdpt_lvc_sk.cu:343-386); each unoccluded one
+ * is added to L in connection order. */
+static f3 lvc_connect(const orc_scene* S, const orc_lvc* B, uint32_t gid, uint32_t bounce, uint32_t max_depth, f3 p,
+                      float eps, f3 wo, f3 ng, const orc_material* m, f3 T, uint32_t cm, f3 L)
+{
+    for (uint32_t i = 0; i < B->nconn; i++) {
+        uint32_t rc[4];
+        rng_draw(B->crng, gid, bounce, 2 + i / 2, 0, rc);
+        uint32_t ua = (i & 1) ? rc[2] : rc[0], ub = (i & 1) ? rc[3] : rc[1];
+        uint32_t j = (uint32_t)(u01(ua) * (float)B->paths);
+        if (j > B->paths - 1) j = B->paths - 1;
+        uint32_t nj = B->count[j];
+        if (!nj) continue;
+        uint32_t k = (uint32_t)(u01(ub) * (float)nj);
+        if (k > nj - 1) k = nj - 1;
+        size_t sl = (size_t)j * B->depth + k;
+        f3 xp = mk3(B->pos[3 * sl], B->pos[3 * sl + 1], B->pos[3 * sl + 2]);
+        f3 xn = mk3(B->ng[3 * sl], B->ng[3 * sl + 1], B->ng[3 * sl + 2]);
+        f3 xw = mk3(B->wo[3 * sl], B->wo[3 * sl + 1], B->wo[3 * sl + 2]);
+        f3 xa = mk3(B->alpha[3 * sl], B->alpha[3 * sl + 1], B->alpha[3 * sl + 2]);
+        const orc_material* mx = &S->mats[B->mid[sl]];
+        f3 seg = sub3(xp, p);
+        float d2 = dot3(seg, seg);
+        float dist = sqrtf(d2);
+        f3 dir = normalize3(seg);
+        f3 ndir = mk3(-dir.x, -dir.y, -dir.z);
+        f3 fy = lambert_f3(wo, dir, ng, m);
+        f3 fx = mx->kind == 0u ? lambert_f3(xw, ndir, xn, mx) : mk3(0.0f, 0.0f, 0.0f);
+        if (black3(fy) || black3(fx)) continue;
+        float G = fabsf(dot3(ng, dir)) * fabsf(dot3(xn, ndir)) / d2;
+        /* full path: camera vertices of bounces 0..bounce, then the light
+         * vertices of depths k..0 */
+        uint32_t M = cm;
+        for (uint32_t e = 0; e <= k; e++)
+            if ((B->mask[sl] >> (k - e)) & 1u) M |= 1u << (bounce + 1 + e);
+        float wgt = bdpt_weight(M, bounce + k + 3, max_depth, B->nconn ? B->depth : 0u);
+        float sc = (float)nj / (float)B->nconn;
+        f3 c = mk3(T.x * xa.x * fy.x * G * fx.x * wgt * sc, T.y * xa.y * fy.y * G * fx.y * wgt * sc,
+                   T.z * xa.z * fy.z * G * fx.z * wgt * sc);
+        float sr[8] = {p.x, p.y, p.z, eps, dir.x, dir.y, dir.z, dist * (1.0f - 1.0e-3f)};
+        int32_t sid; float st;
+        trace_ray8(S, sr, 1, &sid, &st);
+        if (sid < 0) L = add3(L, c);
+    }
+    return L;
+}
+
 /* One path of the path tracer, integrator_path.cu:52-254 with the fixes
  * listed in the design notes (roulette divides by the continue probability,
  * a fresh random number per bounce and dimension, true triangle areas). */
 static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng, uint32_t max_depth,
-                  uint32_t rr_start, float rr_prob)
+                  uint32_t rr_start, float rr_prob, const orc_lvc* B)
 {
+    uint32_t cm = 0; /* BDPT: bit b = camera vertex of bounce b is specular */
     f3 L = mk3(0.0f, 0.0f, 0.0f), T = mk3(1.0f, 1.0f, 1.0f);
     float ray[8];
     memcpy(ray, R0, 32);
@@ -1230,6 +1342,7 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
         if (m->kind == 2u) {
             if (bounce == 0 || spec) {
                 f3 le = dot3(ng, wo) > 0.0f ? mk3(m->color[0], m->color[1], m->color[2]) : mk3(0.0f, 0.0f, 0.0f);
+                if (B) le = mul3s(le, bdpt_weight(cm, bounce + 1, max_depth, B->nconn ? B->depth : 0u));
                 L = mk3(L.x + T.x * le.x, L.y + T.y * le.y, L.z + T.z * le.z);
             }
             break;
@@ -1238,6 +1351,8 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
         rng_draw(rng, gid, bounce, 0, 0, r0);
         rng_draw(rng, gid, bounce, 1, 0, r1);
         int lambert = m->kind == 0u;
+        if (B && !lambert) cm |= 1u << bounce;
+        float wdir = B ? bdpt_weight(cm, bounce + 2, max_depth, B->nconn ? B->depth : 0u) : 1.0f;
         f3 Ld = mk3(0.0f, 0.0f, 0.0f);
         if (S->nl > 0) {
             uint32_t lid = (uint32_t)(u01(r0[0]) * (float)S->nl);
@@ -1293,7 +1408,9 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
             }
         }
         Ld = mul3s(Ld, (float)S->nl);
+        if (B) Ld = mul3s(Ld, wdir);
         L = mk3(L.x + T.x * Ld.x, L.y + T.y * Ld.y, L.z + T.z * Ld.z);
+        if (B && lambert && S->nl > 0) L = lvc_connect(S, B, gid, bounce, max_depth, p, eps, wo, ng, m, T, cm, L);
         /* next bounce */
         f3 wi3, f3v;
         float pdf3;
@@ -1333,11 +1450,7 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
     return L;
 }
 
-void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
-                   const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
-                   const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
-                   uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
-                   uint32_t max_depth, uint32_t rr_start, float rr_prob, float* out_rgb)
+static orc_material* mats_from4(const float* mats4, uint32_t nmats)
 {
     orc_material* mats = (orc_material*)malloc(sizeof(orc_material) * (nmats ? nmats : 1));
     for (uint32_t i = 0; i < nmats; i++) {
@@ -1345,6 +1458,16 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
         mats[i].kind = k;
         mats[i].color[0] = mats4[4 * i + 1]; mats[i].color[1] = mats4[4 * i + 2]; mats[i].color[2] = mats4[4 * i + 3];
     }
+    return mats;
+}
+
+void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                   const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
+                   const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
+                   uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
+                   uint32_t max_depth, uint32_t rr_start, float rr_prob, float* out_rgb)
+{
+    orc_material* mats = mats_from4(mats4, nmats);
     orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
@@ -1355,12 +1478,142 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
         float r = 0.0f, g = 0.0f, b = 0.0f;
         for (uint32_t s = 0; s < spp; s++) {
             uint32_t i = (uint32_t)px * spp + s;
-            f3 L = pt_path(&S, rays + 8 * (size_t)i, sample_gid(width, x0, y0, w, spp, i), rng, max_depth, rr_start, rr_prob);
+            f3 L = pt_path(&S, rays + 8 * (size_t)i, sample_gid(width, x0, y0, w, spp, i), rng, max_depth, rr_start, rr_prob, NULL);
             r += L.x; g += L.y; b += L.z;
         }
         float inv = (float)spp;
         out_rgb[3 * px] = r / inv; out_rgb[3 * px + 1] = g / inv; out_rgb[3 * px + 2] = b / inv;
     }
     free(rays);
+    free(mats);
+}
+
+
+/* One light sub-path (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168):
+ * light chosen uniformly, point uniform on it, direction uniform over the
+ * emitting hemisphere (DiffuseAreaLight, src/core/lights.cu:56-79); each
+ * diffuse or mirror hit is stored at slot (j, depth) with the throughput
+ * arriving there, then the path continues by a BSDF sample with roulette
+ * (continue probability min(1, luminance)); emitters end it. */
+static void light_path(const orc_scene* S, orc_rng rng, uint32_t j, uint32_t D, uint32_t* count, float* pos, float* ngo,
+                       float* woo, float* alpha, uint32_t* mido, uint32_t* masko)
+{
+    count[j] = 0;
+    if (S->nl == 0) return;
+    uint32_t r[4], r2[4];
+    rng_draw(rng, j, 0, 0, 0, r);
+    rng_draw(rng, j, 0, 1, 0, r2);
+    uint32_t lid = (uint32_t)(u01(r[0]) * (float)S->nl);
+    if (lid > S->nl - 1) lid = S->nl - 1;
+    uint32_t lt = S->lights[lid];
+    f3 l0, l1, l2;
+    tri_verts(S, lt, &l0, &l1, &l2);
+    const orc_material* lm = &S->mats[S->tri_mat[lt]];
+    f3 ln = tri_norm(l0, l1, l2);
+    f3 cr = cross3(sub3(l1, l0), sub3(l2, l0));
+    float area = 0.5f * sqrtf(dot3(cr, cr));
+    float su1 = sqrtf(u01(r[1]));
+    float uu = 1.0f - su1, vv = u01(r[2]) * su1;
+    f3 pl = add3(add3(mul3s(l0, uu), mul3s(l1, vv)), mul3s(l2, 1.0f - uu - vv));
+    f3 dir = uniform_dir(ln, u01(r[3]), u01(r2[0]));
+    float cth = fabsf(dot3(ln, dir));
+    float pdf = (1.0f / area) * 0.159154943091895336f / cth;
+    float kk = (float)S->nl / pdf;
+    f3 a = mk3(lm->color[0] * kk, lm->color[1] * kk, lm->color[2] * kk);
+    float ray[8] = {pl.x, pl.y, pl.z, 1.0e-4f, dir.x, dir.y, dir.z, FLT_MAX};
+    uint32_t mask = 0;
+    for (uint32_t d = 0; d < D; d++) {
+        int32_t id; float t;
+        trace_ray8(S, ray, 0, &id, &t);
+        if (id < 0) return;
+        f3 dv = mk3(ray[4], ray[5], ray[6]);
+        f3 p = add3(mk3(ray[0], ray[1], ray[2]), mul3s(dv, t));
+        f3 v0, v1, v2;
+        tri_verts(S, (uint32_t)id, &v0, &v1, &v2);
+        uint32_t mid = S->tri_mat[id];
+        const orc_material* m = &S->mats[mid];
+        if (m->kind == 2u) return;
+        f3 ng = tri_norm(v0, v1, v2);
+        f3 wo = mk3(-dv.x, -dv.y, -dv.z);
+        int mirror = m->kind == 1u;
+        if (mirror) mask |= 1u << d;
+        size_t sl = (size_t)j * D + d;
+        pos[3 * sl] = p.x; pos[3 * sl + 1] = p.y; pos[3 * sl + 2] = p.z;
+        ngo[3 * sl] = ng.x; ngo[3 * sl + 1] = ng.y; ngo[3 * sl + 2] = ng.z;
+        woo[3 * sl] = wo.x; woo[3 * sl + 1] = wo.y; woo[3 * sl + 2] = wo.z;
+        alpha[3 * sl] = a.x; alpha[3 * sl + 1] = a.y; alpha[3 * sl + 2] = a.z;
+        mido[sl] = mid;
+        masko[sl] = mask;
+        count[j] = d + 1;
+        if (d + 1 >= D) return;
+        uint32_t rr[4];
+        rng_draw(rng, j, d + 1, 0, 0, rr);
+        f3 nf = dot3(ng, wo) >= 0.0f ? ng : mk3(-ng.x, -ng.y, -ng.z);
+        f3 wi, f;
+        float bp;
+        if (mirror) {
+            wi = sub3(mul3s(nf, 2.0f * dot3(nf, wo)), wo);
+            float ac = fabsf(dot3(wi, nf));
+            f = mk3(m->color[0] / ac, m->color[1] / ac, m->color[2] / ac);
+            bp = 1.0f;
+        } else {
+            wi = cosine_dir(nf, u01(rr[0]), u01(rr[1]));
+            f = lambert_f3(wo, wi, ng, m);
+            bp = fabsf(dot3(wi, nf)) * 0.318309886183790672f;
+        }
+        if (black3(f) || !(bp > 0.0f)) return;
+        float s = fabsf(dot3(ng, wi));
+        f3 tmp = mk3(f.x * s / bp, f.y * s / bp, f.z * s / bp);
+        float lum = 0.212671f * tmp.x + 0.715160f * tmp.y + 0.072169f * tmp.z;
+        float cp = lum < 1.0f ? lum : 1.0f;
+        if (u01(rr[2]) > cp) return;
+        f3 na = mk3(a.x * (tmp.x / cp), a.y * (tmp.y / cp), a.z * (tmp.z / cp));
+        if (black3(na)) return;
+        a = na;
+        float nr[8] = {p.x, p.y, p.z, 1.0e-3f * t, wi.x, wi.y, wi.z, FLT_MAX};
+        memcpy(ray, nr, 32);
+    }
+}
+
+void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                     const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
+                     const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
+                     uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
+                     uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t light_paths,
+                     uint32_t light_max_depth, uint32_t connections, float* out_rgb)
+{
+    orc_material* mats = mats_from4(mats4, nmats);
+    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
+    const uint32_t NL = light_paths, D = light_max_depth;
+    size_t slots = (size_t)NL * D;
+    uint32_t* count = (uint32_t*)calloc(NL ? NL : 1, 4);
+    float* pos = (float*)calloc(slots * 3 + 1, 4);
+    float* ngs = (float*)calloc(slots * 3 + 1, 4);
+    float* wos = (float*)calloc(slots * 3 + 1, 4);
+    float* alpha = (float*)calloc(slots * 3 + 1, 4);
+    uint32_t* mid = (uint32_t*)calloc(slots + 1, 4);
+    uint32_t* mask = (uint32_t*)calloc(slots + 1, 4);
+    orc_rng lrng = rng_make(seed, 6u);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long j = 0; j < (long)NL; j++)
+        light_path(&S, lrng, (uint32_t)j, D, count, pos, ngs, wos, alpha, mid, mask);
+    orc_lvc B = {NL, D, nl ? connections : 0u, count, pos, ngs, wos, alpha, mid, mask, rng_make(seed, 8u)};
+    uint32_t n = w * h * spp;
+    float* rays = (float*)malloc((size_t)n * 32);
+    orc_raygen_camera(r2c, c2w, width, x0, y0, w, h, spp, seed, 1, rays);
+    orc_rng rng = rng_make(seed, 5u);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (long px = 0; px < (long)(w * h); px++) {
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        for (uint32_t s = 0; s < spp; s++) {
+            uint32_t i = (uint32_t)px * spp + s;
+            f3 L = pt_path(&S, rays + 8 * (size_t)i, sample_gid(width, x0, y0, w, spp, i), rng, max_depth, rr_start,
+                           rr_prob, &B);
+            r += L.x; g += L.y; b += L.z;
+        }
+        float inv = (float)spp;
+        out_rgb[3 * px] = r / inv; out_rgb[3 * px + 1] = g / inv; out_rgb[3 * px + 2] = b / inv;
+    }
+    free(rays); free(count); free(pos); free(ngs); free(wos); free(alpha); free(mid); free(mask);
     free(mats);
 }

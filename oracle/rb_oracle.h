@@ -12,7 +12,8 @@
  * multiply and add rounds on its own unless an fmaf() is written out.
  *
  * The pins of each function (closed forms, FP64 brute force, invariants,
- * known-answer vectors) live in tests/test_oracle_*.py.
+ * known-answer vectors) live in tests/test_oracle.py and
+ * tests/test_oracle_render.py.
  */
 #ifndef RB_ORACLE_H
 #define RB_ORACLE_H
@@ -136,6 +137,20 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                    const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                    uint32_t max_depth, uint32_t rr_start, float rr_prob, float* out_rgb);
+/* Number of BDPT techniques able to build a path z_1..z_n (mask bit i-1 =
+ * z_i specular) when camera vertices exist for bounces 0..max_depth and the
+ * light-vertex cache holds depths 0..lvc_depth-1; a path's weight is 1/that. */
+uint32_t orc_bdpt_techniques(uint32_t mask, uint32_t n, uint32_t max_depth, uint32_t lvc_depth);
+/* BDPT with a light-vertex cache (src/integrators/integrator_bdpt_lvc_sk.cu:47-435):
+ * light_paths light sub-paths of up to light_max_depth cached vertices, then
+ * the camera pass of the path tracer plus `connections` cache connections
+ * per diffuse camera vertex. */
+void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                     const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
+                     const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
+                     uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
+                     uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t light_paths,
+                     uint32_t light_max_depth, uint32_t connections, float* out_rgb);
 
 #ifdef __cplusplus
 }

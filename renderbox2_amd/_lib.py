@@ -124,7 +124,39 @@ _SIGS = {
     "rb_render_params_default": (None, [P]),
     "rb_render_ao": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_pt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
+    "rb_render_bdpt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
+    "rb_server_create": (C.c_int, [P, P]),
+    "rb_server_destroy": (C.c_int, [P]),
+    "rb_server_add_scene": (C.c_int, [P, P, P]),
+    "rb_server_remove_scene": (C.c_int, [P, U32]),
+    "rb_server_submit": (C.c_int, [P, P, P]),
+    "rb_server_poll": (C.c_int, [P, U64, P]),
+    "rb_server_wait": (C.c_int, [P, U64, P]),
+    "rb_server_drain": (C.c_int, [P]),
+    "rb_server_get_stats": (C.c_int, [P, P]),
+    "rb_server_reset_stats": (C.c_int, [P]),
+    "rb_server_request_bytes": (C.c_int, [P, P]),
+    "rb_server_inject_alloc_failures": (C.c_int, [P, U32]),
 }
+
+
+RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT, RB_REQ_TRACE = 0, 1, 2, 3
+RB_REQ_QUEUED, RB_REQ_RUNNING, RB_REQ_DONE, RB_REQ_FAILED = 0, 1, 2, 3
+
+
+class rb_server_config(C.Structure):
+    _fields_ = [("max_inflight", U32), ("hbm_budget", U64), ("max_queue", U32)]
+
+
+class rb_request_status(C.Structure):
+    _fields_ = [("state", C.c_int32), ("result", C.c_int32), ("queue_ms", C.c_double), ("latency_ms", C.c_double),
+                ("rays", U64)]
+
+
+class rb_server_stats(C.Structure):
+    _fields_ = [("submitted", U64), ("completed", U64), ("failed", U64), ("rejected", U64), ("deferrals", U64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("rays", U64), ("first_submit_to_last_done_s", C.c_double), ("peak_budget_bytes", U64)]
 
 
 class rb_batch_desc(C.Structure):
@@ -135,6 +167,12 @@ class rb_render_params(C.Structure):
     _fields_ = [("spp", U32), ("seed", U64), ("ao_samples", U32), ("ao_radius", F32), ("max_depth", U32),
                 ("rr_start", U32), ("rr_prob", F32), ("light_paths", U32), ("lvc_capacity", U32),
                 ("connections", U32), ("light_max_depth", U32), ("ray_counter", P)]
+
+
+class rb_request(C.Structure):
+    _fields_ = [("kind", U32), ("scene", U32), ("camera", rb_camera), ("x0", U32), ("y0", U32), ("w", U32),
+                ("h", U32), ("params", rb_render_params), ("out_host", P), ("rays", P), ("hits", P), ("n", U32),
+                ("flags", U32)]
 
 _L = None
 

@@ -434,7 +434,7 @@ class SamplerRenderer:
     """Camera rays + integrator + film for one tile (SamplerRenderer::render,
     src/renderers/samplerrenderer.cpp:58-107; Integrator::render, src/core/integrator.h:59)."""
 
-    INTEGRATORS = {"ao": "rb_render_ao", "pt": "rb_render_pt"}
+    INTEGRATORS = {"ao": "rb_render_ao", "pt": "rb_render_pt", "bdpt": "rb_render_bdpt"}
 
     def __init__(self, tracer, camera, integrator="ao", **params):
         if integrator not in self.INTEGRATORS:
@@ -466,3 +466,111 @@ def _sync(stream):
 def raygen_diffuse_device(tracer, d_prim, d_hits, n_primary, n_out, seed, d_out, stream=None):
     check(lib().rb_raygen_diffuse(tracer.handle, _dp(d_prim), _dp(d_hits), n_primary, n_out, seed, _dp(d_out),
                                   _sp(stream)))
+
+
+class RenderServer:
+    """Continuous-batching request server over resident scenes (rb_server_*).
+
+    The reference renders one configured frame per process run
+    (src/core/application.cpp:225-239); this keeps scenes resident and admits
+    AO / PT / BDPT tiles and raw trace batches as they arrive, each on its own
+    HIP stream, within a device-memory budget.  Results are independent of
+    what else runs: every random number is keyed by (request seed, pixel,
+    sample, bounce)."""
+
+    KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT}
+
+    def __init__(self, max_inflight=8, hbm_budget=0, max_queue=0):
+        cfg = L.rb_server_config(max_inflight, hbm_budget, max_queue)
+        h = C.c_void_p()
+        check(lib().rb_server_create(C.byref(cfg), C.byref(h)))
+        self.handle = h.value
+        self._tracers = {}
+        self._live = {}  # request id -> (rb_request, host image or None)
+
+    def add_scene(self, tracer):
+        sid = C.c_uint32()
+        check(lib().rb_server_add_scene(self.handle, tracer.handle, C.byref(sid)))
+        self._tracers[sid.value] = tracer
+        return sid.value
+
+    def remove_scene(self, sid):
+        check(lib().rb_server_remove_scene(self.handle, sid))
+        self._tracers.pop(sid, None)
+
+    @staticmethod
+    def render_request(kind, scene, camera, tile=None, **params):
+        x0, y0, w, h = tile if tile else (0, 0, camera.width, camera.height)
+        r = L.rb_request()
+        r.kind = RenderServer.KINDS[kind]
+        r.scene = scene
+        r.camera = camera.c
+        r.x0, r.y0, r.w, r.h = x0, y0, w, h
+        r.params = render_params(**params)
+        return r
+
+    @staticmethod
+    def request_bytes(req):
+        b = C.c_uint64()
+        check(lib().rb_server_request_bytes(C.byref(req), C.byref(b)))
+        return b.value
+
+    def submit(self, req, out=None):
+        if out is not None:
+            req.out_host = out.ctypes.data_as(C.c_void_p)
+        rid = C.c_uint64()
+        check(lib().rb_server_submit(self.handle, C.byref(req), C.byref(rid)))
+        self._live[rid.value] = (req, out)
+        return rid.value
+
+    def submit_render(self, kind, scene, camera, tile=None, **params):
+        """Queues a tile render; wait() returns its float32[h, w, 3] image."""
+        req = self.render_request(kind, scene, camera, tile, **params)
+        out = np.zeros((req.h, req.w, 3), np.float32)
+        return self.submit(req, out)
+
+    def submit_trace(self, scene, d_rays, d_hits, n, flags=L.RB_CLOSEST):
+        req = L.rb_request()
+        req.kind = L.RB_REQ_TRACE
+        req.scene = scene
+        req.rays, req.hits, req.n, req.flags = _dp(d_rays), _dp(d_hits), n, flags
+        return self.submit(req)
+
+    def poll(self, rid):
+        st = L.rb_request_status()
+        s = lib().rb_server_poll(self.handle, rid, C.byref(st))
+        if s not in (L.RB_OK, L.RB_ERR_NOT_READY):
+            check(s)
+        return st
+
+    def wait(self, rid):
+        """Blocks until the request ends; returns (rb_request_status, image or None)."""
+        st = L.rb_request_status()
+        check(lib().rb_server_wait(self.handle, rid, C.byref(st)))
+        _, out = self._live.pop(rid, (None, None))
+        return st, out
+
+    def drain(self):
+        check(lib().rb_server_drain(self.handle))
+
+    def stats(self):
+        st = L.rb_server_stats()
+        check(lib().rb_server_get_stats(self.handle, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in L.rb_server_stats._fields_}
+
+    def reset_stats(self):
+        check(lib().rb_server_reset_stats(self.handle))
+
+    def inject_alloc_failures(self, k):
+        check(lib().rb_server_inject_alloc_failures(self.handle, k))
+
+    def close(self):
+        if self.handle:
+            lib().rb_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

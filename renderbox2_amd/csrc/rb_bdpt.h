@@ -1,0 +1,215 @@
+// rb_bdpt.hip -- bidirectional path tracing with a light-vertex cache (LVC).
+//
+// Restates the reference's BDPT-LVC (src/integrators/integrator_bdpt_lvc_sk.cu:47-435,
+// wavefront form integrator_bdpt_lvc_mk.cu) as wavefront passes:
+//  light pass  : light sub-paths from uniformly chosen area lights (point
+//                uniform on the triangle, direction uniform over the
+//                emitting hemisphere); every non-emitter hit is stored in the
+//                cache at slot (path, depth) -- fixed slots, so the cache is
+//                identical run to run (the reference appends with atomics);
+//  camera pass : per camera vertex, direct light (NEE + BSDF sample with
+//                MIS, as the path tracer), `connections` shadow-ray
+//                connections to cache vertices, then the next bounce; any-hit
+//                and closest-hit rays of a bounce go through one persistent
+//                launch each, live paths are compacted.
+// Weights: every path is weighted by 1 / (number of techniques that can
+// build it): the direct-light pair counts as one technique and each
+// connection whose two end vertices are non-specular, and whose camera and
+// light sub-paths fit max_depth and the cache depth, as one more; with no
+// connections the weights are all 1 and the camera pass is the path tracer.
+// The reference weights a connection by 1 / ((bounce + 1) + lvc_depth)
+// whatever the path (integrator_bdpt_lvc_sk.cu:379-384) and adds the direct
+// term at full weight, which counts paths more than once.
+// Connection estimator: pick a light path j uniformly, then one of its n_j
+// stored vertices uniformly, and scale by n_j / connections (unbiased).
+#pragma once
+#include "rb_shade.h"
+
+namespace rb {
+
+struct LvcBufs {
+    float4* pos;    // p.xyz | (unused)
+    float4* ng;     // geometric normal | material id bits
+    float4* wo;     // direction towards the previous light vertex | specular mask bits
+    float4* alpha;  // light sub-path throughput arriving at the vertex
+    uint32_t* count;  // vertices stored per light path
+    uint32_t depth;   // slots per light path
+    uint32_t paths;
+};
+
+struct LightPath {
+    float4* ray;    // 2 per path
+    float4* alpha;
+    uint32_t* mask;
+    uint32_t* q[2];
+    uint32_t* cnt;  // [0],[1] queue lengths
+    int2* hit;
+};
+
+__device__ __forceinline__ uint32_t wave_append2(uint32_t* counter, bool want) {
+    const uint64_t m = __ballot(want);
+    uint32_t base = 0;
+    if (m) {
+        const int leader = __builtin_ctzll(m);
+        if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+        base = __shfl(base, leader);
+    }
+    const uint32_t lane = __lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    return want ? base + (uint32_t)__popcll(m & lt) : 0xFFFFFFFFu;
+}
+
+// uniform_sample_hemisphere (src/core/montecarlo.cu:116-124) about unit n.
+__device__ __forceinline__ V3 uniform_hemisphere(V3 n, float u1, float u2) {
+    float s, c;
+    sincos_2pi(u2, &s, &c);
+    const float z = u1;
+    const float rr = 1.0f - z * z;
+    const float r = sqrtf(rr > 0.0f ? rr : 0.0f);
+    V3 b1, b2;
+    onb(n, &b1, &b2);
+    return vadd(vadd(vmul(b1, r * c), vmul(b2, r * s)), vmul(n, z));
+}
+
+// 1 / (number of techniques that can build a path z_1 .. z_n; z_0 is the
+// camera, z_n the point on the light, mask bit i-1 = z_i specular, z_n
+// never).  Techniques, each counted only when its sub-paths fit the depth
+// limits (camera vertices exist for bounces 0..max_depth, the cache holds
+// light vertices at depths 0..lvc_depth-1):
+//  * direct light: light sample + MIS BSDF sample at a diffuse z_(n-1)
+//    (bounce n-2), or the emitter hit at bounce n-1 when z_(n-1) is
+//    specular or n = 1;
+//  * connection edge (z_i, z_(i+1)), 1 <= i <= n-2, both ends diffuse,
+//    camera end at bounce i-1 <= max_depth, light end at depth n-i-2 <
+//    lvc_depth.
+// n <= 32 (the renderer bounds max_depth + lvc_depth).
+RB_HD float technique_weight(uint32_t mask, uint32_t n, uint32_t max_depth, uint32_t lvc_depth) {
+    uint32_t c = 1;
+    if (n >= 2) c = ((mask >> (n - 2)) & 1u) ? (n - 1 <= max_depth) : (n - 2 <= max_depth);
+    if (n >= 3) {
+        uint32_t ok = ~mask & ~(mask >> 1) & ((1u << (n - 2)) - 1u);  // bit i-1: edge i
+        const uint32_t lo = n - 2 > lvc_depth ? n - 2 - lvc_depth : 0u;
+        ok &= ~((1u << lo) - 1u);
+        if (max_depth < 31) ok &= (2u << max_depth) - 1u;
+        c += (uint32_t)__builtin_popcount(ok);
+    }
+    return c ? 1.0f / (float)c : 0.0f;
+}
+
+__global__ void k_light_emit(uint32_t npaths, SceneDev S, Rng rng, LightPath P, LvcBufs C) {
+    const uint32_t end = (npaths + 63u) & ~63u;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
+        const bool valid = j < npaths;
+        if (valid) {
+            C.count[j] = 0;
+            const U4 r = rng.draw(j, 0, 0, 0);
+            const U4 r2 = rng.draw(j, 0, 1, 0);
+            uint32_t lid = (uint32_t)(u01(r.x) * (float)S.n_lights);
+            if (lid > S.n_lights - 1) lid = S.n_lights - 1;
+            const uint32_t lt = S.light_tris[lid];
+            V3 l0, l1, l2;
+            uint32_t lmid;
+            tri_fetch(S, lt, &l0, &l1, &l2, &lmid);
+            const rb_material lm = S.mats[lmid];
+            const V3 ln = tri_normal(l0, l1, l2);
+            const float area = tri_area(l0, l1, l2);
+            const V3 pl = sample_triangle(l0, l1, l2, u01(r.y), u01(r.z));
+            const V3 dir = uniform_hemisphere(ln, u01(r.w), u01(r2.x));
+            const float cth = fabsf(vdot(ln, dir));
+            // DiffuseAreaLight::Le (src/core/lights.cu:68-79): pdf over area x
+            // projected solid angle; times the light count (uniform choice)
+            const float pdf = (1.0f / area) * 0.159154943091895336f / cth;
+            const float k = (float)S.n_lights / pdf;
+            P.alpha[j] = make_float4(lm.color[0] * k, lm.color[1] * k, lm.color[2] * k, 0.f);
+            P.ray[2 * (size_t)j] = make_float4(pl.x, pl.y, pl.z, 1.0e-4f);
+            P.ray[2 * (size_t)j + 1] = make_float4(dir.x, dir.y, dir.z, 3.402823466e+38f);
+            P.mask[j] = 0;
+        }
+        const uint32_t q = wave_append2(&P.cnt[0], valid && S.n_lights > 0);
+        if (valid && S.n_lights > 0) P.q[0][q] = j;
+    }
+}
+
+__global__ void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, Rng rng, LightPath P, LvcBufs C) {
+    const uint32_t count = P.cnt[qi];
+    const uint32_t end = (count + 63u) & ~63u;
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += gridDim.x * blockDim.x) {
+        const bool valid = q < count;
+        bool next = false;
+        uint32_t j = 0;
+        if (valid) {
+            j = P.q[qi][q];
+            const int2 h = P.hit[q];
+            if (h.x >= 0) {
+                const float4 o = P.ray[2 * (size_t)j], dd = P.ray[2 * (size_t)j + 1];
+                const float t = __int_as_float(h.y);
+                const V3 dv = v3(dd.x, dd.y, dd.z);
+                const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
+                V3 v0, v1, v2;
+                uint32_t mid;
+                tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid);
+                const rb_material m = S.mats[mid];
+                if (m.kind != RB_MAT_EMITTER) {
+                    const V3 ng = tri_normal(v0, v1, v2);
+                    const V3 wo = vneg(dv);
+                    const float4 a = P.alpha[j];
+                    uint32_t mask = P.mask[j];
+                    const bool mirror = m.kind == RB_MAT_MIRROR;
+                    if (mirror) mask |= 1u << d;
+                    const size_t slot = (size_t)j * C.depth + d;
+                    C.pos[slot] = make_float4(p.x, p.y, p.z, 0.f);
+                    C.ng[slot] = make_float4(ng.x, ng.y, ng.z, __uint_as_float(mid));
+                    C.wo[slot] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mask));
+                    C.alpha[slot] = a;
+                    C.count[j] = d + 1;
+                    if (d + 1 < C.depth) {
+                        const U4 r = rng.draw(j, d + 1, 0, 0);
+                        const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
+                        V3 wi, f;
+                        float pdf;
+                        if (mirror) {
+                            wi = vsub(vmul(nf, 2.0f * vdot(nf, wo)), wo);
+                            const float ac = fabsf(vdot(wi, nf));
+                            f = v3(m.color[0] / ac, m.color[1] / ac, m.color[2] / ac);
+                            pdf = 1.0f;
+                        } else {
+                            wi = cosine_hemisphere(nf, u01(r.x), u01(r.y));
+                            f = lambert_f(wo, wi, ng, m);
+                            pdf = fabsf(vdot(wi, nf)) * 0.318309886183790672f;
+                        }
+                        if (!is_black(f) && pdf > 0.0f) {
+                            const float s = fabsf(vdot(ng, wi));
+                            const V3 tmp = v3(f.x * s / pdf, f.y * s / pdf, f.z * s / pdf);
+                            const float lum = luminance(tmp);
+                            const float cp = lum < 1.0f ? lum : 1.0f;
+                            if (!(u01(r.z) > cp)) {
+                                const float4 na = make_float4(a.x * (tmp.x / cp), a.y * (tmp.y / cp), a.z * (tmp.z / cp), 0.f);
+                                if (!(na.x == 0.f && na.y == 0.f && na.z == 0.f)) {
+                                    P.alpha[j] = na;
+                                    P.mask[j] = mask;
+                                    P.ray[2 * (size_t)j] = make_float4(p.x, p.y, p.z, 1.0e-3f * t);
+                                    P.ray[2 * (size_t)j + 1] = make_float4(wi.x, wi.y, wi.z, 3.402823466e+38f);
+                                    next = true;
+                                }
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        const uint32_t o = wave_append2(&P.cnt[qi ^ 1], next);
+        if (next) P.q[qi ^ 1][o] = j;
+    }
+}
+
+// Gather the rays of the queued light paths into a dense buffer for tracing.
+__global__ void k_light_gather(uint32_t qi, LightPath P, float4* rays) {
+    const uint32_t count = P.cnt[qi];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        const uint32_t j = P.q[qi][q];
+        rays[2 * (size_t)q] = P.ray[2 * (size_t)j];
+        rays[2 * (size_t)q + 1] = P.ray[2 * (size_t)j + 1];
+    }
+}
+
+}  // namespace rb

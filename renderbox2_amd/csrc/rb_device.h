@@ -37,6 +37,16 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
                        rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg,
                        const uint32_t* n_dev = nullptr);
 
+// Render passes (rb_render.hip), asynchronous on stream s.
+rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    const rb_render_params* p, float* out, hipStream_t s);
+rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    const rb_render_params* p, float* out, hipStream_t s);
+rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                      const rb_render_params* p, float* out, hipStream_t s);
+// Device scratch bytes one render call of this kind and size allocates.
+uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_render_params* p);
+
 // Device allocation tracked against the library's HBM accounting
 // (MemoryAllocator::allocate, src/memory/memoryallocator.cpp:54-67).
 rb_status dalloc(void** p, size_t bytes);

@@ -91,10 +91,13 @@ struct PathBufs {
     float4* L;       // radiance
     float4* ray;     // 2 per path: current ray
     int2* hit;       // current hit
-    float4* ca;      // light-sample contribution if unoccluded; w = n/a
+    float4* ca;      // light-sample contribution if unoccluded; w = direct-light technique weight
     float4* cb;      // MIS contribution if the MIS ray hits light `lprim`
     int4* slot;      // x: shadow slot, y: MIS slot, z: bounce slot, w: light triangle
     uint32_t* info;  // bounce | specular << 8 | next specular << 9 | emitter-done << 10
+    uint32_t* cmask; // BDPT: specular camera vertices (bit b = vertex of bounce b)
+    float4* cc;      // BDPT: connection contributions, kMaxConn per path
+    int* cslot;      // BDPT: connection shadow-ray slots, kMaxConn per path
     uint32_t* q[2];  // path queues
     uint32_t* cnt;   // [0],[1] queue lengths, [2] shadow rays, [3] closest rays
     float4* ra;      // shadow rays
@@ -103,19 +106,22 @@ struct PathBufs {
     int2* hbc;
 };
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) {
-    // one atomic per wave: lanes that want a slot get consecutive ones
-    const uint64_t m = __ballot(want);
-    uint32_t base = 0;
-    if (m) {
-        const int leader = __builtin_ctzll(m);
-        if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-    }
-    const uint32_t lane = __lane_id();
-    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    return want ? base + (uint32_t)__popcll(m & lt) : 0xFFFFFFFFu;
-}
+constexpr uint32_t kMaxConn = 8;
+
+}  // namespace rb
+
+#include "rb_bdpt.h"
+
+namespace rb {
+
+struct BdArgs {
+    LvcBufs C;
+    uint32_t nconn;
+    uint32_t tdepth;  // cache depth the technique count may use: 0 when no connections are made
+    Rng crng;
+};
+
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) { return wave_append2(counter, want); }
 
 __global__ void k_pt_init(const int2* __restrict__ hits, const float4* __restrict__ prim, uint32_t n, PathBufs P) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -126,6 +132,7 @@ __global__ void k_pt_init(const int2* __restrict__ hits, const float4* __restric
         P.ray[2 * (size_t)i + 1] = prim[2 * (size_t)i + 1];
         P.hit[i] = h;
         P.info[i] = 0;
+        if (P.cmask) P.cmask[i] = 0;
         const uint32_t q = wave_append(&P.cnt[0], h.x >= 0);
         if (h.x >= 0) P.q[0][q] = i;
     }
@@ -143,14 +150,21 @@ __global__ void k_add64(unsigned long long* total, unsigned long long v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(total, v);
 }
 
+__global__ void k_add_count(const uint32_t* c, unsigned long long* total) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicAdd(total, (unsigned long long)*c);
+}
+
 // Adds the rays of this bounce (shadow + closest queues) to the caller's counter.
 __global__ void k_count_rays(const uint32_t* c, unsigned long long* total, unsigned long long extra) {
     if (threadIdx.x == 0 && blockIdx.x == 0 && total)
         atomicAdd(total, (unsigned long long)c[2] + (unsigned long long)c[3] + extra);
 }
 
+// One bounce of shading for the queued camera paths.  BD: also the
+// light-vertex-cache connections and the technique weights.
+template <bool BD>
 __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng rng, uint32_t max_depth,
-                           uint32_t rr_start, float rr_prob) {
+                           uint32_t rr_start, float rr_prob, BdArgs B) {
     const uint32_t count = P.cnt[qi];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t end = (count + 63u) & ~63u;  // whole waves iterate together (wave_append)
@@ -161,6 +175,9 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
         float4 ra0, ra1, rb1, rc1;
         V3 p = v3(0.f, 0.f, 0.f);
         float eps = 0.0f;
+        uint32_t conn_want = 0;  // BD: bit i = connection i emits a shadow ray
+        V3 cdir[kMaxConn];
+        float cmax[kMaxConn];
         if (valid) {
             const float4 o = P.ray[2 * (size_t)pid], d = P.ray[2 * (size_t)pid + 1];
             const int2 h = P.hit[pid];
@@ -181,11 +198,13 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
             const float4 Tp = P.T[pid];
             const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
             int4 slot = make_int4(-1, -1, -1, -1);
+            uint32_t cm = BD ? P.cmask[pid] : 0u;
             if (m.kind == RB_MAT_EMITTER) {
                 // emission seen directly or through mirrors (integrator_path.cu:94-109), then stop
                 uint32_t ninfo = info | (1u << 10);
                 if (bounce == 0 || spec) {
-                    const V3 le = vdot(ng, wo) > 0.0f ? v3(m.color[0], m.color[1], m.color[2]) : v3(0.f, 0.f, 0.f);
+                    V3 le = vdot(ng, wo) > 0.0f ? v3(m.color[0], m.color[1], m.color[2]) : v3(0.f, 0.f, 0.f);
+                    if (BD) le = vmul(le, technique_weight(cm, bounce + 1, max_depth, B.tdepth));
                     const float4 L = P.L[pid];
                     P.L[pid] = make_float4(L.x + Tv.x * le.x, L.y + Tv.y * le.y, L.z + Tv.z * le.z, 0.f);
                 }
@@ -197,6 +216,8 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
                 const U4 r1 = rng.draw(gid, bounce, 1, 0);
                 V3 ca = v3(0.f, 0.f, 0.f), cb = v3(0.f, 0.f, 0.f);
                 const bool lambert = m.kind == RB_MAT_LAMBERT;
+                if (BD && !lambert) cm |= 1u << bounce;
+                const float wdir = BD ? technique_weight(cm, bounce + 2, max_depth, B.tdepth) : 1.0f;
                 if (S.n_lights > 0) {
                     uint32_t lid = (uint32_t)(u01(r0.x) * (float)S.n_lights);
                     if (lid > S.n_lights - 1) lid = S.n_lights - 1;
@@ -247,6 +268,49 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
                         }
                     }
                 }
+                if (BD && lambert) {
+                    // connections to the light-vertex cache (integrator_bdpt_lvc_sk.cu:343-386)
+                    for (uint32_t i = 0; i < B.nconn; i++) {
+                        const U4 rc = B.crng.draw(gid, bounce, 2 + i / 2, 0);
+                        const uint32_t ua = (i & 1) ? rc.z : rc.x, ub = (i & 1) ? rc.w : rc.y;
+                        float4 contrib = make_float4(0.f, 0.f, 0.f, 0.f);
+                        uint32_t j = (uint32_t)(u01(ua) * (float)B.C.paths);
+                        if (j > B.C.paths - 1) j = B.C.paths - 1;
+                        const uint32_t nj = B.C.count[j];
+                        if (nj) {
+                            uint32_t k = (uint32_t)(u01(ub) * (float)nj);
+                            if (k > nj - 1) k = nj - 1;
+                            const size_t sl = (size_t)j * B.C.depth + k;
+                            const float4 xp = B.C.pos[sl], xn = B.C.ng[sl], xw = B.C.wo[sl], xa = B.C.alpha[sl];
+                            const rb_material mx = S.mats[__float_as_uint(xn.w)];
+                            const V3 seg = vsub(v3(xp.x, xp.y, xp.z), p);
+                            const float d2 = vdot(seg, seg);
+                            const float dist = sqrtf(d2);
+                            const V3 dir = vnorm(seg);
+                            const V3 ngx = v3(xn.x, xn.y, xn.z);
+                            const V3 fy = lambert_f(wo, dir, ng, m);
+                            const V3 fx = mx.kind == RB_MAT_LAMBERT ? lambert_f(v3(xw.x, xw.y, xw.z), vneg(dir), ngx, mx)
+                                                                    : v3(0.f, 0.f, 0.f);
+                            if (!is_black(fy) && !is_black(fx)) {
+                                const float G = fabsf(vdot(ng, dir)) * fabsf(vdot(ngx, vneg(dir))) / d2;
+                                // full path: camera vertices 0..bounce, then light vertices k+1..1
+                                const uint32_t lmask = __float_as_uint(xw.w);
+                                uint32_t M = cm;
+                                for (uint32_t e = 0; e <= k; e++)
+                                    if ((lmask >> (k - e)) & 1u) M |= 1u << (bounce + 1 + e);
+                                const float w = technique_weight(M, bounce + k + 3, max_depth, B.tdepth);
+                                const float sc = (float)nj / (float)B.nconn;
+                                contrib = make_float4(Tv.x * xa.x * fy.x * G * fx.x * w * sc,
+                                                      Tv.y * xa.y * fy.y * G * fx.y * w * sc,
+                                                      Tv.z * xa.z * fy.z * G * fx.z * w * sc, 0.f);
+                                cdir[i] = dir;
+                                cmax[i] = dist * (1.0f - 1.0e-3f);
+                                conn_want |= 1u << i;
+                            }
+                        }
+                        P.cc[(size_t)pid * kMaxConn + i] = contrib;
+                    }
+                }
                 // next bounce (material_sample_f with every component)
                 V3 wi3, f3;
                 float pdf3;
@@ -271,9 +335,9 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
                     bool alive = true;
                     if (bounce > rr_start) {
                         const float lum = luminance(Tn);
-                        const float q = rr_prob < lum ? rr_prob : lum;
-                        if (u01(r1.w) > q) alive = false;
-                        else Tn = v3(Tn.x / q, Tn.y / q, Tn.z / q);
+                        const float qq = rr_prob < lum ? rr_prob : lum;
+                        if (u01(r1.w) > qq) alive = false;
+                        else Tn = v3(Tn.x / qq, Tn.y / qq, Tn.z / qq);
                     }
                     P.Tn[pid] = make_float4(Tn.x, Tn.y, Tn.z, 0.f);
                     if (alive && bounce != max_depth) {
@@ -282,7 +346,8 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
                         if (spec3) ninfo |= 1u << 9;
                     }
                 }
-                P.ca[pid] = make_float4(ca.x, ca.y, ca.z, 0.f);
+                if (BD) P.cmask[pid] = cm;
+                P.ca[pid] = make_float4(ca.x, ca.y, ca.z, wdir);
                 P.cb[pid] = make_float4(cb.x, cb.y, cb.z, 0.f);
                 P.info[pid] = ninfo;
                 P.slot[pid] = slot;
@@ -311,10 +376,22 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
             s.z = wantC ? (int)sc : -1;
             P.slot[pid] = s;
         }
+        if (BD) {
+            for (uint32_t i = 0; i < B.nconn; i++) {
+                const bool want = (conn_want >> i) & 1u;
+                const uint32_t si = wave_append(&P.cnt[2], want);
+                if (valid) P.cslot[(size_t)pid * kMaxConn + i] = want ? (int)si : -1;
+                if (want) {
+                    P.ra[2 * (size_t)si] = make_float4(p.x, p.y, p.z, eps);
+                    P.ra[2 * (size_t)si + 1] = make_float4(cdir[i].x, cdir[i].y, cdir[i].z, cmax[i]);
+                }
+            }
+        }
     }
 }
 
-__global__ void k_pt_connect(PathBufs P, uint32_t qi, float nlights) {
+template <bool BD>
+__global__ void k_pt_connect(PathBufs P, uint32_t qi, float nlights, uint32_t nconn) {
     const uint32_t count = P.cnt[qi];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t end = (count + 63u) & ~63u;
@@ -328,18 +405,27 @@ __global__ void k_pt_connect(PathBufs P, uint32_t qi, float nlights) {
             if (!(info & (1u << 10))) {
                 const int4 s = P.slot[pid];
                 V3 Ld = v3(0.f, 0.f, 0.f);
-                if (s.x >= 0 && P.ha[s.x].x < 0) {
-                    const float4 c = P.ca[pid];
-                    Ld = vadd(Ld, v3(c.x, c.y, c.z));
-                }
+                const float4 ca = P.ca[pid];
+                if (s.x >= 0 && P.ha[s.x].x < 0) Ld = vadd(Ld, v3(ca.x, ca.y, ca.z));
                 if (s.y >= 0 && P.hbc[s.y].x == s.w) {
                     const float4 c = P.cb[pid];
                     Ld = vadd(Ld, v3(c.x, c.y, c.z));
                 }
                 Ld = vmul(Ld, nlights);
+                if (BD) Ld = vmul(Ld, ca.w);
                 const float4 Tp = P.T[pid];
-                const float4 L = P.L[pid];
-                P.L[pid] = make_float4(L.x + Tp.x * Ld.x, L.y + Tp.y * Ld.y, L.z + Tp.z * Ld.z, 0.f);
+                float4 L = P.L[pid];
+                L = make_float4(L.x + Tp.x * Ld.x, L.y + Tp.y * Ld.y, L.z + Tp.z * Ld.z, 0.f);
+                if (BD) {
+                    for (uint32_t i = 0; i < nconn; i++) {
+                        const int cs = P.cslot[(size_t)pid * kMaxConn + i];
+                        if (cs >= 0 && P.ha[cs].x < 0) {
+                            const float4 c = P.cc[(size_t)pid * kMaxConn + i];
+                            L = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.f);
+                        }
+                    }
+                }
+                P.L[pid] = L;
                 if (s.z >= 0) {
                     const int2 h = P.hbc[s.z];
                     if (h.x >= 0) {
@@ -457,16 +543,17 @@ rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
     return e == hipSuccess ? RB_OK : hip_fail(e, "AO launch");
 }
 
-rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                    const rb_render_params* p, float* out, hipStream_t s) {
-    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
-    if (st != RB_OK) return st;
-    if (p->max_depth > 250) return fail(RB_ERR_INVALID, "max_depth must be <= 250");
+// Camera pass shared by the path tracer (BD = false) and BDPT (BD = true).
+static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                             const rb_render_params* p, float* out, hipStream_t s, Scratch& M, bool bd,
+                             const BdArgs& B) {
+    rb_status st;
     const uint32_t n = w * h * p->spp;
-    Scratch M(s);
+    const uint32_t na = bd ? n * (1 + B.nconn) : n;
     float4* prim = M.get<float4>(2 * (size_t)n);
     int2* phits = M.get<int2>(n);
     PathBufs P;
+    std::memset(&P, 0, sizeof(P));
     P.T = M.get<float4>(n);
     P.Tn = M.get<float4>(n);
     P.L = M.get<float4>(n);
@@ -479,10 +566,15 @@ rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
     P.q[0] = M.get<uint32_t>(n);
     P.q[1] = M.get<uint32_t>(n);
     P.cnt = M.get<uint32_t>(4);
-    P.ra = M.get<float4>(2 * (size_t)n);
-    P.ha = M.get<int2>(n);
+    P.ra = M.get<float4>(2 * (size_t)na);
+    P.ha = M.get<int2>(na);
     P.rbc = M.get<float4>(4 * (size_t)n);
     P.hbc = M.get<int2>(2 * (size_t)n);
+    if (bd) {
+        P.cmask = M.get<uint32_t>(n);
+        P.cc = M.get<float4>((size_t)n * kMaxConn);
+        P.cslot = M.get<int>((size_t)n * kMaxConn);
+    }
     if (M.st != RB_OK) return M.st;
     if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)prim, s)) != RB_OK) return st;
     if ((st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg)) != RB_OK)
@@ -494,22 +586,115 @@ rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
     const Rng rng = Rng::make(p->seed, RNG_PATH);
     for (uint32_t b = 0, qi = 0; b <= p->max_depth; b++, qi ^= 1) {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
-        hipLaunchKernelGGL(k_pt_shade, dim3(grid_of(n)), dim3(256), 0, s, P, qi, T, S, rng, p->max_depth, p->rr_start,
-                           p->rr_prob);
-        if ((st = launch_trace(bvh, (rb_ray*)P.ra, n, RB_ANY, nullptr, (rb_hit*)P.ha, nullptr, s, dev().cfg,
+        if (bd)
+            hipLaunchKernelGGL(k_pt_shade<true>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, T, S, rng, p->max_depth,
+                               p->rr_start, p->rr_prob, B);
+        else
+            hipLaunchKernelGGL(k_pt_shade<false>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, T, S, rng, p->max_depth,
+                               p->rr_start, p->rr_prob, B);
+        if ((st = launch_trace(bvh, (rb_ray*)P.ra, na, RB_ANY, nullptr, (rb_hit*)P.ha, nullptr, s, dev().cfg,
                                P.cnt + 2)) != RB_OK)
             return st;
         if ((st = launch_trace(bvh, (rb_ray*)P.rbc, 2 * n, RB_CLOSEST, nullptr, (rb_hit*)P.hbc, nullptr, s, dev().cfg,
                                P.cnt + 3)) != RB_OK)
             return st;
-        hipLaunchKernelGGL(k_pt_connect, dim3(grid_of(n)), dim3(256), 0, s, P, qi, (float)bvh->n_lights);
+        if (bd)
+            hipLaunchKernelGGL(k_pt_connect<true>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, (float)bvh->n_lights,
+                               B.nconn);
+        else
+            hipLaunchKernelGGL(k_pt_connect<false>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, (float)bvh->n_lights,
+                               0u);
         if (p->ray_counter)
             hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt, p->ray_counter,
                                b == 0 ? (unsigned long long)n : 0ull);
     }
     hipLaunchKernelGGL(k_film, dim3(grid_of(w * h)), dim3(256), 0, s, P.L, w * h, p->spp, out);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? RB_OK : hip_fail(e, "PT launch");
+    return e == hipSuccess ? RB_OK : hip_fail(e, "camera pass launch");
+}
+
+rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    const rb_render_params* p, float* out, hipStream_t s) {
+    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
+    if (st != RB_OK) return st;
+    if (p->max_depth > 250) return fail(RB_ERR_INVALID, "max_depth must be <= 250");
+    Scratch M(s);
+    BdArgs B;
+    std::memset(&B, 0, sizeof(B));
+    return camera_pass(bvh, cam, x0, y0, w, h, p, out, s, M, false, B);
+}
+
+uint64_t bdpt_scratch_bytes(uint32_t w, uint32_t h, const rb_render_params* p) {
+    const uint64_t n = (uint64_t)w * h * (p->spp ? p->spp : 1);
+    const uint64_t D = p->light_max_depth ? p->light_max_depth : 1;
+    const uint64_t nl = p->light_paths ? p->light_paths : p->lvc_capacity / D;
+    const uint64_t nc = p->connections;
+    return n * 308 + n * (4 + kMaxConn * 20) + n * nc * 40 + nl * D * 64 + nl * (32 + 16 + 4 + 8 + 4 + 8 + 32);
+}
+
+rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                      const rb_render_params* p, float* out, hipStream_t s) {
+    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
+    if (st != RB_OK) return st;
+    if (p->max_depth > 24) return fail(RB_ERR_INVALID, "BDPT max_depth must be <= 24");
+    if (p->light_max_depth == 0 || p->light_max_depth > 8) return fail(RB_ERR_INVALID, "light_max_depth must be 1..8");
+    // full-path specular masks are 32-bit: path length max_depth + light_max_depth + 2 <= 32
+    if (p->max_depth + p->light_max_depth > 30) return fail(RB_ERR_INVALID, "max_depth + light_max_depth must be <= 30");
+    if (p->connections > kMaxConn) return fail(RB_ERR_INVALID, "connections must be <= %u", kMaxConn);
+    if (p->light_paths > (1u << 24) || (p->light_paths == 0 && p->lvc_capacity == 0))
+        return fail(RB_ERR_INVALID, "light_paths must be 1..2^24 (or 0 with an lvc_capacity)");
+    Scratch M(s);
+    const uint32_t D = p->light_max_depth;
+    // light sub-paths: given, or as many as fill a cache of lvc_capacity vertices
+    const uint32_t NL = p->light_paths ? p->light_paths : (p->lvc_capacity / D ? p->lvc_capacity / D : 1);
+    BdArgs B;
+    B.nconn = bvh->n_lights ? p->connections : 0;
+    B.tdepth = B.nconn ? D : 0;
+    B.crng = Rng::make(p->seed, RNG_CONNECT);
+    B.C.depth = D;
+    B.C.paths = NL;
+    B.C.pos = M.get<float4>((size_t)NL * D);
+    B.C.ng = M.get<float4>((size_t)NL * D);
+    B.C.wo = M.get<float4>((size_t)NL * D);
+    B.C.alpha = M.get<float4>((size_t)NL * D);
+    B.C.count = M.get<uint32_t>(NL);
+    LightPath LP;
+    LP.ray = M.get<float4>(2 * (size_t)NL);
+    LP.alpha = M.get<float4>(NL);
+    LP.mask = M.get<uint32_t>(NL);
+    LP.q[0] = M.get<uint32_t>(NL);
+    LP.q[1] = M.get<uint32_t>(NL);
+    LP.cnt = M.get<uint32_t>(4);
+    LP.hit = M.get<int2>(NL);
+    float4* lrays = M.get<float4>(2 * (size_t)NL);
+    if (M.st != RB_OK) return M.st;
+    const SceneDev S = scene_of(bvh);
+    const Rng lrng = Rng::make(p->seed, RNG_LIGHT);
+    // light pass (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168)
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
+    hipLaunchKernelGGL(k_light_emit, dim3(grid_of(NL)), dim3(256), 0, s, NL, S, lrng, LP, B.C);
+    for (uint32_t d = 0, qi = 0; d < D; d++, qi ^= 1) {
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, qi ^ 1, 2u, 3u);
+        hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, qi, LP, lrays);
+        if ((st = launch_trace(bvh, (rb_ray*)lrays, NL, RB_CLOSEST, nullptr, (rb_hit*)LP.hit, nullptr, s, dev().cfg,
+                               LP.cnt + qi)) != RB_OK)
+            return st;
+        if (p->ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->ray_counter);
+        hipLaunchKernelGGL(k_light_hit, dim3(grid_of(NL)), dim3(256), 0, s, qi, d, S, lrng, LP, B.C);
+    }
+    // camera pass (kernel_sk_camera_trace, integrator_bdpt_lvc_sk.cu:170-435)
+    return camera_pass(bvh, cam, x0, y0, w, h, p, out, s, M, true, B);
+}
+
+uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_render_params* p) {
+    const uint64_t n = (uint64_t)w * h * (p->spp ? p->spp : 1);
+    const uint64_t slack = 64 * 40;  // per-allocation padding and alignment
+    switch (kind) {
+        case RB_REQ_AO: return n * 40 + n * (p->ao_samples ? p->ao_samples : 1) * 41 + slack;
+        case RB_REQ_PT: return n * 308 + slack;
+        case RB_REQ_BDPT: return bdpt_scratch_bytes(w, h, p) + slack;
+        default: return slack;
+    }
 }
 
 }  // namespace rb
@@ -527,7 +712,7 @@ void rb_render_params_default(rb_render_params* p) {
     p->max_depth = 8;
     p->rr_start = 3;
     p->rr_prob = 0.5f;
-    p->light_paths = 1u << 20;
+    p->light_paths = 1u << 18;
     p->lvc_capacity = 1u << 20;
     p->connections = 5;
     p->light_max_depth = 6;
@@ -546,6 +731,13 @@ rb_status rb_render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t
     rb_status st = ensure_device();
     if (st != RB_OK) return st;
     return render_pt(bvh, cam, x0, y0, w, h, p, out, stream ? (hipStream_t)stream : dev().stream);
+}
+
+rb_status rb_render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                         const rb_render_params* p, float* out, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    return render_bdpt(bvh, cam, x0, y0, w, h, p, out, stream ? (hipStream_t)stream : dev().stream);
 }
 
 }  // extern "C"

@@ -1,0 +1,463 @@
+// rb_server.hip -- continuous-batching request server (host code).
+//
+// One scheduler thread per server owns every HIP call of the serving path:
+// it admits queued requests while a slot (HIP stream + event + pinned
+// staging) is free and the request's estimated device memory fits the
+// budget, launches the request's passes on the slot's stream, polls the
+// slot events for completion, copies results out, and records latency
+// (submit -> result ready).  Requests that do not fit wait without blocking
+// smaller ones behind them (bounded bypass); a request larger than the whole
+// budget is rejected at submit.  Results do not depend on what else runs:
+// every random number is keyed by (request seed, pixel, sample, bounce).
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "rb_device.h"
+
+namespace rb {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Req {
+    uint64_t id = 0;
+    rb_request r{};
+    rb_bvh_t bvh = nullptr;
+    uint64_t bytes = 0;
+    int state = RB_REQ_QUEUED;
+    rb_status result = RB_OK;
+    bool deferred = false;
+    Clock::time_point t_submit, t_admit, t_done;
+    float* d_out = nullptr;
+    size_t out_bytes = 0;
+    uint64_t rays = 0;
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    float* pinned = nullptr;
+    size_t pinned_cap = 0;
+    unsigned long long* d_counter = nullptr;
+    unsigned long long* h_counter = nullptr;  // pinned
+    Req* req = nullptr;
+};
+
+struct SceneEntry {
+    rb_bvh_t bvh;
+    uint32_t refs;
+};
+
+}  // namespace
+
+uint64_t request_bytes(const rb_request& r) {
+    if (r.kind == RB_REQ_TRACE) return 64;
+    const uint64_t out = (uint64_t)r.w * r.h * 12;
+    return out + render_scratch_bytes(r.kind, r.w, r.h, &r.params) + 4096;
+}
+
+}  // namespace rb
+
+struct rb_server_s {
+    rb_server_config cfg{};
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<rb::Req*> queue;
+    std::unordered_map<uint64_t, std::unique_ptr<rb::Req>> reqs;
+    std::map<uint32_t, rb::SceneEntry> scenes;
+    uint32_t next_scene = 0;
+    uint64_t next_id = 1;
+    std::vector<rb::Slot> slots;
+    uint64_t used = 0, peak = 0;
+    uint32_t inject_failures = 0;
+    bool stop = false;
+    std::thread th;
+    // stats
+    uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0, deferrals = 0, rays = 0;
+    std::vector<double> lat_ms;
+    bool have_first = false;
+    rb::Clock::time_point first_submit, last_done;
+
+    void loop();
+    void launch(rb::Slot& sl, rb::Req* q);
+    void finish(rb::Slot& sl, rb_status result);
+};
+
+using namespace rb;
+
+void rb_server_s::finish(Slot& sl, rb_status result) {
+    Req* q = sl.req;
+    if (result == RB_OK) {
+        if (q->r.out_host && q->d_out) std::memcpy(q->r.out_host, sl.pinned, q->out_bytes);
+        q->rays = *sl.h_counter;
+    }
+    if (q->d_out) (void)hipFreeAsync(q->d_out, sl.stream);
+    q->d_out = nullptr;
+    std::lock_guard<std::mutex> g(mu);
+    q->t_done = Clock::now();
+    q->result = result;
+    q->state = result == RB_OK ? RB_REQ_DONE : RB_REQ_FAILED;
+    used -= q->bytes;
+    auto it = scenes.find(q->r.scene);
+    if (it != scenes.end()) it->second.refs--;
+    if (result == RB_OK) {
+        completed++;
+        rays += q->rays;
+        lat_ms.push_back(std::chrono::duration<double, std::milli>(q->t_done - q->t_submit).count());
+        last_done = q->t_done;
+    } else {
+        failed++;
+    }
+    sl.req = nullptr;
+    cv_done.notify_all();
+}
+
+void rb_server_s::launch(Slot& sl, Req* q) {
+    sl.req = q;
+    rb_status st = RB_OK;
+    hipError_t e = hipMemsetAsync(sl.d_counter, 0, sizeof(unsigned long long), sl.stream);
+    if (e != hipSuccess) st = hip_fail(e, "server counter reset");
+    if (st == RB_OK && q->r.kind != RB_REQ_TRACE) {
+        q->out_bytes = (size_t)q->r.w * q->r.h * 12;
+        bool inject = false;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            if (inject_failures) {
+                inject_failures--;
+                inject = true;
+            }
+        }
+        if (inject) {
+            st = fail(RB_ERR_OOM, "request %llu: device allocation failed (injected)", (unsigned long long)q->id);
+        } else if ((e = hipMallocAsync((void**)&q->d_out, q->out_bytes, sl.stream)) != hipSuccess) {
+            (void)hipGetLastError();
+            q->d_out = nullptr;
+            st = fail(RB_ERR_OOM, "request %llu: output allocation failed", (unsigned long long)q->id);
+        }
+        if (st == RB_OK && q->r.out_host && sl.pinned_cap < q->out_bytes) {
+            if (sl.pinned) (void)hipHostFree(sl.pinned);
+            sl.pinned = nullptr;
+            sl.pinned_cap = 0;
+            if (hipHostMalloc((void**)&sl.pinned, q->out_bytes, hipHostMallocDefault) != hipSuccess)
+                st = fail(RB_ERR_OOM, "pinned staging allocation failed");
+            else
+                sl.pinned_cap = q->out_bytes;
+        }
+    }
+    if (st == RB_OK) {
+        rb_render_params p = q->r.params;
+        p.ray_counter = sl.d_counter;
+        switch (q->r.kind) {
+            case RB_REQ_AO: st = render_ao(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;
+            case RB_REQ_PT: st = render_pt(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;
+            case RB_REQ_BDPT: st = render_bdpt(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;
+            case RB_REQ_TRACE:
+                st = launch_trace(q->bvh, q->r.rays, q->r.n, q->r.flags & RB_ANY, nullptr, q->r.hits, nullptr, sl.stream,
+                                  dev().cfg);
+                if (st == RB_OK && (e = hipMemcpyAsync(sl.d_counter, &q->r.n, 4, hipMemcpyHostToDevice, sl.stream)) != hipSuccess)
+                    st = hip_fail(e, "server trace count");
+                break;
+            default: st = fail(RB_ERR_INVALID, "unknown request kind %u", q->r.kind);
+        }
+    }
+    if (st == RB_OK && q->r.out_host && q->d_out) {
+        if ((e = hipMemcpyAsync(sl.pinned, q->d_out, q->out_bytes, hipMemcpyDeviceToHost, sl.stream)) != hipSuccess)
+            st = hip_fail(e, "server result copy");
+    }
+    if (st == RB_OK &&
+        (e = hipMemcpyAsync(sl.h_counter, sl.d_counter, sizeof(unsigned long long), hipMemcpyDeviceToHost, sl.stream)) !=
+            hipSuccess)
+        st = hip_fail(e, "server ray count copy");
+    if (st == RB_OK && (e = hipEventRecord(sl.ev, sl.stream)) != hipSuccess) st = hip_fail(e, "server event");
+    if (st != RB_OK) {
+        (void)hipStreamSynchronize(sl.stream);
+        finish(sl, st);
+    }
+}
+
+void rb_server_s::loop() {
+    hipSetDevice(dev().device);
+    std::vector<std::pair<Slot*, Req*>> to_launch;
+    while (true) {
+        // completions
+        bool busy = false;
+        for (Slot& sl : slots) {
+            if (!sl.req) continue;
+            hipError_t e = hipEventQuery(sl.ev);
+            if (e == hipSuccess) finish(sl, RB_OK);
+            else if (e != hipErrorNotReady) {
+                (void)hipGetLastError();
+                finish(sl, hip_fail(e, "request execution"));
+            } else busy = true;
+        }
+        // admission
+        to_launch.clear();
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            if (stop && queue.empty() && !busy) break;
+            size_t looked = 0;
+            for (auto it = queue.begin(); it != queue.end() && looked < 16;) {
+                Slot* free_slot = nullptr;
+                for (Slot& sl : slots)
+                    if (!sl.req) {
+                        bool taken = false;
+                        for (auto& p : to_launch) taken |= p.first == &sl;
+                        if (!taken) {
+                            free_slot = &sl;
+                            break;
+                        }
+                    }
+                if (!free_slot) break;
+                Req* q = *it;
+                if (used + q->bytes <= cfg.hbm_budget) {
+                    used += q->bytes;
+                    peak = std::max(peak, used);
+                    q->state = RB_REQ_RUNNING;
+                    q->t_admit = Clock::now();
+                    to_launch.emplace_back(free_slot, q);
+                    it = queue.erase(it);
+                } else {
+                    if (!q->deferred) {
+                        q->deferred = true;
+                        deferrals++;
+                    }
+                    ++it;
+                    ++looked;
+                }
+            }
+            if (to_launch.empty() && !busy) {
+                if (stop && queue.empty()) break;
+                cv_work.wait_for(lk, std::chrono::milliseconds(queue.empty() ? 50 : 1));
+                continue;
+            }
+        }
+        for (auto& p : to_launch) launch(*p.first, p.second);
+        if (to_launch.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+extern "C" {
+
+rb_status rb_server_request_bytes(const rb_request* req, uint64_t* bytes) {
+    if (!req || !bytes) return fail(RB_ERR_INVALID, "NULL argument");
+    *bytes = request_bytes(*req);
+    return RB_OK;
+}
+
+rb_status rb_server_create(const rb_server_config* cfg, rb_server_t* out) {
+    if (!out) return fail(RB_ERR_INVALID, "NULL out");
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    std::unique_ptr<rb_server_s> S(new rb_server_s());
+    if (cfg) S->cfg = *cfg;
+    if (S->cfg.max_inflight == 0) S->cfg.max_inflight = 8;
+    if (S->cfg.max_inflight > 32) return fail(RB_ERR_INVALID, "max_inflight must be <= 32");
+    if (S->cfg.max_queue == 0) S->cfg.max_queue = 65536;
+    if (S->cfg.hbm_budget == 0) {
+        size_t f = 0, t = 0;
+        if (hipMemGetInfo(&f, &t) != hipSuccess) return fail(RB_ERR_DEVICE, "hipMemGetInfo failed");
+        S->cfg.hbm_budget = (uint64_t)(f * 0.6);
+    }
+    S->slots.resize(S->cfg.max_inflight);
+    for (Slot& sl : S->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
+            hipMalloc((void**)&sl.d_counter, sizeof(unsigned long long)) != hipSuccess ||
+            hipHostMalloc((void**)&sl.h_counter, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
+            return fail(RB_ERR_DEVICE, "server slot creation failed");
+        *sl.h_counter = 0;
+    }
+    rb_server_s* raw = S.get();
+    S->th = std::thread([raw] { raw->loop(); });
+    *out = S.release();
+    return RB_OK;
+}
+
+rb_status rb_server_destroy(rb_server_t s) {
+    if (!s) return RB_OK;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (Req* q : s->queue) {
+            q->state = RB_REQ_FAILED;
+            q->result = RB_ERR_BUSY;
+            auto it = s->scenes.find(q->r.scene);
+            if (it != s->scenes.end()) it->second.refs--;
+        }
+        s->queue.clear();
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    if (s->th.joinable()) s->th.join();
+    for (Slot& sl : s->slots) {
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.d_counter) (void)hipFree(sl.d_counter);
+        if (sl.h_counter) (void)hipHostFree(sl.h_counter);
+        if (sl.pinned) (void)hipHostFree(sl.pinned);
+    }
+    delete s;
+    return RB_OK;
+}
+
+rb_status rb_server_add_scene(rb_server_t s, rb_bvh_t bvh, uint32_t* id) {
+    if (!s || !bvh || !id) return fail(RB_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    *id = s->next_scene++;
+    s->scenes[*id] = SceneEntry{bvh, 0};
+    return RB_OK;
+}
+
+rb_status rb_server_remove_scene(rb_server_t s, uint32_t id) {
+    if (!s) return fail(RB_ERR_INVALID, "NULL server");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->scenes.find(id);
+    if (it == s->scenes.end()) return fail(RB_ERR_INVALID, "unknown scene %u", id);
+    if (it->second.refs) return fail(RB_ERR_BUSY, "scene %u has %u queued or running requests", id, it->second.refs);
+    s->scenes.erase(it);
+    return RB_OK;
+}
+
+rb_status rb_server_submit(rb_server_t s, const rb_request* req, uint64_t* id) {
+    if (!s || !req) return fail(RB_ERR_INVALID, "NULL argument");
+    if (req->kind > RB_REQ_TRACE) return fail(RB_ERR_INVALID, "unknown request kind %u", req->kind);
+    std::unique_ptr<Req> q(new Req());
+    q->r = *req;
+    q->bytes = request_bytes(*req);
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->scenes.find(req->scene);
+    if (it == s->scenes.end()) return fail(RB_ERR_INVALID, "unknown scene %u", req->scene);
+    if (req->kind != RB_REQ_TRACE) {
+        if (req->w == 0 || req->h == 0 || req->x0 + req->w > req->camera.width || req->y0 + req->h > req->camera.height)
+            return fail(RB_ERR_INVALID, "bad tile");
+        if (req->params.spp == 0) return fail(RB_ERR_INVALID, "spp must be >= 1");
+        if (!it->second.bvh->tri_v) return fail(RB_ERR_INVALID, "scene %u has no shading data", req->scene);
+    } else if (req->n && (!req->rays || !req->hits)) {
+        return fail(RB_ERR_INVALID, "trace request without rays/hits");
+    }
+    if (q->bytes > s->cfg.hbm_budget) {
+        s->rejected++;
+        return fail(RB_ERR_OOM, "request needs %llu bytes, the server budget is %llu",
+                    (unsigned long long)q->bytes, (unsigned long long)s->cfg.hbm_budget);
+    }
+    if (s->queue.size() >= s->cfg.max_queue) {
+        s->rejected++;
+        return fail(RB_ERR_BUSY, "server queue full (%u)", s->cfg.max_queue);
+    }
+    q->bvh = it->second.bvh;
+    it->second.refs++;
+    q->id = s->next_id++;
+    q->t_submit = Clock::now();
+    if (!s->have_first) {
+        s->have_first = true;
+        s->first_submit = q->t_submit;
+    }
+    s->submitted++;
+    if (id) *id = q->id;
+    s->queue.push_back(q.get());
+    s->reqs[q->id] = std::move(q);
+    s->cv_work.notify_one();
+    return RB_OK;
+}
+
+static void fill_status(const Req& q, rb_request_status* st) {
+    st->state = q.state;
+    st->result = q.result;
+    st->rays = q.rays;
+    st->queue_ms = q.state >= RB_REQ_RUNNING && q.t_admit.time_since_epoch().count()
+                       ? std::chrono::duration<double, std::milli>(q.t_admit - q.t_submit).count()
+                       : 0.0;
+    st->latency_ms = q.state >= RB_REQ_DONE ? std::chrono::duration<double, std::milli>(q.t_done - q.t_submit).count()
+                                            : 0.0;
+}
+
+rb_status rb_server_poll(rb_server_t s, uint64_t id, rb_request_status* st) {
+    if (!s || !st) return fail(RB_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(RB_ERR_INVALID, "unknown request %llu", (unsigned long long)id);
+    fill_status(*it->second, st);
+    return st->state >= RB_REQ_DONE ? RB_OK : RB_ERR_NOT_READY;
+}
+
+rb_status rb_server_wait(rb_server_t s, uint64_t id, rb_request_status* st) {
+    if (!s) return fail(RB_ERR_INVALID, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return fail(RB_ERR_INVALID, "unknown request %llu", (unsigned long long)id);
+    Req* q = it->second.get();
+    s->cv_done.wait(lk, [&] { return q->state >= RB_REQ_DONE; });
+    if (st) fill_status(*q, st);
+    return RB_OK;
+}
+
+rb_status rb_server_drain(rb_server_t s) {
+    if (!s) return fail(RB_ERR_INVALID, "NULL server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] {
+        if (!s->queue.empty()) return false;
+        for (auto& kv : s->reqs)
+            if (kv.second->state < RB_REQ_DONE) return false;
+        return true;
+    });
+    return RB_OK;
+}
+
+rb_status rb_server_get_stats(rb_server_t s, rb_server_stats* out) {
+    if (!s || !out) return fail(RB_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    std::memset(out, 0, sizeof(*out));
+    out->submitted = s->submitted;
+    out->completed = s->completed;
+    out->failed = s->failed;
+    out->rejected = s->rejected;
+    out->deferrals = s->deferrals;
+    out->rays = s->rays;
+    out->peak_budget_bytes = s->peak;
+    if (!s->lat_ms.empty()) {
+        std::vector<double> v = s->lat_ms;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) {
+            double pos = p * (v.size() - 1);
+            size_t i = (size_t)pos;
+            double f = pos - i;
+            return i + 1 < v.size() ? v[i] * (1 - f) + v[i + 1] * f : v[i];
+        };
+        out->p50_ms = pct(0.5);
+        out->p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        out->mean_ms = sum / v.size();
+        out->max_ms = v.back();
+        out->first_submit_to_last_done_s = std::chrono::duration<double>(s->last_done - s->first_submit).count();
+    }
+    return RB_OK;
+}
+
+rb_status rb_server_reset_stats(rb_server_t s) {
+    if (!s) return fail(RB_ERR_INVALID, "NULL server");
+    std::lock_guard<std::mutex> g(s->mu);
+    s->submitted = s->completed = s->failed = s->rejected = s->deferrals = s->rays = 0;
+    s->lat_ms.clear();
+    s->have_first = false;
+    s->peak = s->used;
+    for (auto it = s->reqs.begin(); it != s->reqs.end();)
+        it = it->second->state >= RB_REQ_DONE ? s->reqs.erase(it) : std::next(it);
+    return RB_OK;
+}
+
+rb_status rb_server_inject_alloc_failures(rb_server_t s, uint32_t k) {
+    if (!s) return fail(RB_ERR_INVALID, "NULL server");
+    std::lock_guard<std::mutex> g(s->mu);
+    s->inject_failures = k;
+    return RB_OK;
+}
+
+}  // extern "C"

@@ -80,3 +80,40 @@ def test_render_rejects_bad_arguments(gpu, ao_setup):
     bare = gpu.CudaTracer(b)  # no scene data uploaded
     with pytest.raises(gpu.RbError):
         gpu.SamplerRenderer(bare, cam, "pt").render()
+
+
+BD = dict(max_depth=8, rr_start=3, rr_prob=0.5, light_paths=4096, light_max_depth=4, connections=3)
+
+
+def test_bdpt_image_bit_exact(gpu, orc, pt_setup):
+    s, b, tr = pt_setup
+    cam = gpu.scenes.pt_camera(48, 48)
+    r = gpu.SamplerRenderer(tr, cam, "bdpt", spp=2, seed=5, **BD)
+    img = r.render()
+    v, idx, tm, lt = s.geometry()
+    ref = orc.render_bdpt(_c2(b), v, idx, tm, s.materials(), lt, _cam_tuple(cam), (0, 0, 48, 48), 2, 5,
+                          BD["max_depth"], BD["rr_start"], BD["rr_prob"], BD["light_paths"], BD["light_max_depth"],
+                          BD["connections"])
+    assert np.isfinite(img).all() and img.max() > 0
+    bad = img.view(np.uint32) != ref.view(np.uint32)
+    assert not bad.any(), f"{bad.sum()} channels differ, max abs {np.abs(img - ref).max()}"
+    # a tile is the crop of the frame (the light pass is keyed by the seed only)
+    assert np.array_equal(r.render((8, 16, 24, 20)), img[16:36, 8:32])
+
+
+def test_bdpt_without_connections_is_the_path_tracer(gpu, pt_setup):
+    s, b, tr = pt_setup
+    cam = gpu.scenes.pt_camera(32, 32)
+    p = dict(BD, connections=0)
+    bd = gpu.SamplerRenderer(tr, cam, "bdpt", spp=2, seed=6, **p).render()
+    pt = gpu.SamplerRenderer(tr, cam, "pt", spp=2, seed=6, max_depth=8, rr_start=3, rr_prob=0.5).render()
+    assert np.array_equal(bd.view(np.uint32), pt.view(np.uint32))
+
+
+def test_bdpt_rejects_bad_arguments(gpu, pt_setup):
+    s, b, tr = pt_setup
+    cam = gpu.scenes.pt_camera(16, 16)
+    for bad in [dict(connections=9), dict(light_max_depth=0), dict(light_max_depth=9),
+                dict(max_depth=24, light_max_depth=8), dict(light_paths=0, lvc_capacity=0)]:
+        with pytest.raises(gpu.RbError):
+            gpu.SamplerRenderer(tr, cam, "bdpt", **dict(BD, **bad)).render()
