@@ -65,16 +65,18 @@ def read_traffic(path):
         return None
 
 
-def bench_render(rb, kind, scene, bvh_max_leaf, cam, frames, **params):
+def bench_render(rb, kind, scene, bvh_max_leaf, cam, frames, built=None, **params):
     """Time whole frames (raygen + integrator + film) with HIP events; rays
-    traced per frame come from the device ray counter."""
+    traced per frame come from the device ray counter.  built: (tracer,
+    build seconds) of `scene` when already uploaded."""
     import ctypes as C
 
     import numpy as np
-    t0 = time.time()
-    bvh = rb.scenes.build(scene, max_leaf=bvh_max_leaf)
-    build_s = time.time() - t0
-    tracer = rb.CudaTracer(bvh, scene)
+    if built is None:
+        t0 = time.time()
+        bvh = rb.scenes.build(scene, max_leaf=bvh_max_leaf)
+        built = (rb.CudaTracer(bvh, scene), time.time() - t0)
+    tracer, build_s = built
     stream = rb.Stream()
     counter = rb.DeviceBuffer(8)
     counter.zero()
@@ -167,6 +169,152 @@ def bench_batched(rb, reps=5, seed=0):
             "setup_s": round(build_s, 1)}
 
 
+SERVED_SCENES = [100_000, 500_000, 1_000_000, 2_000_000]
+
+
+def served_mix(nreq, seed, pt_cam, bd_cam):
+    """Config 5 request list: 40% AO tiles (side 64-512, 8 AO rays), 40% PT
+    tiles (side 128-1024, 4-16 spp, depth 8), 20% BDPT frames (512x512,
+    4 spp, light-vertex cache of 2^20 vertices, depth 6), scenes uniform."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(nreq):
+        u, sc = rng.uniform(), int(rng.integers(0, len(SERVED_SCENES)))
+        rs = 10_000 + i
+        if u < 0.4:
+            side = 8 * int(rng.integers(8, 65))
+            x0, y0 = (int(v) for v in rng.integers(0, pt_cam.width - side + 1, 2))
+            out.append(("ao", sc, pt_cam, (x0, y0, side, side), dict(spp=1, seed=rs, ao_samples=8, ao_radius=1.0)))
+        elif u < 0.8:
+            side = 8 * int(rng.integers(16, 129))
+            x0, y0 = (int(v) for v in rng.integers(0, pt_cam.width - side + 1, 2))
+            out.append(("pt", sc, pt_cam, (x0, y0, side, side),
+                        dict(spp=int(rng.integers(4, 17)), seed=rs, max_depth=8, rr_start=3, rr_prob=0.5)))
+        else:
+            out.append(("bdpt", sc, bd_cam, None, dict(spp=4, seed=rs, max_depth=6, rr_start=3, rr_prob=0.5,
+                                                       light_max_depth=6, lvc_capacity=1 << 20, light_paths=0,
+                                                       connections=5)))
+    return out
+
+
+def _digest(img):
+    try:
+        import xxhash
+        return xxhash.xxh3_64_intdigest(img.tobytes())
+    except ImportError:
+        import hashlib
+        return hashlib.blake2b(img.tobytes(), digest_size=8).hexdigest()
+
+
+def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, seed=0):
+    """Config 5.  The same seeded request list three ways:
+    plain  : one request at a time on one stream, host sync + result copy
+             after each (the reference's synchronous call pattern);
+    burst  : every request submitted to the server at once -> capacity;
+    poisson: arrivals at `load` x capacity -> p50/p99 latency (submit ->
+             result on the host) and aggregate Mrays/s.
+    Rays per request come from the device ray counter; every served image is
+    checked against the plain one by digest."""
+    import ctypes as C
+    import queue
+    import threading
+
+    import numpy as np
+    t0 = time.time()
+    tracers, tris = [], []
+    for k, T in enumerate(SERVED_SCENES):
+        s = rb.scenes.pt_scaled(T, seed=k)
+        tracers.append(rb.CudaTracer(rb.scenes.build(s), s))
+        tris.append(s.get_num_triangles())
+        log(f"served: scene {k} ({s.get_num_triangles()} tris) ready, {time.time() - t0:.1f}s")
+    setup_s = time.time() - t0
+    pt_cam, bd_cam = rb.scenes.pt_camera(1024, 1024), rb.scenes.pt_camera(512, 512)
+    reqs = served_mix(nreq, seed, pt_cam, bd_cam)
+
+    # plain
+    stream = rb.Stream()
+    counter = rb.DeviceBuffer(8)
+    dout = rb.DeviceBuffer(1024 * 1024 * 12)
+    ref, rays = [], []
+    t_plain = 0.0
+    tl = time.time()
+    for i, (kind, sc, cam, tile, p) in enumerate(reqs):
+        r = rb.SamplerRenderer(tracers[sc], cam, kind, **p)
+        r.params.ray_counter = C.c_void_p(counter.ptr)
+        counter.zero(stream.handle)
+        stream.synchronize()
+        t1 = time.perf_counter()
+        w, h = r.render_device(dout, tile, stream.handle)
+        img = dout.download(np.float32, w * h * 3, stream.handle)
+        stream.synchronize()
+        t_plain += time.perf_counter() - t1
+        ref.append(_digest(img))
+        rays.append(int(counter.download(np.uint64, 1)[0]))
+        if time.time() - tl > 10:
+            log(f"served/plain: {i + 1}/{nreq} requests, {t_plain:.1f}s of work")
+            tl = time.time()
+    total_rays = sum(rays)
+    kinds = [q[0] for q in reqs]
+
+    def run(arrivals):
+        srv = rb.RenderServer(max_inflight=max_inflight)
+        sids = [srv.add_scene(t) for t in tracers]
+        rq = queue.Queue()
+
+        def submitter():
+            ts = time.perf_counter()
+            for i, (kind, sc, cam, tile, p) in enumerate(reqs):
+                if arrivals is not None:
+                    dt = arrivals[i] - (time.perf_counter() - ts)
+                    if dt > 0:
+                        time.sleep(dt)
+                rq.put(srv.submit_render(kind, sids[sc], cam, tile, **p))
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        ok, lat, got_rays = True, [], 0
+        tl = time.time()
+        for i in range(nreq):
+            st, img = srv.wait(rq.get())
+            ok &= st.result == 0 and _digest(img) == ref[i]
+            lat.append(st.latency_ms)
+            got_rays += st.rays
+            if time.time() - tl > 10:
+                log(f"served/{'poisson' if arrivals is not None else 'burst'}: {i + 1}/{nreq} done")
+                tl = time.time()
+        th.join()
+        stt = srv.stats()
+        srv.close()
+        return ok and got_rays == total_rays, np.array(lat), stt
+
+    ok_b, lat_b, st_b = run(None)
+    span_b = st_b["first_submit_to_last_done_s"]
+    cap = nreq / span_b
+    arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / (load * cap), nreq))
+    ok_p, lat_p, st_p = run(arr)
+    span_p = st_p["first_submit_to_last_done_s"]
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)), 2) if len(a) else None
+
+    per_kind = {}
+    for k in ("ao", "pt", "bdpt"):
+        sel = np.array([x == k for x in kinds])
+        per_kind[k] = {"n": int(sel.sum()), "p50_ms": pct(lat_p[sel], 50), "p99_ms": pct(lat_p[sel], 99),
+                       "mrays_each": round(sum(r for r, x in zip(rays, kinds) if x == k) / max(1, sel.sum()) / 1e6, 2)}
+    return {"requests": nreq, "scene_tris": tris, "rays": total_rays,
+            "plain_mrays": round(total_rays / t_plain / 1e6, 1), "plain_s": round(t_plain, 2),
+            "served_capacity_mrays": round(total_rays / span_b / 1e6, 1), "burst_s": round(span_b, 2),
+            "speedup": round(t_plain / span_b, 3),
+            "poisson_load": load, "poisson_rate_per_s": round(load * cap, 2),
+            "poisson_mrays": round(total_rays / span_p / 1e6, 1), "poisson_s": round(span_p, 2),
+            "p50_ms": round(st_p["p50_ms"], 2), "p99_ms": round(st_p["p99_ms"], 2),
+            "burst_p50_ms": round(st_b["p50_ms"], 2), "burst_p99_ms": round(st_b["p99_ms"], 2),
+            "per_kind": per_kind, "max_inflight": max_inflight, "deferrals": st_p["deferrals"],
+            "match": bool(ok_b and ok_p), "setup_s": round(setup_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,11 +323,16 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="also time coherent / any-hit / mixed sets")
     ap.add_argument("--autotune", type=int, default=1, help="pick the traversal config on this batch first")
-    ap.add_argument("--config", default="", help="block,lds_stack,chunk,fetch_threshold,waves_per_cu")
+    ap.add_argument("--config", default="",
+                    help="block,lds_stack,chunk,fetch_threshold,waves_per_cu[,speculative[,leaf_gate]]")
+    ap.add_argument("--speculative", type=int, default=0,
+                    help="1: the reference's speculative leaf order (faster, wave-dependent in rare cases)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one copy kernel of this size first (traffic counter calibration)")
     ap.add_argument("--render", type=int, default=1, help="also time the AO and path-traced frames")
     ap.add_argument("--batched", type=int, default=1, help="also time batched vs plain (8 scenes, 64 batches)")
+    ap.add_argument("--served", type=int, default=2000, help="served-mix requests (0 = skip)")
+    ap.add_argument("--served-inflight", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -222,8 +375,9 @@ def main():
         stream.synchronize()
         del src, dst
     tuned_ms = None
+    rb.set_trace_config(speculative=args.speculative)
     if args.config:
-        keys = ["block", "lds_stack", "chunk", "fetch_threshold", "waves_per_cu"]
+        keys = ["block", "lds_stack", "chunk", "fetch_threshold", "waves_per_cu", "speculative", "leaf_gate"]
         rb.set_trace_config(**dict(zip(keys, (int(x) for x in args.config.split(",")))))
     elif args.autotune:
         _, tuned_ms = rb.autotune(tracer, d_diff, n, d_hits, reps=3, stream=stream)
@@ -281,7 +435,7 @@ def main():
     if bytes_per_ray is None:
         bytes_per_ray = gpu_bpr
 
-    if args.render or args.batched:
+    if args.render or args.batched or args.served:
         del tracer, d_prim, d_diff, d_mixed, d_hits
     if args.batched:
         bt = bench_batched(rb)
@@ -292,10 +446,23 @@ def main():
                           spp=1, seed=0, ao_samples=8, ao_radius=1.0)
         log("ao frame (config 1: 78,848 tris, 512x512, 1 spp, 8 AO rays)", ao)
         extra["ao_512"] = ao
-        pt = bench_render(rb, "pt", rb.scenes.pt_room(seed=0), None, rb.scenes.pt_camera(1024, 1024), 2,
+        room = rb.scenes.pt_room(seed=0)
+        t0 = time.time()
+        built = (rb.CudaTracer(rb.scenes.build(room), room), time.time() - t0)
+        pt = bench_render(rb, "pt", room, None, rb.scenes.pt_camera(1024, 1024), 2, built,
                           spp=8, seed=0, max_depth=8, rr_start=3, rr_prob=0.5)
         log("pt frame (config 3: 2,000,136 tris, 1024x1024, 8 spp, depth 8)", pt)
         extra["pt_1024_8spp"] = pt
+        bd = bench_render(rb, "bdpt", room, None, rb.scenes.pt_camera(512, 512), 3, built,
+                          spp=4, seed=0, max_depth=6, rr_start=3, rr_prob=0.5, light_max_depth=6,
+                          lvc_capacity=1 << 20, light_paths=0, connections=5)
+        log("bdpt frame (config 5 BDPT: 2,000,136 tris, 512x512, 4 spp, 2^20-vertex cache, depth 6)", bd)
+        extra["bdpt_512_4spp"] = bd
+        del built
+    if args.served:
+        sv = bench_served(rb, args.served, max_inflight=args.served_inflight)
+        log("served mix (config 5)", sv)
+        extra["served"] = sv
 
     achieved = bytes_per_ray * n / (ms * 1e-3) / 1e9
     traffic = read_traffic(os.path.join(ROOT, "profiles", "trace_incoherent_1m_pmc.txt"))

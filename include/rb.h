@@ -253,13 +253,24 @@ typedef struct rb_trace_config {
     uint32_t chunk;          /* rays claimed per work-counter atomic (multiple of 64) */
     uint32_t fetch_threshold;/* refill when fewer active lanes than this (0..64) */
     uint32_t waves_per_cu;   /* persistent grid size = CUs x this / (block/64) */
+    uint32_t speculative;    /* 0 (default): each ray tests its leaves in its own
+                                traversal order, results are reproducible and equal
+                                the single-ray restatement; 1: the reference's
+                                speculative while-while (kepler_dynamic_fetch.h:222-243),
+                                whose leaf order depends on the other rays of the
+                                wave -- in rare boundary cases ids and t may differ */
+    uint32_t leaf_gate;      /* non-speculative order: a wave leaves its inner-node phase
+                                once at most this many lanes still search for a leaf
+                                (0..63; 0 = when all have one).  Any value gives the
+                                same results: only the interleaving of lanes changes */
 } rb_trace_config;
 rb_status rb_trace_get_config(rb_trace_config* c);
 rb_status rb_trace_set_config(const rb_trace_config* c);
 /* Autotuner (no reference counterpart; the reference hard-codes the launch
  * shape): times candidate configurations on the caller's device rays by
- * coordinate descent over block, lds_stack, chunk, fetch_threshold and
- * waves_per_cu, installs the fastest and returns it with its time per launch. */
+ * coordinate descent over block, lds_stack, chunk, fetch_threshold,
+ * waves_per_cu and leaf_gate (speculative is kept as set), installs the
+ * fastest and returns it with its time per launch. */
 rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits,
                             uint32_t reps, rb_trace_config* best, float* best_ms, void* stream);
 /* Per-ray traversal counters of the device kernel (inner nodes, triangles,

@@ -35,6 +35,8 @@ static rb_status init_locked(int device) {
     g_dev.cfg.chunk = 64;
     g_dev.cfg.fetch_threshold = 24;
     g_dev.cfg.waves_per_cu = 16;
+    g_dev.cfg.speculative = 0;
+    g_dev.cfg.leaf_gate = 0;
     g_dev.device = device;
     return RB_OK;
 }

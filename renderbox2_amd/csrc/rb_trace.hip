@@ -15,6 +15,15 @@
 //  * the work counter lives in a per-launch slot that the last block resets
 //    (the reference copies a zero into a global symbol before each launch,
 //    cudatracer.cu:164-165), so concurrent launches and graph replays work;
+//  * leaf order: by default a lane tests a leaf as soon as it reaches one,
+//    so every ray visits nodes and leaves in the same order whatever the
+//    other rays of its wave do -- the order of the single-ray restatement --
+//    and results are the same run to run.  SPEC = the reference's
+//    speculative form (a lane holding a leaf keeps walking inner nodes until
+//    the whole wave holds leaves, kepler_dynamic_fetch.h:222-243): fewer
+//    idle lanes, but the leaf order then depends on the wave, and where a
+//    slab test rounds a box entry past a hit inside it, which hit survives
+//    culling can change with it (rare; ids and t may then differ);
 //  * batched mode: one launch serves up to kMaxBatch ray batches against
 //    different resident BVHs; every lane carries the scene pointers and the
 //    any/closest flag of the batch its current ray belongs to, so a wave can
@@ -38,6 +47,7 @@ struct TraceArgs {
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
     uint32_t chunk;
     uint32_t threshold;
+    uint32_t leaf_gate;  // non-speculative: leave the inner-node phase once <= this many lanes still search
 };
 
 constexpr uint32_t kMaxBatch = 64;
@@ -58,7 +68,7 @@ struct BatchTable {
     BatchDesc d[kMaxBatch];
 };
 
-template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI>
+template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC>
 __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays) {
     __shared__ int lstack[S * BLOCK];
     int spill[kMaxStack - S];
@@ -158,7 +168,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
 
         // ---- traversal ----
         while (nodeAddr != kSentinel) {
-            while ((uint32_t)nodeAddr < (uint32_t)kSentinel) {
+            while ((uint32_t)nodeAddr < (uint32_t)kSentinel && (SPEC || leafAddr >= 0)) {
                 const float4* np = nodes + 4 * (size_t)nodeAddr;
                 const float4 n0 = np[0];
                 const float4 n1 = np[1];
@@ -196,7 +206,11 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                     leafAddr = nodeAddr;
                     RB_POP(nodeAddr);
                 }
-                if (!__any(leafAddr >= 0)) break;
+                if (SPEC) {
+                    if (!__any(leafAddr >= 0)) break;
+                } else if (__popcll(__ballot(1)) <= a.leaf_gate) {
+                    break;  // the rest keep their place and go on after the leaf phase
+                }
             }
 
             while (leafAddr < 0) {
@@ -269,18 +283,18 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     }
 }
 
-template <bool ANY, bool COUNT, int S, int BLOCK>
+template <bool ANY, bool COUNT, int S, int BLOCK, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     const uint32_t nrays = a.n_dev ? __builtin_nontemporal_load(a.n_dev) : a.n;
-    trace_body<ANY, COUNT, S, BLOCK, false>(a, nullptr, 0, nrays);
+    trace_body<ANY, COUNT, S, BLOCK, false, SPEC>(a, nullptr, 0, nrays);
 }
 
-template <int S, int BLOCK>
+template <int S, int BLOCK, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a, BatchTable t) {
     __shared__ BatchDesc tab[kMaxBatch];
     for (uint32_t i = threadIdx.x; i < t.nb; i += BLOCK) tab[i] = t.d[i];
     __syncthreads();
-    trace_body<false, false, S, BLOCK, true>(a, tab, t.nb, t.total);
+    trace_body<false, false, S, BLOCK, true, SPEC>(a, tab, t.nb, t.total);
 }
 
 // ---------------------------------------------------------------- launch --
@@ -294,17 +308,19 @@ typedef void (*BatchKernel)(TraceArgs, BatchTable);
     X(24, 64) X(24, 128) X(24, 256)
 
 template <bool ANY, bool COUNT>
-static TraceKernel pick(uint32_t S, uint32_t B) {
-#define RB_K(s, b) \
-    if (S == s && B == b) return k_trace<ANY, COUNT, s, b>;
+static TraceKernel pick(const rb_trace_config& c) {
+#define RB_K(s, b)                             \
+    if (c.lds_stack == s && c.block == b)      \
+        return c.speculative ? k_trace<ANY, COUNT, s, b, true> : k_trace<ANY, COUNT, s, b, false>;
     RB_FOR_CONFIGS(RB_K)
 #undef RB_K
     return nullptr;
 }
 
-static BatchKernel pick_batch(uint32_t S, uint32_t B) {
-#define RB_K(s, b) \
-    if (S == s && B == b) return k_trace_batch<s, b>;
+static BatchKernel pick_batch(const rb_trace_config& c) {
+#define RB_K(s, b)                             \
+    if (c.lds_stack == s && c.block == b)      \
+        return c.speculative ? k_trace_batch<s, b, true> : k_trace_batch<s, b, false>;
     RB_FOR_CONFIGS(RB_K)
 #undef RB_K
     return nullptr;
@@ -326,8 +342,8 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     if (n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "too many rays in one call (%u)", n);
     if ((flags & RB_MASKED) && !mask) return fail(RB_ERR_INVALID, "RB_MASKED without a mask");
     const bool any = (flags & RB_ANY) != 0;
-    TraceKernel k = counts ? (any ? pick<true, true>(cfg.lds_stack, cfg.block) : pick<false, true>(cfg.lds_stack, cfg.block))
-                           : (any ? pick<true, false>(cfg.lds_stack, cfg.block) : pick<false, false>(cfg.lds_stack, cfg.block));
+    TraceKernel k = counts ? (any ? pick<true, true>(cfg) : pick<false, true>(cfg))
+                           : (any ? pick<true, false>(cfg) : pick<false, false>(cfg));
     if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
     TraceArgs a;
     a.nodes = bvh->nodes;
@@ -342,6 +358,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.n_dev = n_dev;
     a.chunk = cfg.chunk;
     a.threshold = cfg.fetch_threshold;
+    a.leaf_gate = cfg.leaf_gate;
     hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), 0, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "trace launch");
@@ -349,7 +366,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
 }
 
 rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg) {
-    BatchKernel k = pick_batch(cfg.lds_stack, cfg.block);
+    BatchKernel k = pick_batch(cfg);
     if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
     for (uint32_t i0 = 0; i0 < nb; i0 += kMaxBatch) {
         BatchTable t;
@@ -378,6 +395,7 @@ rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t st
         a.n = t.total;
         a.chunk = cfg.chunk;
         a.threshold = cfg.fetch_threshold;
+        a.leaf_gate = cfg.leaf_gate;
         hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, total)), dim3(cfg.block), 0, stream, a, t);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "batched trace launch");
@@ -391,6 +409,8 @@ static rb_status check_cfg(const rb_trace_config& c) {
     if (c.chunk < 64 || c.chunk % 64 || c.chunk > 65536) return fail(RB_ERR_INVALID, "chunk must be a multiple of 64 up to 65536");
     if (c.fetch_threshold > 64) return fail(RB_ERR_INVALID, "fetch_threshold must be <= 64");
     if (c.waves_per_cu < 1 || c.waves_per_cu > 64) return fail(RB_ERR_INVALID, "waves_per_cu must be 1..64");
+    if (c.speculative > 1) return fail(RB_ERR_INVALID, "speculative must be 0 or 1");
+    if (c.leaf_gate > 63) return fail(RB_ERR_INVALID, "leaf_gate must be 0..63");
     return RB_OK;
 }
 
@@ -465,8 +485,9 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
         const uint32_t chunks[] = {64, 128, 256, 1024};
         const uint32_t thresholds[] = {0, 8, 16, 24, 32, 40, 48};
         const uint32_t waves[] = {8, 12, 16, 24, 32};
+        const uint32_t gates[] = {0, 4, 8, 16, 24, 32};
         for (int pass = 0; pass < 2; pass++) {
-            for (int dim = 0; dim < 5; dim++) {
+            for (int dim = 0; dim < 6; dim++) {
                 const uint32_t* vals;
                 int nv;
                 switch (dim) {
@@ -474,12 +495,14 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
                     case 1: vals = stacks; nv = 5; break;
                     case 2: vals = chunks; nv = 4; break;
                     case 3: vals = thresholds; nv = 7; break;
-                    default: vals = waves; nv = 5; break;
+                    case 4: vals = waves; nv = 5; break;
+                    default: vals = gates; nv = 6; break;
                 }
+                if (dim == 5 && cur.speculative) continue;  // the gate only acts on the non-speculative order
                 for (int k = 0; k < nv; k++) {
                     rb_trace_config c = cur;
                     uint32_t* f = dim == 0 ? &c.block : dim == 1 ? &c.lds_stack : dim == 2 ? &c.chunk
-                                : dim == 3 ? &c.fetch_threshold : &c.waves_per_cu;
+                                : dim == 3 ? &c.fetch_threshold : dim == 4 ? &c.waves_per_cu : &c.leaf_gate;
                     if (*f == vals[k]) continue;
                     *f = vals[k];
                     float ms = 0.f;

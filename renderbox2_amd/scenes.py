@@ -7,6 +7,8 @@ the explicit choices, recorded with every result):
   pt_room      room (2,048) + 187 icospheres s4 + 813 s3 (1,998,080)
                + 4 emissive quads (8)                                     = 2,000,136 tris
   trace_scaled(T): round(0.96 T / 1280) icospheres s3 + the rest as thin quads = T tris
+  pt_scaled(T) : pt_room with the sphere count (and s4 share) set so the
+                 total is within one sphere of T; pt_scaled(2e6) = pt_room
 """
 from . import _lib as L
 from .core import BVH, Platform, PerspectiveCamera, Scene
@@ -58,6 +60,18 @@ def pt_room(seed=0, spheres_hi=187, spheres_lo=813, emit=(12.0, 12.0, 12.0)):
                seed=seed, r=(0.1, 0.5), c=(-4.0, 4.0), mix_materials=True)
     s.generate(L.RB_GEN_EMITTER_QUADS, count=4, seed=seed, r=(0.25, 0.0), c=(-5.0, 5.0), emit=emit)
     return s
+
+
+def pt_scaled(total_tris, seed=0):
+    """The path-tracing room with about total_tris triangles: up to 1000
+    spheres of subdivision 3, then s4 spheres replacing s3 ones."""
+    fixed = 2048 + 8
+    lo = int(round((total_tris - fixed) / ICO3))
+    if lo <= 1000:
+        return pt_room(seed=seed, spheres_hi=0, spheres_lo=max(lo, 1))
+    hi = int(round((total_tris - fixed - 1000 * ICO3) / (ICO4 - ICO3)))
+    hi = min(hi, 1000)
+    return pt_room(seed=seed, spheres_hi=hi, spheres_lo=1000 - hi)
 
 
 def pt_camera(width=1024, height=1024):

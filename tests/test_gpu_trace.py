@@ -2,7 +2,10 @@
 
 Closest hit: triangle id and t bit-identical to the oracle's restatement of
 the reference traversal on the same SBVH (ties to the lowest id).  Any hit:
-hit/miss identical, and the reported (id, t) is a real hit of that ray.
+with the default (non-speculative) leaf order every ray walks the tree in the
+oracle's order, so the first hit found is the oracle's, bit for bit; with the
+speculative order hit/miss is identical and the reported (id, t) is a real
+hit of that ray.
 """
 import ctypes as C
 
@@ -14,9 +17,10 @@ from conftest import random_rays, random_triangles
 pytestmark = pytest.mark.gpu
 
 CONFIGS = [
-    dict(block=128, lds_stack=16, chunk=64, fetch_threshold=24, waves_per_cu=16),
-    dict(block=64, lds_stack=8, chunk=256, fetch_threshold=0, waves_per_cu=8),
-    dict(block=256, lds_stack=24, chunk=64, fetch_threshold=48, waves_per_cu=32),
+    dict(block=128, lds_stack=16, chunk=64, fetch_threshold=24, waves_per_cu=16, speculative=0, leaf_gate=0),
+    dict(block=64, lds_stack=8, chunk=256, fetch_threshold=0, waves_per_cu=8, speculative=0, leaf_gate=40),
+    dict(block=256, lds_stack=24, chunk=64, fetch_threshold=48, waves_per_cu=32, speculative=0, leaf_gate=16),
+    dict(block=256, lds_stack=16, chunk=64, fetch_threshold=40, waves_per_cu=32, speculative=1, leaf_gate=0),
 ]
 
 
@@ -65,6 +69,10 @@ def test_trace_small_bit_exact(gpu, orc, small, cfg):
         assert np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
         aid, at = tr.trace(rays, any_hit=True)
         _check_any(orc, (v, idx), rays, ids, ts, aid, at)
+        if not CONFIGS[cfg]["speculative"]:
+            oaid, oat = orc.trace_compact2(*c2, rays, any_hit=True)
+            assert np.array_equal(aid, oaid)
+            assert np.array_equal(at.view(np.uint32), oat.view(np.uint32))
     finally:
         gpu.set_trace_config(**CONFIGS[0])
 
@@ -207,6 +215,18 @@ def test_million_triangles_properties(gpu, orc):
     sel = np.random.default_rng(0).choice(len(diff), 65536, replace=False)
     oid, ot = orc.trace_compact2(*c2, diff[sel])
     assert np.array_equal(ids[sel], oid) and np.array_equal(ts[sel].view(np.uint32), ot.view(np.uint32))
+    oaid, _ = orc.trace_compact2(*c2, diff[sel], any_hit=True)
+    assert np.array_equal(aid[sel], oaid)
+    # every launch shape gives the same answers: each ray's visit order is its own
+    try:
+        for cfg in CONFIGS[1:3]:
+            gpu.set_trace_config(**cfg)
+            i3, t3 = tr.trace(diff)
+            a3, _ = tr.trace(diff, any_hit=True)
+            assert np.array_equal(i3, ids) and np.array_equal(t3.view(np.uint32), ts.view(np.uint32))
+            assert np.array_equal(a3, aid)
+    finally:
+        gpu.set_trace_config(**CONFIGS[0])
 
 
 def test_batched_trace_equals_per_batch_calls(gpu):
