@@ -207,7 +207,7 @@ def _digest(img):
         return hashlib.blake2b(img.tobytes(), digest_size=8).hexdigest()
 
 
-def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, seed=0):
+def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, policy="deadline", seed=0):
     """Config 5.  The same seeded request list three ways:
     plain  : one request at a time on one stream, host sync + result copy
              after each (the reference's synchronous call pattern);
@@ -258,7 +258,7 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, seed=0):
     kinds = [q[0] for q in reqs]
 
     def run(arrivals):
-        srv = rb.RenderServer(max_inflight=max_inflight)
+        srv = rb.RenderServer(max_inflight=max_inflight, policy=policy)
         sids = [srv.add_scene(t) for t in tracers]
         rq = queue.Queue()
 
@@ -311,7 +311,7 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, seed=0):
             "poisson_mrays": round(total_rays / span_p / 1e6, 1), "poisson_s": round(span_p, 2),
             "p50_ms": round(st_p["p50_ms"], 2), "p99_ms": round(st_p["p99_ms"], 2),
             "burst_p50_ms": round(st_b["p50_ms"], 2), "burst_p99_ms": round(st_b["p99_ms"], 2),
-            "per_kind": per_kind, "max_inflight": max_inflight, "deferrals": st_p["deferrals"],
+            "per_kind": per_kind, "max_inflight": max_inflight, "policy": policy, "deferrals": st_p["deferrals"],
             "match": bool(ok_b and ok_p), "setup_s": round(setup_s, 1)}
 
 
@@ -333,6 +333,7 @@ def main():
     ap.add_argument("--batched", type=int, default=1, help="also time batched vs plain (8 scenes, 64 batches)")
     ap.add_argument("--served", type=int, default=2000, help="served-mix requests (0 = skip)")
     ap.add_argument("--served-inflight", type=int, default=8)
+    ap.add_argument("--served-policy", default="deadline", choices=["deadline", "fifo"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -460,7 +461,7 @@ def main():
         extra["bdpt_512_4spp"] = bd
         del built
     if args.served:
-        sv = bench_served(rb, args.served, max_inflight=args.served_inflight)
+        sv = bench_served(rb, args.served, max_inflight=args.served_inflight, policy=args.served_policy)
         log("served mix (config 5)", sv)
         extra["served"] = sv
 

@@ -206,6 +206,17 @@ rb_status rb_bvh_load_text_compact2(const char* path, void* nodes, size_t* nodes
 rb_status rb_bvh_upload(rb_host_bvh_t b, rb_scene_t scene, rb_bvh_t* out);
 rb_status rb_bvh_release(rb_bvh_t b);
 rb_status rb_bvh_device_bytes(rb_bvh_t b, size_t* bytes);
+/* Device view of a resident BVH for kernels that trace in their own code
+ * through rb_trace_one() (include/rb_trace_device.h); the reference hands
+ * its device tracer the same three buffers as texture objects
+ * (CudaTracer::get_bvh / BvhStruct, src/accelerators/sbvh/cudatracer.h:44-55,89-101). */
+typedef struct rb_bvh_dev {
+    const void* nodes;      /* float4[4 * n_nodes]: MI355X node records */
+    const void* tris;       /* float4[3 * n_refs]: Woop rows */
+    const uint32_t* index;  /* uint32[n_refs]: original triangle id per reference */
+    uint32_t n_nodes, n_refs;
+} rb_bvh_dev;
+rb_status rb_bvh_device_view(rb_bvh_t b, rb_bvh_dev* out);
 
 /* ---- trace ---------------------------------------------------------------- */
 
@@ -228,6 +239,12 @@ rb_status rb_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
 rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
                         const uint8_t* mask, rb_hit* hits, const rb_isect_soa* out,
                         void* stream);
+
+/* One ray per thread through the device function rb_trace_one(): the
+ * reference's kernel_trace_ray form (cuda_tracer_functions.cu:42-279)
+ * rather than the persistent kernel.  Same results as rb_trace, bit for
+ * bit; kept for in-kernel callers' reference and comparison. */
+rb_status rb_trace_simple(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits, void* stream);
 
 /* Batched trace (no reference counterpart: the reference issues one
  * synchronous launch per call, cudatracer.cu:134-183).  Ray batches of
@@ -363,10 +380,14 @@ rb_status rb_render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32
  * launches, tracks completion and records latency (submit -> result ready). */
 typedef struct rb_server_s* rb_server_t;
 
+enum { RB_SCHED_DEADLINE = 0, RB_SCHED_FIFO = 1 };
+
 typedef struct rb_server_config {
     uint32_t max_inflight; /* requests on the device at once (1..32) */
     uint64_t hbm_budget;   /* bytes of request work memory (0: 60% of free HBM) */
     uint32_t max_queue;    /* waiting requests before submit says RB_ERR_BUSY (0: 65536) */
+    uint32_t policy;       /* RB_SCHED_DEADLINE (default): earliest of submit + 4 x the
+                              estimated GPU time first; RB_SCHED_FIFO: arrival order */
 } rb_server_config;
 
 enum { RB_REQ_AO = 0, RB_REQ_PT = 1, RB_REQ_BDPT = 2, RB_REQ_TRACE = 3 };

@@ -121,6 +121,8 @@ _SIGS = {
     "rb_raygen_camera": (C.c_int, [P, U32, U32, U32, U32, U32, U64, C.c_int, P, P]),
     "rb_raygen_diffuse": (C.c_int, [P, P, P, U32, U32, U64, P, P]),
     "rb_trace_batch": (C.c_int, [P, U32, P]),
+    "rb_bvh_device_view": (C.c_int, [P, P]),
+    "rb_trace_simple": (C.c_int, [P, P, U32, U32, P, P]),
     "rb_render_params_default": (None, [P]),
     "rb_render_ao": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_pt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
@@ -144,8 +146,11 @@ RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT, RB_REQ_TRACE = 0, 1, 2, 3
 RB_REQ_QUEUED, RB_REQ_RUNNING, RB_REQ_DONE, RB_REQ_FAILED = 0, 1, 2, 3
 
 
+RB_SCHED_DEADLINE, RB_SCHED_FIFO = 0, 1
+
+
 class rb_server_config(C.Structure):
-    _fields_ = [("max_inflight", U32), ("hbm_budget", U64), ("max_queue", U32)]
+    _fields_ = [("max_inflight", U32), ("hbm_budget", U64), ("max_queue", U32), ("policy", U32)]
 
 
 class rb_request_status(C.Structure):
@@ -157,6 +162,10 @@ class rb_server_stats(C.Structure):
     _fields_ = [("submitted", U64), ("completed", U64), ("failed", U64), ("rejected", U64), ("deferrals", U64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
                 ("rays", U64), ("first_submit_to_last_done_s", C.c_double), ("peak_budget_bytes", U64)]
+
+
+class rb_bvh_dev(C.Structure):
+    _fields_ = [("nodes", P), ("tris", P), ("index", P), ("n_nodes", U32), ("n_refs", U32)]
 
 
 class rb_batch_desc(C.Structure):

@@ -333,6 +333,22 @@ class CudaTracer:
         h = dh.download(HIT_DTYPE, n, _sp(stream))
         return h["id"].copy(), h["t"].copy()
 
+    def trace_simple(self, rays, any_hit=False):
+        """One ray per thread through the device function rb_trace_one()
+        (include/rb_trace_device.h); same results as trace()."""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        n = len(rays)
+        dr = DeviceBuffer.from_array(rays)
+        dh = DeviceBuffer(max(n, 1) * 8)
+        check(lib().rb_trace_simple(self._h, dr.ptr, n, L.RB_ANY if any_hit else L.RB_CLOSEST, dh.ptr, None))
+        h = dh.download(HIT_DTYPE, n)
+        return h["id"].copy(), h["t"].copy()
+
+    def device_view(self):
+        v = L.rb_bvh_dev()
+        check(lib().rb_bvh_device_view(self._h, C.byref(v)))
+        return v
+
     def trace_counts(self, rays, any_hit=False):
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         n = len(rays)
@@ -480,8 +496,9 @@ class RenderServer:
 
     KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT}
 
-    def __init__(self, max_inflight=8, hbm_budget=0, max_queue=0):
-        cfg = L.rb_server_config(max_inflight, hbm_budget, max_queue)
+    def __init__(self, max_inflight=8, hbm_budget=0, max_queue=0, policy="deadline"):
+        pol = {"deadline": L.RB_SCHED_DEADLINE, "fifo": L.RB_SCHED_FIFO}[policy]
+        cfg = L.rb_server_config(max_inflight, hbm_budget, max_queue, pol)
         h = C.c_void_p()
         check(lib().rb_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h.value

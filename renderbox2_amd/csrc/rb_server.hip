@@ -5,15 +5,19 @@
 // staging) is free and the request's estimated device memory fits the
 // budget, launches the request's passes on the slot's stream, polls the
 // slot events for completion, copies results out, and records latency
-// (submit -> result ready).  Requests that do not fit wait without blocking
-// smaller ones behind them (bounded bypass); a request larger than the whole
-// budget is rejected at submit.  Results do not depend on what else runs:
-// every random number is keyed by (request seed, pixel, sample, bounce).
+// (submit -> result ready).  Admission order: earliest virtual deadline
+// first (deadline = submit time + 4 x the request's estimated GPU time, so
+// short requests overtake long ones but nothing waits forever), or plain
+// FIFO; requests that do not fit the budget wait without blocking the ones
+// behind them (bounded bypass); a request larger than the whole budget is
+// rejected at submit.  Results do not depend on what else runs: every
+// random number is keyed by (request seed, pixel, sample, bounce).
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <unordered_map>
@@ -30,6 +34,8 @@ struct Req {
     rb_request r{};
     rb_bvh_t bvh = nullptr;
     uint64_t bytes = 0;
+    double est_rays = 0;
+    Clock::time_point deadline;
     int state = RB_REQ_QUEUED;
     rb_status result = RB_OK;
     bool deferred = false;
@@ -62,13 +68,32 @@ uint64_t request_bytes(const rb_request& r) {
     return out + render_scratch_bytes(r.kind, r.w, r.h, &r.params) + 4096;
 }
 
+// Rays a request is expected to trace (scheduling estimate only), from the
+// per-sample ray counts of the passes at their default depths: AO 1 + n_ao,
+// path tracing ~7.5 at depth 8, BDPT ~(2 + connections) per camera vertex
+// plus the light pass.
+double estimate_rays(const rb_request& r) {
+    const double n = (double)r.w * r.h * (r.params.spp ? r.params.spp : 1);
+    const double depth = (double)r.params.max_depth + 1.0;
+    switch (r.kind) {
+        case RB_REQ_AO: return n * (1.0 + r.params.ao_samples);
+        case RB_REQ_PT: return n * 7.5 * depth / 9.0;
+        case RB_REQ_BDPT: {
+            const double D = r.params.light_max_depth ? r.params.light_max_depth : 1;
+            const double nl = r.params.light_paths ? r.params.light_paths : r.params.lvc_capacity / D;
+            return n * (2.0 + r.params.connections) * depth / 3.0 + nl * D;
+        }
+        default: return (double)r.n;
+    }
+}
+
 }  // namespace rb
 
 struct rb_server_s {
     rb_server_config cfg{};
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
-    std::deque<rb::Req*> queue;
+    std::multimap<rb::Clock::time_point, rb::Req*> queue;  // by admission key (deadline or submit time)
     std::unordered_map<uint64_t, std::unique_ptr<rb::Req>> reqs;
     std::map<uint32_t, rb::SceneEntry> scenes;
     uint32_t next_scene = 0;
@@ -202,7 +227,7 @@ void rb_server_s::loop() {
             std::unique_lock<std::mutex> lk(mu);
             if (stop && queue.empty() && !busy) break;
             size_t looked = 0;
-            for (auto it = queue.begin(); it != queue.end() && looked < 16;) {
+            for (auto it = queue.begin(); it != queue.end() && looked < 16;) {  // earliest key first
                 Slot* free_slot = nullptr;
                 for (Slot& sl : slots)
                     if (!sl.req) {
@@ -214,7 +239,7 @@ void rb_server_s::loop() {
                         }
                     }
                 if (!free_slot) break;
-                Req* q = *it;
+                Req* q = it->second;
                 if (used + q->bytes <= cfg.hbm_budget) {
                     used += q->bytes;
                     peak = std::max(peak, used);
@@ -259,6 +284,7 @@ rb_status rb_server_create(const rb_server_config* cfg, rb_server_t* out) {
     if (S->cfg.max_inflight == 0) S->cfg.max_inflight = 8;
     if (S->cfg.max_inflight > 32) return fail(RB_ERR_INVALID, "max_inflight must be <= 32");
     if (S->cfg.max_queue == 0) S->cfg.max_queue = 65536;
+    if (S->cfg.policy > RB_SCHED_FIFO) return fail(RB_ERR_INVALID, "unknown scheduling policy %u", S->cfg.policy);
     if (S->cfg.hbm_budget == 0) {
         size_t f = 0, t = 0;
         if (hipMemGetInfo(&f, &t) != hipSuccess) return fail(RB_ERR_DEVICE, "hipMemGetInfo failed");
@@ -283,7 +309,8 @@ rb_status rb_server_destroy(rb_server_t s) {
     if (!s) return RB_OK;
     {
         std::lock_guard<std::mutex> g(s->mu);
-        for (Req* q : s->queue) {
+        for (auto& kv : s->queue) {
+            Req* q = kv.second;
             q->state = RB_REQ_FAILED;
             q->result = RB_ERR_BUSY;
             auto it = s->scenes.find(q->r.scene);
@@ -355,13 +382,16 @@ rb_status rb_server_submit(rb_server_t s, const rb_request* req, uint64_t* id) {
     it->second.refs++;
     q->id = s->next_id++;
     q->t_submit = Clock::now();
+    q->est_rays = estimate_rays(*req);
+    // virtual deadline: 4x the estimated time at 1 Grays/s, plus 1 ms
+    q->deadline = q->t_submit + std::chrono::microseconds((int64_t)(1000.0 + 4.0 * q->est_rays / 1.0e3));
     if (!s->have_first) {
         s->have_first = true;
         s->first_submit = q->t_submit;
     }
     s->submitted++;
     if (id) *id = q->id;
-    s->queue.push_back(q.get());
+    s->queue.emplace(s->cfg.policy == RB_SCHED_FIFO ? q->t_submit : q->deadline, q.get());
     s->reqs[q->id] = std::move(q);
     s->cv_work.notify_one();
     return RB_OK;

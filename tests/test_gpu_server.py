@@ -143,3 +143,27 @@ def test_scene_in_use_cannot_be_removed(gpu, scenes):
         assert srv.poll(r).state == L.RB_REQ_DONE
     srv.remove_scene(sid)
     srv.close()
+
+
+@pytest.mark.parametrize("policy", ["deadline", "fifo"])
+def test_deadline_policy_lets_short_requests_overtake(gpu, scenes, policy):
+    import time
+    srv = gpu.RenderServer(max_inflight=1, policy=policy)
+    sid = srv.add_scene(scenes[0])
+    cam = gpu.scenes.pt_camera(256, 256)
+    t_sub, rids = [], []
+    for s in range(3):
+        t_sub.append(time.perf_counter())
+        rids.append(srv.submit_render("pt", sid, cam, None, spp=8, seed=s))
+    t_sub.append(time.perf_counter())
+    rids.append(srv.submit_render("ao", sid, cam, (0, 0, 32, 32), spp=1, seed=9))
+    done = []
+    for t0, r in zip(t_sub, rids):
+        st, _ = srv.wait(r)
+        assert st.result == L.RB_OK
+        done.append(t0 + st.latency_ms * 1e-3)
+    if policy == "deadline":
+        assert done[3] < done[1] and done[3] < done[2]  # admitted ahead of the queued long ones
+    else:
+        assert done[3] > done[2]  # arrival order
+    srv.close()

@@ -261,3 +261,24 @@ def test_batched_trace_equals_per_batch_calls(gpu):
             assert np.array_equal(h["id"] >= 0, e[0] >= 0)
         else:
             assert np.array_equal(h["id"], e[0]) and np.array_equal(h["t"].view(np.uint32), e[1].view(np.uint32))
+
+
+def test_device_function_trace_equals_persistent_kernel(gpu, small, room):
+    """rb_trace_one (the header-only device tracer) walks each ray in the
+    persistent kernel's order with the same arithmetic: identical (id, t)."""
+    v, idx, b, tr, c2 = small
+    rays = random_rays(20000, seed=77)
+    for any_hit in (False, True):
+        a = tr.trace(rays, any_hit=any_hit)
+        s = tr.trace_simple(rays, any_hit=any_hit)
+        assert np.array_equal(a[0], s[0]) and np.array_equal(a[1].view(np.uint32), s[1].view(np.uint32))
+    sc, bb, rt, _ = room
+    cam = gpu.scenes.ao_camera(128, 128)
+    d = gpu.DeviceBuffer(cam.get_total_pixels() * 32)
+    cam.generate_rays_device(d, seed=3)
+    prim = d.download(np.float32, cam.get_total_pixels() * 8).reshape(-1, 8)
+    a = rt.trace(prim)
+    s = rt.trace_simple(prim)
+    assert np.array_equal(a[0], s[0]) and np.array_equal(a[1].view(np.uint32), s[1].view(np.uint32))
+    view = rt.device_view()
+    assert view.nodes and view.tris and view.index and view.n_refs >= view.n_nodes // 2
