@@ -99,6 +99,55 @@ def bench_render(rb, kind, scene, bvh_max_leaf, cam, frames, built=None, **param
             "tris": scene.get_num_triangles(), "bvh_build_s": round(build_s, 1), "mean": round(float(img.mean()), 5)}
 
 
+def bench_graphs(rb, built, iters=50):
+    """Progressive rendering of one tile, `iters` iterations with a new seed
+    each: eager (every kernel launched from the host per iteration) vs the
+    pass captured once into a HIP graph and replayed (rb_render_graph_*).
+    Small tiles are where launch overhead shows."""
+    import numpy as np
+    tracer = built[0]
+    cam = rb.scenes.pt_camera(1024, 1024)
+    stream = rb.Stream()
+    out = {}
+    cases = [("ao_64x64", "ao", (480, 480, 64, 64), dict(spp=1, ao_samples=8)),
+             ("pt_128x128", "pt", (448, 448, 128, 128), dict(spp=1, max_depth=8)),
+             ("bdpt_128x128", "bdpt", (448, 448, 128, 128),
+              dict(spp=1, max_depth=6, light_max_depth=6, light_paths=16384, connections=5))]
+    for name, kind, tile, p in cases:
+        d = rb.DeviceBuffer(tile[2] * tile[3] * 12)
+        r = rb.SamplerRenderer(tracer, cam, kind, **p)
+        g = rb.RenderGraph(tracer, cam, kind, d, tile, **p)
+
+        def eager():
+            for i in range(iters):
+                r.params.seed = i
+                r.render_device(d, tile, stream.handle)
+
+        def graph():
+            for i in range(iters):
+                g.launch(i, stream.handle)
+
+        res = {}
+        for tag, fn in (("eager", eager), ("graph", graph)):
+            fn()
+            stream.synchronize()
+            e0, e1 = rb.Event(), rb.Event()
+            e0.record(stream)
+            fn()
+            e1.record(stream)
+            e1.synchronize()
+            res[tag] = e0.elapsed_ms(e1) / iters
+        img_g = d.download(np.float32)
+        r.params.seed = iters - 1
+        r.render_device(d, tile, stream.handle)
+        stream.synchronize()
+        same = bool(np.array_equal(img_g.view(np.uint32), d.download(np.float32).view(np.uint32)))
+        g.close()
+        out[name] = {"eager_ms": round(res["eager"], 4), "graph_ms": round(res["graph"], 4),
+                     "speedup": round(res["eager"] / res["graph"], 3), "match": same}
+    return out
+
+
 BATCH_SCENES = [10_000, 50_000, 200_000, 500_000, 1_000_000, 2_000_000, 3_000_000, 4_000_000]
 
 
@@ -459,6 +508,9 @@ def main():
                           lvc_capacity=1 << 20, light_paths=0, connections=5)
         log("bdpt frame (config 5 BDPT: 2,000,136 tris, 512x512, 4 spp, 2^20-vertex cache, depth 6)", bd)
         extra["bdpt_512_4spp"] = bd
+        gr = bench_graphs(rb, built)
+        log("progressive tiles, eager vs hipGraph replay (2,000,136 tris)", gr)
+        extra["graph_replay"] = gr
         del built
     if args.served:
         sv = bench_served(rb, args.served, max_inflight=args.served_inflight, policy=args.served_policy)

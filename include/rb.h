@@ -369,6 +369,23 @@ rb_status rb_render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t
 rb_status rb_render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                          const rb_render_params* p, float* out_rgb, void* stream);
 
+/* A render pass captured into a HIP graph (no reference counterpart; the
+ * reference launches every kernel of every iteration from the host,
+ * src/core/application.cpp:225-239).  kind: RB_REQ_AO, RB_REQ_PT or
+ * RB_REQ_BDPT (the serving kinds below).  The graph records the whole pass --
+ * camera rays, every bounce up to max_depth (kernels skip work past the
+ * live-path count held in device memory), film -- into out_rgb, and owns its
+ * scratch.  rb_render_graph_launch(g, seed, stream) replays it for `seed`:
+ * the image equals rb_render_* with that seed, bit for bit.  Replays of one
+ * graph must not overlap (one stream, or synchronised); ray_counter must be
+ * NULL.  Destroy waits for the device. */
+typedef struct rb_render_graph_s* rb_render_graph_t;
+rb_status rb_render_graph_create(uint32_t kind, rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0,
+                                 uint32_t w, uint32_t h, const rb_render_params* p, float* out_rgb,
+                                 rb_render_graph_t* graph);
+rb_status rb_render_graph_launch(rb_render_graph_t graph, uint64_t seed, void* stream);
+rb_status rb_render_graph_destroy(rb_render_graph_t graph);
+
 /* ---- serving ------------------------------------------------------------ */
 
 /* Continuous-batching request server (no reference counterpart: the

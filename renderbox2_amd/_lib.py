@@ -127,6 +127,9 @@ _SIGS = {
     "rb_render_ao": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_pt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_bdpt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
+    "rb_render_graph_create": (C.c_int, [U32, P, P, U32, U32, U32, U32, P, P, P]),
+    "rb_render_graph_launch": (C.c_int, [P, U64, P]),
+    "rb_render_graph_destroy": (C.c_int, [P]),
     "rb_server_create": (C.c_int, [P, P]),
     "rb_server_destroy": (C.c_int, [P]),
     "rb_server_add_scene": (C.c_int, [P, P, P]),

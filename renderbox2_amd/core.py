@@ -475,6 +475,38 @@ class SamplerRenderer:
         return d.download(np.float32, w * h * 3).reshape(h, w, 3)
 
 
+class RenderGraph:
+    """A render pass captured into a HIP graph over a fixed tile and output
+    buffer (rb_render_graph_*): launch(seed) replays the whole pass --
+    raygen, every bounce, film -- for a new seed with one graph launch."""
+
+    KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT}
+
+    def __init__(self, tracer, camera, integrator, d_out, tile=None, **params):
+        x0, y0, w, h = tile if tile else (0, 0, camera.width, camera.height)
+        self.tile = (x0, y0, w, h)
+        self.params = render_params(**params)
+        self._keep = (tracer, camera, d_out)
+        g = C.c_void_p()
+        check(lib().rb_render_graph_create(self.KINDS[integrator], tracer.handle, C.byref(camera.c), x0, y0, w, h,
+                                           C.byref(self.params), _dp(d_out), C.byref(g)))
+        self.handle = g.value
+
+    def launch(self, seed, stream=None):
+        check(lib().rb_render_graph_launch(self.handle, seed, _sp(stream)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().rb_render_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _sync(stream):
     check(lib().rb_stream_synchronize(_sp(stream)))
 

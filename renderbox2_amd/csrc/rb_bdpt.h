@@ -1,4 +1,4 @@
-// rb_bdpt.hip -- bidirectional path tracing with a light-vertex cache (LVC).
+// rb_bdpt.h -- bidirectional path tracing with a light-vertex cache (LVC).
 //
 // Restates the reference's BDPT-LVC (src/integrators/integrator_bdpt_lvc_sk.cu:47-435,
 // wavefront form integrator_bdpt_lvc_mk.cu) as wavefront passes:
@@ -96,7 +96,8 @@ RB_HD float technique_weight(uint32_t mask, uint32_t n, uint32_t max_depth, uint
     return c ? 1.0f / (float)c : 0.0f;
 }
 
-__global__ void k_light_emit(uint32_t npaths, SceneDev S, Rng rng, LightPath P, LvcBufs C) {
+__global__ void k_light_emit(uint32_t npaths, SceneDev S, RngArg rng_a, LightPath P, LvcBufs C) {
+    const Rng rng = rng_a.get();
     const uint32_t end = (npaths + 63u) & ~63u;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
         const bool valid = j < npaths;
@@ -130,7 +131,8 @@ __global__ void k_light_emit(uint32_t npaths, SceneDev S, Rng rng, LightPath P, 
     }
 }
 
-__global__ void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, Rng rng, LightPath P, LvcBufs C) {
+__global__ void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, RngArg rng_a, LightPath P, LvcBufs C) {
+    const Rng rng = rng_a.get();
     const uint32_t count = P.cnt[qi];
     const uint32_t end = (count + 63u) & ~63u;
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += gridDim.x * blockDim.x) {

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "rb_internal.h"
+#include "rb_rng.h"
 
 namespace rb {
 
@@ -27,7 +28,27 @@ struct DeviceState {
 };
 DeviceState& dev();
 rb_status ensure_device();
+// A launch's work-counter pair: from the ring, or from the pool of the render
+// graph being captured on this thread (nullptr when that pool is used up).
 uint32_t* take_counter_slot();
+
+// Set on the capturing thread while a render pass is recorded into a graph
+// (rb_render_graph_create): kernels read their stream keys from `keys`
+// (indexed by tag), trace launches take counter pairs from the graph's own
+// pool, and scratch comes from persistent allocations the graph frees.
+struct CaptureCtx {
+    const Rng* keys = nullptr;
+    uint32_t* counters = nullptr;
+    uint32_t n_pairs = 0, next_pair = 0;
+    std::vector<void*> allocs;
+};
+extern thread_local CaptureCtx* t_capture;
+inline RngArg rng_arg(uint64_t seed, uint32_t tag) {
+    RngArg a;
+    a.r = Rng::make(seed, tag);
+    a.dev = t_capture && t_capture->keys ? t_capture->keys + tag : nullptr;
+    return a;
+}
 rb_status hip_fail(hipError_t e, const char* what);
 
 // Persistent traversal launch (rb_trace.hip).  n_dev, when set, holds the ray

@@ -8,6 +8,7 @@ namespace rb {
 
 static DeviceState g_dev;
 static std::mutex g_init_mu;
+thread_local CaptureCtx* t_capture = nullptr;
 
 DeviceState& dev() { return g_dev; }
 
@@ -30,20 +31,21 @@ static rb_status init_locked(int device) {
     if ((e = hipMalloc(&c, sizeof(uint32_t) * 2 * kCounterSlots)) != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
     if ((e = hipMemset(c, 0, sizeof(uint32_t) * 2 * kCounterSlots)) != hipSuccess) return hip_fail(e, "hipMemset(counters)");
     g_dev.counters = (uint32_t*)c;
+    // the autotuner's pick on 2 Mi incoherent rays at 1M triangles
     g_dev.cfg.block = 128;
-    g_dev.cfg.lds_stack = 16;
+    g_dev.cfg.lds_stack = 24;
     g_dev.cfg.chunk = 64;
-    g_dev.cfg.fetch_threshold = 24;
-    g_dev.cfg.waves_per_cu = 16;
+    g_dev.cfg.fetch_threshold = 48;
+    g_dev.cfg.waves_per_cu = 32;
     g_dev.cfg.speculative = 0;
-    g_dev.cfg.leaf_gate = 0;
+    g_dev.cfg.leaf_gate = 16;
     g_dev.device = device;
     return RB_OK;
 }
 
 rb_status ensure_device() {
     if (g_dev.device >= 0) {
-        hipSetDevice(g_dev.device);
+        if (!t_capture) (void)hipSetDevice(g_dev.device);  // no device calls inside a graph capture
         return RB_OK;
     }
     std::lock_guard<std::mutex> g(g_init_mu);
@@ -52,6 +54,10 @@ rb_status ensure_device() {
 }
 
 uint32_t* take_counter_slot() {
+    if (t_capture) {
+        if (t_capture->next_pair >= t_capture->n_pairs) return nullptr;
+        return t_capture->counters + 2 * t_capture->next_pair++;
+    }
     uint32_t s = g_dev.next_slot.fetch_add(1) % kCounterSlots;
     return g_dev.counters + 2 * s;
 }

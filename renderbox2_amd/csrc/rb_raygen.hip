@@ -41,7 +41,8 @@ __device__ __forceinline__ void camera_ray(const CamArgs& c, float px, float py,
 }
 
 __global__ void k_raygen_camera(CamArgs c, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                                uint32_t spp, Rng rng, int jitter, float4* __restrict__ rays) {
+                                uint32_t spp, RngArg rng_a, int jitter, float4* __restrict__ rays) {
+    const Rng rng = rng_a.get();
     const uint64_t total = (uint64_t)w * h * spp;
     for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < total;
          i += (uint64_t)gridDim.x * blockDim.x) {
@@ -153,7 +154,7 @@ rb_status rb_raygen_camera(const rb_camera* cam, uint32_t x0, uint32_t y0, uint3
     const uint64_t total = (uint64_t)w * h * spp;
     hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
     hipLaunchKernelGGL(k_raygen_camera, dim3(grid_for(total, 256)), dim3(256), 0, s, c, cam->width, x0, y0, w, h, spp,
-                       Rng::make(seed, RNG_CAMERA), jitter, reinterpret_cast<float4*>(rays));
+                       rng_arg(seed, RNG_CAMERA), jitter, reinterpret_cast<float4*>(rays));
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? RB_OK : hip_fail(e, "raygen launch");
 }

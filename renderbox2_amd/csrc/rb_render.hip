@@ -32,8 +32,9 @@ __device__ __forceinline__ uint32_t sample_gid(const TileArgs& T, uint32_t i) {
 // ---------------------------------------------------------------- AO ----
 
 __global__ void k_ao_spawn(const float4* __restrict__ prim, const int2* __restrict__ hits, uint32_t n, TileArgs T,
-                           SceneDev S, uint32_t nao, float radius, Rng rng, float4* __restrict__ rays,
+                           SceneDev S, uint32_t nao, float radius, RngArg rng_a, float4* __restrict__ rays,
                            uint8_t* __restrict__ mask, unsigned long long* ray_counter) {
+    const Rng rng = rng_a.get();
     const uint64_t total = (uint64_t)n * nao;
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < total; j += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t i = (uint32_t)(j / nao), k = (uint32_t)(j % nao);
@@ -118,7 +119,7 @@ struct BdArgs {
     LvcBufs C;
     uint32_t nconn;
     uint32_t tdepth;  // cache depth the technique count may use: 0 when no connections are made
-    Rng crng;
+    RngArg crng;
 };
 
 __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) { return wave_append2(counter, want); }
@@ -163,8 +164,10 @@ __global__ void k_count_rays(const uint32_t* c, unsigned long long* total, unsig
 // One bounce of shading for the queued camera paths.  BD: also the
 // light-vertex-cache connections and the technique weights.
 template <bool BD>
-__global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng rng, uint32_t max_depth,
+__global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngArg rng_a, uint32_t max_depth,
                            uint32_t rr_start, float rr_prob, BdArgs B) {
+    const Rng rng = rng_a.get();
+    const Rng crng = BD ? B.crng.get() : rng;
     const uint32_t count = P.cnt[qi];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t end = (count + 63u) & ~63u;  // whole waves iterate together (wave_append)
@@ -271,7 +274,7 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, Rng 
                 if (BD && lambert) {
                     // connections to the light-vertex cache (integrator_bdpt_lvc_sk.cu:343-386)
                     for (uint32_t i = 0; i < B.nconn; i++) {
-                        const U4 rc = B.crng.draw(gid, bounce, 2 + i / 2, 0);
+                        const U4 rc = crng.draw(gid, bounce, 2 + i / 2, 0);
                         const uint32_t ua = (i & 1) ? rc.z : rc.x, ub = (i & 1) ? rc.w : rc.y;
                         float4 contrib = make_float4(0.f, 0.f, 0.f, 0.f);
                         uint32_t j = (uint32_t)(u01(ua) * (float)B.C.paths);
@@ -468,7 +471,8 @@ static uint32_t grid_of(uint64_t n) {
     return (uint32_t)(g ? g : 1);
 }
 
-// Stream-ordered scratch allocations released at the end of a render call.
+// Stream-ordered scratch allocations released at the end of a render call;
+// while a render graph is captured, persistent allocations the graph owns.
 struct Scratch {
     hipStream_t s;
     std::vector<void*> ptrs;
@@ -478,6 +482,16 @@ struct Scratch {
     T* get(size_t count) {
         void* p = nullptr;
         if (st != RB_OK) return nullptr;
+        if (t_capture) {
+            hipError_t e = hipMalloc(&p, count * sizeof(T) + 16);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                st = fail(RB_ERR_OOM, "render graph scratch (%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
+                return nullptr;
+            }
+            t_capture->allocs.push_back(p);
+            return (T*)p;
+        }
         hipError_t e = hipMallocAsync(&p, count * sizeof(T) + 16, s);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -532,7 +546,7 @@ rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
         return st;
     TileArgs T{cam->width, x0, y0, w, p->spp};
     hipLaunchKernelGGL(k_ao_spawn, dim3(grid_of(na)), dim3(256), 0, s, rays, hits, n, T, scene_of(bvh), p->ao_samples,
-                       p->ao_radius, Rng::make(p->seed, RNG_AO), arays, amask, p->ray_counter);
+                       p->ao_radius, rng_arg(p->seed, RNG_AO), arays, amask, p->ray_counter);
     if (p->ray_counter) hipLaunchKernelGGL(k_add64, dim3(1), dim3(64), 0, s, p->ray_counter, (unsigned long long)n);
     if ((st = launch_trace(bvh, (rb_ray*)arays, (uint32_t)na, RB_ANY | RB_MASKED, amask, (rb_hit*)ahits, nullptr, s,
                            dev().cfg)) != RB_OK)
@@ -583,7 +597,7 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     hipLaunchKernelGGL(k_pt_init, dim3(grid_of(n)), dim3(256), 0, s, phits, prim, n, P);
     TileArgs T{cam->width, x0, y0, w, p->spp};
     const SceneDev S = scene_of(bvh);
-    const Rng rng = Rng::make(p->seed, RNG_PATH);
+    const RngArg rng = rng_arg(p->seed, RNG_PATH);
     for (uint32_t b = 0, qi = 0; b <= p->max_depth; b++, qi ^= 1) {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
         if (bd)
@@ -650,7 +664,7 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
     BdArgs B;
     B.nconn = bvh->n_lights ? p->connections : 0;
     B.tdepth = B.nconn ? D : 0;
-    B.crng = Rng::make(p->seed, RNG_CONNECT);
+    B.crng = rng_arg(p->seed, RNG_CONNECT);
     B.C.depth = D;
     B.C.paths = NL;
     B.C.pos = M.get<float4>((size_t)NL * D);
@@ -669,7 +683,7 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
     float4* lrays = M.get<float4>(2 * (size_t)NL);
     if (M.st != RB_OK) return M.st;
     const SceneDev S = scene_of(bvh);
-    const Rng lrng = Rng::make(p->seed, RNG_LIGHT);
+    const RngArg lrng = rng_arg(p->seed, RNG_LIGHT);
     // light pass (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168)
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
     hipLaunchKernelGGL(k_light_emit, dim3(grid_of(NL)), dim3(256), 0, s, NL, S, lrng, LP, B.C);

@@ -62,6 +62,17 @@ struct Rng {
     }
 };
 
+#if defined(__HIPCC__)
+// A kernel's stream key: known on the host, or -- inside a captured render
+// graph, whose seed changes from launch to launch -- read from a device
+// table of keys indexed by stream tag.
+struct RngArg {
+    Rng r;
+    const Rng* dev;
+    __device__ __forceinline__ Rng get() const { return dev ? *dev : r; }
+};
+#endif
+
 // Stream tags.
 enum : uint32_t {
     RNG_CAMERA = 1,

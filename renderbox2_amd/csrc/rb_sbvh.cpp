@@ -231,9 +231,13 @@ struct Builder {
         return s;
     }
 
+    // used[d][i]: bin i of axis d was grown at least once.  Box::grow inserts
+    // the other box's corners, so growing with an untouched (empty) bin would
+    // make the result infinite: a parallel merge must skip those.
     struct Bins {
         Box b[3][kBins];
         uint32_t enter[3][kBins] = {}, exit[3][kBins] = {};
+        bool used[3][kBins] = {};
     };
 
     void chop(const std::vector<Ref>& refs, size_t lo, size_t hi, F3 origin, F3 binSize, F3 inv, Bins& B) {
@@ -252,9 +256,11 @@ struct Builder {
                     Ref L, R;
                     split_reference(C, L, R, cur, d, get(origin, d) + get(binSize, d) * (float)(i + 1));
                     B.b[d][i].grow(L.b);
+                    B.used[d][i] = true;
                     cur = R;
                 }
                 B.b[d][l].grow(cur.b);
+                B.used[d][l] = true;
                 B.enter[d][f]++;
                 B.exit[d][l]++;
             }
@@ -262,7 +268,8 @@ struct Builder {
     }
 
     // find_spatial_split, splitbvhbuilder.cpp:420-497.  Bin bounds merge by
-    // min/max and counts by sums, so chopping in parallel is exact.
+    // min/max over the bins each range touched and counts by sums, so chopping
+    // in parallel gives the serial bins exactly.
     SpSplit spatial_split(const std::vector<Ref>& refs, const Box& nb, float nodeSAH, int threads) {
         F3 origin = nb.lo;
         F3 binSize = (nb.hi - origin) * (1.0f / (float)kBins);
@@ -281,7 +288,7 @@ struct Builder {
             for (int t = 0; t < threads; t++)
                 for (int d = 0; d < 3; d++)
                     for (int i = 0; i < kBins; i++) {
-                        B->b[d][i].grow(part[t]->b[d][i]);
+                        if (part[t]->used[d][i]) B->b[d][i].grow(part[t]->b[d][i]);
                         B->enter[d][i] += part[t]->enter[d][i];
                         B->exit[d][i] += part[t]->exit[d][i];
                     }

@@ -354,6 +354,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.hits = reinterpret_cast<int2*>(hits);
     a.counts = counts;
     a.counter = take_counter_slot();
+    if (!a.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
     a.n = n;
     a.n_dev = n_dev;
     a.chunk = cfg.chunk;
@@ -392,6 +393,7 @@ rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t st
         TraceArgs a;
         std::memset(&a, 0, sizeof(a));
         a.counter = take_counter_slot();
+        if (!a.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
         a.n = t.total;
         a.chunk = cfg.chunk;
         a.threshold = cfg.fetch_threshold;

@@ -117,3 +117,39 @@ def test_bdpt_rejects_bad_arguments(gpu, pt_setup):
                 dict(max_depth=24, light_max_depth=8), dict(light_paths=0, lvc_capacity=0)]:
         with pytest.raises(gpu.RbError):
             gpu.SamplerRenderer(tr, cam, "bdpt", **dict(BD, **bad)).render()
+
+
+@pytest.mark.parametrize("kind,params", [
+    ("ao", dict(spp=2, ao_samples=4)),
+    ("pt", dict(spp=2, max_depth=6)),
+    ("bdpt", dict(spp=1, max_depth=5, light_paths=2048, light_max_depth=3, connections=2)),
+])
+def test_render_graph_replays_equal_eager_renders(gpu, pt_setup, kind, params):
+    """The captured pass replayed for several seeds, with eager renders and
+    synchronisations in between: every replay equals the eager image."""
+    s, b, tr = pt_setup
+    cam = gpu.scenes.pt_camera(64, 48)
+    tile = (8, 4, 40, 36)
+    d = gpu.DeviceBuffer(40 * 36 * 12)
+    g = gpu.RenderGraph(tr, cam, kind, d, tile, seed=0, **params)
+    try:
+        for seed in (3, 11, 3, 0):
+            g.launch(seed)
+            L.check(L.lib().rb_stream_synchronize(None))
+            got = d.download(np.float32).reshape(36, 40, 3)
+            ref = gpu.SamplerRenderer(tr, cam, kind, seed=seed, **params).render(tile)
+            assert np.array_equal(got.view(np.uint32), ref.view(np.uint32)), (kind, seed)
+    finally:
+        g.close()
+
+
+def test_render_graph_rejects_bad_arguments(gpu, pt_setup):
+    import ctypes as C
+    s, b, tr = pt_setup
+    cam = gpu.scenes.pt_camera(16, 16)
+    d = gpu.DeviceBuffer(16 * 16 * 12)
+    with pytest.raises(gpu.RbError):
+        gpu.RenderGraph(tr, cam, "pt", d, None, spp=0)
+    c = gpu.DeviceBuffer(8)
+    with pytest.raises(gpu.RbError):
+        gpu.RenderGraph(tr, cam, "pt", d, None, ray_counter=C.c_void_p(c.ptr))

@@ -80,6 +80,19 @@ def test_builder_is_thread_count_independent(rbm):
             assert np.array_equal(a, b)
 
 
+def test_parallel_sort_and_binning_match_the_serial_oracle(rbm, orc):
+    """Nodes of >= 65,536 references sort and bin on several threads; the
+    tree must still be the single-threaded reference's, byte for byte.
+    (An empty per-thread spatial bin once leaked into the merged bins and
+    changed split decisions at this size.)"""
+    s = rbm.scenes.trace_scaled(80_000, seed=3)
+    v, idx, _, _ = s.geometry()
+    on, ow, ox = orc.Bvh(v, idx).compact2()
+    for t in (2, 8):
+        _, n, w, x = _product_compact2(rbm, v, idx, threads=t)
+        assert np.array_equal(n, on) and np.array_equal(w, ow) and np.array_equal(x, ox), t
+
+
 def test_builder_degenerates_and_small_scenes(rbm, orc):
     # one triangle: root leaf, wrapped by the exporters
     v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
