@@ -396,13 +396,23 @@ def main():
     from renderbox2_amd import _lib as L
 
     rb.device_init(0)
-    t0 = time.time()
+    import shutil
+    import tempfile
     scene = rb.scenes.trace_1m(seed=0)
-    bvh = rb.scenes.build(scene)
+    cache_dir = tempfile.mkdtemp(prefix="rbvh_")
+    try:
+        t0 = time.time()
+        bvh, _ = rb.BVH.build_cached(scene, cache_dir)  # miss: build + write the binary cache
+        build_s = time.time() - t0
+        t0 = time.time()
+        bvh, hit = rb.BVH.build_cached(scene, cache_dir)  # hit: load and check the key
+        load_s = time.time() - t0
+        cache_mb = sum(os.path.getsize(os.path.join(cache_dir, f)) for f in os.listdir(cache_dir)) / 1e6
+    finally:
+        shutil.rmtree(cache_dir, ignore_errors=True)
     st = bvh.stats()
-    build_s = time.time() - t0
     log(f"scene {scene.get_num_triangles()} tris, SBVH {st['n_inner']} inner / {st['n_tri_refs']} refs "
-        f"in {build_s:.1f}s")
+        f"in {build_s:.1f}s; binary cache {cache_mb:.1f} MB loads in {load_s:.3f}s (hit={hit})")
     tracer = rb.CudaTracer(bvh, scene)
     cam = rb.scenes.trace_camera(2048, 1024)
     n = cam.get_total_pixels()
@@ -536,7 +546,8 @@ def main():
                                "20,000 thin quads), 2,097,152 diffuse rays from 2048x1024 primary hits",
                    "tris": scene.get_num_triangles(), "rays": n, "hit_fraction": round(hit_frac, 4),
                    "bvh_inner_nodes": st["n_inner"], "bvh_tri_refs": st["n_tri_refs"],
-                   "bvh_build_s": round(build_s, 1), "trace_config": cfg},
+                   "bvh_build_s": round(build_s, 1), "bvh_cache_load_s": round(load_s, 3),
+                   "bvh_cache_mb": round(cache_mb, 1), "trace_config": cfg},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1), "gpu_visit_bytes_per_ray": round(gpu_bpr, 1)},

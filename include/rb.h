@@ -199,6 +199,20 @@ rb_status rb_bvh_load_text_compact2(const char* path, void* nodes, size_t* nodes
                                     void* tris, size_t* tris_bytes, void* index,
                                     size_t* index_bytes);
 
+/* Binary BVH cache (the reference's cache is the decimal text form above,
+ * used or rebuilt by a config flag with nothing tying it to the scene,
+ * src/core/application.cpp:131-161).  The file holds the host tree (every
+ * layout exports from it) and a key hashed from the scene's vertices and
+ * triangles and the tree-shaping build parameters (not `threads`): loading
+ * it for other geometry or parameters fails with RB_ERR_INVALID, as does a
+ * corrupt or truncated file (checksum).  rb_bvh_build_cached loads
+ * <cache_dir>/rbvh_<key>.bin if present and valid, else builds and writes it
+ * (atomically: temporary file + rename); *from_cache tells which. */
+rb_status rb_bvh_save_binary(rb_host_bvh_t b, const char* path);
+rb_status rb_bvh_load_binary(const char* path, rb_scene_t scene, const rb_build_params* p, rb_host_bvh_t* out);
+rb_status rb_bvh_build_cached(rb_scene_t scene, const rb_build_params* p, const char* cache_dir, rb_host_bvh_t* out,
+                              int* from_cache);
+
 /* CudaTracer(const CudaBvh&, MemoryAllocator&) (src/accelerators/sbvh/cudatracer.cu:43-127)
  * plus Scene::gpu_prepare (src/core/scene.cpp:166-301): uploads the MI355X
  * layout and the scene's shading data (vertices, materials, lights).

@@ -107,6 +107,9 @@ _SIGS = {
     "rb_bvh_export": (C.c_int, [P, U32, P, P, P, P, P, P]),
     "rb_bvh_save_text": (C.c_int, [P, C.c_char_p]),
     "rb_bvh_load_text_compact2": (C.c_int, [C.c_char_p, P, P, P, P, P, P]),
+    "rb_bvh_save_binary": (C.c_int, [P, C.c_char_p]),
+    "rb_bvh_load_binary": (C.c_int, [C.c_char_p, P, P, P]),
+    "rb_bvh_build_cached": (C.c_int, [P, P, C.c_char_p, P, P]),
     "rb_bvh_upload": (C.c_int, [P, P, P]),
     "rb_bvh_release": (C.c_int, [P]),
     "rb_bvh_device_bytes": (C.c_int, [P, P]),

@@ -216,11 +216,43 @@ class BuildParams:
 class BVH:
     """SBVH built on the host (BVH::BVH, src/accelerators/sbvh/splitbvhbuilder.cpp:177-214)."""
 
-    def __init__(self, scene_or_arrays, platform=None, params=None):
+    @staticmethod
+    def _params(platform, params):
         platform = platform or Platform()
         params = params or BuildParams()
-        bp = L.rb_build_params(platform.min_leaf, platform.max_leaf, params.split_alpha,
-                               platform.node_cost, platform.tri_cost, params.threads)
+        return L.rb_build_params(platform.min_leaf, platform.max_leaf, params.split_alpha,
+                                 platform.node_cost, platform.tri_cost, params.threads)
+
+    @classmethod
+    def _wrap(cls, handle):
+        b = cls.__new__(cls)
+        b._h = handle
+        return b
+
+    @classmethod
+    def load_binary(cls, path, scene, platform=None, params=None):
+        """The tree saved by save_binary, checked against this scene and these
+        build parameters (rb_bvh_load_binary)."""
+        bp = cls._params(platform, params)
+        h = C.c_void_p()
+        check(lib().rb_bvh_load_binary(str(path).encode(), scene.handle, C.byref(bp), C.byref(h)))
+        return cls._wrap(h.value)
+
+    @classmethod
+    def build_cached(cls, scene, cache_dir, platform=None, params=None):
+        """(bvh, from_cache): load <cache_dir>/rbvh_<key>.bin or build and write it."""
+        bp = cls._params(platform, params)
+        h = C.c_void_p()
+        hit = C.c_int()
+        check(lib().rb_bvh_build_cached(scene.handle, C.byref(bp), str(cache_dir).encode(), C.byref(h),
+                                        C.byref(hit)))
+        return cls._wrap(h.value), bool(hit.value)
+
+    def save_binary(self, path):
+        check(lib().rb_bvh_save_binary(self._h, str(path).encode()))
+
+    def __init__(self, scene_or_arrays, platform=None, params=None):
+        bp = self._params(platform, params)
         h = C.c_void_p()
         if isinstance(scene_or_arrays, Scene):
             check(lib().rb_bvh_build_host(scene_or_arrays.handle, C.byref(bp), C.byref(h)))

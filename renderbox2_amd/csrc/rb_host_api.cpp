@@ -452,6 +452,57 @@ rb_status rb_bvh_export(rb_host_bvh_t b, uint32_t layout, void* nodes, size_t* n
     return fail(RB_ERR_INVALID, "unknown layout %u", layout);
 }
 
+rb_status rb_bvh_save_binary(rb_host_bvh_t b, const char* path) {
+    RB_CHECK_ARG(b && path, "NULL argument");
+    return save_bvh_binary(*b->b, path);
+}
+
+rb_status rb_bvh_load_binary(const char* path, rb_scene_t s, const rb_build_params* p, rb_host_bvh_t* out) {
+    RB_CHECK_ARG(path && s && out, "NULL argument");
+    rb_build_params d;
+    rb_build_params_default(&d);
+    HostBvh* h = nullptr;
+    rb_status st;
+    try {
+        st = load_bvh_binary(path, s->s.verts.data(), s->s.num_verts(), s->s.idx.data(), s->s.num_tris(), p ? *p : d, &h);
+    } catch (...) {
+        return fail(RB_ERR_OOM, "BVH load failed (host allocation)");
+    }
+    if (st != RB_OK) return st;
+    *out = new rb_host_bvh_s{h};
+    return RB_OK;
+}
+
+rb_status rb_bvh_build_cached(rb_scene_t s, const rb_build_params* p, const char* cache_dir, rb_host_bvh_t* out,
+                              int* from_cache) {
+    RB_CHECK_ARG(s && cache_dir && out, "NULL argument");
+    RB_CHECK_ARG(s->s.num_tris() > 0, "empty geometry");
+    rb_build_params d;
+    rb_build_params_default(&d);
+    if (!p) p = &d;
+    char name[64];
+    std::snprintf(name, sizeof name, "/rbvh_%016llx.bin",
+                  (unsigned long long)bvh_cache_key(s->s.verts.data(), s->s.num_verts(), s->s.idx.data(),
+                                                     s->s.num_tris(), *p));
+    const std::string path = std::string(cache_dir) + name;
+    if (from_cache) *from_cache = 0;
+    {
+        std::ifstream probe(path, std::ios::binary);
+        if (probe && rb_bvh_load_binary(path.c_str(), s, p, out) == RB_OK) {
+            if (from_cache) *from_cache = 1;
+            return RB_OK;
+        }
+    }
+    rb_status st = rb_bvh_build_host(s, p, out);
+    if (st != RB_OK) return st;
+    if ((st = save_bvh_binary(*(*out)->b, path.c_str())) != RB_OK) {
+        rb_bvh_host_release(*out);
+        *out = nullptr;
+        return st;
+    }
+    return RB_OK;
+}
+
 // CudaBvh::serialize (cudabvh.cpp:176-214): magic, three sizes, layout
 // (1 = BVHLayout_Compact2), decimal uint4 rows, decimal indices, EOF marker.
 rb_status rb_bvh_save_text(rb_host_bvh_t b, const char* path) {

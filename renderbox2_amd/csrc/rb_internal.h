@@ -89,6 +89,12 @@ struct HostBvh {
 
 HostBvh* build_sbvh(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
                     const rb_build_params& p, rb_status* st);
+// Binary tree cache (rb_bvh_cache.cpp): key = hash of geometry + the build
+// parameters that shape the tree.
+uint64_t bvh_cache_key(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt, const rb_build_params& p);
+rb_status save_bvh_binary(const HostBvh& b, const char* path);
+rb_status load_bvh_binary(const char* path, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                          const rb_build_params& p, HostBvh** out);
 void woopify(const float* v0, const float* v1, const float* v2, float out[12]);
 
 // MI355X layout arrays (host copies).

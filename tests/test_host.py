@@ -296,3 +296,53 @@ def test_camera_geometry_closed_form(orc):
     assert abs(np.degrees(np.arctan2(top[1], top[2])) - np.degrees(np.arctan(1 - 1.0 / H))) < 0.05
     # image y grows downward, x mirrors so that +x in raster is to the camera's right
     assert d[0, W // 2, 1] > 0 > d[H - 1, W // 2, 1]
+
+
+# ------------------------------------------------------- binary BVH cache ---
+
+
+def test_binary_bvh_cache_round_trip_and_key_checks(rbm, tmp_path):
+    from renderbox2_amd import _lib as L
+    s = rbm.scenes.pt_scaled(30_000, seed=2)
+    b = rbm.BVH(s)
+    path = tmp_path / "room.rbvh"
+    b.save_binary(path)
+    c = rbm.BVH.load_binary(path, s)
+    for layout in (L.RB_LAYOUT_REF_COMPACT2, L.RB_LAYOUT_MI355X):
+        for x, y in zip(b.export(layout), c.export(layout)):
+            assert np.array_equal(x, y)
+    assert b.stats() == c.stats()
+    # other build parameters: refused
+    p = rbm.Platform()
+    p.set_leaf_preferences(1, 4)
+    with pytest.raises(rbm.RbError):
+        rbm.BVH.load_binary(path, s, p)
+    # other geometry: refused
+    s2 = rbm.scenes.pt_scaled(30_000, seed=3)
+    with pytest.raises(rbm.RbError):
+        rbm.BVH.load_binary(path, s2)
+    # corrupt payload (one byte) and truncation: refused
+    raw = bytearray(path.read_bytes())
+    raw[len(raw) // 2] ^= 0x40
+    bad = tmp_path / "bad.rbvh"
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(rbm.RbError):
+        rbm.BVH.load_binary(bad, s)
+    bad.write_bytes(path.read_bytes()[:-100])
+    with pytest.raises(rbm.RbError):
+        rbm.BVH.load_binary(bad, s)
+
+
+def test_build_cached_builds_once_then_loads(rbm, tmp_path):
+    from renderbox2_amd import _lib as L
+    s = rbm.scenes.trace_scaled(20_000, seed=4)
+    b1, hit1 = rbm.BVH.build_cached(s, tmp_path)
+    b2, hit2 = rbm.BVH.build_cached(s, tmp_path)
+    assert (hit1, hit2) == (False, True)
+    assert len(list(tmp_path.glob("rbvh_*.bin"))) == 1
+    assert not list(tmp_path.glob("*.tmp"))
+    for x, y in zip(b1.export(L.RB_LAYOUT_MI355X), b2.export(L.RB_LAYOUT_MI355X)):
+        assert np.array_equal(x, y)
+    # the thread count does not enter the key (the tree does not depend on it)
+    _, hit3 = rbm.BVH.build_cached(s, tmp_path, params=rbm.BuildParams(threads=3))
+    assert hit3
