@@ -332,6 +332,22 @@ rb_status rb_camera_matrices(const rb_camera* cam, float raster_to_camera[16], f
 rb_status rb_raygen_camera(const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                            uint32_t spp, uint64_t seed, int jitter, rb_ray* rays, void* stream);
 
+/* ---- scene files ------------------------------------------------------- */
+
+/* Wavefront OBJ (ObjFileReader::read_file, src/io/objfilereader.cpp:117-217):
+ * every object of the file is added with `material_id`.  Vertices and
+ * triangles keep file order and file-global indexing (Scene::add_mesh,
+ * src/core/scene.cpp:83-113); polygons are fanned into triangles, negative
+ * indices resolved, `#` starts a comment anywhere on a line. */
+rb_status rb_scene_load_obj(rb_scene_t s, const char* path, uint32_t material_id);
+/* The reference's XML scene file (CustomFileReader::process and get_camera,
+ * src/io/customfilereader.cpp:47-340): its mesh file (path relative to the
+ * XML file), materials (single-layer lambertian / mirror; other bxdfs and
+ * layered materials give RB_ERR_UNSUPPORTED), object -> material
+ * assignments, diffuse lights and, when `cam` is not NULL, the camera for a
+ * width x height film (fov from <fov>, 90 degrees when absent). */
+rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint32_t height, rb_camera* cam);
+
 /* Incoherent diffuse rays from primary hits: ray i starts at the hit of
  * primary i (or a Philox-redrawn hit primary when i missed), with a
  * cosine-distributed direction about the geometric normal facing the

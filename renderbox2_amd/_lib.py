@@ -99,6 +99,8 @@ _SIGS = {
     "rb_scene_materials": (C.c_int, [P, P]),
     "rb_scene_bounds": (C.c_int, [P, P, P]),
     "rb_scene_generate": (C.c_int, [P, P]),
+    "rb_scene_load_obj": (C.c_int, [P, C.c_char_p, U32]),
+    "rb_scene_load_xml": (C.c_int, [P, C.c_char_p, U32, U32, P]),
     "rb_build_params_default": (None, [P]),
     "rb_bvh_build_host": (C.c_int, [P, P, P]),
     "rb_bvh_build_arrays": (C.c_int, [P, U32, P, U32, P, P]),

@@ -148,6 +148,19 @@ class Scene:
     def add_diffuse_light_bsdf(self, radiance):
         return self.add_material(L.RB_MAT_EMITTER, radiance)
 
+    def load_obj(self, path, material_id=0):
+        """Every object of a Wavefront OBJ file with one material (ObjFileReader, src/io/objfilereader.cpp)."""
+        check(lib().rb_scene_load_obj(self._h, str(path).encode(), material_id))
+
+    def load_xml(self, path, width=512, height=512):
+        """The reference's XML scene file (CustomFileReader, src/io/customfilereader.cpp):
+        mesh, materials, lights; returns the file's camera for a width x height film."""
+        c = L.rb_camera()
+        check(lib().rb_scene_load_xml(self._h, str(path).encode(), width, height, C.byref(c)))
+        cam = PerspectiveCamera.__new__(PerspectiveCamera)
+        cam.c = c
+        return cam
+
     def add_mesh(self, verts, idx, material_id):
         v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)

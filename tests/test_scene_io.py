@@ -1,0 +1,140 @@
+"""Scene ingestion: OBJ meshes and the reference's XML scene description
+(src/io/objfilereader.cpp:117-217, src/io/customfilereader.cpp:47-340).  The
+reference ships no scene files, so the fixtures are written here in the
+formats its readers parse; expectations come from the file text."""
+import numpy as np
+import pytest
+
+from renderbox2_amd import _lib as L
+
+CUBE_ROOM_OBJ = """# two objects; faces use every corner syntax of the format
+o floor
+v -1 0 -1
+v  1 0 -1
+v  1 0  1
+v -1 0  1
+vt 0 0
+vt 1 0
+vn 0 1 0
+f 1/1/1 2/2/1 3/1/1
+f 1//1 3//1 4//1
+o lamp
+v -0.25 1.9 -0.25   # a trailing comment
+v  0.25 1.9 -0.25
+v  0.25 1.9  0.25
+v -0.25 1.9  0.25
+f 5 8 7 6
+o mirrorwall
+v 1 0 -1
+v 1 2 -1
+v 1 2 1
+f -3 -2 -1
+"""
+
+SCENE_XML = """<?xml version="1.0"?>
+<!-- cornell-style test scene -->
+<scene>
+  <files>
+    <count>1</count>
+    <file>room.obj</file>
+  </files>
+  <camera>
+    <position>{0, 1, -3}</position>
+    <lookat>{0,1,0}</lookat>
+    <up>0 1 0</up>
+    <fov>45</fov>
+  </camera>
+  <material>
+    <name>white</name>
+    <bsdf><layers>1</layers>
+      <bxdf><type>lambertian</type><reflectance>{0.7, 0.6, 0.5}</reflectance></bxdf>
+    </bsdf>
+  </material>
+  <material>
+    <name>silver</name>
+    <bsdf><layers>1</layers>
+      <bxdf><type>mirror</type><reflectance>{0.9,0.9,0.9}</reflectance></bxdf>
+    </bsdf>
+  </material>
+  <object><name>floor</name><material>white</material></object>
+  <object><name>mirrorwall</name><material>silver</material></object>
+  <lightobjects>
+    <light><name>lamp</name><type>diffuse</type><Le>{10, 9, 8}</Le></light>
+  </lightobjects>
+</scene>
+"""
+
+
+def _write(tmp_path, obj=CUBE_ROOM_OBJ, xml=SCENE_XML):
+    (tmp_path / "room.obj").write_text(obj)
+    (tmp_path / "room.xml").write_text(xml)
+    return tmp_path / "room.xml"
+
+
+def test_obj_reader_keeps_file_order_and_global_indices(rbm, tmp_path):
+    _write(tmp_path)
+    s = rbm.Scene()
+    m = s.add_diffuse_bsdf((0.5, 0.5, 0.5))
+    s.load_obj(tmp_path / "room.obj", m)
+    v, idx, tm, lt = s.geometry()
+    assert v.shape == (11, 3) and np.allclose(v[4], [-0.25, 1.9, -0.25]) and np.allclose(v[10], [1, 2, 1])
+    # floor: 2 triangles; lamp quad fanned (5 8 7 6 -> 5 8 7, 5 7 6); wall: -3 -2 -1 = 9 10 11
+    expect = [[0, 1, 2], [0, 2, 3], [4, 7, 6], [4, 6, 5], [8, 9, 10]]
+    assert idx.tolist() == expect
+    assert (tm == m).all() and len(lt) == 0
+
+
+def test_xml_scene_materials_lights_and_camera(rbm, tmp_path):
+    path = _write(tmp_path)
+    s = rbm.Scene()
+    cam = s.load_xml(path, 320, 240)
+    v, idx, tm, lt = s.geometry()
+    mats = s.materials()
+    # materials in file order, then one emitter per light
+    assert mats == [(L.RB_MAT_LAMBERT, pytest.approx((0.7, 0.6, 0.5))), (L.RB_MAT_MIRROR, pytest.approx((0.9,) * 3)),
+                    (L.RB_MAT_EMITTER, pytest.approx((10.0, 9.0, 8.0)))]
+    assert tm.tolist() == [0, 0, 2, 2, 1]
+    assert lt.tolist() == [2, 3]
+    c = cam.c
+    assert list(c.position) == [0, 1, -3] and list(c.lookat) == [0, 1, 0] and list(c.up) == [0, 1, 0]
+    assert c.hfov_deg == 45.0 and (c.width, c.height) == (320, 240)
+
+
+def test_xml_scene_equals_the_same_scene_built_by_hand(rbm, tmp_path):
+    path = _write(tmp_path)
+    a = rbm.Scene()
+    a.load_xml(path)
+    b = rbm.Scene()
+    w = b.add_diffuse_bsdf((0.7, 0.6, 0.5))
+    mi = b.add_mirror_bsdf((0.9, 0.9, 0.9))
+    e = b.add_diffuse_light_bsdf((10.0, 9.0, 8.0))
+    va, ia, _, _ = a.geometry()
+    b.add_mesh(va[0:4], [[0, 1, 2], [0, 2, 3]], w)
+    b.add_mesh(va[4:8], [[0, 3, 2], [0, 2, 1]], e)
+    b.add_mesh(va[8:11], [[0, 1, 2]], mi)
+    for x, y in zip(a.geometry(), b.geometry()):
+        assert np.array_equal(x, y)
+    ba = rbm.BVH(a)
+    bb = rbm.BVH(b)
+    for x, y in zip(ba.export(L.RB_LAYOUT_MI355X), bb.export(L.RB_LAYOUT_MI355X)):
+        assert np.array_equal(x, y)
+
+
+def test_scene_file_errors(rbm, tmp_path):
+    s = rbm.Scene()
+    s.add_diffuse_bsdf((1, 1, 1))
+    with pytest.raises(rbm.RbError):
+        s.load_obj(tmp_path / "missing.obj", 0)
+    (tmp_path / "bad.obj").write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\n")
+    with pytest.raises(rbm.RbError):  # vertex 3 does not exist
+        s.load_obj(tmp_path / "bad.obj", 0)
+    glass = SCENE_XML.replace("<type>mirror</type>", "<type>glass</type>")
+    with pytest.raises(rbm.RbError) as e:
+        rbm.Scene().load_xml(_write(tmp_path, xml=glass))
+    assert e.value.status == L.RB_ERR_UNSUPPORTED
+    with pytest.raises(rbm.RbError) as e:
+        rbm.Scene().load_xml(_write(tmp_path, xml=SCENE_XML.replace("</scene>", "")))
+    assert e.value.status == L.RB_ERR_INVALID
+    unknown = SCENE_XML.replace("<material>silver</material>", "<material>gold</material>")
+    with pytest.raises(rbm.RbError):
+        rbm.Scene().load_xml(_write(tmp_path, xml=unknown))
