@@ -374,11 +374,22 @@ typedef struct rb_render_params {
     uint32_t connections;  /* BDPT: LVC connections per camera vertex */
     uint32_t light_max_depth; /* BDPT: light sub-path depth */
     unsigned long long* ray_counter; /* optional device counter: += rays traced by the call */
+    /* Film reconstruction filter (Filter*, src/core/filter.h:69-132; the
+     * renderer's <filter> config): RB_FILTER_BOX / _TRIANGLE / _GAUSSIAN with
+     * half-widths in pixels and the Gaussian falloff.  A pixel takes every
+     * sample of the tile whose splat window (kernel_update_film,
+     * src/renderers/samplerrenderer.cu:103-169) covers it, weighted through
+     * the 16x16 filter table, and writes max(0, sum(w L) / sum(w)); the box
+     * of half-width 0.5 (the default) is the mean of the pixel's own samples. */
+    uint32_t filter;
+    float filter_width[2];
+    float filter_alpha;
 } rb_render_params;
+enum { RB_FILTER_BOX = 0, RB_FILTER_TRIANGLE = 1, RB_FILTER_GAUSSIAN = 2 };
 void rb_render_params_default(rb_render_params* p);
 
 /* One render of the tile [x0,x0+w) x [y0,y0+h): camera rays (spp per pixel),
- * the integrator, and the per-pixel box-filtered average into out_rgb
+ * the integrator, and the filtered film (see `filter` above) into out_rgb
  * (device float[w*h*3], row-major).  Replaces SamplerRenderer::render
  * (src/renderers/samplerrenderer.cpp:58-107) + Integrator::render
  * (src/core/integrator.h:59) + kernel_update_film

@@ -176,8 +176,25 @@ def camera_matrices(pos, at, up, hfov, width, height):
     return r2c, c2w
 
 
-def render_ao(compact2, verts, idx, cam, tile, spp, seed, nao, radius):
-    """cam = (pos, at, up, hfov, width, height); tile = (x0, y0, w, h) -> float32[h, w, 3]"""
+FILM_DEFAULT = (0, 0.5, 0.5, 2.0)  # box of half-width 0.5: the mean of the pixel's samples
+
+
+def film(samples, width, tile, spp, seed, film=FILM_DEFAULT):
+    """Filtered film of per-sample radiance samples[w*h*spp, 3] (orc_film)."""
+    x0, y0, w, h = tile
+    ls = np.ascontiguousarray(samples, np.float32).reshape(-1, 3)
+    assert len(ls) == w * h * spp
+    out = np.empty((h, w, 3), np.float32)
+    L = lib()
+    U = C.c_uint32
+    L.orc_film.argtypes = [C.c_void_p] + [U] * 6 + [C.c_uint64, U, C.c_float, C.c_float, C.c_float, C.c_void_p]
+    L.orc_film(_p(ls), width, x0, y0, w, h, spp, seed, *film, _p(out))
+    return out
+
+
+def render_ao(compact2, verts, idx, cam, tile, spp, seed, nao, radius, film=FILM_DEFAULT):
+    """cam = (pos, at, up, hfov, width, height); tile = (x0, y0, w, h); film = (filter, xw, yw, alpha)
+    -> float32[h, w, 3]"""
     nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
     verts = np.ascontiguousarray(verts, np.float32)
     idx = np.ascontiguousarray(idx, np.uint32)
@@ -187,13 +204,14 @@ def render_ao(compact2, verts, idx, cam, tile, spp, seed, nao, radius):
     L = lib()
     P = C.c_void_p
     U = C.c_uint32
-    L.orc_render_ao.argtypes = [P] * 7 + [U] * 7 + [C.c_uint64, U, C.c_float, P]
+    L.orc_render_ao.argtypes = [P] * 7 + [U] * 7 + [C.c_uint64, U, C.c_float, C.c_uint32, C.c_float, C.c_float, C.c_float, P]
     L.orc_render_ao(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(r2c), _p(c2w), cam[4], cam[5], x0, y0,
-                    w, h, spp, seed, nao, radius, _p(out))
+                    w, h, spp, seed, nao, radius, *film, _p(out))
     return out
 
 
-def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob):
+def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob,
+              film=FILM_DEFAULT):
     """materials: list of (kind, (r, g, b)); returns float32[h, w, 3]."""
     nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
     verts = np.ascontiguousarray(verts, np.float32)
@@ -210,15 +228,15 @@ def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, 
     L = lib()
     P = C.c_void_p
     U = C.c_uint32
-    L.orc_render_pt.argtypes = [P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, P]
+    L.orc_render_pt.argtypes = [P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, C.c_uint32, C.c_float, C.c_float, C.c_float, P]
     L.orc_render_pt(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m4), len(materials),
                     _p(lights), len(lights), _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, max_depth, rr_start,
-                    rr_prob, _p(out))
+                    rr_prob, *film, _p(out))
     return out
 
 
 def render_bdpt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob,
-                light_paths, light_max_depth, connections):
+                light_paths, light_max_depth, connections, film=FILM_DEFAULT):
     """BDPT-LVC image of the tile; returns float32[h, w, 3]."""
     nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
     verts = np.ascontiguousarray(verts, np.float32)
@@ -235,10 +253,11 @@ def render_bdpt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp
     L = lib()
     P = C.c_void_p
     U = C.c_uint32
-    L.orc_render_bdpt.argtypes = [P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, U, U, U, P]
+    L.orc_render_bdpt.argtypes = ([P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, U, U, U] +
+                                  [C.c_uint32, C.c_float, C.c_float, C.c_float, P])
     L.orc_render_bdpt(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m4), len(materials),
                       _p(lights), len(lights), _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, max_depth,
-                      rr_start, rr_prob, light_paths, light_max_depth, connections, _p(out))
+                      rr_start, rr_prob, light_paths, light_max_depth, connections, *film, _p(out))
     return out
 
 

@@ -1123,22 +1123,103 @@ static uint32_t sample_gid(uint32_t width, uint32_t x0, uint32_t y0, uint32_t w,
     return ((y0 + p / w) * width + (x0 + p % w)) * spp + s;
 }
 
+/* ------------------------------------------------------------------------ */
+/* Film: Film::Film's filter table (src/core/film.cpp:45-58, x offsets from  */
+/* the x index) and kernel_update_film's splat (src/renderers/               */
+/* samplerrenderer.cu:103-169) restated per output pixel: pixel X gets the   */
+/* sample at raster position d + 0.5 when ceil(d - w) <= X <= floor(d + w)  */
+/* on both axes, with weight table[|X - d| / w * 16]; output                */
+/* max(0, sum(wL) * (1 / sum(w))) (film.cpp:68-86).  Filters of              */
+/* src/core/filter.h:69-132.  The box of half-width 0.5 is the mean of the  */
+/* pixel's own samples.  Sums run over neighbour rows, columns, samples.     */
+/* ------------------------------------------------------------------------ */
+
+static float filter_eval(uint32_t kind, float xw, float yw, float alpha, float x, float y)
+{
+    if (kind == 1u) return fmaxf(0.0f, xw - fabsf(x)) * fmaxf(0.0f, yw - fabsf(y));
+    if (kind == 2u) {
+        float ex = expf(-alpha * xw * xw), ey = expf(-alpha * yw * yw);
+        return fmaxf(0.0f, expf(-alpha * x * x) - ex) * fmaxf(0.0f, expf(-alpha * y * y) - ey);
+    }
+    return 1.0f;
+}
+
+void orc_film(const float* Ls, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp,
+              uint64_t seed, uint32_t kind, float xw, float yw, float alpha, float* out_rgb)
+{
+    if (kind == 0u && xw == 0.5f && yw == 0.5f) {
+        for (uint32_t px = 0; px < w * h; px++) {
+            float r = 0.0f, g = 0.0f, b = 0.0f;
+            for (uint32_t s = 0; s < spp; s++) {
+                const float* L = Ls + 3 * ((size_t)px * spp + s);
+                r += L[0]; g += L[1]; b += L[2];
+            }
+            float inv = (float)spp;
+            out_rgb[3 * px] = r / inv; out_rgb[3 * px + 1] = g / inv; out_rgb[3 * px + 2] = b / inv;
+        }
+        return;
+    }
+    float table[256];
+    for (int y = 0; y < 16; y++)
+        for (int x = 0; x < 16; x++)
+            table[y * 16 + x] = filter_eval(kind, xw, yw, alpha, ((float)x + 0.5f) * xw / 16.0f,
+                                            ((float)y + 0.5f) * yw / 16.0f);
+    const float ixw = 1.0f / xw, iyw = 1.0f / yw;
+    const int rx = (int)ceilf(xw) + 1, ry = (int)ceilf(yw) + 1;
+    orc_rng rng = rng_make(seed, 1u);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long px = 0; px < (long)(w * h); px++) {
+        int lx = (int)(px % w), ly = (int)(px / w);
+        float X = (float)(x0 + (uint32_t)lx), Y = (float)(y0 + (uint32_t)ly);
+        float r = 0.0f, g = 0.0f, b = 0.0f, ws = 0.0f;
+        for (int ny = ly - ry; ny <= ly + ry; ny++) {
+            if (ny < 0 || ny >= (int)h) continue;
+            uint32_t gy = y0 + (uint32_t)ny;
+            for (int nx = lx - rx; nx <= lx + rx; nx++) {
+                if (nx < 0 || nx >= (int)w) continue;
+                uint32_t gx = x0 + (uint32_t)nx;
+                for (uint32_t s = 0; s < spp; s++) {
+                    uint32_t q[4];
+                    rng_draw(rng, gy * width + gx, s, 0, 0, q);
+                    float dx = ((float)gx + u01(q[0])) - 0.5f, dy = ((float)gy + u01(q[1])) - 0.5f;
+                    if (ceilf(dx - xw) > X || X > floorf(dx + xw)) continue;
+                    if (ceilf(dy - yw) > Y || Y > floorf(dy + yw)) continue;
+                    int ix = (int)floorf(fabsf((X - dx) * ixw * 16.0f));
+                    int iy = (int)floorf(fabsf((Y - dy) * iyw * 16.0f));
+                    if (ix > 15) ix = 15;
+                    if (iy > 15) iy = 15;
+                    float wt = table[iy * 16 + ix];
+                    const float* L = Ls + 3 * (((size_t)ny * w + (size_t)nx) * spp + s);
+                    r += wt * L[0]; g += wt * L[1]; b += wt * L[2];
+                    ws += wt;
+                }
+            }
+        }
+        float o[3] = {0.0f, 0.0f, 0.0f};
+        if (ws != 0.0f) {
+            float inv = 1.0f / ws;
+            o[0] = fmaxf(0.0f, r * inv); o[1] = fmaxf(0.0f, g * inv); o[2] = fmaxf(0.0f, b * inv);
+        }
+        out_rgb[3 * px] = o[0]; out_rgb[3 * px + 1] = o[1]; out_rgb[3 * px + 2] = o[2];
+    }
+}
+
 /* AO: per sample, nao cosine rays from the hit (normal turned towards the
  * viewer), tmin = 1e-3 t, tmax = radius; value = unoccluded / nao. */
 void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
                    const uint32_t* idx, const float* r2c, const float* c2w, uint32_t width, uint32_t height,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, uint32_t nao,
-                   float radius, float* out_rgb)
+                   float radius, uint32_t filter, float fxw, float fyw, float falpha, float* out_rgb)
 {
     (void)height;
     orc_scene S = {nodes, woop, tidx, verts, idx, NULL, NULL, NULL, 0};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
+    float* Ls = (float*)malloc((size_t)n * 12);
     orc_raygen_camera(r2c, c2w, width, x0, y0, w, h, spp, seed, 1, rays);
     orc_rng rng = rng_make(seed, 2u);
 #pragma omp parallel for schedule(dynamic, 16)
     for (long px = 0; px < (long)(w * h); px++) {
-        float sum = 0.0f;
         for (uint32_t s = 0; s < spp; s++) {
             uint32_t i = (uint32_t)px * spp + s;
             const float* R = rays + 8 * (size_t)i;
@@ -1165,12 +1246,12 @@ void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                 }
                 c = (float)open / (float)nao;
             }
-            sum += c;
+            Ls[3 * (size_t)i] = c; Ls[3 * (size_t)i + 1] = c; Ls[3 * (size_t)i + 2] = c;
         }
-        float v = sum / (float)spp;
-        out_rgb[3 * px] = v; out_rgb[3 * px + 1] = v; out_rgb[3 * px + 2] = v;
     }
+    orc_film(Ls, width, x0, y0, w, h, spp, seed, filter, fxw, fyw, falpha, out_rgb);
     free(rays);
+    free(Ls);
 }
 
 static float light_pdf_mt(f3 p, f3 d, f3 v0, f3 v1, f3 v2, f3 ln, float area)
@@ -1465,26 +1546,27 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                    const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
                    const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
-                   uint32_t max_depth, uint32_t rr_start, float rr_prob, float* out_rgb)
+                   uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t filter, float fxw, float fyw,
+                   float falpha, float* out_rgb)
 {
     orc_material* mats = mats_from4(mats4, nmats);
     orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
+    float* Ls = (float*)malloc((size_t)n * 12);
     orc_raygen_camera(r2c, c2w, width, x0, y0, w, h, spp, seed, 1, rays);
     orc_rng rng = rng_make(seed, 5u);
 #pragma omp parallel for schedule(dynamic, 8)
     for (long px = 0; px < (long)(w * h); px++) {
-        float r = 0.0f, g = 0.0f, b = 0.0f;
         for (uint32_t s = 0; s < spp; s++) {
             uint32_t i = (uint32_t)px * spp + s;
             f3 L = pt_path(&S, rays + 8 * (size_t)i, sample_gid(width, x0, y0, w, spp, i), rng, max_depth, rr_start, rr_prob, NULL);
-            r += L.x; g += L.y; b += L.z;
+            Ls[3 * (size_t)i] = L.x; Ls[3 * (size_t)i + 1] = L.y; Ls[3 * (size_t)i + 2] = L.z;
         }
-        float inv = (float)spp;
-        out_rgb[3 * px] = r / inv; out_rgb[3 * px + 1] = g / inv; out_rgb[3 * px + 2] = b / inv;
     }
+    orc_film(Ls, width, x0, y0, w, h, spp, seed, filter, fxw, fyw, falpha, out_rgb);
     free(rays);
+    free(Ls);
     free(mats);
 }
 
@@ -1580,7 +1662,8 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
                      const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
                      uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                      uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t light_paths,
-                     uint32_t light_max_depth, uint32_t connections, float* out_rgb)
+                     uint32_t light_max_depth, uint32_t connections, uint32_t filter, float fxw, float fyw,
+                     float falpha, float* out_rgb)
 {
     orc_material* mats = mats_from4(mats4, nmats);
     orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
@@ -1600,20 +1683,19 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
     orc_lvc B = {NL, D, nl ? connections : 0u, count, pos, ngs, wos, alpha, mid, mask, rng_make(seed, 8u)};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
+    float* Ls = (float*)malloc((size_t)n * 12);
     orc_raygen_camera(r2c, c2w, width, x0, y0, w, h, spp, seed, 1, rays);
     orc_rng rng = rng_make(seed, 5u);
 #pragma omp parallel for schedule(dynamic, 8)
     for (long px = 0; px < (long)(w * h); px++) {
-        float r = 0.0f, g = 0.0f, b = 0.0f;
         for (uint32_t s = 0; s < spp; s++) {
             uint32_t i = (uint32_t)px * spp + s;
             f3 L = pt_path(&S, rays + 8 * (size_t)i, sample_gid(width, x0, y0, w, spp, i), rng, max_depth, rr_start,
                            rr_prob, &B);
-            r += L.x; g += L.y; b += L.z;
+            Ls[3 * (size_t)i] = L.x; Ls[3 * (size_t)i + 1] = L.y; Ls[3 * (size_t)i + 2] = L.z;
         }
-        float inv = (float)spp;
-        out_rgb[3 * px] = r / inv; out_rgb[3 * px + 1] = g / inv; out_rgb[3 * px + 2] = b / inv;
     }
-    free(rays); free(count); free(pos); free(ngs); free(wos); free(alpha); free(mid); free(mask);
+    orc_film(Ls, width, x0, y0, w, h, spp, seed, filter, fxw, fyw, falpha, out_rgb);
+    free(rays); free(Ls); free(count); free(pos); free(ngs); free(wos); free(alpha); free(mid); free(mask);
     free(mats);
 }

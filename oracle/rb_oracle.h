@@ -124,19 +124,27 @@ void orc_raygen_diffuse(const float* verts, const uint32_t* idx, const float* pr
 
 /* ---- render passes ------------------------------------------------------ */
 
-/* AO (src/integrators/integrator_ao.cu:42-85) over the tile, box-filtered
- * average of spp samples per pixel into out_rgb[w*h*3]. */
+/* Film of a tile from per-sample radiance Ls[w*h*spp*3] (sample = pixel*spp
+ * + s): filter 0 box / 1 triangle / 2 Gaussian with half-widths fxw, fyw and
+ * Gaussian falloff falpha (src/core/filter.h:69-132, film.cpp:45-86,
+ * samplerrenderer.cu:103-169).  Sample positions are the camera rays' jitter
+ * (Philox, stream 1).  The render passes below end with it. */
+void orc_film(const float* Ls, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp,
+              uint64_t seed, uint32_t filter, float fxw, float fyw, float falpha, float* out_rgb);
+/* AO (src/integrators/integrator_ao.cu:42-85) over the tile, filtered film
+ * of spp samples per pixel into out_rgb[w*h*3]. */
 void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
                    const uint32_t* idx, const float* r2c, const float* c2w, uint32_t width, uint32_t height,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, uint32_t nao,
-                   float radius, float* out_rgb);
+                   float radius, uint32_t filter, float fxw, float fyw, float falpha, float* out_rgb);
 /* Path tracer (src/integrators/integrator_path.cu:43-261): NEE with MIS,
  * Lambert / mirror / emitter materials (mats4: kind bits + rgb per material). */
 void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
                    const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
                    const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
-                   uint32_t max_depth, uint32_t rr_start, float rr_prob, float* out_rgb);
+                   uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t filter, float fxw, float fyw,
+                   float falpha, float* out_rgb);
 /* Number of BDPT techniques able to build a path z_1..z_n (mask bit i-1 =
  * z_i specular) when camera vertices exist for bounces 0..max_depth and the
  * light-vertex cache holds depths 0..lvc_depth-1; a path's weight is 1/that. */
@@ -150,7 +158,8 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
                      const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
                      uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                      uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t light_paths,
-                     uint32_t light_max_depth, uint32_t connections, float* out_rgb);
+                     uint32_t light_max_depth, uint32_t connections, uint32_t filter, float fxw, float fyw,
+                     float falpha, float* out_rgb);
 
 #ifdef __cplusplus
 }

@@ -151,6 +151,7 @@ _SIGS = {
 
 
 RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT, RB_REQ_TRACE = 0, 1, 2, 3
+RB_FILTER_BOX, RB_FILTER_TRIANGLE, RB_FILTER_GAUSSIAN = 0, 1, 2
 RB_REQ_QUEUED, RB_REQ_RUNNING, RB_REQ_DONE, RB_REQ_FAILED = 0, 1, 2, 3
 
 
@@ -183,7 +184,8 @@ class rb_batch_desc(C.Structure):
 class rb_render_params(C.Structure):
     _fields_ = [("spp", U32), ("seed", U64), ("ao_samples", U32), ("ao_radius", F32), ("max_depth", U32),
                 ("rr_start", U32), ("rr_prob", F32), ("light_paths", U32), ("lvc_capacity", U32),
-                ("connections", U32), ("light_max_depth", U32), ("ray_counter", P)]
+                ("connections", U32), ("light_max_depth", U32), ("ray_counter", P), ("filter", U32),
+                ("filter_width", F32 * 2), ("filter_alpha", F32)]
 
 
 class rb_request(C.Structure):

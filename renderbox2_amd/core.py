@@ -482,11 +482,19 @@ def trace_batch(batches, stream=None):
     check(lib().rb_trace_batch(arr, len(batches), _sp(stream)))
 
 
+FILTERS = {"box": L.RB_FILTER_BOX, "triangle": L.RB_FILTER_TRIANGLE, "gaussian": L.RB_FILTER_GAUSSIAN}
+
+
 def render_params(**kw):
-    """rb_render_params with the library defaults, overridden by keywords."""
+    """rb_render_params with the library defaults, overridden by keywords
+    (filter may be a name: "box", "triangle", "gaussian")."""
     p = L.rb_render_params()
     lib().rb_render_params_default(C.byref(p))
     for k, v in kw.items():
+        if k == "filter" and isinstance(v, str):
+            v = FILTERS[v]
+        if k == "filter_width":
+            v = (C.c_float * 2)(*v)
         setattr(p, k, v)
     return p
 

@@ -62,25 +62,17 @@ __global__ void k_ao_spawn(const float4* __restrict__ prim, const int2* __restri
     }
 }
 
-// Per pixel: mean over samples of (unoccluded AO rays / nao); misses give 0.
-__global__ void k_ao_resolve(const int2* __restrict__ prim_hits, const int2* __restrict__ ao_hits, uint32_t npix,
-                             uint32_t spp, uint32_t nao, float* __restrict__ out) {
-    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
-        float sum = 0.0f;
-        for (uint32_t s = 0; s < spp; s++) {
-            const uint32_t i = px * spp + s;
-            float c = 0.0f;
-            if (prim_hits[i].x >= 0) {
-                uint32_t open = 0;
-                for (uint32_t k = 0; k < nao; k++) open += ao_hits[(size_t)i * nao + k].x < 0 ? 1u : 0u;
-                c = (float)open / (float)nao;
-            }
-            sum += c;
+// Per sample: unoccluded AO rays / nao (misses give 0), for the film.
+__global__ void k_ao_resolve(const int2* __restrict__ prim_hits, const int2* __restrict__ ao_hits, uint32_t n,
+                             uint32_t nao, float4* __restrict__ L) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float c = 0.0f;
+        if (prim_hits[i].x >= 0) {
+            uint32_t open = 0;
+            for (uint32_t k = 0; k < nao; k++) open += ao_hits[(size_t)i * nao + k].x < 0 ? 1u : 0u;
+            c = (float)open / (float)nao;
         }
-        const float v = sum / (float)spp;
-        out[3 * (size_t)px] = v;
-        out[3 * (size_t)px + 1] = v;
-        out[3 * (size_t)px + 2] = v;
+        L[i] = make_float4(c, c, c, 0.f);
     }
 }
 
@@ -464,6 +456,99 @@ __global__ void k_film(const float4* __restrict__ L, uint32_t npix, uint32_t spp
     }
 }
 
+// Filtered film (kernel_update_film, src/renderers/samplerrenderer.cu:103-169,
+// as a gather): pixel X takes sample s of tile pixel g when the sample's splat
+// window [ceil(d - w), floor(d + w)] (d = sample position - 0.5) contains X;
+// the weight is the filter table entry of (|X - d| / w * 16) per axis.
+// Sample positions are those of the camera rays (same Philox draw), so
+// nothing is stored; sums run in a fixed order (neighbour row, column,
+// sample), so images are reproducible.
+struct FilmFilter {
+    float xw, yw, inv_xw, inv_yw;
+    float table[256];  // [y * 16 + x]
+};
+
+__global__ void k_film_filter(const float4* __restrict__ L, TileArgs T, uint32_t h, FilmFilter F, RngArg rng_a,
+                              float* __restrict__ out) {
+    const Rng rng = rng_a.get();
+    const uint32_t npix = T.w * h;
+    const int rx = (int)ceilf(F.xw) + 1, ry = (int)ceilf(F.yw) + 1;
+    for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
+        const int lx = (int)(px % T.w), ly = (int)(px / T.w);
+        const float X = (float)(T.x0 + (uint32_t)lx), Y = (float)(T.y0 + (uint32_t)ly);
+        float r = 0.f, g = 0.f, b = 0.f, ws = 0.f;
+        const int ny0 = ly - ry < 0 ? 0 : ly - ry, ny1 = ly + ry > (int)h - 1 ? (int)h - 1 : ly + ry;
+        const int nx0 = lx - rx < 0 ? 0 : lx - rx, nx1 = lx + rx > (int)T.w - 1 ? (int)T.w - 1 : lx + rx;
+        for (int ny = ny0; ny <= ny1; ny++) {
+            const uint32_t gy = T.y0 + (uint32_t)ny;
+            for (int nx = nx0; nx <= nx1; nx++) {
+                const uint32_t gx = T.x0 + (uint32_t)nx;
+                for (uint32_t s = 0; s < T.spp; s++) {
+                    const U4 q = rng.draw(gy * T.width + gx, s, 0, 0);
+                    const float dx = ((float)gx + u01(q.x)) - 0.5f;
+                    const float dy = ((float)gy + u01(q.y)) - 0.5f;
+                    if (!(ceilf(dx - F.xw) <= X && X <= floorf(dx + F.xw))) continue;
+                    if (!(ceilf(dy - F.yw) <= Y && Y <= floorf(dy + F.yw))) continue;
+                    int ix = (int)floorf(fabsf((X - dx) * F.inv_xw * 16.0f));
+                    int iy = (int)floorf(fabsf((Y - dy) * F.inv_yw * 16.0f));
+                    ix = ix < 15 ? ix : 15;
+                    iy = iy < 15 ? iy : 15;
+                    const float wgt = F.table[iy * 16 + ix];
+                    const float4 v = L[((size_t)ny * T.w + (size_t)nx) * T.spp + s];
+                    r += wgt * v.x;
+                    g += wgt * v.y;
+                    b += wgt * v.z;
+                    ws += wgt;
+                }
+            }
+        }
+        float o0 = 0.f, o1 = 0.f, o2 = 0.f;
+        if (ws != 0.0f) {
+            const float inv = 1.0f / ws;
+            o0 = fmaxf(0.0f, r * inv);
+            o1 = fmaxf(0.0f, g * inv);
+            o2 = fmaxf(0.0f, b * inv);
+        }
+        out[3 * (size_t)px] = o0;
+        out[3 * (size_t)px + 1] = o1;
+        out[3 * (size_t)px + 2] = o2;
+    }
+}
+
+// Filter table of Film::Film (src/core/film.cpp:45-58; x offsets from the x
+// index, where the reference reuses y): entry (x, y) = filter at
+// ((x + 0.5) w_x / 16, (y + 0.5) w_y / 16).  Filters of src/core/filter.h:69-132.
+static rb_status make_filter(const rb_render_params* p, FilmFilter* f, bool* plain) {
+    const float xw = p->filter_width[0], yw = p->filter_width[1];
+    if (p->filter > RB_FILTER_GAUSSIAN) return fail(RB_ERR_INVALID, "unknown filter %u", p->filter);
+    if (!(xw > 0.0f && xw <= 8.0f && yw > 0.0f && yw <= 8.0f))
+        return fail(RB_ERR_INVALID, "filter half-widths must be in (0, 8]");
+    *plain = p->filter == RB_FILTER_BOX && xw == 0.5f && yw == 0.5f;
+    f->xw = xw;
+    f->yw = yw;
+    f->inv_xw = 1.0f / xw;
+    f->inv_yw = 1.0f / yw;
+    const float a = p->filter_alpha;
+    const float ex = expf(-a * xw * xw), ey = expf(-a * yw * yw);
+    for (int y = 0; y < 16; y++) {
+        const float fy = ((float)y + 0.5f) * yw / 16.0f;
+        for (int x = 0; x < 16; x++) {
+            const float fx = ((float)x + 0.5f) * xw / 16.0f;
+            float v = 1.0f;
+            if (p->filter == RB_FILTER_TRIANGLE) v = fmaxf(0.0f, xw - fabsf(fx)) * fmaxf(0.0f, yw - fabsf(fy));
+            else if (p->filter == RB_FILTER_GAUSSIAN)
+                v = fmaxf(0.0f, expf(-a * fx * fx) - ex) * fmaxf(0.0f, expf(-a * fy * fy) - ey);
+            f->table[y * 16 + x] = v;
+        }
+    }
+    return RB_OK;
+}
+
+// Film of a tile from per-sample radiance: the pixel mean for the default
+// box of half-width 0.5, the filtered gather otherwise.
+static rb_status film(const float4* L, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                      const rb_render_params* p, float* out, hipStream_t s);
+
 static uint32_t grid_of(uint64_t n) {
     uint64_t g = (n + 255) / 256;
     const uint64_t cap = (uint64_t)dev().num_cus * 16;
@@ -514,7 +599,26 @@ static rb_status check_render(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, u
     if (w == 0 || h == 0 || x0 + w > cam->width || y0 + h > cam->height) return fail(RB_ERR_INVALID, "bad tile");
     if (p->spp == 0) return fail(RB_ERR_INVALID, "spp must be >= 1");
     if ((uint64_t)w * h * p->spp > (1ull << 30)) return fail(RB_ERR_INVALID, "tile x spp too large for one call");
-    return RB_OK;
+    FilmFilter f;
+    bool plain;
+    return make_filter(p, &f, &plain);
+}
+
+static rb_status film(const float4* L, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                      const rb_render_params* p, float* out, hipStream_t s) {
+    FilmFilter f;
+    bool plain;
+    rb_status st = make_filter(p, &f, &plain);
+    if (st != RB_OK) return st;
+    if (plain) {
+        hipLaunchKernelGGL(k_film, dim3(grid_of(w * h)), dim3(256), 0, s, L, w * h, p->spp, out);
+    } else {
+        TileArgs T{cam->width, x0, y0, w, p->spp};
+        hipLaunchKernelGGL(k_film_filter, dim3(grid_of(w * h)), dim3(256), 0, s, L, T, h, f,
+                           rng_arg(p->seed, RNG_CAMERA), out);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RB_OK : hip_fail(e, "film launch");
 }
 
 static SceneDev scene_of(rb_bvh_t b) {
@@ -540,6 +644,7 @@ rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
     float4* arays = M.get<float4>(2 * na);
     uint8_t* amask = M.get<uint8_t>(na);
     int2* ahits = M.get<int2>(na);
+    float4* Ls = M.get<float4>(n);
     if (M.st != RB_OK) return M.st;
     if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)rays, s)) != RB_OK) return st;
     if ((st = launch_trace(bvh, (rb_ray*)rays, n, RB_CLOSEST, nullptr, (rb_hit*)hits, nullptr, s, dev().cfg)) != RB_OK)
@@ -551,10 +656,10 @@ rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
     if ((st = launch_trace(bvh, (rb_ray*)arays, (uint32_t)na, RB_ANY | RB_MASKED, amask, (rb_hit*)ahits, nullptr, s,
                            dev().cfg)) != RB_OK)
         return st;
-    hipLaunchKernelGGL(k_ao_resolve, dim3(grid_of(w * h)), dim3(256), 0, s, hits, ahits, w * h, p->spp,
-                       p->ao_samples, out);
+    hipLaunchKernelGGL(k_ao_resolve, dim3(grid_of(n)), dim3(256), 0, s, hits, ahits, n, p->ao_samples, Ls);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? RB_OK : hip_fail(e, "AO launch");
+    if (e != hipSuccess) return hip_fail(e, "AO launch");
+    return film(Ls, cam, x0, y0, w, h, p, out, s);
 }
 
 // Camera pass shared by the path tracer (BD = false) and BDPT (BD = true).
@@ -622,9 +727,9 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
             hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt, p->ray_counter,
                                b == 0 ? (unsigned long long)n : 0ull);
     }
-    hipLaunchKernelGGL(k_film, dim3(grid_of(w * h)), dim3(256), 0, s, P.L, w * h, p->spp, out);
     hipError_t e = hipGetLastError();
-    return e == hipSuccess ? RB_OK : hip_fail(e, "camera pass launch");
+    if (e != hipSuccess) return hip_fail(e, "camera pass launch");
+    return film(P.L, cam, x0, y0, w, h, p, out, s);
 }
 
 rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
@@ -704,7 +809,7 @@ uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_re
     const uint64_t n = (uint64_t)w * h * (p->spp ? p->spp : 1);
     const uint64_t slack = 64 * 40;  // per-allocation padding and alignment
     switch (kind) {
-        case RB_REQ_AO: return n * 40 + n * (p->ao_samples ? p->ao_samples : 1) * 41 + slack;
+        case RB_REQ_AO: return n * 56 + n * (p->ao_samples ? p->ao_samples : 1) * 41 + slack;
         case RB_REQ_PT: return n * 308 + slack;
         case RB_REQ_BDPT: return bdpt_scratch_bytes(w, h, p) + slack;
         default: return slack;
@@ -731,6 +836,10 @@ void rb_render_params_default(rb_render_params* p) {
     p->connections = 5;
     p->light_max_depth = 6;
     p->ray_counter = nullptr;
+    p->filter = RB_FILTER_BOX;
+    p->filter_width[0] = 0.5f;
+    p->filter_width[1] = 0.5f;
+    p->filter_alpha = 2.0f;
 }
 
 rb_status rb_render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,

@@ -234,3 +234,92 @@ def test_bdpt_agrees_with_the_path_tracer_in_expectation(orc):
     pt, bd = np.mean(pt, 0), np.mean(bd, 0)
     assert np.all(pt > 0.1)
     assert np.all(np.abs(bd / pt - 1.0) < 0.01), (pt, bd)
+
+
+# ---------------------------------------------------------------- film ----
+
+def _jitter(orc, seed, width, gx, gy, s):
+    key = [seed & 0xFFFFFFFF, ((seed >> 32) ^ 0x85EBCA6B) & 0xFFFFFFFF]  # stream tag 1 (camera)
+    q = orc.philox([gy * width + gx, s, 0, 0], key)
+    return (q[0] >> 8) * 2.0 ** -24, (q[1] >> 8) * 2.0 ** -24
+
+
+def _filter64(kind, xw, yw, alpha, x, y):
+    if kind == 1:
+        return max(0.0, xw - abs(x)) * max(0.0, yw - abs(y))
+    if kind == 2:
+        g = lambda d, w: max(0.0, np.exp(-alpha * d * d) - np.exp(-alpha * w * w))  # noqa: E731
+        return g(x, xw) * g(y, yw)
+    return 1.0
+
+
+def _splat64(orc, samples, width, tile, spp, seed, film):
+    """kernel_update_film's splat (samplerrenderer.cu:103-169) in float64:
+    every sample adds w*L and w to the tile pixels of its window."""
+    kind, xw, yw, alpha = film
+    x0, y0, w, h = tile
+    table = np.array([[_filter64(kind, xw, yw, alpha, (x + 0.5) * xw / 16, (y + 0.5) * yw / 16)
+                       for x in range(16)] for y in range(16)])
+    acc = np.zeros((h, w, 3))
+    wsum = np.zeros((h, w))
+    for ly in range(h):
+        for lx in range(w):
+            for s in range(spp):
+                jx, jy = _jitter(orc, seed, width, x0 + lx, y0 + ly, s)
+                dx, dy = x0 + lx + jx - 0.5, y0 + ly + jy - 0.5
+                for py in range(max(int(np.ceil(dy - yw)), y0), min(int(np.floor(dy + yw)), y0 + h - 1) + 1):
+                    for px in range(max(int(np.ceil(dx - xw)), x0), min(int(np.floor(dx + xw)), x0 + w - 1) + 1):
+                        ix = min(int(np.floor(abs((px - dx) / xw * 16))), 15)
+                        iy = min(int(np.floor(abs((py - dy) / yw * 16))), 15)
+                        wt = table[iy, ix]
+                        acc[py - y0, px - x0] += wt * samples[(ly * w + lx) * spp + s]
+                        wsum[py - y0, px - x0] += wt
+    out = np.zeros((h, w, 3))
+    nz = wsum > 0
+    out[nz] = np.maximum(0.0, acc[nz] / wsum[nz][:, None])
+    return out
+
+
+@pytest.mark.parametrize("film", [(0, 1.5, 1.0, 2.0), (1, 2.0, 1.5, 2.0), (2, 2.0, 2.0, 1.5), (2, 1.0, 0.5, 4.0)])
+def test_film_gather_equals_the_reference_splat(orc, film):
+    rng = np.random.default_rng(11)
+    width, tile, spp, seed = 64, (5, 9, 13, 11), 3, 77
+    x0, y0, w, h = tile
+    samples = rng.uniform(0.0, 4.0, (w * h * spp, 3)).astype(np.float32)
+    got = orc.film(samples, width, tile, spp, seed, film)
+    ref = _splat64(orc, samples.astype(np.float64), width, tile, spp, seed, film)
+    assert np.allclose(got, ref, rtol=2e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("film", [(0, 2.5, 0.75, 2.0), (1, 1.0, 3.0, 2.0), (2, 2.0, 2.0, 0.5)])
+def test_film_of_constant_radiance_is_that_constant(orc, film):
+    width, tile, spp, seed = 40, (0, 0, 40, 30), 2, 5
+    samples = np.tile(np.array([[0.25, 1.5, 3.0]], np.float32), (40 * 30 * spp, 1))
+    got = orc.film(samples, width, tile, spp, seed, film)
+    assert np.allclose(got, [0.25, 1.5, 3.0], rtol=1e-6)
+
+
+def test_film_default_box_is_the_pixel_mean(orc):
+    rng = np.random.default_rng(3)
+    samples = rng.uniform(0, 1, (6 * 4 * 5, 3)).astype(np.float32)
+    got = orc.film(samples, 6, (0, 0, 6, 4), 5, 1)
+    s = samples.reshape(4, 6, 5, 3)
+    exp = s[:, :, 0]
+    for k in range(1, 5):
+        exp = exp + s[:, :, k]  # float32, sample order
+    assert np.array_equal(got, exp / np.float32(5))
+
+
+def test_film_gaussian_weights_are_radial(orc):
+    # one bright sample in a dark tile: the filtered image is the filter's
+    # footprint, symmetric about the sample position
+    width, tile, spp, seed = 32, (0, 0, 32, 32), 1, 9
+    samples = np.zeros((32 * 32, 3), np.float32)
+    c = 16 * 32 + 16
+    samples[c] = 1.0
+    got = orc.film(samples, width, tile, spp, seed, (2, 2.0, 2.0, 1.0))
+    jx, jy = _jitter(orc, seed, width, 16, 16, 0)
+    dx, dy = 16 + jx - 0.5, 16 + jy - 0.5
+    ys, xs = np.nonzero(got[..., 0])
+    assert len(xs) > 4
+    assert np.all(np.abs(xs - dx) <= 2.0) and np.all(np.abs(ys - dy) <= 2.0)
