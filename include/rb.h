@@ -348,6 +348,12 @@ rb_status rb_scene_load_obj(rb_scene_t s, const char* path, uint32_t material_id
  * width x height film (fov from <fov>, 90 degrees when absent). */
 rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint32_t height, rb_camera* cam);
 
+/* Writes a host RGB float image (w*h*3, row-major, top row first) by suffix
+ * (ImageWriter::write, src/io/imagewriter.cpp:34-105): .exr (OpenEXR,
+ * uncompressed half B/G/R), .pfm (float32), .ppm / .png (8-bit: clamp to
+ * [0,1], gamma 1/2.2, round, as ImageWriter::to_byte). */
+rb_status rb_image_write(const char* path, const float* rgb, uint32_t w, uint32_t h);
+
 /* Incoherent diffuse rays from primary hits: ray i starts at the hit of
  * primary i (or a Philox-redrawn hit primary when i missed), with a
  * cosine-distributed direction about the geometric normal facing the

@@ -9,5 +9,5 @@ from .core import (BVH, FLT_MAX, HIT_DTYPE, RAY_DTYPE, BuildParams, CudaBvh, Cud
                    DeviceBuffer, Event, PerspectiveCamera, Platform, RenderGraph, RenderServer, Scene, Stream, autotune, copy_kernel,
                    device_init,
                    SamplerRenderer, get_trace_config, raygen_diffuse_device, render_params, set_trace_config,
-                   trace_batch)
+                   trace_batch, write_image)
 from . import scenes  # noqa: F401

@@ -101,6 +101,7 @@ _SIGS = {
     "rb_scene_generate": (C.c_int, [P, P]),
     "rb_scene_load_obj": (C.c_int, [P, C.c_char_p, U32]),
     "rb_scene_load_xml": (C.c_int, [P, C.c_char_p, U32, U32, P]),
+    "rb_image_write": (C.c_int, [C.c_char_p, P, U32, U32]),
     "rb_build_params_default": (None, [P]),
     "rb_bvh_build_host": (C.c_int, [P, P, P]),
     "rb_bvh_build_arrays": (C.c_int, [P, U32, P, U32, P, P]),

@@ -560,6 +560,13 @@ class RenderGraph:
             pass
 
 
+def write_image(path, img):
+    """Host float32 image [h, w, 3] to .exr / .pfm / .ppm / .png (rb_image_write)."""
+    a = np.ascontiguousarray(img, np.float32)
+    h, w = a.shape[:2]
+    check(lib().rb_image_write(str(path).encode(), ptr(a), w, h))
+
+
 def _sync(stream):
     check(lib().rb_stream_synchronize(_sp(stream)))
 

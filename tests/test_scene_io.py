@@ -138,3 +138,89 @@ def test_scene_file_errors(rbm, tmp_path):
     unknown = SCENE_XML.replace("<material>silver</material>", "<material>gold</material>")
     with pytest.raises(rbm.RbError):
         rbm.Scene().load_xml(_write(tmp_path, xml=unknown))
+
+
+# ------------------------------------------------------------ image output --
+
+def _img():
+    rng = np.random.default_rng(5)
+    img = rng.uniform(-0.2, 1.3, (7, 9, 3)).astype(np.float32)
+    img[0, 0] = [0.0, -0.0, 1.0]
+    img[0, 1] = [65504.0, 70000.0, 1e-7]    # max half, overflow, subnormal
+    img[0, 2] = [6e-8, 2.9802322e-08, 1e-9]  # smallest subnormal, tie at half of it, underflow
+    img[1, 0] = [1.0009765625 + 2 ** -11, 1.0 + 2 ** -11, 0.5]  # ties: to even
+    return img
+
+
+def test_pfm_round_trip_is_exact(rbm, tmp_path):
+    img = _img()
+    rbm.write_image(tmp_path / "a.pfm", img)
+    raw = (tmp_path / "a.pfm").read_bytes()
+    head, rest = raw.split(b"\n", 3)[:3], raw.split(b"\n", 3)[3]
+    assert head[0] == b"PF" and head[1] == b"9 7" and float(head[2]) < 0
+    back = np.frombuffer(rest, "<f4").reshape(7, 9, 3)[::-1]
+    assert np.array_equal(back.view(np.uint32), img.view(np.uint32))
+
+
+def _to_byte(img):
+    c = np.clip(img, 0.0, 1.0).astype(np.float32)
+    return (np.power(c, np.float32(1.0 / 2.2)) * np.float32(255) + 0.5).astype(np.int64)
+
+
+def test_png_and_ppm_are_valid_and_agree(rbm, tmp_path):
+    import zlib
+    img = _img()
+    rbm.write_image(tmp_path / "a.ppm", img)
+    rbm.write_image(tmp_path / "a.png", img)
+    ppm = (tmp_path / "a.ppm").read_bytes()
+    assert ppm.startswith(b"P6\n9 7\n255\n")
+    pix = np.frombuffer(ppm[len(b"P6\n9 7\n255\n"):], np.uint8).reshape(7, 9, 3)
+    assert np.abs(pix.astype(np.int64) - _to_byte(img)).max() <= 1  # libm powf vs numpy float32 power
+    png = (tmp_path / "a.png").read_bytes()
+    assert png[:8] == b"\x89PNG\r\n\x1a\n"
+    pos, chunks = 8, {}
+    while pos < len(png):
+        n = int.from_bytes(png[pos:pos + 4], "big")
+        typ, data = png[pos + 4:pos + 8], png[pos + 8:pos + 8 + n]
+        crc = int.from_bytes(png[pos + 8 + n:pos + 12 + n], "big")
+        assert zlib.crc32(typ + data) == crc
+        chunks[typ] = data
+        pos += 12 + n
+    assert set(chunks) == {b"IHDR", b"IDAT", b"IEND"}
+    assert chunks[b"IHDR"] == (9).to_bytes(4, "big") + (7).to_bytes(4, "big") + bytes([8, 2, 0, 0, 0])
+    raw = np.frombuffer(zlib.decompress(chunks[b"IDAT"]), np.uint8).reshape(7, 1 + 27)
+    assert (raw[:, 0] == 0).all()
+    assert np.array_equal(raw[:, 1:].reshape(7, 9, 3), pix)
+
+
+def test_exr_holds_the_half_values(rbm, tmp_path):
+    img = _img()
+    rbm.write_image(tmp_path / "a.exr", img)
+    b = (tmp_path / "a.exr").read_bytes()
+    assert int.from_bytes(b[:4], "little") == 20000630 and int.from_bytes(b[4:8], "little") == 2
+    pos, attrs = 8, {}
+    while b[pos] != 0:
+        name_end = b.index(b"\0", pos)
+        type_end = b.index(b"\0", name_end + 1)
+        size = int.from_bytes(b[type_end + 1:type_end + 5], "little")
+        attrs[b[pos:name_end].decode()] = (b[name_end + 1:type_end].decode(), b[type_end + 5:type_end + 5 + size])
+        pos = type_end + 5 + size
+    pos += 1
+    assert attrs["compression"][1] == b"\0"
+    assert np.frombuffer(attrs["dataWindow"][1], "<i4").tolist() == [0, 0, 8, 6]
+    ch = attrs["channels"][1]
+    assert ch.startswith(b"B\0") and b"G\0" in ch and b"R\0" in ch
+    offsets = np.frombuffer(b[pos:pos + 8 * 7], "<u8")
+    with np.errstate(over="ignore"):
+        half = img.astype(np.float16)  # round to nearest even; 70000 -> inf
+    for y in range(7):
+        o = int(offsets[y])
+        assert int.from_bytes(b[o:o + 4], "little") == y and int.from_bytes(b[o + 4:o + 8], "little") == 9 * 3 * 2
+        line = np.frombuffer(b[o + 8:o + 8 + 54], "<u2").reshape(3, 9)  # B, G, R
+        for c, k in ((0, 2), (1, 1), (2, 0)):
+            assert np.array_equal(line[c], half[y, :, k].view(np.uint16)), (y, k)
+
+
+def test_image_write_rejects_unknown_suffix(rbm, tmp_path):
+    with pytest.raises(rbm.RbError):
+        rbm.write_image(tmp_path / "a.tiff", _img())
