@@ -101,12 +101,23 @@ rb_status rb_event_synchronize(void* ev);
 
 typedef struct rb_scene_s* rb_scene_t;
 
-/* Material: the Lambertian / mirror / diffuse-emitter subset of
- * Material + *BsdfParams (src/core/material.h, src/core/params.h). */
-enum { RB_MAT_LAMBERT = 0, RB_MAT_MIRROR = 1, RB_MAT_EMITTER = 2 };
+/* Material: one BSDF layer of Material + *BsdfParams (src/core/material.h,
+ * src/core/params.h:150-240) or a diffuse emitter.  `color` is the albedo /
+ * reflectance / transmittance tint, or the emitted radiance.  param[]:
+ *   RB_MAT_GLASS      [0] eta outside (etai), [1] eta inside (etat)
+ *   RB_MAT_ORENNAYAR  [0] sigma in degrees (clamped to [0, 90]); [1] A and
+ *                     [2] B are filled in by rb_scene_add_material
+ *   RB_MAT_MICROFACET [0] Blinn exponent, [1] Fresnel: 0 none, 1 conductor,
+ *                     2 dielectric; conductor: [2..4] eta rgb, [5..7] k rgb;
+ *                     dielectric: [2] etai, [3] etat
+ * Mirror and glass are specular (delta) BSDFs. */
+enum { RB_MAT_LAMBERT = 0, RB_MAT_MIRROR = 1, RB_MAT_EMITTER = 2, RB_MAT_GLASS = 3, RB_MAT_ORENNAYAR = 4,
+       RB_MAT_MICROFACET = 5 };
+enum { RB_FRESNEL_NONE = 0, RB_FRESNEL_CONDUCTOR = 1, RB_FRESNEL_DIELECTRIC = 2 };
 typedef struct rb_material {
     uint32_t kind;
-    float color[3]; /* albedo, mirror reflectance or emitted radiance */
+    float color[3];
+    float param[8];
 } rb_material;
 
 rb_status rb_scene_create(rb_scene_t* out);
@@ -342,8 +353,8 @@ rb_status rb_raygen_camera(const rb_camera* cam, uint32_t x0, uint32_t y0, uint3
 rb_status rb_scene_load_obj(rb_scene_t s, const char* path, uint32_t material_id);
 /* The reference's XML scene file (CustomFileReader::process and get_camera,
  * src/io/customfilereader.cpp:47-340): its mesh file (path relative to the
- * XML file), materials (single-layer lambertian / mirror; other bxdfs and
- * layered materials give RB_ERR_UNSUPPORTED), object -> material
+ * XML file), materials (single-layer lambertian, mirror, glass, orennayar,
+ * Blinn microfacet; layered materials give RB_ERR_UNSUPPORTED), object -> material
  * assignments, diffuse lights and, when `cam` is not NULL, the camera for a
  * width x height film (fov from <fov>, 90 degrees when absent). */
 rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint32_t height, rb_camera* cam);

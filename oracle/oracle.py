@@ -176,6 +176,17 @@ def camera_matrices(pos, at, up, hfov, width, height):
     return r2c, c2w
 
 
+def _pack_materials(materials):
+    """(kind, rgb[, param8]) per material -> float32[n, 12]: kind bits, rgb, params."""
+    m = np.zeros((max(len(materials), 1), 12), np.float32)
+    for i, mat in enumerate(materials):
+        m[i, 0] = np.array([mat[0]], np.uint32).view(np.float32)[0]
+        m[i, 1:4] = mat[1]
+        if len(mat) > 2:
+            m[i, 4:4 + len(mat[2])] = mat[2]
+    return m
+
+
 FILM_DEFAULT = (0, 0.5, 0.5, 2.0)  # box of half-width 0.5: the mean of the pixel's samples
 
 
@@ -218,10 +229,7 @@ def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, 
     idx = np.ascontiguousarray(idx, np.uint32)
     tri_mat = np.ascontiguousarray(tri_mat, np.uint32)
     lights = np.ascontiguousarray(lights, np.uint32)
-    m4 = np.zeros((max(len(materials), 1), 4), np.float32)
-    for i, (k, col) in enumerate(materials):
-        m4[i, 0] = np.array([k], np.uint32).view(np.float32)[0]
-        m4[i, 1:] = col
+    m4 = _pack_materials(materials)
     r2c, c2w = camera_matrices(*cam)
     x0, y0, w, h = tile
     out = np.empty((h, w, 3), np.float32)
@@ -243,10 +251,7 @@ def render_bdpt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp
     idx = np.ascontiguousarray(idx, np.uint32)
     tri_mat = np.ascontiguousarray(tri_mat, np.uint32)
     lights = np.ascontiguousarray(lights, np.uint32)
-    m4 = np.zeros((max(len(materials), 1), 4), np.float32)
-    for i, (k, col) in enumerate(materials):
-        m4[i, 0] = np.array([k], np.uint32).view(np.float32)[0]
-        m4[i, 1:] = col
+    m4 = _pack_materials(materials)
     r2c, c2w = camera_matrices(*cam)
     x0, y0, w, h = tile
     out = np.empty((h, w, 3), np.float32)
@@ -266,6 +271,33 @@ def bdpt_techniques(mask, n, max_depth, lvc_depth):
     L.orc_bdpt_techniques.restype = C.c_uint32
     L.orc_bdpt_techniques.argtypes = [C.c_uint32] * 4
     return int(L.orc_bdpt_techniques(mask, n, max_depth, lvc_depth))
+
+
+def powf(x, y):
+    L = lib()
+    L.orc_powf.restype = C.c_float
+    L.orc_powf.argtypes = [C.c_float, C.c_float]
+    return float(L.orc_powf(x, y))
+
+
+def bsdf_eval(material, wo, wi, ng):
+    """material = (kind, rgb, param8); returns (f rgb, pdf)."""
+    m = _pack_materials([material])
+    out = np.empty(4, np.float32)
+    a = [np.ascontiguousarray(v, np.float32) for v in (wo, wi, ng)]
+    lib().orc_bsdf_eval.argtypes = [C.c_void_p] * 5
+    lib().orc_bsdf_eval(_p(m), _p(a[0]), _p(a[1]), _p(a[2]), _p(out))
+    return out[:3].copy(), float(out[3])
+
+
+def bsdf_sample(material, wo, ng, u1, u2):
+    """returns (wi, f or throughput weight for delta materials, pdf)."""
+    m = _pack_materials([material])
+    out = np.empty(7, np.float32)
+    a = [np.ascontiguousarray(v, np.float32) for v in (wo, ng)]
+    lib().orc_bsdf_sample.argtypes = [C.c_void_p] * 3 + [C.c_float, C.c_float, C.c_void_p]
+    lib().orc_bsdf_sample(_p(m), _p(a[0]), _p(a[1]), u1, u2, _p(out))
+    return out[:3].copy(), out[3:6].copy(), float(out[6])
 
 
 def bytes_per_ray(counts, hit):

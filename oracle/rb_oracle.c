@@ -1093,7 +1093,20 @@ void orc_woop_hits(const float* verts, const uint32_t* idx, const int32_t* ids, 
 /* (src/renderers/samplerrenderer.cu:103-169 with the box filter).           */
 /* ------------------------------------------------------------------------ */
 
-typedef struct { uint32_t kind; float color[3]; } orc_material;
+typedef struct { uint32_t kind; float color[3]; float param[8]; } orc_material;
+
+static orc_material* mats_from12(const float* m12, uint32_t nmats)
+{
+    orc_material* mats = (orc_material*)calloc(nmats ? nmats : 1, sizeof(orc_material));
+    for (uint32_t i = 0; i < nmats; i++) {
+        uint32_t k; memcpy(&k, &m12[12 * i], 4);
+        mats[i].kind = k;
+        for (int c = 0; c < 3; c++) mats[i].color[c] = m12[12 * i + 1 + c];
+        for (int c = 0; c < 8; c++) mats[i].param[c] = m12[12 * i + 4 + c];
+    }
+    return mats;
+}
+
 
 typedef struct {
     const uint32_t *nodes, *woop, *tidx;      /* Compact2 */
@@ -1278,11 +1291,285 @@ static float light_pdf_mt(f3 p, f3 d, f3 v0, f3 v1, f3 v2, f3 ln, float area)
 
 static int black3(f3 c) { return c.x == 0.0f && c.y == 0.0f && c.z == 0.0f; }
 
+/* ------------------------------------------------------------------------ */
+/* pow for the BSDFs without a math library: 2^(y log2 x), log2 from        */
+/* 2 atanh((m-1)/(m+1)) on m in [sqrt(1/2), sqrt(2)), 2^f from its Taylor   */
+/* series on [-1/2, 1/2]; the renderer's kernels use the same operations.   */
+/* ------------------------------------------------------------------------ */
+
+static float log2_pos(float x)
+{
+    uint32_t u = f2u(x);
+    int e = 0;
+    if (u < 0x00800000u) { x = x * 16777216.0f; u = f2u(x); e = -24; }
+    e += (int)(u >> 23) - 127;
+    float m = u2f((u & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f) { m = m * 0.5f; e += 1; }
+    const float t = (m - 1.0f) / (m + 1.0f);
+    const float t2 = t * t;
+    float q = fmaf(t2, 0.222222222f, 0.285714286f);
+    q = fmaf(t2, q, 0.4f);
+    q = fmaf(t2, q, 0.666666667f);
+    q = fmaf(t2, q, 2.0f);
+    return fmaf(t * q, 1.44269504f, (float)e);
+}
+
+static float exp2_series(float y)
+{
+    if (y != y) return y;
+    if (y >= 128.0f) return u2f(0x7F800000u);
+    if (y < -150.0f) return 0.0f;
+    const float k = floorf(y + 0.5f);
+    const float z = (y - k) * 0.693147181f;
+    float q = fmaf(z, 1.98412698e-4f, 1.38888889e-3f);
+    q = fmaf(z, q, 8.33333333e-3f);
+    q = fmaf(z, q, 4.16666667e-2f);
+    q = fmaf(z, q, 0.166666667f);
+    q = fmaf(z, q, 0.5f);
+    q = fmaf(z, q, 1.0f);
+    q = fmaf(z, q, 1.0f);
+    int ki = (int)k;
+    if (ki > 127) { q = q * 2.0f; ki = 127; }
+    if (ki < -126) { q = q * 5.96046448e-8f; ki += 24; }
+    return q * u2f((uint32_t)(ki + 127) << 23);
+}
+
+float orc_powf(float x, float y)
+{
+    if (x > 0.0f) return exp2_series(y * log2_pos(x));
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : u2f(0x7F800000u));
+    return u2f(0x7FC00000u);
+}
+
+/* ------------------------------------------------------------------------ */
+/* BSDFs (src/core/reflection.cu:42-300, one layer per material).  Non-delta */
+/* ones reflect only, in the Duff et al. frame of the normal facing wo.      */
+/* ------------------------------------------------------------------------ */
+
+static int delta_mat(const orc_material* m) { return m->kind == 1u || m->kind == 3u; }
+
+typedef struct { f3 b1, b2, n; } frame3;
+static frame3 frame_of(f3 n)
+{
+    frame3 f;
+    const float sign = copysignf(1.0f, n.z);
+    const float a = -1.0f / (sign + n.z);
+    const float b = n.x * n.y * a;
+    f.b1 = mk3(1.0f + sign * n.x * n.x * a, sign * b, -sign * n.x);
+    f.b2 = mk3(b, sign + n.y * n.y * a, -n.y);
+    f.n = n;
+    return f;
+}
+static f3 to_local(const frame3* f, f3 v) { return mk3(dot3(v, f->b1), dot3(v, f->b2), dot3(v, f->n)); }
+static f3 to_world(const frame3* f, f3 l) { return add3(add3(mul3s(f->b1, l.x), mul3s(f->b2, l.y)), mul3s(f->n, l.z)); }
+
+static float sin_theta_l(f3 w) { float s2 = 1.0f - w.z * w.z; return sqrtf(s2 > 0.0f ? s2 : 0.0f); }
+static float cos_phi_l(f3 w)
+{
+    float st = sin_theta_l(w);
+    if (st == 0.0f) return 1.0f;
+    float c = w.x / st;
+    return c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
+}
+static float sin_phi_l(f3 w)
+{
+    float st = sin_theta_l(w);
+    if (st == 0.0f) return 0.0f;
+    float c = w.y / st;
+    return c < -1.0f ? -1.0f : (c > 1.0f ? 1.0f : c);
+}
+
+/* evaluate_orennayar, reflection.cu:144-178 (A = param[1], B = param[2]) */
+static f3 orennayar_local(f3 wi, f3 wo, const orc_material* m)
+{
+    float sti = sin_theta_l(wi), sto = sin_theta_l(wo);
+    float maxcos = 0.0f;
+    if (sti > 1.0e-4f && sto > 1.0e-4f) {
+        float dcos = cos_phi_l(wi) * cos_phi_l(wo) + sin_phi_l(wi) * sin_phi_l(wo);
+        maxcos = dcos > 0.0f ? dcos : 0.0f;
+    }
+    float sinalpha, tanbeta;
+    if (fabsf(wi.z) > fabsf(wo.z)) { sinalpha = sto; tanbeta = sti / fabsf(wi.z); }
+    else { sinalpha = sti; tanbeta = sto / fabsf(wo.z); }
+    float k = m->param[1] + m->param[2] * maxcos * sinalpha * tanbeta;
+    float ip = 0.318309886183790672f;
+    return mk3(m->color[0] * ip * k, m->color[1] * ip * k, m->color[2] * ip * k);
+}
+
+/* Fresnel::evaluate, src/core/reflection.h:147-217 */
+static f3 fresnel_layer(const orc_material* m, float cosi)
+{
+    if (m->param[1] == 1.0f) {
+        float rc[3];
+        for (int c = 0; c < 3; c++) {
+            float eta = m->param[2 + c], k = m->param[5 + c];
+            float temp = (eta * eta + k * k) * cosi * cosi;
+            float rparl2 = (temp - 2.0f * eta * cosi + 1.0f) / (temp + 2.0f * eta * cosi + 1.0f);
+            float tf = eta * eta + k * k;
+            float rperp2 = (tf - 2.0f * eta * cosi + cosi * cosi) / (tf + 2.0f * eta * cosi + cosi * cosi);
+            rc[c] = (rparl2 + rperp2) / 2.0f;
+        }
+        return mk3(rc[0], rc[1], rc[2]);
+    }
+    if (m->param[1] == 2.0f) {
+        float ei = m->param[2], et = m->param[3];
+        if (!(cosi > 0.0f)) { float t = ei; ei = et; et = t; }
+        float c2 = 1.0f - cosi * cosi;
+        float sint = ei / et * sqrtf(c2 > 0.0f ? c2 : 0.0f);
+        if (sint >= 1.0f) return mk3(1.0f, 1.0f, 1.0f);
+        float t2 = 1.0f - sint * sint;
+        float cost = sqrtf(t2 > 0.0f ? t2 : 0.0f);
+        float ci = fabsf(cosi);
+        float rparl = ((et * ci) - (ei * cost)) / ((et * ci) + (ei * cost));
+        float rperp = ((ei * ci) - (et * cost)) / ((ei * ci) + (et * cost));
+        float f = (rparl * rparl + rperp * rperp) / 2.0f;
+        return mk3(f, f, f);
+    }
+    return mk3(1.0f, 1.0f, 1.0f);
+}
+
+/* Blinn D, Torrance-Sparrow G with both cosines (reflection.cu:198-251) */
+static float blinn_d(f3 wh, float e) { return (e + 2.0f) * 0.159154943091895336f * orc_powf(fabsf(wh.z), e); }
+static float micro_g(f3 wi, f3 wo, f3 wh)
+{
+    float nh = fabsf(wh.z), no = fabsf(wo.z), ni = fabsf(wi.z), oh = fabsf(dot3(wo, wh));
+    float g1 = 2.0f * nh * no / oh, g2 = 2.0f * nh * ni / oh;
+    float g = g1 < g2 ? g1 : g2;
+    return g < 1.0f ? g : 1.0f;
+}
+static f3 microfacet_local(f3 wi, f3 wo, const orc_material* m)
+{
+    float ci = fabsf(wi.z), co = fabsf(wo.z);
+    if (ci == 0.0f || co == 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+    f3 wh = add3(wi, wo);
+    if (wh.x == 0.0f && wh.y == 0.0f && wh.z == 0.0f) return mk3(0.0f, 0.0f, 0.0f);
+    wh = normalize3(wh);
+    f3 F = fresnel_layer(m, dot3(wi, wh));
+    float D = blinn_d(wh, m->param[0]), G = micro_g(wi, wo, wh);
+    float den = 4.0f * ci * co;
+    return mk3(m->color[0] * D * G * F.x / den, m->color[1] * D * G * F.y / den, m->color[2] * D * G * F.z / den);
+}
+static float microfacet_pdf_local(f3 wi, f3 wo, float e)
+{
+    if (!(wi.z * wo.z > 0.0f)) return 0.0f;
+    f3 wh = normalize3(add3(wo, wi));
+    float d = dot3(wo, wh);
+    if (d <= 0.0f) return 0.0f;
+    return ((e + 1.0f) * orc_powf(fabsf(wh.z), e)) / (6.28318531f * 4.0f * d);
+}
+
+
 static f3 lambert_f3(f3 wo, f3 wi, f3 ng, const orc_material* m)
 {
     if (!(dot3(wo, ng) * dot3(wi, ng) > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
     const float k = 0.318309886183790672f;
     return mk3(m->color[0] * k, m->color[1] * k, m->color[2] * k);
+}
+
+static f3 bsdf_f3(const orc_material* m, f3 wo, f3 wi, f3 ng, f3 nf)
+{
+    if (m->kind == 0u) return lambert_f3(wo, wi, ng, m);
+    if (delta_mat(m) || m->kind == 2u) return mk3(0.0f, 0.0f, 0.0f);
+    if (!(dot3(wo, ng) * dot3(wi, ng) > 0.0f)) return mk3(0.0f, 0.0f, 0.0f);
+    frame3 fr = frame_of(nf);
+    f3 wil = to_local(&fr, wi), wol = to_local(&fr, wo);
+    return m->kind == 4u ? orennayar_local(wil, wol, m) : microfacet_local(wil, wol, m);
+}
+
+static float bsdf_pdf3(const orc_material* m, f3 wo, f3 wi, f3 nf)
+{
+    if (m->kind == 5u) {
+        frame3 fr = frame_of(nf);
+        return microfacet_pdf_local(to_local(&fr, wi), to_local(&fr, wo), m->param[0]);
+    }
+    return fabsf(dot3(wi, nf)) * 0.318309886183790672f;
+}
+
+static void bsdf_sample3(const orc_material* m, f3 wo, f3 ng, f3 nf, float u1, float u2, f3* wi, f3* f, float* pdf)
+{
+    if (m->kind != 5u) {
+        *wi = cosine_dir(nf, u1, u2);
+        *f = bsdf_f3(m, wo, *wi, ng, nf);
+        *pdf = fabsf(dot3(*wi, nf)) * 0.318309886183790672f;
+        return;
+    }
+    /* sample_blinn_microfacet, reflection.cu:208-225 */
+    frame3 fr = frame_of(nf);
+    f3 wol = to_local(&fr, wo);
+    float e = m->param[0];
+    float ct = orc_powf(u1, 1.0f / (e + 1.0f));
+    float s2 = 1.0f - ct * ct;
+    float st = sqrtf(s2 > 0.0f ? s2 : 0.0f);
+    float sp, cp;
+    orc_sincos_2pi(u2, &sp, &cp);
+    f3 wh = mk3(st * cp, st * sp, ct);
+    if (!(wol.z * wh.z > 0.0f)) wh = mk3(-wh.x, -wh.y, -wh.z);
+    float d = dot3(wol, wh);
+    f3 wil = add3(mk3(-wol.x, -wol.y, -wol.z), mul3s(wh, 2.0f * d));
+    *pdf = d > 0.0f ? ((e + 1.0f) * orc_powf(ct, e)) / (6.28318531f * 4.0f * d) : 0.0f;
+    *wi = normalize3(to_world(&fr, wil));
+    *f = (wil.z * wol.z > 0.0f) ? bsdf_f3(m, wo, *wi, ng, nf) : mk3(0.0f, 0.0f, 0.0f);
+}
+
+/* Glass: Fresnel-weighted reflection / refraction about the outward normal
+ * ng (sample_glass, reflection.cu:109-140, plus the reflection branch);
+ * returns f |cos| / pdf; radiance transport scales refraction by
+ * (eta_wo / eta_wi)^2. */
+static f3 glass_sample3(const orc_material* m, f3 wo, f3 ng, float u, int radiance, f3* wi)
+{
+    frame3 fr = frame_of(ng);
+    f3 wol = to_local(&fr, wo);
+    float cosi = wol.z;
+    int entering = cosi > 0.0f;
+    float ei = entering ? m->param[0] : m->param[1], et = entering ? m->param[1] : m->param[0];
+    orc_material fm = *m;
+    fm.param[1] = 2.0f;
+    fm.param[2] = m->param[0];
+    fm.param[3] = m->param[1];
+    float F = fresnel_layer(&fm, cosi).x;
+    float s2 = 1.0f - cosi * cosi;
+    float eta = ei / et;
+    float st2 = eta * eta * (s2 > 0.0f ? s2 : 0.0f);
+    f3 wil, w;
+    if (u < F || st2 >= 1.0f) {
+        wil = mk3(-wol.x, -wol.y, wol.z);
+        w = mk3(m->color[0], m->color[1], m->color[2]);
+    } else {
+        float c2 = 1.0f - st2;
+        float cost = sqrtf(c2 > 0.0f ? c2 : 0.0f);
+        if (entering) cost = -cost;
+        wil = mk3(eta * -wol.x, eta * -wol.y, cost);
+        float sc = radiance ? (ei * ei) / (et * et) : 1.0f;
+        w = mk3(m->color[0] * sc, m->color[1] * sc, m->color[2] * sc);
+    }
+    *wi = normalize3(to_world(&fr, wil));
+    return w;
+}
+
+/* BSDF value of a material for the tests: wo, wi, ng world; nf = ng facing wo. */
+void orc_bsdf_eval(const float* m12, const float* wo, const float* wi, const float* ng, float* out)
+{
+    orc_material* m = mats_from12(m12, 1);
+    f3 o = mk3(wo[0], wo[1], wo[2]), i = mk3(wi[0], wi[1], wi[2]), n = mk3(ng[0], ng[1], ng[2]);
+    f3 nf = dot3(n, o) >= 0.0f ? n : mk3(-n.x, -n.y, -n.z);
+    f3 f = bsdf_f3(m, o, i, n, nf);
+    out[0] = f.x; out[1] = f.y; out[2] = f.z;
+    out[3] = bsdf_pdf3(m, o, i, nf);
+    free(m);
+}
+
+/* One BSDF sample: out = wi.xyz, f.rgb, pdf (delta materials: f = weight, pdf = 1). */
+void orc_bsdf_sample(const float* m12, const float* wo, const float* ng, float u1, float u2, float* out)
+{
+    orc_material* m = mats_from12(m12, 1);
+    f3 o = mk3(wo[0], wo[1], wo[2]), n = mk3(ng[0], ng[1], ng[2]);
+    f3 nf = dot3(n, o) >= 0.0f ? n : mk3(-n.x, -n.y, -n.z);
+    f3 wi, f;
+    float pdf = 1.0f;
+    if (m->kind == 3u) f = glass_sample3(m, o, n, u1, 1, &wi);
+    else bsdf_sample3(m, o, n, nf, u1, u2, &wi, &f, &pdf);
+    out[0] = wi.x; out[1] = wi.y; out[2] = wi.z; out[3] = f.x; out[4] = f.y; out[5] = f.z; out[6] = pdf;
+    free(m);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1352,7 +1639,7 @@ static f3 uniform_dir(f3 n, float u1, float u2)
  * T) to the cache (integrator_bdpt_lvc_sk.cu:343-386); each unoccluded one
  * is added to L in connection order. */
 static f3 lvc_connect(const orc_scene* S, const orc_lvc* B, uint32_t gid, uint32_t bounce, uint32_t max_depth, f3 p,
-                      float eps, f3 wo, f3 ng, const orc_material* m, f3 T, uint32_t cm, f3 L)
+                      float eps, f3 wo, f3 ng, f3 nf, const orc_material* m, f3 T, uint32_t cm, f3 L)
 {
     for (uint32_t i = 0; i < B->nconn; i++) {
         uint32_t rc[4];
@@ -1375,8 +1662,9 @@ static f3 lvc_connect(const orc_scene* S, const orc_lvc* B, uint32_t gid, uint32
         float dist = sqrtf(d2);
         f3 dir = normalize3(seg);
         f3 ndir = mk3(-dir.x, -dir.y, -dir.z);
-        f3 fy = lambert_f3(wo, dir, ng, m);
-        f3 fx = mx->kind == 0u ? lambert_f3(xw, ndir, xn, mx) : mk3(0.0f, 0.0f, 0.0f);
+        f3 fy = bsdf_f3(m, wo, dir, ng, nf);
+        f3 nfx = dot3(xn, xw) >= 0.0f ? xn : mk3(-xn.x, -xn.y, -xn.z);
+        f3 fx = bsdf_f3(mx, xw, ndir, xn, nfx);
         if (black3(fy) || black3(fx)) continue;
         float G = fabsf(dot3(ng, dir)) * fabsf(dot3(xn, ndir)) / d2;
         /* full path: camera vertices of bounces 0..bounce, then the light
@@ -1431,8 +1719,8 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
         uint32_t r0[4], r1[4];
         rng_draw(rng, gid, bounce, 0, 0, r0);
         rng_draw(rng, gid, bounce, 1, 0, r1);
-        int lambert = m->kind == 0u;
-        if (B && !lambert) cm |= 1u << bounce;
+        int delta = delta_mat(m);
+        if (B && delta) cm |= 1u << bounce;
         float wdir = B ? bdpt_weight(cm, bounce + 2, max_depth, B->nconn ? B->depth : 0u) : 1.0f;
         f3 Ld = mk3(0.0f, 0.0f, 0.0f);
         if (S->nl > 0) {
@@ -1453,10 +1741,10 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
             f3 wi = normalize3(sub3(pl, p));
             float lpdf = light_pdf_mt(p, wi, l0, l1, l2, ln, area);
             f3 Li = dot3(ln, mk3(-wi.x, -wi.y, -wi.z)) > 0.0f ? emit : mk3(0.0f, 0.0f, 0.0f);
-            if (lambert && !black3(Li) && lpdf > 0.0f) {
-                f3 f = lambert_f3(wo, wi, ng, m);
+            if (!delta && !black3(Li) && lpdf > 0.0f) {
+                f3 f = bsdf_f3(m, wo, wi, ng, nf);
                 if (!black3(f)) {
-                    float bpdf = fabsf(dot3(wi, nf)) * 0.318309886183790672f;
+                    float bpdf = bsdf_pdf3(m, wo, wi, nf);
                     float w = lpdf / (lpdf + bpdf);
                     float s = fabsf(dot3(wi, ng)) * w;
                     f3 ca = mk3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
@@ -1469,10 +1757,10 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
                     if (sid < 0) Ld = add3(Ld, ca);
                 }
             }
-            if (lambert) {
-                f3 wi2 = cosine_dir(nf, u01(r0[3]), u01(r1[0]));
-                f3 f2 = lambert_f3(wo, wi2, ng, m);
-                float pdf2 = fabsf(dot3(wi2, nf)) * 0.318309886183790672f;
+            if (!delta) {
+                f3 wi2, f2;
+                float pdf2;
+                bsdf_sample3(m, wo, ng, nf, u01(r0[3]), u01(r1[0]), &wi2, &f2, &pdf2);
                 if (!black3(f2) && pdf2 > 0.0f) {
                     float lpdf2 = light_pdf_mt(p, wi2, l0, l1, l2, ln, area);
                     if (lpdf2 > 0.0f) {
@@ -1491,16 +1779,20 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
         Ld = mul3s(Ld, (float)S->nl);
         if (B) Ld = mul3s(Ld, wdir);
         L = mk3(L.x + T.x * Ld.x, L.y + T.y * Ld.y, L.z + T.z * Ld.z);
-        if (B && lambert && S->nl > 0) L = lvc_connect(S, B, gid, bounce, max_depth, p, eps, wo, ng, m, T, cm, L);
+        if (B && !delta && S->nl > 0) L = lvc_connect(S, B, gid, bounce, max_depth, p, eps, wo, ng, nf, m, T, cm, L);
         /* next bounce */
         f3 wi3, f3v;
         float pdf3;
         int spec3;
-        if (lambert) {
-            wi3 = cosine_dir(nf, u01(r1[1]), u01(r1[2]));
-            f3v = lambert_f3(wo, wi3, ng, m);
-            pdf3 = fabsf(dot3(wi3, nf)) * 0.318309886183790672f;
+        int weight3 = 0; /* f3v already holds f |cos| / pdf */
+        if (!delta) {
+            bsdf_sample3(m, wo, ng, nf, u01(r1[1]), u01(r1[2]), &wi3, &f3v, &pdf3);
             spec3 = 0;
+        } else if (m->kind == 3u) {
+            f3v = glass_sample3(m, wo, ng, u01(r1[1]), 1, &wi3);
+            pdf3 = 1.0f;
+            spec3 = 1;
+            weight3 = 1;
         } else {
             float c = 2.0f * dot3(nf, wo);
             wi3 = sub3(mul3s(nf, c), wo);
@@ -1510,7 +1802,7 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
             spec3 = 1;
         }
         if (black3(f3v) || !(pdf3 > 0.0f)) break;
-        float s3 = fabsf(dot3(wi3, ng));
+        float s3 = weight3 ? 1.0f : fabsf(dot3(wi3, ng));
         f3 Tn = mk3(T.x * (f3v.x * s3 / pdf3), T.y * (f3v.y * s3 / pdf3), T.z * (f3v.z * s3 / pdf3));
         int alive = 1;
         if (bounce > rr_start) {
@@ -1531,16 +1823,6 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
     return L;
 }
 
-static orc_material* mats_from4(const float* mats4, uint32_t nmats)
-{
-    orc_material* mats = (orc_material*)malloc(sizeof(orc_material) * (nmats ? nmats : 1));
-    for (uint32_t i = 0; i < nmats; i++) {
-        uint32_t k; memcpy(&k, &mats4[4 * i], 4);
-        mats[i].kind = k;
-        mats[i].color[0] = mats4[4 * i + 1]; mats[i].color[1] = mats4[4 * i + 2]; mats[i].color[2] = mats4[4 * i + 3];
-    }
-    return mats;
-}
 
 void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
                    const uint32_t* idx, const uint32_t* tri_mat, const float* mats4, uint32_t nmats,
@@ -1549,7 +1831,7 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                    uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t filter, float fxw, float fyw,
                    float falpha, float* out_rgb)
 {
-    orc_material* mats = mats_from4(mats4, nmats);
+    orc_material* mats = mats_from12(mats4, nmats);
     orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
@@ -1617,8 +1899,8 @@ static void light_path(const orc_scene* S, orc_rng rng, uint32_t j, uint32_t D, 
         if (m->kind == 2u) return;
         f3 ng = tri_norm(v0, v1, v2);
         f3 wo = mk3(-dv.x, -dv.y, -dv.z);
-        int mirror = m->kind == 1u;
-        if (mirror) mask |= 1u << d;
+        int delta = delta_mat(m);
+        if (delta) mask |= 1u << d;
         size_t sl = (size_t)j * D + d;
         pos[3 * sl] = p.x; pos[3 * sl + 1] = p.y; pos[3 * sl + 2] = p.z;
         ngo[3 * sl] = ng.x; ngo[3 * sl + 1] = ng.y; ngo[3 * sl + 2] = ng.z;
@@ -1633,18 +1915,21 @@ static void light_path(const orc_scene* S, orc_rng rng, uint32_t j, uint32_t D, 
         f3 nf = dot3(ng, wo) >= 0.0f ? ng : mk3(-ng.x, -ng.y, -ng.z);
         f3 wi, f;
         float bp;
-        if (mirror) {
+        int weight = 0; /* f already holds f |cos| / pdf */
+        if (m->kind == 1u) {
             wi = sub3(mul3s(nf, 2.0f * dot3(nf, wo)), wo);
             float ac = fabsf(dot3(wi, nf));
             f = mk3(m->color[0] / ac, m->color[1] / ac, m->color[2] / ac);
             bp = 1.0f;
+        } else if (m->kind == 3u) {
+            f = glass_sample3(m, wo, ng, u01(rr[0]), 0, &wi);
+            bp = 1.0f;
+            weight = 1;
         } else {
-            wi = cosine_dir(nf, u01(rr[0]), u01(rr[1]));
-            f = lambert_f3(wo, wi, ng, m);
-            bp = fabsf(dot3(wi, nf)) * 0.318309886183790672f;
+            bsdf_sample3(m, wo, ng, nf, u01(rr[0]), u01(rr[1]), &wi, &f, &bp);
         }
         if (black3(f) || !(bp > 0.0f)) return;
-        float s = fabsf(dot3(ng, wi));
+        float s = weight ? 1.0f : fabsf(dot3(ng, wi));
         f3 tmp = mk3(f.x * s / bp, f.y * s / bp, f.z * s / bp);
         float lum = 0.212671f * tmp.x + 0.715160f * tmp.y + 0.072169f * tmp.z;
         float cp = lum < 1.0f ? lum : 1.0f;
@@ -1665,7 +1950,7 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
                      uint32_t light_max_depth, uint32_t connections, uint32_t filter, float fxw, float fyw,
                      float falpha, float* out_rgb)
 {
-    orc_material* mats = mats_from4(mats4, nmats);
+    orc_material* mats = mats_from12(mats4, nmats);
     orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
     const uint32_t NL = light_paths, D = light_max_depth;
     size_t slots = (size_t)NL * D;

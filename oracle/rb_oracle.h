@@ -145,6 +145,13 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                    uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t filter, float fxw, float fyw,
                    float falpha, float* out_rgb);
+/* x^y from log2 / exp2 series (no math library): the renderer's pow. */
+float orc_powf(float x, float y);
+/* BSDF of one material (m12: kind bits, rgb, param[8] as rb_material) for
+ * wo -> wi about the geometric normal ng: out = f.rgb, pdf. */
+void orc_bsdf_eval(const float* m12, const float* wo, const float* wi, const float* ng, float* out);
+/* One BSDF sample from (u1, u2): out = wi.xyz, weight or f .rgb, pdf. */
+void orc_bsdf_sample(const float* m12, const float* wo, const float* ng, float u1, float u2, float* out);
 /* Number of BDPT techniques able to build a path z_1..z_n (mask bit i-1 =
  * z_i specular) when camera vertices exist for bounces 0..max_depth and the
  * light-vertex cache holds depths 0..lvc_depth-1; a path's weight is 1/that. */

@@ -17,7 +17,8 @@ SZ = C.c_size_t
 RB_OK, RB_ERR_INVALID, RB_ERR_OOM, RB_ERR_DEVICE, RB_ERR_BUSY, RB_ERR_UNSUPPORTED, RB_ERR_NOT_READY = range(7)
 RB_CLOSEST, RB_ANY, RB_MASKED = 0, 1, 2
 RB_LAYOUT_REF_COMPACT2, RB_LAYOUT_MI355X = 1, 2
-RB_MAT_LAMBERT, RB_MAT_MIRROR, RB_MAT_EMITTER = 0, 1, 2
+RB_MAT_LAMBERT, RB_MAT_MIRROR, RB_MAT_EMITTER, RB_MAT_GLASS, RB_MAT_ORENNAYAR, RB_MAT_MICROFACET = range(6)
+RB_FRESNEL_NONE, RB_FRESNEL_CONDUCTOR, RB_FRESNEL_DIELECTRIC = 0, 1, 2
 RB_GEN_ICOSPHERES, RB_GEN_THIN_QUADS, RB_GEN_ROOM, RB_GEN_EMITTER_QUADS = 0, 1, 2, 3
 
 
@@ -32,7 +33,7 @@ _STATUS = {0: "RB_OK", 1: "RB_ERR_INVALID", 2: "RB_ERR_OOM", 3: "RB_ERR_DEVICE",
 
 
 class rb_material(C.Structure):
-    _fields_ = [("kind", U32), ("color", F32 * 3)]
+    _fields_ = [("kind", U32), ("color", F32 * 3), ("param", F32 * 8)]
 
 
 class rb_gen_params(C.Structure):

@@ -132,11 +132,27 @@ class Scene:
     def handle(self):
         return self._h
 
-    def add_material(self, kind, color):
-        m = L.rb_material(kind, (C.c_float * 3)(*color))
+    def add_material(self, kind, color, param=()):
+        m = L.rb_material(kind, (C.c_float * 3)(*color), (C.c_float * 8)(*param))
         mid = C.c_uint32()
         check(lib().rb_scene_add_material(self._h, C.byref(m), C.byref(mid)))
         return mid.value
+
+    def add_glass_bsdf(self, color, etai=1.0, etat=1.5):
+        return self.add_material(L.RB_MAT_GLASS, color, (etai, etat))
+
+    def add_orennayar_bsdf(self, color, sigma_deg):
+        return self.add_material(L.RB_MAT_ORENNAYAR, color, (sigma_deg,))
+
+    def add_microfacet_bsdf(self, color, exponent, fresnel=None):
+        """fresnel: None, ("conductor", eta_rgb, k_rgb) or ("dielectric", etai, etat)."""
+        if fresnel is None:
+            p = (exponent, L.RB_FRESNEL_NONE)
+        elif fresnel[0] == "conductor":
+            p = (exponent, L.RB_FRESNEL_CONDUCTOR, *fresnel[1], *fresnel[2])
+        else:
+            p = (exponent, L.RB_FRESNEL_DIELECTRIC, fresnel[1], fresnel[2])
+        return self.add_material(L.RB_MAT_MICROFACET, color, p)
 
     # Scene::add_diffuse_bsdf / add_mirror_bsdf / add_diffuse_light_bsdf
     def add_diffuse_bsdf(self, albedo):
@@ -198,7 +214,7 @@ class Scene:
         arr = (L.rb_material * nm)()
         if nm:
             check(lib().rb_scene_materials(self._h, arr))
-        return [(m.kind, tuple(m.color)) for m in arr]
+        return [(m.kind, tuple(m.color), tuple(m.param)) for m in arr]
 
     def get_scene_aabb(self):
         lo = np.empty(3, np.float32)

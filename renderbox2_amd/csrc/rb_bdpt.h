@@ -156,8 +156,8 @@ __global__ void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, RngArg rng_a, L
                     const V3 wo = vneg(dv);
                     const float4 a = P.alpha[j];
                     uint32_t mask = P.mask[j];
-                    const bool mirror = m.kind == RB_MAT_MIRROR;
-                    if (mirror) mask |= 1u << d;
+                    const bool delta = is_delta(m);
+                    if (delta) mask |= 1u << d;
                     const size_t slot = (size_t)j * C.depth + d;
                     C.pos[slot] = make_float4(p.x, p.y, p.z, 0.f);
                     C.ng[slot] = make_float4(ng.x, ng.y, ng.z, __uint_as_float(mid));
@@ -169,18 +169,21 @@ __global__ void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, RngArg rng_a, L
                         const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
                         V3 wi, f;
                         float pdf;
-                        if (mirror) {
+                        bool weight = false;  // f already holds f |cos| / pdf
+                        if (m.kind == RB_MAT_MIRROR) {
                             wi = vsub(vmul(nf, 2.0f * vdot(nf, wo)), wo);
                             const float ac = fabsf(vdot(wi, nf));
                             f = v3(m.color[0] / ac, m.color[1] / ac, m.color[2] / ac);
                             pdf = 1.0f;
+                        } else if (m.kind == RB_MAT_GLASS) {
+                            f = glass_sample(m, wo, ng, u01(r.x), false, &wi);
+                            pdf = 1.0f;
+                            weight = true;
                         } else {
-                            wi = cosine_hemisphere(nf, u01(r.x), u01(r.y));
-                            f = lambert_f(wo, wi, ng, m);
-                            pdf = fabsf(vdot(wi, nf)) * 0.318309886183790672f;
+                            bsdf_sample(m, wo, ng, nf, u01(r.x), u01(r.y), &wi, &f, &pdf);
                         }
                         if (!is_black(f) && pdf > 0.0f) {
-                            const float s = fabsf(vdot(ng, wi));
+                            const float s = weight ? 1.0f : fabsf(vdot(ng, wi));
                             const V3 tmp = v3(f.x * s / pdf, f.y * s / pdf, f.z * s / pdf);
                             const float lum = luminance(tmp);
                             const float cp = lum < 1.0f ? lum : 1.0f;

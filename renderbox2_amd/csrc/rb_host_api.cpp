@@ -87,8 +87,26 @@ rb_status rb_scene_destroy(rb_scene_t s) {
 
 rb_status rb_scene_add_material(rb_scene_t s, const rb_material* m, uint32_t* id) {
     RB_CHECK_ARG(s && m, "NULL scene or material");
-    RB_CHECK_ARG(m->kind <= RB_MAT_EMITTER, "unknown material kind");
-    s->s.mats.push_back(*m);
+    RB_CHECK_ARG(m->kind <= RB_MAT_MICROFACET, "unknown material kind");
+    rb_material c = *m;
+    if (c.kind == RB_MAT_GLASS) RB_CHECK_ARG(c.param[0] > 0.0f && c.param[1] > 0.0f, "glass needs etai, etat > 0");
+    if (c.kind == RB_MAT_ORENNAYAR) {
+        // A and B from sigma (CustomFileReader::process, src/io/customfilereader.cpp:137-146)
+        float sigma = c.param[0] < 0.0f ? 0.0f : (c.param[0] > 90.0f ? 90.0f : c.param[0]);
+        sigma = (3.14159265358979f / 180.0f) * sigma;
+        const float s2 = sigma * sigma;
+        c.param[1] = 1.0f - (s2 / (2.0f * (s2 + 0.33f)));
+        c.param[2] = 0.45f * s2 / (s2 + 0.09f);
+    }
+    if (c.kind == RB_MAT_MICROFACET) {
+        RB_CHECK_ARG(c.param[0] > 0.0f, "microfacet exponent must be > 0");
+        const float ft = c.param[1];
+        RB_CHECK_ARG(ft == RB_FRESNEL_NONE || ft == RB_FRESNEL_CONDUCTOR || ft == RB_FRESNEL_DIELECTRIC,
+                     "microfacet Fresnel type must be 0, 1 or 2");
+        if (ft == RB_FRESNEL_DIELECTRIC)
+            RB_CHECK_ARG(c.param[2] > 0.0f && c.param[3] > 0.0f, "dielectric etas must be > 0");
+    }
+    s->s.mats.push_back(c);
     if (id) *id = (uint32_t)(s->s.mats.size() - 1);
     return RB_OK;
 }

@@ -25,9 +25,9 @@
 // (<name> of an OBJ object, <Le>{r,g,b}</Le>) making that object a diffuse
 // emitter, and <camera> (<position>, <lookat>, <up>, <fov>).  Materials are
 // numbered in file order, then one emitter material per light, as the
-// reference numbers them.  Single-layer lambertian and mirror bxdfs map to
-// the renderer's materials; other bxdf types and layered materials are
-// refused with RB_ERR_UNSUPPORTED.  The camera's <fov> is used when present
+// reference numbers them.  Single-layer lambertian, mirror, glass,
+// orennayar and (Blinn) microfacet bxdfs map to the renderer's materials;
+// layered materials are refused with RB_ERR_UNSUPPORTED.  The camera's <fov> is used when present
 // (the reference hard-codes 90 degrees, customfilereader.cpp:331-337).
 #include <cstdio>
 #include <cstdlib>
@@ -337,15 +337,58 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
         if (bxdfs.size() != 1)
             return fail(RB_ERR_UNSUPPORTED, "%s: material '%s' has %zu layers; only single-layer materials are supported",
                         path, name->text.c_str(), bxdfs.size());
-        const XNode* type = bxdfs[0]->child("type");
-        const XNode* refl = bxdfs[0]->child("reflectance");
+        const XNode* bx = bxdfs[0];
+        const XNode* type = bx->child("type");
+        const XNode* refl = bx->child("reflectance");
         rb_material mat{};
         if (!type) return fail(RB_ERR_INVALID, "%s: material '%s' bxdf without <type>", path, name->text.c_str());
-        if (type->text == "lambertian") mat.kind = RB_MAT_LAMBERT;
-        else if (type->text == "mirror") mat.kind = RB_MAT_MIRROR;
-        else
-            return fail(RB_ERR_UNSUPPORTED, "%s: material '%s': bxdf type '%s' is not supported (lambertian, mirror)",
-                        path, name->text.c_str(), type->text.c_str());
+        auto num = [&](const char* tag, float* out) -> bool {
+            const XNode* e = bx->child(tag);
+            if (!e) return false;
+            std::istringstream is(e->text);
+            is.imbue(std::locale::classic());
+            return bool(is >> *out);
+        };
+        bool ok = true;
+        if (type->text == "lambertian") {
+            mat.kind = RB_MAT_LAMBERT;
+        } else if (type->text == "mirror") {
+            mat.kind = RB_MAT_MIRROR;
+        } else if (type->text == "glass") {  // customfilereader.cpp:160-174
+            mat.kind = RB_MAT_GLASS;
+            ok = num("etai", &mat.param[0]) && num("etat", &mat.param[1]);
+        } else if (type->text == "orennayar") {  // :130-146 (A, B derived when the material is added)
+            mat.kind = RB_MAT_ORENNAYAR;
+            ok = num("sigma", &mat.param[0]);
+        } else if (type->text == "microfacet") {  // :175-218
+            mat.kind = RB_MAT_MICROFACET;
+            const XNode* dist = bx->child("distribution");
+            if (dist && dist->text != "blinn")
+                return fail(RB_ERR_UNSUPPORTED, "%s: material '%s': microfacet distribution '%s' (blinn only)", path,
+                            name->text.c_str(), dist->text.c_str());
+            ok = num("exponent", &mat.param[0]);
+            const XNode* ft = bx->child("fresnel_type");
+            const std::string f = ft ? ft->text : "fresnel_noop";
+            if (f == "fresnel_conductor") {
+                mat.param[1] = (float)RB_FRESNEL_CONDUCTOR;
+                const XNode* eta = bx->child("eta");
+                const XNode* k = bx->child("k");
+                ok = ok && eta && k && parse_float3(eta->text, &mat.param[2]) && parse_float3(k->text, &mat.param[5]);
+            } else if (f == "fresnel_dielectric") {
+                mat.param[1] = (float)RB_FRESNEL_DIELECTRIC;
+                ok = ok && num("etai", &mat.param[2]) && num("etat", &mat.param[3]);
+            } else if (f == "fresnel_noop") {
+                mat.param[1] = (float)RB_FRESNEL_NONE;
+            } else {
+                return fail(RB_ERR_INVALID, "%s: material '%s': unknown fresnel_type '%s'", path, name->text.c_str(),
+                            f.c_str());
+            }
+        } else {
+            return fail(RB_ERR_UNSUPPORTED, "%s: material '%s': bxdf type '%s' is not supported", path,
+                        name->text.c_str(), type->text.c_str());
+        }
+        if (!ok) return fail(RB_ERR_INVALID, "%s: material '%s': missing or bad %s parameters", path,
+                             name->text.c_str(), type->text.c_str());
         if (!refl || !parse_float3(refl->text, mat.color))
             return fail(RB_ERR_INVALID, "%s: material '%s' needs a <reflectance> triple", path, name->text.c_str());
         by_name[name->text] = mat_base + (uint32_t)mats.size();
@@ -412,10 +455,11 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
         c.width = width;
         c.height = height;
     }
-    try {
-        S.mats.insert(S.mats.end(), mats.begin(), mats.end());
-    } catch (...) {
-        return fail(RB_ERR_OOM, "scene growth failed");
+    for (const rb_material& m : mats) {  // validation and derived parameters as for any material
+        if ((st = rb_scene_add_material(s, &m, nullptr)) != RB_OK) {
+            S.mats.resize(mat_base);
+            return st;
+        }
     }
     if ((st = add_obj_to_scene(S, verts, objs, obj_mat)) != RB_OK) {
         S.mats.resize(mat_base);

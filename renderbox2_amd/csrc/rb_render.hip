@@ -210,8 +210,8 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                 const U4 r0 = rng.draw(gid, bounce, 0, 0);
                 const U4 r1 = rng.draw(gid, bounce, 1, 0);
                 V3 ca = v3(0.f, 0.f, 0.f), cb = v3(0.f, 0.f, 0.f);
-                const bool lambert = m.kind == RB_MAT_LAMBERT;
-                if (BD && !lambert) cm |= 1u << bounce;
+                const bool delta = is_delta(m);
+                if (BD && delta) cm |= 1u << bounce;
                 const float wdir = BD ? technique_weight(cm, bounce + 2, max_depth, B.tdepth) : 1.0f;
                 if (S.n_lights > 0) {
                     uint32_t lid = (uint32_t)(u01(r0.x) * (float)S.n_lights);
@@ -230,10 +230,10 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                     const V3 wi = vnorm(vsub(pl, p));
                     const float lpdf = light_pdf(p, wi, l0, l1, l2, ln, area);
                     const V3 Li = vdot(ln, vneg(wi)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
-                    if (lambert && !is_black(Li) && lpdf > 0.0f) {
-                        const V3 f = lambert_f(wo, wi, ng, m);
+                    if (!delta && !is_black(Li) && lpdf > 0.0f) {
+                        const V3 f = bsdf_f(m, wo, wi, ng, nf);
                         if (!is_black(f)) {
-                            const float bpdf = fabsf(vdot(wi, nf)) * 0.318309886183790672f;
+                            const float bpdf = bsdf_pdf(m, wo, wi, nf);
                             const float w = lpdf / (lpdf + bpdf);
                             const float s = fabsf(vdot(wi, ng)) * w;
                             ca = v3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
@@ -246,10 +246,10 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                         }
                     }
                     // BSDF sample with MIS (non-specular components only)
-                    if (lambert) {
-                        const V3 wi2 = cosine_hemisphere(nf, u01(r0.w), u01(r1.x));
-                        const V3 f2 = lambert_f(wo, wi2, ng, m);
-                        const float pdf2 = fabsf(vdot(wi2, nf)) * 0.318309886183790672f;
+                    if (!delta) {
+                        V3 wi2, f2;
+                        float pdf2;
+                        bsdf_sample(m, wo, ng, nf, u01(r0.w), u01(r1.x), &wi2, &f2, &pdf2);
                         if (!is_black(f2) && pdf2 > 0.0f) {
                             const float lpdf2 = light_pdf(p, wi2, l0, l1, l2, ln, area);
                             if (lpdf2 > 0.0f) {
@@ -263,7 +263,7 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                         }
                     }
                 }
-                if (BD && lambert) {
+                if (BD && !delta) {
                     // connections to the light-vertex cache (integrator_bdpt_lvc_sk.cu:343-386)
                     for (uint32_t i = 0; i < B.nconn; i++) {
                         const U4 rc = crng.draw(gid, bounce, 2 + i / 2, 0);
@@ -283,9 +283,10 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                             const float dist = sqrtf(d2);
                             const V3 dir = vnorm(seg);
                             const V3 ngx = v3(xn.x, xn.y, xn.z);
-                            const V3 fy = lambert_f(wo, dir, ng, m);
-                            const V3 fx = mx.kind == RB_MAT_LAMBERT ? lambert_f(v3(xw.x, xw.y, xw.z), vneg(dir), ngx, mx)
-                                                                    : v3(0.f, 0.f, 0.f);
+                            const V3 fy = bsdf_f(m, wo, dir, ng, nf);
+                            const V3 xwv = v3(xw.x, xw.y, xw.z);
+                            const V3 nfx = vdot(ngx, xwv) >= 0.0f ? ngx : vneg(ngx);
+                            const V3 fx = bsdf_f(mx, xwv, vneg(dir), ngx, nfx);
                             if (!is_black(fy) && !is_black(fx)) {
                                 const float G = fabsf(vdot(ng, dir)) * fabsf(vdot(ngx, vneg(dir))) / d2;
                                 // full path: camera vertices 0..bounce, then light vertices k+1..1
@@ -310,11 +311,15 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                 V3 wi3, f3;
                 float pdf3;
                 bool spec3;
-                if (lambert) {
-                    wi3 = cosine_hemisphere(nf, u01(r1.y), u01(r1.z));
-                    f3 = lambert_f(wo, wi3, ng, m);
-                    pdf3 = fabsf(vdot(wi3, nf)) * 0.318309886183790672f;
+                bool weight3 = false;  // f3 already holds f |cos| / pdf
+                if (!delta) {
+                    bsdf_sample(m, wo, ng, nf, u01(r1.y), u01(r1.z), &wi3, &f3, &pdf3);
                     spec3 = false;
+                } else if (m.kind == RB_MAT_GLASS) {
+                    f3 = glass_sample(m, wo, ng, u01(r1.y), true, &wi3);
+                    pdf3 = 1.0f;
+                    spec3 = true;
+                    weight3 = true;
                 } else {  // mirror (sample_mirror, src/core/reflection.cu:76-92)
                     const float c = 2.0f * vdot(nf, wo);
                     wi3 = vsub(vmul(nf, c), wo);
@@ -325,7 +330,7 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                 }
                 uint32_t ninfo = info & 0x1FFu;
                 if (!is_black(f3) && pdf3 > 0.0f) {
-                    const float s3 = fabsf(vdot(wi3, ng));
+                    const float s3 = weight3 ? 1.0f : fabsf(vdot(wi3, ng));
                     V3 Tn = v3(Tv.x * (f3.x * s3 / pdf3), Tv.y * (f3.y * s3 / pdf3), Tv.z * (f3.z * s3 / pdf3));
                     bool alive = true;
                     if (bounce > rr_start) {

@@ -6,6 +6,7 @@
 // Built with -ffp-contract=off; every fused multiply-add is written out.
 #pragma once
 #include <math.h>
+#include <string.h>
 #include <stdint.h>
 
 #include "rb_rng.h"
@@ -80,6 +81,79 @@ RB_HD V3 cosine_hemisphere(V3 n, float u1, float u2) {
     V3 b1, b2;
     onb(n, &b1, &b2);
     return vadd(vadd(vmul(b1, x), vmul(b2, y)), vmul(n, z));
+}
+
+RB_HD uint32_t fbits_(float f) {
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    return u;
+}
+RB_HD float bitsf_(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+// log2 of a positive finite x: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(t) = 2 (t + t^3/3 + ... + t^9/9), t = (m - 1) / (m + 1),
+// |t| <= 0.1716 (truncation < 1e-9); subnormals rescaled by 2^24 first.
+RB_HD float log2_pos(float x) {
+    uint32_t u = fbits_(x);
+    int e = 0;
+    if (u < 0x00800000u) {
+        x = x * 16777216.0f;
+        u = fbits_(x);
+        e = -24;
+    }
+    e += (int)(u >> 23) - 127;
+    float m = bitsf_((u & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421356f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float t = (m - 1.0f) / (m + 1.0f);
+    const float t2 = t * t;
+    float q = fmaf(t2, 0.222222222f, 0.285714286f);
+    q = fmaf(t2, q, 0.4f);
+    q = fmaf(t2, q, 0.666666667f);
+    q = fmaf(t2, q, 2.0f);
+    return fmaf(t * q, 1.44269504f, (float)e);
+}
+
+// 2^y: k = nearest integer, f = y - k in [-1/2, 1/2], 2^f = e^(f ln 2) from
+// its Taylor series to degree 7 (truncation < 6e-9), scaled by 2^k through
+// the exponent field.
+RB_HD float exp2_series(float y) {
+    if (y != y) return y;
+    if (y >= 128.0f) return bitsf_(0x7F800000u);
+    if (y < -150.0f) return 0.0f;
+    const float k = floorf(y + 0.5f);
+    const float z = (y - k) * 0.693147181f;
+    float q = fmaf(z, 1.98412698e-4f, 1.38888889e-3f);
+    q = fmaf(z, q, 8.33333333e-3f);
+    q = fmaf(z, q, 4.16666667e-2f);
+    q = fmaf(z, q, 0.166666667f);
+    q = fmaf(z, q, 0.5f);
+    q = fmaf(z, q, 1.0f);
+    q = fmaf(z, q, 1.0f);
+    int ki = (int)k;
+    if (ki > 127) {
+        q = q * 2.0f;
+        ki = 127;
+    }
+    if (ki < -126) {
+        q = q * 5.96046448e-8f;  // 2^-24, exact
+        ki += 24;
+    }
+    return q * bitsf_((uint32_t)(ki + 127) << 23);
+}
+
+// x^y for x >= 0 as 2^(y log2 x): the same bits on every IEEE device (no
+// math-library call); relative error ~1e-6 for |y log2 x| up to ~30.
+RB_HD float pow_series(float x, float y) {
+    if (x > 0.0f) return exp2_series(y * log2_pos(x));
+    if (x == 0.0f) return y > 0.0f ? 0.0f : (y == 0.0f ? 1.0f : bitsf_(0x7F800000u));
+    return bitsf_(0x7FC00000u);
 }
 
 // Uniform point on a triangle (uniform_sample_triangle, src/core/montecarlo.cu:157-164).

@@ -181,3 +181,40 @@ def test_filter_arguments_checked(gpu, pt_setup):
     for bad in [dict(filter=3), dict(filter_width=(0.0, 1.0)), dict(filter_width=(1.0, 9.0))]:
         with pytest.raises(gpu.RbError):
             gpu.SamplerRenderer(tr, cam, "pt", **bad).render()
+
+
+@pytest.fixture(scope="module")
+def materials_setup(gpu):
+    """A room of every BSDF: Oren-Nayar walls, glass, conductor and dielectric
+    microfacet spheres, Lambert / mirror spheres, four emitters."""
+    s = gpu.Scene()
+    wall = s.add_orennayar_bsdf((0.6, 0.55, 0.5), 25.0)
+    s.generate(L.RB_GEN_ROOM, c=(-5.0, 5.0), seed=1, material_id=wall)
+    glass = s.add_glass_bsdf((1.0, 1.0, 1.0), 1.0, 1.5)
+    s.generate(L.RB_GEN_ICOSPHERES, count=5, subdiv=2, seed=2, r=(0.5, 1.0), c=(-3.0, 3.0), material_id=glass)
+    metal = s.add_microfacet_bsdf((0.9, 0.8, 0.6), 30.0, ("conductor", (0.2, 0.4, 1.4), (3.6, 2.4, 1.8)))
+    s.generate(L.RB_GEN_ICOSPHERES, count=5, subdiv=2, seed=3, r=(0.5, 1.0), c=(-3.0, 3.0), material_id=metal)
+    plastic = s.add_microfacet_bsdf((0.8, 0.8, 0.8), 8.0, ("dielectric", 1.0, 1.5))
+    s.generate(L.RB_GEN_ICOSPHERES, count=5, subdiv=2, seed=4, r=(0.5, 1.0), c=(-3.0, 3.0), material_id=plastic)
+    s.generate(L.RB_GEN_ICOSPHERES, count=6, subdiv=2, seed=5, r=(0.3, 0.8), c=(-3.0, 3.0), mix_materials=True)
+    s.generate(L.RB_GEN_EMITTER_QUADS, count=4, seed=6, r=(0.25, 0.0), c=(-5.0, 5.0), emit=(12.0, 12.0, 12.0))
+    b = gpu.scenes.build(s)
+    return s, b, gpu.CudaTracer(b, s)
+
+
+@pytest.mark.parametrize("kind", ["pt", "bdpt"])
+def test_all_materials_bit_exact(gpu, orc, materials_setup, kind):
+    s, b, tr = materials_setup
+    cam = gpu.scenes.pt_camera(40, 40)
+    v, idx, tm, lt = s.geometry()
+    if kind == "pt":
+        img = gpu.SamplerRenderer(tr, cam, "pt", spp=2, seed=21, max_depth=8).render()
+        ref = orc.render_pt(_c2(b), v, idx, tm, s.materials(), lt, _cam_tuple(cam), (0, 0, 40, 40), 2, 21, 8, 3, 0.5)
+    else:
+        img = gpu.SamplerRenderer(tr, cam, "bdpt", spp=1, seed=22, **BD).render()
+        ref = orc.render_bdpt(_c2(b), v, idx, tm, s.materials(), lt, _cam_tuple(cam), (0, 0, 40, 40), 1, 22,
+                              BD["max_depth"], BD["rr_start"], BD["rr_prob"], BD["light_paths"],
+                              BD["light_max_depth"], BD["connections"])
+    assert np.isfinite(img).all() and img.max() > 0
+    bad = img.view(np.uint32) != ref.view(np.uint32)
+    assert not bad.any(), f"{bad.sum()} channels differ, max abs {np.abs(img - ref).max()}"

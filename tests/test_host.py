@@ -203,7 +203,7 @@ def test_generated_scene_sizes_are_exact(rbm):
     s = rbm.scenes.pt_room()
     nv, nt, nl, nm = s.counts()
     assert nt == 2_000_136 and nl == 8
-    kinds = [k for k, _ in s.materials()]
+    kinds = [m[0] for m in s.materials()]
     assert kinds.count(2) == 1 and kinds.count(1) > 200 and kinds.count(0) > 600
 
 

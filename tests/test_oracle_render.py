@@ -323,3 +323,166 @@ def test_film_gaussian_weights_are_radial(orc):
     ys, xs = np.nonzero(got[..., 0])
     assert len(xs) > 4
     assert np.all(np.abs(xs - dx) <= 2.0) and np.all(np.abs(ys - dy) <= 2.0)
+
+
+# --------------------------------------------------------------- BSDFs ----
+
+def _dir(theta, phi):
+    return np.array([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)], np.float32)
+
+
+def test_pow_series_against_float64(orc):
+    rng = np.random.default_rng(1)
+    xs = rng.uniform(1e-6, 1.0, 3000)
+    ys = rng.uniform(0.0, 60.0, 3000)
+    for x, y in zip(xs.astype(np.float32), ys.astype(np.float32)):
+        exp = float(np.float64(x) ** np.float64(y))
+        if exp < 1e-30:
+            continue
+        # one float rounding of y log2 x dominates: error ~ 2^-24 ln2 |y log2 x|
+        assert abs(orc.powf(x, y) / exp - 1.0) < 2.5e-7 * (1.0 + abs(float(y) * np.log2(float(x)))), (x, y)
+    assert orc.powf(0.0, 2.0) == 0.0 and orc.powf(0.0, 0.0) == 1.0 and orc.powf(1.0, 123.0) == 1.0
+    assert orc.powf(0.5, 3.0) == 0.125 and orc.powf(4.0, 0.5) == pytest.approx(2.0, rel=1e-7)
+
+
+def _on(sigma_deg, rgb=(0.5, 0.6, 0.7)):
+    s = np.float32(min(max(sigma_deg, 0.0), 90.0)) * np.float32(np.pi / 180)
+    s2 = s * s
+    return (4, rgb, (sigma_deg, 1 - s2 / (2 * (s2 + 0.33)), 0.45 * s2 / (s2 + 0.09)))
+
+
+def test_orennayar_sigma_zero_is_lambert(orc):
+    rng = np.random.default_rng(2)
+    ng = np.array([0, 0, 1], np.float32)
+    for _ in range(200):
+        wo, wi = _dir(rng.uniform(0, 1.5), rng.uniform(0, 6.28)), _dir(rng.uniform(0, 1.5), rng.uniform(0, 6.28))
+        a, pa = orc.bsdf_eval(_on(0.0), wo, wi, ng)
+        b, pb = orc.bsdf_eval((0, (0.5, 0.6, 0.7)), wo, wi, ng)
+        assert np.array_equal(a, b) and pa == pb
+
+
+def test_orennayar_closed_forms_and_reciprocity(orc):
+    m = _on(30.0)
+    A = m[2][1]
+    ng = np.array([0, 0, 1], np.float32)
+    # normal incidence: sin(theta_i) = 0 -> f = rho / pi * A
+    f, _ = orc.bsdf_eval(m, _dir(0.7, 1.0), np.array([0, 0, 1], np.float32), ng)
+    assert np.allclose(f, np.array(m[1]) / np.pi * A, rtol=1e-6)
+    rng = np.random.default_rng(3)
+    for _ in range(100):
+        a, b = _dir(rng.uniform(0.05, 1.5), rng.uniform(0, 6.28)), _dir(rng.uniform(0.05, 1.5), rng.uniform(0, 6.28))
+        assert np.allclose(orc.bsdf_eval(m, a, b, ng)[0], orc.bsdf_eval(m, b, a, ng)[0], rtol=2e-5)
+    # rougher surfaces scatter more to grazing back-scatter: f(back) > Lambert's at grazing
+    g, _ = orc.bsdf_eval(m, _dir(1.3, 0.0), _dir(1.3, 0.0), ng)
+    assert g[0] > m[1][0] / np.pi * A
+
+
+def _albedo_quadrature(orc, m, wo, n=96):
+    ng = np.array([0, 0, 1], np.float32)
+    tot = 0.0
+    th = (np.arange(n) + 0.5) / n * (np.pi / 2)
+    ph = (np.arange(2 * n) + 0.5) / (2 * n) * 2 * np.pi
+    for t in th:
+        for p in ph:
+            f, _ = orc.bsdf_eval(m, wo, _dir(t, p), ng)
+            tot += f[0] * np.cos(t) * np.sin(t)
+    return tot * (np.pi / 2 / n) * (2 * np.pi / (2 * n))
+
+
+@pytest.mark.parametrize("exponent", [5.0, 60.0])
+def test_microfacet_sampling_matches_its_pdf_and_conserves_energy(orc, exponent):
+    m = (5, (1.0, 1.0, 1.0), (exponent, 0.0))  # no Fresnel: maximal energy
+    ng = np.array([0, 0, 1], np.float32)
+    wo = _dir(0.6, 0.3)
+    albedo = _albedo_quadrature(orc, m, wo)
+    assert 0.3 < albedo <= 1.0 + 1e-3
+    rng = np.random.default_rng(4)
+    est = []
+    for u1, u2 in rng.uniform(0, 1, (20000, 2)).astype(np.float32):
+        wi, f, pdf = orc.bsdf_sample(m, wo, ng, u1, u2)
+        if pdf > 0 and f[0] > 0:
+            _, pdf2 = orc.bsdf_eval(m, wo, wi, ng)
+            assert pdf2 == pytest.approx(pdf, rel=2e-3)  # sampling and evaluation agree on the pdf
+            est.append(f[0] * abs(wi[2]) / pdf)
+        else:
+            est.append(0.0)
+    est = np.array(est)
+    se = est.std() / np.sqrt(len(est))
+    assert abs(est.mean() - albedo) < 4 * se + 2e-3
+    # reciprocity
+    for _ in range(50):
+        a, b = _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 6.28)), _dir(rng.uniform(0.05, 1.4), rng.uniform(0, 6.28))
+        assert np.allclose(orc.bsdf_eval(m, a, b, ng)[0], orc.bsdf_eval(m, b, a, ng)[0], rtol=1e-4)
+
+
+def test_glass_fresnel_split_at_normal_incidence(orc):
+    m = (3, (1.0, 1.0, 1.0), (1.0, 1.5))
+    ng = np.array([0, 0, 1], np.float32)
+    R = ((1.5 - 1.0) / 2.5) ** 2
+    wo = np.array([0, 0, 1], np.float32)
+    us = (np.arange(2000) + 0.5) / 2000
+    refl = []
+    for u in us.astype(np.float32):
+        wi, w, pdf = orc.bsdf_sample(m, wo, ng, u, 0.5)
+        refl.append(wi[2] > 0)
+        if wi[2] < 0:  # entering: radiance scaled by (1 / 1.5)^2
+            assert np.allclose(w, 1 / 2.25, rtol=1e-6) and np.allclose(wi, [0, 0, -1], atol=1e-7)
+    assert abs(np.mean(refl) - R) < 1e-3
+    # from inside beyond the critical angle: always reflected
+    wo_in = _dir(np.pi - 0.9, 0.2)  # 0.9 rad from the inward normal > asin(1/1.5) = 0.73
+    for u in (0.01, 0.5, 0.99):
+        wi, _, _ = orc.bsdf_sample(m, wo_in, ng, u, 0.5)
+        assert wi[2] < 0  # stays inside
+
+
+def test_glass_pane_transmits_le_t2_over_1_minus_r2(orc):
+    # camera -> glass pane (two faces, n = 1.5) -> emitter: radiance
+    # Le T^2 / (1 - R^2), T = 1 - R, R = 0.04 at normal incidence (the
+    # (1/n)^2 and n^2 radiance scalings cancel through both faces)
+    def q(z, flip):
+        v = np.array([[-5, -5, z], [5, -5, z], [5, 5, z], [-5, 5, z]], np.float32)
+        return v, np.array([[0, 2, 1], [0, 3, 2]] if flip else [[0, 1, 2], [0, 2, 3]], np.uint32)
+    front, back = q(-1.0, True), q(-0.9, False)     # outward normals -z and +z
+    light = q(0.0, True)                             # normal -z: faces the camera
+    v, idx = _merge(front, back, light)
+    tri_mat = np.array([0, 0, 0, 0, 1, 1], np.uint32)
+    mats = [(3, (1.0, 1.0, 1.0), (1.0, 1.5)), (2, (5.0, 5.0, 5.0))]
+    c2 = orc.Bvh(v, idx).compact2()
+    cam = ((0.0, 0.0, -3.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 10.0, 24, 24)
+    img = orc.render_pt(c2, v, idx, tri_mat, mats, np.array([4, 5], np.uint32), cam, (0, 0, 24, 24), 64, 7, 10, 8,
+                        0.5)
+    R = 0.04
+    exp = 5.0 * (1 - R) ** 2 / (1 - R * R)
+    assert abs(img.mean() / exp - 1.0) < 0.01, (img.mean(), exp)
+
+
+def test_orennayar_sigma_zero_renders_as_lambert(orc):
+    v, idx, tm, mats, lights = _box_room()
+    c2 = orc.Bvh(v, idx).compact2()
+    cam = ((0.0, 1.0, -0.95), (0.0, 1.0, 1.0), (0.0, 1.0, 0.0), 70.0, 16, 16)
+    on = [_on(0.0, mats[0][1])] + mats[1:]
+    a = orc.render_pt(c2, v, idx, tm, mats, lights, cam, (0, 0, 16, 16), 4, 3, 8, 3, 0.5)
+    b = orc.render_pt(c2, v, idx, tm, on, lights, cam, (0, 0, 16, 16), 4, 3, 8, 3, 0.5)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_bdpt_agrees_with_pt_on_glossy_and_rough_materials(orc):
+    # the box with an Oren-Nayar floor and a conductor microfacet left wall:
+    # connections now evaluate glossy BSDFs at both ends
+    v, idx, tm, mats, lights = _box_room()
+    tm = tm.copy()
+    tm[0:2] = 3   # floor
+    tm[8:10] = 4  # left wall
+    mats = mats + [_on(30.0, (0.7, 0.6, 0.5)), (5, (0.9, 0.8, 0.6), (25.0, 1.0, 0.2, 0.4, 1.4, 3.6, 2.4, 1.8))]
+    c2 = orc.Bvh(v, idx).compact2()
+    W = H = 20
+    cam = ((0.0, 1.0, -0.95), (0.0, 1.0, 1.0), (0.0, 1.0, 0.0), 70.0, W, H)
+    pt, bd = [], []
+    for seed in range(8):
+        pt.append(orc.render_pt(c2, v, idx, tm, mats, lights, cam, (0, 0, W, H), 512, seed, 24, 3, 0.5)
+                  .reshape(-1, 3).mean(0))
+        bd.append(orc.render_bdpt(c2, v, idx, tm, mats, lights, cam, (0, 0, W, H), 512, seed, 24, 3, 0.5, 4096, 4, 3)
+                  .reshape(-1, 3).mean(0))
+    pt, bd = np.mean(pt, 0), np.mean(bd, 0)
+    assert np.all(pt > 0.05)
+    assert np.all(np.abs(bd / pt - 1.0) < 0.015), (pt, bd)

@@ -89,7 +89,7 @@ def test_xml_scene_materials_lights_and_camera(rbm, tmp_path):
     s = rbm.Scene()
     cam = s.load_xml(path, 320, 240)
     v, idx, tm, lt = s.geometry()
-    mats = s.materials()
+    mats = [m[:2] for m in s.materials()]
     # materials in file order, then one emitter per light
     assert mats == [(L.RB_MAT_LAMBERT, pytest.approx((0.7, 0.6, 0.5))), (L.RB_MAT_MIRROR, pytest.approx((0.9,) * 3)),
                     (L.RB_MAT_EMITTER, pytest.approx((10.0, 9.0, 8.0)))]
@@ -128,10 +128,16 @@ def test_scene_file_errors(rbm, tmp_path):
     (tmp_path / "bad.obj").write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\n")
     with pytest.raises(rbm.RbError):  # vertex 3 does not exist
         s.load_obj(tmp_path / "bad.obj", 0)
-    glass = SCENE_XML.replace("<type>mirror</type>", "<type>glass</type>")
+    layered = SCENE_XML.replace("<bxdf><type>mirror</type><reflectance>{0.9,0.9,0.9}</reflectance></bxdf>",
+                                "<bxdf><type>mirror</type><reflectance>{0.9,0.9,0.9}</reflectance></bxdf>"
+                                "<bxdf><type>lambertian</type><reflectance>{0.1,0.1,0.1}</reflectance></bxdf>")
     with pytest.raises(rbm.RbError) as e:
-        rbm.Scene().load_xml(_write(tmp_path, xml=glass))
+        rbm.Scene().load_xml(_write(tmp_path, xml=layered))
     assert e.value.status == L.RB_ERR_UNSUPPORTED
+    glass_no_eta = SCENE_XML.replace("<type>mirror</type>", "<type>glass</type>")
+    with pytest.raises(rbm.RbError) as e:
+        rbm.Scene().load_xml(_write(tmp_path, xml=glass_no_eta))
+    assert e.value.status == L.RB_ERR_INVALID
     with pytest.raises(rbm.RbError) as e:
         rbm.Scene().load_xml(_write(tmp_path, xml=SCENE_XML.replace("</scene>", "")))
     assert e.value.status == L.RB_ERR_INVALID
@@ -224,3 +230,40 @@ def test_exr_holds_the_half_values(rbm, tmp_path):
 def test_image_write_rejects_unknown_suffix(rbm, tmp_path):
     with pytest.raises(rbm.RbError):
         rbm.write_image(tmp_path / "a.tiff", _img())
+
+
+def test_xml_reads_glass_orennayar_and_microfacet(rbm, tmp_path):
+    xml = SCENE_XML.replace("""  <object><name>floor</name>""", """  <material>
+    <name>pane</name>
+    <bsdf><layers>1</layers><bxdf><type>glass</type><reflectance>{1,1,1}</reflectance>
+      <etai>1.0</etai><etat>1.5</etat></bxdf></bsdf>
+  </material>
+  <material>
+    <name>clay</name>
+    <bsdf><layers>1</layers><bxdf><type>orennayar</type><reflectance>{0.5,0.4,0.3}</reflectance>
+      <sigma>20</sigma></bxdf></bsdf>
+  </material>
+  <material>
+    <name>gold</name>
+    <bsdf><layers>1</layers><bxdf><type>microfacet</type><reflectance>{1,1,1}</reflectance>
+      <distribution>blinn</distribution><exponent>40</exponent><fresnel_type>fresnel_conductor</fresnel_type>
+      <eta>{0.2, 0.4, 1.4}</eta><k>{3.6, 2.4, 1.8}</k></bxdf></bsdf>
+  </material>
+  <object><name>floor</name>""").replace("<object><name>floor</name><material>white</material>",
+                                          "<object><name>floor</name><material>clay</material>")
+    s = rbm.Scene()
+    s.load_xml(_write(tmp_path, xml=xml))
+    mats = s.materials()
+    kinds = [m[0] for m in mats]
+    assert kinds == [L.RB_MAT_LAMBERT, L.RB_MAT_MIRROR, L.RB_MAT_GLASS, L.RB_MAT_ORENNAYAR, L.RB_MAT_MICROFACET,
+                     L.RB_MAT_EMITTER]
+    assert mats[2][2][:2] == (1.0, 1.5)
+    sig = np.float32(20.0) * np.float32(np.pi / 180)
+    s2 = np.float32(sig * sig)
+    assert mats[3][2][0] == 20.0
+    assert mats[3][2][1] == pytest.approx(1 - s2 / (2 * (s2 + 0.33)), rel=1e-6)  # A
+    assert mats[3][2][2] == pytest.approx(0.45 * s2 / (s2 + 0.09), rel=1e-6)     # B
+    assert mats[4][2][:2] == (40.0, L.RB_FRESNEL_CONDUCTOR)
+    assert mats[4][2][2:8] == pytest.approx((0.2, 0.4, 1.4, 3.6, 2.4, 1.8))
+    _, _, tm, _ = s.geometry()
+    assert tm.tolist()[:2] == [3, 3]  # the floor got the Oren-Nayar material
