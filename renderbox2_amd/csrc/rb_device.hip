@@ -31,6 +31,14 @@ static rb_status init_locked(int device) {
     if ((e = hipMalloc(&c, sizeof(uint32_t) * 2 * kCounterSlots)) != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
     if ((e = hipMemset(c, 0, sizeof(uint32_t) * 2 * kCounterSlots)) != hipSuccess) return hip_fail(e, "hipMemset(counters)");
     g_dev.counters = (uint32_t*)c;
+    // Render scratch comes from the stream-ordered pool; keep freed blocks in
+    // the pool instead of returning them at every synchronisation, so a
+    // steady stream of renders reuses its memory rather than remapping it.
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+        uint64_t keep = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    }
     // the autotuner's pick on 2 Mi incoherent rays at 1M triangles
     g_dev.cfg.block = 128;
     g_dev.cfg.lds_stack = 24;

@@ -50,6 +50,7 @@ struct Slot {
     hipEvent_t ev = nullptr;
     float* pinned = nullptr;
     size_t pinned_cap = 0;
+    std::vector<float*> retired;  // outgrown staging, freed with the server (freeing pinned memory syncs the device)
     unsigned long long* d_counter = nullptr;
     unsigned long long* h_counter = nullptr;  // pinned
     Req* req = nullptr;
@@ -166,13 +167,18 @@ void rb_server_s::launch(Slot& sl, Req* q) {
             st = fail(RB_ERR_OOM, "request %llu: output allocation failed", (unsigned long long)q->id);
         }
         if (st == RB_OK && q->r.out_host && sl.pinned_cap < q->out_bytes) {
-            if (sl.pinned) (void)hipHostFree(sl.pinned);
-            sl.pinned = nullptr;
-            sl.pinned_cap = 0;
-            if (hipHostMalloc((void**)&sl.pinned, q->out_bytes, hipHostMallocDefault) != hipSuccess)
+            // grow geometrically from 8 MB; the old buffer is kept until the
+            // server goes, since freeing pinned memory synchronises the device
+            size_t cap = sl.pinned_cap ? 2 * sl.pinned_cap : ((size_t)8 << 20);
+            while (cap < q->out_bytes) cap *= 2;
+            float* np_ = nullptr;
+            if (hipHostMalloc((void**)&np_, cap, hipHostMallocDefault) != hipSuccess) {
                 st = fail(RB_ERR_OOM, "pinned staging allocation failed");
-            else
-                sl.pinned_cap = q->out_bytes;
+            } else {
+                if (sl.pinned) sl.retired.push_back(sl.pinned);
+                sl.pinned = np_;
+                sl.pinned_cap = cap;
+            }
         }
     }
     if (st == RB_OK) {
@@ -329,6 +335,7 @@ rb_status rb_server_destroy(rb_server_t s) {
         if (sl.d_counter) (void)hipFree(sl.d_counter);
         if (sl.h_counter) (void)hipHostFree(sl.h_counter);
         if (sl.pinned) (void)hipHostFree(sl.pinned);
+        for (float* r : sl.retired) (void)hipHostFree(r);
     }
     delete s;
     return RB_OK;
