@@ -361,6 +361,7 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, policy="deadline", see
             "p50_ms": round(st_p["p50_ms"], 2), "p99_ms": round(st_p["p99_ms"], 2),
             "burst_p50_ms": round(st_b["p50_ms"], 2), "burst_p99_ms": round(st_b["p99_ms"], 2),
             "per_kind": per_kind, "max_inflight": max_inflight, "policy": policy, "deferrals": st_p["deferrals"],
+            "arena_misses": st_p["arena_misses"], "peak_arena_mb": round(st_p["peak_arena_bytes"] / 2**20, 1),
             "match": bool(ok_b and ok_p), "setup_s": round(setup_s, 1)}
 
 
@@ -375,7 +376,7 @@ def main():
     ap.add_argument("--config", default="",
                     help="block,lds_stack,chunk,fetch_threshold,waves_per_cu[,speculative[,leaf_gate]]")
     ap.add_argument("--speculative", type=int, default=0,
-                    help="1: the reference's speculative leaf order (faster, wave-dependent in rare cases)")
+                    help="1: the reference's speculative leaf order (wave-dependent results in rare cases)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one copy kernel of this size first (traffic counter calibration)")
     ap.add_argument("--render", type=int, default=1, help="also time the AO and path-traced frames")
@@ -457,10 +458,23 @@ def main():
             m = time_launches(rb, tracer, buf, cnt, d_hits, anyh, max(3, args.steps // 2), 1, stream)
             extra[name + "_mrays"] = round(cnt / (m * 1e-3) / 1e6, 1)
             log(f"{name}: {m:.3f} ms/launch, {extra[name + '_mrays']} Mrays/s")
+        # one GPU's share of the mixed 4 Mi set split over G GPUs (every G-th
+        # ray, the interleaved split), traced at its real size on this GPU
+        mixed = d_mixed.download(np.float32, 2 * n * 8).reshape(2 * n, 8)
+        share = {}
+        for G in (2, 4, 8):
+            part = np.ascontiguousarray(mixed[::G])
+            d_part = rb.DeviceBuffer.from_array(part)
+            m = time_launches(rb, tracer, d_part, len(part), d_hits, False, max(3, args.steps // 2), 1, stream)
+            share[str(G)] = {"rays": len(part), "ms": round(m, 4), "mrays": round(len(part) / (m * 1e-3) / 1e6, 1)}
+            del d_part
+        extra["mixed_share_per_gpu"] = share
+        log(f"mixed set, one GPU's share: {share}")
+        del mixed
 
-    # device-side traversal counters (this kernel's own visit order)
-    ids, _, cnt = tracer.trace_counts(d_diff.download(np.float32, n * 8).reshape(n, 8)[: 1 << 18])
-    gpu_bpr = float((40 + 64 * cnt[:, 0] + 48 * cnt[:, 1] + 16 * cnt[:, 2] + 4 * (ids >= 0)).mean())
+    # device-side traversal counters (this kernel's own visit order), all rays
+    ids, _, gcnt = tracer.trace_counts(d_diff.download(np.float32, n * 8).reshape(n, 8))
+    gpu_bpr = float((40 + 64 * gcnt[:, 0] + 48 * gcnt[:, 1] + 16 * gcnt[:, 2] + 4 * (ids >= 0)).mean())
 
     # CPU baseline (the oracle's restatement of the reference traversal) and
     # the algorithmic bytes per ray it counts on the reference visit order.
@@ -485,12 +499,13 @@ def main():
             times.append(time.time() - t1)
         tcpu = sorted(times)[1]
         bytes_per_ray = float(oracle.bytes_per_ray(ocnt, oid >= 0).mean())
+        counts_equal = bool(np.array_equal(ocnt, gcnt[:size]))
         same = bool(np.array_equal(oid, hits["id"][:size]) and
                     np.array_equal(ot.view(np.uint32), hits["t"][:size].view(np.uint32)))
         cpu = {"value": round(size / tcpu / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
                "sample": f"first {size} of the {n} incoherent rays, closest hit, median of 3 "
                          f"(OpenMP, {cpu_model()})",
-               "matches_gpu": same}
+               "matches_gpu": same, "visit_counts_equal": counts_equal}
         log(f"cpu baseline: {cpu['value']} Mrays/s on {cores} cores, B/ray {bytes_per_ray:.0f}, match={same}")
     if bytes_per_ray is None:
         bytes_per_ray = gpu_bpr

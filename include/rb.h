@@ -361,7 +361,8 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
 
 /* Writes a host RGB float image (w*h*3, row-major, top row first) by suffix
  * (ImageWriter::write, src/io/imagewriter.cpp:34-105): .exr (OpenEXR,
- * uncompressed half B/G/R), .pfm (float32), .ppm / .png (8-bit: clamp to
+ * uncompressed half B/G/R), .pfm (float32), .jpg / .jpeg (baseline JPEG,
+ * quality 100 as the reference writes it), .ppm / .png (8-bit: clamp to
  * [0,1], gamma 1/2.2, round, as ImageWriter::to_byte). */
 rb_status rb_image_write(const char* path, const float* rgb, uint32_t w, uint32_t h);
 
@@ -401,8 +402,13 @@ typedef struct rb_render_params {
     uint32_t filter;
     float filter_width[2];
     float filter_alpha;
+    /* Ray-cast debug shading (RayCastIntegratorParams::m_shade,
+     * src/core/params.h:139-151): RB_SHADE_* below. */
+    uint32_t shade;
 } rb_render_params;
 enum { RB_FILTER_BOX = 0, RB_FILTER_TRIANGLE = 1, RB_FILTER_GAUSSIAN = 2 };
+enum { RB_SHADE_PRIMITIVE_ID = 0, RB_SHADE_MATERIAL_ID = 1, RB_SHADE_NORMALS = 2, RB_SHADE_UVS = 3,
+       RB_SHADE_UNIFORM = 4 };
 void rb_render_params_default(rb_render_params* p);
 
 /* One render of the tile [x0,x0+w) x [y0,y0+h): camera rays (spp per pixel),
@@ -426,11 +432,20 @@ rb_status rb_render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t
  * direct light and `connections` cache connections per camera vertex. */
 rb_status rb_render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                          const rb_render_params* p, float* out_rgb, void* stream);
+/* Ray-cast debug integrator (IntegratorRayCast, src/integrators/integrator_raycast.cu:37-160):
+ * one closest-hit trace of the camera rays, each sample coloured by
+ * p->shade -- PRIMITIVE_ID: the reference's 15-colour palette indexed by
+ * triangle id mod 15; MATERIAL_ID: the hit material's colour; NORMALS: the
+ * geometric normal (unit cross(v1 - v0, v2 - v0)); UVS: the barycentric
+ * weights (u, v, 0) of v0 and v1; UNIFORM: white -- black on a miss, then
+ * the film. */
+rb_status rb_render_raycast(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                            const rb_render_params* p, float* out_rgb, void* stream);
 
 /* A render pass captured into a HIP graph (no reference counterpart; the
  * reference launches every kernel of every iteration from the host,
- * src/core/application.cpp:225-239).  kind: RB_REQ_AO, RB_REQ_PT or
- * RB_REQ_BDPT (the serving kinds below).  The graph records the whole pass --
+ * src/core/application.cpp:225-239).  kind: RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT
+ * or RB_REQ_RAYCAST (the serving kinds below).  The graph records the whole pass --
  * camera rays, every bounce up to max_depth (kernels skip work past the
  * live-path count held in device memory), film -- into out_rgb, and owns its
  * scratch.  rb_render_graph_launch(g, seed, stream) replays it for `seed`:
@@ -465,7 +480,7 @@ typedef struct rb_server_config {
                               estimated GPU time first; RB_SCHED_FIFO: arrival order */
 } rb_server_config;
 
-enum { RB_REQ_AO = 0, RB_REQ_PT = 1, RB_REQ_BDPT = 2, RB_REQ_TRACE = 3 };
+enum { RB_REQ_AO = 0, RB_REQ_PT = 1, RB_REQ_BDPT = 2, RB_REQ_TRACE = 3, RB_REQ_RAYCAST = 4 };
 enum { RB_REQ_QUEUED = 0, RB_REQ_RUNNING = 1, RB_REQ_DONE = 2, RB_REQ_FAILED = 3 };
 
 typedef struct rb_request {
@@ -495,6 +510,11 @@ typedef struct rb_server_stats {
     uint64_t rays;
     double first_submit_to_last_done_s;
     uint64_t peak_budget_bytes;
+    /* device memory the slots' arenas held at most (output + scratch of the
+     * largest request each slot ran), and scratch allocations that did not
+     * fit their arena and came from the stream-ordered pool instead */
+    uint64_t peak_arena_bytes;
+    uint64_t arena_misses;
 } rb_server_stats;
 
 rb_status rb_server_create(const rb_server_config* cfg, rb_server_t* out);

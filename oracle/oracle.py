@@ -243,6 +243,27 @@ def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, 
     return out
 
 
+def render_raycast(compact2, verts, idx, tri_mat, materials, cam, tile, spp, seed, shade, film=FILM_DEFAULT):
+    """Ray-cast debug image of the tile (shade 0..4: primitive id, material,
+    normals, uvs, uniform); returns float32[h, w, 3]."""
+    nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
+    verts = np.ascontiguousarray(verts, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    tri_mat = np.ascontiguousarray(tri_mat, np.uint32)
+    m12 = _pack_materials(materials)
+    r2c, c2w = camera_matrices(*cam)
+    x0, y0, w, h = tile
+    out = np.empty((h, w, 3), np.float32)
+    L = lib()
+    P = C.c_void_p
+    U = C.c_uint32
+    L.orc_render_raycast.argtypes = [P] * 7 + [U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, C.c_float,
+                                                                    C.c_float, P]
+    L.orc_render_raycast(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m12), len(materials),
+                         _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, shade, *film, _p(out))
+    return out
+
+
 def render_bdpt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob,
                 light_paths, light_max_depth, connections, film=FILM_DEFAULT):
     """BDPT-LVC image of the tile; returns float32[h, w, 3]."""

@@ -1984,3 +1984,63 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
     free(rays); free(Ls); free(count); free(pos); free(ngs); free(wos); free(alpha); free(mid); free(mask);
     free(mats);
 }
+
+/* ---- ray-cast debug integrator ------------------------------------------ */
+
+/* kernel_ray_cast_update's colour table (src/integrators/integrator_raycast.cu:47-61) */
+static const float kPalette[15][3] = {
+    {1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}, {0.0f, 1.0f, 1.0f}, {1.0f, 0.0f, 1.0f},
+    {1.0f, 1.0f, 0.0f}, {0.1f, 0.1f, 0.1f}, {0.5f, 0.5f, 0.5f}, {1.0f, 1.0f, 1.0f}, {0.0f, 0.25f, 0.5f},
+    {1.0f, 0.5f, 0.0f}, {0.5f, 0.25f, 0.0f}, {0.2f, 0.9f, 1.0f}, {0.7f, 0.3f, 0.1f}, {0.34f, 0.89f, 0.45f}};
+
+/* kernel_ray_cast_update (integrator_raycast.cu:37-128) per camera sample,
+ * IntegratorRayCast::compute (:136-160) for the trace. */
+void orc_render_raycast(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                        const uint32_t* idx, const uint32_t* tri_mat, const float* m12, uint32_t nmats,
+                        const float* r2c, const float* c2w, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w,
+                        uint32_t h, uint32_t spp, uint64_t seed, uint32_t shade, uint32_t filter, float fxw,
+                        float fyw, float falpha, float* out_rgb)
+{
+    orc_material* mats = mats_from12(m12, nmats);
+    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, NULL, 0};
+    uint32_t n = w * h * spp;
+    float* rays = (float*)malloc((size_t)n * 32);
+    float* Ls = (float*)malloc((size_t)n * 12);
+    orc_raygen_camera(r2c, c2w, width, x0, y0, w, h, spp, seed, 1, rays);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (long i = 0; i < (long)n; i++) {
+        const float* R = rays + 8 * (size_t)i;
+        int32_t id; float t;
+        trace_ray8(&S, R, 0, &id, &t);
+        f3 c = mk3(0.0f, 0.0f, 0.0f);
+        if (id >= 0) {
+            f3 a, b, cc;
+            tri_verts(&S, (uint32_t)id, &a, &b, &cc);
+            if (shade == 0) {
+                const float* q = kPalette[(uint32_t)id % 15u];
+                c = mk3(q[0], q[1], q[2]);
+            } else if (shade == 1) {
+                const orc_material* m = &mats[tri_mat[id]];
+                c = mk3(m->color[0], m->color[1], m->color[2]);
+            } else if (shade == 2) {
+                c = tri_norm(a, b, cc);
+            } else if (shade == 3) {
+                /* barycentric weights of v0, v1 at the hit point */
+                f3 p = add3(mk3(R[0], R[1], R[2]), mul3s(mk3(R[4], R[5], R[6]), t));
+                f3 e0 = sub3(a, cc), e1 = sub3(b, cc), q = sub3(p, cc);
+                float d00 = dot3(e0, e0), d01 = dot3(e0, e1), d11 = dot3(e1, e1);
+                float d20 = dot3(q, e0), d21 = dot3(q, e1);
+                float den = d00 * d11 - d01 * d01;
+                c.x = den != 0.0f ? (d11 * d20 - d01 * d21) / den : 0.0f;
+                c.y = den != 0.0f ? (d00 * d21 - d01 * d20) / den : 0.0f;
+            } else {
+                c = mk3(1.0f, 1.0f, 1.0f);
+            }
+        }
+        Ls[3 * (size_t)i] = c.x; Ls[3 * (size_t)i + 1] = c.y; Ls[3 * (size_t)i + 2] = c.z;
+    }
+    orc_film(Ls, width, x0, y0, w, h, spp, seed, filter, fxw, fyw, falpha, out_rgb);
+    free(rays);
+    free(Ls);
+    free(mats);
+}

@@ -167,6 +167,15 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
                      uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t light_paths,
                      uint32_t light_max_depth, uint32_t connections, uint32_t filter, float fxw, float fyw,
                      float falpha, float* out_rgb);
+/* Ray-cast debug integrator (src/integrators/integrator_raycast.cu:37-160):
+ * closest hit of the camera rays coloured by shade mode 0 primitive id
+ * (15-colour palette, id mod 15), 1 material colour, 2 geometric normal,
+ * 3 barycentrics (u, v, 0), 4 uniform white; black on a miss; then the film. */
+void orc_render_raycast(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
+                        const uint32_t* idx, const uint32_t* tri_mat, const float* m12, uint32_t nmats,
+                        const float* r2c, const float* c2w, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w,
+                        uint32_t h, uint32_t spp, uint64_t seed, uint32_t shade, uint32_t filter, float fxw,
+                        float fyw, float falpha, float* out_rgb);
 
 #ifdef __cplusplus
 }

@@ -134,6 +134,7 @@ _SIGS = {
     "rb_render_ao": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_pt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_bdpt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
+    "rb_render_raycast": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_graph_create": (C.c_int, [U32, P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_graph_launch": (C.c_int, [P, U64, P]),
     "rb_render_graph_destroy": (C.c_int, [P]),
@@ -152,7 +153,8 @@ _SIGS = {
 }
 
 
-RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT, RB_REQ_TRACE = 0, 1, 2, 3
+RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT, RB_REQ_TRACE, RB_REQ_RAYCAST = 0, 1, 2, 3, 4
+RB_SHADE_PRIMITIVE_ID, RB_SHADE_MATERIAL_ID, RB_SHADE_NORMALS, RB_SHADE_UVS, RB_SHADE_UNIFORM = 0, 1, 2, 3, 4
 RB_FILTER_BOX, RB_FILTER_TRIANGLE, RB_FILTER_GAUSSIAN = 0, 1, 2
 RB_REQ_QUEUED, RB_REQ_RUNNING, RB_REQ_DONE, RB_REQ_FAILED = 0, 1, 2, 3
 
@@ -172,7 +174,8 @@ class rb_request_status(C.Structure):
 class rb_server_stats(C.Structure):
     _fields_ = [("submitted", U64), ("completed", U64), ("failed", U64), ("rejected", U64), ("deferrals", U64),
                 ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
-                ("rays", U64), ("first_submit_to_last_done_s", C.c_double), ("peak_budget_bytes", U64)]
+                ("rays", U64), ("first_submit_to_last_done_s", C.c_double), ("peak_budget_bytes", U64),
+                ("peak_arena_bytes", U64), ("arena_misses", U64)]
 
 
 class rb_bvh_dev(C.Structure):
@@ -187,7 +190,8 @@ class rb_render_params(C.Structure):
     _fields_ = [("spp", U32), ("seed", U64), ("ao_samples", U32), ("ao_radius", F32), ("max_depth", U32),
                 ("rr_start", U32), ("rr_prob", F32), ("light_paths", U32), ("lvc_capacity", U32),
                 ("connections", U32), ("light_max_depth", U32), ("ray_counter", P), ("filter", U32),
-                ("filter_width", F32 * 2), ("filter_alpha", F32)]
+                ("filter_width", F32 * 2), ("filter_alpha", F32),
+                ("shade", U32)]
 
 
 class rb_request(C.Structure):

@@ -499,16 +499,21 @@ def trace_batch(batches, stream=None):
 
 
 FILTERS = {"box": L.RB_FILTER_BOX, "triangle": L.RB_FILTER_TRIANGLE, "gaussian": L.RB_FILTER_GAUSSIAN}
+# RayCastShade names of the integrator config (src/config/configuration.cpp:222-250)
+SHADES = {"primitive_id": L.RB_SHADE_PRIMITIVE_ID, "material_id": L.RB_SHADE_MATERIAL_ID,
+          "normals": L.RB_SHADE_NORMALS, "uvs": L.RB_SHADE_UVS, "uniform": L.RB_SHADE_UNIFORM}
 
 
 def render_params(**kw):
     """rb_render_params with the library defaults, overridden by keywords
-    (filter may be a name: "box", "triangle", "gaussian")."""
+    (filter may be a name: "box", "triangle", "gaussian"; shade one of SHADES)."""
     p = L.rb_render_params()
     lib().rb_render_params_default(C.byref(p))
     for k, v in kw.items():
         if k == "filter" and isinstance(v, str):
             v = FILTERS[v]
+        if k == "shade" and isinstance(v, str):
+            v = SHADES[v]
         if k == "filter_width":
             v = (C.c_float * 2)(*v)
         setattr(p, k, v)
@@ -519,7 +524,8 @@ class SamplerRenderer:
     """Camera rays + integrator + film for one tile (SamplerRenderer::render,
     src/renderers/samplerrenderer.cpp:58-107; Integrator::render, src/core/integrator.h:59)."""
 
-    INTEGRATORS = {"ao": "rb_render_ao", "pt": "rb_render_pt", "bdpt": "rb_render_bdpt"}
+    INTEGRATORS = {"ao": "rb_render_ao", "pt": "rb_render_pt", "bdpt": "rb_render_bdpt",
+                   "raycast": "rb_render_raycast"}
 
     def __init__(self, tracer, camera, integrator="ao", **params):
         if integrator not in self.INTEGRATORS:
@@ -549,7 +555,7 @@ class RenderGraph:
     buffer (rb_render_graph_*): launch(seed) replays the whole pass --
     raygen, every bounce, film -- for a new seed with one graph launch."""
 
-    KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT}
+    KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT, "raycast": L.RB_REQ_RAYCAST}
 
     def __init__(self, tracer, camera, integrator, d_out, tile=None, **params):
         x0, y0, w, h = tile if tile else (0, 0, camera.width, camera.height)
@@ -602,7 +608,7 @@ class RenderServer:
     what else runs: every random number is keyed by (request seed, pixel,
     sample, bounce)."""
 
-    KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT}
+    KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT, "raycast": L.RB_REQ_RAYCAST}
 
     def __init__(self, max_inflight=8, hbm_budget=0, max_queue=0, policy="deadline"):
         pol = {"deadline": L.RB_SCHED_DEADLINE, "fifo": L.RB_SCHED_FIFO}[policy]

@@ -43,6 +43,28 @@ struct CaptureCtx {
     std::vector<void*> allocs;
 };
 extern thread_local CaptureCtx* t_capture;
+
+// Set on the server's scheduler thread while it launches a request: render
+// scratch is carved from the slot's own device arena, which only that slot's
+// stream touches, so requests on different streams never reuse each other's
+// memory (no allocator call, no cross-stream ordering).  Allocations that do
+// not fit fall back to the stream-ordered pool.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0, off = 0;
+    uint32_t misses = 0;  // allocations that did not fit
+    void* take(size_t bytes) {
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        if (!base || off + need > cap) {
+            misses++;
+            return nullptr;
+        }
+        void* p = base + off;
+        off += need;
+        return p;
+    }
+};
+extern thread_local Arena* t_arena;
 inline RngArg rng_arg(uint64_t seed, uint32_t tag) {
     RngArg a;
     a.r = Rng::make(seed, tag);
@@ -65,6 +87,8 @@ rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
                     const rb_render_params* p, float* out, hipStream_t s);
 rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                       const rb_render_params* p, float* out, hipStream_t s);
+rb_status render_raycast(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                         const rb_render_params* p, float* out, hipStream_t s);
 // Device scratch bytes one render call of this kind and size allocates.
 uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_render_params* p);
 

@@ -9,6 +9,7 @@ namespace rb {
 static DeviceState g_dev;
 static std::mutex g_init_mu;
 thread_local CaptureCtx* t_capture = nullptr;
+thread_local Arena* t_arena = nullptr;
 
 DeviceState& dev() { return g_dev; }
 

@@ -56,8 +56,8 @@ rb_status rb_render_graph_create(uint32_t kind, rb_bvh_t bvh, const rb_camera* c
     if (st != RB_OK) return st;
     if (!out || !p) return fail(RB_ERR_INVALID, "NULL argument");
     *out = nullptr;
-    if (kind != RB_REQ_AO && kind != RB_REQ_PT && kind != RB_REQ_BDPT)
-        return fail(RB_ERR_INVALID, "render graphs take RB_REQ_AO, RB_REQ_PT or RB_REQ_BDPT");
+    if (kind != RB_REQ_AO && kind != RB_REQ_PT && kind != RB_REQ_BDPT && kind != RB_REQ_RAYCAST)
+        return fail(RB_ERR_INVALID, "render graphs take RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT or RB_REQ_RAYCAST");
     if (p->ray_counter) return fail(RB_ERR_INVALID, "render graphs take no ray counter");
     rb_render_graph_s* G = new rb_render_graph_s();
     const uint32_t pairs = 2 * (p->max_depth + 1) + p->light_max_depth + 8;
@@ -87,6 +87,7 @@ rb_status rb_render_graph_create(uint32_t kind, rb_bvh_t bvh, const rb_camera* c
     switch (kind) {
         case RB_REQ_AO: st = render_ao(bvh, cam, x0, y0, w, h, p, out_rgb, cs); break;
         case RB_REQ_PT: st = render_pt(bvh, cam, x0, y0, w, h, p, out_rgb, cs); break;
+        case RB_REQ_RAYCAST: st = render_raycast(bvh, cam, x0, y0, w, h, p, out_rgb, cs); break;
         default: st = render_bdpt(bvh, cam, x0, y0, w, h, p, out_rgb, cs); break;
     }
     t_capture = nullptr;

@@ -445,6 +445,55 @@ __global__ void k_pt_connect(PathBufs P, uint32_t qi, float nlights, uint32_t nc
     }
 }
 
+// ------------------------------------------------------------ ray cast --
+
+// Debug colours of kernel_ray_cast_update (src/integrators/integrator_raycast.cu:47-61).
+__constant__ float kRaycastPalette[15][3] = {
+    {1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f},   {0.0f, 1.0f, 1.0f},  {1.0f, 0.0f, 1.0f},
+    {1.0f, 1.0f, 0.0f}, {0.1f, 0.1f, 0.1f}, {0.5f, 0.5f, 0.5f},   {1.0f, 1.0f, 1.0f},  {0.0f, 0.25f, 0.5f},
+    {1.0f, 0.5f, 0.0f}, {0.5f, 0.25f, 0.0f}, {0.2f, 0.9f, 1.0f}, {0.7f, 0.3f, 0.1f}, {0.34f, 0.89f, 0.45f}};
+
+// Per sample debug colour of the closest hit (kernel_ray_cast_update,
+// integrator_raycast.cu:37-128); misses are black.  PRIMITIVE_ID: palette
+// entry of the triangle id mod 15; MATERIAL_ID: the material's colour;
+// NORMALS: the geometric normal cross(v1 - v0, v2 - v0), normalised;
+// UVS: the hit's barycentric weights of v0 and v1; UNIFORM: white.
+__global__ void k_raycast_shade(const float4* __restrict__ rays, const int2* __restrict__ hits, uint32_t n,
+                                SceneDev S, uint32_t mode, float4* __restrict__ L) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int2 h = hits[i];
+        V3 c = v3(0.0f, 0.0f, 0.0f);
+        if (h.x >= 0) {
+            const uint32_t t = (uint32_t)h.x;
+            V3 v0, v1, v2;
+            uint32_t mat;
+            tri_fetch(S, t, &v0, &v1, &v2, &mat);
+            if (mode == RB_SHADE_PRIMITIVE_ID) {
+                const uint32_t k = t % 15u;
+                c = v3(kRaycastPalette[k][0], kRaycastPalette[k][1], kRaycastPalette[k][2]);
+            } else if (mode == RB_SHADE_MATERIAL_ID) {
+                const rb_material& m = S.mats[mat];
+                c = v3(m.color[0], m.color[1], m.color[2]);
+            } else if (mode == RB_SHADE_NORMALS) {
+                c = tri_normal(v0, v1, v2);
+            } else if (mode == RB_SHADE_UVS) {
+                const float4 o = rays[2 * (size_t)i], d = rays[2 * (size_t)i + 1];
+                const float tt = __int_as_float(h.y);
+                const V3 p = vadd(v3(o.x, o.y, o.z), vmul(v3(d.x, d.y, d.z), tt));
+                const V3 e0 = vsub(v0, v2), e1 = vsub(v1, v2), q = vsub(p, v2);
+                const float d00 = vdot(e0, e0), d01 = vdot(e0, e1), d11 = vdot(e1, e1);
+                const float d20 = vdot(q, e0), d21 = vdot(q, e1);
+                const float den = d00 * d11 - d01 * d01;
+                c.x = den != 0.0f ? (d11 * d20 - d01 * d21) / den : 0.0f;
+                c.y = den != 0.0f ? (d00 * d21 - d01 * d20) / den : 0.0f;
+            } else {
+                c = v3(1.0f, 1.0f, 1.0f);
+            }
+        }
+        L[i] = make_float4(c.x, c.y, c.z, 0.0f);
+    }
+}
+
 __global__ void k_film(const float4* __restrict__ L, uint32_t npix, uint32_t spp, float* __restrict__ out) {
     for (uint32_t px = blockIdx.x * blockDim.x + threadIdx.x; px < npix; px += gridDim.x * blockDim.x) {
         float r = 0.f, g = 0.f, b = 0.f;
@@ -562,7 +611,8 @@ static uint32_t grid_of(uint64_t n) {
 }
 
 // Stream-ordered scratch allocations released at the end of a render call;
-// while a render graph is captured, persistent allocations the graph owns.
+// while a render graph is captured, persistent allocations the graph owns;
+// inside the server, carved from the launching slot's arena.
 struct Scratch {
     hipStream_t s;
     std::vector<void*> ptrs;
@@ -582,6 +632,7 @@ struct Scratch {
             t_capture->allocs.push_back(p);
             return (T*)p;
         }
+        if (t_arena && (p = t_arena->take(count * sizeof(T) + 16)) != nullptr) return (T*)p;
         hipError_t e = hipMallocAsync(&p, count * sizeof(T) + 16, s);
         if (e != hipSuccess) {
             (void)hipGetLastError();
@@ -664,6 +715,27 @@ rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0
     hipLaunchKernelGGL(k_ao_resolve, dim3(grid_of(n)), dim3(256), 0, s, hits, ahits, n, p->ao_samples, Ls);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "AO launch");
+    return film(Ls, cam, x0, y0, w, h, p, out, s);
+}
+
+rb_status render_raycast(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                         const rb_render_params* p, float* out, hipStream_t s) {
+    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
+    if (st != RB_OK) return st;
+    if (p->shade > RB_SHADE_UNIFORM) return fail(RB_ERR_INVALID, "unknown ray-cast shade mode %u", p->shade);
+    const uint32_t n = w * h * p->spp;
+    Scratch M(s);
+    float4* rays = M.get<float4>(2 * (size_t)n);
+    int2* hits = M.get<int2>(n);
+    float4* Ls = M.get<float4>(n);
+    if (M.st != RB_OK) return M.st;
+    if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)rays, s)) != RB_OK) return st;
+    if ((st = launch_trace(bvh, (rb_ray*)rays, n, RB_CLOSEST, nullptr, (rb_hit*)hits, nullptr, s, dev().cfg)) != RB_OK)
+        return st;
+    if (p->ray_counter) hipLaunchKernelGGL(k_add64, dim3(1), dim3(64), 0, s, p->ray_counter, (unsigned long long)n);
+    hipLaunchKernelGGL(k_raycast_shade, dim3(grid_of(n)), dim3(256), 0, s, rays, hits, n, scene_of(bvh), p->shade, Ls);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "ray-cast launch");
     return film(Ls, cam, x0, y0, w, h, p, out, s);
 }
 
@@ -817,6 +889,7 @@ uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_re
         case RB_REQ_AO: return n * 56 + n * (p->ao_samples ? p->ao_samples : 1) * 41 + slack;
         case RB_REQ_PT: return n * 308 + slack;
         case RB_REQ_BDPT: return bdpt_scratch_bytes(w, h, p) + slack;
+        case RB_REQ_RAYCAST: return n * 56 + slack;
         default: return slack;
     }
 }
@@ -845,6 +918,7 @@ void rb_render_params_default(rb_render_params* p) {
     p->filter_width[0] = 0.5f;
     p->filter_width[1] = 0.5f;
     p->filter_alpha = 2.0f;
+    p->shade = RB_SHADE_PRIMITIVE_ID;
 }
 
 rb_status rb_render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
@@ -866,6 +940,13 @@ rb_status rb_render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32
     rb_status st = ensure_device();
     if (st != RB_OK) return st;
     return render_bdpt(bvh, cam, x0, y0, w, h, p, out, stream ? (hipStream_t)stream : dev().stream);
+}
+
+rb_status rb_render_raycast(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                            const rb_render_params* p, float* out, void* stream) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    return render_raycast(bvh, cam, x0, y0, w, h, p, out, stream ? (hipStream_t)stream : dev().stream);
 }
 
 }  // extern "C"

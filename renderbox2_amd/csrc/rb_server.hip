@@ -13,6 +13,7 @@
 // rejected at submit.  Results do not depend on what else runs: every
 // random number is keyed by (request seed, pixel, sample, bounce).
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -53,8 +54,15 @@ struct Slot {
     std::vector<float*> retired;  // outgrown staging, freed with the server (freeing pinned memory syncs the device)
     unsigned long long* d_counter = nullptr;
     unsigned long long* h_counter = nullptr;  // pinned
+    char* arena = nullptr;  // device memory for the output and scratch of this slot's requests
+    size_t arena_cap = 0;
     Req* req = nullptr;
 };
+
+// Arena headroom over a request's estimate: 256-byte alignment of each
+// scratch allocation.
+constexpr size_t kArenaSlack = (size_t)64 << 10;
+constexpr size_t kArenaGrain = (size_t)2 << 20;
 
 struct SceneEntry {
     rb_bvh_t bvh;
@@ -78,6 +86,7 @@ double estimate_rays(const rb_request& r) {
     const double depth = (double)r.params.max_depth + 1.0;
     switch (r.kind) {
         case RB_REQ_AO: return n * (1.0 + r.params.ao_samples);
+        case RB_REQ_RAYCAST: return n;
         case RB_REQ_PT: return n * 7.5 * depth / 9.0;
         case RB_REQ_BDPT: {
             const double D = r.params.light_max_depth ? r.params.light_max_depth : 1;
@@ -102,6 +111,8 @@ struct rb_server_s {
     std::vector<rb::Slot> slots;
     uint64_t used = 0, peak = 0;
     uint32_t inject_failures = 0;
+    std::atomic<uint64_t> reserved{0};          // arena bytes held (changed by the scheduler thread only)
+    uint64_t peak_reserved = 0, arena_misses = 0;  // under mu
     bool stop = false;
     std::thread th;
     // stats
@@ -113,6 +124,8 @@ struct rb_server_s {
     void loop();
     void launch(rb::Slot& sl, rb::Req* q);
     void finish(rb::Slot& sl, rb_status result);
+    rb_status ensure_arena(rb::Slot& sl, size_t need);
+    void drop_arena(rb::Slot& sl);
 };
 
 using namespace rb;
@@ -123,8 +136,7 @@ void rb_server_s::finish(Slot& sl, rb_status result) {
         if (q->r.out_host && q->d_out) std::memcpy(q->r.out_host, sl.pinned, q->out_bytes);
         q->rays = *sl.h_counter;
     }
-    if (q->d_out) (void)hipFreeAsync(q->d_out, sl.stream);
-    q->d_out = nullptr;
+    q->d_out = nullptr;  // lives in the slot's arena
     std::lock_guard<std::mutex> g(mu);
     q->t_done = Clock::now();
     q->result = result;
@@ -144,9 +156,46 @@ void rb_server_s::finish(Slot& sl, rb_status result) {
     cv_done.notify_all();
 }
 
+void rb_server_s::drop_arena(Slot& sl) {
+    if (!sl.arena) return;
+    (void)hipFreeAsync(sl.arena, sl.stream);
+    reserved -= sl.arena_cap;
+    sl.arena = nullptr;
+    sl.arena_cap = 0;
+}
+
+// Each slot keeps one device arena for its requests' output and scratch.
+// Only the slot's stream uses it, in order, so a request reuses the memory of
+// the previous one on that slot without an allocator call and without
+// waiting for other streams (a shared stream-ordered pool may order a stream
+// behind the one that freed the block it hands out).  The arena grows to the
+// largest request the slot has run; when the arenas together would pass the
+// budget, idle slots give theirs back first.
+rb_status rb_server_s::ensure_arena(Slot& sl, size_t need) {
+    if (sl.arena_cap >= need) return RB_OK;
+    const size_t cap = (need + kArenaGrain - 1) / kArenaGrain * kArenaGrain;
+    if (reserved - sl.arena_cap + cap > cfg.hbm_budget)
+        for (Slot& o : slots)
+            if (&o != &sl && !o.req) drop_arena(o);
+    drop_arena(sl);
+    void* p = nullptr;
+    hipError_t e = hipMallocAsync(&p, cap, sl.stream);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(RB_ERR_OOM, "server arena (%zu bytes): %s", cap, hipGetErrorString(e));
+    }
+    sl.arena = (char*)p;
+    sl.arena_cap = cap;
+    reserved += cap;
+    std::lock_guard<std::mutex> g(mu);
+    peak_reserved = std::max<uint64_t>(peak_reserved, reserved.load());
+    return RB_OK;
+}
+
 void rb_server_s::launch(Slot& sl, Req* q) {
     sl.req = q;
     rb_status st = RB_OK;
+    Arena arena;
     hipError_t e = hipMemsetAsync(sl.d_counter, 0, sizeof(unsigned long long), sl.stream);
     if (e != hipSuccess) st = hip_fail(e, "server counter reset");
     if (st == RB_OK && q->r.kind != RB_REQ_TRACE) {
@@ -161,10 +210,9 @@ void rb_server_s::launch(Slot& sl, Req* q) {
         }
         if (inject) {
             st = fail(RB_ERR_OOM, "request %llu: device allocation failed (injected)", (unsigned long long)q->id);
-        } else if ((e = hipMallocAsync((void**)&q->d_out, q->out_bytes, sl.stream)) != hipSuccess) {
-            (void)hipGetLastError();
-            q->d_out = nullptr;
-            st = fail(RB_ERR_OOM, "request %llu: output allocation failed", (unsigned long long)q->id);
+        } else if ((st = ensure_arena(sl, q->bytes + kArenaSlack)) == RB_OK) {
+            arena = Arena{sl.arena, sl.arena_cap, 0};
+            q->d_out = (float*)arena.take(q->out_bytes);  // first in the arena: always fits
         }
         if (st == RB_OK && q->r.out_host && sl.pinned_cap < q->out_bytes) {
             // grow geometrically from 8 MB; the old buffer is kept until the
@@ -184,10 +232,12 @@ void rb_server_s::launch(Slot& sl, Req* q) {
     if (st == RB_OK) {
         rb_render_params p = q->r.params;
         p.ray_counter = sl.d_counter;
+        t_arena = q->r.kind == RB_REQ_TRACE ? nullptr : &arena;
         switch (q->r.kind) {
             case RB_REQ_AO: st = render_ao(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;
             case RB_REQ_PT: st = render_pt(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;
             case RB_REQ_BDPT: st = render_bdpt(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;
+            case RB_REQ_RAYCAST: st = render_raycast(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;
             case RB_REQ_TRACE:
                 st = launch_trace(q->bvh, q->r.rays, q->r.n, q->r.flags & RB_ANY, nullptr, q->r.hits, nullptr, sl.stream,
                                   dev().cfg);
@@ -195,6 +245,11 @@ void rb_server_s::launch(Slot& sl, Req* q) {
                     st = hip_fail(e, "server trace count");
                 break;
             default: st = fail(RB_ERR_INVALID, "unknown request kind %u", q->r.kind);
+        }
+        t_arena = nullptr;
+        if (arena.misses) {
+            std::lock_guard<std::mutex> g(mu);
+            arena_misses += arena.misses;
         }
     }
     if (st == RB_OK && q->r.out_host && q->d_out) {
@@ -329,6 +384,7 @@ rb_status rb_server_destroy(rb_server_t s) {
     s->cv_done.notify_all();
     if (s->th.joinable()) s->th.join();
     for (Slot& sl : s->slots) {
+        if (sl.arena) (void)hipFreeAsync(sl.arena, sl.stream);
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.ev) (void)hipEventDestroy(sl.ev);
         if (sl.stream) (void)hipStreamDestroy(sl.stream);
@@ -361,7 +417,7 @@ rb_status rb_server_remove_scene(rb_server_t s, uint32_t id) {
 
 rb_status rb_server_submit(rb_server_t s, const rb_request* req, uint64_t* id) {
     if (!s || !req) return fail(RB_ERR_INVALID, "NULL argument");
-    if (req->kind > RB_REQ_TRACE) return fail(RB_ERR_INVALID, "unknown request kind %u", req->kind);
+    if (req->kind > RB_REQ_RAYCAST) return fail(RB_ERR_INVALID, "unknown request kind %u", req->kind);
     std::unique_ptr<Req> q(new Req());
     q->r = *req;
     q->bytes = request_bytes(*req);
@@ -458,6 +514,8 @@ rb_status rb_server_get_stats(rb_server_t s, rb_server_stats* out) {
     out->deferrals = s->deferrals;
     out->rays = s->rays;
     out->peak_budget_bytes = s->peak;
+    out->peak_arena_bytes = s->peak_reserved;
+    out->arena_misses = s->arena_misses;
     if (!s->lat_ms.empty()) {
         std::vector<double> v = s->lat_ms;
         std::sort(v.begin(), v.end());
@@ -485,6 +543,8 @@ rb_status rb_server_reset_stats(rb_server_t s) {
     s->lat_ms.clear();
     s->have_first = false;
     s->peak = s->used;
+    s->peak_reserved = s->reserved.load();
+    s->arena_misses = 0;
     for (auto it = s->reqs.begin(); it != s->reqs.end();)
         it = it->second->state >= RB_REQ_DONE ? s->reqs.erase(it) : std::next(it);
     return RB_OK;

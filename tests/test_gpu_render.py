@@ -123,6 +123,7 @@ def test_bdpt_rejects_bad_arguments(gpu, pt_setup):
     ("ao", dict(spp=2, ao_samples=4)),
     ("pt", dict(spp=2, max_depth=6)),
     ("bdpt", dict(spp=1, max_depth=5, light_paths=2048, light_max_depth=3, connections=2)),
+    ("raycast", dict(spp=2, shade=L.RB_SHADE_NORMALS, filter=L.RB_FILTER_TRIANGLE, filter_width=(1.5, 1.5))),
 ])
 def test_render_graph_replays_equal_eager_renders(gpu, pt_setup, kind, params):
     """The captured pass replayed for several seeds, with eager renders and
@@ -218,3 +219,36 @@ def test_all_materials_bit_exact(gpu, orc, materials_setup, kind):
     assert np.isfinite(img).all() and img.max() > 0
     bad = img.view(np.uint32) != ref.view(np.uint32)
     assert not bad.any(), f"{bad.sum()} channels differ, max abs {np.abs(img - ref).max()}"
+
+
+@pytest.mark.parametrize("shade", ["primitive_id", "material_id", "normals", "uvs", "uniform"])
+def test_raycast_debug_shading_bit_exact(gpu, orc, materials_setup, shade):
+    """The ray-cast debug integrator, every shade mode, on a tile with a
+    Gaussian film, equals the oracle's; its HIP-graph replay equals it too."""
+    s, b, tr = materials_setup
+    cam = gpu.scenes.pt_camera(48, 40)
+    tile = (3, 2, 41, 35)
+    mode = gpu.core.SHADES[shade]
+    film = (2, 1.5, 1.5, 2.0)
+    fp = dict(filter=film[0], filter_width=(film[1], film[2]), filter_alpha=film[3])
+    img = gpu.SamplerRenderer(tr, cam, "raycast", spp=3, seed=31, shade=shade, **fp).render(tile)
+    v, idx, tm, _ = s.geometry()
+    ref = orc.render_raycast(_c2(b), v, idx, tm, s.materials(), _cam_tuple(cam), tile, 3, 31, mode, film=film)
+    assert np.isfinite(img).all() and np.abs(img).max() > 0
+    bad = img.view(np.uint32) != ref.view(np.uint32)
+    assert not bad.any(), f"{bad.sum()} channels differ, max abs {np.abs(img - ref).max()}"
+    d = gpu.DeviceBuffer(tile[2] * tile[3] * 12)
+    g = gpu.RenderGraph(tr, cam, "raycast", d, tile, spp=3, seed=0, shade=shade, **fp)
+    try:
+        g.launch(31)
+        L.check(L.lib().rb_stream_synchronize(None))
+        got = d.download(np.float32, tile[2] * tile[3] * 3)
+        assert np.array_equal(got.view(np.uint32), ref.reshape(-1).view(np.uint32))
+    finally:
+        g.close()
+
+
+def test_raycast_rejects_unknown_shade(gpu, pt_setup):
+    s, b, tr = pt_setup
+    with pytest.raises(gpu.RbError):
+        gpu.SamplerRenderer(tr, gpu.scenes.pt_camera(16, 16), "raycast", shade=5).render()

@@ -32,6 +32,7 @@ def _mix(gpu, ids):
                                                connections=4)),
         ("pt", ao, ac, None, dict(spp=1, seed=7)),
         ("ao", ao, ac, (8, 8, 40, 40), dict(spp=3, seed=8, ao_samples=2)),
+        ("raycast", room, pc, (8, 8, 40, 32), dict(spp=2, seed=9, shade=L.RB_SHADE_MATERIAL_ID)),
     ]
 
 
@@ -54,6 +55,8 @@ def test_served_results_equal_standalone_renders(gpu, scenes):
     s = srv.stats()
     assert s["completed"] == len(mix) and s["failed"] == 0
     assert s["p99_ms"] >= s["p50_ms"] > 0.0
+    # every scratch allocation came from the slots' arenas (the estimate holds)
+    assert s["arena_misses"] == 0 and s["peak_arena_bytes"] > 0
     srv.close()
 
 

@@ -77,6 +77,30 @@ def test_trace_small_bit_exact(gpu, orc, small, cfg):
         gpu.set_trace_config(**CONFIGS[0])
 
 
+def test_visit_counts_equal_the_oracle(gpu, orc):
+    """In the default leaf order each ray walks the tree in the oracle's
+    order, so the kernel's own per-ray counts of inner nodes, triangles and
+    leaves (closest hit) equal the oracle's, whatever the launch shape.
+    Leaves of at most 16 references keep the MI355X records one-to-one with
+    the Compact2 nodes."""
+    v, idx = random_triangles(4000, seed=52)
+    p = gpu.Platform()
+    p.set_leaf_preferences(1, 16)
+    b = gpu.BVH((v, idx), p)
+    tr = gpu.CudaTracer(b)
+    rays = random_rays(30000, seed=71)
+    oid, ot, ocnt = orc.trace_compact2(*_compact2(gpu, b), rays, counts=True)
+    assert (oid >= 0).sum() > 5000
+    try:
+        for cfg in CONFIGS[:3]:
+            gpu.set_trace_config(**cfg)
+            ids, ts, cnt = tr.trace_counts(rays)
+            assert np.array_equal(ids, oid) and np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
+            assert np.array_equal(cnt, ocnt), cfg
+    finally:
+        gpu.set_trace_config(**CONFIGS[0])
+
+
 def test_trace_room_camera_and_diffuse_rays(gpu, orc, room):
     s, b, tr, c2 = room
     cam = gpu.scenes.ao_camera(256, 256)

@@ -486,3 +486,75 @@ def test_bdpt_agrees_with_pt_on_glossy_and_rough_materials(orc):
     pt, bd = np.mean(pt, 0), np.mean(bd, 0)
     assert np.all(pt > 0.05)
     assert np.all(np.abs(bd / pt - 1.0) < 0.015), (pt, bd)
+
+
+# ------------------------------------------------------- ray-cast debug ----
+
+# kernel_ray_cast_update's colour table (src/integrators/integrator_raycast.cu:47-61)
+RAYCAST_PALETTE = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [0, 1, 1], [1, 0, 1], [1, 1, 0], [0.1, 0.1, 0.1],
+                            [0.5, 0.5, 0.5], [1, 1, 1], [0, 0.25, 0.5], [1, 0.5, 0], [0.5, 0.25, 0],
+                            [0.2, 0.9, 1.0], [0.7, 0.3, 0.1], [0.34, 0.89, 0.45]], np.float32)
+
+
+def _open_room():
+    """The box room without its front wall and ceiling, seen from outside:
+    some camera rays miss everything."""
+    v, idx, tm, mats, _ = _box_room()
+    keep = np.r_[0:2, 4:6, 8:12, 12:14]  # floor, back, left, right, light
+    mats = mats + [(5, (0.9, 0.8, 0.6), (25.0, 1.0, 0.2, 0.4, 1.4, 3.6, 2.4, 1.8)), (3, (0.8, 1.0, 0.9), (1.0, 1.5))]
+    tm = tm[keep].copy()
+    tm[2:4] = 3  # back wall: microfacet
+    tm[4:6] = 4  # left wall: glass
+    return v, idx[keep], tm, mats
+
+
+RAYCAST_CAM = ((0.3, 2.6, -3.0), (0.0, 0.8, 0.5), (0.0, 1.0, 0.0), 60.0, 24, 20)
+
+
+@pytest.mark.parametrize("shade", range(5))
+def test_raycast_colours_the_traced_hits(orc, shade):
+    # one sample per pixel, box film: each pixel is the colour of its camera
+    # ray's closest hit, traced independently here
+    v, idx, tm, mats = _open_room()
+    c2 = orc.Bvh(v, idx).compact2()
+    W, H = RAYCAST_CAM[4], RAYCAST_CAM[5]
+    img = orc.render_raycast(c2, v, idx, tm, mats, RAYCAST_CAM, (0, 0, W, H), 1, 5, shade).reshape(-1, 3)
+    rays, ids, t = _primary(orc, c2, RAYCAST_CAM, 1, 5)
+    hit = ids >= 0
+    assert 0 < hit.sum() < len(ids)  # both hits and misses
+    assert np.all(img[~hit] == 0.0)
+    h = np.nonzero(hit)[0]
+    tri = idx[ids[h]].astype(np.int64)
+    v0, v1, v2 = (v[tri[:, k]].astype(np.float64) for k in range(3))
+    if shade == 0:
+        assert np.array_equal(img[h], RAYCAST_PALETTE[ids[h] % 15])
+    elif shade == 1:
+        cols = np.array([m[1] for m in mats], np.float32)
+        assert np.array_equal(img[h], cols[tm[ids[h]]])
+    elif shade == 2:
+        n = np.cross(v1 - v0, v2 - v0)
+        n /= np.linalg.norm(n, axis=1, keepdims=True)
+        assert np.allclose(img[h], n, atol=1e-6)
+    elif shade == 3:
+        u, w_ = img[h, 0].astype(np.float64), img[h, 1].astype(np.float64)
+        assert np.all(img[h, 2] == 0.0)
+        assert np.all((u > -1e-5) & (w_ > -1e-5) & (u + w_ < 1 + 1e-5))
+        p = rays[h, :3].astype(np.float64) + rays[h, 4:7].astype(np.float64) * t[h, None].astype(np.float64)
+        q = u[:, None] * v0 + w_[:, None] * v1 + (1 - u - w_)[:, None] * v2
+        assert np.allclose(q, p, atol=2e-5)
+    else:
+        assert np.all(img[h] == 1.0)
+
+
+def test_raycast_uniform_is_the_hit_fraction(orc):
+    # uniform shading with spp samples per pixel: the pixel mean is the
+    # fraction of its camera rays that hit anything
+    v, idx, tm, mats = _open_room()
+    c2 = orc.Bvh(v, idx).compact2()
+    W, H = RAYCAST_CAM[4], RAYCAST_CAM[5]
+    spp = 6
+    img = orc.render_raycast(c2, v, idx, tm, mats, RAYCAST_CAM, (0, 0, W, H), spp, 9, 4)
+    _, ids, _ = _primary(orc, c2, RAYCAST_CAM, spp, 9)
+    frac = (ids >= 0).reshape(H, W, spp).mean(axis=2)
+    assert np.allclose(img[..., 0], frac, atol=1e-6)
+    assert np.any((frac > 0) & (frac < 1))  # silhouette pixels are partly covered

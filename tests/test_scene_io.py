@@ -227,6 +227,37 @@ def test_exr_holds_the_half_values(rbm, tmp_path):
             assert np.array_equal(line[c], half[y, :, k].view(np.uint16)), (y, k)
 
 
+@pytest.mark.parametrize("shape", [(29, 37), (8, 8), (1, 1), (17, 3)])
+def test_jpeg_decodes_to_the_8bit_image(rbm, tmp_path, shape):
+    # a baseline JPEG that an independent decoder (Pillow) reads back: at the
+    # reference's quality 100 every quantiser step is 1, so the decoded pixels
+    # are the 8-bit image up to the colour transform's and the DCT's rounding
+    Image = pytest.importorskip("PIL.Image")
+    h, w = shape
+    y, x = np.mgrid[0:h, 0:w]
+    rng = np.random.default_rng(h * 100 + w)
+    img = np.stack([x / max(w - 1, 1), y / max(h - 1, 1), 0.5 + 0.5 * np.sin(x * 0.7 + y * 0.3)], -1)
+    img = (img + rng.uniform(-0.1, 0.1, img.shape)).astype(np.float32)
+    img[0, 0] = [1.5, -0.5, 0.25]  # clamped like the 8-bit writers
+    rbm.write_image(tmp_path / "a.jpg", img)
+    data = (tmp_path / "a.jpg").read_bytes()
+    assert data[:4] == b"\xff\xd8\xff\xe0" and data[-2:] == b"\xff\xd9"
+    with Image.open(tmp_path / "a.jpg") as im:
+        assert im.format == "JPEG" and im.mode == "RGB" and im.size == (w, h)
+        got = np.asarray(im, np.int64)
+    err = np.abs(got - _to_byte(img))
+    assert err.max() <= 3 and err.mean() < 1.0, (err.max(), err.mean())
+
+
+def test_jpeg_of_a_flat_image_is_exact(rbm, tmp_path):
+    Image = pytest.importorskip("PIL.Image")
+    img = np.full((16, 24, 3), 0.5, np.float32)
+    rbm.write_image(tmp_path / "flat.jpeg", img)
+    with Image.open(tmp_path / "flat.jpeg") as im:
+        got = np.asarray(im, np.int64)
+    assert np.abs(got - _to_byte(img)).max() <= 1
+
+
 def test_image_write_rejects_unknown_suffix(rbm, tmp_path):
     with pytest.raises(rbm.RbError):
         rbm.write_image(tmp_path / "a.tiff", _img())
