@@ -558,3 +558,69 @@ def test_raycast_uniform_is_the_hit_fraction(orc):
     frac = (ids >= 0).reshape(H, W, spp).mean(axis=2)
     assert np.allclose(img[..., 0], frac, atol=1e-6)
     assert np.any((frac > 0) & (frac < 1))  # silhouette pixels are partly covered
+
+
+# ------------------------------------------------------------- furnace ----
+
+def _icosphere(sub):
+    t = (1 + 5 ** 0.5) / 2
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    v = [np.array(p, float) / np.linalg.norm(p) for p in v]
+    for _ in range(sub):
+        cache, nf = {}, []
+
+        def mid(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in cache:
+                m = v[a] + v[b]
+                v.append(m / np.linalg.norm(m))
+                cache[k] = len(v) - 1
+            return cache[k]
+        for a, b, c in f:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    return np.array(v, np.float32), np.array(f, np.uint32)
+
+
+def _furnace(albedo, le):
+    """Inside a sphere every surface element sees every other with a form
+    factor proportional to its area, so with a non-reflecting emitter cap of
+    area fraction f (radiance Le) and Lambert walls of albedo a the walls'
+    radiance solves L = a (f Le + (1 - f) L): L = a f Le / (1 - a (1 - f))."""
+    v, idx = _icosphere(3)
+    c = v[idx].mean(1)
+    n = np.cross(v[idx[:, 1]] - v[idx[:, 0]], v[idx[:, 2]] - v[idx[:, 0]])
+    flip = (n * c).sum(1) > 0  # wind every face to face the centre
+    idx[flip] = idx[flip][:, [0, 2, 1]]
+    cap = c[:, 1] > 0.75
+    area = 0.5 * np.linalg.norm(np.cross(v[idx[:, 1]] - v[idx[:, 0]], v[idx[:, 2]] - v[idx[:, 0]]), axis=1)
+    f = area[cap].sum() / area.sum()
+    tm = cap.astype(np.uint32)
+    mats = [(0, (albedo,) * 3), (2, (le,) * 3)]
+    lights = np.nonzero(cap)[0].astype(np.uint32)
+    cam = ((0.0, 0.0, 0.0), (0.0, -1.0, 0.0), (0.0, 0.0, 1.0), 60.0, 16, 16)  # looks at the walls only
+    return v, idx, tm, mats, lights, cam, albedo * f * le / (1 - albedo * (1 - f))
+
+
+@pytest.mark.parametrize("albedo", [0.5, 0.8])
+def test_pt_spherical_furnace_closed_form(orc, albedo):
+    # every bounce of the multi-bounce estimator (NEE + MIS, roulette, depth)
+    # enters the geometric series; the faceted sphere is within 0.1% of round
+    v, idx, tm, mats, lights, cam, exp = _furnace(albedo, 4.0)
+    c2 = orc.Bvh(v, idx).compact2()
+    got = np.mean([orc.render_pt(c2, v, idx, tm, mats, lights, cam, (0, 0, 16, 16), 64, s, 60, 3, 0.5).mean()
+                   for s in range(4)])
+    assert abs(got / exp - 1) < 0.004, (got, exp)
+
+
+def test_bdpt_spherical_furnace_closed_form(orc):
+    v, idx, tm, mats, lights, cam, exp = _furnace(0.5, 4.0)
+    c2 = orc.Bvh(v, idx).compact2()
+    got = np.mean([orc.render_bdpt(c2, v, idx, tm, mats, lights, cam, (0, 0, 16, 16), 64, s, 24, 3, 0.5, 4096, 6,
+                                   3).mean() for s in range(4)])
+    assert abs(got / exp - 1) < 0.006, (got, exp)
