@@ -306,3 +306,24 @@ def test_device_function_trace_equals_persistent_kernel(gpu, small, room):
     assert np.array_equal(a[0], s[0]) and np.array_equal(a[1].view(np.uint32), s[1].view(np.uint32))
     view = rt.device_view()
     assert view.nodes and view.tris and view.index and view.n_refs >= view.n_nodes // 2
+
+
+def test_trace_matches_the_golden_fixture(gpu):
+    """The HIP path on the seeded 2k-triangle scene against the committed
+    fixture: closest and any-hit (id, t) and per-ray visit counts."""
+    import importlib.util
+    import os
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("make_fixtures", os.path.join(golden, "make_fixtures.py"))
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    g = np.load(os.path.join(golden, "trace_2k.npz"))
+    v, idx, rays = mf.trace_2k_inputs()
+    p = gpu.Platform()
+    p.set_leaf_preferences(1, mf.SCENE["max_leaf"])
+    tr = gpu.CudaTracer(gpu.BVH((v, idx), p))
+    ids, ts, cnt = tr.trace_counts(rays)
+    assert np.array_equal(ids, g["ids"]) and np.array_equal(ts.view(np.uint32), g["t"].view(np.uint32))
+    assert np.array_equal(cnt, g["counts"])
+    aid, at = tr.trace(rays, any_hit=True)
+    assert np.array_equal(aid, g["any_ids"]) and np.array_equal(at.view(np.uint32), g["any_t"].view(np.uint32))

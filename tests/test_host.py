@@ -346,3 +346,26 @@ def test_build_cached_builds_once_then_loads(rbm, tmp_path):
     # the thread count does not enter the key (the tree does not depend on it)
     _, hit3 = rbm.BVH.build_cached(s, tmp_path, params=rbm.BuildParams(threads=3))
     assert hit3
+
+
+def test_product_builder_matches_the_golden_fixture(rbm):
+    """librb.so's parallel builder and Compact2 exporter give the bytes the
+    committed fixture holds for the seeded 2k-triangle scene."""
+    import hashlib
+    import importlib.util
+    import os
+    from renderbox2_amd import _lib as L
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    spec = importlib.util.spec_from_file_location("make_fixtures", os.path.join(golden, "make_fixtures.py"))
+    mf = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mf)
+    g = np.load(os.path.join(golden, "trace_2k.npz"))
+    v, idx, _ = mf.trace_2k_inputs()
+    p = rbm.Platform()
+    p.set_leaf_preferences(1, mf.SCENE["max_leaf"])
+    for threads in (1, 8):
+        b = rbm.BVH((v, idx), p, rbm.BuildParams(threads=threads))
+        n, w, x = b.export(L.RB_LAYOUT_REF_COMPACT2)
+        assert [n.size, w.size, x.size] == g["sizes"].tolist()
+        for name, a in (("sha_nodes", n), ("sha_woop", w), ("sha_index", x)):
+            assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest() == g[name].tobytes(), (name, threads)

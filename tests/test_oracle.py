@@ -4,10 +4,14 @@ No reference test, fixture or golden vector exists for this path (the
 reference ships no tests), so every oracle function is pinned here by
 closed forms, FP64 brute force, invariants and published known answers.
 """
+import os
+
 import numpy as np
 import pytest
 
 from conftest import random_rays, random_triangles
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # ---------------------------------------------------------------- math --
 
@@ -56,12 +60,43 @@ def test_woopify_gives_plane_distance_and_barycentrics(orc):
 
 
 def test_philox_known_answers(orc):
-    # Random123 kat_vectors for philox4x32_10 (Salmon et al., SC'11).
-    assert [hex(x) for x in orc.philox([0, 0, 0, 0], [0, 0])] == ["0x6627e8d5", "0xe169c58d", "0xbc57ac4c", "0x9b00dbd8"]
-    assert [hex(x) for x in orc.philox([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2)] == \
-        ["0x408f276d", "0x41c83b0e", "0xa20bc7c6", "0x6d5451fd"]
-    assert [hex(x) for x in orc.philox([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0])] == \
-        ["0xd16cfe09", "0x94fdcceb", "0x5001e420", "0x24126ea1"]
+    # Random123 kat_vectors for philox4x32_10 (Salmon et al., SC'11), tests/golden/philox4x32_10_kat.json
+    import json
+    with open(os.path.join(GOLDEN, "philox4x32_10_kat.json")) as f:
+        kat = json.load(f)
+    assert len(kat) == 3
+    for v in kat:
+        assert list(orc.philox(v["counter"], v["key"])) == v["out"]
+
+
+def _golden_module():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_fixtures", os.path.join(GOLDEN, "make_fixtures.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_oracle_reproduces_the_golden_trace_fixture(orc):
+    """The oracle's SBVH, Compact2 bytes and traversal (ids, t, visit counts)
+    on the seeded 2k-triangle scene equal the committed fixture."""
+    import hashlib
+    mf = _golden_module()
+    g = np.load(os.path.join(GOLDEN, "trace_2k.npz"))
+    v, idx, rays = mf.trace_2k_inputs()
+    b = orc.Bvh(v, idx, max_leaf=mf.SCENE["max_leaf"])
+    nodes, woop, tidx = b.compact2()
+    assert [nodes.size, woop.size, tidx.size] == g["sizes"].tolist()
+    for name, a in (("sha_nodes", nodes), ("sha_woop", woop), ("sha_index", tidx)):
+        assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest() == g[name].tobytes(), name
+    st = b.stats()
+    assert [st[k] for k in ("n_inner", "n_leaf", "n_tris", "n_child_nodes", "duplicates", "max_depth")] == \
+        g["stats"].tolist()
+    ids, t, cnt = orc.trace_compact2(nodes, woop, tidx, rays, counts=True)
+    assert np.array_equal(ids, g["ids"]) and np.array_equal(t.view(np.uint32), g["t"].view(np.uint32))
+    assert np.array_equal(cnt, g["counts"])
+    aid, at = orc.trace_compact2(nodes, woop, tidx, rays, any_hit=True)
+    assert np.array_equal(aid, g["any_ids"]) and np.array_equal(at.view(np.uint32), g["any_t"].view(np.uint32))
 
 
 def test_sincos_2pi_against_float64(orc):
