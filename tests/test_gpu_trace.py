@@ -327,3 +327,36 @@ def test_trace_matches_the_golden_fixture(gpu):
     assert np.array_equal(cnt, g["counts"])
     aid, at = tr.trace(rays, any_hit=True)
     assert np.array_equal(aid, g["any_ids"]) and np.array_equal(at.view(np.uint32), g["any_t"].view(np.uint32))
+
+
+def test_largest_batch_one_call_takes(gpu, small):
+    """2^31 - 1 rays in one rb_trace call (68.7 GB of rays, 17.2 GB of hits,
+    every ray and hit offset past 2^31 bytes exercised): a 2^20-ray set tiled
+    over the whole batch, so every hit must equal the hit of its ray in the
+    small set, traced alone."""
+    from renderbox2_amd import _lib as L
+    v, idx, b, tr, c2 = small
+    nb = 1 << 20
+    base = random_rays(nb, seed=91)
+    bid, bt = tr.trace(base)
+    n = 0x7FFFFFFF
+    d_rays = gpu.DeviceBuffer(n * 32)
+    d_rays.upload(base)
+    filled = nb
+    while filled < n:  # doubling copies of the filled prefix
+        k = min(filled, n - filled)
+        L.check(L.lib().rb_memcpy_dtod(d_rays.ptr + filled * 32, d_rays.ptr, k * 32, None))
+        filled += k
+    d_hits = gpu.DeviceBuffer(n * 8)
+    tr.trace_device(d_rays, n, d_hits)
+    L.check(L.lib().rb_stream_synchronize(None))
+    d_rays.free()
+    chunk = 1 << 24
+    exp_id, exp_t = np.tile(bid, chunk // nb), np.tile(bt.view(np.uint32), chunk // nb)
+    for s in range(0, n, chunk):
+        cnt = min(chunk, n - s)
+        out = np.empty(cnt, gpu.HIT_DTYPE)
+        L.check(L.lib().rb_memcpy_dtoh(out.ctypes.data, d_hits.ptr + s * 8, cnt * 8, None))
+        assert np.array_equal(out["id"], exp_id[:cnt]), s
+        assert np.array_equal(out["t"].view(np.uint32), exp_t[:cnt]), s
+    d_hits.free()
