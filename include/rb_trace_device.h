@@ -67,6 +67,10 @@ __device__ __forceinline__ rb_hit rb_trace_one(const rb_bvh_dev& b, const rb_ray
             const float4* np = nodes + 4 * (size_t)nodeAddr;
             const float4 n0 = np[0], n1 = np[1], nz = np[2];
             const int4 lk = *reinterpret_cast<const int4*>(np + 3);
+            /* keep the four loads of the record together (the link load would
+               otherwise be sunk below the slab tests: a second round trip) */
+            asm volatile("" ::"v"(n0.x), "v"(n0.y), "v"(n0.z), "v"(n0.w), "v"(n1.x), "v"(n1.y), "v"(n1.z),
+                         "v"(n1.w), "v"(nz.x), "v"(nz.y), "v"(nz.z), "v"(nz.w), "v"(lk.x), "v"(lk.y));
             const float c0lox = fmaf(n0.x, idx, -oodx), c0hix = fmaf(n0.y, idx, -oodx);
             const float c0loy = fmaf(n0.z, idy, -oody), c0hiy = fmaf(n0.w, idy, -oody);
             const float c0loz = fmaf(nz.x, idz, -oodz), c0hiz = fmaf(nz.y, idz, -oodz);

@@ -41,11 +41,11 @@ static rb_status init_locked(int device) {
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
     // the autotuner's pick on 2 Mi incoherent rays at 1M triangles
-    g_dev.cfg.block = 128;
+    g_dev.cfg.block = 256;
     g_dev.cfg.lds_stack = 24;
     g_dev.cfg.chunk = 64;
     g_dev.cfg.fetch_threshold = 48;
-    g_dev.cfg.waves_per_cu = 32;
+    g_dev.cfg.waves_per_cu = 24;
     g_dev.cfg.speculative = 0;
     g_dev.cfg.leaf_gate = 16;
     g_dev.device = device;

@@ -68,6 +68,21 @@ struct BatchTable {
     BatchDesc d[kMaxBatch];
 };
 
+// Global-memory pointer types.  A pointer read from memory (the LDS batch
+// table) is generic to the compiler and loads through it become flat loads,
+// which wait on both the vector-memory and the LDS counters; the scene and
+// ray pointers of a lane are held in these types so every access is a
+// global one.
+// Plain clang vectors (no constructors) so the same loads compile in the
+// host pass of the source.
+typedef float V4f __attribute__((ext_vector_type(4)));
+typedef int V4i __attribute__((ext_vector_type(4)));
+typedef int V2i __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) const V4f* GF4;
+typedef __attribute__((address_space(1))) const V4i* GI4;
+typedef __attribute__((address_space(1))) const uint32_t* GU32;
+typedef __attribute__((address_space(1))) V2i* GI2;
+
 template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC>
 __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays) {
     __shared__ int lstack[S * BLOCK];
@@ -86,9 +101,17 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                 spill[sp - S] = (v);                          \
         }                                                     \
     } while (0)
+    // Pops happen only at sp >= 0 (the sentinel at entry 0 ends the ray).  The
+    // LDS read is unconditional (the empty asm takes its value, so it cannot
+    // be folded away) and the spill read a separate branch: a select between
+    // the two arrays would become one generic (flat) load, which waits on
+    // both the vector-memory and the LDS counters.
 #define RB_POP(dst)                                                        \
     do {                                                                   \
-        (dst) = sp < S ? lstack[sp * BLOCK + tid] : spill[sp - S];         \
+        int v_ = lstack[(sp < S ? sp : S - 1) * BLOCK + tid];              \
+        asm volatile("" : "+v"(v_));                                       \
+        if (sp >= S) v_ = spill[sp - S];                                   \
+        (dst) = v_;                                                        \
         --sp;                                                              \
     } while (0)
 
@@ -100,10 +123,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     float idx_ = 0.f, idy_ = 0.f, idz_ = 0.f, oodx = 0.f, oody = 0.f, oodz = 0.f;
     uint32_t c_inner = 0, c_tri = 0, c_leaf = 0;
     // scene of the lane's current ray (uniform in single-batch mode)
-    const float4* nodes = a.nodes;
-    const float4* tris = a.tris;
-    const uint32_t* index = a.index;
-    int2* hits = a.hits;
+    GF4 nodes = (GF4)a.nodes;
+    GF4 tris = (GF4)a.tris;
+    GU32 index = (GU32)a.index;
+    GI2 hits = (GI2)a.hits;
     bool any = ANY_T;
 
     while (true) {
@@ -129,7 +152,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
             }
             if (term) {
                 if (mine >= nrays) break;
-                const float4* rays = a.rays;
+                GF4 rays = (GF4)a.rays;
                 rayidx = mine;
                 if (MULTI) {
                     uint32_t lo = 0, hi = nb;
@@ -139,16 +162,16 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                         else hi = mid;
                     }
                     const BatchDesc& B = tab[lo];
-                    nodes = B.nodes;
-                    tris = B.tris;
-                    index = B.index;
-                    hits = B.hits;
-                    rays = B.rays;
+                    nodes = (GF4)B.nodes;
+                    tris = (GF4)B.tris;
+                    index = (GU32)B.index;
+                    hits = (GI2)B.hits;
+                    rays = (GF4)B.rays;
                     any = B.any != 0;
                     rayidx = mine - B.start;
                 }
-                const float4 o = rays[2 * (size_t)rayidx];
-                const float4 d = rays[2 * (size_t)rayidx + 1];
+                const V4f o = rays[2 * (size_t)rayidx];
+                const V4f d = rays[2 * (size_t)rayidx + 1];
                 ox = o.x; oy = o.y; oz = o.z; tmin = o.w;
                 dx = d.x; dy = d.y; dz = d.z; hitT = d.w;
                 const float ooeps = 0x1p-80f;
@@ -169,11 +192,17 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
         // ---- traversal ----
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel && (SPEC || leafAddr >= 0)) {
-                const float4* np = nodes + 4 * (size_t)nodeAddr;
-                const float4 n0 = np[0];
-                const float4 n1 = np[1];
-                const float4 nz = np[2];
-                const int4 lk = *reinterpret_cast<const int4*>(np + 3);
+                const GF4 np = nodes + 4 * (size_t)nodeAddr;
+                const V4f n0 = np[0];
+                const V4f n1 = np[1];
+                const V4f nz = np[2];
+                const V4i lk = *(GI4)(np + 3);
+                // All four loads of the record in flight at once: without this
+                // the compiler sinks the link load below the slab tests (it is
+                // only read when a child is hit), a second memory round trip
+                // per visited node.  The empty asm consumes every loaded value.
+                asm volatile("" ::"v"(n0.x), "v"(n0.y), "v"(n0.z), "v"(n0.w), "v"(n1.x), "v"(n1.y), "v"(n1.z),
+                             "v"(n1.w), "v"(nz.x), "v"(nz.y), "v"(nz.z), "v"(nz.w), "v"(lk.x), "v"(lk.y));
                 if (COUNT) c_inner++;
                 const float c0lox = fmaf(n0.x, idx_, -oodx), c0hix = fmaf(n0.y, idx_, -oodx);
                 const float c0loy = fmaf(n0.z, idy_, -oody), c0hiy = fmaf(n0.w, idy_, -oody);
@@ -219,10 +248,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                 const uint32_t cnt = l & 31u;
                 if (COUNT) c_leaf++;
                 for (uint32_t k = 0; k < cnt; k++) {
-                    const float4* w = tris + 3 * (size_t)(first + k);
-                    const float4 v0 = w[0];
-                    const float4 v1 = w[1];
-                    const float4 v2 = w[2];
+                    const GF4 w = tris + 3 * (size_t)(first + k);
+                    const V4f v0 = w[0];
+                    const V4f v1 = w[1];
+                    const V4f v2 = w[2];
                     if (COUNT) c_tri++;
                     const float Oz = fmaf(-oz, v0.z, fmaf(-oy, v0.y, fmaf(-ox, v0.x, v0.w)));
                     const float invDz = 1.0f / fmaf(dz, v0.z, fmaf(dy, v0.y, dx * v0.x));
@@ -259,7 +288,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
         }
 
         if (nodeAddr == kSentinel) {
-            int2 r;
+            V2i r;
             r.x = hitIndex < 0 ? -1 : (int)index[hitIndex];
             r.y = __float_as_int(hitT);
             hits[rayidx] = r;
