@@ -386,9 +386,9 @@ def main():
     ap.add_argument("--served-policy", default="deadline", choices=["deadline", "fifo"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU design is described in DESIGN.md, no figure at several GPUs is produced)",
-              file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(rays shard by replicating the scene and splitting each batch; no figure at several GPUs is "
+              "produced here)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
