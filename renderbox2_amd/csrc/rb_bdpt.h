@@ -46,18 +46,48 @@ struct LightPath {
     int2* hit;
 };
 
-__device__ __forceinline__ uint32_t wave_append2(uint32_t* counter, bool want) {
-    const uint64_t m = __ballot(want);
-    uint32_t base = 0;
-    if (m) {
-        const int leader = __builtin_ctzll(m);
-        if ((int)__lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-        base = __shfl(base, leader);
-    }
-    const uint32_t lane = __lane_id();
+// Queue kernels (shade, connect, path and light-path init / extension) run
+// kQueueBlock-thread workgroups whose loops take the same number of trips in
+// every thread of a workgroup, as block_reserve needs.
+constexpr uint32_t kQueueBlock = 512;
+
+// Reserves c (0..31) consecutive slots of the queue whose length is
+// *counter for the calling thread and returns the first (meaningless when
+// c == 0).  The whole workgroup reserves at once: wave prefix sums from
+// bit-plane ballots, wave offsets through LDS, one global atomic per call.
+// Appends to one queue counter from every wave of the chip serialise where
+// the device-scope atomics are performed (~10 ns each, one per wave made the
+// queue kernels atomic-bound); per workgroup they are kQueueBlock / 64 times
+// fewer.  Every thread of the workgroup must call it.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t c) {
+    __shared__ uint32_t s_off[kQueueBlock / 64 + 1];
+    const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-    return want ? base + (uint32_t)__popcll(m & lt) : 0xFFFFFFFFu;
+    uint32_t pre = 0, tot = 0;
+    for (uint32_t b = 0; b < 5; b++) {
+        const uint64_t m = __ballot((c >> b) & 1u);
+        pre += (uint32_t)__popcll(m & lt) << b;
+        tot += (uint32_t)__popcll(m) << b;
+    }
+    if (lane == 0) s_off[wave] = tot;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t sum = 0;
+        for (uint32_t w = 0; w < nw; w++) {
+            const uint32_t t = s_off[w];
+            s_off[w] = sum;
+            sum += t;
+        }
+        s_off[kQueueBlock / 64] = sum ? atomicAdd(counter, sum) : 0u;
+    }
+    __syncthreads();
+    const uint32_t base = s_off[kQueueBlock / 64] + s_off[wave] + pre;
+    __syncthreads();  // s_off is reused by the next call
+    return base;
 }
+
+// End of a workgroup-uniform loop over n items.
+__device__ __forceinline__ uint32_t uniform_end(uint32_t n) { return (n + blockDim.x - 1) / blockDim.x * blockDim.x; }
 
 // uniform_sample_hemisphere (src/core/montecarlo.cu:116-124) about unit n.
 __device__ __forceinline__ V3 uniform_hemisphere(V3 n, float u1, float u2) {
@@ -96,9 +126,10 @@ RB_HD float technique_weight(uint32_t mask, uint32_t n, uint32_t max_depth, uint
     return c ? 1.0f / (float)c : 0.0f;
 }
 
-__global__ void k_light_emit(uint32_t npaths, SceneDev S, RngArg rng_a, LightPath P, LvcBufs C) {
+__global__ __launch_bounds__(kQueueBlock) void k_light_emit(uint32_t npaths, SceneDev S, RngArg rng_a, LightPath P,
+                                                             LvcBufs C) {
     const Rng rng = rng_a.get();
-    const uint32_t end = (npaths + 63u) & ~63u;
+    const uint32_t end = uniform_end(npaths);
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
         const bool valid = j < npaths;
         if (valid) {
@@ -126,15 +157,16 @@ __global__ void k_light_emit(uint32_t npaths, SceneDev S, RngArg rng_a, LightPat
             P.ray[2 * (size_t)j + 1] = make_float4(dir.x, dir.y, dir.z, 3.402823466e+38f);
             P.mask[j] = 0;
         }
-        const uint32_t q = wave_append2(&P.cnt[0], valid && S.n_lights > 0);
+        const uint32_t q = block_reserve(&P.cnt[0], valid && S.n_lights > 0 ? 1u : 0u);
         if (valid && S.n_lights > 0) P.q[0][q] = j;
     }
 }
 
-__global__ void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, RngArg rng_a, LightPath P, LvcBufs C) {
+__global__ __launch_bounds__(kQueueBlock) void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, RngArg rng_a,
+                                                            LightPath P, LvcBufs C) {
     const Rng rng = rng_a.get();
     const uint32_t count = P.cnt[qi];
-    const uint32_t end = (count + 63u) & ~63u;
+    const uint32_t end = uniform_end(count);
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += gridDim.x * blockDim.x) {
         const bool valid = q < count;
         bool next = false;
@@ -202,7 +234,7 @@ __global__ void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, RngArg rng_a, L
                 }
             }
         }
-        const uint32_t o = wave_append2(&P.cnt[qi ^ 1], next);
+        const uint32_t o = block_reserve(&P.cnt[qi ^ 1], next ? 1u : 0u);
         if (next) P.q[qi ^ 1][o] = j;
     }
 }

@@ -114,20 +114,25 @@ struct BdArgs {
     RngArg crng;
 };
 
-__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool want) { return wave_append2(counter, want); }
-
-__global__ void k_pt_init(const int2* __restrict__ hits, const float4* __restrict__ prim, uint32_t n, PathBufs P) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int2 h = hits[i];
-        P.T[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-        P.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        P.ray[2 * (size_t)i] = prim[2 * (size_t)i];
-        P.ray[2 * (size_t)i + 1] = prim[2 * (size_t)i + 1];
-        P.hit[i] = h;
-        P.info[i] = 0;
-        if (P.cmask) P.cmask[i] = 0;
-        const uint32_t q = wave_append(&P.cnt[0], h.x >= 0);
-        if (h.x >= 0) P.q[0][q] = i;
+__global__ __launch_bounds__(kQueueBlock) void k_pt_init(const int2* __restrict__ hits,
+                                                          const float4* __restrict__ prim, uint32_t n, PathBufs P) {
+    const uint32_t end = uniform_end(n);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
+        const bool valid = i < n;
+        bool live = false;
+        if (valid) {
+            const int2 h = hits[i];
+            P.T[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+            P.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+            P.ray[2 * (size_t)i] = prim[2 * (size_t)i];
+            P.ray[2 * (size_t)i + 1] = prim[2 * (size_t)i + 1];
+            P.hit[i] = h;
+            P.info[i] = 0;
+            if (P.cmask) P.cmask[i] = 0;
+            live = h.x >= 0;
+        }
+        const uint32_t q = block_reserve(&P.cnt[0], live ? 1u : 0u);
+        if (live) P.q[0][q] = i;
     }
 }
 
@@ -156,13 +161,15 @@ __global__ void k_count_rays(const uint32_t* c, unsigned long long* total, unsig
 // One bounce of shading for the queued camera paths.  BD: also the
 // light-vertex-cache connections and the technique weights.
 template <bool BD>
-__global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngArg rng_a, uint32_t max_depth,
-                           uint32_t rr_start, float rr_prob, BdArgs B) {
+__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pt_shade(
+    PathBufs P, uint32_t qi, TileArgs T, SceneDev S,
+                                                          RngArg rng_a, uint32_t max_depth, uint32_t rr_start,
+                                                          float rr_prob, BdArgs B) {
     const Rng rng = rng_a.get();
     const Rng crng = BD ? B.crng.get() : rng;
     const uint32_t count = P.cnt[qi];
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t end = (count + 63u) & ~63u;  // whole waves iterate together (wave_append)
+    const uint32_t end = uniform_end(count);  // whole workgroups iterate together (block_reserve)
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
         const bool valid = q < count;
         const uint32_t pid = valid ? P.q[qi][q] : 0u;
@@ -353,10 +360,13 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
                 P.slot[pid] = slot;
             }
         }
-        // wave-aggregated appends (every lane of the wave reaches these)
-        const uint32_t sa = wave_append(&P.cnt[2], wantA);
-        const uint32_t sb = wave_append(&P.cnt[3], wantB);
-        const uint32_t sc = wave_append(&P.cnt[3], wantC);
+        // workgroup-aggregated appends (every thread of the workgroup reaches
+        // these): shadow queue = light sample, then the connections; closest
+        // queue = MIS ray, then the bounce ray
+        const uint32_t nconn_w = BD ? (uint32_t)__popc(conn_want) : 0u;
+        const uint32_t sa = block_reserve(&P.cnt[2], (wantA ? 1u : 0u) + nconn_w);
+        const uint32_t sb = block_reserve(&P.cnt[3], (wantB ? 1u : 0u) + (wantC ? 1u : 0u));
+        const uint32_t sc = sb + (wantB ? 1u : 0u);
         if (wantA) {
             P.ra[2 * (size_t)sa] = ra0;
             P.ra[2 * (size_t)sa + 1] = ra1;
@@ -377,9 +387,10 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
             P.slot[pid] = s;
         }
         if (BD) {
+            uint32_t next_slot = sa + (wantA ? 1u : 0u);
             for (uint32_t i = 0; i < B.nconn; i++) {
                 const bool want = (conn_want >> i) & 1u;
-                const uint32_t si = wave_append(&P.cnt[2], want);
+                const uint32_t si = want ? next_slot++ : 0u;
                 if (valid) P.cslot[(size_t)pid * kMaxConn + i] = want ? (int)si : -1;
                 if (want) {
                     P.ra[2 * (size_t)si] = make_float4(p.x, p.y, p.z, eps);
@@ -391,10 +402,10 @@ __global__ void k_pt_shade(PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngA
 }
 
 template <bool BD>
-__global__ void k_pt_connect(PathBufs P, uint32_t qi, float nlights, uint32_t nconn) {
+__global__ __launch_bounds__(kQueueBlock) void k_pt_connect(PathBufs P, uint32_t qi, float nlights, uint32_t nconn) {
     const uint32_t count = P.cnt[qi];
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t end = (count + 63u) & ~63u;
+    const uint32_t end = uniform_end(count);
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
         const bool valid = q < count;
         bool next = false;
@@ -440,7 +451,7 @@ __global__ void k_pt_connect(PathBufs P, uint32_t qi, float nlights, uint32_t nc
                 }
             }
         }
-        const uint32_t o = wave_append(&P.cnt[qi ^ 1], next);
+        const uint32_t o = block_reserve(&P.cnt[qi ^ 1], next ? 1u : 0u);
         if (next) P.q[qi ^ 1][o] = pid;
     }
 }
@@ -606,6 +617,14 @@ static rb_status film(const float4* L, const rb_camera* cam, uint32_t x0, uint32
 static uint32_t grid_of(uint64_t n) {
     uint64_t g = (n + 255) / 256;
     const uint64_t cap = (uint64_t)dev().num_cus * 16;
+    if (g > cap) g = cap;
+    return (uint32_t)(g ? g : 1);
+}
+
+// Grid of the queue kernels (kQueueBlock-thread workgroups, same thread cap).
+static uint32_t grid_q(uint64_t n) {
+    uint64_t g = (n + kQueueBlock - 1) / kQueueBlock;
+    const uint64_t cap = (uint64_t)dev().num_cus * 16 * 256 / kQueueBlock;
     if (g > cap) g = cap;
     return (uint32_t)(g ? g : 1);
 }
@@ -776,17 +795,17 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     if ((st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg)) != RB_OK)
         return st;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, 0u, 1u, 2u);
-    hipLaunchKernelGGL(k_pt_init, dim3(grid_of(n)), dim3(256), 0, s, phits, prim, n, P);
+    hipLaunchKernelGGL(k_pt_init, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, phits, prim, n, P);
     TileArgs T{cam->width, x0, y0, w, p->spp};
     const SceneDev S = scene_of(bvh);
     const RngArg rng = rng_arg(p->seed, RNG_PATH);
     for (uint32_t b = 0, qi = 0; b <= p->max_depth; b++, qi ^= 1) {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
         if (bd)
-            hipLaunchKernelGGL(k_pt_shade<true>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, T, S, rng, p->max_depth,
+            hipLaunchKernelGGL(k_pt_shade<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
                                p->rr_start, p->rr_prob, B);
         else
-            hipLaunchKernelGGL(k_pt_shade<false>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, T, S, rng, p->max_depth,
+            hipLaunchKernelGGL(k_pt_shade<false>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
                                p->rr_start, p->rr_prob, B);
         if ((st = launch_trace(bvh, (rb_ray*)P.ra, na, RB_ANY, nullptr, (rb_hit*)P.ha, nullptr, s, dev().cfg,
                                P.cnt + 2)) != RB_OK)
@@ -795,10 +814,10 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
                                P.cnt + 3)) != RB_OK)
             return st;
         if (bd)
-            hipLaunchKernelGGL(k_pt_connect<true>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, (float)bvh->n_lights,
+            hipLaunchKernelGGL(k_pt_connect<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, (float)bvh->n_lights,
                                B.nconn);
         else
-            hipLaunchKernelGGL(k_pt_connect<false>, dim3(grid_of(n)), dim3(256), 0, s, P, qi, (float)bvh->n_lights,
+            hipLaunchKernelGGL(k_pt_connect<false>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, (float)bvh->n_lights,
                                0u);
         if (p->ray_counter)
             hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt, p->ray_counter,
@@ -868,7 +887,7 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
     const RngArg lrng = rng_arg(p->seed, RNG_LIGHT);
     // light pass (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168)
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
-    hipLaunchKernelGGL(k_light_emit, dim3(grid_of(NL)), dim3(256), 0, s, NL, S, lrng, LP, B.C);
+    hipLaunchKernelGGL(k_light_emit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, NL, S, lrng, LP, B.C);
     for (uint32_t d = 0, qi = 0; d < D; d++, qi ^= 1) {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, qi ^ 1, 2u, 3u);
         hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, qi, LP, lrays);
@@ -876,7 +895,7 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
                                LP.cnt + qi)) != RB_OK)
             return st;
         if (p->ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->ray_counter);
-        hipLaunchKernelGGL(k_light_hit, dim3(grid_of(NL)), dim3(256), 0, s, qi, d, S, lrng, LP, B.C);
+        hipLaunchKernelGGL(k_light_hit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, qi, d, S, lrng, LP, B.C);
     }
     // camera pass (kernel_sk_camera_trace, integrator_bdpt_lvc_sk.cu:170-435)
     return camera_pass(bvh, cam, x0, y0, w, h, p, out, s, M, true, B);
