@@ -39,7 +39,11 @@ __global__ void k_ao_spawn(const float4* __restrict__ prim, const int2* __restri
     for (uint64_t j = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; j < total; j += (uint64_t)gridDim.x * blockDim.x) {
         const uint32_t i = (uint32_t)(j / nao), k = (uint32_t)(j % nao);
         const int2 h = hits[i];
-        if (ray_counter && k == 0 && h.x >= 0) atomicAdd(ray_counter, (unsigned long long)nao);
+        if (ray_counter) {  // AO rays traced: nao per primary hit, one atomic per wave
+            const uint64_t m = __ballot(k == 0 && h.x >= 0);
+            if (m && __lane_id() == (uint32_t)__builtin_ctzll(m))
+                atomicAdd(ray_counter, (unsigned long long)__popcll(m) * nao);
+        }
         if (h.x < 0) {
             mask[j] = 0;
             rays[2 * j] = make_float4(0.f, 0.f, 0.f, 0.f);
