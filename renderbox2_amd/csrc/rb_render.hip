@@ -165,7 +165,7 @@ __global__ void k_count_rays(const uint32_t* c, unsigned long long* total, unsig
 // One bounce of shading for the queued camera paths.  BD: also the
 // light-vertex-cache connections and the technique weights.
 template <bool BD>
-__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pt_shade(
+__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(BD ? 2 : 4))) void k_pt_shade(
     PathBufs P, uint32_t qi, TileArgs T, SceneDev S,
                                                           RngArg rng_a, uint32_t max_depth, uint32_t rr_start,
                                                           float rr_prob, BdArgs B) {
