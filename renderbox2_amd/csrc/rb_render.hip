@@ -776,7 +776,13 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
                              const BdArgs& B) {
     rb_status st;
     const uint32_t n = w * h * p->spp;
-    const uint32_t na = bd ? n * (1 + B.nconn) : n;
+    // one launch traces at most 2^31 - 1 rays: the shadow queue (light
+    // samples + connections) and the MIS + bounce queue (2 per path)
+    const uint64_t na64 = bd ? (uint64_t)n * (1 + B.nconn) : n;
+    if (na64 > 0x7FFFFFFFull || 2ull * n > 0x7FFFFFFFull)
+        return fail(RB_ERR_INVALID, "tile x spp too large: %u paths need %llu shadow rays and %llu bounce rays per launch "
+                    "(at most 2^31 - 1)", n, (unsigned long long)na64, 2ull * n);
+    const uint32_t na = (uint32_t)na64;
     float4* prim = M.get<float4>(2 * (size_t)n);
     int2* phits = M.get<int2>(n);
     PathBufs P;
@@ -870,6 +876,12 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
     if (p->connections > kMaxConn) return fail(RB_ERR_INVALID, "connections must be <= %u", kMaxConn);
     if (p->light_paths > (1u << 24) || (p->light_paths == 0 && p->lvc_capacity == 0))
         return fail(RB_ERR_INVALID, "light_paths must be 1..2^24 (or 0 with an lvc_capacity)");
+    {
+        const uint64_t npaths = (uint64_t)w * h * p->spp;
+        if (npaths * (1 + p->connections) > 0x7FFFFFFFull || 2 * npaths > 0x7FFFFFFFull)
+            return fail(RB_ERR_INVALID, "tile x spp too large for one call: %llu paths x (1 + %u connections) shadow "
+                        "rays (at most 2^31 - 1 per launch)", (unsigned long long)npaths, p->connections);
+    }
     Scratch M(s);
     const uint32_t D = p->light_max_depth;
     // light sub-paths: given, or as many as fill a cache of lvc_capacity vertices

@@ -110,6 +110,21 @@ def test_bdpt_without_connections_is_the_path_tracer(gpu, pt_setup):
     assert np.array_equal(bd.view(np.uint32), pt.view(np.uint32))
 
 
+def test_queues_beyond_one_launch_are_rejected_before_any_work(gpu, pt_setup):
+    """A launch traces at most 2^31 - 1 rays: path counts whose shadow
+    queue (light samples + connections) or bounce queue (2 per path) would
+    pass that are refused on the host, with the reason."""
+    s, b, tr = pt_setup
+    cam = gpu.scenes.pt_camera(1024, 1024)
+    d = gpu.DeviceBuffer(1024 * 1024 * 12)
+    # 2^29 paths: 2^30 bounce rays fit, 2^29 x (1 + 8) shadow rays do not
+    with pytest.raises(gpu.RbError, match="tile x spp too large"):
+        gpu.SamplerRenderer(tr, cam, "bdpt", **dict(BD, spp=512, connections=8)).render_device(d)
+    # 2^30 paths (the largest tile x spp): 2^31 bounce rays do not fit
+    with pytest.raises(gpu.RbError, match="tile x spp too large"):
+        gpu.SamplerRenderer(tr, cam, "pt", spp=1024).render_device(d)
+
+
 def test_bdpt_rejects_bad_arguments(gpu, pt_setup):
     s, b, tr = pt_setup
     cam = gpu.scenes.pt_camera(16, 16)
