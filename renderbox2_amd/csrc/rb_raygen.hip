@@ -148,7 +148,8 @@ rb_status rb_raygen_camera(const rb_camera* cam, uint32_t x0, uint32_t y0, uint3
     if (st != RB_OK) return st;
     if (!cam || !rays) return fail(RB_ERR_INVALID, "NULL camera or rays");
     if (spp == 0 || w == 0 || h == 0) return RB_OK;
-    if (x0 + w > cam->width || y0 + h > cam->height) return fail(RB_ERR_INVALID, "tile outside the film");
+    if ((uint64_t)x0 + w > cam->width || (uint64_t)y0 + h > cam->height)
+        return fail(RB_ERR_INVALID, "tile outside the film");
     CamArgs c;
     if ((st = rb_camera_matrices(cam, c.r2c, c.c2w)) != RB_OK) return st;
     const uint64_t total = (uint64_t)w * h * spp;

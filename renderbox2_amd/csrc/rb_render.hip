@@ -675,7 +675,8 @@ static rb_status check_render(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, u
                               const rb_render_params* p, float* out) {
     if (!bvh || !cam || !p || !out) return fail(RB_ERR_INVALID, "NULL argument");
     if (!bvh->tri_v) return fail(RB_ERR_INVALID, "bvh was uploaded without its scene");
-    if (w == 0 || h == 0 || x0 + w > cam->width || y0 + h > cam->height) return fail(RB_ERR_INVALID, "bad tile");
+    if (w == 0 || h == 0 || (uint64_t)x0 + w > cam->width || (uint64_t)y0 + h > cam->height)
+        return fail(RB_ERR_INVALID, "bad tile");
     if (p->spp == 0) return fail(RB_ERR_INVALID, "spp must be >= 1");
     if ((uint64_t)w * h * p->spp > (1ull << 30)) return fail(RB_ERR_INVALID, "tile x spp too large for one call");
     FilmFilter f;

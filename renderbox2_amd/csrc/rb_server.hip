@@ -425,7 +425,8 @@ rb_status rb_server_submit(rb_server_t s, const rb_request* req, uint64_t* id) {
     auto it = s->scenes.find(req->scene);
     if (it == s->scenes.end()) return fail(RB_ERR_INVALID, "unknown scene %u", req->scene);
     if (req->kind != RB_REQ_TRACE) {
-        if (req->w == 0 || req->h == 0 || req->x0 + req->w > req->camera.width || req->y0 + req->h > req->camera.height)
+        if (req->w == 0 || req->h == 0 || (uint64_t)req->x0 + req->w > req->camera.width ||
+            (uint64_t)req->y0 + req->h > req->camera.height)
             return fail(RB_ERR_INVALID, "bad tile");
         if (req->params.spp == 0) return fail(RB_ERR_INVALID, "spp must be >= 1");
         if (!it->second.bvh->tri_v) return fail(RB_ERR_INVALID, "scene %u has no shading data", req->scene);

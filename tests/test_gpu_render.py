@@ -77,6 +77,8 @@ def test_render_rejects_bad_arguments(gpu, ao_setup):
         gpu.SamplerRenderer(tr, cam, "ao", spp=0).render()
     with pytest.raises(gpu.RbError):
         gpu.SamplerRenderer(tr, cam, "ao").render((20, 20, 16, 16))
+    with pytest.raises(gpu.RbError):  # x0 + w wraps around 2^32
+        gpu.SamplerRenderer(tr, cam, "ao").render((0xFFFFFFF0, 0, 32, 16))
     bare = gpu.CudaTracer(b)  # no scene data uploaded
     with pytest.raises(gpu.RbError):
         gpu.SamplerRenderer(bare, cam, "pt").render()
