@@ -120,6 +120,8 @@ rb_status rb_scene_add_mesh(rb_scene_t s, const float* verts, uint32_t nv, const
         if (idx[i] >= nv) return fail(RB_ERR_INVALID, "mesh triangle %llu indexes vertex %u of %u",
                                       (unsigned long long)(i / 3), idx[i], nv);
     Scene& S = s->s;
+    if ((uint64_t)S.num_verts() + nv > 0xFFFFFFFFull || (uint64_t)S.num_tris() + nt > 0x7FFFFFFFull)
+        return fail(RB_ERR_INVALID, "scene would exceed 2^32 vertices or 2^31 triangles");
     const uint32_t base = S.num_verts();
     const uint32_t tbase = S.num_tris();
     try {
