@@ -118,7 +118,7 @@ void split_reference(const Ctx& C, Ref& L, Ref& R, const Ref& ref, int dim, floa
         if (v0p >= pos) R.b.insert(v0);
         if ((v0p < pos && v1p > pos) || (v0p > pos && v1p < pos)) {
             float t = (pos - v0p) / (v1p - v0p);
-            t = std::fmax(0.0f, std::fmin(t, 1.0f));
+            t = fmax_(0.0f, fmin_(t, 1.0f));
             F3 p = v0 + (v1 - v0) * t;
             L.b.insert(p);
             R.b.insert(p);
