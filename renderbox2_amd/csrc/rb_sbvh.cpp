@@ -104,30 +104,40 @@ struct Ctx {
 };
 
 // split_reference, splitbvhbuilder.cpp:586-627.
-void split_reference(const Ctx& C, Ref& L, Ref& R, const Ref& ref, int dim, float pos) {
-    L.tri = R.tri = ref.tri;
-    L.b = Box();
-    R.b = Box();
-    const uint32_t* ind = C.idx + 3 * (size_t)ref.tri;
-    F3 v1 = C.v(ind[2]);
+inline void split_tri(const F3 v[3], const Box& within, int dim, float pos, Box& L, Box& R) {
+    L = Box();
+    R = Box();
+    F3 v1 = v[2];
     for (int i = 0; i < 3; i++) {
         F3 v0 = v1;
-        v1 = C.v(ind[i]);
+        v1 = v[i];
         float v0p = get(v0, dim), v1p = get(v1, dim);
-        if (v0p <= pos) L.b.insert(v0);
-        if (v0p >= pos) R.b.insert(v0);
+        if (v0p <= pos) L.insert(v0);
+        if (v0p >= pos) R.insert(v0);
         if ((v0p < pos && v1p > pos) || (v0p > pos && v1p < pos)) {
             float t = (pos - v0p) / (v1p - v0p);
             t = fmax_(0.0f, fmin_(t, 1.0f));
             F3 p = v0 + (v1 - v0) * t;
-            L.b.insert(p);
-            R.b.insert(p);
+            L.insert(p);
+            R.insert(p);
         }
     }
-    getr(L.b.hi, dim) = pos;
-    getr(R.b.lo, dim) = pos;
-    L.b.intersect(ref.b);
-    R.b.intersect(ref.b);
+    getr(L.hi, dim) = pos;
+    getr(R.lo, dim) = pos;
+    L.intersect(within);
+    R.intersect(within);
+}
+
+inline void tri_verts(const Ctx& C, uint32_t tri, F3 v[3]) {
+    const uint32_t* ind = C.idx + 3 * (size_t)tri;
+    for (int i = 0; i < 3; i++) v[i] = C.v(ind[i]);
+}
+
+void split_reference(const Ctx& C, Ref& L, Ref& R, const Ref& ref, int dim, float pos) {
+    L.tri = R.tri = ref.tri;
+    F3 v[3];
+    tri_verts(C, ref.tri, v);
+    split_tri(v, ref.b, dim, pos, L.b, R.b);
 }
 
 // Sort (key, position) pairs; parallel merge sort above a size threshold.
@@ -245,21 +255,27 @@ struct Builder {
             const Ref& ref = refs[ri];
             F3 rmin = f3((ref.b.lo.x - origin.x) * inv.x, (ref.b.lo.y - origin.y) * inv.y, (ref.b.lo.z - origin.z) * inv.z);
             F3 rmax = f3((ref.b.hi.x - origin.x) * inv.x, (ref.b.hi.y - origin.y) * inv.y, (ref.b.hi.z - origin.z) * inv.z);
+            F3 tv[3];
+            bool have_tv = false;
             for (int d = 0; d < 3; d++) {
                 int f = trunc_int(get(rmin, d));
                 f = f < 0 ? 0 : (f > kBins - 1 ? kBins - 1 : f);
                 int l = trunc_int(get(rmax, d));
                 l = l > kBins - 1 ? kBins - 1 : l;
                 l = l < f ? f : l;
-                Ref cur = ref;
+                Box cur = ref.b;
+                if (f < l && !have_tv) {  // the triangle's vertices, loaded once per reference
+                    tri_verts(C, ref.tri, tv);
+                    have_tv = true;
+                }
                 for (int i = f; i < l; i++) {
-                    Ref L, R;
-                    split_reference(C, L, R, cur, d, get(origin, d) + get(binSize, d) * (float)(i + 1));
-                    B.b[d][i].grow(L.b);
+                    Box L, R;
+                    split_tri(tv, cur, d, get(origin, d) + get(binSize, d) * (float)(i + 1), L, R);
+                    B.b[d][i].grow(L);
                     B.used[d][i] = true;
                     cur = R;
                 }
-                B.b[d][l].grow(cur.b);
+                B.b[d][l].grow(cur);
                 B.used[d][l] = true;
                 B.enter[d][f]++;
                 B.exit[d][l]++;
