@@ -12,9 +12,15 @@
 //    lengths live in device memory, so the bounce loop needs no host
 //    synchronisation (the reference counts live paths with thrust on the
 //    host every bounce, integrator_bdpt_lvc_mk.cu:529-544);
-//  * film: per-pixel box-filtered average of the spp samples in sample order
-//    (kernel_update_film, src/renderers/samplerrenderer.cu:103-169, without
-//    float atomics, so images are bit-reproducible).
+//    Queue appends are reserved per workgroup (block_reserve), one global
+//    atomic per 512 threads;
+//  * BDPT: the light pass and the camera pass with cache connections
+//    (rb_bdpt.h);
+//  * ray-cast debug shading: camera rays -> closest hit -> colour;
+//  * film: the reference's filtered splat restated as a gather (box,
+//    triangle or Gaussian; kernel_update_film,
+//    src/renderers/samplerrenderer.cu:103-169), in a fixed order and without
+//    float atomics, so images are bit-reproducible.
 #include "rb_shade.h"
 
 namespace rb {
