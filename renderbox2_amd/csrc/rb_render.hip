@@ -11,9 +11,9 @@
 //    next hit) -> compaction of the live paths into the next queue.  Queue
 //    lengths live in device memory, so the bounce loop needs no host
 //    synchronisation (the reference counts live paths with thrust on the
-//    host every bounce, integrator_bdpt_lvc_mk.cu:529-544);
-//    Queue appends are reserved per workgroup (block_reserve), one global
-//    atomic per 512 threads;
+//    host every bounce, integrator_bdpt_lvc_mk.cu:529-544).  Queue appends
+//    are reserved per workgroup (block_reserve): one global atomic per 512
+//    threads;
 //  * BDPT: the light pass and the camera pass with cache connections
 //    (rb_bdpt.h);
 //  * ray-cast debug shading: camera rays -> closest hit -> colour;
