@@ -380,6 +380,7 @@ def main():
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="launch one copy kernel of this size first (traffic counter calibration)")
     ap.add_argument("--render", type=int, default=1, help="also time the AO and path-traced frames")
+    ap.add_argument("--graphs", type=int, default=1, help="with --render: also time hipGraph replay of tiles")
     ap.add_argument("--batched", type=int, default=1, help="also time batched vs plain (8 scenes, 64 batches)")
     ap.add_argument("--served", type=int, default=2000, help="served-mix requests (0 = skip)")
     ap.add_argument("--served-inflight", type=int, default=8)
@@ -533,9 +534,10 @@ def main():
                           lvc_capacity=1 << 20, light_paths=0, connections=5)
         log("bdpt frame (config 5 BDPT: 2,000,136 tris, 512x512, 4 spp, 2^20-vertex cache, depth 6)", bd)
         extra["bdpt_512_4spp"] = bd
-        gr = bench_graphs(rb, built)
-        log("progressive tiles, eager vs hipGraph replay (2,000,136 tris)", gr)
-        extra["graph_replay"] = gr
+        if args.graphs:
+            gr = bench_graphs(rb, built)
+            log("progressive tiles, eager vs hipGraph replay (2,000,136 tris)", gr)
+            extra["graph_replay"] = gr
         del built
     if args.served:
         sv = bench_served(rb, args.served, max_inflight=args.served_inflight, policy=args.served_policy)

@@ -27,11 +27,14 @@
 
 namespace rb {
 
+// Light-vertex cache: one 64-byte record per vertex (one cache line per
+// random gather by a connection), rows
+//   [0] p.xyz | (unused)
+//   [1] geometric normal | material id bits
+//   [2] direction towards the previous light vertex | specular mask bits
+//   [3] light sub-path throughput arriving at the vertex
 struct LvcBufs {
-    float4* pos;    // p.xyz | (unused)
-    float4* ng;     // geometric normal | material id bits
-    float4* wo;     // direction towards the previous light vertex | specular mask bits
-    float4* alpha;  // light sub-path throughput arriving at the vertex
+    float4* v;        // 4 rows per vertex, vertex `k` of light path `j` at slot j * depth + k
     uint32_t* count;  // vertices stored per light path
     uint32_t depth;   // slots per light path
     uint32_t paths;
@@ -191,10 +194,11 @@ __global__ __launch_bounds__(kQueueBlock) void k_light_hit(uint32_t qi, uint32_t
                     const bool delta = is_delta(m);
                     if (delta) mask |= 1u << d;
                     const size_t slot = (size_t)j * C.depth + d;
-                    C.pos[slot] = make_float4(p.x, p.y, p.z, 0.f);
-                    C.ng[slot] = make_float4(ng.x, ng.y, ng.z, __uint_as_float(mid));
-                    C.wo[slot] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mask));
-                    C.alpha[slot] = a;
+                    float4* rec = C.v + 4 * (size_t)slot;
+                    rec[0] = make_float4(p.x, p.y, p.z, 0.f);
+                    rec[1] = make_float4(ng.x, ng.y, ng.z, __uint_as_float(mid));
+                    rec[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mask));
+                    rec[3] = a;
                     C.count[j] = d + 1;
                     if (d + 1 < C.depth) {
                         const U4 r = rng.draw(j, d + 1, 0, 0);

@@ -197,7 +197,8 @@ rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t f
         if (e != hipSuccess) st = hip_fail(e, "fill launch");
     }
     if (tmp) {
-        hipStreamSynchronize(s);
+        const hipError_t e = hipStreamSynchronize(s);
+        if (e != hipSuccess && st == RB_OK) st = hip_fail(e, "fill sync");
         dfree(tmp, (size_t)n * sizeof(rb_hit));
     }
     return st;

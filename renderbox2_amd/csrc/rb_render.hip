@@ -168,15 +168,14 @@ __global__ void k_count_rays(const uint32_t* c, unsigned long long* total, unsig
         atomicAdd(total, (unsigned long long)c[2] + (unsigned long long)c[3] + extra);
 }
 
-// One bounce of shading for the queued camera paths.  BD: also the
-// light-vertex-cache connections and the technique weights.
+// One bounce of shading for the queued camera paths.  BD: with the
+// technique weights; the light-vertex-cache connections follow in
+// k_lvc_connect.
 template <bool BD>
-__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(BD ? 2 : 4))) void k_pt_shade(
-    PathBufs P, uint32_t qi, TileArgs T, SceneDev S,
-                                                          RngArg rng_a, uint32_t max_depth, uint32_t rr_start,
-                                                          float rr_prob, BdArgs B) {
+__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pt_shade(
+    PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngArg rng_a, uint32_t max_depth, uint32_t rr_start,
+    float rr_prob, BdArgs B) {
     const Rng rng = rng_a.get();
-    const Rng crng = BD ? B.crng.get() : rng;
     const uint32_t count = P.cnt[qi];
     const uint32_t stride = gridDim.x * blockDim.x;
     const uint32_t end = uniform_end(count);  // whole workgroups iterate together (block_reserve)
@@ -187,9 +186,6 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(BD 
         float4 ra0, ra1, rb1, rc1;
         V3 p = v3(0.f, 0.f, 0.f);
         float eps = 0.0f;
-        uint32_t conn_want = 0;  // BD: bit i = connection i emits a shadow ray
-        V3 cdir[kMaxConn];
-        float cmax[kMaxConn];
         if (valid) {
             const float4 o = P.ray[2 * (size_t)pid], d = P.ray[2 * (size_t)pid + 1];
             const int2 h = P.hit[pid];
@@ -280,50 +276,6 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(BD 
                         }
                     }
                 }
-                if (BD && !delta) {
-                    // connections to the light-vertex cache (integrator_bdpt_lvc_sk.cu:343-386)
-                    for (uint32_t i = 0; i < B.nconn; i++) {
-                        const U4 rc = crng.draw(gid, bounce, 2 + i / 2, 0);
-                        const uint32_t ua = (i & 1) ? rc.z : rc.x, ub = (i & 1) ? rc.w : rc.y;
-                        float4 contrib = make_float4(0.f, 0.f, 0.f, 0.f);
-                        uint32_t j = (uint32_t)(u01(ua) * (float)B.C.paths);
-                        if (j > B.C.paths - 1) j = B.C.paths - 1;
-                        const uint32_t nj = B.C.count[j];
-                        if (nj) {
-                            uint32_t k = (uint32_t)(u01(ub) * (float)nj);
-                            if (k > nj - 1) k = nj - 1;
-                            const size_t sl = (size_t)j * B.C.depth + k;
-                            const float4 xp = B.C.pos[sl], xn = B.C.ng[sl], xw = B.C.wo[sl], xa = B.C.alpha[sl];
-                            const rb_material mx = S.mats[__float_as_uint(xn.w)];
-                            const V3 seg = vsub(v3(xp.x, xp.y, xp.z), p);
-                            const float d2 = vdot(seg, seg);
-                            const float dist = sqrtf(d2);
-                            const V3 dir = vnorm(seg);
-                            const V3 ngx = v3(xn.x, xn.y, xn.z);
-                            const V3 fy = bsdf_f(m, wo, dir, ng, nf);
-                            const V3 xwv = v3(xw.x, xw.y, xw.z);
-                            const V3 nfx = vdot(ngx, xwv) >= 0.0f ? ngx : vneg(ngx);
-                            const V3 fx = bsdf_f(mx, xwv, vneg(dir), ngx, nfx);
-                            if (!is_black(fy) && !is_black(fx)) {
-                                const float G = fabsf(vdot(ng, dir)) * fabsf(vdot(ngx, vneg(dir))) / d2;
-                                // full path: camera vertices 0..bounce, then light vertices k+1..1
-                                const uint32_t lmask = __float_as_uint(xw.w);
-                                uint32_t M = cm;
-                                for (uint32_t e = 0; e <= k; e++)
-                                    if ((lmask >> (k - e)) & 1u) M |= 1u << (bounce + 1 + e);
-                                const float w = technique_weight(M, bounce + k + 3, max_depth, B.tdepth);
-                                const float sc = (float)nj / (float)B.nconn;
-                                contrib = make_float4(Tv.x * xa.x * fy.x * G * fx.x * w * sc,
-                                                      Tv.y * xa.y * fy.y * G * fx.y * w * sc,
-                                                      Tv.z * xa.z * fy.z * G * fx.z * w * sc, 0.f);
-                                cdir[i] = dir;
-                                cmax[i] = dist * (1.0f - 1.0e-3f);
-                                conn_want |= 1u << i;
-                            }
-                        }
-                        P.cc[(size_t)pid * kMaxConn + i] = contrib;
-                    }
-                }
                 // next bounce (material_sample_f with every component)
                 V3 wi3, f3;
                 float pdf3;
@@ -371,10 +323,9 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(BD 
             }
         }
         // workgroup-aggregated appends (every thread of the workgroup reaches
-        // these): shadow queue = light sample, then the connections; closest
-        // queue = MIS ray, then the bounce ray
-        const uint32_t nconn_w = BD ? (uint32_t)__popc(conn_want) : 0u;
-        const uint32_t sa = block_reserve(&P.cnt[2], (wantA ? 1u : 0u) + nconn_w);
+        // these): shadow queue = light sample; closest queue = MIS ray, then
+        // the bounce ray.  BDPT connection rays are appended by k_lvc_connect.
+        const uint32_t sa = block_reserve(&P.cnt[2], wantA ? 1u : 0u);
         const uint32_t sb = block_reserve(&P.cnt[3], (wantB ? 1u : 0u) + (wantC ? 1u : 0u));
         const uint32_t sc = sb + (wantB ? 1u : 0u);
         if (wantA) {
@@ -396,17 +347,96 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(BD 
             s.z = wantC ? (int)sc : -1;
             P.slot[pid] = s;
         }
-        if (BD) {
-            uint32_t next_slot = sa + (wantA ? 1u : 0u);
-            for (uint32_t i = 0; i < B.nconn; i++) {
-                const bool want = (conn_want >> i) & 1u;
-                const uint32_t si = want ? next_slot++ : 0u;
-                if (valid) P.cslot[(size_t)pid * kMaxConn + i] = want ? (int)si : -1;
-                if (want) {
-                    P.ra[2 * (size_t)si] = make_float4(p.x, p.y, p.z, eps);
-                    P.ra[2 * (size_t)si + 1] = make_float4(cdir[i].x, cdir[i].y, cdir[i].z, cmax[i]);
+    }
+}
+
+// BDPT connections of the queued camera vertices to the light-vertex cache
+// (integrator_bdpt_lvc_sk.cu:343-386), one thread per (path, connection):
+// thread t serves queue entry t / nconn, connection t % nconn.  Runs after
+// k_pt_shade of the same bounce, which leaves the camera vertex (ray, hit,
+// throughput, specular mask, bounce) unchanged for the non-specular vertices
+// that connect.  Writes the contribution and the shadow-ray slot of every
+// connection (-1: no ray) and appends the shadow rays to the shadow queue.
+__global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_t qi, TileArgs T, SceneDev S,
+                                                             uint32_t max_depth, BdArgs B) {
+    const Rng crng = B.crng.get();
+    const uint32_t nconn = B.nconn;
+    const uint32_t total = P.cnt[qi] * nconn;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    const uint32_t end = uniform_end(total);  // whole workgroups iterate together (block_reserve)
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < end; t += stride) {
+        const bool valid = t < total;
+        const uint32_t q = valid ? t / nconn : 0u;
+        const uint32_t i = valid ? t - q * nconn : 0u;
+        const uint32_t pid = valid ? P.q[qi][q] : 0u;
+        bool want = false;
+        float4 r0, r1;
+        if (valid) {
+            const float4 o = P.ray[2 * (size_t)pid], d = P.ray[2 * (size_t)pid + 1];
+            const int2 h = P.hit[pid];
+            const uint32_t bounce = P.info[pid] & 255u;
+            const float th = __int_as_float(h.y);
+            const V3 dv = v3(d.x, d.y, d.z);
+            const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, th));
+            V3 v0, v1, v2;
+            uint32_t mid;
+            tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid);
+            const rb_material m = S.mats[mid];
+            if (m.kind != RB_MAT_EMITTER && !is_delta(m)) {
+                const V3 ng = tri_normal(v0, v1, v2);
+                const V3 wo = vneg(dv);
+                const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
+                const float4 Tp = P.T[pid];
+                const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
+                const uint32_t cm = P.cmask[pid];
+                const uint32_t gid = sample_gid(T, pid);
+                const U4 rc = crng.draw(gid, bounce, 2 + i / 2, 0);
+                const uint32_t ua = (i & 1) ? rc.z : rc.x, ub = (i & 1) ? rc.w : rc.y;
+                float4 contrib = make_float4(0.f, 0.f, 0.f, 0.f);
+                uint32_t j = (uint32_t)(u01(ua) * (float)B.C.paths);
+                if (j > B.C.paths - 1) j = B.C.paths - 1;
+                const uint32_t nj = B.C.count[j];
+                if (nj) {
+                    uint32_t k = (uint32_t)(u01(ub) * (float)nj);
+                    if (k > nj - 1) k = nj - 1;
+                    const size_t sl = (size_t)j * B.C.depth + k;
+                    const float4* rec = B.C.v + 4 * sl;
+                    const float4 xp = rec[0], xn = rec[1], xw = rec[2], xa = rec[3];
+                    const rb_material mx = S.mats[__float_as_uint(xn.w)];
+                    const V3 seg = vsub(v3(xp.x, xp.y, xp.z), p);
+                    const float d2 = vdot(seg, seg);
+                    const float dist = sqrtf(d2);
+                    const V3 dir = vnorm(seg);
+                    const V3 ngx = v3(xn.x, xn.y, xn.z);
+                    const V3 fy = bsdf_f(m, wo, dir, ng, nf);
+                    const V3 xwv = v3(xw.x, xw.y, xw.z);
+                    const V3 nfx = vdot(ngx, xwv) >= 0.0f ? ngx : vneg(ngx);
+                    const V3 fx = bsdf_f(mx, xwv, vneg(dir), ngx, nfx);
+                    if (!is_black(fy) && !is_black(fx)) {
+                        const float G = fabsf(vdot(ng, dir)) * fabsf(vdot(ngx, vneg(dir))) / d2;
+                        // full path: camera vertices 0..bounce, then light vertices k+1..1
+                        const uint32_t lmask = __float_as_uint(xw.w);
+                        uint32_t M = cm;
+                        for (uint32_t e = 0; e <= k; e++)
+                            if ((lmask >> (k - e)) & 1u) M |= 1u << (bounce + 1 + e);
+                        const float w = technique_weight(M, bounce + k + 3, max_depth, B.tdepth);
+                        const float sc = (float)nj / (float)B.nconn;
+                        contrib = make_float4(Tv.x * xa.x * fy.x * G * fx.x * w * sc,
+                                              Tv.y * xa.y * fy.y * G * fx.y * w * sc,
+                                              Tv.z * xa.z * fy.z * G * fx.z * w * sc, 0.f);
+                        r0 = make_float4(p.x, p.y, p.z, 1.0e-3f * th);
+                        r1 = make_float4(dir.x, dir.y, dir.z, dist * (1.0f - 1.0e-3f));
+                        want = true;
+                    }
                 }
+                P.cc[(size_t)pid * kMaxConn + i] = contrib;
             }
+        }
+        const uint32_t si = block_reserve(&P.cnt[2], want ? 1u : 0u);
+        if (valid) P.cslot[(size_t)pid * kMaxConn + i] = want ? (int)si : -1;
+        if (want) {
+            P.ra[2 * (size_t)si] = r0;
+            P.ra[2 * (size_t)si + 1] = r1;
         }
     }
 }
@@ -826,10 +856,13 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     const RngArg rng = rng_arg(p->seed, RNG_PATH);
     for (uint32_t b = 0, qi = 0; b <= p->max_depth; b++, qi ^= 1) {
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
-        if (bd)
+        if (bd) {
             hipLaunchKernelGGL(k_pt_shade<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
                                p->rr_start, p->rr_prob, B);
-        else
+            if (B.nconn)
+                hipLaunchKernelGGL(k_lvc_connect, dim3(grid_q(n * B.nconn)), dim3(kQueueBlock), 0, s, P, qi, T, S,
+                                   p->max_depth, B);
+        } else
             hipLaunchKernelGGL(k_pt_shade<false>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
                                p->rr_start, p->rr_prob, B);
         if ((st = launch_trace(bvh, (rb_ray*)P.ra, na, RB_ANY, nullptr, (rb_hit*)P.ha, nullptr, s, dev().cfg,
@@ -899,10 +932,7 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
     B.crng = rng_arg(p->seed, RNG_CONNECT);
     B.C.depth = D;
     B.C.paths = NL;
-    B.C.pos = M.get<float4>((size_t)NL * D);
-    B.C.ng = M.get<float4>((size_t)NL * D);
-    B.C.wo = M.get<float4>((size_t)NL * D);
-    B.C.alpha = M.get<float4>((size_t)NL * D);
+    B.C.v = M.get<float4>(4 * (size_t)NL * D);
     B.C.count = M.get<uint32_t>(NL);
     LightPath LP;
     LP.ray = M.get<float4>(2 * (size_t)NL);

@@ -24,7 +24,9 @@ struct U4 {
 RB_HD uint32_t mulhi32(uint32_t a, uint32_t b) { return (uint32_t)(((uint64_t)a * b) >> 32); }
 
 RB_HD U4 philox4x32_10(U4 c, uint32_t k0, uint32_t k1) {
+#ifdef __HIPCC__
 #pragma unroll
+#endif
     for (int r = 0; r < 10; r++) {
         const uint32_t hi0 = mulhi32(0xD2511F53u, c.x), lo0 = 0xD2511F53u * c.x;
         const uint32_t hi1 = mulhi32(0xCD9E8D57u, c.z), lo1 = 0xCD9E8D57u * c.z;

@@ -497,13 +497,14 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
     auto measure = [&](const rb_trace_config& c, float* ms) -> rb_status {
         rb_status r = launch_trace(bvh, rays, n, flags, nullptr, hits, nullptr, s, c);  // warm-up
         if (r != RB_OK) return r;
-        hipEventRecord(e0, s);
+        hipError_t e = hipEventRecord(e0, s);
+        if (e != hipSuccess) return hip_fail(e, "autotune event");
         for (uint32_t i = 0; i < reps; i++)
             if ((r = launch_trace(bvh, rays, n, flags, nullptr, hits, nullptr, s, c)) != RB_OK) return r;
-        hipEventRecord(e1, s);
-        hipError_t e = hipEventSynchronize(e1);
+        if ((e = hipEventRecord(e1, s)) != hipSuccess) return hip_fail(e, "autotune event");
+        e = hipEventSynchronize(e1);
         if (e != hipSuccess) return hip_fail(e, "autotune sync");
-        hipEventElapsedTime(ms, e0, e1);
+        if ((e = hipEventElapsedTime(ms, e0, e1)) != hipSuccess) return hip_fail(e, "autotune timing");
         *ms /= (float)reps;
         return RB_OK;
     };
@@ -550,8 +551,8 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
     if (best) *best = cur;
     if (best_ms) *best_ms = cur_ms;
 done:
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return st;
 }
 
