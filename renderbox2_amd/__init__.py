@@ -10,4 +10,4 @@ from .core import (BVH, FLT_MAX, HIT_DTYPE, RAY_DTYPE, BuildParams, CudaBvh, Cud
                    device_init,
                    SamplerRenderer, get_trace_config, raygen_diffuse_device, render_params, set_trace_config,
                    trace_batch, write_image)
-from . import scenes  # noqa: F401
+from . import config, scenes  # noqa: F401
