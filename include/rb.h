@@ -88,6 +88,15 @@ rb_status rb_memset(void* dst, int value, size_t bytes, void* stream);
  * HBM traffic counters. bytes must be a multiple of 16 and pointers 16-aligned. */
 rb_status rb_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 rb_status rb_mem_info(size_t* free_bytes, size_t* total_bytes, size_t* library_bytes);
+/* Self-test, no counterpart in the reference: runs every one of the 2^32
+ * single-precision bit patterns through the traversal's reciprocal (the
+ * hardware reciprocal refined by one fused Newton step, taken when the
+ * refined value is normal, else the division) on the current device.
+ * *fast_inputs = inputs that take the refined value (4,227,858,434 on
+ * gfx950), *mismatches = inputs whose reciprocal differs from the IEEE
+ * division 1.0f / x (must be 0: the traversal's bit-exact results rest on
+ * it).  Synchronous; tens of milliseconds of device time. */
+rb_status rb_selftest_rcp(unsigned long long* fast_inputs, unsigned long long* mismatches);
 rb_status rb_stream_create(void** stream);
 rb_status rb_stream_destroy(void* stream);
 rb_status rb_stream_synchronize(void* stream);

@@ -124,6 +124,7 @@ _SIGS = {
     "rb_trace_counts": (C.c_int, [P, P, U32, U32, P, P, P]),
     "rb_trace_autotune": (C.c_int, [P, P, U32, U32, P, U32, P, P, P]),
     "rb_copy_kernel": (C.c_int, [P, P, SZ, P]),
+    "rb_selftest_rcp": (C.c_int, [P, P]),
     "rb_camera_matrices": (C.c_int, [P, P, P]),
     "rb_raygen_camera": (C.c_int, [P, U32, U32, U32, U32, U32, U64, C.c_int, P, P]),
     "rb_raygen_diffuse": (C.c_int, [P, P, P, U32, U32, U64, P, P]),

@@ -460,6 +460,14 @@ def copy_kernel(d_dst, d_src, nbytes, stream=None):
     check(lib().rb_copy_kernel(_dp(d_dst), _dp(d_src), nbytes, _sp(stream)))
 
 
+def selftest_rcp():
+    """(inputs taking the refined reciprocal, inputs differing from the IEEE
+    division) over all 2^32 single-precision inputs (rb_selftest_rcp)."""
+    fast, bad = C.c_ulonglong(0), C.c_ulonglong(0)
+    check(lib().rb_selftest_rcp(C.byref(fast), C.byref(bad)))
+    return fast.value, bad.value
+
+
 class PerspectiveCamera:
     """Pinhole camera (PerspectiveCamera, src/core/camera.h:48-71)."""
 

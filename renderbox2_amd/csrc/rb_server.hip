@@ -268,7 +268,7 @@ void rb_server_s::launch(Slot& sl, Req* q) {
 }
 
 void rb_server_s::loop() {
-    hipSetDevice(dev().device);
+    (void)hipSetDevice(dev().device);  // the device rb_server_create validated
     std::vector<std::pair<Slot*, Req*>> to_launch;
     while (true) {
         // completions

@@ -77,6 +77,15 @@ def test_trace_small_bit_exact(gpu, orc, small, cfg):
         gpu.set_trace_config(**CONFIGS[0])
 
 
+def test_reciprocal_equals_the_division_for_every_input(gpu):
+    # the traversal's 1/x (hardware reciprocal + one fused Newton step when
+    # that is normal, else the division) must be the IEEE quotient for all
+    # 2^32 inputs: the bit-exact (id, t) of every other test rests on it
+    fast, bad = gpu.selftest_rcp()
+    assert bad == 0
+    assert fast == 4_227_858_434  # every normal |x| < 2^126, and x = +-2^126
+
+
 def test_visit_counts_equal_the_oracle(gpu, orc):
     """In the default leaf order each ray walks the tree in the oracle's
     order, so the kernel's own per-ray counts of inner nodes, triangles and
