@@ -197,6 +197,15 @@ rb_status load_bvh_binary(const char* path, const float* verts, uint32_t nv, con
         return fail(RB_ERR_INVALID, "%s was built from other geometry or build parameters", path);
     if (hd.n_nodes == 0 || hd.n_nodes > (1u << 31) || hd.n_refs > (1u << 31))
         return fail(RB_ERR_INVALID, "%s: bad sizes", path);
+    // the header's sizes must account for the file exactly before anything
+    // is allocated from them
+    const uint64_t want = sizeof(FileHeader) + (uint64_t)hd.n_nodes * sizeof(FileNode) + (uint64_t)hd.n_refs * 4 + 8;
+    in.seekg(0, std::ios::end);
+    const std::streamoff have = in.tellg();
+    if (have < 0 || (uint64_t)have != want)
+        return fail(RB_ERR_INVALID, "%s: %lld bytes, the header describes %llu", path, (long long)have,
+                    (unsigned long long)want);
+    in.seekg((std::streamoff)sizeof(FileHeader), std::ios::beg);
     std::vector<FileNode> nodes(hd.n_nodes);
     std::vector<uint32_t> refs(hd.n_refs);
     uint64_t check = 0;

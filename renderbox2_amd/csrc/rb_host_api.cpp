@@ -474,7 +474,11 @@ rb_status rb_bvh_export(rb_host_bvh_t b, uint32_t layout, void* nodes, size_t* n
 
 rb_status rb_bvh_save_binary(rb_host_bvh_t b, const char* path) {
     RB_CHECK_ARG(b && path, "NULL argument");
-    return save_bvh_binary(*b->b, path);
+    try {
+        return save_bvh_binary(*b->b, path);
+    } catch (...) {
+        return fail(RB_ERR_OOM, "BVH save failed (host allocation)");
+    }
 }
 
 rb_status rb_bvh_load_binary(const char* path, rb_scene_t s, const rb_build_params* p, rb_host_bvh_t* out) {
@@ -515,7 +519,7 @@ rb_status rb_bvh_build_cached(rb_scene_t s, const rb_build_params* p, const char
     }
     rb_status st = rb_bvh_build_host(s, p, out);
     if (st != RB_OK) return st;
-    if ((st = save_bvh_binary(*(*out)->b, path.c_str())) != RB_OK) {
+    if ((st = rb_bvh_save_binary(*out, path.c_str())) != RB_OK) {
         rb_bvh_host_release(*out);
         *out = nullptr;
         return st;

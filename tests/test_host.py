@@ -331,6 +331,14 @@ def test_binary_bvh_cache_round_trip_and_key_checks(rbm, tmp_path):
     bad.write_bytes(path.read_bytes()[:-100])
     with pytest.raises(rbm.RbError):
         rbm.BVH.load_binary(bad, s)
+    # a header whose node count does not match the file (here 2^31 nodes,
+    # 64 GiB if allocated) is refused before anything is allocated; the
+    # count sits after magic, version, reserved, key, nv and nt
+    raw = bytearray(path.read_bytes())
+    raw[32:36] = (1 << 31).to_bytes(4, "little")
+    bad.write_bytes(bytes(raw))
+    with pytest.raises(rbm.RbError, match="header describes"):
+        rbm.BVH.load_binary(bad, s)
 
 
 def test_build_cached_builds_once_then_loads(rbm, tmp_path):
