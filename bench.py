@@ -52,17 +52,22 @@ def time_launches(rb, tracer, d_rays, n, d_hits, any_hit, steps, warmup, stream)
     return e0.elapsed_ms(e1) / steps
 
 
-def read_traffic(path):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+def read_profile_value(path, key):
+    """A `key = value` figure from a committed rocprofv3 --pmc summary, if present."""
     try:
         d = {}
         for line in open(path):
             if "=" in line and not line.startswith("#"):
                 k, v = line.split("=", 1)
                 d[k.strip()] = v.strip()
-        return float(d["traffic_bytes_per_launch"])
+        return float(d[key])
     except (OSError, KeyError, ValueError):
         return None
+
+
+def read_traffic(path):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, if present."""
+    return read_profile_value(path, "traffic_bytes_per_launch")
 
 
 def bench_render(rb, kind, scene, bvh_max_leaf, cam, frames, built=None, **params):
@@ -546,6 +551,7 @@ def main():
 
     achieved = bytes_per_ray * n / (ms * 1e-3) / 1e9
     traffic = read_traffic(os.path.join(ROOT, "profiles", "trace_incoherent_1m_pmc.txt"))
+    l2_hit = read_profile_value(os.path.join(ROOT, "profiles", "trace_incoherent_1m_l2.txt"), "l2_hit_rate")
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -567,7 +573,8 @@ def main():
                    "bvh_cache_mb": round(cache_mb, 1), "trace_config": cfg},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_ray": round(bytes_per_ray, 1), "gpu_visit_bytes_per_ray": round(gpu_bpr, 1)},
+                     "bytes_per_ray": round(bytes_per_ray, 1), "gpu_visit_bytes_per_ray": round(gpu_bpr, 1),
+                     "l2_hit_rate": l2_hit},
         "cpu_baseline": cpu,
     }
     out.update(extra)
