@@ -87,15 +87,19 @@ def test_render_rejects_bad_arguments(gpu, ao_setup):
 BD = dict(max_depth=8, rr_start=3, rr_prob=0.5, light_paths=4096, light_max_depth=4, connections=3)
 
 
-def test_bdpt_image_bit_exact(gpu, orc, pt_setup):
+@pytest.mark.parametrize("conn", [3, 8])
+def test_bdpt_image_bit_exact(gpu, orc, pt_setup, conn):
+    # 3 connections: an odd count (the last one uses half of a random draw);
+    # 8: the most one call takes, one thread per (path, connection)
     s, b, tr = pt_setup
+    bd = dict(BD, connections=conn)
     cam = gpu.scenes.pt_camera(48, 48)
-    r = gpu.SamplerRenderer(tr, cam, "bdpt", spp=2, seed=5, **BD)
+    r = gpu.SamplerRenderer(tr, cam, "bdpt", spp=2, seed=5, **bd)
     img = r.render()
     v, idx, tm, lt = s.geometry()
     ref = orc.render_bdpt(_c2(b), v, idx, tm, s.materials(), lt, _cam_tuple(cam), (0, 0, 48, 48), 2, 5,
-                          BD["max_depth"], BD["rr_start"], BD["rr_prob"], BD["light_paths"], BD["light_max_depth"],
-                          BD["connections"])
+                          bd["max_depth"], bd["rr_start"], bd["rr_prob"], bd["light_paths"], bd["light_max_depth"],
+                          bd["connections"])
     assert np.isfinite(img).all() and img.max() > 0
     bad = img.view(np.uint32) != ref.view(np.uint32)
     assert not bad.any(), f"{bad.sum()} channels differ, max abs {np.abs(img - ref).max()}"
