@@ -379,7 +379,7 @@ def main():
     ap.add_argument("--extras", type=int, default=1, help="also time coherent / any-hit / mixed sets")
     ap.add_argument("--autotune", type=int, default=1, help="pick the traversal config on this batch first")
     ap.add_argument("--config", default="",
-                    help="block,lds_stack,chunk,fetch_threshold,waves_per_cu[,speculative[,leaf_gate]]")
+                    help="block,lds_stack,chunk,fetch_threshold,waves_per_cu[,speculative[,leaf_gate[,lds_nodes]]]")
     ap.add_argument("--speculative", type=int, default=0,
                     help="1: the reference's speculative leaf order (wave-dependent results in rare cases)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -444,7 +444,8 @@ def main():
     tuned_ms = None
     rb.set_trace_config(speculative=args.speculative)
     if args.config:
-        keys = ["block", "lds_stack", "chunk", "fetch_threshold", "waves_per_cu", "speculative", "leaf_gate"]
+        keys = ["block", "lds_stack", "chunk", "fetch_threshold", "waves_per_cu", "speculative", "leaf_gate",
+                "lds_nodes"]
         rb.set_trace_config(**dict(zip(keys, (int(x) for x in args.config.split(",")))))
     elif args.autotune:
         _, tuned_ms = rb.autotune(tracer, d_diff, n, d_hits, reps=3, stream=stream)

@@ -314,6 +314,11 @@ typedef struct rb_trace_config {
                                 once at most this many lanes still search for a leaf
                                 (0..63; 0 = when all have one).  Any value gives the
                                 same results: only the interleaving of lanes changes */
+    uint32_t lds_nodes;      /* top-of-tree node records each block copies into LDS at
+                                start and reads from there (0..512, multiple of 64; 0 =
+                                none).  The MI355X layout stores the first 512 records
+                                breadth-first, so these are the top levels of the tree.
+                                Any value gives the same results */
 } rb_trace_config;
 rb_status rb_trace_get_config(rb_trace_config* c);
 rb_status rb_trace_set_config(const rb_trace_config* c);

@@ -55,7 +55,7 @@ class rb_bvh_stats(C.Structure):
 
 class rb_trace_config(C.Structure):
     _fields_ = [("block", U32), ("lds_stack", U32), ("chunk", U32), ("fetch_threshold", U32),
-                ("waves_per_cu", U32), ("speculative", U32), ("leaf_gate", U32)]
+                ("waves_per_cu", U32), ("speculative", U32), ("leaf_gate", U32), ("lds_nodes", U32)]
 
 
 class rb_camera(C.Structure):

@@ -103,7 +103,10 @@ rb_status load_bvh_binary(const char* path, const float* verts, uint32_t nv, con
                           const rb_build_params& p, HostBvh** out);
 void woopify(const float* v0, const float* v1, const float* v2, float out[12]);
 
-// MI355X layout arrays (host copies).
+// MI355X layout arrays (host copies).  The first kLdsNodesMax inner
+// records are the top of the tree in breadth-first order (the records a
+// trace block may keep in LDS); the rest follow in depth-first order.
+constexpr uint32_t kLdsNodesMax = 512;
 struct Mi355Layout {
     std::vector<uint32_t> nodes;  // 16 words per node
     std::vector<float> tris;      // 12 floats per triangle reference

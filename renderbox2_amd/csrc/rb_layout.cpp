@@ -160,6 +160,43 @@ static const BNode* split_big_leaf(const BNode* leaf, std::vector<std::unique_pt
     return in;
 }
 
+// Moves the first `k` inner records in breadth-first order (root first) to
+// the front of the node array and renumbers every inner link; the other
+// records keep their relative order, so subtrees below the front stay
+// contiguous.  The traversal reads the front from LDS (lds_nodes).
+static void breadth_first_front(std::vector<uint32_t>& nodes, uint32_t k) {
+    const uint32_t n = (uint32_t)(nodes.size() / 16);
+    if (k > n) k = n;
+    std::vector<uint32_t> order;  // old record ids, breadth-first
+    order.reserve(k);
+    std::vector<uint8_t> front(n, 0);
+    order.push_back(0);
+    front[0] = 1;
+    for (size_t h = 0; h < order.size() && order.size() < k; h++) {
+        const uint32_t* d = &nodes[(size_t)order[h] * 16];
+        for (int c = 0; c < 2 && order.size() < k; c++) {
+            const int32_t link = (int32_t)d[12 + c];
+            if (link >= 0 && !front[link]) {
+                front[link] = 1;
+                order.push_back((uint32_t)link);
+            }
+        }
+    }
+    std::vector<uint32_t> newid(n);
+    uint32_t next = 0;
+    for (uint32_t r : order) newid[r] = next++;
+    for (uint32_t r = 0; r < n; r++)
+        if (!front[r]) newid[r] = next++;
+    std::vector<uint32_t> out(nodes.size());
+    for (uint32_t r = 0; r < n; r++) {
+        uint32_t* d = &out[(size_t)newid[r] * 16];
+        std::memcpy(d, &nodes[(size_t)r * 16], 64);
+        for (int c = 0; c < 2; c++)
+            if ((int32_t)d[12 + c] >= 0) d[12 + c] = newid[d[12 + c]];
+    }
+    nodes.swap(out);
+}
+
 rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
     RootView rv(H.root);
     std::vector<std::unique_ptr<BNode>> pool;
@@ -212,6 +249,7 @@ rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
         d[14] = cnt[0];
         d[15] = cnt[1];
     }
+    breadth_first_front(L.nodes, kLdsNodesMax);
     return RB_OK;
 }
 

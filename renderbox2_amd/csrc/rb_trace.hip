@@ -48,6 +48,7 @@ struct TraceArgs {
     uint32_t chunk;
     uint32_t threshold;
     uint32_t leaf_gate;  // non-speculative: leave the inner-node phase once <= this many lanes still search
+    uint32_t top_nodes;  // node records 0..top_nodes-1 are read from a copy in LDS
 };
 
 constexpr uint32_t kMaxBatch = 64;
@@ -192,6 +193,14 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     } while (0)
 
     lstack[tid] = kSentinel;
+    // top of the tree (the layout's first records, breadth-first) in LDS
+    extern __shared__ V4f lnodes[];
+    const uint32_t top = MULTI ? 0u : a.top_nodes;
+    if (top) {
+        const GF4 src = (GF4)a.nodes;
+        for (uint32_t i = tid; i < 4 * top; i += BLOCK) lnodes[i] = src[i];
+        __syncthreads();
+    }
     uint32_t pool_next = 0, pool_end = 0;
     int nodeAddr = kSentinel, leafAddr = 0, sp = 0, hitIndex = -1;
     uint32_t rayidx = 0;
@@ -273,17 +282,28 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
         // ---- traversal ----
         while (nodeAddr != kSentinel) {
             while ((uint32_t)nodeAddr < (uint32_t)kSentinel && (SPEC || leafAddr >= 0)) {
-                const GF4 np = nodes + 4 * (size_t)nodeAddr;
-                const V4f n0 = np[0];
-                const V4f n1 = np[1];
-                const V4f nz = np[2];
-                const V4i lk = *(GI4)(np + 3);
-                // All four loads of the record in flight at once: without this
-                // the compiler sinks the link load below the slab tests (it is
-                // only read when a child is hit), a second memory round trip
-                // per visited node.  The empty asm consumes every loaded value.
-                asm volatile("" ::"v"(n0.x), "v"(n0.y), "v"(n0.z), "v"(n0.w), "v"(n1.x), "v"(n1.y), "v"(n1.z),
-                             "v"(n1.w), "v"(nz.x), "v"(nz.y), "v"(nz.z), "v"(nz.w), "v"(lk.x), "v"(lk.y));
+                V4f n0, n1, nz;
+                V4i lk;
+                if ((uint32_t)nodeAddr < top) {
+                    const V4f* ln = lnodes + 4 * nodeAddr;
+                    n0 = ln[0];
+                    n1 = ln[1];
+                    nz = ln[2];
+                    lk = __builtin_bit_cast(V4i, ln[3]);
+                } else {
+                    const GF4 np = nodes + 4 * (size_t)nodeAddr;
+                    n0 = np[0];
+                    n1 = np[1];
+                    nz = np[2];
+                    lk = *(GI4)(np + 3);
+                    // All four loads of the record in flight at once: without
+                    // this the compiler sinks the link load below the slab tests
+                    // (it is only read when a child is hit), a second memory
+                    // round trip per visited node.  The empty asm consumes every
+                    // loaded value.
+                    asm volatile("" ::"v"(n0.x), "v"(n0.y), "v"(n0.z), "v"(n0.w), "v"(n1.x), "v"(n1.y), "v"(n1.z),
+                                 "v"(n1.w), "v"(nz.x), "v"(nz.y), "v"(nz.z), "v"(nz.w), "v"(lk.x), "v"(lk.y));
+                }
                 if (COUNT) c_inner++;
                 const float c0lox = fmaf(n0.x, idx_, -oodx), c0hix = fmaf(n0.y, idx_, -oodx);
                 const float c0loy = fmaf(n0.z, idy_, -oody), c0hiy = fmaf(n0.w, idy_, -oody);
@@ -470,7 +490,8 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.chunk = cfg.chunk;
     a.threshold = cfg.fetch_threshold;
     a.leaf_gate = cfg.leaf_gate;
-    hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), 0, stream, a);
+    a.top_nodes = cfg.lds_nodes < bvh->n_nodes ? cfg.lds_nodes : bvh->n_nodes;
+    hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), (size_t)a.top_nodes * 64, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "trace launch");
     return RB_OK;
@@ -523,6 +544,7 @@ static rb_status check_cfg(const rb_trace_config& c) {
     if (c.waves_per_cu < 1 || c.waves_per_cu > 64) return fail(RB_ERR_INVALID, "waves_per_cu must be 1..64");
     if (c.speculative > 1) return fail(RB_ERR_INVALID, "speculative must be 0 or 1");
     if (c.leaf_gate > 63) return fail(RB_ERR_INVALID, "leaf_gate must be 0..63");
+    if (c.lds_nodes > 512 || c.lds_nodes % 64) return fail(RB_ERR_INVALID, "lds_nodes must be 0..512, step 64");
     return RB_OK;
 }
 
@@ -599,8 +621,9 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
         const uint32_t thresholds[] = {0, 8, 16, 24, 32, 40, 48};
         const uint32_t waves[] = {8, 12, 16, 24, 32};
         const uint32_t gates[] = {0, 4, 8, 16, 24, 32};
+        const uint32_t tops[] = {0, 64, 128, 192, 256};
         for (int pass = 0; pass < 2; pass++) {
-            for (int dim = 0; dim < 6; dim++) {
+            for (int dim = 0; dim < 7; dim++) {
                 const uint32_t* vals;
                 int nv;
                 switch (dim) {
@@ -609,13 +632,15 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
                     case 2: vals = chunks; nv = 4; break;
                     case 3: vals = thresholds; nv = 7; break;
                     case 4: vals = waves; nv = 5; break;
-                    default: vals = gates; nv = 6; break;
+                    case 5: vals = gates; nv = 6; break;
+                    default: vals = tops; nv = 5; break;
                 }
                 if (dim == 5 && cur.speculative) continue;  // the gate only acts on the non-speculative order
                 for (int k = 0; k < nv; k++) {
                     rb_trace_config c = cur;
                     uint32_t* f = dim == 0 ? &c.block : dim == 1 ? &c.lds_stack : dim == 2 ? &c.chunk
-                                : dim == 3 ? &c.fetch_threshold : dim == 4 ? &c.waves_per_cu : &c.leaf_gate;
+                                : dim == 3 ? &c.fetch_threshold : dim == 4 ? &c.waves_per_cu
+                                : dim == 5 ? &c.leaf_gate : &c.lds_nodes;
                     if (*f == vals[k]) continue;
                     *f = vals[k];
                     float ms = 0.f;

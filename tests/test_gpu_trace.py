@@ -16,11 +16,16 @@ from conftest import random_rays, random_triangles
 
 pytestmark = pytest.mark.gpu
 
+# lds_nodes: how many top-of-tree node records are read from LDS (0 = none)
 CONFIGS = [
-    dict(block=128, lds_stack=16, chunk=64, fetch_threshold=24, waves_per_cu=16, speculative=0, leaf_gate=0),
-    dict(block=64, lds_stack=8, chunk=256, fetch_threshold=0, waves_per_cu=8, speculative=0, leaf_gate=40),
-    dict(block=256, lds_stack=24, chunk=64, fetch_threshold=48, waves_per_cu=32, speculative=0, leaf_gate=16),
-    dict(block=256, lds_stack=16, chunk=64, fetch_threshold=40, waves_per_cu=32, speculative=1, leaf_gate=0),
+    dict(block=128, lds_stack=16, chunk=64, fetch_threshold=24, waves_per_cu=16, speculative=0, leaf_gate=0,
+         lds_nodes=0),
+    dict(block=64, lds_stack=8, chunk=256, fetch_threshold=0, waves_per_cu=8, speculative=0, leaf_gate=40,
+         lds_nodes=128),
+    dict(block=256, lds_stack=24, chunk=64, fetch_threshold=48, waves_per_cu=32, speculative=0, leaf_gate=16,
+         lds_nodes=512),
+    dict(block=256, lds_stack=16, chunk=64, fetch_threshold=40, waves_per_cu=32, speculative=1, leaf_gate=0,
+         lds_nodes=256),
 ]
 
 
@@ -59,6 +64,7 @@ def _check_any(orc, geom, rays, ids, ts, aid, at):
 @pytest.mark.parametrize("cfg", range(len(CONFIGS)))
 def test_trace_small_bit_exact(gpu, orc, small, cfg):
     v, idx, b, tr, c2 = small
+    saved = gpu.get_trace_config()
     gpu.set_trace_config(**CONFIGS[cfg])
     try:
         rays = random_rays(50000, seed=60 + cfg)
@@ -74,7 +80,7 @@ def test_trace_small_bit_exact(gpu, orc, small, cfg):
             assert np.array_equal(aid, oaid)
             assert np.array_equal(at.view(np.uint32), oat.view(np.uint32))
     finally:
-        gpu.set_trace_config(**CONFIGS[0])
+        gpu.set_trace_config(**saved)
 
 
 def test_reciprocal_equals_the_division_for_every_input(gpu):
@@ -100,6 +106,7 @@ def test_visit_counts_equal_the_oracle(gpu, orc):
     rays = random_rays(30000, seed=71)
     oid, ot, ocnt = orc.trace_compact2(*_compact2(gpu, b), rays, counts=True)
     assert (oid >= 0).sum() > 5000
+    saved = gpu.get_trace_config()
     try:
         for cfg in CONFIGS[:3]:
             gpu.set_trace_config(**cfg)
@@ -107,7 +114,7 @@ def test_visit_counts_equal_the_oracle(gpu, orc):
             assert np.array_equal(ids, oid) and np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
             assert np.array_equal(cnt, ocnt), cfg
     finally:
-        gpu.set_trace_config(**CONFIGS[0])
+        gpu.set_trace_config(**saved)
 
 
 def test_trace_room_camera_and_diffuse_rays(gpu, orc, room):
@@ -251,6 +258,7 @@ def test_million_triangles_properties(gpu, orc):
     oaid, _ = orc.trace_compact2(*c2, diff[sel], any_hit=True)
     assert np.array_equal(aid[sel], oaid)
     # every launch shape gives the same answers: each ray's visit order is its own
+    saved = gpu.get_trace_config()
     try:
         for cfg in CONFIGS[1:3]:
             gpu.set_trace_config(**cfg)
@@ -259,7 +267,7 @@ def test_million_triangles_properties(gpu, orc):
             assert np.array_equal(i3, ids) and np.array_equal(t3.view(np.uint32), ts.view(np.uint32))
             assert np.array_equal(a3, aid)
     finally:
-        gpu.set_trace_config(**CONFIGS[0])
+        gpu.set_trace_config(**saved)
 
 
 def test_batched_trace_equals_per_batch_calls(gpu):
