@@ -299,8 +299,10 @@ rb_status rb_trace_batch(const rb_batch_desc* batches, uint32_t nb, void* stream
  * hard-codes STACK_SIZE 64 / DYNAMIC_FETCH_THRESHOLD 20 / 720 warps of
  * (32,4), kepler_dynamic_fetch.h:62-63 and cudatracer.cu:170-171). */
 typedef struct rb_trace_config {
-    uint32_t block;          /* threads per block: 64, 128, 256 */
-    uint32_t lds_stack;      /* stack entries per lane kept in LDS: 8..24 */
+    uint32_t block;          /* threads per block: 64, 128, 256, 512, 1024 */
+    uint32_t lds_stack;      /* stack entries per lane kept in LDS: 8..24 (8..16 at
+                                block 512, 8..12 at 1024); stack plus cached nodes
+                                must fit the device's LDS per block */
     uint32_t chunk;          /* rays claimed per work-counter atomic (multiple of 64) */
     uint32_t fetch_threshold;/* refill when fewer active lanes than this (0..64) */
     uint32_t waves_per_cu;   /* persistent grid size = CUs x this / (block/64) */

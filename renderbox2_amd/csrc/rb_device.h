@@ -20,6 +20,7 @@ constexpr int kSentinel = 0x76543210;
 struct DeviceState {
     int device = -1;
     uint32_t num_cus = 0;
+    size_t lds_per_block = 65536;  // LDS one workgroup may use (static + dynamic)
     hipStream_t stream = nullptr;  // library default stream
     uint32_t* counters = nullptr;  // 2 * kCounterSlots
     std::atomic<uint32_t> next_slot{0};

@@ -26,6 +26,7 @@ static rb_status init_locked(int device) {
     hipDeviceProp_t prop;
     if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) return hip_fail(e, "hipGetDeviceProperties");
     g_dev.num_cus = (uint32_t)prop.multiProcessorCount;
+    if (prop.sharedMemPerBlock) g_dev.lds_per_block = prop.sharedMemPerBlock;
     if ((e = hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking)) != hipSuccess)
         return hip_fail(e, "hipStreamCreate");
     void* c = nullptr;
@@ -41,16 +42,16 @@ static rb_status init_locked(int device) {
         (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
     }
     // the best measured shape on 2 Mi incoherent rays at 1M triangles: the
-    // 12-entry LDS stack and 192 cached top-of-tree records take 24 KB per
-    // block, which keeps 6 blocks (24 waves) on a CU
-    g_dev.cfg.block = 256;
+    // 12-entry LDS stack and 384 cached top-of-tree records take 48 KB per
+    // 512-thread block, which keeps 3 blocks (24 waves) on a CU
+    g_dev.cfg.block = 512;
     g_dev.cfg.lds_stack = 12;
     g_dev.cfg.chunk = 64;
     g_dev.cfg.fetch_threshold = 48;
     g_dev.cfg.waves_per_cu = 24;
     g_dev.cfg.speculative = 0;
     g_dev.cfg.leaf_gate = 16;
-    g_dev.cfg.lds_nodes = 192;
+    g_dev.cfg.lds_nodes = 384;
     g_dev.device = device;
     return RB_OK;
 }

@@ -432,10 +432,10 @@ __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a, BatchTable t
 typedef void (*TraceKernel)(TraceArgs);
 typedef void (*BatchKernel)(TraceArgs, BatchTable);
 
-#define RB_FOR_CONFIGS(X)                                          \
-    X(8, 64) X(8, 128) X(8, 256) X(12, 64) X(12, 128) X(12, 256)   \
-    X(16, 64) X(16, 128) X(16, 256) X(20, 64) X(20, 128) X(20, 256) \
-    X(24, 64) X(24, 128) X(24, 256)
+#define RB_FOR_CONFIGS(X)                                                        \
+    X(8, 64) X(8, 128) X(8, 256) X(8, 512) X(8, 1024) X(12, 64) X(12, 128)       \
+    X(12, 256) X(12, 512) X(12, 1024) X(16, 64) X(16, 128) X(16, 256) X(16, 512) \
+    X(20, 64) X(20, 128) X(20, 256) X(24, 64) X(24, 128) X(24, 256)
 
 template <bool ANY, bool COUNT>
 static TraceKernel pick(const rb_trace_config& c) {
@@ -491,6 +491,10 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.threshold = cfg.fetch_threshold;
     a.leaf_gate = cfg.leaf_gate;
     a.top_nodes = cfg.lds_nodes < bvh->n_nodes ? cfg.lds_nodes : bvh->n_nodes;
+    const size_t lds = (size_t)cfg.lds_stack * cfg.block * 4 + (size_t)a.top_nodes * 64;
+    if (lds > dev().lds_per_block)
+        return fail(RB_ERR_INVALID, "trace config needs %zu bytes of LDS per block (stack %u x %u lanes + %u node records); "
+                    "the device allows %zu", lds, cfg.lds_stack, cfg.block, a.top_nodes, dev().lds_per_block);
     hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), (size_t)a.top_nodes * 64, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "trace launch");
@@ -537,7 +541,11 @@ rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t st
 }
 
 static rb_status check_cfg(const rb_trace_config& c) {
-    if (c.block != 64 && c.block != 128 && c.block != 256) return fail(RB_ERR_INVALID, "block must be 64/128/256");
+    if (c.block != 64 && c.block != 128 && c.block != 256 && c.block != 512 && c.block != 1024)
+        return fail(RB_ERR_INVALID, "block must be 64/128/256/512/1024");
+    if (!pick<false, false>(c))
+        return fail(RB_ERR_INVALID, "no kernel for lds_stack %u at block %u (512: stack 8-16; 1024: stack 8-12)",
+                    c.lds_stack, c.block);
     if (c.lds_stack < 8 || c.lds_stack > 24 || c.lds_stack % 4) return fail(RB_ERR_INVALID, "lds_stack must be 8..24, step 4");
     if (c.chunk < 64 || c.chunk % 64 || c.chunk > 65536) return fail(RB_ERR_INVALID, "chunk must be a multiple of 64 up to 65536");
     if (c.fetch_threshold > 64) return fail(RB_ERR_INVALID, "fetch_threshold must be <= 64");
@@ -615,25 +623,25 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
     float cur_ms = 0.f;
     if ((st = measure(cur, &cur_ms)) != RB_OK) goto done;
     {
-        const uint32_t blocks[] = {64, 128, 256};
+        const uint32_t blocks[] = {64, 128, 256, 512, 1024};
         const uint32_t stacks[] = {8, 12, 16, 20, 24};
         const uint32_t chunks[] = {64, 128, 256, 1024};
         const uint32_t thresholds[] = {0, 8, 16, 24, 32, 40, 48};
         const uint32_t waves[] = {8, 12, 16, 24, 32};
         const uint32_t gates[] = {0, 4, 8, 16, 24, 32};
-        const uint32_t tops[] = {0, 64, 128, 192, 256};
+        const uint32_t tops[] = {0, 64, 128, 192, 256, 384};
         for (int pass = 0; pass < 2; pass++) {
             for (int dim = 0; dim < 7; dim++) {
                 const uint32_t* vals;
                 int nv;
                 switch (dim) {
-                    case 0: vals = blocks; nv = 3; break;
+                    case 0: vals = blocks; nv = 5; break;
                     case 1: vals = stacks; nv = 5; break;
                     case 2: vals = chunks; nv = 4; break;
                     case 3: vals = thresholds; nv = 7; break;
                     case 4: vals = waves; nv = 5; break;
                     case 5: vals = gates; nv = 6; break;
-                    default: vals = tops; nv = 5; break;
+                    default: vals = tops; nv = 6; break;
                 }
                 if (dim == 5 && cur.speculative) continue;  // the gate only acts on the non-speculative order
                 for (int k = 0; k < nv; k++) {
@@ -643,6 +651,9 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
                                 : dim == 5 ? &c.leaf_gate : &c.lds_nodes;
                     if (*f == vals[k]) continue;
                     *f = vals[k];
+                    if (!pick<false, false>(c) ||
+                        (size_t)c.lds_stack * c.block * 4 + (size_t)c.lds_nodes * 64 > dev().lds_per_block)
+                        continue;  // no kernel for this shape, or more LDS than a block gets
                     float ms = 0.f;
                     if ((st = measure(c, &ms)) != RB_OK) goto done;
                     if (ms < cur_ms * 0.99f) {

@@ -26,6 +26,8 @@ CONFIGS = [
          lds_nodes=512),
     dict(block=256, lds_stack=16, chunk=64, fetch_threshold=40, waves_per_cu=32, speculative=1, leaf_gate=0,
          lds_nodes=256),
+    dict(block=1024, lds_stack=8, chunk=128, fetch_threshold=48, waves_per_cu=32, speculative=0, leaf_gate=16,
+         lds_nodes=512),
 ]
 
 
