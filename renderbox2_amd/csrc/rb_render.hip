@@ -186,6 +186,7 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
         float4 ra0, ra1, rb1, rc1;
         V3 p = v3(0.f, 0.f, 0.f);
         float eps = 0.0f;
+        int4 slot = make_int4(-1, -1, -1, -1);  // written once, after the queue reservations
         if (valid) {
             const float4 o = P.ray[2 * (size_t)pid], d = P.ray[2 * (size_t)pid + 1];
             const int2 h = P.hit[pid];
@@ -205,7 +206,6 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
             const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
             const float4 Tp = P.T[pid];
             const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
-            int4 slot = make_int4(-1, -1, -1, -1);
             uint32_t cm = BD ? P.cmask[pid] : 0u;
             if (m.kind == RB_MAT_EMITTER) {
                 // emission seen directly or through mirrors (integrator_path.cu:94-109), then stop
@@ -217,7 +217,6 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
                     P.L[pid] = make_float4(L.x + Tv.x * le.x, L.y + Tv.y * le.y, L.z + Tv.z * le.z, 0.f);
                 }
                 P.info[pid] = ninfo;
-                P.slot[pid] = slot;
             } else {
                 const uint32_t gid = sample_gid(T, pid);
                 const U4 r0 = rng.draw(gid, bounce, 0, 0);
@@ -319,7 +318,6 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
                 P.ca[pid] = make_float4(ca.x, ca.y, ca.z, wdir);
                 P.cb[pid] = make_float4(cb.x, cb.y, cb.z, 0.f);
                 P.info[pid] = ninfo;
-                P.slot[pid] = slot;
             }
         }
         // workgroup-aggregated appends (every thread of the workgroup reaches
@@ -340,12 +338,11 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
             P.rbc[2 * (size_t)sc] = make_float4(p.x, p.y, p.z, eps);
             P.rbc[2 * (size_t)sc + 1] = rc1;
         }
-        if (valid && (wantA || wantB || wantC)) {
-            int4 s = P.slot[pid];
-            s.x = wantA ? (int)sa : -1;
-            s.y = wantB ? (int)sb : -1;
-            s.z = wantC ? (int)sc : -1;
-            P.slot[pid] = s;
+        if (valid) {
+            slot.x = wantA ? (int)sa : -1;
+            slot.y = wantB ? (int)sb : -1;
+            slot.z = wantC ? (int)sc : -1;
+            P.slot[pid] = slot;
         }
     }
 }
