@@ -862,11 +862,9 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
         } else
             hipLaunchKernelGGL(k_pt_shade<false>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
                                p->rr_start, p->rr_prob, B);
-        if ((st = launch_trace(bvh, (rb_ray*)P.ra, na, RB_ANY, nullptr, (rb_hit*)P.ha, nullptr, s, dev().cfg,
-                               P.cnt + 2)) != RB_OK)
-            return st;
-        if ((st = launch_trace(bvh, (rb_ray*)P.rbc, 2 * n, RB_CLOSEST, nullptr, (rb_hit*)P.hbc, nullptr, s, dev().cfg,
-                               P.cnt + 3)) != RB_OK)
+        // shadow rays (any hit) and MIS + bounce rays (closest hit) in one launch
+        if ((st = launch_trace_pair(bvh, (rb_ray*)P.ra, na, P.cnt + 2, (rb_hit*)P.ha, (rb_ray*)P.rbc, 2 * n, P.cnt + 3,
+                                    (rb_hit*)P.hbc, s, dev().cfg)) != RB_OK)
             return st;
         if (bd)
             hipLaunchKernelGGL(k_pt_connect<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, (float)bvh->n_lights,

@@ -45,6 +45,10 @@ struct TraceArgs {
     uint32_t* counter;
     uint32_t n;
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
+    // paired launch: a second ray set (closest hit) after the first (any hit)
+    const float4* __restrict__ rays2;
+    int2* __restrict__ hits2;
+    const uint32_t* n_dev2;
     uint32_t chunk;
     uint32_t threshold;
     uint32_t leaf_gate;  // non-speculative: leave the inner-node phase once <= this many lanes still search
@@ -160,8 +164,11 @@ __global__ __launch_bounds__(256) void k_rcp_check(unsigned long long* out) {
     }
 }
 
-template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC>
-__device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays) {
+// PAIR: rays [0, n0) are the any-hit set (a.rays / a.hits), rays
+// [n0, nrays) the closest-hit set (a.rays2 / a.hits2).
+template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC, bool PAIR = false>
+__device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays,
+                                           uint32_t n0 = 0) {
     __shared__ int lstack[S * BLOCK];
     int spill[kMaxStack - S];
     const int tid = threadIdx.x;
@@ -239,6 +246,13 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                 if (mine >= nrays) break;
                 GF4 rays = (GF4)a.rays;
                 rayidx = mine;
+                if (PAIR) {
+                    const bool second = mine >= n0;
+                    rays = second ? (GF4)a.rays2 : (GF4)a.rays;
+                    hits = second ? (GI2)a.hits2 : (GI2)a.hits;
+                    any = !second;
+                    rayidx = second ? mine - n0 : mine;
+                }
                 if (MULTI) {
                     uint32_t lo = 0, hi = nb;
                     while (hi - lo > 1) {
@@ -419,6 +433,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     trace_body<ANY, COUNT, S, BLOCK, false, SPEC>(a, nullptr, 0, nrays);
 }
 
+template <int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_pair(TraceArgs a) {
+    const uint32_t n0 = __builtin_nontemporal_load(a.n_dev);
+    const uint32_t n1 = __builtin_nontemporal_load(a.n_dev2);
+    trace_body<true, false, S, BLOCK, false, false, true>(a, nullptr, 0, n0 + n1, n0);
+}
+
 template <int S, int BLOCK, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a, BatchTable t) {
     __shared__ BatchDesc tab[kMaxBatch];
@@ -442,6 +463,14 @@ static TraceKernel pick(const rb_trace_config& c) {
 #define RB_K(s, b)                             \
     if (c.lds_stack == s && c.block == b)      \
         return c.speculative ? k_trace<ANY, COUNT, s, b, true> : k_trace<ANY, COUNT, s, b, false>;
+    RB_FOR_CONFIGS(RB_K)
+#undef RB_K
+    return nullptr;
+}
+
+static TraceKernel pick_pair(const rb_trace_config& c) {
+#define RB_K(s, b) \
+    if (c.lds_stack == s && c.block == b) return k_trace_pair<s, b>;
     RB_FOR_CONFIGS(RB_K)
 #undef RB_K
     return nullptr;
@@ -498,6 +527,47 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), (size_t)a.top_nodes * 64, stream, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "trace launch");
+    return RB_OK;
+}
+
+rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, const uint32_t* n0_dev, rb_hit* hits0,
+                            const rb_ray* rays1, uint32_t cap1, const uint32_t* n1_dev, rb_hit* hits1,
+                            hipStream_t stream, const rb_trace_config& cfg) {
+    TraceKernel k = cfg.speculative ? nullptr : pick_pair(cfg);
+    // the speculative order, or more rays than one counter range takes: two launches
+    if (!k || (uint64_t)cap0 + cap1 > 0x7FFFFFFFull) {
+        rb_status st = launch_trace(bvh, rays0, cap0, RB_ANY, nullptr, hits0, nullptr, stream, cfg, n0_dev);
+        if (st != RB_OK) return st;
+        return launch_trace(bvh, rays1, cap1, RB_CLOSEST, nullptr, hits1, nullptr, stream, cfg, n1_dev);
+    }
+    if (cap0 + cap1 == 0) return RB_OK;
+    if (!bvh || !rays0 || !rays1 || !hits0 || !hits1 || !n0_dev || !n1_dev)
+        return fail(RB_ERR_INVALID, "paired trace: NULL argument");
+    TraceArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.nodes = bvh->nodes;
+    a.tris = bvh->tris;
+    a.index = bvh->index;
+    a.rays = reinterpret_cast<const float4*>(rays0);
+    a.hits = reinterpret_cast<int2*>(hits0);
+    a.n_dev = n0_dev;
+    a.rays2 = reinterpret_cast<const float4*>(rays1);
+    a.hits2 = reinterpret_cast<int2*>(hits1);
+    a.n_dev2 = n1_dev;
+    a.counter = take_counter_slot();
+    if (!a.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
+    a.n = cap0 + cap1;
+    a.chunk = cfg.chunk;
+    a.threshold = cfg.fetch_threshold;
+    a.leaf_gate = cfg.leaf_gate;
+    a.top_nodes = cfg.lds_nodes < bvh->n_nodes ? cfg.lds_nodes : bvh->n_nodes;
+    const size_t lds = (size_t)cfg.lds_stack * cfg.block * 4 + (size_t)a.top_nodes * 64;
+    if (lds > dev().lds_per_block)
+        return fail(RB_ERR_INVALID, "trace config needs %zu bytes of LDS per block; the device allows %zu", lds,
+                    dev().lds_per_block);
+    hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, a.n)), dim3(cfg.block), (size_t)a.top_nodes * 64, stream, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "paired trace launch");
     return RB_OK;
 }
 
