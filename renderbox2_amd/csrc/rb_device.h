@@ -80,12 +80,13 @@ rb_status hip_fail(hipError_t e, const char* what);
 rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
                        rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg,
                        const uint32_t* n_dev = nullptr);
-// Two queued ray sets of one BVH in one persistent launch: set 0 any hit,
-// set 1 closest hit, their counts in device memory (cap0 / cap1 bound them
-// and size the grid).  Same per-ray results as two launch_trace calls, with
-// one launch's ramp-up and tail instead of two.
+// Two queued ray sets of one BVH in one persistent launch: set 0 any hit
+// (flags0 = RB_ANY) or closest hit (RB_CLOSEST), set 1 closest hit, their
+// counts in device memory (cap0 / cap1 bound them and size the grid).  Same
+// per-ray results as two launch_trace calls, with one launch's ramp-up and
+// tail instead of two.
 rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, const uint32_t* n0_dev, rb_hit* hits0,
-                            const rb_ray* rays1, uint32_t cap1, const uint32_t* n1_dev, rb_hit* hits1,
+                            uint32_t flags0, const rb_ray* rays1, uint32_t cap1, const uint32_t* n1_dev, rb_hit* hits1,
                             hipStream_t stream, const rb_trace_config& cfg);
 
 // Render passes (rb_render.hip), asynchronous on stream s.

@@ -21,6 +21,8 @@
 //    triangle or Gaussian; kernel_update_film,
 //    src/renderers/samplerrenderer.cu:103-169), in a fixed order and without
 //    float atomics, so images are bit-reproducible.
+#include <functional>
+
 #include "rb_shade.h"
 
 namespace rb {
@@ -152,6 +154,10 @@ __global__ void k_reset(uint32_t* c, uint32_t a, uint32_t b, uint32_t d) {
         c[b] = 0;
         c[d] = 0;
     }
+}
+
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
 __global__ void k_add64(unsigned long long* total, unsigned long long v) {
@@ -804,10 +810,14 @@ rb_status render_raycast(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32
     return film(Ls, cam, x0, y0, w, h, p, out, s);
 }
 
+// Traces the camera rays (prim, n of them) into phits; BDPT runs its light
+// pass here, its first launch paired with the camera rays.
+typedef std::function<rb_status(const float4* prim, int2* phits, uint32_t n)> PrimaryTrace;
+
 // Camera pass shared by the path tracer (BD = false) and BDPT (BD = true).
 static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
                              const rb_render_params* p, float* out, hipStream_t s, Scratch& M, bool bd,
-                             const BdArgs& B) {
+                             const BdArgs& B, const PrimaryTrace* primaries = nullptr) {
     rb_status st;
     const uint32_t n = w * h * p->spp;
     // one launch traces at most 2^31 - 1 rays: the shadow queue (light
@@ -844,8 +854,11 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     }
     if (M.st != RB_OK) return M.st;
     if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)prim, s)) != RB_OK) return st;
-    if ((st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg)) != RB_OK)
-        return st;
+    if (primaries)
+        st = (*primaries)(prim, phits, n);
+    else
+        st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg);
+    if (st != RB_OK) return st;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, 0u, 1u, 2u);
     hipLaunchKernelGGL(k_pt_init, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, phits, prim, n, P);
     TileArgs T{cam->width, x0, y0, w, p->spp};
@@ -863,8 +876,8 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
             hipLaunchKernelGGL(k_pt_shade<false>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
                                p->rr_start, p->rr_prob, B);
         // shadow rays (any hit) and MIS + bounce rays (closest hit) in one launch
-        if ((st = launch_trace_pair(bvh, (rb_ray*)P.ra, na, P.cnt + 2, (rb_hit*)P.ha, (rb_ray*)P.rbc, 2 * n, P.cnt + 3,
-                                    (rb_hit*)P.hbc, s, dev().cfg)) != RB_OK)
+        if ((st = launch_trace_pair(bvh, (rb_ray*)P.ra, na, P.cnt + 2, (rb_hit*)P.ha, RB_ANY, (rb_ray*)P.rbc, 2 * n,
+                                    P.cnt + 3, (rb_hit*)P.hbc, s, dev().cfg)) != RB_OK)
             return st;
         if (bd)
             hipLaunchKernelGGL(k_pt_connect<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, (float)bvh->n_lights,
@@ -938,23 +951,35 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
     LP.cnt = M.get<uint32_t>(4);
     LP.hit = M.get<int2>(NL);
     float4* lrays = M.get<float4>(2 * (size_t)NL);
+    uint32_t* ncam = M.get<uint32_t>(1);  // camera ray count for the paired launch
     if (M.st != RB_OK) return M.st;
     const SceneDev S = scene_of(bvh);
     const RngArg lrng = rng_arg(p->seed, RNG_LIGHT);
-    // light pass (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168)
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
-    hipLaunchKernelGGL(k_light_emit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, NL, S, lrng, LP, B.C);
-    for (uint32_t d = 0, qi = 0; d < D; d++, qi ^= 1) {
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, qi ^ 1, 2u, 3u);
-        hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, qi, LP, lrays);
-        if ((st = launch_trace(bvh, (rb_ray*)lrays, NL, RB_CLOSEST, nullptr, (rb_hit*)LP.hit, nullptr, s, dev().cfg,
-                               LP.cnt + qi)) != RB_OK)
-            return st;
-        if (p->ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->ray_counter);
-        hipLaunchKernelGGL(k_light_hit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, qi, d, S, lrng, LP, B.C);
-    }
+    // Light pass (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168),
+    // run once the camera rays exist: its first trace launch also traces
+    // them (independent work, one launch tail instead of two).
+    const PrimaryTrace light_pass = [&](const float4* prim, int2* phits, uint32_t n) -> rb_status {
+        rb_status r;
+        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, ncam, n);
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
+        hipLaunchKernelGGL(k_light_emit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, NL, S, lrng, LP, B.C);
+        for (uint32_t d = 0, qi = 0; d < D; d++, qi ^= 1) {
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, qi ^ 1, 2u, 3u);
+            hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, qi, LP, lrays);
+            if (d == 0)
+                r = launch_trace_pair(bvh, (rb_ray*)lrays, NL, LP.cnt + qi, (rb_hit*)LP.hit, RB_CLOSEST, (const rb_ray*)prim,
+                                      n, ncam, (rb_hit*)phits, s, dev().cfg);
+            else
+                r = launch_trace(bvh, (rb_ray*)lrays, NL, RB_CLOSEST, nullptr, (rb_hit*)LP.hit, nullptr, s, dev().cfg,
+                                 LP.cnt + qi);
+            if (r != RB_OK) return r;
+            if (p->ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->ray_counter);
+            hipLaunchKernelGGL(k_light_hit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, qi, d, S, lrng, LP, B.C);
+        }
+        return RB_OK;
+    };
     // camera pass (kernel_sk_camera_trace, integrator_bdpt_lvc_sk.cu:170-435)
-    return camera_pass(bvh, cam, x0, y0, w, h, p, out, s, M, true, B);
+    return camera_pass(bvh, cam, x0, y0, w, h, p, out, s, M, true, B, &light_pass);
 }
 
 uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_render_params* p) {

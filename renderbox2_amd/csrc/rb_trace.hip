@@ -45,10 +45,12 @@ struct TraceArgs {
     uint32_t* counter;
     uint32_t n;
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
-    // paired launch: a second ray set (closest hit) after the first (any hit)
+    // paired launch: a second ray set (closest hit) after the first (any hit
+    // when any0 is set, else closest hit)
     const float4* __restrict__ rays2;
     int2* __restrict__ hits2;
     const uint32_t* n_dev2;
+    uint32_t any0;
     uint32_t chunk;
     uint32_t threshold;
     uint32_t leaf_gate;  // non-speculative: leave the inner-node phase once <= this many lanes still search
@@ -164,8 +166,8 @@ __global__ __launch_bounds__(256) void k_rcp_check(unsigned long long* out) {
     }
 }
 
-// PAIR: rays [0, n0) are the any-hit set (a.rays / a.hits), rays
-// [n0, nrays) the closest-hit set (a.rays2 / a.hits2).
+// PAIR: rays [0, n0) are the first set (a.rays / a.hits; any hit when
+// a.any0), rays [n0, nrays) the closest-hit set (a.rays2 / a.hits2).
 template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC, bool PAIR = false>
 __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays,
                                            uint32_t n0 = 0) {
@@ -250,7 +252,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                     const bool second = mine >= n0;
                     rays = second ? (GF4)a.rays2 : (GF4)a.rays;
                     hits = second ? (GI2)a.hits2 : (GI2)a.hits;
-                    any = !second;
+                    any = !second && a.any0 != 0;
                     rayidx = second ? mine - n0 : mine;
                 }
                 if (MULTI) {
@@ -531,12 +533,12 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
 }
 
 rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, const uint32_t* n0_dev, rb_hit* hits0,
-                            const rb_ray* rays1, uint32_t cap1, const uint32_t* n1_dev, rb_hit* hits1,
+                            uint32_t flags0, const rb_ray* rays1, uint32_t cap1, const uint32_t* n1_dev, rb_hit* hits1,
                             hipStream_t stream, const rb_trace_config& cfg) {
     TraceKernel k = cfg.speculative ? nullptr : pick_pair(cfg);
     // the speculative order, or more rays than one counter range takes: two launches
     if (!k || (uint64_t)cap0 + cap1 > 0x7FFFFFFFull) {
-        rb_status st = launch_trace(bvh, rays0, cap0, RB_ANY, nullptr, hits0, nullptr, stream, cfg, n0_dev);
+        rb_status st = launch_trace(bvh, rays0, cap0, flags0 & RB_ANY, nullptr, hits0, nullptr, stream, cfg, n0_dev);
         if (st != RB_OK) return st;
         return launch_trace(bvh, rays1, cap1, RB_CLOSEST, nullptr, hits1, nullptr, stream, cfg, n1_dev);
     }
@@ -554,6 +556,7 @@ rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, co
     a.rays2 = reinterpret_cast<const float4*>(rays1);
     a.hits2 = reinterpret_cast<int2*>(hits1);
     a.n_dev2 = n1_dev;
+    a.any0 = (flags0 & RB_ANY) ? 1u : 0u;
     a.counter = take_counter_slot();
     if (!a.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
     a.n = cap0 + cap1;
