@@ -27,9 +27,16 @@
 //  * batched mode: one launch serves up to kMaxBatch ray batches against
 //    different resident BVHs; every lane carries the scene pointers and the
 //    any/closest flag of the batch its current ray belongs to, so a wave can
-//    mix rays of several scenes while it refills.
-// Arithmetic (fp32, explicit fmaf, exact division) matches the oracle's
-// restatement bit for bit.
+//    mix rays of several scenes while it refills;
+//  * paired mode: two queued ray sets of one BVH (an any-hit or closest-hit
+//    set, then a closest-hit set) in one launch, so a render bounce pays
+//    one launch's ramp-up and tail instead of two;
+//  * top of the tree in LDS: each block copies the first lds_nodes node
+//    records (the layout stores them breadth-first) into LDS at start and
+//    reads them from there; every ray passes through them, often several
+//    times (the reference reads every node through the texture path).
+// Arithmetic (fp32, explicit fmaf, reciprocals equal to the IEEE division)
+// matches the oracle's restatement bit for bit.
 #include "rb_device.h"
 
 namespace rb {
