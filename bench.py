@@ -448,7 +448,7 @@ def main():
                 "lds_nodes"]
         rb.set_trace_config(**dict(zip(keys, (int(x) for x in args.config.split(",")))))
     elif args.autotune:
-        _, tuned_ms = rb.autotune(tracer, d_diff, n, d_hits, reps=3, stream=stream)
+        _, tuned_ms = rb.autotune(tracer, d_diff, n, d_hits, reps=5, stream=stream)
     cfg = rb.get_trace_config()
     log("trace config", cfg, "autotuned %.3f ms" % tuned_ms if tuned_ms else "")
 
