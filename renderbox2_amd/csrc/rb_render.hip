@@ -94,8 +94,14 @@ struct PathBufs {
     float4* T;       // throughput (w unused)
     float4* Tn;      // throughput after this bounce's sample
     float4* L;       // radiance
-    float4* ray;     // 2 per path: current ray
-    int2* hit;       // current hit
+    // The current vertex of a path: ray 2 * cur[pid] of pray and hit
+    // cur[pid] of phit -- the camera rays at bounce 0, then the previous
+    // bounce's closest-hit queue, read in place (the MIS + bounce queues
+    // alternate between two buffers, so this bounce's rays do not overwrite
+    // them)
+    const float4* pray;
+    const int2* phit;
+    uint32_t* cur;
     float4* ca;      // light-sample contribution if unoccluded; w = direct-light technique weight
     float4* cb;      // MIS contribution if the MIS ray hits light `lprim`
     int4* slot;      // x: shadow slot, y: MIS slot, z: bounce slot, w: light triangle
@@ -126,8 +132,7 @@ struct BdArgs {
     RngArg crng;
 };
 
-__global__ __launch_bounds__(kQueueBlock) void k_pt_init(const int2* __restrict__ hits,
-                                                          const float4* __restrict__ prim, uint32_t n, PathBufs P) {
+__global__ __launch_bounds__(kQueueBlock) void k_pt_init(const int2* __restrict__ hits, uint32_t n, PathBufs P) {
     const uint32_t end = uniform_end(n);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
         const bool valid = i < n;
@@ -136,9 +141,7 @@ __global__ __launch_bounds__(kQueueBlock) void k_pt_init(const int2* __restrict_
             const int2 h = hits[i];
             P.T[i] = make_float4(1.f, 1.f, 1.f, 0.f);
             P.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            P.ray[2 * (size_t)i] = prim[2 * (size_t)i];
-            P.ray[2 * (size_t)i + 1] = prim[2 * (size_t)i + 1];
-            P.hit[i] = h;
+            P.cur[i] = i;
             P.info[i] = 0;
             if (P.cmask) P.cmask[i] = 0;
             live = h.x >= 0;
@@ -194,8 +197,9 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
         float eps = 0.0f;
         int4 slot = make_int4(-1, -1, -1, -1);  // written once, after the queue reservations
         if (valid) {
-            const float4 o = P.ray[2 * (size_t)pid], d = P.ray[2 * (size_t)pid + 1];
-            const int2 h = P.hit[pid];
+            const uint32_t c = P.cur[pid];
+            const float4 o = P.pray[2 * (size_t)c], d = P.pray[2 * (size_t)c + 1];
+            const int2 h = P.phit[c];
             const uint32_t info = P.info[pid];
             const uint32_t bounce = info & 255u;
             const bool spec = (info >> 8) & 1u;
@@ -375,8 +379,9 @@ __global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_
         bool want = false;
         float4 r0, r1;
         if (valid) {
-            const float4 o = P.ray[2 * (size_t)pid], d = P.ray[2 * (size_t)pid + 1];
-            const int2 h = P.hit[pid];
+            const uint32_t c = P.cur[pid];
+            const float4 o = P.pray[2 * (size_t)c], d = P.pray[2 * (size_t)c + 1];
+            const int2 h = P.phit[c];
             const uint32_t bounce = P.info[pid] & 255u;
             const float th = __int_as_float(h.y);
             const V3 dv = v3(d.x, d.y, d.z);
@@ -483,9 +488,7 @@ __global__ __launch_bounds__(kQueueBlock) void k_pt_connect(PathBufs P, uint32_t
                 if (s.z >= 0) {
                     const int2 h = P.hbc[s.z];
                     if (h.x >= 0) {
-                        P.ray[2 * (size_t)pid] = P.rbc[2 * (size_t)s.z];
-                        P.ray[2 * (size_t)pid + 1] = P.rbc[2 * (size_t)s.z + 1];
-                        P.hit[pid] = h;
+                        P.cur[pid] = (uint32_t)s.z;  // the next vertex: bounce ray s.z, read in place
                         P.T[pid] = P.Tn[pid];
                         const uint32_t b = (info & 255u) + 1u;
                         P.info[pid] = b | (((info >> 9) & 1u) << 8);
@@ -834,8 +837,7 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     P.T = M.get<float4>(n);
     P.Tn = M.get<float4>(n);
     P.L = M.get<float4>(n);
-    P.ray = M.get<float4>(2 * (size_t)n);
-    P.hit = M.get<int2>(n);
+    P.cur = M.get<uint32_t>(n);
     P.ca = M.get<float4>(n);
     P.cb = M.get<float4>(n);
     P.slot = M.get<int4>(n);
@@ -845,8 +847,8 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     P.cnt = M.get<uint32_t>(4);
     P.ra = M.get<float4>(2 * (size_t)na);
     P.ha = M.get<int2>(na);
-    P.rbc = M.get<float4>(4 * (size_t)n);
-    P.hbc = M.get<int2>(2 * (size_t)n);
+    float4* rbcq[2] = {M.get<float4>(4 * (size_t)n), M.get<float4>(4 * (size_t)n)};
+    int2* hbcq[2] = {M.get<int2>(2 * (size_t)n), M.get<int2>(2 * (size_t)n)};
     if (bd) {
         P.cmask = M.get<uint32_t>(n);
         P.cc = M.get<float4>((size_t)n * kMaxConn);
@@ -860,11 +862,16 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
         st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg);
     if (st != RB_OK) return st;
     hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, 0u, 1u, 2u);
-    hipLaunchKernelGGL(k_pt_init, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, phits, prim, n, P);
+    hipLaunchKernelGGL(k_pt_init, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, phits, n, P);
     TileArgs T{cam->width, x0, y0, w, p->spp};
     const SceneDev S = scene_of(bvh);
     const RngArg rng = rng_arg(p->seed, RNG_PATH);
     for (uint32_t b = 0, qi = 0; b <= p->max_depth; b++, qi ^= 1) {
+        // this bounce's vertices, and the buffers its MIS + bounce rays go to
+        P.pray = b == 0 ? prim : rbcq[(b - 1) & 1];
+        P.phit = b == 0 ? phits : hbcq[(b - 1) & 1];
+        P.rbc = rbcq[b & 1];
+        P.hbc = hbcq[b & 1];
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
         if (bd) {
             hipLaunchKernelGGL(k_pt_shade<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
@@ -910,7 +917,7 @@ uint64_t bdpt_scratch_bytes(uint32_t w, uint32_t h, const rb_render_params* p) {
     const uint64_t D = p->light_max_depth ? p->light_max_depth : 1;
     const uint64_t nl = p->light_paths ? p->light_paths : p->lvc_capacity / D;
     const uint64_t nc = p->connections;
-    return n * 308 + n * (4 + kMaxConn * 20) + n * nc * 40 + nl * D * 64 + nl * (32 + 16 + 4 + 8 + 4 + 8 + 32);
+    return n * 352 + n * (4 + kMaxConn * 20) + n * nc * 40 + nl * D * 64 + nl * (32 + 16 + 4 + 8 + 4 + 8 + 32);
 }
 
 rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
@@ -987,7 +994,7 @@ uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_re
     const uint64_t slack = 64 * 40;  // per-allocation padding and alignment
     switch (kind) {
         case RB_REQ_AO: return n * 56 + n * (p->ao_samples ? p->ao_samples : 1) * 41 + slack;
-        case RB_REQ_PT: return n * 308 + slack;
+        case RB_REQ_PT: return n * 352 + slack;
         case RB_REQ_BDPT: return bdpt_scratch_bytes(w, h, p) + slack;
         case RB_REQ_RAYCAST: return n * 56 + slack;
         default: return slack;
