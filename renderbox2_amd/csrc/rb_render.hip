@@ -91,8 +91,8 @@ __global__ void k_ao_resolve(const int2* __restrict__ prim_hits, const int2* __r
 // ---------------------------------------------------------------- PT ----
 
 struct PathBufs {
-    float4* T;       // throughput (w unused)
-    float4* Tn;      // throughput after this bounce's sample
+    float4* T;       // throughput at this bounce (w unused)
+    float4* Tn;      // throughput after this bounce's sample: the next bounce's T (the two swap per bounce)
     float4* L;       // radiance
     // The current vertex of a path: ray 2 * cur[pid] of pray and hit
     // cur[pid] of phit -- the camera rays at bounce 0, then the previous
@@ -489,7 +489,6 @@ __global__ __launch_bounds__(kQueueBlock) void k_pt_connect(PathBufs P, uint32_t
                     const int2 h = P.hbc[s.z];
                     if (h.x >= 0) {
                         P.cur[pid] = (uint32_t)s.z;  // the next vertex: bounce ray s.z, read in place
-                        P.T[pid] = P.Tn[pid];
                         const uint32_t b = (info & 255u) + 1u;
                         P.info[pid] = b | (((info >> 9) & 1u) << 8);
                         next = true;
@@ -834,8 +833,9 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     int2* phits = M.get<int2>(n);
     PathBufs P;
     std::memset(&P, 0, sizeof(P));
-    P.T = M.get<float4>(n);
-    P.Tn = M.get<float4>(n);
+    float4* Tq[2] = {M.get<float4>(n), M.get<float4>(n)};
+    P.T = Tq[0];  // k_pt_init writes bounce 0's
+    P.Tn = Tq[1];
     P.L = M.get<float4>(n);
     P.cur = M.get<uint32_t>(n);
     P.ca = M.get<float4>(n);
@@ -872,6 +872,8 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
         P.phit = b == 0 ? phits : hbcq[(b - 1) & 1];
         P.rbc = rbcq[b & 1];
         P.hbc = hbcq[b & 1];
+        P.T = Tq[b & 1];
+        P.Tn = Tq[(b + 1) & 1];
         hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
         if (bd) {
             hipLaunchKernelGGL(k_pt_shade<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
