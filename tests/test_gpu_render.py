@@ -56,6 +56,45 @@ def test_pt_image_bit_exact(gpu, orc, pt_setup):
     assert not bad.any(), f"{bad.sum()} channels differ, max abs {np.abs(img - ref).max()}"
 
 
+@pytest.mark.parametrize("kind", ["pt", "bdpt"])
+def test_launch_shapes_give_the_same_image(gpu, pt_setup, kind):
+    # the bounce traces run as one paired launch (any-hit shadow queue +
+    # closest-hit queue) with the top of the tree in LDS; other block sizes,
+    # no LDS records and the speculative order (which falls back to two
+    # launches per bounce) give the same image -- bit for bit except the
+    # speculative order, whose rare boundary cases may move a sample
+    s, b, tr = pt_setup
+    cam = gpu.scenes.pt_camera(48, 48)
+    params = dict(spp=2, seed=9, max_depth=6, light_paths=2048, light_max_depth=3, connections=3) \
+        if kind == "bdpt" else dict(spp=2, seed=9, max_depth=6)
+    saved = gpu.get_trace_config()
+    try:
+        ref = gpu.SamplerRenderer(tr, cam, kind, **params).render()
+        for cfg in (dict(block=256, lds_stack=16, lds_nodes=0), dict(block=1024, lds_stack=8, lds_nodes=512)):
+            gpu.set_trace_config(**cfg)
+            img = gpu.SamplerRenderer(tr, cam, kind, **params).render()
+            assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), cfg
+        gpu.set_trace_config(**dict(saved, speculative=1))
+        img = gpu.SamplerRenderer(tr, cam, kind, **params).render()
+        assert np.isfinite(img).all()
+        assert (img.view(np.uint32) == ref.view(np.uint32)).mean() > 0.99
+        assert abs(img.mean() - ref.mean()) <= 0.01 * ref.mean()
+    finally:
+        gpu.set_trace_config(**saved)
+
+
+def test_trace_configs_without_a_kernel_are_rejected(gpu):
+    saved = gpu.get_trace_config()
+    try:
+        for bad in (dict(block=1024, lds_stack=24), dict(block=512, lds_stack=20), dict(block=384),
+                    dict(lds_nodes=576), dict(lds_nodes=100)):
+            with pytest.raises(gpu.RbError):
+                gpu.set_trace_config(**bad)
+        assert gpu.get_trace_config() == saved
+    finally:
+        gpu.set_trace_config(**saved)
+
+
 def test_tiles_match_full_frame(gpu, pt_setup):
     s, b, tr = pt_setup
     cam = gpu.scenes.pt_camera(48, 40)
