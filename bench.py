@@ -489,7 +489,11 @@ def main():
     bytes_per_ray = None
     if args.cpu_seconds > 0:
         from oracle import oracle
-        cores = min(os.cpu_count() or 1, 16)
+        # every CPU this process may run on, capped by the host's own thread
+        # setting where it has one (the GPU host gives a GPU's job 16 CPUs)
+        affinity = len(os.sched_getaffinity(0))
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        cores = min(affinity, int(omp)) if omp.isdigit() and int(omp) > 0 else affinity
         rays = d_diff.download(np.float32, n * 8).reshape(n, 8)
         cn, cw, cx = bvh.export(L.RB_LAYOUT_REF_COMPACT2)
         cn, cw = cn.reshape(-1, 4), cw.reshape(-1, 4)
@@ -497,23 +501,28 @@ def main():
         t1 = time.time()
         oracle.trace_compact2(cn, cw, cx, probe, nthreads=cores)
         per_ray = (time.time() - t1) / len(probe)
-        size = int(min(n, max(1 << 16, args.cpu_seconds / 3.0 / max(per_ray, 1e-9))))
+        # 1 warm-up + 5 timed runs within the budget
+        size = int(min(n, max(1 << 16, args.cpu_seconds / 6.0 / max(per_ray, 1e-9))))
         sample = rays[:size]
         times = []
-        for _ in range(3):
+        for rep in range(6):
             t1 = time.time()
             oid, ot, ocnt = oracle.trace_compact2(cn, cw, cx, sample, counts=True, nthreads=cores)
-            times.append(time.time() - t1)
-        tcpu = sorted(times)[1]
+            if rep:
+                times.append(time.time() - t1)
+        tcpu = sorted(times)[2]
         bytes_per_ray = float(oracle.bytes_per_ray(ocnt, oid >= 0).mean())
         counts_equal = bool(np.array_equal(ocnt, gcnt[:size]))
         same = bool(np.array_equal(oid, hits["id"][:size]) and
                     np.array_equal(ot.view(np.uint32), hits["t"][:size].view(np.uint32)))
         cpu = {"value": round(size / tcpu / 1e6, 3), "unit": "Mrays/s", "cores": cores, "kind": "port",
-               "sample": f"first {size} of the {n} incoherent rays, closest hit, median of 3 "
-                         f"(OpenMP, {cpu_model()})",
+               "sample": f"first {size} of the {n} incoherent rays, closest hit, 1 warm-up + median of 5, "
+                         f"OpenMP {cores} threads ({affinity} CPUs in the affinity mask, OMP_NUM_THREADS="
+                         f"{omp or 'unset'}), {cpu_model()}",
+               "cpu_model": cpu_model(), "affinity_cpus": affinity,
                "matches_gpu": same, "visit_counts_equal": counts_equal}
-        log(f"cpu baseline: {cpu['value']} Mrays/s on {cores} cores, B/ray {bytes_per_ray:.0f}, match={same}")
+        log(f"cpu baseline: {cpu['value']} Mrays/s on {cores} threads ({affinity} CPUs), B/ray {bytes_per_ray:.0f}, "
+            f"match={same}")
     if bytes_per_ray is None:
         bytes_per_ray = gpu_bpr
 

@@ -249,6 +249,7 @@ typedef struct rb_bvh_dev {
     const void* tris;       /* float4[3 * n_refs]: Woop rows */
     const uint32_t* index;  /* uint32[n_refs]: original triangle id per reference */
     uint32_t n_nodes, n_refs;
+    uint32_t* errors;       /* device word for RB_TRACE_ERR_* flags (rb_trace_errors); may be NULL */
 } rb_bvh_dev;
 rb_status rb_bvh_device_view(rb_bvh_t b, rb_bvh_dev* out);
 
@@ -331,6 +332,18 @@ rb_status rb_trace_set_config(const rb_trace_config* c);
  * fastest and returns it with its time per launch. */
 rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits,
                             uint32_t reps, rb_trace_config* best, float* best_ms, void* stream);
+/* Traversal error flags raised on the device since the last call (the
+ * reference has no error path: its 64-entry local stack is written past,
+ * kepler_dynamic_fetch.h:62-71,199-216).  RB_TRACE_ERR_STACK: a ray needed
+ * more stack entries than the traversal holds -- unreachable for a BVH that
+ * rb_bvh_upload accepted (it refuses trees that deep), set only if the
+ * device data was corrupted or a hand-built rb_bvh_dev was traced.  The
+ * pushes past the stack are dropped, so results of the launches since the
+ * last call are not trustworthy.  Waits for the device, returns the flags
+ * in *flags (0 = none), clears them, and returns RB_ERR_INVALID when any
+ * was set. */
+enum { RB_TRACE_ERR_STACK = 1 };
+rb_status rb_trace_errors(uint32_t* flags);
 /* Per-ray traversal counters of the device kernel (inner nodes, triangles,
  * leaves) for traffic accounting; counts is device uint32[3*n]. */
 rb_status rb_trace_counts(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,

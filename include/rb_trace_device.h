@@ -49,12 +49,16 @@ __device__ __forceinline__ rb_hit rb_trace_one(const rb_bvh_dev& b, const rb_ray
     int sp = 0;
     stack[0] = RB_DEV_SENTINEL;
     int nodeAddr = 0, leafAddr = 0, hitIndex = -1;
-#define RB_DEV_PUSH(v)                    \
-    do {                                  \
-        if (sp + 1 < RB_DEV_STACK) {      \
-            ++sp;                         \
-            stack[sp] = (v);              \
-        }                                 \
+    /* the upload refuses trees deep enough to fill the stack; a push past
+       it is dropped and flagged (rb_trace_errors) */
+#define RB_DEV_PUSH(v)                                               \
+    do {                                                             \
+        if (sp + 1 < RB_DEV_STACK) {                                 \
+            ++sp;                                                    \
+            stack[sp] = (v);                                         \
+        } else if (b.errors) {                                       \
+            atomicOr(b.errors, (uint32_t)RB_TRACE_ERR_STACK);        \
+        }                                                            \
     } while (0)
 #define RB_DEV_POP(dst)      \
     do {                     \

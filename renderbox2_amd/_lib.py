@@ -5,7 +5,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "librb.so")
+# RB_LIB selects another build of the same library (the sanitizer build of
+# `make -C renderbox2_amd/csrc -f asan.mk asan` is librb_asan.so).
+LIB_PATH = os.environ.get("RB_LIB") or os.path.join(_HERE, "librb.so")
 
 P = C.c_void_p
 U32 = C.c_uint32
@@ -122,6 +124,7 @@ _SIGS = {
     "rb_trace_get_config": (C.c_int, [P]),
     "rb_trace_set_config": (C.c_int, [P]),
     "rb_trace_counts": (C.c_int, [P, P, U32, U32, P, P, P]),
+    "rb_trace_errors": (C.c_int, [P]),
     "rb_trace_autotune": (C.c_int, [P, P, U32, U32, P, U32, P, P, P]),
     "rb_copy_kernel": (C.c_int, [P, P, SZ, P]),
     "rb_selftest_rcp": (C.c_int, [P, P]),
@@ -180,7 +183,7 @@ class rb_server_stats(C.Structure):
 
 
 class rb_bvh_dev(C.Structure):
-    _fields_ = [("nodes", P), ("tris", P), ("index", P), ("n_nodes", U32), ("n_refs", U32)]
+    _fields_ = [("nodes", P), ("tris", P), ("index", P), ("n_nodes", U32), ("n_refs", U32), ("errors", P)]
 
 
 class rb_batch_desc(C.Structure):

@@ -135,8 +135,9 @@ __global__ __launch_bounds__(kQueueBlock) void k_light_emit(uint32_t npaths, Sce
     const uint32_t end = uniform_end(npaths);
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < end; j += gridDim.x * blockDim.x) {
         const bool valid = j < npaths;
-        if (valid) {
-            C.count[j] = 0;
+        const bool live = valid && S.n_lights > 0;  // no emitters: no light paths, empty cache
+        if (valid) C.count[j] = 0;
+        if (live) {
             const U4 r = rng.draw(j, 0, 0, 0);
             const U4 r2 = rng.draw(j, 0, 1, 0);
             uint32_t lid = (uint32_t)(u01(r.x) * (float)S.n_lights);
@@ -160,8 +161,8 @@ __global__ __launch_bounds__(kQueueBlock) void k_light_emit(uint32_t npaths, Sce
             P.ray[2 * (size_t)j + 1] = make_float4(dir.x, dir.y, dir.z, 3.402823466e+38f);
             P.mask[j] = 0;
         }
-        const uint32_t q = block_reserve(&P.cnt[0], valid && S.n_lights > 0 ? 1u : 0u);
-        if (valid && S.n_lights > 0) P.q[0][q] = j;
+        const uint32_t q = block_reserve(&P.cnt[0], live ? 1u : 0u);
+        if (live) P.q[0][q] = j;
     }
 }
 

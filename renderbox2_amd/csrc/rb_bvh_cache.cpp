@@ -235,9 +235,21 @@ rb_status load_bvh_binary(const char* path, const float* verts, uint32_t nv, con
     H->tri_blocks.emplace_back(new std::vector<uint32_t>(std::move(refs)));
     std::vector<BNode>& bn = *H->node_blocks.back();
     uint32_t* tri = H->tri_blocks.back()->data();
+    // Children always follow their parent, so one forward pass gives every
+    // node's depth (set by the parent before the node is reached) and how
+    // often it is linked: a tree links every node but the root exactly once,
+    // and the builder's trees are at most kMaxTreeDepth deep (the traversal
+    // stack is sized from that bound).  The checksum detects corruption; these
+    // checks refuse a well-formed file that no build wrote.
+    std::vector<uint8_t> depth(hd.n_nodes, 0), links(hd.n_nodes, 0);
     for (uint32_t i = 0; i < hd.n_nodes; i++) {
         const FileNode& f = nodes[i];
         BNode& n = bn[i];
+        if (i > 0 && links[i] != 1)
+            return fail(RB_ERR_INVALID, "%s: node %u is linked %u times (not a tree)", path, i, links[i]);
+        if (depth[i] > kMaxTreeDepth)
+            return fail(RB_ERR_INVALID, "%s: node %u at depth %u, deeper than a build makes (%u)", path, i, depth[i],
+                        kMaxTreeDepth);
         std::memcpy(&n.b.lo, f.lo, 12);
         std::memcpy(&n.b.hi, f.hi, 12);
         if (f.leaf_count & 0x80000000u) {
@@ -254,6 +266,10 @@ rb_status load_bvh_binary(const char* path, const float* verts, uint32_t nv, con
             n.leaf = false;
             n.c[0] = &bn[i + 1];
             n.c[1] = &bn[f.link];
+            for (uint32_t c : {i + 1, f.link}) {
+                depth[c] = (uint8_t)(depth[i] + 1);
+                if (links[c] < 255) links[c]++;
+            }
         }
     }
     H->root = &bn[0];

@@ -9,8 +9,14 @@ namespace rb {
 
 // Work-counter slots: each launch of a persistent kernel takes one slot
 // (work counter + exit counter); the last block out resets both to zero, so
-// no memset node is needed between launches or inside captured graphs.
+// no memset node is needed between launches or inside captured graphs.  A
+// slot is reused kCounterSlots launches later; when that launch is on
+// another stream it first waits (stream-ordered, no host wait) for the
+// event recorded after the slot's previous launch, so two launches never
+// share a counter whatever the streams and the number in flight.
 constexpr uint32_t kCounterSlots = 4096;
+// Words after the counter pairs: [0] traversal error flags (RB_TRACE_ERR_*).
+constexpr uint32_t kErrorWords = 64;
 
 // Traversal stack bound: the builder stops at depth 64 and leaves of more than
 // 31 references are split by the exporter, so 96 entries never overflow.
@@ -22,7 +28,8 @@ struct DeviceState {
     uint32_t num_cus = 0;
     size_t lds_per_block = 65536;  // LDS one workgroup may use (static + dynamic)
     hipStream_t stream = nullptr;  // library default stream
-    uint32_t* counters = nullptr;  // 2 * kCounterSlots
+    uint32_t* counters = nullptr;  // 2 * kCounterSlots, then kErrorWords
+    uint32_t* errors = nullptr;    // counters + 2 * kCounterSlots
     std::atomic<uint32_t> next_slot{0};
     std::atomic<size_t> library_bytes{0};
     rb_trace_config cfg{};
@@ -30,8 +37,18 @@ struct DeviceState {
 DeviceState& dev();
 rb_status ensure_device();
 // A launch's work-counter pair: from the ring, or from the pool of the render
-// graph being captured on this thread (nullptr when that pool is used up).
-uint32_t* take_counter_slot();
+// graph being captured on this thread (counter == nullptr when that pool is
+// used up).  take_counter_slot makes `stream` wait for the slot's previous
+// launch when that was on another stream; release_counter_slot, called once
+// the launch is enqueued, records the slot's event.  Every take is followed
+// by exactly one release.
+struct SlotLease {
+    uint32_t* counter = nullptr;
+    int32_t slot = -1;  // ring slot, -1 for a graph's own pair
+    rb_status st = RB_OK;
+};
+SlotLease take_counter_slot(hipStream_t stream);
+void release_counter_slot(const SlotLease& l, hipStream_t stream);
 
 // Set on the capturing thread while a render pass is recorded into a graph
 // (rb_render_graph_create): kernels read their stream keys from `keys`
@@ -76,10 +93,12 @@ rb_status hip_fail(hipError_t e, const char* what);
 
 // Persistent traversal launch (rb_trace.hip).  n_dev, when set, holds the ray
 // count in device memory (queues sized by earlier kernels); n is then an
-// upper bound that only sizes the grid.
+// upper bound that only sizes the grid.  counter, when set, is the launch's
+// zeroed work-counter pair (two words the kernel leaves zero again) instead
+// of a slot of the ring.
 rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
                        rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg,
-                       const uint32_t* n_dev = nullptr);
+                       const uint32_t* n_dev = nullptr, uint32_t* counter = nullptr);
 // Two queued ray sets of one BVH in one persistent launch: set 0 any hit
 // (flags0 = RB_ANY) or closest hit (RB_CLOSEST), set 1 closest hit, their
 // counts in device memory (cap0 / cap1 bound them and size the grid).  Same

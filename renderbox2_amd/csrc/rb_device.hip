@@ -30,9 +30,11 @@ static rb_status init_locked(int device) {
     if ((e = hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking)) != hipSuccess)
         return hip_fail(e, "hipStreamCreate");
     void* c = nullptr;
-    if ((e = hipMalloc(&c, sizeof(uint32_t) * 2 * kCounterSlots)) != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
-    if ((e = hipMemset(c, 0, sizeof(uint32_t) * 2 * kCounterSlots)) != hipSuccess) return hip_fail(e, "hipMemset(counters)");
+    const size_t cbytes = sizeof(uint32_t) * (2 * kCounterSlots + kErrorWords);
+    if ((e = hipMalloc(&c, cbytes)) != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
+    if ((e = hipMemset(c, 0, cbytes)) != hipSuccess) return hip_fail(e, "hipMemset(counters)");
     g_dev.counters = (uint32_t*)c;
+    g_dev.errors = g_dev.counters + 2 * kCounterSlots;
     // Render scratch comes from the stream-ordered pool; keep freed blocks in
     // the pool instead of returning them at every synchronisation, so a
     // steady stream of renders reuses its memory rather than remapping it.
@@ -66,13 +68,56 @@ rb_status ensure_device() {
     return init_locked(0);
 }
 
-uint32_t* take_counter_slot() {
+// Per ring slot: the stream of its last launch and an event recorded after
+// it (created on the slot's first use).  Striped locks keep a take and its
+// release atomic against another thread wrapping round to the same slot.
+static hipStream_t g_slot_stream[kCounterSlots];
+static hipEvent_t g_slot_event[kCounterSlots];
+static std::mutex g_slot_mu[64];
+
+SlotLease take_counter_slot(hipStream_t stream) {
+    SlotLease l;
     if (t_capture) {
-        if (t_capture->next_pair >= t_capture->n_pairs) return nullptr;
-        return t_capture->counters + 2 * t_capture->next_pair++;
+        if (t_capture->next_pair < t_capture->n_pairs) l.counter = t_capture->counters + 2 * t_capture->next_pair++;
+        return l;
     }
-    uint32_t s = g_dev.next_slot.fetch_add(1) % kCounterSlots;
-    return g_dev.counters + 2 * s;
+    const uint32_t s = g_dev.next_slot.fetch_add(1) % kCounterSlots;
+    g_slot_mu[s % 64].lock();
+    hipEvent_t& ev = g_slot_event[s];
+    if (!ev) {
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            ev = nullptr;
+            g_slot_mu[s % 64].unlock();
+            l.st = hip_fail(e, "work-counter slot event");
+            return l;
+        }
+    } else if (g_slot_stream[s] != stream) {
+        hipError_t e = hipStreamWaitEvent(stream, ev, 0);
+        if (e != hipSuccess) {
+            g_slot_mu[s % 64].unlock();
+            l.st = hip_fail(e, "work-counter slot wait");
+            return l;
+        }
+    }
+    l.counter = g_dev.counters + 2 * s;
+    l.slot = (int32_t)s;
+    return l;  // the slot's lock is held until release_counter_slot
+}
+
+void release_counter_slot(const SlotLease& l, hipStream_t stream) {
+    if (l.slot < 0) return;
+    const uint32_t s = (uint32_t)l.slot;
+    if (hipEventRecord(g_slot_event[s], stream) == hipSuccess) {
+        g_slot_stream[s] = stream;
+    } else {
+        // no event marks this launch's end: finish it before the slot is
+        // handed out again (error path only)
+        (void)hipGetLastError();
+        (void)hipStreamSynchronize(stream);
+        (void)hipGetLastError();
+    }
+    g_slot_mu[s % 64].unlock();
 }
 
 rb_status dalloc(void** p, size_t bytes) {
@@ -287,6 +332,13 @@ rb_status rb_bvh_upload(rb_host_bvh_t hb, rb_scene_t scene, rb_bvh_t* out) {
         if (st != RB_OK) return st;
     } catch (...) {
         return fail(RB_ERR_OOM, "layout export failed (host allocation)");
+    }
+    {
+        // every stack entry a ray can need must exist: the kernels hold kMaxStack
+        const uint32_t depth = layout_depth(L.nodes);
+        if (depth == UINT32_MAX || depth + 1 >= (uint32_t)kMaxStack)
+            return fail(RB_ERR_UNSUPPORTED, "BVH node layout is %s (the traversal stack holds %d entries)",
+                        depth == UINT32_MAX ? "not a tree" : "too deep", kMaxStack);
     }
     if (scene && scene->s.num_tris() != (uint32_t)(hb->b->idx.size() / 3))
         return fail(RB_ERR_INVALID, "scene has %u triangles, the BVH was built over %zu", scene->s.num_tris(),

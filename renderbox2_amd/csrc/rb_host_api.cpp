@@ -116,12 +116,12 @@ rb_status rb_scene_add_mesh(rb_scene_t s, const float* verts, uint32_t nv, const
     RB_CHECK_ARG(s, "NULL scene");
     RB_CHECK_ARG(nt == 0 || (verts && idx), "NULL mesh arrays");
     RB_CHECK_ARG(material_id < s->s.mats.size(), "material id out of range (add the material first)");
+    Scene& S = s->s;
+    rb_status st = scene_room(S, nv, nt);  // before any array is read
+    if (st != RB_OK) return st;
     for (uint64_t i = 0; i < 3ull * nt; i++)
         if (idx[i] >= nv) return fail(RB_ERR_INVALID, "mesh triangle %llu indexes vertex %u of %u",
                                       (unsigned long long)(i / 3), idx[i], nv);
-    Scene& S = s->s;
-    if ((uint64_t)S.num_verts() + nv > 0xFFFFFFFFull || (uint64_t)S.num_tris() + nt > 0x7FFFFFFFull)
-        return fail(RB_ERR_INVALID, "scene would exceed 2^32 vertices or 2^31 triangles");
     const uint32_t base = S.num_verts();
     const uint32_t tbase = S.num_tris();
     try {

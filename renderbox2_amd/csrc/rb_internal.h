@@ -73,6 +73,19 @@ struct Scene {
     uint32_t num_verts() const { return (uint32_t)(verts.size() / 3); }
 };
 
+// Limits every path that grows a scene checks before it reads or copies
+// anything: at most 2^32 - 1 vertices and 2^31 - 1 triangles (triangle ids
+// travel as int32 in trace results, -1 = miss).
+constexpr uint64_t kMaxSceneVerts = 0xFFFFFFFFull;
+constexpr uint64_t kMaxSceneTris = 0x7FFFFFFFull;
+inline rb_status scene_room(const Scene& S, uint64_t add_verts, uint64_t add_tris) {
+    if ((uint64_t)S.num_verts() + add_verts > kMaxSceneVerts || (uint64_t)S.num_tris() + add_tris > kMaxSceneTris)
+        return fail(RB_ERR_INVALID, "scene would exceed 2^32 - 1 vertices or 2^31 - 1 triangles (%u + %llu "
+                    "vertices, %u + %llu triangles)", S.num_verts(), (unsigned long long)add_verts, S.num_tris(),
+                    (unsigned long long)add_tris);
+    return RB_OK;
+}
+
 // ------------------------------------------------------------ host BVH --
 
 struct BNode {
@@ -92,6 +105,10 @@ struct HostBvh {
     rb_build_params params{};
     rb_bvh_stats stats{};
 };
+
+// Deepest node (root = depth 0) the SBVH builder makes: a node at this
+// depth becomes a leaf whatever its size.
+constexpr uint32_t kMaxTreeDepth = 64;
 
 HostBvh* build_sbvh(const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
                     const rb_build_params& p, rb_status* st);
@@ -113,6 +130,11 @@ struct Mi355Layout {
     std::vector<uint32_t> index;  // 1 per triangle reference
 };
 rb_status export_mi355x(const HostBvh& b, Mi355Layout& out);
+// Depth of the deepest inner record of an MI355X node array (root = 0);
+// UINT32_MAX when a link leaves the array or the links form a cycle.  A
+// traversal holds at most depth + 1 stack entries (the sentinel, then one
+// postponed child per inner node on the current path).
+uint32_t layout_depth(const std::vector<uint32_t>& nodes);
 rb_status export_compact2(const HostBvh& b, std::vector<uint32_t>& nodes, std::vector<uint32_t>& woop,
                           std::vector<uint32_t>& index);
 

@@ -260,8 +260,8 @@ rb_status add_obj_to_scene(Scene& S, const std::vector<float>& verts, const std:
     const uint32_t base = S.num_verts();
     size_t ntris = 0;
     for (const ObjObject& o : objs) ntris += o.tris.size() / 3;
-    if ((uint64_t)base + verts.size() / 3 > 0xFFFFFFFFull || (uint64_t)S.num_tris() + ntris > 0xFFFFFFFFull)
-        return fail(RB_ERR_INVALID, "scene too large");
+    rb_status st = scene_room(S, verts.size() / 3, ntris);
+    if (st != RB_OK) return st;
     try {
         S.verts.insert(S.verts.end(), verts.begin(), verts.end());
         for (size_t k = 0; k < objs.size(); k++) {

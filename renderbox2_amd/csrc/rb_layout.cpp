@@ -253,4 +253,26 @@ rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
     return RB_OK;
 }
 
+uint32_t layout_depth(const std::vector<uint32_t>& nodes) {
+    const uint32_t n = (uint32_t)(nodes.size() / 16);
+    if (n == 0) return 0;
+    uint32_t deepest = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> stack;  // (record, depth)
+    stack.push_back({0u, 0u});
+    while (!stack.empty()) {
+        const auto e = stack.back();
+        stack.pop_back();
+        if (e.second > deepest) deepest = e.second;
+        if (e.second > n) return UINT32_MAX;  // a cycle: no tree is deeper than its record count
+        for (int c = 0; c < 2; c++) {
+            const int32_t link = (int32_t)nodes[(size_t)e.first * 16 + 12 + c];
+            if (link >= 0) {
+                if ((uint32_t)link >= n) return UINT32_MAX;
+                stack.push_back({(uint32_t)link, e.second + 1});
+            }
+        }
+    }
+    return deepest;
+}
+
 }  // namespace rb

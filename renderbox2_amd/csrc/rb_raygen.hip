@@ -136,6 +136,19 @@ static uint32_t grid_for(uint64_t n, uint32_t block) {
     return (uint32_t)(g ? g : 1);
 }
 
+// Trace into `hits`, then write the intersection records from them.
+// `counter`: the trace's work-counter pair when the caller brings one (else
+// a slot of the library's ring).
+static rb_status launch_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
+                             rb_hit* hits, const rb_isect_soa* out, hipStream_t s, uint32_t* counter) {
+    rb_status st = launch_trace(bvh, rays, n, flags, mask, hits, nullptr, s, dev().cfg, nullptr, counter);
+    if (st != RB_OK) return st;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, reinterpret_cast<const float4*>(rays),
+                       reinterpret_cast<const int2*>(hits), n, bvh->tri_v, *out);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? RB_OK : hip_fail(e, "fill launch");
+}
+
 }  // namespace rb
 
 using namespace rb;
@@ -184,22 +197,22 @@ rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t f
     if (!bvh->tri_v) return fail(RB_ERR_INVALID, "bvh was uploaded without its scene");
     if (n == 0) return RB_OK;
     hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
+    // Without a caller buffer the (id, t) pairs live in a stream-ordered
+    // allocation freed on the same stream after the fill: no host wait.
     rb_hit* tmp = nullptr;
     if (!hits) {
-        if ((st = dalloc((void**)&tmp, (size_t)n * sizeof(rb_hit))) != RB_OK) return st;
+        hipError_t e = hipMallocAsync((void**)&tmp, (size_t)n * sizeof(rb_hit), s);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(e == hipErrorOutOfMemory ? RB_ERR_OOM : RB_ERR_DEVICE, "trace-fill hits (%zu bytes): %s",
+                        (size_t)n * sizeof(rb_hit), hipGetErrorString(e));
+        }
         hits = tmp;
     }
-    st = launch_trace(bvh, rays, n, flags, mask, hits, nullptr, s, dev().cfg);
-    if (st == RB_OK) {
-        hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, reinterpret_cast<const float4*>(rays),
-                           reinterpret_cast<const int2*>(hits), n, bvh->tri_v, *out);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) st = hip_fail(e, "fill launch");
-    }
+    st = launch_fill(bvh, rays, n, flags, mask, hits, out, s, nullptr);
     if (tmp) {
-        const hipError_t e = hipStreamSynchronize(s);
-        if (e != hipSuccess && st == RB_OK) st = hip_fail(e, "fill sync");
-        dfree(tmp, (size_t)n * sizeof(rb_hit));
+        const hipError_t e = hipFreeAsync(tmp, s);
+        if (e != hipSuccess && st == RB_OK) st = hip_fail(e, "trace-fill hits free");
     }
     return st;
 }

@@ -23,7 +23,7 @@
 namespace rb {
 namespace {
 
-enum { kMaxDepth = 64, kMaxSpatialDepth = 48, kBins = 128 };
+enum { kMaxDepth = kMaxTreeDepth, kMaxSpatialDepth = 48, kBins = 128 };
 
 struct Ref {
     uint32_t tri;

@@ -50,6 +50,7 @@ struct TraceArgs {
     int2* __restrict__ hits;
     uint32_t* __restrict__ counts;
     uint32_t* counter;
+    uint32_t* errors;  // RB_TRACE_ERR_* flags (device word), set on a traversal-stack overflow
     uint32_t n;
     const uint32_t* n_dev;  // when set, the ray count is read from device memory
     // paired launch: a second ray set (closest hit) after the first (any hit
@@ -184,6 +185,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     const uint32_t lane = __lane_id();
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
+    // The upload refuses trees deep enough to fill the stack, so the else
+    // branch is unreachable for a library BVH; it flags the launch instead of
+    // writing past the stack.
 #define RB_PUSH(v)                                            \
     do {                                                      \
         if (sp + 1 < kMaxStack) {                             \
@@ -192,6 +196,8 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                 lstack[sp * BLOCK + tid] = (v);               \
             else                                              \
                 spill[sp - S] = (v);                          \
+        } else {                                              \
+            atomicOr(a.errors, (uint32_t)RB_TRACE_ERR_STACK); \
         }                                                     \
     } while (0)
     // Pops happen only at sp >= 0 (the sentinel at entry 0 ends the ray).  The
@@ -504,7 +510,7 @@ static uint32_t persistent_blocks(const rb_trace_config& cfg, uint64_t n) {
 
 rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
                        rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg,
-                       const uint32_t* n_dev) {
+                       const uint32_t* n_dev, uint32_t* counter) {
     if (n == 0) return RB_OK;
     if (!bvh || !rays || !hits) return fail(RB_ERR_INVALID, "NULL bvh, rays or hits");
     if (n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "too many rays in one call (%u)", n);
@@ -514,6 +520,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
                            : (any ? pick<true, false>(cfg) : pick<false, false>(cfg));
     if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
     TraceArgs a;
+    std::memset(&a, 0, sizeof(a));
     a.nodes = bvh->nodes;
     a.tris = bvh->tris;
     a.index = bvh->index;
@@ -521,8 +528,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.mask = (flags & RB_MASKED) ? mask : nullptr;
     a.hits = reinterpret_cast<int2*>(hits);
     a.counts = counts;
-    a.counter = take_counter_slot();
-    if (!a.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
+    a.errors = dev().errors;
     a.n = n;
     a.n_dev = n_dev;
     a.chunk = cfg.chunk;
@@ -533,8 +539,18 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     if (lds > dev().lds_per_block)
         return fail(RB_ERR_INVALID, "trace config needs %zu bytes of LDS per block (stack %u x %u lanes + %u node records); "
                     "the device allows %zu", lds, cfg.lds_stack, cfg.block, a.top_nodes, dev().lds_per_block);
+    SlotLease L;
+    if (counter) {
+        L.counter = counter;
+    } else {
+        L = take_counter_slot(stream);
+        if (L.st != RB_OK) return L.st;
+        if (!L.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
+    }
+    a.counter = L.counter;
     hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), (size_t)a.top_nodes * 64, stream, a);
-    hipError_t e = hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    release_counter_slot(L, stream);
     if (e != hipSuccess) return hip_fail(e, "trace launch");
     return RB_OK;
 }
@@ -564,8 +580,7 @@ rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, co
     a.hits2 = reinterpret_cast<int2*>(hits1);
     a.n_dev2 = n1_dev;
     a.any0 = (flags0 & RB_ANY) ? 1u : 0u;
-    a.counter = take_counter_slot();
-    if (!a.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
+    a.errors = dev().errors;
     a.n = cap0 + cap1;
     a.chunk = cfg.chunk;
     a.threshold = cfg.fetch_threshold;
@@ -575,8 +590,13 @@ rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, co
     if (lds > dev().lds_per_block)
         return fail(RB_ERR_INVALID, "trace config needs %zu bytes of LDS per block; the device allows %zu", lds,
                     dev().lds_per_block);
+    const SlotLease L = take_counter_slot(stream);
+    if (L.st != RB_OK) return L.st;
+    if (!L.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
+    a.counter = L.counter;
     hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, a.n)), dim3(cfg.block), (size_t)a.top_nodes * 64, stream, a);
-    hipError_t e = hipGetLastError();
+    const hipError_t e = hipGetLastError();
+    release_counter_slot(L, stream);
     if (e != hipSuccess) return hip_fail(e, "paired trace launch");
     return RB_OK;
 }
@@ -607,14 +627,18 @@ rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t st
         t.total = (uint32_t)total;
         TraceArgs a;
         std::memset(&a, 0, sizeof(a));
-        a.counter = take_counter_slot();
-        if (!a.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
+        a.errors = dev().errors;
         a.n = t.total;
         a.chunk = cfg.chunk;
         a.threshold = cfg.fetch_threshold;
         a.leaf_gate = cfg.leaf_gate;
+        const SlotLease L = take_counter_slot(stream);
+        if (L.st != RB_OK) return L.st;
+        if (!L.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
+        a.counter = L.counter;
         hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, total)), dim3(cfg.block), 0, stream, a, t);
-        hipError_t e = hipGetLastError();
+        const hipError_t e = hipGetLastError();
+        release_counter_slot(L, stream);
         if (e != hipSuccess) return hip_fail(e, "batched trace launch");
     }
     return RB_OK;
@@ -772,6 +796,22 @@ rb_status rb_selftest_rcp(unsigned long long* fast_inputs, unsigned long long* m
     if (e != hipSuccess) return hip_fail(e, "reciprocal self-test");
     *fast_inputs = h[0];
     *mismatches = h[1];
+    return RB_OK;
+}
+
+rb_status rb_trace_errors(uint32_t* flags) {
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
+    if (!flags) return fail(RB_ERR_INVALID, "NULL flags");
+    uint32_t h = 0;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(&h, dev().errors, 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && h) e = hipMemset(dev().errors, 0, 4);
+    if (e != hipSuccess) return hip_fail(e, "trace error flags");
+    *flags = h;
+    if (h & RB_TRACE_ERR_STACK)
+        return fail(RB_ERR_INVALID, "a traversal stack overflowed: hits of the launches since the last check may be wrong");
+    if (h) return fail(RB_ERR_INVALID, "trace error flags 0x%x", h);
     return RB_OK;
 }
 

@@ -19,6 +19,7 @@ static rb_bvh_dev view_of(rb_bvh_t bvh) {
     v.index = bvh->index;
     v.n_nodes = bvh->n_nodes;
     v.n_refs = bvh->n_refs;
+    v.errors = dev().errors;
     return v;
 }
 
@@ -30,6 +31,8 @@ extern "C" {
 
 rb_status rb_bvh_device_view(rb_bvh_t bvh, rb_bvh_dev* out) {
     if (!bvh || !out) return fail(RB_ERR_INVALID, "NULL argument");
+    rb_status st = ensure_device();
+    if (st != RB_OK) return st;
     *out = view_of(bvh);
     return RB_OK;
 }

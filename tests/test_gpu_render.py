@@ -312,3 +312,24 @@ def test_raycast_rejects_unknown_shade(gpu, pt_setup):
     s, b, tr = pt_setup
     with pytest.raises(gpu.RbError):
         gpu.SamplerRenderer(tr, gpu.scenes.pt_camera(16, 16), "raycast", shade=5).render()
+
+
+def test_bdpt_on_a_scene_without_emitters_is_black(gpu, orc):
+    """No emitters: no light paths, an empty light-vertex cache, no light
+    samples -- the image is black, as the oracle's (which stops the light
+    pass when there is no light) -- and nothing reads the absent light list."""
+    s = gpu.scenes.ao_room(seed=2)  # closed room + icospheres, all Lambertian, no emitter
+    b = gpu.scenes.build(s)
+    tr = gpu.CudaTracer(b, s)
+    cam = gpu.scenes.pt_camera(32, 32)
+    bd = dict(BD, connections=3)
+    img = gpu.SamplerRenderer(tr, cam, "bdpt", spp=2, seed=3, **bd).render()
+    vv, ii, tm, lt = s.geometry()
+    assert len(lt) == 0
+    ref = orc.render_bdpt(_c2(b), vv, ii, tm, s.materials(), lt, _cam_tuple(cam), (0, 0, 32, 32), 2, 3,
+                          bd["max_depth"], bd["rr_start"], bd["rr_prob"], bd["light_paths"], bd["light_max_depth"],
+                          bd["connections"])
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+    assert not img.any()
+    pt = gpu.SamplerRenderer(tr, cam, "pt", spp=2, seed=3, max_depth=8).render()
+    assert not pt.any()

@@ -272,59 +272,62 @@ def test_million_triangles_properties(gpu, orc):
         gpu.set_trace_config(**saved)
 
 
-def test_batched_trace_equals_per_batch_calls(gpu):
+def test_batched_trace_equals_the_oracle_per_batch(gpu, orc):
     """Uneven batches against several resident scenes, closest and any hit,
-    more batches than one launch takes (64), empty batches: one batched call
-    gives exactly the per-batch rb_trace results."""
+    more batches than one launch takes (64), empty batches: every batch of
+    one batched call (the kernel's batch-table path: a lane finds its ray's
+    batch and scene when it claims the ray) equals the oracle on that batch's
+    scene, (id, t) bit for bit -- any hit too, since each ray walks its tree
+    in the oracle's order and stops at the same first hit."""
     rng = np.random.default_rng(3)
-    tracers = []
+    scenes = []
     for k, n in enumerate([800, 3000, 20000]):
         v, idx = random_triangles(n, seed=100 + k, spread=4.0 + k)
-        tracers.append(gpu.CudaTracer(gpu.BVH((v, idx))))
-    batches, expect = [], []
+        b = gpu.BVH((v, idx))
+        scenes.append((gpu.CudaTracer(b), _compact2(gpu, b)))
+    batches, inputs = [], []
     for i in range(70):
         n = 0 if i % 17 == 5 else int(np.exp(rng.uniform(np.log(64), np.log(40000))))
-        tr = tracers[i % 3]
+        tr, c2 = scenes[i % 3]
         anyh = i % 4 == 3
         rays = random_rays(max(n, 1), seed=1000 + i, box=7.0)[:n]
         dr = gpu.DeviceBuffer.from_array(rays) if n else gpu.DeviceBuffer(32)
         dh = gpu.DeviceBuffer(max(n, 1) * 8)
         batches.append((tr, dr, n, dh, anyh))
-        if n:
-            ids, ts = tr.trace(rays, any_hit=anyh)
-            expect.append((ids, ts))
-        else:
-            expect.append(None)
+        inputs.append((rays, c2))
     gpu.trace_batch(batches)
-    for (tr, dr, n, dh, anyh), e in zip(batches, expect):
+    checked_any = 0
+    for (tr, dr, n, dh, anyh), (rays, c2) in zip(batches, inputs):
         if not n:
             continue
         h = dh.download(gpu.HIT_DTYPE, n)
-        if anyh:  # any hit: same hit/miss; (id, t) may differ by visit order
-            assert np.array_equal(h["id"] >= 0, e[0] >= 0)
-        else:
-            assert np.array_equal(h["id"], e[0]) and np.array_equal(h["t"].view(np.uint32), e[1].view(np.uint32))
+        oid, ot = orc.trace_compact2(*c2, rays, any_hit=anyh)
+        assert np.array_equal(h["id"], oid) and np.array_equal(h["t"].view(np.uint32), ot.view(np.uint32)), anyh
+        checked_any += anyh
+    assert checked_any >= 15
 
 
-def test_device_function_trace_equals_persistent_kernel(gpu, small, room):
-    """rb_trace_one (the header-only device tracer) walks each ray in the
-    persistent kernel's order with the same arithmetic: identical (id, t)."""
+def test_device_function_trace_equals_the_oracle(gpu, orc, small, room):
+    """rb_trace_one (the header-only device tracer, the reference's trace_ray
+    form: one ray per thread, cuda_tracer_functions.cu:42-279) against the
+    oracle: closest and any hit, (id, t) bit for bit."""
     v, idx, b, tr, c2 = small
     rays = random_rays(20000, seed=77)
     for any_hit in (False, True):
-        a = tr.trace(rays, any_hit=any_hit)
         s = tr.trace_simple(rays, any_hit=any_hit)
-        assert np.array_equal(a[0], s[0]) and np.array_equal(a[1].view(np.uint32), s[1].view(np.uint32))
-    sc, bb, rt, _ = room
+        oid, ot = orc.trace_compact2(*c2, rays, any_hit=any_hit)
+        assert np.array_equal(s[0], oid) and np.array_equal(s[1].view(np.uint32), ot.view(np.uint32)), any_hit
+    sc, bb, rt, rc2 = room
     cam = gpu.scenes.ao_camera(128, 128)
     d = gpu.DeviceBuffer(cam.get_total_pixels() * 32)
     cam.generate_rays_device(d, seed=3)
     prim = d.download(np.float32, cam.get_total_pixels() * 8).reshape(-1, 8)
-    a = rt.trace(prim)
-    s = rt.trace_simple(prim)
-    assert np.array_equal(a[0], s[0]) and np.array_equal(a[1].view(np.uint32), s[1].view(np.uint32))
+    for any_hit in (False, True):
+        s = rt.trace_simple(prim, any_hit=any_hit)
+        oid, ot = orc.trace_compact2(*rc2, prim, any_hit=any_hit)
+        assert np.array_equal(s[0], oid) and np.array_equal(s[1].view(np.uint32), ot.view(np.uint32)), any_hit
     view = rt.device_view()
-    assert view.nodes and view.tris and view.index and view.n_refs >= view.n_nodes // 2
+    assert view.nodes and view.tris and view.index and view.n_refs >= view.n_nodes // 2 and view.errors
 
 
 def test_trace_matches_the_golden_fixture(gpu):
@@ -379,3 +382,48 @@ def test_largest_batch_one_call_takes(gpu, small):
         assert np.array_equal(out["id"], exp_id[:cnt]), s
         assert np.array_equal(out["t"].view(np.uint32), exp_t[:cnt]), s
     d_hits.free()
+
+
+def test_counter_slots_reused_across_streams(gpu, orc):
+    """A launch's work-counter pair comes from a ring of 4,096; a slot is
+    reused 4,096 launches later, and when that launch is on another stream it
+    first waits for the slot's previous launch.  One long launch on stream A
+    (few waves per CU, so other streams' work runs beside it) and 4,200
+    launches on stream B: every hit of both equals the oracle's."""
+    v, idx = random_triangles(60000, seed=120, spread=8.0)
+    b = gpu.BVH((v, idx))
+    tr = gpu.CudaTracer(b)
+    c2 = _compact2(gpu, b)
+    a_rays = random_rays(1 << 22, seed=121, box=9.0)
+    b_rays = random_rays(4200 * 256, seed=122, box=9.0)
+    d_a, d_ah = gpu.DeviceBuffer.from_array(a_rays), gpu.DeviceBuffer(len(a_rays) * 8)
+    d_b, d_bh = gpu.DeviceBuffer.from_array(b_rays), gpu.DeviceBuffer(len(b_rays) * 8)
+    sa, sb = gpu.Stream(), gpu.Stream()
+    saved = gpu.get_trace_config()
+    try:
+        gpu.set_trace_config(**dict(saved, waves_per_cu=1))
+        tr.trace_device(d_a, len(a_rays), d_ah, stream=sa)
+        gpu.set_trace_config(**saved)
+        from renderbox2_amd import _lib as L
+        lib = L.lib()
+        for k in range(4200):
+            L.check(lib.rb_trace(tr.handle, d_b.ptr + k * 256 * 32, 256, L.RB_CLOSEST, None,
+                                 d_bh.ptr + k * 256 * 8, sb.handle))
+    finally:
+        gpu.set_trace_config(**saved)
+    sa.synchronize()
+    sb.synchronize()
+    ha = d_ah.download(gpu.HIT_DTYPE, len(a_rays))
+    hb = d_bh.download(gpu.HIT_DTYPE, len(b_rays))
+    for h, rays in ((ha, a_rays), (hb, b_rays)):
+        oid, ot = orc.trace_compact2(*c2, rays)
+        assert np.array_equal(h["id"], oid) and np.array_equal(h["t"].view(np.uint32), ot.view(np.uint32))
+
+
+def test_no_traversal_errors_were_flagged(gpu):
+    """Every BVH these tests uploaded passed the upload's depth check, so no
+    traversal stack overflowed in any launch."""
+    flags = C.c_uint32(0xFFFFFFFF)
+    from renderbox2_amd import _lib as L
+    assert L.lib().rb_trace_errors(C.byref(flags)) == L.RB_OK, L.last_error()
+    assert flags.value == 0

@@ -377,3 +377,116 @@ def test_product_builder_matches_the_golden_fixture(rbm):
         assert [n.size, w.size, x.size] == g["sizes"].tolist()
         for name, a in (("sha_nodes", n), ("sha_woop", w), ("sha_index", x)):
             assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest() == g[name].tobytes(), (name, threads)
+
+
+# ------------------------------------------- refusals before any read ---
+
+
+def test_scene_growth_limit_is_checked_before_the_arrays(rbm):
+    """2^31 triangles are refused from the counts alone: the arrays (3
+    vertices, 1 triangle here) are never read past, by add_mesh and by the
+    OBJ reader alike (both take the same limit check)."""
+    from renderbox2_amd import _lib as L
+    lib = L.lib()
+    s = rbm.Scene()
+    m = s.add_diffuse_bsdf((0.5, 0.5, 0.5))
+    v = np.zeros((3, 3), np.float32)
+    idx = np.array([[0, 1, 2]], np.uint32)
+    st = lib.rb_scene_add_mesh(s._h, v.ctypes.data, 3, idx.ctypes.data, 1 << 31, m)
+    assert st == L.RB_ERR_INVALID and "2^31" in L.last_error()
+    assert s.get_num_triangles() == 0
+    assert lib.rb_scene_add_mesh(s._h, v.ctypes.data, 3, idx.ctypes.data, 1, m) == L.RB_OK
+    assert s.get_num_triangles() == 1
+
+
+class _Hash64:
+    """The binary cache's checksum (rb_bvh_cache.cpp), for writing test files."""
+    M = (1 << 64) - 1
+
+    def __init__(self):
+        self.h, self.n = 0x9E3779B97F4A7C15, 0
+
+    @staticmethod
+    def _rotl(x, r):
+        return ((x << r) | (x >> (64 - r))) & _Hash64.M
+
+    def bytes(self, b):
+        b = bytes(b)
+        for i in range(0, len(b), 8):
+            w = int.from_bytes(b[i:i + 8].ljust(8, b"\0"), "little")
+            self.h ^= (self._rotl(w * 0xC2B2AE3D27D4EB4F & self.M, 31) * 0x9E3779B185EBCA87) & self.M
+            self.h = (self._rotl(self.h, 27) * 0x9E3779B185EBCA87 + 0x85EBCA77C2B2AE63) & self.M
+        self.n += len(b)
+
+    def done(self):
+        x = self.h ^ self.n
+        x ^= x >> 33
+        x = x * 0xFF51AFD7ED558CCD & self.M
+        x ^= x >> 33
+        x = x * 0xC4CEB9FE1A85EC53 & self.M
+        return x ^ (x >> 33)
+
+
+def _cache_file_with_nodes(good_bytes, nodes, refs):
+    """A cache file with the header (key, sizes, stats) of a real one and
+    the given node records / reference list, with a valid checksum."""
+    import struct
+    hdr = bytearray(good_bytes[:100])  # the packed 100-byte header
+    struct.pack_into("<II", hdr, 32, len(nodes), len(refs))
+    body = b"".join(struct.pack("<6fII", *lo, *hi, lc, link) for lo, hi, lc, link in nodes)
+    body += np.asarray(refs, np.uint32).tobytes()
+    h = _Hash64()
+    h.bytes(hdr)
+    h.bytes(body[:len(nodes) * 32])
+    h.bytes(body[len(nodes) * 32:])
+    return bytes(hdr) + body + h.done().to_bytes(8, "little")
+
+
+def test_binary_cache_refuses_files_no_build_writes(rbm, tmp_path):
+    """The checksum only detects corruption; a well-formed file must also be
+    a tree (every node but the root linked once) no deeper than the builder
+    goes (64), or the traversal stack could overflow."""
+    s = rbm.scenes.trace_scaled(2_000, seed=1)
+    b = rbm.BVH(s)
+    path = tmp_path / "good.rbvh"
+    b.save_binary(path)
+    good = path.read_bytes()
+    assert len(good) > 100 and good[:8] == b"RBVHBIN1"
+    box = ((-20.0,) * 3, (20.0,) * 3)
+    leaf = lambda first: (*box, 0x80000001, first)  # noqa: E731
+
+    def chain(depth):
+        """Inner nodes 0..depth-1, each with the next as first child and a
+        one-reference leaf as second child (preorder, as the writer emits)."""
+        nodes, refs = [], []
+
+        def emit(d):
+            me = len(nodes)
+            if d == depth:
+                nodes.append(leaf(len(refs)))
+                refs.append(0)
+                return
+            nodes.append([*box, 0, 0])
+            emit(d + 1)
+            nodes[me][3] = len(nodes)
+            nodes.append(leaf(len(refs)))
+            refs.append(1)
+
+        emit(0)
+        return [tuple(n) for n in nodes], refs
+
+    # the same file rewritten by this writer loads (the writer is right) ...
+    ok_nodes, ok_refs = chain(64)
+    p = tmp_path / "deep64.rbvh"
+    p.write_bytes(_cache_file_with_nodes(good, ok_nodes, ok_refs))
+    rbm.BVH.load_binary(p, s)
+    # ... one level deeper does not
+    deep_nodes, deep_refs = chain(65)
+    p.write_bytes(_cache_file_with_nodes(good, deep_nodes, deep_refs))
+    with pytest.raises(rbm.RbError, match="deeper than a build makes"):
+        rbm.BVH.load_binary(p, s)
+    # a node linked twice (a DAG, not a tree)
+    dag = [(*box, 0, 3), (*box, 0, 3), leaf(0), leaf(0)]
+    p.write_bytes(_cache_file_with_nodes(good, dag, [0]))
+    with pytest.raises(rbm.RbError, match="not a tree"):
+        rbm.BVH.load_binary(p, s)
