@@ -275,6 +275,18 @@ rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t f
                         const uint8_t* mask, rb_hit* hits, const rb_isect_soa* out,
                         void* stream);
 
+/* rb_trace_fill without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, serving loops, hosts that forbid hipMalloc).  The (id, t) staging
+ * and the trace's work counter live in a static device array of the library
+ * (8 MiB + 256 bytes per device, held until the process ends), so at most
+ * 1,048,576 rays per call (more: RB_ERR_INVALID).  rb_device_init must have
+ * run (RB_ERR_INVALID otherwise).  Every call on a device shares that
+ * array: calls that overlap on one device (different streams or threads)
+ * race and must be serialised by the caller.  Same results as rb_trace_fill. */
+rb_status rb_trace_fill_noalloc(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
+                                const uint8_t* mask, const rb_isect_soa* out, void* stream);
+
 /* One ray per thread through the device function rb_trace_one(): the
  * reference's kernel_trace_ray form (cuda_tracer_functions.cu:42-279)
  * rather than the persistent kernel.  Same results as rb_trace, bit for

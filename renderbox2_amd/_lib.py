@@ -121,6 +121,7 @@ _SIGS = {
     "rb_bvh_device_bytes": (C.c_int, [P, P]),
     "rb_trace": (C.c_int, [P, P, U32, U32, P, P, P]),
     "rb_trace_fill": (C.c_int, [P, P, U32, U32, P, P, P, P]),
+    "rb_trace_fill_noalloc": (C.c_int, [P, P, U32, U32, P, P, P]),
     "rb_trace_get_config": (C.c_int, [P]),
     "rb_trace_set_config": (C.c_int, [P]),
     "rb_trace_counts": (C.c_int, [P, P, U32, U32, P, P, P]),

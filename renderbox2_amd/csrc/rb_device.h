@@ -36,6 +36,10 @@ struct DeviceState {
 };
 DeviceState& dev();
 rb_status ensure_device();
+// Records the device address of the allocation-free entry point's static
+// scratch array for `device` (rb_raygen.hip); run once per device by the
+// init, with that device current.
+rb_status noalloc_setup(int device);
 // A launch's work-counter pair: from the ring, or from the pool of the render
 // graph being captured on this thread (counter == nullptr when that pool is
 // used up).  take_counter_slot makes `stream` wait for the slot's previous

@@ -35,6 +35,10 @@ static rb_status init_locked(int device) {
     if ((e = hipMemset(c, 0, cbytes)) != hipSuccess) return hip_fail(e, "hipMemset(counters)");
     g_dev.counters = (uint32_t*)c;
     g_dev.errors = g_dev.counters + 2 * kCounterSlots;
+    // the allocation-free entry point's scratch address; if the lookup
+    // fails, that entry point reports it and everything else still works
+    (void)noalloc_setup(device);
+    (void)hipGetLastError();
     // Render scratch comes from the stream-ordered pool; keep freed blocks in
     // the pool instead of returning them at every synchronisation, so a
     // steady stream of renders reuses its memory rather than remapping it.

@@ -136,6 +136,24 @@ static uint32_t grid_for(uint64_t n, uint32_t block) {
     return (uint32_t)(g ? g : 1);
 }
 
+// ------------------------------------------ allocation-free trace + fill --
+// rb_trace_fill_noalloc keeps what rb_trace_fill would allocate -- the
+// (id, t) staging between the trace and the fill, and the trace's
+// work-counter pair -- in this file-scope device array: bytes
+// [0, kScratchHead) hold the counter pair, then kNoAllocRays rb_hit
+// records.  Each device holds its own copy for the life of the process.  In
+// host code `scratch` names a host-side stand-in; kernels get its device
+// address, looked up once per device at init (noalloc_setup).
+constexpr uint32_t kNoAllocRays = 1u << 20;
+constexpr size_t kScratchHead = 256;
+static __device__ __align__(256) unsigned char scratch[kScratchHead + (size_t)kNoAllocRays * sizeof(rb_hit)];
+constexpr int kMaxDevices = 64;
+static void* g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device
+
+__global__ void k_zero_words(uint32_t* p, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0u;
+}
+
 // Trace into `hits`, then write the intersection records from them.
 // `counter`: the trace's work-counter pair when the caller brings one (else
 // a slot of the library's ring).
@@ -147,6 +165,15 @@ static rb_status launch_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint3
                        reinterpret_cast<const int2*>(hits), n, bvh->tri_v, *out);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? RB_OK : hip_fail(e, "fill launch");
+}
+
+rb_status noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return fail(RB_ERR_INVALID, "device %d: no scratch slot", device);
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_dev[device] = p;
+    return RB_OK;
 }
 
 }  // namespace rb
@@ -215,6 +242,31 @@ rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t f
         if (e != hipSuccess && st == RB_OK) st = hip_fail(e, "trace-fill hits free");
     }
     return st;
+}
+
+rb_status rb_trace_fill_noalloc(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
+                                const rb_isect_soa* out, void* stream) {
+    // Nothing here allocates: the device must already be initialised (the
+    // init allocates), and every buffer is the caller's or `scratch`.
+    const int d = dev().device;
+    if (d < 0) return fail(RB_ERR_INVALID, "rb_trace_fill_noalloc: call rb_device_init first");
+    if (d >= kMaxDevices || !g_scratch_dev[d]) return fail(RB_ERR_DEVICE, "rb_trace_fill_noalloc: no scratch on device %d", d);
+    if (!bvh || !rays || !out) return fail(RB_ERR_INVALID, "NULL argument");
+    if (!bvh->tri_v) return fail(RB_ERR_INVALID, "bvh was uploaded without its scene");
+    if (n > kNoAllocRays)
+        return fail(RB_ERR_INVALID, "rb_trace_fill_noalloc: %u rays, at most %u per call (its scratch holds that many)",
+                    n, kNoAllocRays);
+    if (n == 0) return RB_OK;
+    hipError_t e = hipSetDevice(d);
+    if (e != hipSuccess) return hip_fail(e, "hipSetDevice");
+    const hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
+    uint32_t* counter = static_cast<uint32_t*>(g_scratch_dev[d]);
+    rb_hit* hits = reinterpret_cast<rb_hit*>(static_cast<char*>(g_scratch_dev[d]) + kScratchHead);
+    // the counter pair must start at zero (a kernel, not a memset: callers
+    // may capture this into a graph)
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counter, 2u);
+    if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "scratch clear");
+    return launch_fill(bvh, rays, n, flags, mask, hits, out, s, counter);
 }
 
 }  // extern "C"
