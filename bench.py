@@ -29,6 +29,26 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def check_tree_fixture(rb, scene, bvh):
+    """The 1M-triangle tree this run built (on every CPU) must be the bytes of
+    the single-threaded oracle's tree recorded in tests/golden/trace_1m_tree.json:
+    the algorithmic bytes per ray are counted on it.  Exits on a mismatch."""
+    import hashlib
+
+    import numpy as np
+    from renderbox2_amd import _lib as L
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "trace_1m_tree.json")))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    v, idx, _, _ = scene.geometry()
+    n, w, x = bvh.export(L.RB_LAYOUT_REF_COMPACT2)
+    got = (sha(v), sha(idx), sha(n), sha(w), sha(x))
+    want = (g["sha_verts"], g["sha_idx"], g["sha_nodes"], g["sha_woop"], g["sha_index"])
+    if got != want:
+        raise SystemExit(f"bench: the 1M-triangle scene or its SBVH differs from the oracle's fixture "
+                         f"(scene {got[:2] == want[:2]}, tree {got[2:] == want[2:]}); results would not be comparable")
+    return "sha256 " + g["sha_nodes"][:16]
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -420,6 +440,8 @@ def main():
     st = bvh.stats()
     log(f"scene {scene.get_num_triangles()} tris, SBVH {st['n_inner']} inner / {st['n_tri_refs']} refs "
         f"in {build_s:.1f}s; binary cache {cache_mb:.1f} MB loads in {load_s:.3f}s (hit={hit})")
+    tree_check = check_tree_fixture(rb, scene, bvh)
+    log(f"1M tree: bytes equal the single-threaded oracle's ({tree_check})")
     tracer = rb.CudaTracer(bvh, scene)
     cam = rb.scenes.trace_camera(2048, 1024)
     n = cam.get_total_pixels()
@@ -580,7 +602,8 @@ def main():
                    "tris": scene.get_num_triangles(), "rays": n, "hit_fraction": round(hit_frac, 4),
                    "bvh_inner_nodes": st["n_inner"], "bvh_tri_refs": st["n_tri_refs"],
                    "bvh_build_s": round(build_s, 1), "bvh_cache_load_s": round(load_s, 3),
-                   "bvh_cache_mb": round(cache_mb, 1), "trace_config": cfg},
+                   "bvh_cache_mb": round(cache_mb, 1), "trace_config": cfg,
+                   "bvh_equals_oracle_tree": tree_check},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1), "gpu_visit_bytes_per_ray": round(gpu_bpr, 1),

@@ -56,7 +56,14 @@ typedef struct rb_hit {
 
 /* Intersection record, the fields of IntersectionBuffer
  * (src/core/intersection.h:63-80) that the tracer writes, plus t.  SoA,
- * device pointers, each of n entries; NULL fields are skipped. */
+ * device pointers, each of n entries; NULL fields are skipped.  For a hit:
+ * position o + t d; geometric normal unit cross(v1 - v0, v2 - v0); shading
+ * normal the triangle's vertex normals weighted by the position's
+ * barycentrics and normalised (the geometric normal for a triangle without
+ * normals); incoming direction -d; uv the barycentric weights of v0 and v1
+ * at the position (p = u v0 + v v1 + (1 - u - v) v2); material id; front =
+ * geometric normal . d < 0; epsilon 1e-3 t.  Misses write intersected = 0
+ * and t = tmax and leave the other fields as they were. */
 typedef struct rb_isect_soa {
     float* position[3];
     float* geometric_normal[3];
@@ -119,9 +126,14 @@ typedef struct rb_scene_s* rb_scene_t;
  *   RB_MAT_MICROFACET [0] Blinn exponent, [1] Fresnel: 0 none, 1 conductor,
  *                     2 dielectric; conductor: [2..4] eta rgb, [5..7] k rgb;
  *                     dielectric: [2] etai, [3] etat
+ *   RB_MAT_LAYERED    [0] number of layers n (1..4), [1..n] ids of single-layer
+ *                     materials (not emitters) already in the scene; `color` unused.
+ *                     A Material of several BSDF layers (src/core/material.h:44-52):
+ *                     the equal mixture f = (1/n) sum f_i, sampled by picking a
+ *                     layer uniformly (material.cu:80-95); see rb_scene_add_layered_material
  * Mirror and glass are specular (delta) BSDFs. */
 enum { RB_MAT_LAMBERT = 0, RB_MAT_MIRROR = 1, RB_MAT_EMITTER = 2, RB_MAT_GLASS = 3, RB_MAT_ORENNAYAR = 4,
-       RB_MAT_MICROFACET = 5 };
+       RB_MAT_MICROFACET = 5, RB_MAT_LAYERED = 6 };
 enum { RB_FRESNEL_NONE = 0, RB_FRESNEL_CONDUCTOR = 1, RB_FRESNEL_DIELECTRIC = 2 };
 typedef struct rb_material {
     uint32_t kind;
@@ -133,15 +145,47 @@ rb_status rb_scene_create(rb_scene_t* out);
 rb_status rb_scene_destroy(rb_scene_t s);
 /* Scene::add_material (src/core/scene.h:133) */
 rb_status rb_scene_add_material(rb_scene_t s, const rb_material* m, uint32_t* id);
+/* A layered material (Material with num_valid_layers = n, src/core/material.h:44-52,
+ * read from the scene file's <layers>, customfilereader.cpp:86-225) over n (1..4)
+ * single-layer, non-emitting materials of the scene.  It is the equal mixture of
+ * its layers: f = (1/n) sum f_i; a sampled direction comes from one layer picked
+ * uniformly, with pdf (1/n) sum pdf_i over the non-specular layers (a specular
+ * layer's sample keeps its own weight); light samples and connections evaluate
+ * the non-specular layers.  The reference's material_sample_f returns sum f_i
+ * with the undivided sum of pdfs (the mixture's f / pdf ratio), while its
+ * material_f returns sum f_i and material_pdf (1/n) sum pdf_i -- its light
+ * samples and BSDF samples estimate different materials; here all three
+ * describe the mixture. */
+rb_status rb_scene_add_layered_material(rb_scene_t s, const uint32_t* layer_ids, uint32_t n, uint32_t* id);
 /* Scene::add_mesh (src/core/scene.cpp:83-123): triangles index the mesh's
  * own vertices; the scene offsets them.  Emissive meshes register each
  * triangle as a DiffuseAreaLight (src/core/scene.cpp:100-118). */
 rb_status rb_scene_add_mesh(rb_scene_t s, const float* verts, uint32_t nv, const uint32_t* idx,
                             uint32_t nt, uint32_t material_id);
+/* Scene::add_mesh with a Mesh's shading data (src/core/scene.cpp:83-123;
+ * Scene h_normals / h_texcoords / h_nor_indices / h_tex_indices,
+ * src/core/scene.h:142-147; SceneBuffer, src/core/buffer.h:322-335):
+ * nn normals (xyz) and ntc texture coordinates (uv) with their own index
+ * triples per triangle (local to the mesh, as idx).  normals (texcoords)
+ * may be NULL: no triangle of the mesh has them; nidx (tidx) may be NULL
+ * with nn (ntc) == nv: one per vertex, indexed like the vertices.  A
+ * triangle whose triple is 0xFFFFFFFF x 3 has none.  Triangles with normals
+ * shade with the interpolated normal (kepler_dynamic_fetch.h:327-334),
+ * the others with the geometric one; texture coordinates are kept with the
+ * scene (no pass of the reference reads them). */
+rb_status rb_scene_add_mesh_ex(rb_scene_t s, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                               const float* normals, uint32_t nn, const uint32_t* nidx, const float* texcoords,
+                               uint32_t ntc, const uint32_t* tidx, uint32_t material_id);
 rb_status rb_scene_counts(rb_scene_t s, uint32_t* nv, uint32_t* nt, uint32_t* nlights,
                           uint32_t* nmaterials);
 rb_status rb_scene_geometry(rb_scene_t s, float* verts, uint32_t* idx, uint32_t* tri_material,
                             uint32_t* light_tris);
+/* The scene's shading data: counts of normals / texcoords and whether any
+ * triangle has them (then nidx / tidx hold 3 entries per triangle, scene
+ * indices or 0xFFFFFFFF; else those arrays are empty and not written). */
+rb_status rb_scene_shading_counts(rb_scene_t s, uint32_t* nnormals, uint32_t* ntexcoords, int* has_nidx,
+                                  int* has_tidx);
+rb_status rb_scene_shading(rb_scene_t s, float* normals, uint32_t* nidx, float* texcoords, uint32_t* tidx);
 rb_status rb_scene_materials(rb_scene_t s, rb_material* out);
 rb_status rb_scene_bounds(rb_scene_t s, float lo[3], float hi[3]);
 
@@ -166,6 +210,7 @@ typedef struct rb_gen_params {
     float emit[3];             /* emitter radiance for RB_GEN_EMITTER_QUADS */
     uint32_t subdiv_hi;        /* >subdiv: a seeded share `hi_count` of spheres uses it */
     uint32_t hi_count;
+    uint32_t smooth;           /* icospheres: 1 = with their analytic vertex normals */
 } rb_gen_params;
 rb_status rb_scene_generate(rb_scene_t s, const rb_gen_params* p);
 
@@ -390,12 +435,18 @@ rb_status rb_raygen_camera(const rb_camera* cam, uint32_t x0, uint32_t y0, uint3
  * every object of the file is added with `material_id`.  Vertices and
  * triangles keep file order and file-global indexing (Scene::add_mesh,
  * src/core/scene.cpp:83-113); polygons are fanned into triangles, negative
- * indices resolved, `#` starts a comment anywhere on a line. */
+ * indices resolved, `#` starts a comment anywhere on a line.  `vn` normals
+ * and `vt` texture coordinates are kept with their corner indices (v, v/vt,
+ * v//vn, v/vt/vn); a triangle has normals (texcoords) when all three of its
+ * corners name one (rb_scene_add_mesh_ex). */
 rb_status rb_scene_load_obj(rb_scene_t s, const char* path, uint32_t material_id);
 /* The reference's XML scene file (CustomFileReader::process and get_camera,
  * src/io/customfilereader.cpp:47-340): its mesh file (path relative to the
- * XML file), materials (single-layer lambertian, mirror, glass, orennayar,
- * Blinn microfacet; layered materials give RB_ERR_UNSUPPORTED), object -> material
+ * XML file), materials (lambertian, mirror, glass, orennayar and Blinn
+ * microfacet <bxdf> layers; a <bsdf> of 2..4 layers becomes an RB_MAT_LAYERED
+ * material whose layer materials are numbered after the emitters, so the named
+ * materials and emitters keep the reference's numbering; a <layers> count that
+ * differs from the <bxdf> count is RB_ERR_INVALID), object -> material
  * assignments, diffuse lights and, when `cam` is not NULL, the camera for a
  * width x height film (fov from <fov>, 90 degrees when absent). */
 rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint32_t height, rb_camera* cam);

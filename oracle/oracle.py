@@ -203,7 +203,7 @@ def film(samples, width, tile, spp, seed, film=FILM_DEFAULT):
     return out
 
 
-def render_ao(compact2, verts, idx, cam, tile, spp, seed, nao, radius, film=FILM_DEFAULT):
+def render_ao(compact2, verts, idx, cam, tile, spp, seed, nao, radius, film=FILM_DEFAULT, normals=None):
     """cam = (pos, at, up, hfov, width, height); tile = (x0, y0, w, h); film = (filter, xw, yw, alpha)
     -> float32[h, w, 3]"""
     nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
@@ -215,15 +215,17 @@ def render_ao(compact2, verts, idx, cam, tile, spp, seed, nao, radius, film=FILM
     L = lib()
     P = C.c_void_p
     U = C.c_uint32
-    L.orc_render_ao.argtypes = [P] * 7 + [U] * 7 + [C.c_uint64, U, C.c_float, C.c_uint32, C.c_float, C.c_float, C.c_float, P]
+    L.orc_render_ao.argtypes = [P] * 7 + [U] * 7 + [C.c_uint64, U, C.c_float, C.c_uint32, C.c_float, C.c_float, C.c_float, P,
+                                                 P, P]
     L.orc_render_ao(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(r2c), _p(c2w), cam[4], cam[5], x0, y0,
-                    w, h, spp, seed, nao, radius, *film, _p(out))
+                    w, h, spp, seed, nao, radius, *film, _p(out), *_np(normals))
     return out
 
 
 def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob,
-              film=FILM_DEFAULT):
-    """materials: list of (kind, (r, g, b)); returns float32[h, w, 3]."""
+              film=FILM_DEFAULT, normals=None):
+    """materials: list of (kind, (r, g, b)); normals: tri_normals() of the
+    scene or None; returns float32[h, w, 3]."""
     nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
     verts = np.ascontiguousarray(verts, np.float32)
     idx = np.ascontiguousarray(idx, np.uint32)
@@ -236,14 +238,16 @@ def render_pt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, 
     L = lib()
     P = C.c_void_p
     U = C.c_uint32
-    L.orc_render_pt.argtypes = [P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, C.c_uint32, C.c_float, C.c_float, C.c_float, P]
+    L.orc_render_pt.argtypes = ([P] * 7 + [U, P, U, P, P] + [U] * 6 +
+                                [C.c_uint64, U, U, C.c_float, C.c_uint32, C.c_float, C.c_float, C.c_float, P, P, P])
     L.orc_render_pt(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m4), len(materials),
                     _p(lights), len(lights), _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, max_depth, rr_start,
-                    rr_prob, *film, _p(out))
+                    rr_prob, *film, _p(out), *_np(normals))
     return out
 
 
-def render_raycast(compact2, verts, idx, tri_mat, materials, cam, tile, spp, seed, shade, film=FILM_DEFAULT):
+def render_raycast(compact2, verts, idx, tri_mat, materials, cam, tile, spp, seed, shade, film=FILM_DEFAULT,
+                   normals=None):
     """Ray-cast debug image of the tile (shade 0..4: primitive id, material,
     normals, uvs, uniform); returns float32[h, w, 3]."""
     nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
@@ -258,14 +262,14 @@ def render_raycast(compact2, verts, idx, tri_mat, materials, cam, tile, spp, see
     P = C.c_void_p
     U = C.c_uint32
     L.orc_render_raycast.argtypes = [P] * 7 + [U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, C.c_float,
-                                                                    C.c_float, P]
+                                                                    C.c_float, P, P, P]
     L.orc_render_raycast(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m12), len(materials),
-                         _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, shade, *film, _p(out))
+                         _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, shade, *film, _p(out), *_np(normals))
     return out
 
 
 def render_bdpt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp, seed, max_depth, rr_start, rr_prob,
-                light_paths, light_max_depth, connections, film=FILM_DEFAULT):
+                light_paths, light_max_depth, connections, film=FILM_DEFAULT, normals=None):
     """BDPT-LVC image of the tile; returns float32[h, w, 3]."""
     nodes, woop, tidx = (np.ascontiguousarray(a, np.uint32) for a in compact2)
     verts = np.ascontiguousarray(verts, np.float32)
@@ -280,10 +284,10 @@ def render_bdpt(compact2, verts, idx, tri_mat, materials, lights, cam, tile, spp
     P = C.c_void_p
     U = C.c_uint32
     L.orc_render_bdpt.argtypes = ([P] * 7 + [U, P, U, P, P] + [U] * 6 + [C.c_uint64, U, U, C.c_float, U, U, U] +
-                                  [C.c_uint32, C.c_float, C.c_float, C.c_float, P])
+                                  [C.c_uint32, C.c_float, C.c_float, C.c_float, P, P, P])
     L.orc_render_bdpt(_p(nodes), _p(woop), _p(tidx), _p(verts), _p(idx), _p(tri_mat), _p(m4), len(materials),
                       _p(lights), len(lights), _p(r2c), _p(c2w), cam[4], x0, y0, w, h, spp, seed, max_depth,
-                      rr_start, rr_prob, light_paths, light_max_depth, connections, *film, _p(out))
+                      rr_start, rr_prob, light_paths, light_max_depth, connections, *film, _p(out), *_np(normals))
     return out
 
 
@@ -326,3 +330,117 @@ def bytes_per_ray(counts, hit):
     32 (ray) + 8 (id,t out) + 64*inner + 48*tris + 16*leaf terminators + 4*[hit]."""
     counts = np.asarray(counts, np.float64)
     return 40.0 + 64.0 * counts[:, 0] + 48.0 * counts[:, 1] + 16.0 * counts[:, 2] + 4.0 * np.asarray(hit, np.float64)
+
+
+def tri_normals(normals, nidx, nt):
+    """Per-triangle vertex normals (nt x 9 float32) and which triangles have
+    them (nt uint8) from a scene's shading arrays (Scene.shading()); (None,
+    None) when no triangle has any."""
+    if nidx is None or len(nidx) == 0 or normals is None or len(normals) == 0:
+        return None, None
+    nidx = np.asarray(nidx, np.uint32).reshape(-1, 3)
+    assert len(nidx) == nt
+    has = (nidx[:, 0] != 0xFFFFFFFF).astype(np.uint8)
+    safe = np.where(nidx == 0xFFFFFFFF, 0, nidx)
+    tn = np.asarray(normals, np.float32).reshape(-1, 3)[safe].reshape(nt, 9)
+    tn[has == 0] = 0.0
+    return np.ascontiguousarray(tn), np.ascontiguousarray(has)
+
+
+def _np(normals):
+    """(tri_n, has_n) arguments: NULL pointers when the scene has no normals.
+    The arrays must be tri_normals()' own (contiguous float32 / uint8): the
+    pointers are only valid while the caller holds them."""
+    if normals is None or normals[0] is None:
+        return None, None
+    tn, hn = normals
+    if not (isinstance(tn, np.ndarray) and tn.dtype == np.float32 and tn.flags.c_contiguous and
+            isinstance(hn, np.ndarray) and hn.dtype == np.uint8 and hn.flags.c_contiguous):
+        raise TypeError("normals must be the (tri_n, has_n) pair tri_normals() returns")
+    return _p(tn), _p(hn)
+
+
+def fill(rays, ids, ts, verts, idx, tri_mat, normals=None):
+    """The intersection records of hits (orc_fill): dict of position, Ng, Ns,
+    -d (n x 3), uv (n x 2), epsilon, t (n), material id, front (n); misses
+    hold NaN / 0xFFFFFFFF."""
+    rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+    n = len(rays)
+    ids = np.ascontiguousarray(ids, np.int32)
+    ts = np.ascontiguousarray(ts, np.float32)
+    verts = np.ascontiguousarray(verts, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    tri_mat = np.ascontiguousarray(tri_mat, np.uint32)
+    out = np.full((n, 16), np.nan, np.float32)
+    mf = np.full((n, 2), 0xFFFFFFFF, np.uint32)
+    L = lib()
+    P = C.c_void_p
+    L.orc_fill.argtypes = [P, P, P, C.c_uint32, P, P, P, P, P, P, P]
+    tn, hn = _np(normals)
+    L.orc_fill(_p(rays), _p(ids), _p(ts), n, _p(verts), _p(idx), _p(tri_mat), tn, hn, _p(out), _p(mf))
+    return {"position": out[:, 0:3], "geometric_normal": out[:, 3:6], "shading_normal": out[:, 6:9],
+            "incoming_direction": out[:, 9:12], "uv": out[:, 12:14], "epsilon": out[:, 14], "t": out[:, 15],
+            "material_id": mf[:, 0], "front": mf[:, 1]}
+
+
+class _NodeLog(C.Structure):
+    _fields_ = [("level", C.c_int32), ("kind", C.c_int32), ("num", C.c_uint32), ("bounds", C.c_float * 6),
+                ("leaf_sah", C.c_float), ("obj_sah", C.c_float), ("sp_sah", C.c_float), ("obj_dim", C.c_int32),
+                ("sp_dim", C.c_int32), ("sp_tried", C.c_int32), ("obj_left", C.c_uint32), ("sp_pos", C.c_float),
+                ("ln", C.c_uint32), ("rn", C.c_uint32), ("ref_off", C.c_uint64)]
+
+
+def build_log(verts, idx, max_leaf=0x7FFFFFF, min_leaf=1, split_alpha=1e-5, node_cost=1.0, tri_cost=1.0):
+    """Builds the oracle SBVH with its decision log on: (list of node dicts,
+    references float32[m, 7] -- tri id bits, lo.xyz, hi.xyz)."""
+    L = lib()
+    L.orc_build_log_start.argtypes = []
+    L.orc_build_log_stop.argtypes = []
+    L.orc_build_log_nodes.restype = C.c_size_t
+    L.orc_build_log_nodes.argtypes = [C.c_void_p]
+    L.orc_build_log_refs.restype = C.c_size_t
+    L.orc_build_log_refs.argtypes = [C.c_void_p]
+    L.orc_build_log_start()
+    try:
+        b = Bvh(verts, idx, max_leaf=max_leaf, min_leaf=min_leaf, split_alpha=split_alpha, node_cost=node_cost,
+                tri_cost=tri_cost)
+        n = L.orc_build_log_nodes(None)
+        arr = (_NodeLog * n)()
+        L.orc_build_log_nodes(arr)
+        m = L.orc_build_log_refs(None)
+        refs = np.empty((m, 7), np.float32)
+        L.orc_build_log_refs(_p(refs))
+    finally:
+        L.orc_build_log_stop()
+    nodes = [{k: (list(getattr(e, k)) if k == "bounds" else getattr(e, k)) for k, _ in _NodeLog._fields_} for e in arr]
+    return b, nodes, refs
+
+
+def raygen_diffuse(verts, idx, prim, hit_id, hit_t, n_out, seed):
+    """Incoherent diffuse rays from primary hits (orc_raygen_diffuse):
+    float32[n_out, 8]."""
+    verts = np.ascontiguousarray(verts, np.float32)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    prim = np.ascontiguousarray(prim, np.float32).reshape(-1, 8)
+    hit_id = np.ascontiguousarray(hit_id, np.int32)
+    hit_t = np.ascontiguousarray(hit_t, np.float32)
+    out = np.empty((n_out, 8), np.float32)
+    L = lib()
+    P = C.c_void_p
+    L.orc_raygen_diffuse.argtypes = [P, P, P, P, P, C.c_uint32, C.c_uint32, C.c_uint64, P]
+    L.orc_raygen_diffuse(_p(verts), _p(idx), _p(prim), _p(hit_id), _p(hit_t), len(prim), n_out, seed, _p(out))
+    return out
+
+
+def raygen_camera(r2c, c2w, width, tile, spp, seed, jitter=True):
+    """Camera rays of a tile (orc_raygen_camera): float32[w*h*spp, 8]."""
+    x0, y0, w, h = tile
+    out = np.empty((w * h * spp, 8), np.float32)
+    L = lib()
+    P = C.c_void_p
+    U = C.c_uint32
+    L.orc_raygen_camera.argtypes = [P, P, U, U, U, U, U, U, C.c_uint64, C.c_int, P]
+    r2c = np.ascontiguousarray(r2c, np.float32)
+    c2w = np.ascontiguousarray(c2w, np.float32)
+    L.orc_raygen_camera(_p(r2c), _p(c2w), width, x0, y0, w, h, spp, seed, 1 if jitter else 0, _p(out))
+    return out

@@ -417,6 +417,75 @@ static int32_t create_leaf(builder_t* B, uint32_t num, const aabb* b)
     return id;
 }
 
+/* Decision log of the builder (checker support): when started, every
+ * build_node call on this thread records its level, bounds, references
+ * (after degenerate removal) and what it decided, so a test can recompute
+ * every candidate independently and check that the decision is the
+ * minimum the reference's rules select. */
+static __thread int g_log_on;
+static __thread orc_node_log* g_log_nodes;
+static __thread size_t g_log_n, g_log_cap;
+static __thread float* g_log_refs;   /* 7 floats per reference: tri id bits, lo.xyz, hi.xyz */
+static __thread size_t g_log_nrefs, g_log_refcap;
+
+void orc_build_log_start(void)
+{
+    g_log_on = 1;
+    g_log_n = g_log_nrefs = 0;
+}
+
+void orc_build_log_stop(void)
+{
+    g_log_on = 0;
+    free(g_log_nodes); free(g_log_refs);
+    g_log_nodes = NULL; g_log_refs = NULL;
+    g_log_n = g_log_cap = g_log_nrefs = g_log_refcap = 0;
+}
+
+size_t orc_build_log_nodes(orc_node_log* out)
+{
+    if (out && g_log_n) memcpy(out, g_log_nodes, g_log_n * sizeof(orc_node_log));
+    return g_log_n;
+}
+
+size_t orc_build_log_refs(float* out7)
+{
+    if (out7 && g_log_nrefs) memcpy(out7, g_log_refs, g_log_nrefs * 7 * sizeof(float));
+    return g_log_nrefs;
+}
+
+/* Appends a node entry with its references; returns its index or -1. */
+static long log_node(const builder_t* B, uint32_t num, const aabb* bounds, int level)
+{
+    if (g_log_n == g_log_cap) {
+        size_t nc = g_log_cap ? 2 * g_log_cap : 1024;
+        orc_node_log* p = (orc_node_log*)realloc(g_log_nodes, nc * sizeof(orc_node_log));
+        if (!p) return -1;
+        g_log_nodes = p; g_log_cap = nc;
+    }
+    if (g_log_nrefs + num > g_log_refcap) {
+        size_t nc = g_log_refcap ? 2 * g_log_refcap : 4096;
+        while (nc < g_log_nrefs + num) nc *= 2;
+        float* p = (float*)realloc(g_log_refs, nc * 7 * sizeof(float));
+        if (!p) return -1;
+        g_log_refs = p; g_log_refcap = nc;
+    }
+    orc_node_log* e = &g_log_nodes[g_log_n];
+    memset(e, 0, sizeof(*e));
+    e->level = level;
+    e->num = num;
+    e->bounds[0] = bounds->lo.x; e->bounds[1] = bounds->lo.y; e->bounds[2] = bounds->lo.z;
+    e->bounds[3] = bounds->hi.x; e->bounds[4] = bounds->hi.y; e->bounds[5] = bounds->hi.z;
+    e->ref_off = g_log_nrefs;
+    for (size_t i = B->n - num; i < B->n; i++) {
+        float* r = g_log_refs + 7 * g_log_nrefs++;
+        memcpy(&r[0], &B->st[i].tri, 4);
+        r[1] = B->st[i].b.lo.x; r[2] = B->st[i].b.lo.y; r[3] = B->st[i].b.lo.z;
+        r[4] = B->st[i].b.hi.x; r[5] = B->st[i].b.hi.y; r[6] = B->st[i].b.hi.z;
+    }
+    return (long)g_log_n++;
+}
+
 /* build_node, splitbvhbuilder.cpp:290-357 */
 static int32_t build_node(builder_t* B, uint32_t num, aabb bounds, int level)
 {
@@ -438,6 +507,10 @@ static int32_t build_node(builder_t* B, uint32_t num, aabb bounds, int level)
         }
         num = (uint32_t)((long)B->n - firstRef);
     }
+    const long lg = g_log_on ? log_node(B, num, &bounds, level) : -1;
+    if (lg >= 0) {
+        g_log_nodes[lg].leaf_sah = g_log_nodes[lg].obj_sah = g_log_nodes[lg].sp_sah = FLT_MAX;
+    }
     if (num <= B->min_leaf || level >= MAX_DEPTH) return create_leaf(B, num, &bounds);
 
     float area = aabb_area(&bounds);
@@ -446,10 +519,19 @@ static int32_t build_node(builder_t* B, uint32_t num, aabb bounds, int level)
     obj_split obj = find_object_split(B, num, nodeSAH);
 
     sp_split sp; sp.sah = FLT_MAX; sp.dim = 0; sp.pos = 0.0f;
+    int sp_tried = 0;
     if (level < MAX_SPATIAL_DEPTH) {
         aabb ov = obj.lb;
         aabb_intersect(&ov, &obj.rb);
-        if (aabb_area(&ov) >= B->min_overlap) sp = find_spatial_split(B, &bounds, num, nodeSAH);
+        if (aabb_area(&ov) >= B->min_overlap) {
+            sp = find_spatial_split(B, &bounds, num, nodeSAH);
+            sp_tried = 1;
+        }
+    }
+    if (lg >= 0) {
+        orc_node_log* e = &g_log_nodes[lg];
+        e->leaf_sah = leafSAH; e->obj_sah = obj.sah; e->obj_dim = obj.dim; e->obj_left = obj.num_left;
+        e->sp_sah = sp.sah; e->sp_dim = sp.dim; e->sp_pos = sp.pos; e->sp_tried = sp_tried;
     }
 
     float minSAH = leafSAH;
@@ -458,13 +540,23 @@ static int32_t build_node(builder_t* B, uint32_t num, aabb bounds, int level)
 
     uint32_t ln = 0, rn = 0;
     aabb lb = aabb_empty(), rb = aabb_empty();
-    if (minSAH == sp.sah) perform_spatial_split(B, &ln, &lb, &rn, &rb, num, &sp);
+    int kind = 1;
+    if (minSAH == sp.sah) {
+        perform_spatial_split(B, &ln, &lb, &rn, &rb, num, &sp);
+        kind = 2;
+    }
     if (B->oom) return -1;
     if (!ln || !rn) {
         /* perform_object_split, :409-418 */
         sort_range(B, B->n - num, B->n, obj.dim);
         ln = obj.num_left; lb = obj.lb;
         rn = num - obj.num_left; rb = obj.rb;
+        kind = 1;
+    }
+    if (lg >= 0) {
+        g_log_nodes[lg].kind = kind;
+        g_log_nodes[lg].ln = ln;
+        g_log_nodes[lg].rn = rn;
     }
     B->bvh->duplicates += (uint64_t)(ln + rn - num);
     int32_t right = build_node(B, rn, rb, level + 1);
@@ -1113,6 +1205,10 @@ typedef struct {
     const float* verts; const uint32_t* idx;  /* scene */
     const uint32_t* tri_mat; const orc_material* mats;
     const uint32_t* lights; uint32_t nl;
+    /* per-triangle shading normals (n0, n1, n2: 9 floats) and which
+     * triangles have them; NULL: every triangle shades with its geometric
+     * normal */
+    const float* tri_n; const uint8_t* has_n;
 } orc_scene;
 
 static void tri_verts(const orc_scene* S, uint32_t t, f3* a, f3* b, f3* c)
@@ -1124,6 +1220,61 @@ static void tri_verts(const orc_scene* S, uint32_t t, f3* a, f3* b, f3* c)
 }
 
 static f3 tri_norm(f3 a, f3 b, f3 c) { return normalize3(cross3(sub3(b, a), sub3(c, a))); }
+
+/* Barycentric weights u (of a) and v (of b; c takes 1 - u - v) of a point p
+ * of the triangle's plane, from the normal equations of
+ * p - c = u (a - c) + v (b - c): the u, v convention of the Woop transform
+ * (M = [v0 - v2 | v1 - v2 | n | v2], cudabvh.cpp:303-322). */
+static void bary_uv(f3 p, f3 a, f3 b, f3 c, float* u, float* v)
+{
+    f3 e0 = sub3(a, c), e1 = sub3(b, c), q = sub3(p, c);
+    float d00 = dot3(e0, e0), d01 = dot3(e0, e1), d11 = dot3(e1, e1);
+    float d20 = dot3(q, e0), d21 = dot3(q, e1);
+    float den = d00 * d11 - d01 * d01;
+    *u = den != 0.0f ? (d11 * d20 - d01 * d21) / den : 0.0f;
+    *v = den != 0.0f ? (d00 * d21 - d01 * d20) / den : 0.0f;
+}
+
+/* Shading normal of triangle t at p (the hit's IntersectionBuffer
+ * m_shading_normal, kepler_dynamic_fetch.h:327-334): the vertex normals
+ * weighted by p's barycentrics, u n0 + v n1 + (1 - u - v) n2, normalised
+ * (the reference weights n0, n0, n1 and does not normalise -- a slip this
+ * restatement does not keep).  *ns = ng and 0 returned for a triangle
+ * without normals or where the weighted sum vanishes; 1 when *ns is the
+ * interpolated normal.  Not oriented: it points where the mesh's normals do. */
+static int interp_normal(const orc_scene* S, uint32_t t, f3 p, f3 a, f3 b, f3 c, f3 ng, f3* ns)
+{
+    *ns = ng;
+    if (!S->tri_n || !S->has_n[t]) return 0;
+    const float* n = S->tri_n + 9 * (size_t)t;
+    float u, v;
+    bary_uv(p, a, b, c, &u, &v);
+    float w = 1.0f - u - v;
+    f3 s = mk3(n[0] * u + n[3] * v + n[6] * w, n[1] * u + n[4] * v + n[7] * w, n[2] * u + n[5] * v + n[8] * w);
+    float l2 = dot3(s, s);
+    if (!(l2 > 0.0f)) return 0;
+    *ns = mul3s(s, 1.0f / sqrtf(l2));
+    return 1;
+}
+
+/* The shading normal the passes shade with: interp_normal turned to the
+ * geometric normal's side (so "facing the ray" means the same for both). */
+static int shade_normal(const orc_scene* S, uint32_t t, f3 p, f3 a, f3 b, f3 c, f3 ng, f3* ns)
+{
+    int sn = interp_normal(S, t, p, a, b, c, ng, ns);
+    if (sn && dot3(*ns, ng) < 0.0f) *ns = mk3(-ns->x, -ns->y, -ns->z);
+    return sn;
+}
+
+/* Importance transport's shading-normal factor for a light sub-path vertex
+ * (Veach 1997, sec. 5.3): |wo.ns| |wi.ng| / (|wo.ng| |wi.ns|), wo towards the
+ * light, wi the continuation; 0 where the denominator is. */
+static float adjoint_factor(f3 wo, f3 wi, f3 ng, f3 ns)
+{
+    float num = fabsf(dot3(wo, ns)) * fabsf(dot3(wi, ng));
+    float den = fabsf(dot3(wo, ng)) * fabsf(dot3(wi, ns));
+    return den != 0.0f ? num / den : 0.0f;
+}
 
 static void trace_ray8(const orc_scene* S, const float* r, int any, int32_t* id, float* t)
 {
@@ -1222,10 +1373,11 @@ void orc_film(const float* Ls, uint32_t width, uint32_t x0, uint32_t y0, uint32_
 void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
                    const uint32_t* idx, const float* r2c, const float* c2w, uint32_t width, uint32_t height,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, uint32_t nao,
-                   float radius, uint32_t filter, float fxw, float fyw, float falpha, float* out_rgb)
+                   float radius, uint32_t filter, float fxw, float fyw, float falpha, float* out_rgb,
+                   const float* tri_n, const uint8_t* has_n)
 {
     (void)height;
-    orc_scene S = {nodes, woop, tidx, verts, idx, NULL, NULL, NULL, 0};
+    orc_scene S = {nodes, woop, tidx, verts, idx, NULL, NULL, NULL, 0, tri_n, has_n};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
     float* Ls = (float*)malloc((size_t)n * 12);
@@ -1244,8 +1396,11 @@ void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                 f3 p = add3(mk3(R[0], R[1], R[2]), mul3s(d, t));
                 f3 a, b, cc;
                 tri_verts(&S, (uint32_t)id, &a, &b, &cc);
-                f3 nn = tri_norm(a, b, cc);
-                if (dot3(nn, d) > 0.0f) nn = mk3(-nn.x, -nn.y, -nn.z);
+                /* kernel_ambient_occlusion samples about the shading normal
+                 * (integrator_ao.cu:55), here turned to face the ray */
+                f3 ng = tri_norm(a, b, cc), nn;
+                shade_normal(&S, (uint32_t)id, p, a, b, cc, ng, &nn);
+                if (dot3(ng, d) > 0.0f) nn = mk3(-nn.x, -nn.y, -nn.z);
                 uint32_t open = 0;
                 uint32_t gid = sample_gid(width, x0, y0, w, spp, i);
                 for (uint32_t k = 0; k < nao; k++) {
@@ -1546,6 +1701,130 @@ static f3 glass_sample3(const orc_material* m, f3 wo, f3 ng, float u, int radian
     return w;
 }
 
+/* One sampled direction of a single-layer material (material_sample_f's
+ * lobes, material.cu:98-135): a non-delta BSDF gives f and pdf at wi; glass
+ * gives f |cos| / pdf in *f with *weight set; a mirror gives color / |cos|
+ * with pdf 1 (sample_mirror, reflection.cu:76-92). */
+static void layer_sample3(const orc_material* m, f3 wo, f3 ng, f3 ns, f3 nf, float u1, float u2, int radiance, f3* wi,
+                          f3* f, float* pdf, int* spec, int* weight)
+{
+    *weight = 0;
+    if (!delta_mat(m)) {
+        bsdf_sample3(m, wo, ng, nf, u1, u2, wi, f, pdf);
+        *spec = 0;
+        return;
+    }
+    *spec = 1;
+    *pdf = 1.0f;
+    if (m->kind == 3u) {
+        *f = glass_sample3(m, wo, ns, u1, radiance, wi);
+        *weight = 1;
+        return;
+    }
+    float c = 2.0f * dot3(nf, wo);
+    *wi = sub3(mul3s(nf, c), wo);
+    float ac = fabsf(dot3(*wi, nf));
+    *f = mk3(m->color[0] / ac, m->color[1] / ac, m->color[2] / ac);
+}
+
+/* Layered materials (kind 6: Material's num_valid_layers layers,
+ * src/core/material.h:44-52; layer choice material.cu:80-95).  The material
+ * is the equal mixture of its n single-layer materials, f = (1/n) sum f_i.
+ * A sampled direction comes from one layer picked uniformly; a specular
+ * layer's sample keeps its own throughput factor, a non-specular one has
+ * the mixture's f and pdf (1/n) sum pdf_i over the non-specular layers.
+ * Light samples and connections evaluate the non-specular layers; the light
+ * sample's MIS weight uses the non-specular technique's pdf
+ * (1/n_ns) sum pdf_i.  The reference's material_f sums the layers while its
+ * sampled continuation divides by the pick (material.cu:137-170, 264-326),
+ * so its two estimators disagree for n > 1; this is the mean, which its
+ * continuation estimates. */
+static uint32_t mat_layers3(const orc_material* m) { return m->kind == 6u ? (uint32_t)m->param[0] : 1u; }
+static const orc_material* mat_layer3(const orc_scene* S, const orc_material* m, uint32_t k)
+{
+    return &S->mats[(uint32_t)m->param[1 + k]];
+}
+static int mat_delta3(const orc_scene* S, const orc_material* m)
+{
+    if (m->kind != 6u) return delta_mat(m);
+    for (uint32_t k = 0; k < mat_layers3(m); k++)
+        if (!delta_mat(mat_layer3(S, m, k))) return 0;
+    return 1;
+}
+static f3 mat_f3(const orc_scene* S, const orc_material* m, f3 wo, f3 wi, f3 ng, f3 nf)
+{
+    if (m->kind != 6u) return bsdf_f3(m, wo, wi, ng, nf);
+    uint32_t n = mat_layers3(m);
+    f3 s = mk3(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = 0; k < n; k++) s = add3(s, bsdf_f3(mat_layer3(S, m, k), wo, wi, ng, nf));
+    float fn = (float)n;
+    return mk3(s.x / fn, s.y / fn, s.z / fn);
+}
+static float mat_pdf_sum3(const orc_scene* S, const orc_material* m, f3 wo, f3 wi, f3 nf, uint32_t* count)
+{
+    float s = 0.0f;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < mat_layers3(m); k++) {
+        const orc_material* L = mat_layer3(S, m, k);
+        if (delta_mat(L)) continue;
+        s += bsdf_pdf3(L, wo, wi, nf);
+        c++;
+    }
+    *count = c;
+    return s;
+}
+static float mat_pdf_ns3(const orc_scene* S, const orc_material* m, f3 wo, f3 wi, f3 nf)
+{
+    if (m->kind != 6u) return bsdf_pdf3(m, wo, wi, nf);
+    uint32_t c;
+    float s = mat_pdf_sum3(S, m, wo, wi, nf, &c);
+    return c ? s / (float)c : 0.0f;
+}
+/* a direction from the non-specular layers (one picked uniformly) */
+static void mat_sample_ns3(const orc_scene* S, const orc_material* m, f3 wo, f3 ng, f3 nf, float upick, float u1,
+                           float u2, f3* wi, f3* f, float* pdf)
+{
+    if (m->kind != 6u) {
+        bsdf_sample3(m, wo, ng, nf, u1, u2, wi, f, pdf);
+        return;
+    }
+    uint32_t n = mat_layers3(m), c = 0;
+    for (uint32_t k = 0; k < n; k++) c += delta_mat(mat_layer3(S, m, k)) ? 0u : 1u;
+    *f = mk3(0.0f, 0.0f, 0.0f);
+    *pdf = 0.0f;
+    if (!c) return;
+    uint32_t j = (uint32_t)(upick * (float)c);
+    if (j > c - 1) j = c - 1;
+    const orc_material* L = m;
+    for (uint32_t k = 0, i = 0; k < n; k++) {
+        if (delta_mat(mat_layer3(S, m, k))) continue;
+        if (i++ == j) { L = mat_layer3(S, m, k); break; }
+    }
+    f3 fl;
+    float pl;
+    bsdf_sample3(L, wo, ng, nf, u1, u2, wi, &fl, &pl);
+    if (!(pl > 0.0f)) return;
+    *f = mat_f3(S, m, wo, *wi, ng, nf);
+    *pdf = mat_pdf_ns3(S, m, wo, *wi, nf);
+}
+/* the next direction from all layers (one picked uniformly) */
+static void mat_sample3(const orc_scene* S, const orc_material* m, f3 wo, f3 ng, f3 ns, f3 nf, float upick, float u1,
+                        float u2, int radiance, f3* wi, f3* f, float* pdf, int* spec, int* weight)
+{
+    if (m->kind != 6u) {
+        layer_sample3(m, wo, ng, ns, nf, u1, u2, radiance, wi, f, pdf, spec, weight);
+        return;
+    }
+    uint32_t n = mat_layers3(m);
+    uint32_t k = (uint32_t)(upick * (float)n);
+    if (k > n - 1) k = n - 1;
+    layer_sample3(mat_layer3(S, m, k), wo, ng, ns, nf, u1, u2, radiance, wi, f, pdf, spec, weight);
+    if (*spec || !(*pdf > 0.0f)) return;
+    *f = mat_f3(S, m, wo, *wi, ng, nf);
+    uint32_t c;
+    *pdf = mat_pdf_sum3(S, m, wo, *wi, nf, &c) / (float)n;
+}
+
 /* BSDF value of a material for the tests: wo, wi, ng world; nf = ng facing wo. */
 void orc_bsdf_eval(const float* m12, const float* wo, const float* wi, const float* ng, float* out)
 {
@@ -1587,6 +1866,8 @@ typedef struct {
     const float *pos, *ng, *wo, *alpha;      /* 3 floats per slot j * depth + d */
     const uint32_t *mid, *mask;              /* material, specular bits of y_1..y_(d+1) */
     orc_rng crng;
+    const float* ns;                         /* shading normal per slot (NULL: the scene has none) */
+    const uint8_t* snf;                      /* 1: ns is an interpolated normal */
 } orc_lvc;
 
 /* Techniques that build the path z_1..z_n (z_0 camera, z_n on the light,
@@ -1639,7 +1920,7 @@ static f3 uniform_dir(f3 n, float u1, float u2)
  * T) to the cache (integrator_bdpt_lvc_sk.cu:343-386); each unoccluded one
  * is added to L in connection order. */
 static f3 lvc_connect(const orc_scene* S, const orc_lvc* B, uint32_t gid, uint32_t bounce, uint32_t max_depth, f3 p,
-                      float eps, f3 wo, f3 ng, f3 nf, const orc_material* m, f3 T, uint32_t cm, f3 L)
+                      float eps, f3 wo, f3 ng, f3 ns, f3 nf, const orc_material* m, f3 T, uint32_t cm, f3 L)
 {
     for (uint32_t i = 0; i < B->nconn; i++) {
         uint32_t rc[4];
@@ -1662,11 +1943,16 @@ static f3 lvc_connect(const orc_scene* S, const orc_lvc* B, uint32_t gid, uint32
         float dist = sqrtf(d2);
         f3 dir = normalize3(seg);
         f3 ndir = mk3(-dir.x, -dir.y, -dir.z);
-        f3 fy = bsdf_f3(m, wo, dir, ng, nf);
-        f3 nfx = dot3(xn, xw) >= 0.0f ? xn : mk3(-xn.x, -xn.y, -xn.z);
-        f3 fx = bsdf_f3(mx, xw, ndir, xn, nfx);
+        f3 fy = mat_f3(S, m, wo, dir, ng, nf);
+        /* the cache vertex's shading normal; importance transport there
+         * (light towards camera) takes the adjoint factor */
+        int xsn = B->ns && B->snf[sl];
+        f3 xs = xsn ? mk3(B->ns[3 * sl], B->ns[3 * sl + 1], B->ns[3 * sl + 2]) : xn;
+        f3 nfx = dot3(xn, xw) >= 0.0f ? xs : mk3(-xs.x, -xs.y, -xs.z);
+        f3 fx = mat_f3(S, mx, xw, ndir, xn, nfx);
+        if (xsn) fx = mul3s(fx, adjoint_factor(xw, ndir, xn, xs));
         if (black3(fy) || black3(fx)) continue;
-        float G = fabsf(dot3(ng, dir)) * fabsf(dot3(xn, ndir)) / d2;
+        float G = fabsf(dot3(ns, dir)) * fabsf(dot3(xs, ndir)) / d2;
         /* full path: camera vertices of bounces 0..bounce, then the light
          * vertices of depths k..0 */
         uint32_t M = cm;
@@ -1705,9 +1991,10 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
         f3 v0, v1, v2;
         tri_verts(S, (uint32_t)id, &v0, &v1, &v2);
         const orc_material* m = &S->mats[S->tri_mat[id]];
-        f3 ng = tri_norm(v0, v1, v2);
+        f3 ng = tri_norm(v0, v1, v2), ns;
+        shade_normal(S, (uint32_t)id, p, v0, v1, v2, ng, &ns);
         f3 wo = mk3(-d.x, -d.y, -d.z);
-        f3 nf = dot3(ng, wo) >= 0.0f ? ng : mk3(-ng.x, -ng.y, -ng.z);
+        f3 nf = dot3(ng, wo) >= 0.0f ? ns : mk3(-ns.x, -ns.y, -ns.z);
         if (m->kind == 2u) {
             if (bounce == 0 || spec) {
                 f3 le = dot3(ng, wo) > 0.0f ? mk3(m->color[0], m->color[1], m->color[2]) : mk3(0.0f, 0.0f, 0.0f);
@@ -1716,11 +2003,11 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
             }
             break;
         }
-        uint32_t r0[4], r1[4];
+        uint32_t r0[4], r1[4], r2[4] = {0u, 0u, 0u, 0u};
         rng_draw(rng, gid, bounce, 0, 0, r0);
         rng_draw(rng, gid, bounce, 1, 0, r1);
-        int delta = delta_mat(m);
-        if (B && delta) cm |= 1u << bounce;
+        if (m->kind == 6u) rng_draw(rng, gid, bounce, 2, 0, r2); /* layer picks */
+        int delta = mat_delta3(S, m);
         float wdir = B ? bdpt_weight(cm, bounce + 2, max_depth, B->nconn ? B->depth : 0u) : 1.0f;
         f3 Ld = mk3(0.0f, 0.0f, 0.0f);
         if (S->nl > 0) {
@@ -1742,11 +2029,11 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
             float lpdf = light_pdf_mt(p, wi, l0, l1, l2, ln, area);
             f3 Li = dot3(ln, mk3(-wi.x, -wi.y, -wi.z)) > 0.0f ? emit : mk3(0.0f, 0.0f, 0.0f);
             if (!delta && !black3(Li) && lpdf > 0.0f) {
-                f3 f = bsdf_f3(m, wo, wi, ng, nf);
+                f3 f = mat_f3(S, m, wo, wi, ng, nf);
                 if (!black3(f)) {
-                    float bpdf = bsdf_pdf3(m, wo, wi, nf);
+                    float bpdf = mat_pdf_ns3(S, m, wo, wi, nf);
                     float w = lpdf / (lpdf + bpdf);
-                    float s = fabsf(dot3(wi, ng)) * w;
+                    float s = fabsf(dot3(wi, ns)) * w;
                     f3 ca = mk3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
                     f3 seg = sub3(pl, p);
                     float dist = sqrtf(dot3(seg, seg));
@@ -1760,13 +2047,13 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
             if (!delta) {
                 f3 wi2, f2;
                 float pdf2;
-                bsdf_sample3(m, wo, ng, nf, u01(r0[3]), u01(r1[0]), &wi2, &f2, &pdf2);
+                mat_sample_ns3(S, m, wo, ng, nf, u01(r2[1]), u01(r0[3]), u01(r1[0]), &wi2, &f2, &pdf2);
                 if (!black3(f2) && pdf2 > 0.0f) {
                     float lpdf2 = light_pdf_mt(p, wi2, l0, l1, l2, ln, area);
                     if (lpdf2 > 0.0f) {
                         float w2 = pdf2 / (pdf2 + lpdf2);
                         f3 Li2 = dot3(ln, mk3(-wi2.x, -wi2.y, -wi2.z)) > 0.0f ? emit : mk3(0.0f, 0.0f, 0.0f);
-                        float s2 = fabsf(dot3(wi2, ng)) * w2;
+                        float s2 = fabsf(dot3(wi2, ns)) * w2;
                         f3 cb = mk3(f2.x * Li2.x * s2 / pdf2, f2.y * Li2.y * s2 / pdf2, f2.z * Li2.z * s2 / pdf2);
                         float mr[8] = {p.x, p.y, p.z, eps, wi2.x, wi2.y, wi2.z, FLT_MAX};
                         int32_t mid2; float mt;
@@ -1779,30 +2066,17 @@ static f3 pt_path(const orc_scene* S, const float* R0, uint32_t gid, orc_rng rng
         Ld = mul3s(Ld, (float)S->nl);
         if (B) Ld = mul3s(Ld, wdir);
         L = mk3(L.x + T.x * Ld.x, L.y + T.y * Ld.y, L.z + T.z * Ld.z);
-        if (B && !delta && S->nl > 0) L = lvc_connect(S, B, gid, bounce, max_depth, p, eps, wo, ng, nf, m, T, cm, L);
+        if (B && !delta && S->nl > 0) L = lvc_connect(S, B, gid, bounce, max_depth, p, eps, wo, ng, ns, nf, m, T, cm, L);
         /* next bounce */
         f3 wi3, f3v;
         float pdf3;
-        int spec3;
-        int weight3 = 0; /* f3v already holds f |cos| / pdf */
-        if (!delta) {
-            bsdf_sample3(m, wo, ng, nf, u01(r1[1]), u01(r1[2]), &wi3, &f3v, &pdf3);
-            spec3 = 0;
-        } else if (m->kind == 3u) {
-            f3v = glass_sample3(m, wo, ng, u01(r1[1]), 1, &wi3);
-            pdf3 = 1.0f;
-            spec3 = 1;
-            weight3 = 1;
-        } else {
-            float c = 2.0f * dot3(nf, wo);
-            wi3 = sub3(mul3s(nf, c), wo);
-            float ac = fabsf(dot3(wi3, nf));
-            f3v = mk3(m->color[0] / ac, m->color[1] / ac, m->color[2] / ac);
-            pdf3 = 1.0f;
-            spec3 = 1;
-        }
+        int spec3, weight3; /* weight3: f3v already holds f |cos| / pdf */
+        mat_sample3(S, m, wo, ng, ns, nf, u01(r2[0]), u01(r1[1]), u01(r1[2]), 1, &wi3, &f3v, &pdf3, &spec3, &weight3);
+        /* the vertex is specular for the paths continuing through a
+         * specular lobe */
+        if (B && spec3) cm |= 1u << bounce;
         if (black3(f3v) || !(pdf3 > 0.0f)) break;
-        float s3 = weight3 ? 1.0f : fabsf(dot3(wi3, ng));
+        float s3 = weight3 ? 1.0f : fabsf(dot3(wi3, ns));
         f3 Tn = mk3(T.x * (f3v.x * s3 / pdf3), T.y * (f3v.y * s3 / pdf3), T.z * (f3v.z * s3 / pdf3));
         int alive = 1;
         if (bounce > rr_start) {
@@ -1829,10 +2103,10 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                    const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                    uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t filter, float fxw, float fyw,
-                   float falpha, float* out_rgb)
+                   float falpha, float* out_rgb, const float* tri_n, const uint8_t* has_n)
 {
     orc_material* mats = mats_from12(mats4, nmats);
-    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
+    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl, tri_n, has_n};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
     float* Ls = (float*)malloc((size_t)n * 12);
@@ -1860,7 +2134,7 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
  * arriving there, then the path continues by a BSDF sample with roulette
  * (continue probability min(1, luminance)); emitters end it. */
 static void light_path(const orc_scene* S, orc_rng rng, uint32_t j, uint32_t D, uint32_t* count, float* pos, float* ngo,
-                       float* woo, float* alpha, uint32_t* mido, uint32_t* masko)
+                       float* woo, float* alpha, uint32_t* mido, uint32_t* masko, float* nso, uint8_t* snfo)
 {
     count[j] = 0;
     if (S->nl == 0) return;
@@ -1897,40 +2171,34 @@ static void light_path(const orc_scene* S, orc_rng rng, uint32_t j, uint32_t D, 
         uint32_t mid = S->tri_mat[id];
         const orc_material* m = &S->mats[mid];
         if (m->kind == 2u) return;
-        f3 ng = tri_norm(v0, v1, v2);
+        f3 ng = tri_norm(v0, v1, v2), ns;
+        int sn = shade_normal(S, (uint32_t)id, p, v0, v1, v2, ng, &ns);
         f3 wo = mk3(-dv.x, -dv.y, -dv.z);
-        int delta = delta_mat(m);
-        if (delta) mask |= 1u << d;
         size_t sl = (size_t)j * D + d;
         pos[3 * sl] = p.x; pos[3 * sl + 1] = p.y; pos[3 * sl + 2] = p.z;
         ngo[3 * sl] = ng.x; ngo[3 * sl + 1] = ng.y; ngo[3 * sl + 2] = ng.z;
         woo[3 * sl] = wo.x; woo[3 * sl + 1] = wo.y; woo[3 * sl + 2] = wo.z;
         alpha[3 * sl] = a.x; alpha[3 * sl + 1] = a.y; alpha[3 * sl + 2] = a.z;
         mido[sl] = mid;
-        masko[sl] = mask;
+        masko[sl] = mask | (mat_delta3(S, m) ? 1u << d : 0u);
+        if (nso) {
+            nso[3 * sl] = ns.x; nso[3 * sl + 1] = ns.y; nso[3 * sl + 2] = ns.z;
+            snfo[sl] = (uint8_t)sn;
+        }
         count[j] = d + 1;
         if (d + 1 >= D) return;
         uint32_t rr[4];
         rng_draw(rng, j, d + 1, 0, 0, rr);
-        f3 nf = dot3(ng, wo) >= 0.0f ? ng : mk3(-ng.x, -ng.y, -ng.z);
+        f3 nf = dot3(ng, wo) >= 0.0f ? ns : mk3(-ns.x, -ns.y, -ns.z);
         f3 wi, f;
         float bp;
-        int weight = 0; /* f already holds f |cos| / pdf */
-        if (m->kind == 1u) {
-            wi = sub3(mul3s(nf, 2.0f * dot3(nf, wo)), wo);
-            float ac = fabsf(dot3(wi, nf));
-            f = mk3(m->color[0] / ac, m->color[1] / ac, m->color[2] / ac);
-            bp = 1.0f;
-        } else if (m->kind == 3u) {
-            f = glass_sample3(m, wo, ng, u01(rr[0]), 0, &wi);
-            bp = 1.0f;
-            weight = 1;
-        } else {
-            bsdf_sample3(m, wo, ng, nf, u01(rr[0]), u01(rr[1]), &wi, &f, &bp);
-        }
+        int spec, weight; /* weight: f already holds f |cos| / pdf */
+        mat_sample3(S, m, wo, ng, ns, nf, u01(rr[3]), u01(rr[0]), u01(rr[1]), 0, &wi, &f, &bp, &spec, &weight);
+        if (spec) mask |= 1u << d;
         if (black3(f) || !(bp > 0.0f)) return;
-        float s = weight ? 1.0f : fabsf(dot3(ng, wi));
+        float s = weight ? 1.0f : fabsf(dot3(ns, wi));
         f3 tmp = mk3(f.x * s / bp, f.y * s / bp, f.z * s / bp);
+        if (sn) tmp = mul3s(tmp, adjoint_factor(wo, wi, ng, ns));
         float lum = 0.212671f * tmp.x + 0.715160f * tmp.y + 0.072169f * tmp.z;
         float cp = lum < 1.0f ? lum : 1.0f;
         if (u01(rr[2]) > cp) return;
@@ -1948,10 +2216,10 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
                      uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                      uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t light_paths,
                      uint32_t light_max_depth, uint32_t connections, uint32_t filter, float fxw, float fyw,
-                     float falpha, float* out_rgb)
+                     float falpha, float* out_rgb, const float* tri_n, const uint8_t* has_n)
 {
     orc_material* mats = mats_from12(mats4, nmats);
-    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl};
+    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, lights, nl, tri_n, has_n};
     const uint32_t NL = light_paths, D = light_max_depth;
     size_t slots = (size_t)NL * D;
     uint32_t* count = (uint32_t*)calloc(NL ? NL : 1, 4);
@@ -1961,11 +2229,13 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
     float* alpha = (float*)calloc(slots * 3 + 1, 4);
     uint32_t* mid = (uint32_t*)calloc(slots + 1, 4);
     uint32_t* mask = (uint32_t*)calloc(slots + 1, 4);
+    float* nss = tri_n ? (float*)calloc(slots * 3 + 1, 4) : NULL;
+    uint8_t* snf = tri_n ? (uint8_t*)calloc(slots + 1, 1) : NULL;
     orc_rng lrng = rng_make(seed, 6u);
 #pragma omp parallel for schedule(dynamic, 64)
     for (long j = 0; j < (long)NL; j++)
-        light_path(&S, lrng, (uint32_t)j, D, count, pos, ngs, wos, alpha, mid, mask);
-    orc_lvc B = {NL, D, nl ? connections : 0u, count, pos, ngs, wos, alpha, mid, mask, rng_make(seed, 8u)};
+        light_path(&S, lrng, (uint32_t)j, D, count, pos, ngs, wos, alpha, mid, mask, nss, snf);
+    orc_lvc B = {NL, D, nl ? connections : 0u, count, pos, ngs, wos, alpha, mid, mask, rng_make(seed, 8u), nss, snf};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
     float* Ls = (float*)malloc((size_t)n * 12);
@@ -1982,6 +2252,7 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
     }
     orc_film(Ls, width, x0, y0, w, h, spp, seed, filter, fxw, fyw, falpha, out_rgb);
     free(rays); free(Ls); free(count); free(pos); free(ngs); free(wos); free(alpha); free(mid); free(mask);
+    free(nss); free(snf);
     free(mats);
 }
 
@@ -1999,10 +2270,10 @@ void orc_render_raycast(const uint32_t* nodes, const uint32_t* woop, const uint3
                         const uint32_t* idx, const uint32_t* tri_mat, const float* m12, uint32_t nmats,
                         const float* r2c, const float* c2w, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w,
                         uint32_t h, uint32_t spp, uint64_t seed, uint32_t shade, uint32_t filter, float fxw,
-                        float fyw, float falpha, float* out_rgb)
+                        float fyw, float falpha, float* out_rgb, const float* tri_n, const uint8_t* has_n)
 {
     orc_material* mats = mats_from12(m12, nmats);
-    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, NULL, 0};
+    orc_scene S = {nodes, woop, tidx, verts, idx, tri_mat, mats, NULL, 0, tri_n, has_n};
     uint32_t n = w * h * spp;
     float* rays = (float*)malloc((size_t)n * 32);
     float* Ls = (float*)malloc((size_t)n * 12);
@@ -2020,10 +2291,15 @@ void orc_render_raycast(const uint32_t* nodes, const uint32_t* woop, const uint3
                 const float* q = kPalette[(uint32_t)id % 15u];
                 c = mk3(q[0], q[1], q[2]);
             } else if (shade == 1) {
+                /* a layered material shows its first layer's colour
+                 * (integrator_raycast.cu:71-76) */
                 const orc_material* m = &mats[tri_mat[id]];
+                if (m->kind == 6u) m = &mats[(uint32_t)m->param[1]];
                 c = mk3(m->color[0], m->color[1], m->color[2]);
             } else if (shade == 2) {
-                c = tri_norm(a, b, cc);
+                /* m_shading_normal (integrator_raycast.cu:109-115) */
+                f3 p = add3(mk3(R[0], R[1], R[2]), mul3s(mk3(R[4], R[5], R[6]), t));
+                interp_normal(&S, (uint32_t)id, p, a, b, cc, tri_norm(a, b, cc), &c);
             } else if (shade == 3) {
                 /* barycentric weights of v0, v1 at the hit point */
                 f3 p = add3(mk3(R[0], R[1], R[2]), mul3s(mk3(R[4], R[5], R[6]), t));
@@ -2043,4 +2319,44 @@ void orc_render_raycast(const uint32_t* nodes, const uint32_t* woop, const uint3
     free(rays);
     free(Ls);
     free(mats);
+}
+
+/* ---- intersection records ----------------------------------------------- */
+
+/* The result-write block of kernel_kepler_trace (kepler_dynamic_fetch.h:312-360)
+ * as its own pass over (id, t): each hit's position o + t d, Ng (unit
+ * cross(v1 - v0, v2 - v0)), Ns (interp_normal), -d, barycentric (u, v) of
+ * the position (the reference stores the u, v of the last triangle tested,
+ * not of the hit -- a slip not kept), material id, front = Ng.d < 0,
+ * epsilon = 1e-3 t; t itself. */
+void orc_fill(const float* rays, const int32_t* ids, const float* ts, uint32_t n, const float* verts,
+              const uint32_t* idx, const uint32_t* tri_mat, const float* tri_n, const uint8_t* has_n, float* out,
+              uint32_t* mid_front)
+{
+    orc_scene S = {NULL, NULL, NULL, verts, idx, tri_mat, NULL, NULL, 0, tri_n, has_n};
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < (long)n; i++) {
+        const int32_t id = ids[i];
+        if (id < 0) continue;
+        const float* R = rays + 8 * (size_t)i;
+        const float t = ts[i];
+        f3 d = mk3(R[4], R[5], R[6]);
+        f3 p = add3(mk3(R[0], R[1], R[2]), mul3s(d, t));
+        f3 a, b, c;
+        tri_verts(&S, (uint32_t)id, &a, &b, &c);
+        f3 ng = tri_norm(a, b, c), ns;
+        interp_normal(&S, (uint32_t)id, p, a, b, c, ng, &ns);
+        float u, v;
+        bary_uv(p, a, b, c, &u, &v);
+        float* o = out + 16 * (size_t)i;
+        o[0] = p.x; o[1] = p.y; o[2] = p.z;
+        o[3] = ng.x; o[4] = ng.y; o[5] = ng.z;
+        o[6] = ns.x; o[7] = ns.y; o[8] = ns.z;
+        o[9] = -d.x; o[10] = -d.y; o[11] = -d.z;
+        o[12] = u; o[13] = v;
+        o[14] = 1.0e-3f * t;
+        o[15] = t;
+        mid_front[2 * (size_t)i] = tri_mat[id];
+        mid_front[2 * (size_t)i + 1] = dot3(ng, d) < 0.0f ? 1u : 0u;
+    }
 }

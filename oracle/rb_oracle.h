@@ -136,7 +136,8 @@ void orc_film(const float* Ls, uint32_t width, uint32_t x0, uint32_t y0, uint32_
 void orc_render_ao(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
                    const uint32_t* idx, const float* r2c, const float* c2w, uint32_t width, uint32_t height,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed, uint32_t nao,
-                   float radius, uint32_t filter, float fxw, float fyw, float falpha, float* out_rgb);
+                   float radius, uint32_t filter, float fxw, float fyw, float falpha, float* out_rgb,
+                   const float* tri_n, const uint8_t* has_n);
 /* Path tracer (src/integrators/integrator_path.cu:43-261): NEE with MIS,
  * Lambert / mirror / emitter materials (mats4: kind bits + rgb per material). */
 void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
@@ -144,7 +145,7 @@ void orc_render_pt(const uint32_t* nodes, const uint32_t* woop, const uint32_t* 
                    const uint32_t* lights, uint32_t nl, const float* r2c, const float* c2w, uint32_t width,
                    uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                    uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t filter, float fxw, float fyw,
-                   float falpha, float* out_rgb);
+                   float falpha, float* out_rgb, const float* tri_n, const uint8_t* has_n);
 /* x^y from log2 / exp2 series (no math library): the renderer's pow. */
 float orc_powf(float x, float y);
 /* BSDF of one material (m12: kind bits, rgb, param[8] as rb_material) for
@@ -152,6 +153,28 @@ float orc_powf(float x, float y);
 void orc_bsdf_eval(const float* m12, const float* wo, const float* wi, const float* ng, float* out);
 /* One BSDF sample from (u1, u2): out = wi.xyz, weight or f .rgb, pdf. */
 void orc_bsdf_sample(const float* m12, const float* wo, const float* ng, float u1, float u2, float* out);
+/* Decision log of the SBVH build (checker support).  After
+ * orc_build_log_start(), every node that orc_bvh_build on this thread
+ * visits is recorded: its level, bounds and references (after degenerate
+ * removal: tri id bits + box, 7 floats each, from ref_off), the leaf /
+ * object / spatial SAH it computed (FLT_MAX where not computed; the spatial
+ * one only when sp_tried), and what it did (kind 0 leaf, 1 object split,
+ * 2 spatial split) with the children's reference counts. */
+typedef struct {
+    int32_t level, kind;
+    uint32_t num;
+    float bounds[6];
+    float leaf_sah, obj_sah, sp_sah;
+    int32_t obj_dim, sp_dim, sp_tried;
+    uint32_t obj_left;
+    float sp_pos;
+    uint32_t ln, rn;
+    uint64_t ref_off;
+} orc_node_log;
+void orc_build_log_start(void);
+void orc_build_log_stop(void);
+size_t orc_build_log_nodes(orc_node_log* out);
+size_t orc_build_log_refs(float* out7);
 /* Number of BDPT techniques able to build a path z_1..z_n (mask bit i-1 =
  * z_i specular) when camera vertices exist for bounces 0..max_depth and the
  * light-vertex cache holds depths 0..lvc_depth-1; a path's weight is 1/that. */
@@ -166,16 +189,30 @@ void orc_render_bdpt(const uint32_t* nodes, const uint32_t* woop, const uint32_t
                      uint32_t x0, uint32_t y0, uint32_t w, uint32_t h, uint32_t spp, uint64_t seed,
                      uint32_t max_depth, uint32_t rr_start, float rr_prob, uint32_t light_paths,
                      uint32_t light_max_depth, uint32_t connections, uint32_t filter, float fxw, float fyw,
-                     float falpha, float* out_rgb);
+                     float falpha, float* out_rgb, const float* tri_n, const uint8_t* has_n);
 /* Ray-cast debug integrator (src/integrators/integrator_raycast.cu:37-160):
  * closest hit of the camera rays coloured by shade mode 0 primitive id
- * (15-colour palette, id mod 15), 1 material colour, 2 geometric normal,
- * 3 barycentrics (u, v, 0), 4 uniform white; black on a miss; then the film. */
+ * (15-colour palette, id mod 15), 1 material colour, 2 shading normal (the
+ * geometric normal for a triangle without normals), 3 barycentrics (u, v,
+ * 0), 4 uniform white; black on a miss; then the film.
+ * tri_n / has_n (every render entry point): per-triangle vertex normals (9
+ * floats) and whether each triangle has them, or NULL for none. */
 void orc_render_raycast(const uint32_t* nodes, const uint32_t* woop, const uint32_t* tidx, const float* verts,
                         const uint32_t* idx, const uint32_t* tri_mat, const float* m12, uint32_t nmats,
                         const float* r2c, const float* c2w, uint32_t width, uint32_t x0, uint32_t y0, uint32_t w,
                         uint32_t h, uint32_t spp, uint64_t seed, uint32_t shade, uint32_t filter, float fxw,
-                        float fyw, float falpha, float* out_rgb);
+                        float fyw, float falpha, float* out_rgb, const float* tri_n, const uint8_t* has_n);
+/* The IntersectionBuffer fields kernel_kepler_trace writes for a hit
+ * (kepler_dynamic_fetch.h:312-360) from (id, t) of ray i: position o + t d,
+ * geometric normal, shading normal (interpolated, or the geometric one),
+ * incoming direction -d, barycentric (u, v) of the position, material id,
+ * front (geometric normal against d), epsilon 1e-3 t.  out, per ray:
+ * [0..2] position, [3..5] Ng, [6..8] Ns, [9..11] -d, [12..13] uv,
+ * [14] epsilon, [15] t; mid_front[2 i] material id, [2 i + 1] front;
+ * misses leave their entries untouched. */
+void orc_fill(const float* rays, const int32_t* ids, const float* ts, uint32_t n, const float* verts,
+              const uint32_t* idx, const uint32_t* tri_mat, const float* tri_n, const uint8_t* has_n, float* out,
+              uint32_t* mid_front);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,8 @@ SZ = C.c_size_t
 RB_OK, RB_ERR_INVALID, RB_ERR_OOM, RB_ERR_DEVICE, RB_ERR_BUSY, RB_ERR_UNSUPPORTED, RB_ERR_NOT_READY = range(7)
 RB_CLOSEST, RB_ANY, RB_MASKED = 0, 1, 2
 RB_LAYOUT_REF_COMPACT2, RB_LAYOUT_MI355X = 1, 2
-RB_MAT_LAMBERT, RB_MAT_MIRROR, RB_MAT_EMITTER, RB_MAT_GLASS, RB_MAT_ORENNAYAR, RB_MAT_MICROFACET = range(6)
+(RB_MAT_LAMBERT, RB_MAT_MIRROR, RB_MAT_EMITTER, RB_MAT_GLASS, RB_MAT_ORENNAYAR, RB_MAT_MICROFACET,
+ RB_MAT_LAYERED) = range(7)
 RB_FRESNEL_NONE, RB_FRESNEL_CONDUCTOR, RB_FRESNEL_DIELECTRIC = 0, 1, 2
 RB_GEN_ICOSPHERES, RB_GEN_THIN_QUADS, RB_GEN_ROOM, RB_GEN_EMITTER_QUADS = 0, 1, 2, 3
 
@@ -42,7 +43,7 @@ class rb_gen_params(C.Structure):
     _fields_ = [("kind", U32), ("count", U32), ("subdiv", U32), ("seed", U64),
                 ("r0", F32), ("r1", F32), ("c0", F32), ("c1", F32),
                 ("material_id", I32), ("mix_materials", U32), ("emit", F32 * 3),
-                ("subdiv_hi", U32), ("hi_count", U32)]
+                ("subdiv_hi", U32), ("hi_count", U32), ("smooth", U32)]
 
 
 class rb_build_params(C.Structure):
@@ -96,7 +97,11 @@ _SIGS = {
     "rb_scene_create": (C.c_int, [P]),
     "rb_scene_destroy": (C.c_int, [P]),
     "rb_scene_add_material": (C.c_int, [P, P, P]),
+    "rb_scene_add_layered_material": (C.c_int, [P, P, U32, P]),
     "rb_scene_add_mesh": (C.c_int, [P, P, U32, P, U32, U32]),
+    "rb_scene_add_mesh_ex": (C.c_int, [P, P, U32, P, U32, P, U32, P, P, U32, P, U32]),
+    "rb_scene_shading_counts": (C.c_int, [P, P, P, P, P]),
+    "rb_scene_shading": (C.c_int, [P, P, P, P, P]),
     "rb_scene_counts": (C.c_int, [P, P, P, P, P]),
     "rb_scene_geometry": (C.c_int, [P, P, P, P, P]),
     "rb_scene_materials": (C.c_int, [P, P]),

@@ -164,6 +164,16 @@ class Scene:
     def add_diffuse_light_bsdf(self, radiance):
         return self.add_material(L.RB_MAT_EMITTER, radiance)
 
+    def add_layered_material(self, layer_ids):
+        """A material of 1..4 layers, each a material id of this scene that is
+        neither an emitter nor layered (Material's layers,
+        src/core/material.h:44-52): the equal mixture of the layers, one
+        picked uniformly per sampled direction."""
+        ids = np.ascontiguousarray(layer_ids, dtype=np.uint32).ravel()
+        mid = C.c_uint32()
+        check(lib().rb_scene_add_layered_material(self._h, ids.ctypes.data_as(C.c_void_p), len(ids), C.byref(mid)))
+        return mid.value
+
     def load_obj(self, path, material_id=0):
         """Every object of a Wavefront OBJ file with one material (ObjFileReader, src/io/objfilereader.cpp)."""
         check(lib().rb_scene_load_obj(self._h, str(path).encode(), material_id))
@@ -177,15 +187,44 @@ class Scene:
         cam.c = c
         return cam
 
-    def add_mesh(self, verts, idx, material_id):
+    def add_mesh(self, verts, idx, material_id, normals=None, nidx=None, texcoords=None, tidx=None):
+        """Scene::add_mesh (src/core/scene.cpp:83-123): triangles index the
+        mesh's own vertices; normals / texcoords with their own index triples
+        (None: one per vertex, indexed like the vertices; a triple of
+        0xFFFFFFFF: that triangle has none)."""
         v = np.ascontiguousarray(verts, np.float32).reshape(-1, 3)
         i = np.ascontiguousarray(idx, np.uint32).reshape(-1, 3)
-        check(lib().rb_scene_add_mesh(self._h, ptr(v), len(v), ptr(i), len(i), material_id))
+        if normals is None and texcoords is None:
+            check(lib().rb_scene_add_mesh(self._h, ptr(v), len(v), ptr(i), len(i), material_id))
+            return
+        n = None if normals is None else np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        ni = None if nidx is None else np.ascontiguousarray(nidx, np.uint32).reshape(-1, 3)
+        t = None if texcoords is None else np.ascontiguousarray(texcoords, np.float32).reshape(-1, 2)
+        ti = None if tidx is None else np.ascontiguousarray(tidx, np.uint32).reshape(-1, 3)
+        check(lib().rb_scene_add_mesh_ex(self._h, ptr(v), len(v), ptr(i), len(i),
+                                         ptr(n) if n is not None else None, 0 if n is None else len(n),
+                                         ptr(ni) if ni is not None else None,
+                                         ptr(t) if t is not None else None, 0 if t is None else len(t),
+                                         ptr(ti) if ti is not None else None, material_id))
+
+    def shading(self):
+        """(normals f32[nn,3], nidx u32[nt,3] or None, texcoords f32[ntc,2],
+        tidx u32[nt,3] or None): the scene's shading data."""
+        nn, ntc, hn, ht = C.c_uint32(), C.c_uint32(), C.c_int(), C.c_int()
+        check(lib().rb_scene_shading_counts(self._h, C.byref(nn), C.byref(ntc), C.byref(hn), C.byref(ht)))
+        nt = self.counts()[1]
+        n = np.empty((nn.value, 3), np.float32)
+        t = np.empty((ntc.value, 2), np.float32)
+        ni = np.empty((nt, 3), np.uint32) if hn.value else None
+        ti = np.empty((nt, 3), np.uint32) if ht.value else None
+        check(lib().rb_scene_shading(self._h, ptr(n), ptr(ni) if ni is not None else None, ptr(t),
+                                     ptr(ti) if ti is not None else None))
+        return n, ni, t, ti
 
     def generate(self, kind, count=0, subdiv=0, seed=0, r=(0.0, 0.0), c=(0.0, 0.0), material_id=-1,
-                 mix_materials=False, emit=(0.0, 0.0, 0.0), subdiv_hi=0, hi_count=0):
+                 mix_materials=False, emit=(0.0, 0.0, 0.0), subdiv_hi=0, hi_count=0, smooth=False):
         p = L.rb_gen_params(kind, count, subdiv, seed, r[0], r[1], c[0], c[1], material_id,
-                            int(mix_materials), (C.c_float * 3)(*emit), subdiv_hi, hi_count)
+                            int(mix_materials), (C.c_float * 3)(*emit), subdiv_hi, hi_count, int(smooth))
         check(lib().rb_scene_generate(self._h, C.byref(p)))
 
     def counts(self):

@@ -7,7 +7,7 @@ include Makefile
 # writers) are instrumented, the HIP sources are the release objects.  The
 # library leaves the sanitizer runtime to the process, so run it with gcc's
 # runtime preloaded (CPU only: `make -f asan.mk asan-test`).
-ASAN_OUT ?= ../librb_asan.so
+ASAN_OUT ?= ../../_build/librb_asan.so
 ASAN_BUILD ?= ../../_build/rb_asan
 ASAN_FLAGS = -O1 -g -fno-omit-frame-pointer -fsanitize=address,undefined -fno-sanitize-recover=undefined
 ASAN_OBJ = $(patsubst %.cpp,$(ASAN_BUILD)/%.o,$(CPP_SRC))

@@ -29,15 +29,17 @@ namespace rb {
 
 // Light-vertex cache: one 64-byte record per vertex (one cache line per
 // random gather by a connection), rows
-//   [0] p.xyz | (unused)
+//   [0] p.xyz | 1 when row 4 holds an interpolated shading normal, else 0
 //   [1] geometric normal | material id bits
 //   [2] direction towards the previous light vertex | specular mask bits
 //   [3] light sub-path throughput arriving at the vertex
+//   [4] (scenes with shading normals only: 80-byte records) shading normal
 struct LvcBufs {
-    float4* v;        // 4 rows per vertex, vertex `k` of light path `j` at slot j * depth + k
+    float4* v;        // `rows` rows per vertex, vertex `k` of light path `j` at slot j * depth + k
     uint32_t* count;  // vertices stored per light path
     uint32_t depth;   // slots per light path
     uint32_t paths;
+    uint32_t rows;    // 4, or 5 in scenes with shading normals
 };
 
 struct LightPath {
@@ -184,44 +186,41 @@ __global__ __launch_bounds__(kQueueBlock) void k_light_hit(uint32_t qi, uint32_t
                 const V3 dv = v3(dd.x, dd.y, dd.z);
                 const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
                 V3 v0, v1, v2;
-                uint32_t mid;
-                tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid);
+                uint32_t mid, tf;
+                tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
                 const rb_material m = S.mats[mid];
                 if (m.kind != RB_MAT_EMITTER) {
                     const V3 ng = tri_normal(v0, v1, v2);
+                    V3 ns;
+                    const bool sn = shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
                     const V3 wo = vneg(dv);
                     const float4 a = P.alpha[j];
                     uint32_t mask = P.mask[j];
-                    const bool delta = is_delta(m);
-                    if (delta) mask |= 1u << d;
+                    // the stored vertex is specular when every layer is; the
+                    // path continues with the sampled lobe's kind
+                    const uint32_t dbit = mat_delta(S, m) ? 1u << d : 0u;
                     const size_t slot = (size_t)j * C.depth + d;
-                    float4* rec = C.v + 4 * (size_t)slot;
-                    rec[0] = make_float4(p.x, p.y, p.z, 0.f);
+                    float4* rec = C.v + C.rows * slot;
+                    rec[0] = make_float4(p.x, p.y, p.z, __uint_as_float(sn ? 1u : 0u));
                     rec[1] = make_float4(ng.x, ng.y, ng.z, __uint_as_float(mid));
-                    rec[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mask));
+                    rec[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mask | dbit));
                     rec[3] = a;
+                    if (C.rows > 4) rec[4] = make_float4(ns.x, ns.y, ns.z, 0.f);
                     C.count[j] = d + 1;
                     if (d + 1 < C.depth) {
                         const U4 r = rng.draw(j, d + 1, 0, 0);
-                        const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
+                        const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
                         V3 wi, f;
                         float pdf;
-                        bool weight = false;  // f already holds f |cos| / pdf
-                        if (m.kind == RB_MAT_MIRROR) {
-                            wi = vsub(vmul(nf, 2.0f * vdot(nf, wo)), wo);
-                            const float ac = fabsf(vdot(wi, nf));
-                            f = v3(m.color[0] / ac, m.color[1] / ac, m.color[2] / ac);
-                            pdf = 1.0f;
-                        } else if (m.kind == RB_MAT_GLASS) {
-                            f = glass_sample(m, wo, ng, u01(r.x), false, &wi);
-                            pdf = 1.0f;
-                            weight = true;
-                        } else {
-                            bsdf_sample(m, wo, ng, nf, u01(r.x), u01(r.y), &wi, &f, &pdf);
-                        }
+                        bool spec, weight;  // weight: f already holds f |cos| / pdf
+                        mat_sample(S, m, wo, ng, ns, nf, u01(r.w), u01(r.x), u01(r.y), false, &wi, &f, &pdf, &spec,
+                                   &weight);
+                        if (spec) mask |= 1u << d;
                         if (!is_black(f) && pdf > 0.0f) {
-                            const float s = weight ? 1.0f : fabsf(vdot(ng, wi));
-                            const V3 tmp = v3(f.x * s / pdf, f.y * s / pdf, f.z * s / pdf);
+                            const float s = weight ? 1.0f : fabsf(vdot(ns, wi));
+                            V3 tmp = v3(f.x * s / pdf, f.y * s / pdf, f.z * s / pdf);
+                            // importance transport: shading-normal adjoint factor
+                            if (sn) tmp = vmul(tmp, adjoint_factor(wo, wi, ng, ns));
                             const float lum = luminance(tmp);
                             const float cp = lum < 1.0f ? lum : 1.0f;
                             if (!(u01(r.z) > cp)) {

@@ -138,7 +138,8 @@ struct rb_bvh_s {
     uint32_t* index = nullptr;  // original triangle id per reference
     uint32_t n_nodes = 0, n_refs = 0;
     // shading data (Scene::gpu_prepare, src/core/scene.cpp:166-301)
-    float4* tri_v = nullptr;          // 3 float4 per triangle: v0.xyz|mat, v1.xyz|0, v2.xyz|0
+    float4* tri_v = nullptr;          // 3 float4 per triangle: v0.xyz|mat, v1.xyz|flags, v2.xyz|0
+    float4* tri_n = nullptr;          // 3 float4 per triangle (n0, n1, n2) when the scene has normals
     rb_material* mats = nullptr;
     uint32_t* light_tris = nullptr;
     uint32_t n_tris = 0, n_mats = 0, n_lights = 0;

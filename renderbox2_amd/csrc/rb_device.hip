@@ -372,17 +372,26 @@ rb_status rb_bvh_upload(rb_host_bvh_t hb, rb_scene_t scene, rb_bvh_t* out) {
         const Scene& Sc = scene->s;
         B->n_tris = Sc.num_tris();
         std::vector<float> tv((size_t)B->n_tris * 12);
+        // shading normals: 3 float4 per triangle, only when some triangle has them
+        std::vector<float> tn(Sc.nidx.empty() ? 0 : (size_t)B->n_tris * 12, 0.0f);
         for (uint32_t t = 0; t < B->n_tris; t++) {
+            const bool has_n = Sc.tri_has_normals(t);
             for (int k = 0; k < 3; k++) {
                 uint32_t vi = Sc.idx[3 * (size_t)t + k];
                 tv[12 * (size_t)t + 4 * k + 0] = Sc.verts[3 * (size_t)vi];
                 tv[12 * (size_t)t + 4 * k + 1] = Sc.verts[3 * (size_t)vi + 1];
                 tv[12 * (size_t)t + 4 * k + 2] = Sc.verts[3 * (size_t)vi + 2];
-                uint32_t w = k == 0 ? Sc.tri_mat[t] : 0u;
+                // v0.w: material; v1.w: flags (kTriHasNormals = 1)
+                uint32_t w = k == 0 ? Sc.tri_mat[t] : (k == 1 && has_n ? 1u : 0u);
                 std::memcpy(&tv[12 * (size_t)t + 4 * k + 3], &w, 4);
+                if (has_n) {
+                    const uint32_t ni = Sc.nidx[3 * (size_t)t + k];
+                    for (int c = 0; c < 3; c++) tn[12 * (size_t)t + 4 * k + c] = Sc.normals[3 * (size_t)ni + c];
+                }
             }
         }
         if ((st = up((void**)&B->tri_v, tv.data(), tv.size() * 4)) != RB_OK) { cleanup(); return st; }
+        if (!tn.empty() && (st = up((void**)&B->tri_n, tn.data(), tn.size() * 4)) != RB_OK) { cleanup(); return st; }
         B->n_mats = (uint32_t)Sc.mats.size();
         if ((st = up((void**)&B->mats, Sc.mats.data(), Sc.mats.size() * sizeof(rb_material))) != RB_OK) { cleanup(); return st; }
         B->n_lights = (uint32_t)Sc.light_tris.size();
@@ -402,6 +411,7 @@ rb_status rb_bvh_release(rb_bvh_t b) {
         dfree(b->tris, ((size_t)b->n_refs * 12 + 4) * 4);
         dfree(b->index, ((size_t)b->n_refs + 1) * 4);
         dfree(b->tri_v, (size_t)b->n_tris * 48);
+        dfree(b->tri_n, (size_t)b->n_tris * 48);
         dfree(b->mats, (size_t)b->n_mats * sizeof(rb_material));
         dfree(b->light_tris, (size_t)b->n_lights * 4);
     }

@@ -33,6 +33,83 @@ rb_status fail(rb_status s, const char* fmt, ...) {
     return s;
 }
 
+// One index array of a mesh's corners: every index of a used triple below
+// `count`, or the whole triple kNoIndex (a triangle without that data).
+static rb_status check_corners(const uint32_t* c, uint32_t nt, uint32_t count, const char* what) {
+    if (!c) return RB_OK;
+    for (uint64_t t = 0; t < nt; t++) {
+        const uint32_t a = c[3 * t], b = c[3 * t + 1], d = c[3 * t + 2];
+        if (a == kNoIndex && b == kNoIndex && d == kNoIndex) continue;
+        if (a >= count || b >= count || d >= count)
+            return fail(RB_ERR_INVALID, "mesh triangle %llu: %s index out of range (%u %u %u of %u; a triangle "
+                        "without %ss takes %u %u %u)", (unsigned long long)t, what, a, b, d, count, what, kNoIndex,
+                        kNoIndex, kNoIndex);
+    }
+    return RB_OK;
+}
+
+// Appends one per-corner attribute (normals: 3 floats, texcoords: 2) with
+// its index triples, offset into the scene's arrays; keeps `sidx` empty
+// while no triangle of the scene has the attribute.
+static void append_corners(std::vector<float>& sdata, std::vector<uint32_t>& sidx, uint32_t tris_before,
+                           const float* data, uint32_t count, uint32_t width, const uint32_t* c, uint32_t nt) {
+    bool any = false;
+    for (uint64_t i = 0; c && i < 3ull * nt && !any; i++) any = c[i] != kNoIndex;
+    if (!any) {
+        if (!sidx.empty()) sidx.insert(sidx.end(), 3 * (size_t)nt, kNoIndex);
+        return;
+    }
+    if (sidx.empty()) sidx.assign(3 * (size_t)tris_before, kNoIndex);
+    const uint32_t base = (uint32_t)(sdata.size() / width);
+    sdata.insert(sdata.end(), data, data + (size_t)width * count);
+    for (uint64_t i = 0; i < 3ull * nt; i++) sidx.push_back(c[i] == kNoIndex ? kNoIndex : c[i] + base);
+}
+
+rb_status append_mesh(Scene& S, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* normals, uint32_t nn, const uint32_t* nidx, const float* texcoords, uint32_t ntc,
+                      const uint32_t* tidx, const uint32_t* tri_mat, uint32_t material_id) {
+    rb_status st = scene_room(S, nv, nt);  // before any array is read
+    if (st != RB_OK) return st;
+    if ((uint64_t)S.normals.size() / 3 + nn > kMaxSceneVerts || (uint64_t)S.texcoords.size() / 2 + ntc > kMaxSceneVerts)
+        return fail(RB_ERR_INVALID, "scene would exceed 2^32 - 1 normals or texture coordinates");
+    if (nt && (!verts || !idx)) return fail(RB_ERR_INVALID, "NULL mesh arrays");
+    if ((nn && !normals) || (ntc && !texcoords)) return fail(RB_ERR_INVALID, "NULL normal or texcoord array");
+    for (uint64_t i = 0; i < 3ull * nt; i++)
+        if (idx[i] >= nv) return fail(RB_ERR_INVALID, "mesh triangle %llu indexes vertex %u of %u",
+                                      (unsigned long long)(i / 3), idx[i], nv);
+    // normals without their own indices: one per vertex, indexed like the vertices
+    if (normals && !nidx) {
+        if (nn != nv) return fail(RB_ERR_INVALID, "%u normals without normal indices need one per vertex (%u)", nn, nv);
+        nidx = idx;
+    }
+    if (texcoords && !tidx) {
+        if (ntc != nv) return fail(RB_ERR_INVALID, "%u texcoords without indices need one per vertex (%u)", ntc, nv);
+        tidx = idx;
+    }
+    if (!normals) nidx = nullptr;
+    if (!texcoords) tidx = nullptr;
+    if ((st = check_corners(nidx, nt, nn, "normal")) != RB_OK) return st;
+    if ((st = check_corners(tidx, nt, ntc, "texcoord")) != RB_OK) return st;
+    for (uint32_t t = 0; tri_mat && t < nt; t++)
+        if (tri_mat[t] >= S.mats.size()) return fail(RB_ERR_INVALID, "material id %u out of range", tri_mat[t]);
+    const uint32_t base = S.num_verts();
+    const uint32_t tbase = S.num_tris();
+    try {
+        S.verts.insert(S.verts.end(), verts, verts + 3 * (size_t)nv);
+        for (uint64_t i = 0; i < 3ull * nt; i++) S.idx.push_back(idx[i] + base);
+        append_corners(S.normals, S.nidx, tbase, normals, nn, 3, nidx, nt);
+        append_corners(S.texcoords, S.tidx, tbase, texcoords, ntc, 2, tidx, nt);
+        for (uint32_t t = 0; t < nt; t++) {
+            const uint32_t m = tri_mat ? tri_mat[t] : material_id;
+            S.tri_mat.push_back(m);
+            if (S.mats[m].kind == RB_MAT_EMITTER) S.light_tris.push_back(tbase + t);
+        }
+    } catch (...) {
+        return fail(RB_ERR_OOM, "scene growth failed");
+    }
+    return RB_OK;
+}
+
 }  // namespace rb
 
 using namespace rb;
@@ -41,6 +118,50 @@ using namespace rb;
     do {                                                \
         if (!(c)) return fail(RB_ERR_INVALID, "%s", msg); \
     } while (0)
+
+namespace rb {
+// Checks a material and fills its derived parameters; the layers of a
+// layered material are looked up in `mats`.
+rb_status prepare_material(const std::vector<rb_material>& mats, rb_material* m) {
+    RB_CHECK_ARG(m->kind <= RB_MAT_LAYERED, "unknown material kind");
+    rb_material& c = *m;
+    if (c.kind == RB_MAT_LAYERED) {
+        const float nf = c.param[0];
+        RB_CHECK_ARG(nf >= 1.0f && nf <= 4.0f && nf == (float)(uint32_t)nf, "a layered material has 1..4 layers");
+        const uint32_t n = (uint32_t)nf;
+        for (uint32_t k = 0; k < n; k++) {
+            const float f = c.param[1 + k];
+            if (!(f >= 0.0f && f == (float)(uint32_t)f && (uint32_t)f < mats.size()))
+                return fail(RB_ERR_INVALID, "layer %u: material id %g is not a material of the scene", k, f);
+            const uint32_t kind = mats[(uint32_t)f].kind;
+            if (kind == RB_MAT_EMITTER || kind == RB_MAT_LAYERED)
+                return fail(RB_ERR_INVALID, "layer %u: material %u is an emitter or layered (layers are single BSDFs)",
+                            k, (uint32_t)f);
+        }
+        for (uint32_t k = n + 1; k < 8; k++) c.param[k] = 0.0f;
+        c.color[0] = c.color[1] = c.color[2] = 0.0f;
+    }
+    if (c.kind == RB_MAT_GLASS) RB_CHECK_ARG(c.param[0] > 0.0f && c.param[1] > 0.0f, "glass needs etai, etat > 0");
+    if (c.kind == RB_MAT_ORENNAYAR) {
+        // A and B from sigma (CustomFileReader::process, src/io/customfilereader.cpp:137-146)
+        float sigma = c.param[0] < 0.0f ? 0.0f : (c.param[0] > 90.0f ? 90.0f : c.param[0]);
+        sigma = (3.14159265358979f / 180.0f) * sigma;
+        const float s2 = sigma * sigma;
+        c.param[1] = 1.0f - (s2 / (2.0f * (s2 + 0.33f)));
+        c.param[2] = 0.45f * s2 / (s2 + 0.09f);
+    }
+    if (c.kind == RB_MAT_MICROFACET) {
+        RB_CHECK_ARG(c.param[0] > 0.0f, "microfacet exponent must be > 0");
+        const float ft = c.param[1];
+        RB_CHECK_ARG(ft == RB_FRESNEL_NONE || ft == RB_FRESNEL_CONDUCTOR || ft == RB_FRESNEL_DIELECTRIC,
+                     "microfacet Fresnel type must be 0, 1 or 2");
+        if (ft == RB_FRESNEL_DIELECTRIC)
+            RB_CHECK_ARG(c.param[2] > 0.0f && c.param[3] > 0.0f, "dielectric etas must be > 0");
+    }
+    return RB_OK;
+}
+
+}  // namespace rb
 
 extern "C" {
 
@@ -87,53 +208,40 @@ rb_status rb_scene_destroy(rb_scene_t s) {
 
 rb_status rb_scene_add_material(rb_scene_t s, const rb_material* m, uint32_t* id) {
     RB_CHECK_ARG(s && m, "NULL scene or material");
-    RB_CHECK_ARG(m->kind <= RB_MAT_MICROFACET, "unknown material kind");
     rb_material c = *m;
-    if (c.kind == RB_MAT_GLASS) RB_CHECK_ARG(c.param[0] > 0.0f && c.param[1] > 0.0f, "glass needs etai, etat > 0");
-    if (c.kind == RB_MAT_ORENNAYAR) {
-        // A and B from sigma (CustomFileReader::process, src/io/customfilereader.cpp:137-146)
-        float sigma = c.param[0] < 0.0f ? 0.0f : (c.param[0] > 90.0f ? 90.0f : c.param[0]);
-        sigma = (3.14159265358979f / 180.0f) * sigma;
-        const float s2 = sigma * sigma;
-        c.param[1] = 1.0f - (s2 / (2.0f * (s2 + 0.33f)));
-        c.param[2] = 0.45f * s2 / (s2 + 0.09f);
-    }
-    if (c.kind == RB_MAT_MICROFACET) {
-        RB_CHECK_ARG(c.param[0] > 0.0f, "microfacet exponent must be > 0");
-        const float ft = c.param[1];
-        RB_CHECK_ARG(ft == RB_FRESNEL_NONE || ft == RB_FRESNEL_CONDUCTOR || ft == RB_FRESNEL_DIELECTRIC,
-                     "microfacet Fresnel type must be 0, 1 or 2");
-        if (ft == RB_FRESNEL_DIELECTRIC)
-            RB_CHECK_ARG(c.param[2] > 0.0f && c.param[3] > 0.0f, "dielectric etas must be > 0");
-    }
+    const rb_status st = prepare_material(s->s.mats, &c);
+    if (st != RB_OK) return st;
     s->s.mats.push_back(c);
     if (id) *id = (uint32_t)(s->s.mats.size() - 1);
     return RB_OK;
 }
 
+rb_status rb_scene_add_layered_material(rb_scene_t s, const uint32_t* layer_ids, uint32_t n, uint32_t* id) {
+    RB_CHECK_ARG(s && layer_ids, "NULL argument");
+    RB_CHECK_ARG(n >= 1 && n <= 4, "a layered material has 1..4 layers");
+    rb_material m{};
+    m.kind = RB_MAT_LAYERED;
+    m.param[0] = (float)n;
+    for (uint32_t k = 0; k < n; k++) {
+        RB_CHECK_ARG(layer_ids[k] < (1u << 24), "layer material id out of range");
+        m.param[1 + k] = (float)layer_ids[k];
+    }
+    return rb_scene_add_material(s, &m, id);
+}
+
 rb_status rb_scene_add_mesh(rb_scene_t s, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
                             uint32_t material_id) {
     RB_CHECK_ARG(s, "NULL scene");
-    RB_CHECK_ARG(nt == 0 || (verts && idx), "NULL mesh arrays");
     RB_CHECK_ARG(material_id < s->s.mats.size(), "material id out of range (add the material first)");
-    Scene& S = s->s;
-    rb_status st = scene_room(S, nv, nt);  // before any array is read
-    if (st != RB_OK) return st;
-    for (uint64_t i = 0; i < 3ull * nt; i++)
-        if (idx[i] >= nv) return fail(RB_ERR_INVALID, "mesh triangle %llu indexes vertex %u of %u",
-                                      (unsigned long long)(i / 3), idx[i], nv);
-    const uint32_t base = S.num_verts();
-    const uint32_t tbase = S.num_tris();
-    try {
-        S.verts.insert(S.verts.end(), verts, verts + 3 * (size_t)nv);
-        for (uint64_t i = 0; i < 3ull * nt; i++) S.idx.push_back(idx[i] + base);
-        S.tri_mat.insert(S.tri_mat.end(), nt, material_id);
-        if (S.mats[material_id].kind == RB_MAT_EMITTER)
-            for (uint32_t t = 0; t < nt; t++) S.light_tris.push_back(tbase + t);
-    } catch (...) {
-        return fail(RB_ERR_OOM, "scene growth failed");
-    }
-    return RB_OK;
+    return append_mesh(s->s, verts, nv, idx, nt, nullptr, 0, nullptr, nullptr, 0, nullptr, nullptr, material_id);
+}
+
+rb_status rb_scene_add_mesh_ex(rb_scene_t s, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                               const float* normals, uint32_t nn, const uint32_t* nidx, const float* texcoords,
+                               uint32_t ntc, const uint32_t* tidx, uint32_t material_id) {
+    RB_CHECK_ARG(s, "NULL scene");
+    RB_CHECK_ARG(material_id < s->s.mats.size(), "material id out of range (add the material first)");
+    return append_mesh(s->s, verts, nv, idx, nt, normals, nn, nidx, texcoords, ntc, tidx, nullptr, material_id);
 }
 
 rb_status rb_scene_counts(rb_scene_t s, uint32_t* nv, uint32_t* nt, uint32_t* nl, uint32_t* nm) {
@@ -148,16 +256,36 @@ rb_status rb_scene_counts(rb_scene_t s, uint32_t* nv, uint32_t* nt, uint32_t* nl
 rb_status rb_scene_geometry(rb_scene_t s, float* verts, uint32_t* idx, uint32_t* tri_mat, uint32_t* lights) {
     RB_CHECK_ARG(s, "NULL scene");
     const Scene& S = s->s;
-    if (verts) std::memcpy(verts, S.verts.data(), S.verts.size() * 4);
-    if (idx) std::memcpy(idx, S.idx.data(), S.idx.size() * 4);
-    if (tri_mat) std::memcpy(tri_mat, S.tri_mat.data(), S.tri_mat.size() * 4);
-    if (lights) std::memcpy(lights, S.light_tris.data(), S.light_tris.size() * 4);
+    copy_vec(verts, S.verts);
+    copy_vec(idx, S.idx);
+    copy_vec(tri_mat, S.tri_mat);
+    copy_vec(lights, S.light_tris);
+    return RB_OK;
+}
+
+rb_status rb_scene_shading_counts(rb_scene_t s, uint32_t* nn, uint32_t* ntc, int* has_nidx, int* has_tidx) {
+    RB_CHECK_ARG(s, "NULL scene");
+    const Scene& S = s->s;
+    if (nn) *nn = (uint32_t)(S.normals.size() / 3);
+    if (ntc) *ntc = (uint32_t)(S.texcoords.size() / 2);
+    if (has_nidx) *has_nidx = S.nidx.empty() ? 0 : 1;
+    if (has_tidx) *has_tidx = S.tidx.empty() ? 0 : 1;
+    return RB_OK;
+}
+
+rb_status rb_scene_shading(rb_scene_t s, float* normals, uint32_t* nidx, float* texcoords, uint32_t* tidx) {
+    RB_CHECK_ARG(s, "NULL scene");
+    const Scene& S = s->s;
+    copy_vec(normals, S.normals);
+    copy_vec(nidx, S.nidx);
+    copy_vec(texcoords, S.texcoords);
+    copy_vec(tidx, S.tidx);
     return RB_OK;
 }
 
 rb_status rb_scene_materials(rb_scene_t s, rb_material* out) {
     RB_CHECK_ARG(s && out, "NULL argument");
-    std::memcpy(out, s->s.mats.data(), s->s.mats.size() * sizeof(rb_material));
+    copy_vec(out, s->s.mats);
     return RB_OK;
 }
 
@@ -292,9 +420,17 @@ extern "C" rb_status rb_scene_generate(rb_scene_t s, const rb_gen_params* p) {
                 if ((st = add_material_mix(s, g, p, &mat)) != RB_OK) return st;
                 v.resize(m.v.size());
                 for (size_t k = 0; k < m.v.size(); k++) v[k] = (float)(c[k % 3] + r * m.v[k]);
-                if ((st = rb_scene_add_mesh(s, v.data(), (uint32_t)(v.size() / 3), m.t.data(),
-                                            (uint32_t)(m.t.size() / 3), mat)) != RB_OK)
-                    return st;
+                if (p->smooth) {
+                    // the unit-sphere positions are the sphere's outward normals
+                    std::vector<float> nrm(m.v.size());
+                    for (size_t k = 0; k < m.v.size(); k++) nrm[k] = (float)m.v[k];
+                    st = append_mesh(s->s, v.data(), (uint32_t)(v.size() / 3), m.t.data(), (uint32_t)(m.t.size() / 3),
+                                     nrm.data(), (uint32_t)(nrm.size() / 3), nullptr, nullptr, 0, nullptr, nullptr, mat);
+                } else {
+                    st = rb_scene_add_mesh(s, v.data(), (uint32_t)(v.size() / 3), m.t.data(), (uint32_t)(m.t.size() / 3),
+                                           mat);
+                }
+                if (st != RB_OK) return st;
             }
             return RB_OK;
         }
@@ -440,7 +576,7 @@ static rb_status copy_out(const void* src, size_t bytes, void* dst, size_t* cap)
     if (!cap) return RB_OK;
     if (dst) {
         if (*cap < bytes) return fail(RB_ERR_INVALID, "output buffer too small (%zu < %zu bytes)", *cap, bytes);
-        std::memcpy(dst, src, bytes);
+        if (bytes) std::memcpy(dst, src, bytes);
     }
     *cap = bytes;
     return RB_OK;

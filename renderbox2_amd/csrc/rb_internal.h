@@ -63,15 +63,48 @@ struct Box {
 
 // ---------------------------------------------------------------- scene --
 
+// Index of a triangle corner that has no shading normal / texture coordinate.
+constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
+
 struct Scene {
     std::vector<float> verts;          // xyz per vertex
     std::vector<uint32_t> idx;         // 3 per triangle (scene-global vertex ids)
     std::vector<uint32_t> tri_mat;     // material per triangle
     std::vector<rb_material> mats;
     std::vector<uint32_t> light_tris;  // triangle ids of emissive triangles
+    // Shading data of Scene (src/core/scene.h:142-147): per-corner indices
+    // into their own arrays, 3 per triangle, kNoIndex for a triangle without
+    // any; an empty index array means no triangle has that data.
+    std::vector<float> normals;        // xyz per shading normal
+    std::vector<uint32_t> nidx;
+    std::vector<float> texcoords;      // uv per texture coordinate
+    std::vector<uint32_t> tidx;
     uint32_t num_tris() const { return (uint32_t)(idx.size() / 3); }
     uint32_t num_verts() const { return (uint32_t)(verts.size() / 3); }
+    bool tri_has_normals(uint32_t t) const { return !nidx.empty() && nidx[3 * (size_t)t] != kNoIndex; }
 };
+
+// Scene::add_mesh (src/core/scene.cpp:83-123) with its checks: vertices and
+// triangles (indices local to the mesh), optional normals / texcoords with
+// per-corner index triples (NULL indices with normals / texcoords: one per
+// vertex, indexed like the vertices; a kNoIndex triple: that triangle has
+// none), a material per triangle (tri_mat) or one for all.  Emissive
+// triangles become lights.  Nothing is added when a check fails.
+rb_status append_mesh(Scene& S, const float* verts, uint32_t nv, const uint32_t* idx, uint32_t nt,
+                      const float* normals, uint32_t nn, const uint32_t* nidx, const float* texcoords, uint32_t ntc,
+                      const uint32_t* tidx, const uint32_t* tri_mat, uint32_t material_id);
+
+// rb_scene_add_material's checks and derived parameters (Oren-Nayar A, B;
+// unused layer slots cleared), in place; a layered material's layer ids are
+// looked up in `mats`.
+rb_status prepare_material(const std::vector<rb_material>& mats, rb_material* m);
+
+// Copies a vector into a caller array (nothing for an empty vector, whose
+// data() may be NULL).
+template <class T>
+inline void copy_vec(void* dst, const std::vector<T>& v) {
+    if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(T));
+}
 
 // Limits every path that grows a scene checks before it reads or copies
 // anything: at most 2^32 - 1 vertices and 2^31 - 1 triangles (triangle ids

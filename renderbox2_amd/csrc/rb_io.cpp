@@ -25,9 +25,10 @@
 // (<name> of an OBJ object, <Le>{r,g,b}</Le>) making that object a diffuse
 // emitter, and <camera> (<position>, <lookat>, <up>, <fov>).  Materials are
 // numbered in file order, then one emitter material per light, as the
-// reference numbers them.  Single-layer lambertian, mirror, glass,
-// orennayar and (Blinn) microfacet bxdfs map to the renderer's materials;
-// layered materials are refused with RB_ERR_UNSUPPORTED.  The camera's <fov> is used when present
+// reference numbers them.  Lambertian, mirror, glass, orennayar and (Blinn)
+// microfacet bxdfs map to the renderer's materials; a material with 2..4
+// <bxdf> layers becomes an RB_MAT_LAYERED material whose layers are added
+// after the emitters.  The camera's <fov> is used when present
 // (the reference hard-codes 90 degrees, customfilereader.cpp:331-337).
 #include <cstdio>
 #include <cstdlib>
@@ -187,20 +188,48 @@ bool parse_float3(const std::string& t, float out[3]) {
 
 struct ObjObject {
     std::string name;
-    std::vector<uint32_t> tris;  // file-global 0-based vertex indices
+    std::vector<uint32_t> tris;   // file-global 0-based vertex indices
+    std::vector<uint32_t> ntris;  // file-global normal indices per corner (kNoIndex: none)
+    std::vector<uint32_t> ttris;  // file-global texcoord indices per corner (kNoIndex: none)
 };
 
-bool resolve_index(const char* tok, size_t nv, uint32_t* out) {
-    char* end = nullptr;
-    long v = std::strtol(tok, &end, 10);
-    if (end == tok || v == 0) return false;
-    long r = v > 0 ? v - 1 : (long)nv + v;
-    if (r < 0 || (size_t)r >= nv) return false;
+struct ObjData {
+    std::vector<float> verts, normals, texcoords;
+    std::vector<ObjObject> objs;
+};
+
+// One OBJ index (1-based, or negative = relative to the end of the list so
+// far) resolved against a list of n entries.
+bool resolve_index(const char* tok, const char* end, size_t n, uint32_t* out) {
+    char* e = nullptr;
+    long v = std::strtol(tok, &e, 10);
+    if (e == tok || e != end || v == 0) return false;
+    long r = v > 0 ? v - 1 : (long)n + v;
+    if (r < 0 || (size_t)r >= n) return false;
     *out = (uint32_t)r;
     return true;
 }
 
-rb_status read_obj(const std::string& path, std::vector<float>& verts, std::vector<ObjObject>& objs) {
+// A face corner: v, v/vt, v//vn or v/vt/vn (ObjFileReader::parse_face,
+// src/io/objfilereader.cpp:57-108, which reads v//vn and v/vt the other way
+// round -- a slip not kept).  Missing fields give kNoIndex.
+bool parse_corner(const std::string& c, const ObjData& D, uint32_t* v, uint32_t* vt, uint32_t* vn) {
+    const size_t s1 = c.find('/');
+    const size_t s2 = s1 == std::string::npos ? std::string::npos : c.find('/', s1 + 1);
+    const char* b = c.c_str();
+    const size_t vend = s1 == std::string::npos ? c.size() : s1;
+    if (!resolve_index(b, b + vend, D.verts.size() / 3, v)) return false;
+    *vt = *vn = kNoIndex;
+    if (s1 == std::string::npos) return true;
+    const size_t tend = s2 == std::string::npos ? c.size() : s2;
+    if (tend > s1 + 1 && !resolve_index(b + s1 + 1, b + tend, D.texcoords.size() / 2, vt)) return false;
+    if (s2 == std::string::npos) return tend > s1 + 1;  // "v/" is malformed
+    return resolve_index(b + s2 + 1, b + c.size(), D.normals.size() / 3, vn);
+}
+
+rb_status read_obj(const std::string& path, ObjData& D) {
+    std::vector<float>& verts = D.verts;
+    std::vector<ObjObject>& objs = D.objs;
     std::ifstream in(path);
     if (!in) return fail(RB_ERR_INVALID, "cannot open %s", path.c_str());
     std::string line;
@@ -223,59 +252,84 @@ rb_status read_obj(const std::string& path, std::vector<float>& verts, std::vect
             ObjObject o;
             ls >> o.name;
             objs.push_back(std::move(o));
-        } else if (tag == "v") {
+        } else if (tag == "v" || tag == "vn") {
             float x, y, z;
-            if (!(ls >> x >> y >> z)) return fail(RB_ERR_INVALID, "%s:%zu: bad vertex", path.c_str(), lineno);
-            verts.push_back(x);
-            verts.push_back(y);
-            verts.push_back(z);
+            if (!(ls >> x >> y >> z))
+                return fail(RB_ERR_INVALID, "%s:%zu: bad %s", path.c_str(), lineno, tag == "v" ? "vertex" : "normal");
+            std::vector<float>& dst = tag == "v" ? verts : D.normals;
+            dst.push_back(x);
+            dst.push_back(y);
+            dst.push_back(z);
+        } else if (tag == "vt") {
+            float u, v = 0.0f;  // a 1D coordinate gets v = 0; a third (w) is ignored
+            if (!(ls >> u)) return fail(RB_ERR_INVALID, "%s:%zu: bad texture coordinate", path.c_str(), lineno);
+            if (!(ls >> v)) v = 0.0f;
+            D.texcoords.push_back(u);
+            D.texcoords.push_back(v);
         } else if (tag == "f") {
             poly.clear();
             std::string c;
-            const size_t nv = verts.size() / 3;
             while (ls >> c) {
-                // v, v/vt, v//vn, v/vt/vn: the vertex index is the first field
-                uint32_t vi;
-                if (!resolve_index(c.c_str(), nv, &vi))
-                    return fail(RB_ERR_INVALID, "%s:%zu: face corner '%s' names no vertex", path.c_str(), lineno,
-                                c.c_str());
+                uint32_t vi, ti, ni;
+                if (!parse_corner(c, D, &vi, &ti, &ni))
+                    return fail(RB_ERR_INVALID, "%s:%zu: face corner '%s' is malformed or names no entry", path.c_str(),
+                                lineno, c.c_str());
                 poly.push_back(vi);
+                poly.push_back(ti);
+                poly.push_back(ni);
             }
-            if (poly.size() < 3) return fail(RB_ERR_INVALID, "%s:%zu: face with %zu corners", path.c_str(), lineno,
-                                             poly.size());
+            if (poly.size() < 9) return fail(RB_ERR_INVALID, "%s:%zu: face with %zu corners", path.c_str(), lineno,
+                                             poly.size() / 3);
             ObjObject& o = current();
-            for (size_t k = 1; k + 1 < poly.size(); k++) {
-                o.tris.push_back(poly[0]);
-                o.tris.push_back(poly[k]);
-                o.tris.push_back(poly[k + 1]);
+            const size_t nc = poly.size() / 3;
+            // a triangle has normals (texcoords) only when all three corners do
+            for (size_t k = 1; k + 1 < nc; k++) {
+                const size_t cs[3] = {0, k, k + 1};
+                bool has_t = true, has_n = true;
+                for (size_t c3 : cs) {
+                    o.tris.push_back(poly[3 * c3]);
+                    has_t = has_t && poly[3 * c3 + 1] != kNoIndex;
+                    has_n = has_n && poly[3 * c3 + 2] != kNoIndex;
+                }
+                for (size_t c3 : cs) {
+                    o.ttris.push_back(has_t ? poly[3 * c3 + 1] : kNoIndex);
+                    o.ntris.push_back(has_n ? poly[3 * c3 + 2] : kNoIndex);
+                }
             }
         }
-        // vn, vt, g, s, usemtl, mtllib: not used by the renderer
+        // g, s, usemtl, mtllib: not used by the renderer
     }
     return RB_OK;
 }
 
-rb_status add_obj_to_scene(Scene& S, const std::vector<float>& verts, const std::vector<ObjObject>& objs,
-                           const std::vector<uint32_t>& obj_mat) {
-    const uint32_t base = S.num_verts();
+// Adds every object of a read file as a mesh of the file's vertices,
+// normals and texture coordinates (file-global indexing, Scene::add_mesh,
+// src/core/scene.cpp:83-113), object k with material obj_mat[k].
+rb_status add_obj_to_scene(Scene& S, const ObjData& D, const std::vector<uint32_t>& obj_mat) {
+    std::vector<uint32_t> idx, nidx, tidx, mats;
     size_t ntris = 0;
-    for (const ObjObject& o : objs) ntris += o.tris.size() / 3;
-    rb_status st = scene_room(S, verts.size() / 3, ntris);
+    for (const ObjObject& o : D.objs) ntris += o.tris.size() / 3;
+    rb_status st = scene_room(S, D.verts.size() / 3, ntris);  // before anything is copied
     if (st != RB_OK) return st;
     try {
-        S.verts.insert(S.verts.end(), verts.begin(), verts.end());
-        for (size_t k = 0; k < objs.size(); k++) {
-            const uint32_t tbase = S.num_tris();
-            const uint32_t n = (uint32_t)(objs[k].tris.size() / 3);
-            for (uint32_t v : objs[k].tris) S.idx.push_back(base + v);
-            S.tri_mat.insert(S.tri_mat.end(), n, obj_mat[k]);
-            if (S.mats[obj_mat[k]].kind == RB_MAT_EMITTER)
-                for (uint32_t t = 0; t < n; t++) S.light_tris.push_back(tbase + t);
+        idx.reserve(3 * ntris);
+        nidx.reserve(3 * ntris);
+        tidx.reserve(3 * ntris);
+        mats.reserve(ntris);
+        for (size_t k = 0; k < D.objs.size(); k++) {
+            const ObjObject& o = D.objs[k];
+            idx.insert(idx.end(), o.tris.begin(), o.tris.end());
+            nidx.insert(nidx.end(), o.ntris.begin(), o.ntris.end());
+            tidx.insert(tidx.end(), o.ttris.begin(), o.ttris.end());
+            mats.insert(mats.end(), o.tris.size() / 3, obj_mat[k]);
         }
     } catch (...) {
-        return fail(RB_ERR_OOM, "scene growth failed");
+        return fail(RB_ERR_OOM, "OBJ read failed (host allocation)");
     }
-    return RB_OK;
+    const uint32_t nn = (uint32_t)(D.normals.size() / 3), ntc = (uint32_t)(D.texcoords.size() / 2);
+    return append_mesh(S, D.verts.data(), (uint32_t)(D.verts.size() / 3), idx.data(), (uint32_t)ntris,
+                       nn ? D.normals.data() : nullptr, nn, nn ? nidx.data() : nullptr,
+                       ntc ? D.texcoords.data() : nullptr, ntc, ntc ? tidx.data() : nullptr, mats.data(), 0);
 }
 
 std::string dir_of(const std::string& p) {
@@ -293,16 +347,15 @@ extern "C" {
 rb_status rb_scene_load_obj(rb_scene_t s, const char* path, uint32_t material_id) {
     if (!s || !path) return fail(RB_ERR_INVALID, "NULL argument");
     if (material_id >= s->s.mats.size()) return fail(RB_ERR_INVALID, "material id out of range (add the material first)");
-    std::vector<float> verts;
-    std::vector<ObjObject> objs;
+    ObjData D;
     rb_status st;
     try {
-        st = read_obj(path, verts, objs);
+        st = read_obj(path, D);
     } catch (...) {
         return fail(RB_ERR_OOM, "OBJ read failed (host allocation)");
     }
     if (st != RB_OK) return st;
-    return add_obj_to_scene(s->s, verts, objs, std::vector<uint32_t>(objs.size(), material_id));
+    return add_obj_to_scene(s->s, D, std::vector<uint32_t>(D.objs.size(), material_id));
 }
 
 rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint32_t height, rb_camera* cam) {
@@ -321,33 +374,24 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
     if (count && std::atoi(count->text.c_str()) != 1)
         return fail(RB_ERR_UNSUPPORTED, "%s: one mesh file per scene (count %s)", path, count->text.c_str());
 
-    // materials in file order (customfilereader.cpp:89-225), then one emitter per light (:254-280)
+    // materials in file order (customfilereader.cpp:89-225), then one emitter
+    // per light (:254-280), then the layers of the layered materials
     Scene& S = s->s;
     const uint32_t mat_base = (uint32_t)S.mats.size();
     std::vector<rb_material> mats;
     std::map<std::string, uint32_t> by_name;
-    for (const XNode& m : doc.kids) {
-        if (m.name != "material") continue;
-        const XNode* name = m.child("name");
-        const XNode* bsdf = m.child("bsdf");
-        if (!name || !bsdf) return fail(RB_ERR_INVALID, "%s: <material> needs <name> and <bsdf>", path);
-        std::vector<const XNode*> bxdfs;
-        for (const XNode& k : bsdf->kids)
-            if (k.name == "bxdf") bxdfs.push_back(&k);
-        if (bxdfs.size() != 1)
-            return fail(RB_ERR_UNSUPPORTED, "%s: material '%s' has %zu layers; only single-layer materials are supported",
-                        path, name->text.c_str(), bxdfs.size());
-        const XNode* bx = bxdfs[0];
+    // one <bxdf> (customfilereader.cpp:113-219)
+    auto parse_bxdf = [&](const XNode* bx, const std::string& mname, rb_material* out) -> rb_status {
         const XNode* type = bx->child("type");
         const XNode* refl = bx->child("reflectance");
         rb_material mat{};
-        if (!type) return fail(RB_ERR_INVALID, "%s: material '%s' bxdf without <type>", path, name->text.c_str());
-        auto num = [&](const char* tag, float* out) -> bool {
+        if (!type) return fail(RB_ERR_INVALID, "%s: material '%s' bxdf without <type>", path, mname.c_str());
+        auto num = [&](const char* tag, float* v) -> bool {
             const XNode* e = bx->child(tag);
             if (!e) return false;
             std::istringstream is(e->text);
             is.imbue(std::locale::classic());
-            return bool(is >> *out);
+            return bool(is >> *v);
         };
         bool ok = true;
         if (type->text == "lambertian") {
@@ -365,7 +409,7 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
             const XNode* dist = bx->child("distribution");
             if (dist && dist->text != "blinn")
                 return fail(RB_ERR_UNSUPPORTED, "%s: material '%s': microfacet distribution '%s' (blinn only)", path,
-                            name->text.c_str(), dist->text.c_str());
+                            mname.c_str(), dist->text.c_str());
             ok = num("exponent", &mat.param[0]);
             const XNode* ft = bx->child("fresnel_type");
             const std::string f = ft ? ft->text : "fresnel_noop";
@@ -380,19 +424,52 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
             } else if (f == "fresnel_noop") {
                 mat.param[1] = (float)RB_FRESNEL_NONE;
             } else {
-                return fail(RB_ERR_INVALID, "%s: material '%s': unknown fresnel_type '%s'", path, name->text.c_str(),
+                return fail(RB_ERR_INVALID, "%s: material '%s': unknown fresnel_type '%s'", path, mname.c_str(),
                             f.c_str());
             }
         } else {
-            return fail(RB_ERR_UNSUPPORTED, "%s: material '%s': bxdf type '%s' is not supported", path,
-                        name->text.c_str(), type->text.c_str());
+            return fail(RB_ERR_UNSUPPORTED, "%s: material '%s': bxdf type '%s' is not supported", path, mname.c_str(),
+                        type->text.c_str());
         }
-        if (!ok) return fail(RB_ERR_INVALID, "%s: material '%s': missing or bad %s parameters", path,
-                             name->text.c_str(), type->text.c_str());
+        if (!ok) return fail(RB_ERR_INVALID, "%s: material '%s': missing or bad %s parameters", path, mname.c_str(),
+                             type->text.c_str());
         if (!refl || !parse_float3(refl->text, mat.color))
-            return fail(RB_ERR_INVALID, "%s: material '%s' needs a <reflectance> triple", path, name->text.c_str());
+            return fail(RB_ERR_INVALID, "%s: material '%s' needs a <reflectance> triple", path, mname.c_str());
+        *out = mat;
+        return RB_OK;
+    };
+    std::vector<std::vector<rb_material>> layered;  // the layers of each named material with more than one
+    for (const XNode& m : doc.kids) {
+        if (m.name != "material") continue;
+        const XNode* name = m.child("name");
+        const XNode* bsdf = m.child("bsdf");
+        if (!name || !bsdf) return fail(RB_ERR_INVALID, "%s: <material> needs <name> and <bsdf>", path);
+        std::vector<rb_material> layers;
+        for (const XNode& k : bsdf->kids) {
+            if (k.name != "bxdf") continue;
+            rb_material l;
+            const rb_status st = parse_bxdf(&k, name->text, &l);
+            if (st != RB_OK) return st;
+            layers.push_back(l);
+        }
+        // <layers> gives the count (material.num_valid_layers, :104-110)
+        const XNode* nl = bsdf->child("layers");
+        if (nl && std::atoi(nl->text.c_str()) != (int)layers.size())
+            return fail(RB_ERR_INVALID, "%s: material '%s': <layers> says %s, %zu <bxdf> given", path,
+                        name->text.c_str(), nl->text.c_str(), layers.size());
+        if (layers.empty() || layers.size() > 4)
+            return fail(RB_ERR_INVALID, "%s: material '%s' has %zu layers (1..4)", path, name->text.c_str(),
+                        layers.size());
         by_name[name->text] = mat_base + (uint32_t)mats.size();
-        mats.push_back(mat);
+        if (layers.size() == 1) {
+            mats.push_back(layers[0]);
+        } else {
+            rb_material lm{};
+            lm.kind = RB_MAT_LAYERED;
+            lm.param[0] = (float)layers.size();
+            mats.push_back(lm);  // layer ids set below, once the lights are numbered
+            layered.push_back(layers);
+        }
     }
     std::map<std::string, uint32_t> light_mat;
     if (const XNode* lo = doc.child("lightobjects")) {
@@ -409,12 +486,20 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
         }
     }
     if (mats.empty()) return fail(RB_ERR_INVALID, "%s: no materials", path);
+    for (size_t i = 0, li = 0; i < mats.size(); i++) {
+        if (mats[i].kind != RB_MAT_LAYERED) continue;
+        const std::vector<rb_material>& layers = layered[li++];
+        for (size_t k = 0; k < layers.size(); k++) {
+            mats[i].param[1 + k] = (float)(mat_base + mats.size());
+            mats.push_back(layers[k]);
+        }
+    }
 
-    std::vector<float> verts;
-    std::vector<ObjObject> objs;
+    ObjData D;
+    std::vector<ObjObject>& objs = D.objs;
     rb_status st;
     try {
-        st = read_obj(dir_of(path) + file->text, verts, objs);
+        st = read_obj(dir_of(path) + file->text, D);
     } catch (...) {
         return fail(RB_ERR_OOM, "OBJ read failed (host allocation)");
     }
@@ -455,13 +540,18 @@ rb_status rb_scene_load_xml(rb_scene_t s, const char* path, uint32_t width, uint
         c.width = width;
         c.height = height;
     }
-    for (const rb_material& m : mats) {  // validation and derived parameters as for any material
-        if ((st = rb_scene_add_material(s, &m, nullptr)) != RB_OK) {
-            S.mats.resize(mat_base);
-            return st;
-        }
+    // validation and derived parameters as for any material: the single
+    // BSDFs first, then the layered ones against the full list
+    for (rb_material& m : mats)
+        if (m.kind != RB_MAT_LAYERED && (st = prepare_material(S.mats, &m)) != RB_OK) return st;
+    {
+        std::vector<rb_material> all(S.mats);
+        all.insert(all.end(), mats.begin(), mats.end());
+        for (rb_material& m : mats)
+            if (m.kind == RB_MAT_LAYERED && (st = prepare_material(all, &m)) != RB_OK) return st;
     }
-    if ((st = add_obj_to_scene(S, verts, objs, obj_mat)) != RB_OK) {
+    S.mats.insert(S.mats.end(), mats.begin(), mats.end());
+    if ((st = add_obj_to_scene(S, D, obj_mat)) != RB_OK) {
         S.mats.resize(mat_base);
         return st;
     }

@@ -8,6 +8,7 @@
 //    (src/accelerators/sbvh/kepler_dynamic_fetch.h:312-360) as its own pass.
 #include "rb_device.h"
 #include "rb_sample.h"
+#include "rb_shade.h"
 
 namespace rb {
 
@@ -87,8 +88,8 @@ __global__ void k_raygen_diffuse(const float4* __restrict__ prim, const int2* __
     }
 }
 
-__global__ void k_fill(const float4* __restrict__ rays, const int2* __restrict__ hits, uint32_t n,
-                       const float4* __restrict__ tri_v, rb_isect_soa o) {
+__global__ void k_fill(const float4* __restrict__ rays, const int2* __restrict__ hits, uint32_t n, SceneDev S,
+                       rb_isect_soa o) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int2 h = hits[i];
         const bool hit = h.x >= 0;
@@ -97,32 +98,34 @@ __global__ void k_fill(const float4* __restrict__ rays, const int2* __restrict__
         if (!hit) continue;
         const float4 ro = rays[2 * (size_t)i], rd = rays[2 * (size_t)i + 1];
         const float t = __int_as_float(h.y);
-        const float4 a = tri_v[3 * (size_t)h.x], b = tri_v[3 * (size_t)h.x + 1], c = tri_v[3 * (size_t)h.x + 2];
-        const V3 v0 = v3(a.x, a.y, a.z), v1 = v3(b.x, b.y, b.z), v2 = v3(c.x, c.y, c.z);
+        V3 v0, v1, v2;
+        uint32_t mid, tf;
+        tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
         const V3 d = v3(rd.x, rd.y, rd.z);
         const V3 p = vadd(v3(ro.x, ro.y, ro.z), vmul(d, t));
         const V3 ng = tri_normal(v0, v1, v2);
+        V3 ns;
+        interp_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
         if (o.primitive_id) o.primitive_id[i] = (uint32_t)h.x;
-        if (o.material_id) o.material_id[i] = __float_as_uint(a.w);
+        if (o.material_id) o.material_id[i] = mid;
         for (int k = 0; k < 3; k++) {
             const float pv = k == 0 ? p.x : (k == 1 ? p.y : p.z);
             const float nv = k == 0 ? ng.x : (k == 1 ? ng.y : ng.z);
+            const float sv = k == 0 ? ns.x : (k == 1 ? ns.y : ns.z);
             const float dv = k == 0 ? d.x : (k == 1 ? d.y : d.z);
             if (o.position[k]) o.position[k][i] = pv;
             if (o.geometric_normal[k]) o.geometric_normal[k][i] = nv;
-            if (o.shading_normal[k]) o.shading_normal[k][i] = nv;
+            if (o.shading_normal[k]) o.shading_normal[k][i] = sv;
             if (o.incoming_direction[k]) o.incoming_direction[k][i] = -dv;
         }
         if (o.front) o.front[i] = vdot(ng, d) < 0.0f ? 1 : 0;
         if (o.epsilon) o.epsilon[i] = 1.0e-3f * t;
         if (o.uv) {
-            // barycentrics of p: weights of v0 and v1 (the Woop u, v convention)
-            const V3 e0 = vsub(v0, v2), e1 = vsub(v1, v2), q = vsub(p, v2);
-            const float d00 = vdot(e0, e0), d01 = vdot(e0, e1), d11 = vdot(e1, e1);
-            const float d20 = vdot(q, e0), d21 = vdot(q, e1);
-            const float den = d00 * d11 - d01 * d01;
-            const float u = den != 0.0f ? (d11 * d20 - d01 * d21) / den : 0.0f;
-            const float v = den != 0.0f ? (d00 * d21 - d01 * d20) / den : 0.0f;
+            // barycentrics of p: weights of v0 and v1 (the Woop u, v
+            // convention), of the hit triangle (the reference stores those of
+            // the last triangle tested)
+            float u, v;
+            bary_uv(p, v0, v1, v2, &u, &v);
             o.uv[2 * (size_t)i] = u;
             o.uv[2 * (size_t)i + 1] = v;
         }
@@ -161,8 +164,12 @@ static rb_status launch_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint3
                              rb_hit* hits, const rb_isect_soa* out, hipStream_t s, uint32_t* counter) {
     rb_status st = launch_trace(bvh, rays, n, flags, mask, hits, nullptr, s, dev().cfg, nullptr, counter);
     if (st != RB_OK) return st;
+    SceneDev S;
+    std::memset(&S, 0, sizeof(S));
+    S.tri_v = bvh->tri_v;
+    S.tri_n = bvh->tri_n;
     hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, s, reinterpret_cast<const float4*>(rays),
-                       reinterpret_cast<const int2*>(hits), n, bvh->tri_v, *out);
+                       reinterpret_cast<const int2*>(hits), n, S, *out);
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? RB_OK : hip_fail(e, "fill launch");
 }

@@ -63,9 +63,13 @@ __global__ void k_ao_spawn(const float4* __restrict__ prim, const int2* __restri
         const V3 dv = v3(d.x, d.y, d.z);
         const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
         V3 v0, v1, v2;
-        tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, nullptr);
-        V3 n = tri_normal(v0, v1, v2);
-        if (vdot(n, dv) > 0.0f) n = vneg(n);
+        uint32_t tf;
+        tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, nullptr, &tf);
+        // about the shading normal (integrator_ao.cu:55), turned to face the ray
+        const V3 ng = tri_normal(v0, v1, v2);
+        V3 n;
+        shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &n);
+        if (vdot(ng, dv) > 0.0f) n = vneg(n);
         const U4 r = rng.draw(sample_gid(T, i), k, 0, 0);
         const V3 dir = cosine_hemisphere(n, u01(r.x), u01(r.y));
         rays[2 * j] = make_float4(p.x, p.y, p.z, 1.0e-3f * t);
@@ -208,12 +212,14 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
             p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
             eps = 1.0e-3f * t;
             V3 v0, v1, v2;
-            uint32_t mid;
-            tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid);
+            uint32_t mid, tf;
+            tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
             const rb_material m = S.mats[mid];
             const V3 ng = tri_normal(v0, v1, v2);
+            V3 ns;  // shading normal on ng's side (ng itself for a flat triangle)
+            shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
             const V3 wo = vneg(dv);
-            const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
+            const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
             const float4 Tp = P.T[pid];
             const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
             uint32_t cm = BD ? P.cmask[pid] : 0u;
@@ -231,9 +237,11 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
                 const uint32_t gid = sample_gid(T, pid);
                 const U4 r0 = rng.draw(gid, bounce, 0, 0);
                 const U4 r1 = rng.draw(gid, bounce, 1, 0);
+                // layer picks of a layered material: next bounce, MIS sample
+                const bool layered = m.kind == RB_MAT_LAYERED;
+                const U4 r2 = layered ? rng.draw(gid, bounce, 2, 0) : U4{0u, 0u, 0u, 0u};
                 V3 ca = v3(0.f, 0.f, 0.f), cb = v3(0.f, 0.f, 0.f);
-                const bool delta = is_delta(m);
-                if (BD && delta) cm |= 1u << bounce;
+                const bool delta = mat_delta(S, m);
                 const float wdir = BD ? technique_weight(cm, bounce + 2, max_depth, B.tdepth) : 1.0f;
                 if (S.n_lights > 0) {
                     uint32_t lid = (uint32_t)(u01(r0.x) * (float)S.n_lights);
@@ -253,11 +261,11 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
                     const float lpdf = light_pdf(p, wi, l0, l1, l2, ln, area);
                     const V3 Li = vdot(ln, vneg(wi)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
                     if (!delta && !is_black(Li) && lpdf > 0.0f) {
-                        const V3 f = bsdf_f(m, wo, wi, ng, nf);
+                        const V3 f = mat_f(S, m, wo, wi, ng, nf);
                         if (!is_black(f)) {
-                            const float bpdf = bsdf_pdf(m, wo, wi, nf);
+                            const float bpdf = mat_pdf_ns(S, m, wo, wi, nf);
                             const float w = lpdf / (lpdf + bpdf);
-                            const float s = fabsf(vdot(wi, ng)) * w;
+                            const float s = fabsf(vdot(wi, ns)) * w;
                             ca = v3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
                             const V3 seg = vsub(pl, p);
                             const float dist = sqrtf(vdot(seg, seg));
@@ -271,13 +279,13 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
                     if (!delta) {
                         V3 wi2, f2;
                         float pdf2;
-                        bsdf_sample(m, wo, ng, nf, u01(r0.w), u01(r1.x), &wi2, &f2, &pdf2);
+                        mat_sample_ns(S, m, wo, ng, nf, u01(r2.y), u01(r0.w), u01(r1.x), &wi2, &f2, &pdf2);
                         if (!is_black(f2) && pdf2 > 0.0f) {
                             const float lpdf2 = light_pdf(p, wi2, l0, l1, l2, ln, area);
                             if (lpdf2 > 0.0f) {
                                 const float w2 = pdf2 / (pdf2 + lpdf2);
                                 const V3 Li2 = vdot(ln, vneg(wi2)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
-                                const float s2 = fabsf(vdot(wi2, ng)) * w2;
+                                const float s2 = fabsf(vdot(wi2, ns)) * w2;
                                 cb = v3(f2.x * Li2.x * s2 / pdf2, f2.y * Li2.y * s2 / pdf2, f2.z * Li2.z * s2 / pdf2);
                                 rb1 = make_float4(wi2.x, wi2.y, wi2.z, 3.402823466e+38f);
                                 wantB = true;
@@ -289,26 +297,15 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
                 V3 wi3, f3;
                 float pdf3;
                 bool spec3;
-                bool weight3 = false;  // f3 already holds f |cos| / pdf
-                if (!delta) {
-                    bsdf_sample(m, wo, ng, nf, u01(r1.y), u01(r1.z), &wi3, &f3, &pdf3);
-                    spec3 = false;
-                } else if (m.kind == RB_MAT_GLASS) {
-                    f3 = glass_sample(m, wo, ng, u01(r1.y), true, &wi3);
-                    pdf3 = 1.0f;
-                    spec3 = true;
-                    weight3 = true;
-                } else {  // mirror (sample_mirror, src/core/reflection.cu:76-92)
-                    const float c = 2.0f * vdot(nf, wo);
-                    wi3 = vsub(vmul(nf, c), wo);
-                    const float ac = fabsf(vdot(wi3, nf));
-                    f3 = v3(m.color[0] / ac, m.color[1] / ac, m.color[2] / ac);
-                    pdf3 = 1.0f;
-                    spec3 = true;
-                }
+                bool weight3;  // f3 already holds f |cos| / pdf
+                mat_sample(S, m, wo, ng, ns, nf, u01(r2.x), u01(r1.y), u01(r1.z), true, &wi3, &f3, &pdf3, &spec3,
+                           &weight3);
+                // the vertex counts as specular for the technique weights of
+                // the paths that continue through a specular lobe
+                if (BD && spec3) cm |= 1u << bounce;
                 uint32_t ninfo = info & 0x1FFu;
                 if (!is_black(f3) && pdf3 > 0.0f) {
-                    const float s3 = weight3 ? 1.0f : fabsf(vdot(wi3, ng));
+                    const float s3 = weight3 ? 1.0f : fabsf(vdot(wi3, ns));
                     V3 Tn = v3(Tv.x * (f3.x * s3 / pdf3), Tv.y * (f3.y * s3 / pdf3), Tv.z * (f3.z * s3 / pdf3));
                     bool alive = true;
                     if (bounce > rr_start) {
@@ -387,16 +384,20 @@ __global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_
             const V3 dv = v3(d.x, d.y, d.z);
             const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, th));
             V3 v0, v1, v2;
-            uint32_t mid;
-            tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid);
+            uint32_t mid, tf;
+            tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
             const rb_material m = S.mats[mid];
-            if (m.kind != RB_MAT_EMITTER && !is_delta(m)) {
+            if (m.kind != RB_MAT_EMITTER && !mat_delta(S, m)) {
                 const V3 ng = tri_normal(v0, v1, v2);
+                V3 ns;
+                shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
                 const V3 wo = vneg(dv);
-                const V3 nf = vdot(ng, wo) >= 0.0f ? ng : vneg(ng);
+                const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
                 const float4 Tp = P.T[pid];
                 const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
-                const uint32_t cm = P.cmask[pid];
+                // earlier vertices only: k_pt_shade has already marked this
+                // vertex when its next direction took a specular lobe
+                const uint32_t cm = P.cmask[pid] & ((1u << bounce) - 1u);
                 const uint32_t gid = sample_gid(T, pid);
                 const U4 rc = crng.draw(gid, bounce, 2 + i / 2, 0);
                 const uint32_t ua = (i & 1) ? rc.z : rc.x, ub = (i & 1) ? rc.w : rc.y;
@@ -408,7 +409,7 @@ __global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_
                     uint32_t k = (uint32_t)(u01(ub) * (float)nj);
                     if (k > nj - 1) k = nj - 1;
                     const size_t sl = (size_t)j * B.C.depth + k;
-                    const float4* rec = B.C.v + 4 * sl;
+                    const float4* rec = B.C.v + B.C.rows * sl;
                     const float4 xp = rec[0], xn = rec[1], xw = rec[2], xa = rec[3];
                     const rb_material mx = S.mats[__float_as_uint(xn.w)];
                     const V3 seg = vsub(v3(xp.x, xp.y, xp.z), p);
@@ -416,12 +417,22 @@ __global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_
                     const float dist = sqrtf(d2);
                     const V3 dir = vnorm(seg);
                     const V3 ngx = v3(xn.x, xn.y, xn.z);
-                    const V3 fy = bsdf_f(m, wo, dir, ng, nf);
+                    const V3 fy = mat_f(S, m, wo, dir, ng, nf);
                     const V3 xwv = v3(xw.x, xw.y, xw.z);
-                    const V3 nfx = vdot(ngx, xwv) >= 0.0f ? ngx : vneg(ngx);
-                    const V3 fx = bsdf_f(mx, xwv, vneg(dir), ngx, nfx);
+                    // the cache vertex's shading normal (row 4, when the scene
+                    // has normals); importance transport there takes the
+                    // adjoint factor
+                    const bool xsn = B.C.rows > 4 && __float_as_uint(xp.w) != 0u;
+                    V3 xs = ngx;
+                    if (xsn) {
+                        const float4 r4 = rec[4];
+                        xs = v3(r4.x, r4.y, r4.z);
+                    }
+                    const V3 nfx = vdot(ngx, xwv) >= 0.0f ? xs : vneg(xs);
+                    V3 fx = mat_f(S, mx, xwv, vneg(dir), ngx, nfx);
+                    if (xsn) fx = vmul(fx, adjoint_factor(xwv, vneg(dir), ngx, xs));
                     if (!is_black(fy) && !is_black(fx)) {
-                        const float G = fabsf(vdot(ng, dir)) * fabsf(vdot(ngx, vneg(dir))) / d2;
+                        const float G = fabsf(vdot(ns, dir)) * fabsf(vdot(xs, vneg(dir))) / d2;
                         // full path: camera vertices 0..bounce, then light vertices k+1..1
                         const uint32_t lmask = __float_as_uint(xw.w);
                         uint32_t M = cm;
@@ -512,7 +523,8 @@ __constant__ float kRaycastPalette[15][3] = {
 // Per sample debug colour of the closest hit (kernel_ray_cast_update,
 // integrator_raycast.cu:37-128); misses are black.  PRIMITIVE_ID: palette
 // entry of the triangle id mod 15; MATERIAL_ID: the material's colour;
-// NORMALS: the geometric normal cross(v1 - v0, v2 - v0), normalised;
+// NORMALS: the shading normal (interpolated vertex normals; for a triangle
+// without them the geometric normal cross(v1 - v0, v2 - v0), normalised);
 // UVS: the hit's barycentric weights of v0 and v1; UNIFORM: white.
 __global__ void k_raycast_shade(const float4* __restrict__ rays, const int2* __restrict__ hits, uint32_t n,
                                 SceneDev S, uint32_t mode, float4* __restrict__ L) {
@@ -522,16 +534,23 @@ __global__ void k_raycast_shade(const float4* __restrict__ rays, const int2* __r
         if (h.x >= 0) {
             const uint32_t t = (uint32_t)h.x;
             V3 v0, v1, v2;
-            uint32_t mat;
-            tri_fetch(S, t, &v0, &v1, &v2, &mat);
+            uint32_t mat, tf;
+            tri_fetch(S, t, &v0, &v1, &v2, &mat, &tf);
             if (mode == RB_SHADE_PRIMITIVE_ID) {
                 const uint32_t k = t % 15u;
                 c = v3(kRaycastPalette[k][0], kRaycastPalette[k][1], kRaycastPalette[k][2]);
             } else if (mode == RB_SHADE_MATERIAL_ID) {
-                const rb_material& m = S.mats[mat];
+                // a layered material shows its first layer's colour
+                // (integrator_raycast.cu:71-76)
+                const rb_material& ml = S.mats[mat];
+                const rb_material& m = ml.kind == RB_MAT_LAYERED ? S.mats[(uint32_t)ml.param[1]] : ml;
                 c = v3(m.color[0], m.color[1], m.color[2]);
             } else if (mode == RB_SHADE_NORMALS) {
-                c = tri_normal(v0, v1, v2);
+                // m_shading_normal (integrator_raycast.cu:109-115)
+                const float4 o = rays[2 * (size_t)i], d = rays[2 * (size_t)i + 1];
+                const float tt = __int_as_float(h.y);
+                const V3 p = vadd(v3(o.x, o.y, o.z), vmul(v3(d.x, d.y, d.z), tt));
+                interp_normal(S, t, tf, p, v0, v1, v2, tri_normal(v0, v1, v2), &c);
             } else if (mode == RB_SHADE_UVS) {
                 const float4 o = rays[2 * (size_t)i], d = rays[2 * (size_t)i + 1];
                 const float tt = __int_as_float(h.y);
@@ -748,6 +767,7 @@ static SceneDev scene_of(rb_bvh_t b) {
     S.mats = b->mats;
     S.light_tris = b->light_tris;
     S.n_lights = b->n_lights;
+    S.tri_n = b->tri_n;
     return S;
 }
 
@@ -919,7 +939,9 @@ uint64_t bdpt_scratch_bytes(uint32_t w, uint32_t h, const rb_render_params* p) {
     const uint64_t D = p->light_max_depth ? p->light_max_depth : 1;
     const uint64_t nl = p->light_paths ? p->light_paths : p->lvc_capacity / D;
     const uint64_t nc = p->connections;
-    return n * 352 + n * (4 + kMaxConn * 20) + n * nc * 40 + nl * D * 64 + nl * (32 + 16 + 4 + 8 + 4 + 8 + 32);
+    // cache records are 64 bytes, 80 in scenes with shading normals: the
+    // estimate (made without the scene) takes the larger
+    return n * 352 + n * (4 + kMaxConn * 20) + n * nc * 40 + nl * D * 80 + nl * (32 + 16 + 4 + 8 + 4 + 8 + 32);
 }
 
 rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
@@ -949,7 +971,8 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
     B.crng = rng_arg(p->seed, RNG_CONNECT);
     B.C.depth = D;
     B.C.paths = NL;
-    B.C.v = M.get<float4>(4 * (size_t)NL * D);
+    B.C.rows = bvh->tri_n ? 5u : 4u;
+    B.C.v = M.get<float4>((size_t)B.C.rows * NL * D);
     B.C.count = M.get<uint32_t>(NL);
     LightPath LP;
     LP.ray = M.get<float4>(2 * (size_t)NL);

@@ -8,18 +8,77 @@
 namespace rb {
 
 struct SceneDev {
-    const float4* __restrict__ tri_v;  // 3 float4 per triangle (v0|material, v1, v2)
+    const float4* __restrict__ tri_v;  // 3 float4 per triangle (v0|material, v1|flags, v2)
     const rb_material* __restrict__ mats;
     const uint32_t* __restrict__ light_tris;
     uint32_t n_lights;
+    // 3 float4 per triangle (n0, n1, n2 | 0) when the scene has shading
+    // normals, else nullptr; a triangle has them when its kTriHasNormals flag is set
+    const float4* __restrict__ tri_n;
 };
 
-__device__ __forceinline__ void tri_fetch(const SceneDev& S, uint32_t t, V3* v0, V3* v1, V3* v2, uint32_t* mat) {
+// Bits of a triangle's flags word (the w of its second vertex row).
+constexpr uint32_t kTriHasNormals = 1u;
+
+__device__ __forceinline__ void tri_fetch(const SceneDev& S, uint32_t t, V3* v0, V3* v1, V3* v2, uint32_t* mat,
+                                          uint32_t* flags = nullptr) {
     const float4 a = S.tri_v[3 * (size_t)t], b = S.tri_v[3 * (size_t)t + 1], c = S.tri_v[3 * (size_t)t + 2];
     *v0 = v3(a.x, a.y, a.z);
     *v1 = v3(b.x, b.y, b.z);
     *v2 = v3(c.x, c.y, c.z);
     if (mat) *mat = __float_as_uint(a.w);
+    if (flags) *flags = __float_as_uint(b.w);
+}
+
+// Barycentric weights u (of v0) and v (of v1; v2 takes 1 - u - v) of a
+// point p of the triangle's plane, from the normal equations of
+// p - v2 = u (v0 - v2) + v (v1 - v2): the Woop transform's u, v convention.
+RB_HD void bary_uv(V3 p, V3 v0, V3 v1, V3 v2, float* u, float* v) {
+    const V3 e0 = vsub(v0, v2), e1 = vsub(v1, v2), q = vsub(p, v2);
+    const float d00 = vdot(e0, e0), d01 = vdot(e0, e1), d11 = vdot(e1, e1);
+    const float d20 = vdot(q, e0), d21 = vdot(q, e1);
+    const float den = d00 * d11 - d01 * d01;
+    *u = den != 0.0f ? (d11 * d20 - d01 * d21) / den : 0.0f;
+    *v = den != 0.0f ? (d00 * d21 - d01 * d20) / den : 0.0f;
+}
+
+// Shading normal of triangle t at p (the IntersectionBuffer m_shading_normal,
+// kepler_dynamic_fetch.h:327-334): vertex normals weighted by p's
+// barycentrics, u n0 + v n1 + (1 - u - v) n2, normalised (the reference
+// weights n0, n0, n1 and does not normalise).  *ns = ng and false for a
+// triangle without normals or where the weighted sum vanishes.  Not
+// oriented: it points where the mesh's normals do.
+__device__ __forceinline__ bool interp_normal(const SceneDev& S, uint32_t t, uint32_t flags, V3 p, V3 v0, V3 v1, V3 v2,
+                                              V3 ng, V3* ns) {
+    *ns = ng;
+    if (!(flags & kTriHasNormals)) return false;
+    const float4 a = S.tri_n[3 * (size_t)t], b = S.tri_n[3 * (size_t)t + 1], c = S.tri_n[3 * (size_t)t + 2];
+    float u, v;
+    bary_uv(p, v0, v1, v2, &u, &v);
+    const float w = 1.0f - u - v;
+    const V3 s = v3(a.x * u + b.x * v + c.x * w, a.y * u + b.y * v + c.y * w, a.z * u + b.z * v + c.z * w);
+    const float l2 = vdot(s, s);
+    if (!(l2 > 0.0f)) return false;
+    *ns = vmul(s, 1.0f / sqrtf(l2));
+    return true;
+}
+
+// The shading normal the passes shade with: interp_normal turned to the
+// geometric normal's side.
+__device__ __forceinline__ bool shade_normal(const SceneDev& S, uint32_t t, uint32_t flags, V3 p, V3 v0, V3 v1, V3 v2,
+                                             V3 ng, V3* ns) {
+    const bool sn = interp_normal(S, t, flags, p, v0, v1, v2, ng, ns);
+    if (sn && vdot(*ns, ng) < 0.0f) *ns = vneg(*ns);
+    return sn;
+}
+
+// Importance transport's shading-normal factor at a light sub-path vertex
+// (Veach 1997, sec. 5.3): |wo.ns| |wi.ng| / (|wo.ng| |wi.ns|), wo towards the
+// light, wi the continuation; 0 where the denominator is.
+RB_HD float adjoint_factor(V3 wo, V3 wi, V3 ng, V3 ns) {
+    const float num = fabsf(vdot(wo, ns)) * fabsf(vdot(wi, ng));
+    const float den = fabsf(vdot(wo, ng)) * fabsf(vdot(wi, ns));
+    return den != 0.0f ? num / den : 0.0f;
 }
 
 RB_HD float vlen(V3 a) { return sqrtf(vdot(a, a)); }
@@ -266,6 +325,146 @@ RB_HD V3 glass_sample(const rb_material& m, V3 wo, V3 ng, float u, bool radiance
     }
     *wi = vnorm(to_world(fr, wil));
     return w;
+}
+
+// One sampled direction from a single-layer material: a non-specular BSDF
+// gives f and pdf at wi (bsdf_sample); glass gives its throughput factor f
+// |cos| / pdf in *f with *weight set; a mirror gives color / |cos| with pdf
+// 1 (material_sample_f's lobes, material.cu:98-135).
+RB_HD void layer_sample(const rb_material& m, V3 wo, V3 ng, V3 ns, V3 nf, float u1, float u2, bool radiance, V3* wi,
+                        V3* f, float* pdf, bool* spec, bool* weight) {
+    *weight = false;
+    if (!is_delta(m)) {
+        bsdf_sample(m, wo, ng, nf, u1, u2, wi, f, pdf);
+        *spec = false;
+        return;
+    }
+    *spec = true;
+    *pdf = 1.0f;
+    if (m.kind == RB_MAT_GLASS) {
+        *f = glass_sample(m, wo, ns, u1, radiance, wi);
+        *weight = true;
+        return;
+    }
+    // sample_mirror (src/core/reflection.cu:76-92)
+    const float c = 2.0f * vdot(nf, wo);
+    *wi = vsub(vmul(nf, c), wo);
+    const float ac = fabsf(vdot(*wi, nf));
+    *f = v3(m.color[0] / ac, m.color[1] / ac, m.color[2] / ac);
+}
+
+// ------------------------------------------------------ layered materials --
+// RB_MAT_LAYERED (Material's num_valid_layers BSDF layers,
+// src/core/material.h:44-52): the equal mixture of its n single-layer
+// materials, f = (1/n) sum f_i.  A sampled direction comes from one layer
+// picked uniformly (material.cu:80-95); a specular layer's sample keeps its
+// own throughput factor ((1/n) delta / (1/n)), a non-specular one has the
+// mixture's f and pdf (1/n) sum pdf_i over the non-specular layers at that
+// direction.  Light samples and connections evaluate the non-specular
+// layers; the light-sample MIS weight uses the pdf of the non-specular
+// sampling technique, (1/n_ns) sum pdf_i.  Single-layer materials take the
+// plain BSDF code paths unchanged.
+
+RB_HD uint32_t mat_layers(const rb_material& m) { return m.kind == RB_MAT_LAYERED ? (uint32_t)m.param[0] : 1u; }
+
+__device__ __forceinline__ rb_material mat_layer(const SceneDev& S, const rb_material& m, uint32_t k) {
+    return S.mats[(uint32_t)m.param[1 + k]];
+}
+
+// every layer specular: no light sample and no connection at this vertex
+__device__ __forceinline__ bool mat_delta(const SceneDev& S, const rb_material& m) {
+    if (m.kind != RB_MAT_LAYERED) return is_delta(m);
+    const uint32_t n = mat_layers(m);
+    for (uint32_t k = 0; k < n; k++)
+        if (!is_delta(mat_layer(S, m, k))) return false;
+    return true;
+}
+
+// the material's (non-specular) f for wo -> wi
+__device__ __forceinline__ V3 mat_f(const SceneDev& S, const rb_material& m, V3 wo, V3 wi, V3 ng, V3 nf) {
+    if (m.kind != RB_MAT_LAYERED) return bsdf_f(m, wo, wi, ng, nf);
+    const uint32_t n = mat_layers(m);
+    V3 s = v3(0.0f, 0.0f, 0.0f);
+    for (uint32_t k = 0; k < n; k++) s = vadd(s, bsdf_f(mat_layer(S, m, k), wo, wi, ng, nf));
+    const float fn = (float)n;
+    return v3(s.x / fn, s.y / fn, s.z / fn);
+}
+
+// sum of the non-specular layers' pdfs at wi, and how many there are
+__device__ __forceinline__ float mat_pdf_sum(const SceneDev& S, const rb_material& m, V3 wo, V3 wi, V3 nf,
+                                             uint32_t* count) {
+    const uint32_t n = mat_layers(m);
+    float s = 0.0f;
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < n; k++) {
+        const rb_material L = mat_layer(S, m, k);
+        if (is_delta(L)) continue;
+        s += bsdf_pdf(L, wo, wi, nf);
+        c++;
+    }
+    *count = c;
+    return s;
+}
+
+// pdf of the non-specular sampling technique at wi (the light-sample MIS weight)
+__device__ __forceinline__ float mat_pdf_ns(const SceneDev& S, const rb_material& m, V3 wo, V3 wi, V3 nf) {
+    if (m.kind != RB_MAT_LAYERED) return bsdf_pdf(m, wo, wi, nf);
+    uint32_t c;
+    const float s = mat_pdf_sum(S, m, wo, wi, nf, &c);
+    return c ? s / (float)c : 0.0f;
+}
+
+// A direction from the non-specular layers only (one picked uniformly with
+// upick), with the material's f and the technique's pdf there.
+__device__ __forceinline__ void mat_sample_ns(const SceneDev& S, const rb_material& m, V3 wo, V3 ng, V3 nf, float upick,
+                                              float u1, float u2, V3* wi, V3* f, float* pdf) {
+    if (m.kind != RB_MAT_LAYERED) {
+        bsdf_sample(m, wo, ng, nf, u1, u2, wi, f, pdf);
+        return;
+    }
+    const uint32_t n = mat_layers(m);
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < n; k++) c += is_delta(mat_layer(S, m, k)) ? 0u : 1u;
+    *f = v3(0.0f, 0.0f, 0.0f);
+    *pdf = 0.0f;
+    if (!c) return;
+    uint32_t j = (uint32_t)(upick * (float)c);
+    if (j > c - 1) j = c - 1;
+    rb_material L = m;
+    for (uint32_t k = 0, i = 0; k < n; k++) {
+        const rb_material Lk = mat_layer(S, m, k);
+        if (is_delta(Lk)) continue;
+        if (i++ == j) {
+            L = Lk;
+            break;
+        }
+    }
+    V3 fl;
+    float pl;
+    bsdf_sample(L, wo, ng, nf, u1, u2, wi, &fl, &pl);
+    if (!(pl > 0.0f)) return;  // the other layers' f may be non-zero where this one's is not
+    *f = mat_f(S, m, wo, *wi, ng, nf);
+    *pdf = mat_pdf_ns(S, m, wo, *wi, nf);
+}
+
+// The next direction from all layers (one picked uniformly with upick):
+// layer_sample of that layer, then for a non-specular layer the mixture's f
+// and its sampling pdf (1/n) sum pdf_i at wi.
+__device__ __forceinline__ void mat_sample(const SceneDev& S, const rb_material& m, V3 wo, V3 ng, V3 ns, V3 nf,
+                                           float upick, float u1, float u2, bool radiance, V3* wi, V3* f, float* pdf,
+                                           bool* spec, bool* weight) {
+    if (m.kind != RB_MAT_LAYERED) {
+        layer_sample(m, wo, ng, ns, nf, u1, u2, radiance, wi, f, pdf, spec, weight);
+        return;
+    }
+    const uint32_t n = mat_layers(m);
+    uint32_t k = (uint32_t)(upick * (float)n);
+    if (k > n - 1) k = n - 1;
+    layer_sample(mat_layer(S, m, k), wo, ng, ns, nf, u1, u2, radiance, wi, f, pdf, spec, weight);
+    if (*spec || !(*pdf > 0.0f)) return;
+    *f = mat_f(S, m, wo, *wi, ng, nf);
+    uint32_t c;
+    *pdf = mat_pdf_sum(S, m, wo, *wi, nf, &c) / (float)n;
 }
 
 }  // namespace rb

@@ -17,11 +17,13 @@ ICO3 = 1280
 ICO4 = 5120
 
 
-def ao_room(seed=0):
+def ao_room(seed=0, smooth=False):
+    """smooth: the icospheres carry their analytic vertex normals (same geometry)."""
     s = Scene()
     m = s.add_diffuse_bsdf((0.7, 0.7, 0.7))
     s.generate(L.RB_GEN_ROOM, c=(-5.0, 5.0), seed=seed, material_id=m)
-    s.generate(L.RB_GEN_ICOSPHERES, count=60, subdiv=3, seed=seed, r=(0.1, 0.5), c=(-4.0, 4.0), material_id=m)
+    s.generate(L.RB_GEN_ICOSPHERES, count=60, subdiv=3, seed=seed, r=(0.1, 0.5), c=(-4.0, 4.0), material_id=m,
+               smooth=smooth)
     return s
 
 
@@ -52,12 +54,12 @@ def trace_camera(width=2048, height=1024):
     return PerspectiveCamera((0.0, 4.0, -28.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), (width, height), 50.0)
 
 
-def pt_room(seed=0, spheres_hi=187, spheres_lo=813, emit=(12.0, 12.0, 12.0)):
+def pt_room(seed=0, spheres_hi=187, spheres_lo=813, emit=(12.0, 12.0, 12.0), smooth=False):
     s = Scene()
     wall = s.add_diffuse_bsdf((0.6, 0.6, 0.6))
     s.generate(L.RB_GEN_ROOM, c=(-5.0, 5.0), seed=seed, material_id=wall)
     s.generate(L.RB_GEN_ICOSPHERES, count=spheres_hi + spheres_lo, subdiv=3, subdiv_hi=4, hi_count=spheres_hi,
-               seed=seed, r=(0.1, 0.5), c=(-4.0, 4.0), mix_materials=True)
+               seed=seed, r=(0.1, 0.5), c=(-4.0, 4.0), mix_materials=True, smooth=smooth)
     s.generate(L.RB_GEN_EMITTER_QUADS, count=4, seed=seed, r=(0.25, 0.0), c=(-5.0, 5.0), emit=emit)
     return s
 

@@ -490,3 +490,25 @@ def test_binary_cache_refuses_files_no_build_writes(rbm, tmp_path):
     p.write_bytes(_cache_file_with_nodes(good, dag, [0]))
     with pytest.raises(rbm.RbError, match="not a tree"):
         rbm.BVH.load_binary(p, s)
+
+
+def test_product_builds_the_single_thread_oracle_tree_at_one_million_triangles(rbm):
+    """The headline scene's SBVH (1,000,000 triangles, built on every CPU)
+    has the bytes of the single-threaded oracle's tree, whose SHA-256 the
+    fixture holds (tests/golden/make_fixtures.py): the tree behind the
+    benchmark's visit counts and algorithmic bytes is the reference's."""
+    import hashlib
+    import json
+    from renderbox2_amd import _lib as L
+    g = json.load(open(os.path.join(ROOT, "tests", "golden", "trace_1m_tree.json")))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()  # noqa: E731
+    s = rbm.scenes.trace_1m(seed=0)
+    v, idx, _, _ = s.geometry()
+    assert (sha(v), sha(idx)) == (g["sha_verts"], g["sha_idx"]), "the scene generator changed"
+    b = rbm.BVH(s)
+    n, w, x = b.export(L.RB_LAYOUT_REF_COMPACT2)
+    assert [n.size, w.size, x.size] == g["sizes"]
+    assert (sha(n), sha(w), sha(x)) == (g["sha_nodes"], g["sha_woop"], g["sha_index"])
+    st = b.stats()
+    assert (st["n_inner"], st["n_leaf"], st["n_tri_refs"], st["duplicates"]) == \
+        (g["stats"]["n_inner"], g["stats"]["n_leaf"], g["stats"]["n_tris"], g["stats"]["duplicates"])

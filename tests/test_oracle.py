@@ -361,3 +361,66 @@ def test_single_triangle_woop_test_agrees_with_traversal(orc):
     # a wrong triangle id is (almost always) not a hit
     t2, ok2 = orc.woop_hits(v, idx, (ids[h] + 1) % len(idx), rays[h])
     assert ok2.mean() < 0.05
+
+
+def _cam(cam):
+    c = cam.c
+    return (tuple(c.position), tuple(c.lookat), tuple(c.up), c.hfov_deg, c.width, c.height)
+
+
+def _camera_rays(orc, r2c, c2w, w, h):
+    return orc.raygen_camera(r2c, c2w, w, (0, 0, w, h), 1, 3)
+
+
+def test_diffuse_rays_by_their_definition(orc, rbm):
+    """The incoherent ray set of the trace benchmark (orc_raygen_diffuse) from
+    its definition: ray i starts at the hit point o + t d of primary i, or of
+    another hit primary when i missed, with tmin = 1e-3 t exactly; its
+    direction is a unit vector on the side of the hit triangle's geometric
+    normal that faces the primary ray; directions are cosine distributed
+    (E[cos] = 2/3, E[cos^2] = 1/2, and the azimuth uniform)."""
+    s = rbm.scenes.trace_scaled(20_000, seed=2)
+    v, idx, _, _ = s.geometry()
+    from renderbox2_amd import _lib as L
+    b = rbm.BVH(s)
+    n, w, x = b.export(L.RB_LAYOUT_REF_COMPACT2)
+    cam = rbm.scenes.trace_camera(256, 128)
+    r2c, c2w = orc.camera_matrices(*_cam(cam))
+    prim = _camera_rays(orc, r2c, c2w, 256, 128)
+    ids, ts = orc.trace_compact2(n.reshape(-1, 4), w.reshape(-1, 4), x, prim)
+    hit = ids >= 0
+    assert 0.02 < hit.mean() < 0.95  # most rays redraw their origin from another primary
+    nout = 2 * len(prim)
+    out = orc.raygen_diffuse(v, idx, prim, ids, ts, nout, 7)
+    origins = prim[:, :3].astype(np.float64) + prim[:, 4:7] * ts[:, None].astype(np.float64)
+    # every origin is the hit point of some hit primary, with that primary's tmin
+    o32 = (prim[:, :3] + prim[:, 4:7] * ts[:, None]).astype(np.float32)
+    own = np.arange(nout) % len(prim)
+    same = hit[own]
+    assert np.array_equal(out[same, :3], o32[own[same]])
+    assert np.array_equal(out[same, 3], (np.float32(1e-3) * ts[own[same]]).astype(np.float32))
+    # a missed primary redraws another primary up to 64 times; a ray that
+    # finds none is the empty ray (zero length) -- (1 - hit fraction)^64 of them
+    empty = out[:, 7] == 0
+    assert np.array_equal(out[empty], np.tile(np.float32([0, 0, 0, 0, 0, 0, 1, 0]), (empty.sum(), 1)))
+    assert empty.mean() < 3 * (1 - hit.mean()) ** 64 + 1e-3
+    redrawn = ~same & ~empty
+    keyed = {tuple(r) for r in o32[hit].view(np.uint32)}
+    assert all(tuple(r) in keyed for r in out[redrawn, :3].view(np.uint32))
+    tmin_of = {tuple(r): t for r, t in zip(o32[hit].view(np.uint32), (np.float32(1e-3) * ts[hit]).astype(np.float32))}
+    assert all(tmin_of[tuple(r)] == t for r, t in zip(out[redrawn, :3].view(np.uint32), out[redrawn, 3]))
+    assert np.all(out[~empty, 7] == np.float32(3.4028235e38))
+    d = out[~empty, 4:7].astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-6)
+    # the facing geometric normal at each origin: from its triangle (own hit rays)
+    tri = v[idx[ids[own[same]]]].astype(np.float64)
+    ng = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    ng /= np.linalg.norm(ng, axis=1, keepdims=True)
+    flip = (ng * prim[own[same], 4:7]).sum(1) > 0
+    ng[flip] *= -1
+    c = (out[same, 4:7].astype(np.float64) * ng).sum(1)
+    assert np.all(c > -1e-6)
+    m = len(c)
+    assert abs(c.mean() - 2 / 3) < 4 * np.sqrt(1 / 18 / m)       # var(cos) = 1/2 - 4/9
+    assert abs((c * c).mean() - 0.5) < 4 * np.sqrt(1 / 12 / m)   # var(cos^2) = 1/3 - 1/4
+    assert np.isfinite(origins[hit]).all()

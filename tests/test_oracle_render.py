@@ -624,3 +624,35 @@ def test_bdpt_spherical_furnace_closed_form(orc):
     got = np.mean([orc.render_bdpt(c2, v, idx, tm, mats, lights, cam, (0, 0, 16, 16), 64, s, 24, 3, 0.5, 4096, 6,
                                    3).mean() for s in range(4)])
     assert abs(got / exp - 1) < 0.006, (got, exp)
+
+
+def test_bdpt_agrees_with_pt_under_shading_normals(orc):
+    # the box with vertex normals tilted ~20 degrees off the floor's and the
+    # back wall's geometric normals: camera paths shade with them, light
+    # paths too, with the adjoint factor |wo.ns||wi.ng| / (|wo.ng||wi.ns|)
+    # importance transport needs there (at the light pass's bounces and at
+    # the cache vertex of each connection); without that factor BDPT and the
+    # path tracer estimate different images
+    v, idx, tm, mats, lights = _box_room()
+    nt = len(idx)
+    tilt = np.array([[0.35, 1.0, 0.3], [-0.3, 1.0, 0.35], [0.3, 1.0, -0.35], [-0.35, 1.0, -0.3]], np.float32)
+    tilt /= np.linalg.norm(tilt, axis=1, keepdims=True)
+    back = tilt[:, [0, 2, 1]].copy()   # about +z for the back wall
+    normals = np.concatenate([tilt, back])
+    nidx = np.full((nt, 3), 0xFFFFFFFF, np.uint32)
+    nidx[0:2] = [[0, 1, 2], [0, 2, 3]]   # floor corners 0..3
+    nidx[4:6] = [[4, 5, 6], [4, 6, 7]]   # back wall
+    nrm = orc.tri_normals(normals, nidx, nt)
+    c2 = orc.Bvh(v, idx).compact2()
+    W = H = 20
+    cam = ((0.0, 1.0, -0.95), (0.0, 1.0, 1.0), (0.0, 1.0, 0.0), 70.0, W, H)
+    pt, bd = [], []
+    for seed in range(8):
+        pt.append(orc.render_pt(c2, v, idx, tm, mats, lights, cam, (0, 0, W, H), 512, seed, 24, 3, 0.5,
+                                normals=nrm).reshape(-1, 3).mean(0))
+        bd.append(orc.render_bdpt(c2, v, idx, tm, mats, lights, cam, (0, 0, W, H), 512, seed, 24, 3, 0.5, 4096, 4, 3,
+                                  normals=nrm).reshape(-1, 3).mean(0))
+    pt, bd = np.mean(pt, 0), np.mean(bd, 0)
+    flat = orc.render_pt(c2, v, idx, tm, mats, lights, cam, (0, 0, W, H), 512, 0, 24, 3, 0.5).reshape(-1, 3).mean(0)
+    assert np.all(np.abs(pt / flat - 1.0) > 0.01), (pt, flat)  # the normals change the image
+    assert np.all(np.abs(bd / pt - 1.0) < 0.015), (pt, bd)

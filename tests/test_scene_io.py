@@ -131,8 +131,18 @@ def test_scene_file_errors(rbm, tmp_path):
     layered = SCENE_XML.replace("<bxdf><type>mirror</type><reflectance>{0.9,0.9,0.9}</reflectance></bxdf>",
                                 "<bxdf><type>mirror</type><reflectance>{0.9,0.9,0.9}</reflectance></bxdf>"
                                 "<bxdf><type>lambertian</type><reflectance>{0.1,0.1,0.1}</reflectance></bxdf>")
-    with pytest.raises(rbm.RbError) as e:
+    with pytest.raises(rbm.RbError) as e:  # <layers>1</layers> with two <bxdf>
         rbm.Scene().load_xml(_write(tmp_path, xml=layered))
+    assert e.value.status == L.RB_ERR_INVALID
+    five = SCENE_XML.replace("<bxdf><type>mirror</type><reflectance>{0.9,0.9,0.9}</reflectance></bxdf>",
+                             "<bxdf><type>mirror</type><reflectance>{0.9,0.9,0.9}</reflectance></bxdf>" * 5)
+    with pytest.raises(rbm.RbError) as e:  # more than four layers
+        rbm.Scene().load_xml(_write(tmp_path, xml=five.replace("<name>silver</name>\n    <bsdf><layers>1</layers>",
+                                                               "<name>silver</name>\n    <bsdf><layers>5</layers>")))
+    assert e.value.status == L.RB_ERR_INVALID and "1..4" in str(e.value)
+    unknown = SCENE_XML.replace("<type>mirror</type>", "<type>velvet</type>")
+    with pytest.raises(rbm.RbError) as e:
+        rbm.Scene().load_xml(_write(tmp_path, xml=unknown))
     assert e.value.status == L.RB_ERR_UNSUPPORTED
     glass_no_eta = SCENE_XML.replace("<type>mirror</type>", "<type>glass</type>")
     with pytest.raises(rbm.RbError) as e:
