@@ -107,21 +107,39 @@ def bench_render(rb, kind, scene, bvh_max_leaf, cam, frames, built=None, **param
     counter.zero()
     r = rb.SamplerRenderer(tracer, cam, kind, **params)
     r.params.ray_counter = C.c_void_p(counter.ptr)
+    # live camera-path vertices per bounce and light sub-paths per depth
+    depth = r.params.max_depth + 1
+    paths = rb.DeviceBuffer(8 * depth)
+    paths.zero()
+    lpaths = rb.DeviceBuffer(8 * max(r.params.light_max_depth, 1))
+    lpaths.zero()
+    if kind in ("pt", "bdpt"):
+        r.params.path_counter = C.c_void_p(paths.ptr)
+    if kind == "bdpt":
+        r.params.light_path_counter = C.c_void_p(lpaths.ptr)
     out = rb.DeviceBuffer(cam.width * cam.height * 12)
     r.render_device(out, stream=stream.handle)
     stream.synchronize()
     rays = int(counter.download(np.uint64, 1)[0])
-    r.params.ray_counter = None
-    e0, e1 = rb.Event(), rb.Event()
-    e0.record(stream)
-    for _ in range(frames):
+    r.params.ray_counter = r.params.path_counter = r.params.light_path_counter = None
+    # one event pair per frame: the median frame, and the mean over all
+    ev = [rb.Event() for _ in range(frames + 1)]
+    ev[0].record(stream)
+    for k in range(frames):
         r.render_device(out, stream=stream.handle)
-    e1.record(stream)
-    e1.synchronize()
-    ms = e0.elapsed_ms(e1) / frames
+        ev[k + 1].record(stream)
+    ev[-1].synchronize()
+    per = [ev[k].elapsed_ms(ev[k + 1]) for k in range(frames)]
+    ms = float(np.median(per))
     img = out.download(np.float32).reshape(cam.height, cam.width, 3)
-    return {"frame_ms": round(ms, 3), "rays_per_frame": rays, "mrays": round(rays / (ms * 1e-3) / 1e6, 1),
-            "tris": scene.get_num_triangles(), "bvh_build_s": round(build_s, 1), "mean": round(float(img.mean()), 5)}
+    res = {"frame_ms": round(ms, 3), "frame_ms_mean": round(float(np.mean(per)), 3), "frames": frames,
+           "rays_per_frame": rays, "mrays": round(rays / (ms * 1e-3) / 1e6, 1),
+           "tris": scene.get_num_triangles(), "bvh_build_s": round(build_s, 1), "mean": round(float(img.mean()), 5)}
+    if kind in ("pt", "bdpt"):
+        res["paths_per_bounce"] = [int(x) for x in paths.download(np.uint64, depth)]
+    if kind == "bdpt":
+        res["light_paths_per_depth"] = [int(x) for x in lpaths.download(np.uint64, r.params.light_max_depth)]
+    return res
 
 
 def bench_graphs(rb, built, iters=50):
@@ -562,11 +580,11 @@ def main():
         room = rb.scenes.pt_room(seed=0)
         t0 = time.time()
         built = (rb.CudaTracer(rb.scenes.build(room), room), time.time() - t0)
-        pt = bench_render(rb, "pt", room, None, rb.scenes.pt_camera(1024, 1024), 2, built,
+        pt = bench_render(rb, "pt", room, None, rb.scenes.pt_camera(1024, 1024), 7, built,
                           spp=8, seed=0, max_depth=8, rr_start=3, rr_prob=0.5)
         log("pt frame (config 3: 2,000,136 tris, 1024x1024, 8 spp, depth 8)", pt)
         extra["pt_1024_8spp"] = pt
-        bd = bench_render(rb, "bdpt", room, None, rb.scenes.pt_camera(512, 512), 3, built,
+        bd = bench_render(rb, "bdpt", room, None, rb.scenes.pt_camera(512, 512), 9, built,
                           spp=4, seed=0, max_depth=6, rr_start=3, rr_prob=0.5, light_max_depth=6,
                           lvc_capacity=1 << 20, light_paths=0, connections=5)
         log("bdpt frame (config 5 BDPT: 2,000,136 tris, 512x512, 4 spp, 2^20-vertex cache, depth 6)", bd)

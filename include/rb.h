@@ -497,6 +497,12 @@ typedef struct rb_render_params {
     /* Ray-cast debug shading (RayCastIntegratorParams::m_shade,
      * src/core/params.h:139-151): RB_SHADE_* below. */
     uint32_t shade;
+    /* Optional device counters (NULL: none), for measurement: camera-path
+     * vertices shaded with a BSDF at each bounce are added to
+     * path_counter[0..max_depth] (PT, BDPT), light sub-paths traced at each
+     * depth to light_path_counter[0..light_max_depth - 1] (BDPT). */
+    unsigned long long* path_counter;
+    unsigned long long* light_path_counter;
 } rb_render_params;
 enum { RB_FILTER_BOX = 0, RB_FILTER_TRIANGLE = 1, RB_FILTER_GAUSSIAN = 2 };
 enum { RB_SHADE_PRIMITIVE_ID = 0, RB_SHADE_MATERIAL_ID = 1, RB_SHADE_NORMALS = 2, RB_SHADE_UVS = 3,
@@ -580,7 +586,7 @@ typedef struct rb_request {
     uint32_t scene;          /* id from rb_server_add_scene */
     rb_camera camera;        /* render kinds */
     uint32_t x0, y0, w, h;   /* render kinds: tile */
-    rb_render_params params; /* render kinds (ray_counter is ignored) */
+    rb_render_params params; /* render kinds (ray_counter and the path counters are ignored) */
     float* out_host;         /* render kinds: optional host image w*h*3 (valid once done) */
     const rb_ray* rays;      /* RB_REQ_TRACE: device rays / hits, n, RB_CLOSEST or RB_ANY */
     rb_hit* hits;

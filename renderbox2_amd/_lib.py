@@ -201,7 +201,7 @@ class rb_render_params(C.Structure):
                 ("rr_start", U32), ("rr_prob", F32), ("light_paths", U32), ("lvc_capacity", U32),
                 ("connections", U32), ("light_max_depth", U32), ("ray_counter", P), ("filter", U32),
                 ("filter_width", F32 * 2), ("filter_alpha", F32),
-                ("shade", U32)]
+                ("shade", U32), ("path_counter", P), ("light_path_counter", P)]
 
 
 class rb_request(C.Structure):

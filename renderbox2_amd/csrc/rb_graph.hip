@@ -58,7 +58,8 @@ rb_status rb_render_graph_create(uint32_t kind, rb_bvh_t bvh, const rb_camera* c
     *out = nullptr;
     if (kind != RB_REQ_AO && kind != RB_REQ_PT && kind != RB_REQ_BDPT && kind != RB_REQ_RAYCAST)
         return fail(RB_ERR_INVALID, "render graphs take RB_REQ_AO, RB_REQ_PT, RB_REQ_BDPT or RB_REQ_RAYCAST");
-    if (p->ray_counter) return fail(RB_ERR_INVALID, "render graphs take no ray counter");
+    if (p->ray_counter || p->path_counter || p->light_path_counter)
+        return fail(RB_ERR_INVALID, "render graphs take no ray or path counters");
     rb_render_graph_s* G = new rb_render_graph_s();
     const uint32_t pairs = 2 * (p->max_depth + 1) + p->light_max_depth + 8;
     hipError_t e;

@@ -95,8 +95,9 @@ __global__ void k_ao_resolve(const int2* __restrict__ prim_hits, const int2* __r
 // ---------------------------------------------------------------- PT ----
 
 struct PathBufs {
-    float4* T;       // throughput at this bounce (w unused)
+    float4* T;       // throughput at this bounce (w unused; bounce 0's is 1 and not stored)
     float4* Tn;      // throughput after this bounce's sample: the next bounce's T (the two swap per bounce)
+    const float4* Tprev;  // the previous bounce's T, for its contributions (the buffer Tn reuses)
     float4* L;       // radiance
     // The current vertex of a path: ray 2 * cur[pid] of pray and hit
     // cur[pid] of phit -- the camera rays at bounce 0, then the previous
@@ -136,25 +137,6 @@ struct BdArgs {
     RngArg crng;
 };
 
-__global__ __launch_bounds__(kQueueBlock) void k_pt_init(const int2* __restrict__ hits, uint32_t n, PathBufs P) {
-    const uint32_t end = uniform_end(n);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) {
-        const bool valid = i < n;
-        bool live = false;
-        if (valid) {
-            const int2 h = hits[i];
-            P.T[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-            P.L[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-            P.cur[i] = i;
-            P.info[i] = 0;
-            if (P.cmask) P.cmask[i] = 0;
-            live = h.x >= 0;
-        }
-        const uint32_t q = block_reserve(&P.cnt[0], live ? 1u : 0u);
-        if (live) P.q[0][q] = i;
-    }
-}
-
 __global__ void k_reset(uint32_t* c, uint32_t a, uint32_t b, uint32_t d) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         c[a] = 0;
@@ -181,175 +163,270 @@ __global__ void k_count_rays(const uint32_t* c, unsigned long long* total, unsig
         atomicAdd(total, (unsigned long long)c[2] + (unsigned long long)c[3] + extra);
 }
 
-// One bounce of shading for the queued camera paths.  BD: with the
-// technique weights; the light-vertex-cache connections follow in
-// k_lvc_connect.
+// Rays asked for by one shaded camera vertex: light sample (A, shadow
+// queue), MIS BSDF sample (B) and next bounce (C, closest queue).
+struct ShadeRays {
+    bool wantA, wantB, wantC;
+    float4 ra0, ra1, rb1, rc1;
+    V3 p;
+    float eps;
+    int lt;  // light triangle of the samples
+};
+
+// Shading of camera path pid at its vertex (ray 2 * c of pray, hit c of
+// phit; info: bounce | specular << 8; throughput Tp; radiance L so far).
+// An emitter adds its emission when seen directly or through specular
+// bounces (integrator_path.cu:94-109) and ends the path: writes L, returns
+// false.  Otherwise light sample + MIS BSDF sample + next-bounce sample with
+// roulette (integrator_path.cu:111-249); writes L, Tn, ca, cb, info (and
+// cmask with BD) and fills R; returns true.  BD: with the technique weights;
+// the light-vertex-cache connections follow in k_lvc_connect.
 template <bool BD>
-__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pt_shade(
-    PathBufs P, uint32_t qi, TileArgs T, SceneDev S, RngArg rng_a, uint32_t max_depth, uint32_t rr_start,
-    float rr_prob, BdArgs B) {
+__device__ __forceinline__ bool shade_vertex(const PathBufs& P, uint32_t pid, uint32_t c, uint32_t info, float4 Tp,
+                                             float4 L, const TileArgs& T, const SceneDev& S, const Rng& rng,
+                                             uint32_t max_depth, uint32_t rr_start, float rr_prob, const BdArgs& B,
+                                             ShadeRays& R) {
+    const float4 o = P.pray[2 * (size_t)c], d = P.pray[2 * (size_t)c + 1];
+    const int2 h = P.phit[c];
+    const uint32_t bounce = info & 255u;
+    const bool spec = (info >> 8) & 1u;
+    const float t = __int_as_float(h.y);
+    const V3 dv = v3(d.x, d.y, d.z);
+    const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
+    const float eps = 1.0e-3f * t;
+    V3 v0, v1, v2;
+    uint32_t mid, tf;
+    tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
+    const rb_material m = S.mats[mid];
+    const V3 ng = tri_normal(v0, v1, v2);
+    V3 ns;  // shading normal on ng's side (ng itself for a flat triangle)
+    shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
+    const V3 wo = vneg(dv);
+    const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
+    const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
+    uint32_t cm = BD ? (bounce ? P.cmask[pid] : 0u) : 0u;
+    if (m.kind == RB_MAT_EMITTER) {
+        if (bounce == 0 || spec) {
+            V3 le = vdot(ng, wo) > 0.0f ? v3(m.color[0], m.color[1], m.color[2]) : v3(0.f, 0.f, 0.f);
+            if (BD) le = vmul(le, technique_weight(cm, bounce + 1, max_depth, B.tdepth));
+            L = make_float4(L.x + Tv.x * le.x, L.y + Tv.y * le.y, L.z + Tv.z * le.z, 0.f);
+        }
+        P.L[pid] = L;
+        return false;
+    }
+    P.L[pid] = L;
+    R.p = p;
+    R.eps = eps;
+    const uint32_t gid = sample_gid(T, pid);
+    const U4 r0 = rng.draw(gid, bounce, 0, 0);
+    const U4 r1 = rng.draw(gid, bounce, 1, 0);
+    // layer picks of a layered material: next bounce, MIS sample
+    const bool layered = m.kind == RB_MAT_LAYERED;
+    const U4 r2 = layered ? rng.draw(gid, bounce, 2, 0) : U4{0u, 0u, 0u, 0u};
+    V3 ca = v3(0.f, 0.f, 0.f), cb = v3(0.f, 0.f, 0.f);
+    const bool delta = mat_delta(S, m);
+    const float wdir = BD ? technique_weight(cm, bounce + 2, max_depth, B.tdepth) : 1.0f;
+    if (S.n_lights > 0) {
+        uint32_t lid = (uint32_t)(u01(r0.x) * (float)S.n_lights);
+        if (lid > S.n_lights - 1) lid = S.n_lights - 1;
+        const uint32_t lt = S.light_tris[lid];
+        V3 l0, l1, l2;
+        uint32_t lmid;
+        tri_fetch(S, lt, &l0, &l1, &l2, &lmid);
+        const rb_material lm = S.mats[lmid];
+        const V3 emit = v3(lm.color[0], lm.color[1], lm.color[2]);
+        const V3 ln = tri_normal(l0, l1, l2);
+        const float area = tri_area(l0, l1, l2);
+        R.lt = (int)lt;
+        // light sample (DiffuseAreaLight::sample_L, src/core/lights.cu:56-65)
+        const V3 pl = sample_triangle(l0, l1, l2, u01(r0.y), u01(r0.z));
+        const V3 wi = vnorm(vsub(pl, p));
+        const float lpdf = light_pdf(p, wi, l0, l1, l2, ln, area);
+        const V3 Li = vdot(ln, vneg(wi)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
+        if (!delta && !is_black(Li) && lpdf > 0.0f) {
+            const V3 f = mat_f(S, m, wo, wi, ng, nf);
+            if (!is_black(f)) {
+                const float bpdf = mat_pdf_ns(S, m, wo, wi, nf);
+                const float w = lpdf / (lpdf + bpdf);
+                const float s = fabsf(vdot(wi, ns)) * w;
+                ca = v3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
+                const V3 seg = vsub(pl, p);
+                const float dist = sqrtf(vdot(seg, seg));
+                const V3 sd = vnorm(seg);
+                R.ra0 = make_float4(p.x, p.y, p.z, eps);
+                R.ra1 = make_float4(sd.x, sd.y, sd.z, dist * (1.0f - 1.0e-3f));
+                R.wantA = true;
+            }
+        }
+        // BSDF sample with MIS (non-specular components only)
+        if (!delta) {
+            V3 wi2, f2;
+            float pdf2;
+            mat_sample_ns(S, m, wo, ng, nf, u01(r2.y), u01(r0.w), u01(r1.x), &wi2, &f2, &pdf2);
+            if (!is_black(f2) && pdf2 > 0.0f) {
+                const float lpdf2 = light_pdf(p, wi2, l0, l1, l2, ln, area);
+                if (lpdf2 > 0.0f) {
+                    const float w2 = pdf2 / (pdf2 + lpdf2);
+                    const V3 Li2 = vdot(ln, vneg(wi2)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
+                    const float s2 = fabsf(vdot(wi2, ns)) * w2;
+                    cb = v3(f2.x * Li2.x * s2 / pdf2, f2.y * Li2.y * s2 / pdf2, f2.z * Li2.z * s2 / pdf2);
+                    R.rb1 = make_float4(wi2.x, wi2.y, wi2.z, 3.402823466e+38f);
+                    R.wantB = true;
+                }
+            }
+        }
+    }
+    // next bounce (material_sample_f with every component)
+    V3 wi3, f3;
+    float pdf3;
+    bool spec3;
+    bool weight3;  // f3 already holds f |cos| / pdf
+    mat_sample(S, m, wo, ng, ns, nf, u01(r2.x), u01(r1.y), u01(r1.z), true, &wi3, &f3, &pdf3, &spec3, &weight3);
+    // the vertex counts as specular for the technique weights of the paths
+    // that continue through a specular lobe
+    if (BD && spec3) cm |= 1u << bounce;
+    uint32_t ninfo = info & 0x1FFu;
+    if (!is_black(f3) && pdf3 > 0.0f) {
+        const float s3 = weight3 ? 1.0f : fabsf(vdot(wi3, ns));
+        V3 Tn = v3(Tv.x * (f3.x * s3 / pdf3), Tv.y * (f3.y * s3 / pdf3), Tv.z * (f3.z * s3 / pdf3));
+        bool alive = true;
+        if (bounce > rr_start) {
+            const float lum = luminance(Tn);
+            const float qq = rr_prob < lum ? rr_prob : lum;
+            if (u01(r1.w) > qq) alive = false;
+            else Tn = v3(Tn.x / qq, Tn.y / qq, Tn.z / qq);
+        }
+        if (alive && bounce != max_depth) {
+            P.Tn[pid] = make_float4(Tn.x, Tn.y, Tn.z, 0.f);
+            R.rc1 = make_float4(wi3.x, wi3.y, wi3.z, 3.402823466e+38f);
+            R.wantC = true;
+            if (spec3) ninfo |= 1u << 9;
+        }
+    }
+    if (BD) P.cmask[pid] = cm;
+    P.ca[pid] = make_float4(ca.x, ca.y, ca.z, wdir);
+    P.cb[pid] = make_float4(cb.x, cb.y, cb.z, 0.f);
+    P.info[pid] = ninfo;
+    return true;
+}
+
+// The contributions of path pid's last shaded vertex once its rays are
+// traced: light sample if unoccluded, MIS sample if it hit the sampled
+// light (times the light count and, with BD, the direct-light technique
+// weight), then BD's cache connections whose shadow rays are unoccluded
+// (integrator_path.cu:136-200).  Returns the new radiance in *L; true, with
+// the next vertex (bounce ray cur, info), when the bounce ray hit.
+template <bool BD>
+__device__ __forceinline__ bool connect_vertex(const PathBufs& P, uint32_t pid, float nlights, uint32_t nconn,
+                                               float4* L, uint32_t* cur, uint32_t* ninfo) {
+    const uint32_t info = P.info[pid];
+    const int4 s = P.slot[pid];
+    V3 Ld = v3(0.f, 0.f, 0.f);
+    const float4 ca = P.ca[pid];
+    if (s.x >= 0 && P.ha[s.x].x < 0) Ld = vadd(Ld, v3(ca.x, ca.y, ca.z));
+    if (s.y >= 0 && P.phit[s.y].x == s.w) {
+        const float4 c = P.cb[pid];
+        Ld = vadd(Ld, v3(c.x, c.y, c.z));
+    }
+    Ld = vmul(Ld, nlights);
+    if (BD) Ld = vmul(Ld, ca.w);
+    // the vertex's throughput: 1 at bounce 0 (not stored)
+    const float4 Tp = (info & 255u) ? P.Tprev[pid] : make_float4(1.f, 1.f, 1.f, 0.f);
+    float4 l = P.L[pid];
+    l = make_float4(l.x + Tp.x * Ld.x, l.y + Tp.y * Ld.y, l.z + Tp.z * Ld.z, 0.f);
+    if (BD) {
+        for (uint32_t i = 0; i < nconn; i++) {
+            const int cs = P.cslot[(size_t)pid * kMaxConn + i];
+            if (cs >= 0 && P.ha[cs].x < 0) {
+                const float4 c = P.cc[(size_t)pid * kMaxConn + i];
+                l = make_float4(l.x + c.x, l.y + c.y, l.z + c.z, 0.f);
+            }
+        }
+    }
+    *L = l;
+    if (s.z >= 0) {
+        const int2 h = P.phit[s.z];
+        if (h.x >= 0) {
+            *cur = (uint32_t)s.z;  // the next vertex: bounce ray s.z, read in place
+            *ninfo = ((info & 255u) + 1u) | (((info >> 9) & 1u) << 8);
+            return true;
+        }
+    }
+    return false;
+}
+
+// One step of the camera pass over a queue of paths, with no host
+// synchronisation between steps.
+//  MODE 0: the n0 camera rays' hits start the paths (L = 0, T = 1) and
+//          bounce 0 is shaded;
+//  MODE 1: each path of queue qi (shaded at the previous bounce, its rays
+//          traced) takes that bounce's contributions (connect_vertex) and,
+//          when its bounce ray hit, is shaded at this bounce by the same
+//          thread (shade_vertex), so its state is read once per bounce.  (A
+//          per-workgroup compaction of the continuing paths before shading
+//          measured slower: its barrier stalls the workgroup on the slowest
+//          connect; the paths that end leave idle lanes only at the bounces
+//          roulette thins, where the queue is short);
+//  MODE 2: contributions only (after the last bounce).
+// Shaded non-emitter vertices are appended to queue qi ^ 1 and their rays to
+// the shadow (cnt[2]) and closest (cnt[3]) queues.
+template <bool BD, int MODE>
+__global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_pt_step(
+    PathBufs P, uint32_t qi, uint32_t n0, TileArgs T, SceneDev S, RngArg rng_a, uint32_t max_depth,
+    uint32_t rr_start, float rr_prob, float nlights, BdArgs B) {
     const Rng rng = rng_a.get();
-    const uint32_t count = P.cnt[qi];
+    const uint32_t count = MODE == 0 ? n0 : P.cnt[qi];
     const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t end = uniform_end(count);  // whole workgroups iterate together (block_reserve)
+    const uint32_t end = uniform_end(count);  // whole workgroups iterate together
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
         const bool valid = q < count;
-        const uint32_t pid = valid ? P.q[qi][q] : 0u;
-        bool wantA = false, wantB = false, wantC = false;
-        float4 ra0, ra1, rb1, rc1;
-        V3 p = v3(0.f, 0.f, 0.f);
-        float eps = 0.0f;
-        int4 slot = make_int4(-1, -1, -1, -1);  // written once, after the queue reservations
+        bool go = false;
+        uint32_t pid = 0, cur = 0, info = 0;
+        float4 L = make_float4(0.f, 0.f, 0.f, 0.f);
         if (valid) {
-            const uint32_t c = P.cur[pid];
-            const float4 o = P.pray[2 * (size_t)c], d = P.pray[2 * (size_t)c + 1];
-            const int2 h = P.phit[c];
-            const uint32_t info = P.info[pid];
-            const uint32_t bounce = info & 255u;
-            const bool spec = (info >> 8) & 1u;
-            const float t = __int_as_float(h.y);
-            const V3 dv = v3(d.x, d.y, d.z);
-            p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
-            eps = 1.0e-3f * t;
-            V3 v0, v1, v2;
-            uint32_t mid, tf;
-            tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
-            const rb_material m = S.mats[mid];
-            const V3 ng = tri_normal(v0, v1, v2);
-            V3 ns;  // shading normal on ng's side (ng itself for a flat triangle)
-            shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
-            const V3 wo = vneg(dv);
-            const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
-            const float4 Tp = P.T[pid];
-            const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
-            uint32_t cm = BD ? P.cmask[pid] : 0u;
-            if (m.kind == RB_MAT_EMITTER) {
-                // emission seen directly or through mirrors (integrator_path.cu:94-109), then stop
-                uint32_t ninfo = info | (1u << 10);
-                if (bounce == 0 || spec) {
-                    V3 le = vdot(ng, wo) > 0.0f ? v3(m.color[0], m.color[1], m.color[2]) : v3(0.f, 0.f, 0.f);
-                    if (BD) le = vmul(le, technique_weight(cm, bounce + 1, max_depth, B.tdepth));
-                    const float4 L = P.L[pid];
-                    P.L[pid] = make_float4(L.x + Tv.x * le.x, L.y + Tv.y * le.y, L.z + Tv.z * le.z, 0.f);
-                }
-                P.info[pid] = ninfo;
+            if (MODE == 0) {
+                pid = cur = q;
+                go = P.phit[q].x >= 0;
+                if (!go) P.L[q] = L;
             } else {
-                const uint32_t gid = sample_gid(T, pid);
-                const U4 r0 = rng.draw(gid, bounce, 0, 0);
-                const U4 r1 = rng.draw(gid, bounce, 1, 0);
-                // layer picks of a layered material: next bounce, MIS sample
-                const bool layered = m.kind == RB_MAT_LAYERED;
-                const U4 r2 = layered ? rng.draw(gid, bounce, 2, 0) : U4{0u, 0u, 0u, 0u};
-                V3 ca = v3(0.f, 0.f, 0.f), cb = v3(0.f, 0.f, 0.f);
-                const bool delta = mat_delta(S, m);
-                const float wdir = BD ? technique_weight(cm, bounce + 2, max_depth, B.tdepth) : 1.0f;
-                if (S.n_lights > 0) {
-                    uint32_t lid = (uint32_t)(u01(r0.x) * (float)S.n_lights);
-                    if (lid > S.n_lights - 1) lid = S.n_lights - 1;
-                    const uint32_t lt = S.light_tris[lid];
-                    V3 l0, l1, l2;
-                    uint32_t lmid;
-                    tri_fetch(S, lt, &l0, &l1, &l2, &lmid);
-                    const rb_material lm = S.mats[lmid];
-                    const V3 emit = v3(lm.color[0], lm.color[1], lm.color[2]);
-                    const V3 ln = tri_normal(l0, l1, l2);
-                    const float area = tri_area(l0, l1, l2);
-                    slot.w = (int)lt;
-                    // light sample (DiffuseAreaLight::sample_L, src/core/lights.cu:56-65)
-                    const V3 pl = sample_triangle(l0, l1, l2, u01(r0.y), u01(r0.z));
-                    const V3 wi = vnorm(vsub(pl, p));
-                    const float lpdf = light_pdf(p, wi, l0, l1, l2, ln, area);
-                    const V3 Li = vdot(ln, vneg(wi)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
-                    if (!delta && !is_black(Li) && lpdf > 0.0f) {
-                        const V3 f = mat_f(S, m, wo, wi, ng, nf);
-                        if (!is_black(f)) {
-                            const float bpdf = mat_pdf_ns(S, m, wo, wi, nf);
-                            const float w = lpdf / (lpdf + bpdf);
-                            const float s = fabsf(vdot(wi, ns)) * w;
-                            ca = v3(f.x * Li.x * s / lpdf, f.y * Li.y * s / lpdf, f.z * Li.z * s / lpdf);
-                            const V3 seg = vsub(pl, p);
-                            const float dist = sqrtf(vdot(seg, seg));
-                            const V3 sd = vnorm(seg);
-                            ra0 = make_float4(p.x, p.y, p.z, eps);
-                            ra1 = make_float4(sd.x, sd.y, sd.z, dist * (1.0f - 1.0e-3f));
-                            wantA = true;
-                        }
-                    }
-                    // BSDF sample with MIS (non-specular components only)
-                    if (!delta) {
-                        V3 wi2, f2;
-                        float pdf2;
-                        mat_sample_ns(S, m, wo, ng, nf, u01(r2.y), u01(r0.w), u01(r1.x), &wi2, &f2, &pdf2);
-                        if (!is_black(f2) && pdf2 > 0.0f) {
-                            const float lpdf2 = light_pdf(p, wi2, l0, l1, l2, ln, area);
-                            if (lpdf2 > 0.0f) {
-                                const float w2 = pdf2 / (pdf2 + lpdf2);
-                                const V3 Li2 = vdot(ln, vneg(wi2)) > 0.0f ? emit : v3(0.f, 0.f, 0.f);
-                                const float s2 = fabsf(vdot(wi2, ns)) * w2;
-                                cb = v3(f2.x * Li2.x * s2 / pdf2, f2.y * Li2.y * s2 / pdf2, f2.z * Li2.z * s2 / pdf2);
-                                rb1 = make_float4(wi2.x, wi2.y, wi2.z, 3.402823466e+38f);
-                                wantB = true;
-                            }
-                        }
-                    }
-                }
-                // next bounce (material_sample_f with every component)
-                V3 wi3, f3;
-                float pdf3;
-                bool spec3;
-                bool weight3;  // f3 already holds f |cos| / pdf
-                mat_sample(S, m, wo, ng, ns, nf, u01(r2.x), u01(r1.y), u01(r1.z), true, &wi3, &f3, &pdf3, &spec3,
-                           &weight3);
-                // the vertex counts as specular for the technique weights of
-                // the paths that continue through a specular lobe
-                if (BD && spec3) cm |= 1u << bounce;
-                uint32_t ninfo = info & 0x1FFu;
-                if (!is_black(f3) && pdf3 > 0.0f) {
-                    const float s3 = weight3 ? 1.0f : fabsf(vdot(wi3, ns));
-                    V3 Tn = v3(Tv.x * (f3.x * s3 / pdf3), Tv.y * (f3.y * s3 / pdf3), Tv.z * (f3.z * s3 / pdf3));
-                    bool alive = true;
-                    if (bounce > rr_start) {
-                        const float lum = luminance(Tn);
-                        const float qq = rr_prob < lum ? rr_prob : lum;
-                        if (u01(r1.w) > qq) alive = false;
-                        else Tn = v3(Tn.x / qq, Tn.y / qq, Tn.z / qq);
-                    }
-                    P.Tn[pid] = make_float4(Tn.x, Tn.y, Tn.z, 0.f);
-                    if (alive && bounce != max_depth) {
-                        rc1 = make_float4(wi3.x, wi3.y, wi3.z, 3.402823466e+38f);
-                        wantC = true;
-                        if (spec3) ninfo |= 1u << 9;
-                    }
-                }
-                if (BD) P.cmask[pid] = cm;
-                P.ca[pid] = make_float4(ca.x, ca.y, ca.z, wdir);
-                P.cb[pid] = make_float4(cb.x, cb.y, cb.z, 0.f);
-                P.info[pid] = ninfo;
+                pid = P.q[qi][q];
+                go = connect_vertex<BD>(P, pid, nlights, B.nconn, &L, &cur, &info);
+                if (MODE == 2 || !go) P.L[pid] = L;
             }
+        }
+        if (MODE == 2) continue;
+        ShadeRays R;
+        R.wantA = R.wantB = R.wantC = false;
+        R.lt = -1;
+        bool shaded = false;
+        if (go) {
+            const float4 Tp = (info & 255u) ? P.T[pid] : make_float4(1.f, 1.f, 1.f, 0.f);
+            shaded = shade_vertex<BD>(P, pid, cur, info, Tp, L, T, S, rng, max_depth, rr_start, rr_prob, B, R);
+            if (BD && shaded) P.cur[pid] = cur;  // k_lvc_connect reads the vertex
         }
         // workgroup-aggregated appends (every thread of the workgroup reaches
         // these): shadow queue = light sample; closest queue = MIS ray, then
-        // the bounce ray.  BDPT connection rays are appended by k_lvc_connect.
-        const uint32_t sa = block_reserve(&P.cnt[2], wantA ? 1u : 0u);
-        const uint32_t sb = block_reserve(&P.cnt[3], (wantB ? 1u : 0u) + (wantC ? 1u : 0u));
-        const uint32_t sc = sb + (wantB ? 1u : 0u);
-        if (wantA) {
-            P.ra[2 * (size_t)sa] = ra0;
-            P.ra[2 * (size_t)sa + 1] = ra1;
+        // the bounce ray; the shaded vertices to the next path queue.  BDPT
+        // connection rays are appended by k_lvc_connect.
+        const uint32_t sa = block_reserve(&P.cnt[2], R.wantA ? 1u : 0u);
+        const uint32_t sb = block_reserve(&P.cnt[3], (R.wantB ? 1u : 0u) + (R.wantC ? 1u : 0u));
+        const uint32_t sq = block_reserve(&P.cnt[qi ^ 1], shaded ? 1u : 0u);
+        const uint32_t sc = sb + (R.wantB ? 1u : 0u);
+        if (R.wantA) {
+            P.ra[2 * (size_t)sa] = R.ra0;
+            P.ra[2 * (size_t)sa + 1] = R.ra1;
         }
-        if (wantB) {
-            P.rbc[2 * (size_t)sb] = make_float4(p.x, p.y, p.z, eps);
-            P.rbc[2 * (size_t)sb + 1] = rb1;
+        if (R.wantB) {
+            P.rbc[2 * (size_t)sb] = make_float4(R.p.x, R.p.y, R.p.z, R.eps);
+            P.rbc[2 * (size_t)sb + 1] = R.rb1;
         }
-        if (wantC) {
-            P.rbc[2 * (size_t)sc] = make_float4(p.x, p.y, p.z, eps);
-            P.rbc[2 * (size_t)sc + 1] = rc1;
+        if (R.wantC) {
+            P.rbc[2 * (size_t)sc] = make_float4(R.p.x, R.p.y, R.p.z, R.eps);
+            P.rbc[2 * (size_t)sc + 1] = R.rc1;
         }
-        if (valid) {
-            slot.x = wantA ? (int)sa : -1;
-            slot.y = wantB ? (int)sb : -1;
-            slot.z = wantC ? (int)sc : -1;
-            P.slot[pid] = slot;
+        if (shaded) {
+            P.slot[pid] = make_int4(R.wantA ? (int)sa : -1, R.wantB ? (int)sb : -1, R.wantC ? (int)sc : -1, R.lt);
+            P.q[qi ^ 1][sq] = pid;
         }
     }
 }
@@ -357,10 +434,15 @@ __global__ __launch_bounds__(kQueueBlock) __attribute__((amdgpu_waves_per_eu(4))
 // BDPT connections of the queued camera vertices to the light-vertex cache
 // (integrator_bdpt_lvc_sk.cu:343-386), one thread per (path, connection):
 // thread t serves queue entry t / nconn, connection t % nconn.  Runs after
-// k_pt_shade of the same bounce, which leaves the camera vertex (ray, hit,
-// throughput, specular mask, bounce) unchanged for the non-specular vertices
-// that connect.  Writes the contribution and the shadow-ray slot of every
-// connection (-1: no ray) and appends the shadow rays to the shadow queue.
+// the k_pt_step that shaded the vertices (queue qi), which leaves the camera
+// vertex (cur, hit, throughput, earlier specular mask, bounce) in place.
+// Writes the contribution and the shadow-ray slot of every connection (-1:
+// no ray) and appends the shadow rays to the shadow queue.  The nconn
+// threads of a vertex read it from the same cache lines; making the
+// connections in the shading thread instead (its state already in
+// registers) measured slower, since one thread then waits on nconn cache
+// records in turn and the kernel needs more registers than four waves per
+// SIMD allow.
 __global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_t qi, TileArgs T, SceneDev S,
                                                              uint32_t max_depth, BdArgs B) {
     const Rng crng = B.crng.get();
@@ -393,7 +475,7 @@ __global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_
                 shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
                 const V3 wo = vneg(dv);
                 const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
-                const float4 Tp = P.T[pid];
+                const float4 Tp = bounce ? P.T[pid] : make_float4(1.f, 1.f, 1.f, 0.f);  // bounce 0's is not stored
                 const V3 Tv = v3(Tp.x, Tp.y, Tp.z);
                 // earlier vertices only: k_pt_shade has already marked this
                 // vertex when its next direction took a specular lobe
@@ -457,58 +539,6 @@ __global__ __launch_bounds__(kQueueBlock) void k_lvc_connect(PathBufs P, uint32_
             P.ra[2 * (size_t)si] = r0;
             P.ra[2 * (size_t)si + 1] = r1;
         }
-    }
-}
-
-template <bool BD>
-__global__ __launch_bounds__(kQueueBlock) void k_pt_connect(PathBufs P, uint32_t qi, float nlights, uint32_t nconn) {
-    const uint32_t count = P.cnt[qi];
-    const uint32_t stride = gridDim.x * blockDim.x;
-    const uint32_t end = uniform_end(count);
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += stride) {
-        const bool valid = q < count;
-        bool next = false;
-        uint32_t pid = 0;
-        if (valid) {
-            pid = P.q[qi][q];
-            const uint32_t info = P.info[pid];
-            if (!(info & (1u << 10))) {
-                const int4 s = P.slot[pid];
-                V3 Ld = v3(0.f, 0.f, 0.f);
-                const float4 ca = P.ca[pid];
-                if (s.x >= 0 && P.ha[s.x].x < 0) Ld = vadd(Ld, v3(ca.x, ca.y, ca.z));
-                if (s.y >= 0 && P.hbc[s.y].x == s.w) {
-                    const float4 c = P.cb[pid];
-                    Ld = vadd(Ld, v3(c.x, c.y, c.z));
-                }
-                Ld = vmul(Ld, nlights);
-                if (BD) Ld = vmul(Ld, ca.w);
-                const float4 Tp = P.T[pid];
-                float4 L = P.L[pid];
-                L = make_float4(L.x + Tp.x * Ld.x, L.y + Tp.y * Ld.y, L.z + Tp.z * Ld.z, 0.f);
-                if (BD) {
-                    for (uint32_t i = 0; i < nconn; i++) {
-                        const int cs = P.cslot[(size_t)pid * kMaxConn + i];
-                        if (cs >= 0 && P.ha[cs].x < 0) {
-                            const float4 c = P.cc[(size_t)pid * kMaxConn + i];
-                            L = make_float4(L.x + c.x, L.y + c.y, L.z + c.z, 0.f);
-                        }
-                    }
-                }
-                P.L[pid] = L;
-                if (s.z >= 0) {
-                    const int2 h = P.hbc[s.z];
-                    if (h.x >= 0) {
-                        P.cur[pid] = (uint32_t)s.z;  // the next vertex: bounce ray s.z, read in place
-                        const uint32_t b = (info & 255u) + 1u;
-                        P.info[pid] = b | (((info >> 9) & 1u) << 8);
-                        next = true;
-                    }
-                }
-            }
-        }
-        const uint32_t o = block_reserve(&P.cnt[qi ^ 1], next ? 1u : 0u);
-        if (next) P.q[qi ^ 1][o] = pid;
     }
 }
 
@@ -854,8 +884,6 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     PathBufs P;
     std::memset(&P, 0, sizeof(P));
     float4* Tq[2] = {M.get<float4>(n), M.get<float4>(n)};
-    P.T = Tq[0];  // k_pt_init writes bounce 0's
-    P.Tn = Tq[1];
     P.L = M.get<float4>(n);
     P.cur = M.get<uint32_t>(n);
     P.ca = M.get<float4>(n);
@@ -881,43 +909,55 @@ static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, ui
     else
         st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg);
     if (st != RB_OK) return st;
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, 0u, 1u, 2u);
-    hipLaunchKernelGGL(k_pt_init, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, phits, n, P);
     TileArgs T{cam->width, x0, y0, w, p->spp};
     const SceneDev S = scene_of(bvh);
     const RngArg rng = rng_arg(p->seed, RNG_PATH);
-    for (uint32_t b = 0, qi = 0; b <= p->max_depth; b++, qi ^= 1) {
-        // this bounce's vertices, and the buffers its MIS + bounce rays go to
+    const float nl = (float)bvh->n_lights;
+    auto step = [&](int mode, uint32_t qin) {
+        const dim3 g(grid_q(n)), blk(kQueueBlock);
+        const uint32_t md = p->max_depth, rs = p->rr_start;
+        const float rp = p->rr_prob;
+        if (bd) {
+            if (mode == 0) hipLaunchKernelGGL((k_pt_step<true, 0>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else if (mode == 1) hipLaunchKernelGGL((k_pt_step<true, 1>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else hipLaunchKernelGGL((k_pt_step<true, 2>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+        } else {
+            if (mode == 0) hipLaunchKernelGGL((k_pt_step<false, 0>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else if (mode == 1) hipLaunchKernelGGL((k_pt_step<false, 1>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else hipLaunchKernelGGL((k_pt_step<false, 2>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+        }
+    };
+    // Step b takes bounce b - 1's contributions and shades bounce b: its
+    // input queue (the vertices shaded at b - 1) is (b + 1) & 1, its output
+    // queue b & 1; each bounce's rays are traced between the steps.
+    for (uint32_t b = 0; b <= p->max_depth; b++) {
+        const uint32_t qin = (b + 1) & 1, qout = b & 1;
         P.pray = b == 0 ? prim : rbcq[(b - 1) & 1];
         P.phit = b == 0 ? phits : hbcq[(b - 1) & 1];
         P.rbc = rbcq[b & 1];
         P.hbc = hbcq[b & 1];
         P.T = Tq[b & 1];
         P.Tn = Tq[(b + 1) & 1];
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qi ^ 1, 2u, 3u);
-        if (bd) {
-            hipLaunchKernelGGL(k_pt_shade<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
-                               p->rr_start, p->rr_prob, B);
-            if (B.nconn)
-                hipLaunchKernelGGL(k_lvc_connect, dim3(grid_q(n * B.nconn)), dim3(kQueueBlock), 0, s, P, qi, T, S,
-                                   p->max_depth, B);
-        } else
-            hipLaunchKernelGGL(k_pt_shade<false>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, T, S, rng, p->max_depth,
-                               p->rr_start, p->rr_prob, B);
+        P.Tprev = Tq[(b + 1) & 1];  // bounce b - 1's T; this bounce's Tn overwrites it path by path
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qout, 2u, 3u);
+        step(b == 0 ? 0 : 1, qin);
+        if (bd && B.nconn)
+            hipLaunchKernelGGL(k_lvc_connect, dim3(grid_q(n * B.nconn)), dim3(kQueueBlock), 0, s, P, qout, T, S,
+                               p->max_depth, B);
+        if (p->path_counter)
+            hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt + qout, p->path_counter + b);
         // shadow rays (any hit) and MIS + bounce rays (closest hit) in one launch
         if ((st = launch_trace_pair(bvh, (rb_ray*)P.ra, na, P.cnt + 2, (rb_hit*)P.ha, RB_ANY, (rb_ray*)P.rbc, 2 * n,
                                     P.cnt + 3, (rb_hit*)P.hbc, s, dev().cfg)) != RB_OK)
             return st;
-        if (bd)
-            hipLaunchKernelGGL(k_pt_connect<true>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, (float)bvh->n_lights,
-                               B.nconn);
-        else
-            hipLaunchKernelGGL(k_pt_connect<false>, dim3(grid_q(n)), dim3(kQueueBlock), 0, s, P, qi, (float)bvh->n_lights,
-                               0u);
         if (p->ray_counter)
             hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt, p->ray_counter,
                                b == 0 ? (unsigned long long)n : 0ull);
     }
+    // the last bounce's contributions
+    P.phit = hbcq[p->max_depth & 1];
+    P.Tprev = Tq[p->max_depth & 1];
+    step(2, p->max_depth & 1);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "camera pass launch");
     return film(P.L, cam, x0, y0, w, h, p, out, s);
@@ -1006,6 +1046,8 @@ rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t 
                                  LP.cnt + qi);
             if (r != RB_OK) return r;
             if (p->ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->ray_counter);
+            if (p->light_path_counter)
+                hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->light_path_counter + d);
             hipLaunchKernelGGL(k_light_hit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, qi, d, S, lrng, LP, B.C);
         }
         return RB_OK;
@@ -1046,6 +1088,8 @@ void rb_render_params_default(rb_render_params* p) {
     p->connections = 5;
     p->light_max_depth = 6;
     p->ray_counter = nullptr;
+    p->path_counter = nullptr;
+    p->light_path_counter = nullptr;
     p->filter = RB_FILTER_BOX;
     p->filter_width[0] = 0.5f;
     p->filter_width[1] = 0.5f;

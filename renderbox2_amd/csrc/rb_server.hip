@@ -232,6 +232,7 @@ void rb_server_s::launch(Slot& sl, Req* q) {
     if (st == RB_OK) {
         rb_render_params p = q->r.params;
         p.ray_counter = sl.d_counter;
+        p.path_counter = p.light_path_counter = nullptr;
         t_arena = q->r.kind == RB_REQ_TRACE ? nullptr : &arena;
         switch (q->r.kind) {
             case RB_REQ_AO: st = render_ao(q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out, sl.stream); break;

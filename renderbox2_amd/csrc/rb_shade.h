@@ -368,7 +368,10 @@ RB_HD void layer_sample(const rb_material& m, V3 wo, V3 ng, V3 ns, V3 nf, float 
 RB_HD uint32_t mat_layers(const rb_material& m) { return m.kind == RB_MAT_LAYERED ? (uint32_t)m.param[0] : 1u; }
 
 __device__ __forceinline__ rb_material mat_layer(const SceneDev& S, const rb_material& m, uint32_t k) {
-    return S.mats[(uint32_t)m.param[1 + k]];
+    // a select, not m.param[1 + k]: indexing the register copy at run time
+    // would move the whole material to scratch memory
+    const float id = k == 0 ? m.param[1] : k == 1 ? m.param[2] : k == 2 ? m.param[3] : m.param[4];
+    return S.mats[(uint32_t)id];
 }
 
 // every layer specular: no light sample and no connection at this vertex
