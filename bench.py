@@ -306,6 +306,10 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, policy="deadline", see
     burst  : every request submitted to the server at once -> capacity;
     poisson: arrivals at `load` x capacity -> p50/p99 latency (submit ->
              result on the host) and aggregate Mrays/s.
+    The server runs burst and poisson in both modes: each request on its own
+    stream (the top-level figures) and in lockstep (one batched trace launch
+    of every in-flight request's rays per round; "lockstep"), the poisson
+    arrivals of both at `load` x the stream mode's capacity.
     Rays per request come from the device ray counter; every served image is
     checked against the plain one by digest."""
     import ctypes as C
@@ -349,8 +353,8 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, policy="deadline", see
     total_rays = sum(rays)
     kinds = [q[0] for q in reqs]
 
-    def run(arrivals):
-        srv = rb.RenderServer(max_inflight=max_inflight, policy=policy)
+    def run(arrivals, mode="streams"):
+        srv = rb.RenderServer(max_inflight=max_inflight, policy=policy, mode=mode)
         sids = [srv.add_scene(t) for t in tracers]
         rq = queue.Queue()
 
@@ -373,7 +377,7 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, policy="deadline", see
             lat.append(st.latency_ms)
             got_rays += st.rays
             if time.time() - tl > 10:
-                log(f"served/{'poisson' if arrivals is not None else 'burst'}: {i + 1}/{nreq} done")
+                log(f"served/{mode}/{'poisson' if arrivals is not None else 'burst'}: {i + 1}/{nreq} done")
                 tl = time.time()
         th.join()
         stt = srv.stats()
@@ -386,6 +390,16 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, policy="deadline", see
     arr = np.cumsum(np.random.default_rng(seed + 1).exponential(1.0 / (load * cap), nreq))
     ok_p, lat_p, st_p = run(arr)
     span_p = st_p["first_submit_to_last_done_s"]
+    ok_lb, lat_lb, st_lb = run(None, "lockstep")
+    span_lb = st_lb["first_submit_to_last_done_s"]
+    ok_lp, lat_lp, st_lp = run(arr, "lockstep")  # the same arrivals
+    span_lp = st_lp["first_submit_to_last_done_s"]
+    lockstep = {"served_capacity_mrays": round(total_rays / span_lb / 1e6, 1), "burst_s": round(span_lb, 2),
+                "speedup": round(t_plain / span_lb, 3), "poisson_rate_per_s": round(load * cap, 2),
+                "poisson_mrays": round(total_rays / span_lp / 1e6, 1), "poisson_s": round(span_lp, 2),
+                "p50_ms": round(st_lp["p50_ms"], 2), "p99_ms": round(st_lp["p99_ms"], 2),
+                "burst_p50_ms": round(st_lb["p50_ms"], 2), "burst_p99_ms": round(st_lb["p99_ms"], 2),
+                "arena_misses": st_lp["arena_misses"], "match": bool(ok_lb and ok_lp)}
 
     def pct(a, q):
         return round(float(np.percentile(a, q)), 2) if len(a) else None
@@ -405,7 +419,7 @@ def bench_served(rb, nreq=2000, load=0.8, max_inflight=8, policy="deadline", see
             "burst_p50_ms": round(st_b["p50_ms"], 2), "burst_p99_ms": round(st_b["p99_ms"], 2),
             "per_kind": per_kind, "max_inflight": max_inflight, "policy": policy, "deferrals": st_p["deferrals"],
             "arena_misses": st_p["arena_misses"], "peak_arena_mb": round(st_p["peak_arena_bytes"] / 2**20, 1),
-            "match": bool(ok_b and ok_p), "setup_s": round(setup_s, 1)}
+            "match": bool(ok_b and ok_p), "setup_s": round(setup_s, 1), "lockstep": lockstep}
 
 
 def main():

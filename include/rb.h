@@ -563,9 +563,11 @@ rb_status rb_render_graph_destroy(rb_render_graph_t graph);
  * reference renders one configured frame per process run,
  * src/core/application.cpp:225-239).  Scenes stay resident; requests are
  * admitted while fewer than max_inflight run and their estimated device
- * memory fits the budget; each running request owns a HIP stream so their
- * kernels overlap on the GPU; a scheduler thread inside the library
- * launches, tracks completion and records latency (submit -> result ready). */
+ * memory fits the budget; a scheduler thread inside the library launches,
+ * tracks completion and records latency (submit -> result ready).  The
+ * running requests either each own a HIP stream so their kernels overlap on
+ * the GPU, or advance in lockstep with one batched trace per round (`mode`
+ * below).  Results are the same either way, bit for bit. */
 typedef struct rb_server_s* rb_server_t;
 
 enum { RB_SCHED_DEADLINE = 0, RB_SCHED_FIFO = 1 };
@@ -576,7 +578,13 @@ typedef struct rb_server_config {
     uint32_t max_queue;    /* waiting requests before submit says RB_ERR_BUSY (0: 65536) */
     uint32_t policy;       /* RB_SCHED_DEADLINE (default): earliest of submit + 4 x the
                               estimated GPU time first; RB_SCHED_FIFO: arrival order */
+    uint32_t mode;         /* RB_SERVE_STREAMS (default): each request's pass on its own
+                              stream, the streams overlapping on the GPU; RB_SERVE_LOCKSTEP:
+                              the in-flight requests advance together on one stream, one
+                              integrator step each per round and one batched trace launch
+                              of all their rays per round */
 } rb_server_config;
+enum { RB_SERVE_STREAMS = 0, RB_SERVE_LOCKSTEP = 1 };
 
 enum { RB_REQ_AO = 0, RB_REQ_PT = 1, RB_REQ_BDPT = 2, RB_REQ_TRACE = 3, RB_REQ_RAYCAST = 4 };
 enum { RB_REQ_QUEUED = 0, RB_REQ_RUNNING = 1, RB_REQ_DONE = 2, RB_REQ_FAILED = 3 };

@@ -170,10 +170,11 @@ RB_REQ_QUEUED, RB_REQ_RUNNING, RB_REQ_DONE, RB_REQ_FAILED = 0, 1, 2, 3
 
 
 RB_SCHED_DEADLINE, RB_SCHED_FIFO = 0, 1
+RB_SERVE_STREAMS, RB_SERVE_LOCKSTEP = 0, 1
 
 
 class rb_server_config(C.Structure):
-    _fields_ = [("max_inflight", U32), ("hbm_budget", U64), ("max_queue", U32), ("policy", U32)]
+    _fields_ = [("max_inflight", U32), ("hbm_budget", U64), ("max_queue", U32), ("policy", U32), ("mode", U32)]
 
 
 class rb_request_status(C.Structure):

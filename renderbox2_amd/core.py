@@ -650,16 +650,18 @@ class RenderServer:
 
     The reference renders one configured frame per process run
     (src/core/application.cpp:225-239); this keeps scenes resident and admits
-    AO / PT / BDPT tiles and raw trace batches as they arrive, each on its own
-    HIP stream, within a device-memory budget.  Results are independent of
-    what else runs: every random number is keyed by (request seed, pixel,
-    sample, bounce)."""
+    AO / PT / BDPT tiles and raw trace batches as they arrive within a
+    device-memory budget, each on its own HIP stream (mode "streams") or all
+    in lockstep with one batched trace launch per round (mode "lockstep").
+    Results are independent of what else runs and of the mode: every random
+    number is keyed by (request seed, pixel, sample, bounce)."""
 
     KINDS = {"ao": L.RB_REQ_AO, "pt": L.RB_REQ_PT, "bdpt": L.RB_REQ_BDPT, "raycast": L.RB_REQ_RAYCAST}
 
-    def __init__(self, max_inflight=8, hbm_budget=0, max_queue=0, policy="deadline"):
+    def __init__(self, max_inflight=8, hbm_budget=0, max_queue=0, policy="deadline", mode="streams"):
         pol = {"deadline": L.RB_SCHED_DEADLINE, "fifo": L.RB_SCHED_FIFO}[policy]
-        cfg = L.rb_server_config(max_inflight, hbm_budget, max_queue, pol)
+        md = {"streams": L.RB_SERVE_STREAMS, "lockstep": L.RB_SERVE_LOCKSTEP}[mode]
+        cfg = L.rb_server_config(max_inflight, hbm_budget, max_queue, pol, md)
         h = C.c_void_p()
         check(lib().rb_server_create(C.byref(cfg), C.byref(h)))
         self.handle = h.value

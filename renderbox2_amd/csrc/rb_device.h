@@ -2,6 +2,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <memory>
+
 #include "rb_internal.h"
 #include "rb_rng.h"
 
@@ -111,6 +113,52 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
 rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, const uint32_t* n0_dev, rb_hit* hits0,
                             uint32_t flags0, const rb_ray* rays1, uint32_t cap1, const uint32_t* n1_dev, rb_hit* hits1,
                             hipStream_t stream, const rb_trace_config& cfg);
+
+// A ray set of one BVH for a batched launch: n rays, or *n_dev of them
+// (bounded by n) when the count is in device memory.
+struct BatchSet {
+    rb_bvh_t bvh;
+    const float4* rays;
+    int2* hits;
+    uint32_t n;
+    const uint32_t* n_dev;
+    bool any;
+    const uint8_t* mask;  // NULL: every ray is traced
+};
+// Ray sets of any resident BVHs through batched persistent launches (up to
+// 64 sets and 2^31 - 1 rays of bound per launch).  Same per-ray results as
+// one launch_trace per set.
+rb_status launch_trace_sets(const BatchSet* sets, uint32_t ns, hipStream_t stream, const rb_trace_config& cfg);
+
+// A ray set a render step leaves to trace.
+struct TraceSet {
+    const float4* rays;
+    int2* hits;
+    uint32_t n;             // rays, or the bound of *n_dev
+    const uint32_t* n_dev;  // device count (NULL: n)
+    uint32_t flags;         // RB_CLOSEST or RB_ANY, with RB_MASKED
+    const uint8_t* mask;
+    rb_trace_config cfg;    // launch shape when the set is traced on its own
+};
+
+// A render call as a sequence of steps (rb_render.hip).  step() enqueues on
+// s the kernels up to the job's next trace and returns that trace's ray sets
+// (one or two); *done once the last step has written the film.  All device
+// memory is taken when the job is made (from the server's arena when
+// t_arena is set) and released when it is destroyed.
+class RenderJob {
+  public:
+    virtual ~RenderJob() {}
+    virtual rb_status step(hipStream_t s, TraceSet* sets, uint32_t* nsets, bool* done) = 0;
+    rb_bvh_t bvh = nullptr;
+};
+// The job of one render call of kind RB_REQ_AO / _PT / _BDPT / _RAYCAST
+// (the checks of the render_* calls; s: the stream its memory is freed on).
+rb_status make_render_job(uint32_t kind, rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w,
+                          uint32_t h, const rb_render_params* p, float* out, hipStream_t s,
+                          std::unique_ptr<RenderJob>* job);
+// Runs a job on one stream, each step's sets in one launch.
+rb_status run_job(RenderJob& J, hipStream_t s);
 
 // Render passes (rb_render.hip), asynchronous on stream s.
 rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,

@@ -21,8 +21,6 @@
 //    triangle or Gaussian; kernel_update_film,
 //    src/renderers/samplerrenderer.cu:103-169), in a fixed order and without
 //    float atomics, so images are bit-reproducible.
-#include <functional>
-
 #include "rb_shade.h"
 
 namespace rb {
@@ -801,179 +799,6 @@ static SceneDev scene_of(rb_bvh_t b) {
     return S;
 }
 
-rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                    const rb_render_params* p, float* out, hipStream_t s) {
-    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
-    if (st != RB_OK) return st;
-    if (p->ao_samples == 0) return fail(RB_ERR_INVALID, "ao_samples must be >= 1");
-    const uint32_t n = w * h * p->spp;
-    const uint64_t na = (uint64_t)n * p->ao_samples;
-    if (na > 0x7FFFFFFFull) return fail(RB_ERR_INVALID, "too many AO rays for one call");
-    Scratch M(s);
-    float4* rays = M.get<float4>(2 * (size_t)n);
-    int2* hits = M.get<int2>(n);
-    float4* arays = M.get<float4>(2 * na);
-    uint8_t* amask = M.get<uint8_t>(na);
-    int2* ahits = M.get<int2>(na);
-    float4* Ls = M.get<float4>(n);
-    if (M.st != RB_OK) return M.st;
-    if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)rays, s)) != RB_OK) return st;
-    if ((st = launch_trace(bvh, (rb_ray*)rays, n, RB_CLOSEST, nullptr, (rb_hit*)hits, nullptr, s, dev().cfg)) != RB_OK)
-        return st;
-    TileArgs T{cam->width, x0, y0, w, p->spp};
-    hipLaunchKernelGGL(k_ao_spawn, dim3(grid_of(na)), dim3(256), 0, s, rays, hits, n, T, scene_of(bvh), p->ao_samples,
-                       p->ao_radius, rng_arg(p->seed, RNG_AO), arays, amask, p->ray_counter);
-    if (p->ray_counter) hipLaunchKernelGGL(k_add64, dim3(1), dim3(64), 0, s, p->ray_counter, (unsigned long long)n);
-    // AO rays are short (tmax = the AO radius): a wave finishes a claim of
-    // rays quickly and the launch's work-counter atomics set the pace;
-    // claims of 128 rays halve them (AO frame -19% at 2 Mi rays) once the
-    // launch is large enough for every resident wave to get several claims
-    // (small tiles keep the finer claims).
-    rb_trace_config ao_cfg = dev().cfg;
-    const uint64_t resident = (uint64_t)dev().num_cus * ao_cfg.waves_per_cu;
-    if (ao_cfg.chunk < 128 && na >= resident * 128 * 2) ao_cfg.chunk = 128;
-    if ((st = launch_trace(bvh, (rb_ray*)arays, (uint32_t)na, RB_ANY | RB_MASKED, amask, (rb_hit*)ahits, nullptr, s,
-                           ao_cfg)) != RB_OK)
-        return st;
-    hipLaunchKernelGGL(k_ao_resolve, dim3(grid_of(n)), dim3(256), 0, s, hits, ahits, n, p->ao_samples, Ls);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "AO launch");
-    return film(Ls, cam, x0, y0, w, h, p, out, s);
-}
-
-rb_status render_raycast(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                         const rb_render_params* p, float* out, hipStream_t s) {
-    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
-    if (st != RB_OK) return st;
-    if (p->shade > RB_SHADE_UNIFORM) return fail(RB_ERR_INVALID, "unknown ray-cast shade mode %u", p->shade);
-    const uint32_t n = w * h * p->spp;
-    Scratch M(s);
-    float4* rays = M.get<float4>(2 * (size_t)n);
-    int2* hits = M.get<int2>(n);
-    float4* Ls = M.get<float4>(n);
-    if (M.st != RB_OK) return M.st;
-    if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)rays, s)) != RB_OK) return st;
-    if ((st = launch_trace(bvh, (rb_ray*)rays, n, RB_CLOSEST, nullptr, (rb_hit*)hits, nullptr, s, dev().cfg)) != RB_OK)
-        return st;
-    if (p->ray_counter) hipLaunchKernelGGL(k_add64, dim3(1), dim3(64), 0, s, p->ray_counter, (unsigned long long)n);
-    hipLaunchKernelGGL(k_raycast_shade, dim3(grid_of(n)), dim3(256), 0, s, rays, hits, n, scene_of(bvh), p->shade, Ls);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "ray-cast launch");
-    return film(Ls, cam, x0, y0, w, h, p, out, s);
-}
-
-// Traces the camera rays (prim, n of them) into phits; BDPT runs its light
-// pass here, its first launch paired with the camera rays.
-typedef std::function<rb_status(const float4* prim, int2* phits, uint32_t n)> PrimaryTrace;
-
-// Camera pass shared by the path tracer (BD = false) and BDPT (BD = true).
-static rb_status camera_pass(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                             const rb_render_params* p, float* out, hipStream_t s, Scratch& M, bool bd,
-                             const BdArgs& B, const PrimaryTrace* primaries = nullptr) {
-    rb_status st;
-    const uint32_t n = w * h * p->spp;
-    // one launch traces at most 2^31 - 1 rays: the shadow queue (light
-    // samples + connections) and the MIS + bounce queue (2 per path)
-    const uint64_t na64 = bd ? (uint64_t)n * (1 + B.nconn) : n;
-    if (na64 > 0x7FFFFFFFull || 2ull * n > 0x7FFFFFFFull)
-        return fail(RB_ERR_INVALID, "tile x spp too large: %u paths need %llu shadow rays and %llu bounce rays per launch "
-                    "(at most 2^31 - 1)", n, (unsigned long long)na64, 2ull * n);
-    const uint32_t na = (uint32_t)na64;
-    float4* prim = M.get<float4>(2 * (size_t)n);
-    int2* phits = M.get<int2>(n);
-    PathBufs P;
-    std::memset(&P, 0, sizeof(P));
-    float4* Tq[2] = {M.get<float4>(n), M.get<float4>(n)};
-    P.L = M.get<float4>(n);
-    P.cur = M.get<uint32_t>(n);
-    P.ca = M.get<float4>(n);
-    P.cb = M.get<float4>(n);
-    P.slot = M.get<int4>(n);
-    P.info = M.get<uint32_t>(n);
-    P.q[0] = M.get<uint32_t>(n);
-    P.q[1] = M.get<uint32_t>(n);
-    P.cnt = M.get<uint32_t>(4);
-    P.ra = M.get<float4>(2 * (size_t)na);
-    P.ha = M.get<int2>(na);
-    float4* rbcq[2] = {M.get<float4>(4 * (size_t)n), M.get<float4>(4 * (size_t)n)};
-    int2* hbcq[2] = {M.get<int2>(2 * (size_t)n), M.get<int2>(2 * (size_t)n)};
-    if (bd) {
-        P.cmask = M.get<uint32_t>(n);
-        P.cc = M.get<float4>((size_t)n * kMaxConn);
-        P.cslot = M.get<int>((size_t)n * kMaxConn);
-    }
-    if (M.st != RB_OK) return M.st;
-    if ((st = rb_raygen_camera(cam, x0, y0, w, h, p->spp, p->seed, 1, (rb_ray*)prim, s)) != RB_OK) return st;
-    if (primaries)
-        st = (*primaries)(prim, phits, n);
-    else
-        st = launch_trace(bvh, (rb_ray*)prim, n, RB_CLOSEST, nullptr, (rb_hit*)phits, nullptr, s, dev().cfg);
-    if (st != RB_OK) return st;
-    TileArgs T{cam->width, x0, y0, w, p->spp};
-    const SceneDev S = scene_of(bvh);
-    const RngArg rng = rng_arg(p->seed, RNG_PATH);
-    const float nl = (float)bvh->n_lights;
-    auto step = [&](int mode, uint32_t qin) {
-        const dim3 g(grid_q(n)), blk(kQueueBlock);
-        const uint32_t md = p->max_depth, rs = p->rr_start;
-        const float rp = p->rr_prob;
-        if (bd) {
-            if (mode == 0) hipLaunchKernelGGL((k_pt_step<true, 0>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
-            else if (mode == 1) hipLaunchKernelGGL((k_pt_step<true, 1>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
-            else hipLaunchKernelGGL((k_pt_step<true, 2>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
-        } else {
-            if (mode == 0) hipLaunchKernelGGL((k_pt_step<false, 0>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
-            else if (mode == 1) hipLaunchKernelGGL((k_pt_step<false, 1>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
-            else hipLaunchKernelGGL((k_pt_step<false, 2>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
-        }
-    };
-    // Step b takes bounce b - 1's contributions and shades bounce b: its
-    // input queue (the vertices shaded at b - 1) is (b + 1) & 1, its output
-    // queue b & 1; each bounce's rays are traced between the steps.
-    for (uint32_t b = 0; b <= p->max_depth; b++) {
-        const uint32_t qin = (b + 1) & 1, qout = b & 1;
-        P.pray = b == 0 ? prim : rbcq[(b - 1) & 1];
-        P.phit = b == 0 ? phits : hbcq[(b - 1) & 1];
-        P.rbc = rbcq[b & 1];
-        P.hbc = hbcq[b & 1];
-        P.T = Tq[b & 1];
-        P.Tn = Tq[(b + 1) & 1];
-        P.Tprev = Tq[(b + 1) & 1];  // bounce b - 1's T; this bounce's Tn overwrites it path by path
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qout, 2u, 3u);
-        step(b == 0 ? 0 : 1, qin);
-        if (bd && B.nconn)
-            hipLaunchKernelGGL(k_lvc_connect, dim3(grid_q(n * B.nconn)), dim3(kQueueBlock), 0, s, P, qout, T, S,
-                               p->max_depth, B);
-        if (p->path_counter)
-            hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt + qout, p->path_counter + b);
-        // shadow rays (any hit) and MIS + bounce rays (closest hit) in one launch
-        if ((st = launch_trace_pair(bvh, (rb_ray*)P.ra, na, P.cnt + 2, (rb_hit*)P.ha, RB_ANY, (rb_ray*)P.rbc, 2 * n,
-                                    P.cnt + 3, (rb_hit*)P.hbc, s, dev().cfg)) != RB_OK)
-            return st;
-        if (p->ray_counter)
-            hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt, p->ray_counter,
-                               b == 0 ? (unsigned long long)n : 0ull);
-    }
-    // the last bounce's contributions
-    P.phit = hbcq[p->max_depth & 1];
-    P.Tprev = Tq[p->max_depth & 1];
-    step(2, p->max_depth & 1);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "camera pass launch");
-    return film(P.L, cam, x0, y0, w, h, p, out, s);
-}
-
-rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                    const rb_render_params* p, float* out, hipStream_t s) {
-    rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
-    if (st != RB_OK) return st;
-    if (p->max_depth > 250) return fail(RB_ERR_INVALID, "max_depth must be <= 250");
-    Scratch M(s);
-    BdArgs B;
-    std::memset(&B, 0, sizeof(B));
-    return camera_pass(bvh, cam, x0, y0, w, h, p, out, s, M, false, B);
-}
-
 uint64_t bdpt_scratch_bytes(uint32_t w, uint32_t h, const rb_render_params* p) {
     const uint64_t n = (uint64_t)w * h * (p->spp ? p->spp : 1);
     const uint64_t D = p->light_max_depth ? p->light_max_depth : 1;
@@ -984,76 +809,408 @@ uint64_t bdpt_scratch_bytes(uint32_t w, uint32_t h, const rb_render_params* p) {
     return n * 352 + n * (4 + kMaxConn * 20) + n * nc * 40 + nl * D * 80 + nl * (32 + 16 + 4 + 8 + 4 + 8 + 32);
 }
 
-rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
-                      const rb_render_params* p, float* out, hipStream_t s) {
+// ------------------------------------------------------------- jobs ----
+// A render call as a sequence of steps.  Each step enqueues the integrator
+// kernels that lead up to a trace and names the ray sets that trace takes
+// (one or two); the last step writes the film.  render_* run a job's steps
+// with one launch per step (a pair launch for two sets); the server's
+// lockstep mode traces the sets of every in-flight job in one batched launch
+// per step.  Every buffer is allocated when the job is made.
+
+struct AoJob : RenderJob {
+    rb_camera cam;
+    rb_render_params p;
+    uint32_t x0, y0, w, h, n;
+    uint64_t na;
+    float* out;
+    Scratch M;
+    float4 *rays, *arays, *Ls;
+    int2 *hits, *ahits;
+    uint8_t* amask;
+    int state = 0;
+    explicit AoJob(hipStream_t s) : M(s) {}
+    rb_status step(hipStream_t s, TraceSet* sets, uint32_t* nsets, bool* done) override {
+        rb_status st;
+        *nsets = 0;
+        *done = false;
+        if (state == 0) {
+            if ((st = rb_raygen_camera(&cam, x0, y0, w, h, p.spp, p.seed, 1, (rb_ray*)rays, s)) != RB_OK) return st;
+            sets[0] = TraceSet{rays, hits, n, nullptr, RB_CLOSEST, nullptr, dev().cfg};
+            *nsets = 1;
+        } else if (state == 1) {
+            TileArgs T{cam.width, x0, y0, w, p.spp};
+            hipLaunchKernelGGL(k_ao_spawn, dim3(grid_of(na)), dim3(256), 0, s, rays, hits, n, T, scene_of(bvh),
+                               p.ao_samples, p.ao_radius, rng_arg(p.seed, RNG_AO), arays, amask, p.ray_counter);
+            if (p.ray_counter) hipLaunchKernelGGL(k_add64, dim3(1), dim3(64), 0, s, p.ray_counter, (unsigned long long)n);
+            // AO rays are short (tmax = the AO radius): a wave finishes a
+            // claim of rays quickly and the launch's work-counter atomics set
+            // the pace; claims of 128 rays halve them (AO frame -19% at 2 Mi
+            // rays) once the launch is large enough for every resident wave
+            // to get several claims (small tiles keep the finer claims).
+            rb_trace_config ao_cfg = dev().cfg;
+            const uint64_t resident = (uint64_t)dev().num_cus * ao_cfg.waves_per_cu;
+            if (ao_cfg.chunk < 128 && na >= resident * 128 * 2) ao_cfg.chunk = 128;
+            sets[0] = TraceSet{arays, ahits, (uint32_t)na, nullptr, RB_ANY | RB_MASKED, amask, ao_cfg};
+            *nsets = 1;
+        } else {
+            hipLaunchKernelGGL(k_ao_resolve, dim3(grid_of(n)), dim3(256), 0, s, hits, ahits, n, p.ao_samples, Ls);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return hip_fail(e, "AO launch");
+            if ((st = film(Ls, &cam, x0, y0, w, h, &p, out, s)) != RB_OK) return st;
+            *done = true;
+        }
+        state++;
+        return RB_OK;
+    }
+};
+
+struct RaycastJob : RenderJob {
+    rb_camera cam;
+    rb_render_params p;
+    uint32_t x0, y0, w, h, n;
+    float* out;
+    Scratch M;
+    float4 *rays, *Ls;
+    int2* hits;
+    int state = 0;
+    explicit RaycastJob(hipStream_t s) : M(s) {}
+    rb_status step(hipStream_t s, TraceSet* sets, uint32_t* nsets, bool* done) override {
+        rb_status st;
+        *nsets = 0;
+        *done = false;
+        if (state == 0) {
+            if ((st = rb_raygen_camera(&cam, x0, y0, w, h, p.spp, p.seed, 1, (rb_ray*)rays, s)) != RB_OK) return st;
+            sets[0] = TraceSet{rays, hits, n, nullptr, RB_CLOSEST, nullptr, dev().cfg};
+            *nsets = 1;
+        } else {
+            if (p.ray_counter) hipLaunchKernelGGL(k_add64, dim3(1), dim3(64), 0, s, p.ray_counter, (unsigned long long)n);
+            hipLaunchKernelGGL(k_raycast_shade, dim3(grid_of(n)), dim3(256), 0, s, rays, hits, n, scene_of(bvh), p.shade,
+                               Ls);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return hip_fail(e, "ray-cast launch");
+            if ((st = film(Ls, &cam, x0, y0, w, h, &p, out, s)) != RB_OK) return st;
+            *done = true;
+        }
+        state++;
+        return RB_OK;
+    }
+};
+
+// The path tracer (bd = false) and BDPT (bd = true): BDPT's light pass
+// (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168), its first
+// trace paired with the camera rays (independent work, one launch tail
+// instead of two), then the camera pass (integrator_path.cu:43-261;
+// kernel_sk_camera_trace, integrator_bdpt_lvc_sk.cu:170-435): step b takes
+// bounce b - 1's contributions and shades bounce b -- its input queue (the
+// vertices shaded at b - 1) is (b + 1) & 1, its output queue b & 1 -- and
+// each bounce's shadow rays (any hit) and MIS + bounce rays (closest hit)
+// are traced between the steps.
+struct CameraJob : RenderJob {
+    rb_camera cam;
+    rb_render_params p;
+    uint32_t x0, y0, w, h, n, na;
+    float* out;
+    bool bd;
+    Scratch M;
+    PathBufs P;
+    BdArgs B;
+    float4* prim;
+    int2* phits;
+    float4* Tq[2];
+    float4* rbcq[2];
+    int2* hbcq[2];
+    // BDPT light pass
+    LightPath LP;
+    float4* lrays = nullptr;
+    uint32_t* ncam = nullptr;
+    uint32_t NL = 0, D = 0;
+    // progress: 0 start, 1 light depths, 2 camera bounces, 3 last contributions
+    int phase = 0;
+    uint32_t d = 0, b = 0;
+    explicit CameraJob(hipStream_t s) : M(s) {}
+
+    TileArgs tile() const { return TileArgs{cam.width, x0, y0, w, p.spp}; }
+
+    void launch_step(hipStream_t s, int mode, uint32_t qin) {
+        const dim3 g(grid_q(n)), blk(kQueueBlock);
+        const TileArgs T = tile();
+        const SceneDev S = scene_of(bvh);
+        const RngArg rng = rng_arg(p.seed, RNG_PATH);
+        const uint32_t md = p.max_depth, rs = p.rr_start;
+        const float rp = p.rr_prob, nl = (float)bvh->n_lights;
+        if (bd) {
+            if (mode == 0) hipLaunchKernelGGL((k_pt_step<true, 0>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else if (mode == 1) hipLaunchKernelGGL((k_pt_step<true, 1>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else hipLaunchKernelGGL((k_pt_step<true, 2>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+        } else {
+            if (mode == 0) hipLaunchKernelGGL((k_pt_step<false, 0>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else if (mode == 1) hipLaunchKernelGGL((k_pt_step<false, 1>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+            else hipLaunchKernelGGL((k_pt_step<false, 2>), g, blk, 0, s, P, qin, n, T, S, rng, md, rs, rp, nl, B);
+        }
+    }
+
+    // the light rays of depth dd (queue dd & 1) for the next trace
+    void light_rays(hipStream_t s, uint32_t dd) {
+        const uint32_t qi = dd & 1;
+        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, qi ^ 1, 2u, 3u);
+        hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, qi, LP, lrays);
+    }
+
+    rb_status step(hipStream_t s, TraceSet* sets, uint32_t* nsets, bool* done) override {
+        rb_status st;
+        *nsets = 0;
+        *done = false;
+        const TileArgs T = tile();
+        const SceneDev S = scene_of(bvh);
+        if (phase == 0) {
+            if ((st = rb_raygen_camera(&cam, x0, y0, w, h, p.spp, p.seed, 1, (rb_ray*)prim, s)) != RB_OK) return st;
+            if (!bd) {
+                sets[0] = TraceSet{prim, phits, n, nullptr, RB_CLOSEST, nullptr, dev().cfg};
+                *nsets = 1;
+                phase = 2;
+                return RB_OK;
+            }
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, ncam, n);
+            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
+            hipLaunchKernelGGL(k_light_emit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, NL, S,
+                               rng_arg(p.seed, RNG_LIGHT), LP, B.C);
+            light_rays(s, 0);
+            sets[0] = TraceSet{lrays, LP.hit, NL, LP.cnt + 0, RB_CLOSEST, nullptr, dev().cfg};
+            sets[1] = TraceSet{prim, phits, n, ncam, RB_CLOSEST, nullptr, dev().cfg};
+            *nsets = 2;
+            phase = 1;
+            return RB_OK;
+        }
+        if (phase == 1) {  // light depth d has been traced
+            const uint32_t qi = d & 1;
+            if (p.ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p.ray_counter);
+            if (p.light_path_counter)
+                hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p.light_path_counter + d);
+            hipLaunchKernelGGL(k_light_hit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, qi, d, S, rng_arg(p.seed, RNG_LIGHT),
+                               LP, B.C);
+            if (++d < D) {
+                light_rays(s, d);
+                sets[0] = TraceSet{lrays, LP.hit, NL, LP.cnt + (d & 1), RB_CLOSEST, nullptr, dev().cfg};
+                *nsets = 1;
+                return RB_OK;
+            }
+            phase = 2;  // the camera pass starts in this step
+        }
+        if (phase == 2) {
+            if (b > 0 && p.ray_counter)  // the previous bounce's rays
+                hipLaunchKernelGGL(k_count_rays, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt, p.ray_counter,
+                                   b == 1 ? (unsigned long long)n : 0ull);
+            if (b > p.max_depth) {
+                phase = 3;
+            } else {
+                const uint32_t qin = (b + 1) & 1, qout = b & 1;
+                P.pray = b == 0 ? prim : rbcq[(b - 1) & 1];
+                P.phit = b == 0 ? phits : hbcq[(b - 1) & 1];
+                P.rbc = rbcq[b & 1];
+                P.hbc = hbcq[b & 1];
+                P.T = Tq[b & 1];
+                P.Tn = Tq[(b + 1) & 1];
+                P.Tprev = Tq[(b + 1) & 1];  // bounce b - 1's T; this bounce's Tn overwrites it path by path
+                hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, P.cnt, qout, 2u, 3u);
+                launch_step(s, b == 0 ? 0 : 1, qin);
+                if (bd && B.nconn)
+                    hipLaunchKernelGGL(k_lvc_connect, dim3(grid_q((uint64_t)n * B.nconn)), dim3(kQueueBlock), 0, s, P,
+                                       qout, T, S, p.max_depth, B);
+                if (p.path_counter)
+                    hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt + qout,
+                                       p.path_counter + b);
+                sets[0] = TraceSet{P.ra, P.ha, na, P.cnt + 2, RB_ANY, nullptr, dev().cfg};
+                sets[1] = TraceSet{P.rbc, P.hbc, 2 * n, P.cnt + 3, RB_CLOSEST, nullptr, dev().cfg};
+                *nsets = 2;
+                b++;
+                return RB_OK;
+            }
+        }
+        // the last bounce's contributions, then the film
+        P.phit = hbcq[p.max_depth & 1];
+        P.Tprev = Tq[p.max_depth & 1];
+        launch_step(s, 2, p.max_depth & 1);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "camera pass launch");
+        if ((st = film(P.L, &cam, x0, y0, w, h, &p, out, s)) != RB_OK) return st;
+        *done = true;
+        return RB_OK;
+    }
+};
+
+rb_status make_render_job(uint32_t kind, rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w,
+                          uint32_t h, const rb_render_params* p, float* out, hipStream_t s,
+                          std::unique_ptr<RenderJob>* job) {
     rb_status st = check_render(bvh, cam, x0, y0, w, h, p, out);
     if (st != RB_OK) return st;
-    if (p->max_depth > 24) return fail(RB_ERR_INVALID, "BDPT max_depth must be <= 24");
-    if (p->light_max_depth == 0 || p->light_max_depth > 8) return fail(RB_ERR_INVALID, "light_max_depth must be 1..8");
-    // full-path specular masks are 32-bit: path length max_depth + light_max_depth + 2 <= 32
-    if (p->max_depth + p->light_max_depth > 30) return fail(RB_ERR_INVALID, "max_depth + light_max_depth must be <= 30");
-    if (p->connections > kMaxConn) return fail(RB_ERR_INVALID, "connections must be <= %u", kMaxConn);
-    if (p->light_paths > (1u << 24) || (p->light_paths == 0 && p->lvc_capacity == 0))
-        return fail(RB_ERR_INVALID, "light_paths must be 1..2^24 (or 0 with an lvc_capacity)");
-    {
-        const uint64_t npaths = (uint64_t)w * h * p->spp;
-        if (npaths * (1 + p->connections) > 0x7FFFFFFFull || 2 * npaths > 0x7FFFFFFFull)
-            return fail(RB_ERR_INVALID, "tile x spp too large for one call: %llu paths x (1 + %u connections) shadow "
-                        "rays (at most 2^31 - 1 per launch)", (unsigned long long)npaths, p->connections);
-    }
-    Scratch M(s);
-    const uint32_t D = p->light_max_depth;
-    // light sub-paths: given, or as many as fill a cache of lvc_capacity vertices
-    const uint32_t NL = p->light_paths ? p->light_paths : (p->lvc_capacity / D ? p->lvc_capacity / D : 1);
-    BdArgs B;
-    B.nconn = bvh->n_lights ? p->connections : 0;
-    B.tdepth = B.nconn ? D : 0;
-    B.crng = rng_arg(p->seed, RNG_CONNECT);
-    B.C.depth = D;
-    B.C.paths = NL;
-    B.C.rows = bvh->tri_n ? 5u : 4u;
-    B.C.v = M.get<float4>((size_t)B.C.rows * NL * D);
-    B.C.count = M.get<uint32_t>(NL);
-    LightPath LP;
-    LP.ray = M.get<float4>(2 * (size_t)NL);
-    LP.alpha = M.get<float4>(NL);
-    LP.mask = M.get<uint32_t>(NL);
-    LP.q[0] = M.get<uint32_t>(NL);
-    LP.q[1] = M.get<uint32_t>(NL);
-    LP.cnt = M.get<uint32_t>(4);
-    LP.hit = M.get<int2>(NL);
-    float4* lrays = M.get<float4>(2 * (size_t)NL);
-    uint32_t* ncam = M.get<uint32_t>(1);  // camera ray count for the paired launch
-    if (M.st != RB_OK) return M.st;
-    const SceneDev S = scene_of(bvh);
-    const RngArg lrng = rng_arg(p->seed, RNG_LIGHT);
-    // Light pass (kernel_sk_light_trace, integrator_bdpt_lvc_sk.cu:47-168),
-    // run once the camera rays exist: its first trace launch also traces
-    // them (independent work, one launch tail instead of two).
-    const PrimaryTrace light_pass = [&](const float4* prim, int2* phits, uint32_t n) -> rb_status {
-        rb_status r;
-        hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, s, ncam, n);
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
-        hipLaunchKernelGGL(k_light_emit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, NL, S, lrng, LP, B.C);
-        for (uint32_t d = 0, qi = 0; d < D; d++, qi ^= 1) {
-            hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, qi ^ 1, 2u, 3u);
-            hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, qi, LP, lrays);
-            if (d == 0)
-                r = launch_trace_pair(bvh, (rb_ray*)lrays, NL, LP.cnt + qi, (rb_hit*)LP.hit, RB_CLOSEST, (const rb_ray*)prim,
-                                      n, ncam, (rb_hit*)phits, s, dev().cfg);
-            else
-                r = launch_trace(bvh, (rb_ray*)lrays, NL, RB_CLOSEST, nullptr, (rb_hit*)LP.hit, nullptr, s, dev().cfg,
-                                 LP.cnt + qi);
-            if (r != RB_OK) return r;
-            if (p->ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->ray_counter);
-            if (p->light_path_counter)
-                hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p->light_path_counter + d);
-            hipLaunchKernelGGL(k_light_hit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, qi, d, S, lrng, LP, B.C);
+    const uint32_t n = w * h * p->spp;
+    try {
+        if (kind == RB_REQ_AO) {
+            if (p->ao_samples == 0) return fail(RB_ERR_INVALID, "ao_samples must be >= 1");
+            const uint64_t na = (uint64_t)n * p->ao_samples;
+            if (na > 0x7FFFFFFFull) return fail(RB_ERR_INVALID, "too many AO rays for one call");
+            std::unique_ptr<AoJob> J(new AoJob(s));
+            J->bvh = bvh;
+            J->cam = *cam;
+            J->p = *p;
+            J->x0 = x0, J->y0 = y0, J->w = w, J->h = h, J->n = n, J->na = na, J->out = out;
+            Scratch& M = J->M;
+            J->rays = M.get<float4>(2 * (size_t)n);
+            J->hits = M.get<int2>(n);
+            J->arays = M.get<float4>(2 * na);
+            J->amask = M.get<uint8_t>(na);
+            J->ahits = M.get<int2>(na);
+            J->Ls = M.get<float4>(n);
+            if (M.st != RB_OK) return M.st;
+            *job = std::move(J);
+            return RB_OK;
         }
+        if (kind == RB_REQ_RAYCAST) {
+            if (p->shade > RB_SHADE_UNIFORM) return fail(RB_ERR_INVALID, "unknown ray-cast shade mode %u", p->shade);
+            std::unique_ptr<RaycastJob> J(new RaycastJob(s));
+            J->bvh = bvh;
+            J->cam = *cam;
+            J->p = *p;
+            J->x0 = x0, J->y0 = y0, J->w = w, J->h = h, J->n = n, J->out = out;
+            Scratch& M = J->M;
+            J->rays = M.get<float4>(2 * (size_t)n);
+            J->hits = M.get<int2>(n);
+            J->Ls = M.get<float4>(n);
+            if (M.st != RB_OK) return M.st;
+            *job = std::move(J);
+            return RB_OK;
+        }
+        if (kind != RB_REQ_PT && kind != RB_REQ_BDPT) return fail(RB_ERR_INVALID, "unknown render kind %u", kind);
+        const bool bd = kind == RB_REQ_BDPT;
+        if (!bd && p->max_depth > 250) return fail(RB_ERR_INVALID, "max_depth must be <= 250");
+        if (bd) {
+            if (p->max_depth > 24) return fail(RB_ERR_INVALID, "BDPT max_depth must be <= 24");
+            if (p->light_max_depth == 0 || p->light_max_depth > 8)
+                return fail(RB_ERR_INVALID, "light_max_depth must be 1..8");
+            // full-path specular masks are 32-bit: path length max_depth + light_max_depth + 2 <= 32
+            if (p->max_depth + p->light_max_depth > 30)
+                return fail(RB_ERR_INVALID, "max_depth + light_max_depth must be <= 30");
+            if (p->connections > kMaxConn) return fail(RB_ERR_INVALID, "connections must be <= %u", kMaxConn);
+            if (p->light_paths > (1u << 24) || (p->light_paths == 0 && p->lvc_capacity == 0))
+                return fail(RB_ERR_INVALID, "light_paths must be 1..2^24 (or 0 with an lvc_capacity)");
+        }
+        // one launch traces at most 2^31 - 1 rays: the shadow queue (light
+        // samples + connections) and the MIS + bounce queue (2 per path)
+        const uint32_t nconn = bd && bvh->n_lights ? p->connections : 0;
+        const uint64_t na64 = (uint64_t)n * (1 + (bd ? p->connections : 0));
+        if (na64 > 0x7FFFFFFFull || 2ull * n > 0x7FFFFFFFull)
+            return fail(RB_ERR_INVALID, "tile x spp too large: %u paths need %llu shadow rays and %llu bounce rays per "
+                        "launch (at most 2^31 - 1)", n, (unsigned long long)na64, 2ull * n);
+        std::unique_ptr<CameraJob> J(new CameraJob(s));
+        J->bvh = bvh;
+        J->cam = *cam;
+        J->p = *p;
+        J->x0 = x0, J->y0 = y0, J->w = w, J->h = h, J->n = n, J->out = out, J->bd = bd;
+        J->na = (uint32_t)(bd ? (uint64_t)n * (1 + nconn) : n);
+        Scratch& M = J->M;
+        PathBufs& P = J->P;
+        BdArgs& B = J->B;
+        std::memset(&P, 0, sizeof(P));
+        std::memset(&B, 0, sizeof(B));
+        if (bd) {
+            const uint32_t D = p->light_max_depth;
+            // light sub-paths: given, or as many as fill a cache of lvc_capacity vertices
+            const uint32_t NL = p->light_paths ? p->light_paths : (p->lvc_capacity / D ? p->lvc_capacity / D : 1);
+            J->D = D;
+            J->NL = NL;
+            B.nconn = nconn;
+            B.tdepth = B.nconn ? D : 0;
+            B.crng = rng_arg(p->seed, RNG_CONNECT);
+            B.C.depth = D;
+            B.C.paths = NL;
+            B.C.rows = bvh->tri_n ? 5u : 4u;
+            B.C.v = M.get<float4>((size_t)B.C.rows * NL * D);
+            B.C.count = M.get<uint32_t>(NL);
+            LightPath& LP = J->LP;
+            LP.ray = M.get<float4>(2 * (size_t)NL);
+            LP.alpha = M.get<float4>(NL);
+            LP.mask = M.get<uint32_t>(NL);
+            LP.q[0] = M.get<uint32_t>(NL);
+            LP.q[1] = M.get<uint32_t>(NL);
+            LP.cnt = M.get<uint32_t>(4);
+            LP.hit = M.get<int2>(NL);
+            J->lrays = M.get<float4>(2 * (size_t)NL);
+            J->ncam = M.get<uint32_t>(1);  // camera ray count for the paired launch
+        }
+        J->prim = M.get<float4>(2 * (size_t)n);
+        J->phits = M.get<int2>(n);
+        J->Tq[0] = M.get<float4>(n);
+        J->Tq[1] = M.get<float4>(n);
+        P.L = M.get<float4>(n);
+        P.cur = M.get<uint32_t>(n);
+        P.ca = M.get<float4>(n);
+        P.cb = M.get<float4>(n);
+        P.slot = M.get<int4>(n);
+        P.info = M.get<uint32_t>(n);
+        P.q[0] = M.get<uint32_t>(n);
+        P.q[1] = M.get<uint32_t>(n);
+        P.cnt = M.get<uint32_t>(4);
+        P.ra = M.get<float4>(2 * (size_t)J->na);
+        P.ha = M.get<int2>(J->na);
+        for (int k = 0; k < 2; k++) {
+            J->rbcq[k] = M.get<float4>(4 * (size_t)n);
+            J->hbcq[k] = M.get<int2>(2 * (size_t)n);
+        }
+        if (bd) {
+            P.cmask = M.get<uint32_t>(n);
+            P.cc = M.get<float4>((size_t)n * kMaxConn);
+            P.cslot = M.get<int>((size_t)n * kMaxConn);
+        }
+        if (M.st != RB_OK) return M.st;
+        *job = std::move(J);
         return RB_OK;
-    };
-    // camera pass (kernel_sk_camera_trace, integrator_bdpt_lvc_sk.cu:170-435)
-    return camera_pass(bvh, cam, x0, y0, w, h, p, out, s, M, true, B, &light_pass);
+    } catch (...) {
+        return fail(RB_ERR_OOM, "render job (host allocation)");
+    }
+}
+
+// Runs a job on one stream: each step's ray sets in one launch (a pair
+// launch for two sets).
+rb_status run_job(RenderJob& J, hipStream_t s) {
+    TraceSet sets[2];
+    for (;;) {
+        uint32_t ns = 0;
+        bool done = false;
+        rb_status st = J.step(s, sets, &ns, &done);
+        if (st != RB_OK || done) return st;
+        if (ns == 1)
+            st = launch_trace(J.bvh, (const rb_ray*)sets[0].rays, sets[0].n, sets[0].flags, sets[0].mask,
+                              (rb_hit*)sets[0].hits, nullptr, s, sets[0].cfg, sets[0].n_dev);
+        else
+            st = launch_trace_pair(J.bvh, (const rb_ray*)sets[0].rays, sets[0].n, sets[0].n_dev, (rb_hit*)sets[0].hits,
+                                   sets[0].flags, (const rb_ray*)sets[1].rays, sets[1].n, sets[1].n_dev,
+                                   (rb_hit*)sets[1].hits, s, sets[0].cfg);
+        if (st != RB_OK) return st;
+    }
+}
+
+static rb_status render_kind(uint32_t kind, rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w,
+                             uint32_t h, const rb_render_params* p, float* out, hipStream_t s) {
+    std::unique_ptr<RenderJob> J;
+    rb_status st = make_render_job(kind, bvh, cam, x0, y0, w, h, p, out, s, &J);
+    return st == RB_OK ? run_job(*J, s) : st;
+}
+
+rb_status render_ao(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    const rb_render_params* p, float* out, hipStream_t s) {
+    return render_kind(RB_REQ_AO, bvh, cam, x0, y0, w, h, p, out, s);
+}
+rb_status render_raycast(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                         const rb_render_params* p, float* out, hipStream_t s) {
+    return render_kind(RB_REQ_RAYCAST, bvh, cam, x0, y0, w, h, p, out, s);
+}
+rb_status render_pt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                    const rb_render_params* p, float* out, hipStream_t s) {
+    return render_kind(RB_REQ_PT, bvh, cam, x0, y0, w, h, p, out, s);
+}
+rb_status render_bdpt(rb_bvh_t bvh, const rb_camera* cam, uint32_t x0, uint32_t y0, uint32_t w, uint32_t h,
+                      const rb_render_params* p, float* out, hipStream_t s) {
+    return render_kind(RB_REQ_BDPT, bvh, cam, x0, y0, w, h, p, out, s);
 }
 
 uint64_t render_scratch_bytes(uint32_t kind, uint32_t w, uint32_t h, const rb_render_params* p) {

@@ -12,6 +12,18 @@
 // behind them (bounded bypass); a request larger than the whole budget is
 // rejected at submit.  Results do not depend on what else runs: every
 // random number is keyed by (request seed, pixel, sample, bounce).
+//
+// Two ways to run the admitted requests:
+//  * streams (default): each request's whole pass is enqueued on its slot's
+//    stream at admission, and the streams' kernels overlap on the GPU;
+//  * lockstep: every request is a render job (make_render_job) and all of
+//    them advance together on one stream: per round, each job enqueues its
+//    integrator kernels up to its next trace, then one batched launch
+//    (launch_trace_sets) traces the ray sets of every in-flight request, so
+//    a round pays one persistent launch's ramp-up and tail instead of one
+//    per request.  The scheduler keeps at most kRoundsAhead rounds queued on
+//    the device, so a request admitted meanwhile joins within a round or
+//    two.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -57,7 +69,12 @@ struct Slot {
     char* arena = nullptr;  // device memory for the output and scratch of this slot's requests
     size_t arena_cap = 0;
     Req* req = nullptr;
+    bool tail = false;                // the request's result copies and event are enqueued
+    std::unique_ptr<RenderJob> job;   // lockstep: the request's remaining steps
 };
+
+// Lockstep rounds queued on the device at most.
+constexpr uint32_t kRoundsAhead = 2;
 
 // Arena headroom over a request's estimate: 256-byte alignment of each
 // scratch allocation.
@@ -120,9 +137,19 @@ struct rb_server_s {
     std::vector<double> lat_ms;
     bool have_first = false;
     rb::Clock::time_point first_submit, last_done;
+    // lockstep mode
+    hipStream_t lstream = nullptr;
+    hipEvent_t round_ev[rb::kRoundsAhead] = {};
+    uint32_t round_head = 0, rounds_out = 0;
+    uint64_t rounds = 0;
+    std::vector<rb::BatchSet> round_sets;
 
     void loop();
+    rb_status prepare(rb::Slot& sl, rb::Req* q, rb::Arena* arena);
     void launch(rb::Slot& sl, rb::Req* q);
+    void launch_lockstep(rb::Slot& sl, rb::Req* q);
+    rb_status tail(rb::Slot& sl);
+    bool round();
     void finish(rb::Slot& sl, rb_status result);
     rb_status ensure_arena(rb::Slot& sl, size_t need);
     void drop_arena(rb::Slot& sl);
@@ -153,6 +180,8 @@ void rb_server_s::finish(Slot& sl, rb_status result) {
         failed++;
     }
     sl.req = nullptr;
+    sl.tail = false;
+    sl.job.reset();
     cv_done.notify_all();
 }
 
@@ -192,10 +221,12 @@ rb_status rb_server_s::ensure_arena(Slot& sl, size_t need) {
     return RB_OK;
 }
 
-void rb_server_s::launch(Slot& sl, Req* q) {
+// Counter reset, output and scratch arena, pinned staging of a request
+// admitted to slot sl (arena: where its scratch comes from).
+rb_status rb_server_s::prepare(Slot& sl, Req* q, Arena* arena) {
     sl.req = q;
+    sl.tail = false;
     rb_status st = RB_OK;
-    Arena arena;
     hipError_t e = hipMemsetAsync(sl.d_counter, 0, sizeof(unsigned long long), sl.stream);
     if (e != hipSuccess) st = hip_fail(e, "server counter reset");
     if (st == RB_OK && q->r.kind != RB_REQ_TRACE) {
@@ -211,8 +242,8 @@ void rb_server_s::launch(Slot& sl, Req* q) {
         if (inject) {
             st = fail(RB_ERR_OOM, "request %llu: device allocation failed (injected)", (unsigned long long)q->id);
         } else if ((st = ensure_arena(sl, q->bytes + kArenaSlack)) == RB_OK) {
-            arena = Arena{sl.arena, sl.arena_cap, 0};
-            q->d_out = (float*)arena.take(q->out_bytes);  // first in the arena: always fits
+            *arena = Arena{sl.arena, sl.arena_cap, 0};
+            q->d_out = (float*)arena->take(q->out_bytes);  // first in the arena: always fits
         }
         if (st == RB_OK && q->r.out_host && sl.pinned_cap < q->out_bytes) {
             // grow geometrically from 8 MB; the old buffer is kept until the
@@ -229,6 +260,28 @@ void rb_server_s::launch(Slot& sl, Req* q) {
             }
         }
     }
+    return st;
+}
+
+// The request's result and ray count to the host, then its event.
+rb_status rb_server_s::tail(Slot& sl) {
+    Req* q = sl.req;
+    hipError_t e;
+    if (q->r.out_host && q->d_out &&
+        (e = hipMemcpyAsync(sl.pinned, q->d_out, q->out_bytes, hipMemcpyDeviceToHost, sl.stream)) != hipSuccess)
+        return hip_fail(e, "server result copy");
+    if ((e = hipMemcpyAsync(sl.h_counter, sl.d_counter, sizeof(unsigned long long), hipMemcpyDeviceToHost, sl.stream)) !=
+        hipSuccess)
+        return hip_fail(e, "server ray count copy");
+    if ((e = hipEventRecord(sl.ev, sl.stream)) != hipSuccess) return hip_fail(e, "server event");
+    sl.tail = true;
+    return RB_OK;
+}
+
+void rb_server_s::launch(Slot& sl, Req* q) {
+    Arena arena;
+    rb_status st = prepare(sl, q, &arena);
+    hipError_t e;
     if (st == RB_OK) {
         rb_render_params p = q->r.params;
         p.ray_counter = sl.d_counter;
@@ -253,19 +306,102 @@ void rb_server_s::launch(Slot& sl, Req* q) {
             arena_misses += arena.misses;
         }
     }
-    if (st == RB_OK && q->r.out_host && q->d_out) {
-        if ((e = hipMemcpyAsync(sl.pinned, q->d_out, q->out_bytes, hipMemcpyDeviceToHost, sl.stream)) != hipSuccess)
-            st = hip_fail(e, "server result copy");
-    }
-    if (st == RB_OK &&
-        (e = hipMemcpyAsync(sl.h_counter, sl.d_counter, sizeof(unsigned long long), hipMemcpyDeviceToHost, sl.stream)) !=
-            hipSuccess)
-        st = hip_fail(e, "server ray count copy");
-    if (st == RB_OK && (e = hipEventRecord(sl.ev, sl.stream)) != hipSuccess) st = hip_fail(e, "server event");
+    if (st == RB_OK) st = tail(sl);
     if (st != RB_OK) {
         (void)hipStreamSynchronize(sl.stream);
         finish(sl, st);
     }
+}
+
+// Lockstep admission: the request becomes a render job whose steps the
+// rounds run (a trace request is traced in the next round).
+void rb_server_s::launch_lockstep(Slot& sl, Req* q) {
+    Arena arena;
+    rb_status st = prepare(sl, q, &arena);
+    if (st == RB_OK && q->r.kind != RB_REQ_TRACE) {
+        rb_render_params p = q->r.params;
+        p.ray_counter = sl.d_counter;
+        p.path_counter = p.light_path_counter = nullptr;
+        t_arena = &arena;
+        st = make_render_job(q->r.kind, q->bvh, &q->r.camera, q->r.x0, q->r.y0, q->r.w, q->r.h, &p, q->d_out,
+                             sl.stream, &sl.job);
+        t_arena = nullptr;
+        if (arena.misses) {
+            std::lock_guard<std::mutex> g(mu);
+            arena_misses += arena.misses;
+        }
+    }
+    if (st != RB_OK) {
+        sl.job.reset();
+        (void)hipStreamSynchronize(sl.stream);
+        finish(sl, st);
+    }
+}
+
+// One lockstep round: every running job's next step, then one batched trace
+// of all their ray sets (and of the trace requests).  Returns whether any
+// work was enqueued.
+bool rb_server_s::round() {
+    while (rounds_out && hipEventQuery(round_ev[(round_head + kRoundsAhead - rounds_out) % kRoundsAhead]) == hipSuccess)
+        rounds_out--;
+    if (rounds_out >= kRoundsAhead) return false;
+    round_sets.clear();
+    std::vector<Slot*> traced;
+    bool work = false;
+    for (Slot& sl : slots) {
+        if (!sl.req || sl.tail) continue;
+        Req* q = sl.req;
+        work = true;
+        if (q->r.kind == RB_REQ_TRACE) {
+            if (q->r.n)
+                round_sets.push_back(BatchSet{q->bvh, reinterpret_cast<const float4*>(q->r.rays),
+                                              reinterpret_cast<int2*>(q->r.hits), q->r.n, nullptr,
+                                              (q->r.flags & RB_ANY) != 0, nullptr});
+            traced.push_back(&sl);
+            continue;
+        }
+        TraceSet ts[2];
+        uint32_t ns = 0;
+        bool done = false;
+        rb_status st = sl.job->step(lstream, ts, &ns, &done);
+        if (st == RB_OK && done) {
+            sl.job.reset();  // its scratch lives in the slot's arena
+            st = tail(sl);
+        } else if (st == RB_OK) {
+            for (uint32_t k = 0; k < ns; k++)
+                round_sets.push_back(BatchSet{q->bvh, ts[k].rays, ts[k].hits, ts[k].n, ts[k].n_dev,
+                                              (ts[k].flags & RB_ANY) != 0,
+                                              (ts[k].flags & RB_MASKED) ? ts[k].mask : nullptr});
+        }
+        if (st != RB_OK) {
+            sl.job.reset();
+            (void)hipStreamSynchronize(lstream);
+            finish(sl, st);
+        }
+    }
+    if (!round_sets.empty()) {
+        const rb_status st = launch_trace_sets(round_sets.data(), (uint32_t)round_sets.size(), lstream, dev().cfg);
+        if (st != RB_OK) {  // every job of the round stops with the error
+            (void)hipStreamSynchronize(lstream);
+            for (Slot& sl : slots)
+                if (sl.req && !sl.tail) finish(sl, st);
+            return true;
+        }
+    }
+    for (Slot* sl : traced) {
+        hipError_t e = hipMemcpyAsync(sl->d_counter, &sl->req->r.n, 4, hipMemcpyHostToDevice, lstream);
+        rb_status st = e == hipSuccess ? tail(*sl) : hip_fail(e, "server trace count");
+        if (st != RB_OK) {
+            (void)hipStreamSynchronize(lstream);
+            finish(*sl, st);
+        }
+    }
+    if (work && hipEventRecord(round_ev[round_head], lstream) == hipSuccess) {
+        round_head = (round_head + 1) % kRoundsAhead;
+        rounds_out++;
+        rounds++;
+    }
+    return work;
 }
 
 void rb_server_s::loop() {
@@ -276,6 +412,10 @@ void rb_server_s::loop() {
         bool busy = false;
         for (Slot& sl : slots) {
             if (!sl.req) continue;
+            if (!sl.tail) {  // lockstep: steps to go
+                busy = true;
+                continue;
+            }
             hipError_t e = hipEventQuery(sl.ev);
             if (e == hipSuccess) finish(sl, RB_OK);
             else if (e != hipErrorNotReady) {
@@ -324,8 +464,13 @@ void rb_server_s::loop() {
                 continue;
             }
         }
-        for (auto& p : to_launch) launch(*p.first, p.second);
-        if (to_launch.empty()) std::this_thread::sleep_for(std::chrono::microseconds(20));
+        const bool lockstep = cfg.mode == RB_SERVE_LOCKSTEP;
+        for (auto& p : to_launch) {
+            if (lockstep) launch_lockstep(*p.first, p.second);
+            else launch(*p.first, p.second);
+        }
+        const bool stepped = lockstep && round();
+        if (to_launch.empty() && !stepped) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 
@@ -347,14 +492,23 @@ rb_status rb_server_create(const rb_server_config* cfg, rb_server_t* out) {
     if (S->cfg.max_inflight > 32) return fail(RB_ERR_INVALID, "max_inflight must be <= 32");
     if (S->cfg.max_queue == 0) S->cfg.max_queue = 65536;
     if (S->cfg.policy > RB_SCHED_FIFO) return fail(RB_ERR_INVALID, "unknown scheduling policy %u", S->cfg.policy);
+    if (S->cfg.mode > RB_SERVE_LOCKSTEP) return fail(RB_ERR_INVALID, "unknown serving mode %u", S->cfg.mode);
     if (S->cfg.hbm_budget == 0) {
         size_t f = 0, t = 0;
         if (hipMemGetInfo(&f, &t) != hipSuccess) return fail(RB_ERR_DEVICE, "hipMemGetInfo failed");
         S->cfg.hbm_budget = (uint64_t)(f * 0.6);
     }
     S->slots.resize(S->cfg.max_inflight);
+    if (S->cfg.mode == RB_SERVE_LOCKSTEP) {  // one stream for every request
+        if (hipStreamCreateWithFlags(&S->lstream, hipStreamNonBlocking) != hipSuccess)
+            return fail(RB_ERR_DEVICE, "server stream creation failed");
+        for (hipEvent_t& ev : S->round_ev)
+            if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess)
+                return fail(RB_ERR_DEVICE, "server event creation failed");
+    }
     for (Slot& sl : S->slots) {
-        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+        if (S->lstream) sl.stream = S->lstream;
+        if ((!S->lstream && hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess) ||
             hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
             hipMalloc((void**)&sl.d_counter, sizeof(unsigned long long)) != hipSuccess ||
             hipHostMalloc((void**)&sl.h_counter, sizeof(unsigned long long), hipHostMallocDefault) != hipSuccess)
@@ -385,15 +539,19 @@ rb_status rb_server_destroy(rb_server_t s) {
     s->cv_done.notify_all();
     if (s->th.joinable()) s->th.join();
     for (Slot& sl : s->slots) {
+        sl.job.reset();
         if (sl.arena) (void)hipFreeAsync(sl.arena, sl.stream);
         if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         if (sl.ev) (void)hipEventDestroy(sl.ev);
-        if (sl.stream) (void)hipStreamDestroy(sl.stream);
+        if (sl.stream && sl.stream != s->lstream) (void)hipStreamDestroy(sl.stream);
         if (sl.d_counter) (void)hipFree(sl.d_counter);
         if (sl.h_counter) (void)hipHostFree(sl.h_counter);
         if (sl.pinned) (void)hipHostFree(sl.pinned);
         for (float* r : sl.retired) (void)hipHostFree(r);
     }
+    for (hipEvent_t ev : s->round_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (s->lstream) (void)hipStreamDestroy(s->lstream);
     delete s;
     return RB_OK;
 }

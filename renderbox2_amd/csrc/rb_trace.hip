@@ -73,13 +73,15 @@ struct BatchDesc {
     const uint32_t* index;
     const float4* rays;
     int2* hits;
-    uint32_t start;  // first global ray index of the batch
-    uint32_t any;    // 1: terminate on the first hit
+    const uint8_t* mask;    // rays with mask 0 report a miss untraced (NULL: none)
+    const uint32_t* n_dev;  // ray count in device memory (NULL: n)
+    uint32_t n;             // ray count, or the bound of *n_dev
+    uint32_t start;         // first global ray index of the batch (set by the kernel)
+    uint32_t any;           // 1: terminate on the first hit
 };
 
 struct BatchTable {
     uint32_t nb;
-    uint32_t total;
     BatchDesc d[kMaxBatch];
 };
 
@@ -268,7 +270,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                     any = !second && a.any0 != 0;
                     rayidx = second ? mine - n0 : mine;
                 }
+                const uint8_t* mk = a.mask;
                 if (MULTI) {
+                    // the last batch starting at or before `mine` (empty
+                    // batches share their successor's start)
                     uint32_t lo = 0, hi = nb;
                     while (hi - lo > 1) {
                         const uint32_t mid = (lo + hi) >> 1;
@@ -283,6 +288,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                     rays = (GF4)B.rays;
                     any = B.any != 0;
                     rayidx = mine - B.start;
+                    mk = B.mask;
                 }
                 const V4f o = rays[2 * (size_t)rayidx];
                 const V4f d = rays[2 * (size_t)rayidx + 1];
@@ -303,7 +309,7 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                 sp = 0;
                 leafAddr = 0;
                 hitIndex = -1;
-                nodeAddr = (!MULTI && a.mask && a.mask[rayidx] == 0) ? kSentinel : 0;
+                nodeAddr = (mk && mk[rayidx] == 0) ? kSentinel : 0;
                 if (COUNT) { c_inner = 0; c_tri = 0; c_leaf = 0; }
             }
         }
@@ -458,9 +464,24 @@ __global__ __launch_bounds__(BLOCK) void k_trace_pair(TraceArgs a) {
 template <int S, int BLOCK, bool SPEC>
 __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a, BatchTable t) {
     __shared__ BatchDesc tab[kMaxBatch];
+    __shared__ uint32_t s_total;
     for (uint32_t i = threadIdx.x; i < t.nb; i += BLOCK) tab[i] = t.d[i];
     __syncthreads();
-    trace_body<false, false, S, BLOCK, true, SPEC>(a, tab, t.nb, t.total);
+    if (threadIdx.x == 0) {  // batch starts from the counts (device counts were final before the launch)
+        uint32_t sum = 0;
+        for (uint32_t i = 0; i < t.nb; i++) {
+            uint32_t n = tab[i].n;
+            if (tab[i].n_dev) {
+                const uint32_t c = __builtin_nontemporal_load(tab[i].n_dev);
+                n = c < n ? c : n;
+            }
+            tab[i].start = sum;
+            sum += n;
+        }
+        s_total = sum;
+    }
+    __syncthreads();
+    trace_body<false, false, S, BLOCK, true, SPEC>(a, tab, t.nb, s_total);
 }
 
 // ---------------------------------------------------------------- launch --
@@ -601,34 +622,37 @@ rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, co
     return RB_OK;
 }
 
-rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg) {
+rb_status launch_trace_sets(const BatchSet* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg) {
     BatchKernel k = pick_batch(cfg);
     if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
-    for (uint32_t i0 = 0; i0 < nb; i0 += kMaxBatch) {
+    for (uint32_t i0 = 0; i0 < nb;) {
         BatchTable t;
         std::memset(&t, 0, sizeof(t));
-        uint64_t total = 0;
-        for (uint32_t i = i0; i < nb && i < i0 + kMaxBatch; i++) {
+        uint64_t total = 0;  // bound of the launch's rays (sizes the grid)
+        uint32_t i = i0;
+        for (; i < nb && t.nb < kMaxBatch; i++) {
             if (b[i].n == 0) continue;
             if (!b[i].bvh || !b[i].rays || !b[i].hits) return fail(RB_ERR_INVALID, "batch %u: NULL bvh, rays or hits", i);
-            if (b[i].flags & RB_MASKED) return fail(RB_ERR_INVALID, "batch %u: masks are not supported in batches", i);
+            if (b[i].n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "batch %u: more than 2^31 - 1 rays", i);
+            if (total + b[i].n > 0x7FFFFFFFull) break;  // the next launch takes it
             BatchDesc& d = t.d[t.nb++];
             d.nodes = b[i].bvh->nodes;
             d.tris = b[i].bvh->tris;
             d.index = b[i].bvh->index;
-            d.rays = reinterpret_cast<const float4*>(b[i].rays);
-            d.hits = reinterpret_cast<int2*>(b[i].hits);
-            d.start = (uint32_t)total;
-            d.any = (b[i].flags & RB_ANY) ? 1u : 0u;
+            d.rays = b[i].rays;
+            d.hits = b[i].hits;
+            d.mask = b[i].mask;
+            d.n_dev = b[i].n_dev;
+            d.n = b[i].n;
+            d.any = b[i].any ? 1u : 0u;
             total += b[i].n;
-            if (total > 0x7FFFFFFFull) return fail(RB_ERR_INVALID, "more than 2^31 rays in one batched call");
         }
+        i0 = i;
         if (t.nb == 0) continue;
-        t.total = (uint32_t)total;
         TraceArgs a;
         std::memset(&a, 0, sizeof(a));
         a.errors = dev().errors;
-        a.n = t.total;
+        a.n = (uint32_t)total;
         a.chunk = cfg.chunk;
         a.threshold = cfg.fetch_threshold;
         a.leaf_gate = cfg.leaf_gate;
@@ -642,6 +666,27 @@ rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t st
         if (e != hipSuccess) return hip_fail(e, "batched trace launch");
     }
     return RB_OK;
+}
+
+rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg) {
+    uint64_t total = 0;
+    std::vector<BatchSet> sets;
+    try {
+        sets.reserve(nb);
+    } catch (...) {
+        return fail(RB_ERR_OOM, "batch table (host allocation)");
+    }
+    for (uint32_t i = 0; i < nb; i++) {
+        if (b[i].n == 0) continue;
+        if (!b[i].bvh || !b[i].rays || !b[i].hits) return fail(RB_ERR_INVALID, "batch %u: NULL bvh, rays or hits", i);
+        if (b[i].flags & RB_MASKED) return fail(RB_ERR_INVALID, "batch %u: masks are not supported in batches", i);
+        // the rays of one table of kMaxBatch batches go through one launch
+        total = (sets.size() % kMaxBatch == 0 ? 0 : total) + b[i].n;
+        if (total > 0x7FFFFFFFull) return fail(RB_ERR_INVALID, "more than 2^31 rays in one batched call");
+        sets.push_back(BatchSet{b[i].bvh, reinterpret_cast<const float4*>(b[i].rays), reinterpret_cast<int2*>(b[i].hits),
+                                b[i].n, nullptr, (b[i].flags & RB_ANY) != 0, nullptr});
+    }
+    return launch_trace_sets(sets.data(), (uint32_t)sets.size(), stream, cfg);
 }
 
 static rb_status check_cfg(const rb_trace_config& c) {

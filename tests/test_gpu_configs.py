@@ -13,7 +13,8 @@
 * Served mix: 4 resident scenes of 100k-2M triangles, 40% AO tiles / 40%
   path-traced tiles (one of 1024 x 512 at 16 spp) / 20% BDPT frames (512 x
   512, 4 spp, 2^20-vertex cache): every served image equals the same request
-  rendered alone, bit for bit.
+  rendered alone, bit for bit, with the requests on their own streams and
+  in lockstep (one batched trace of every request's rays per round).
 """
 import numpy as np
 import pytest
@@ -121,11 +122,18 @@ def test_batched_trace_over_the_eight_resident_scenes(gpu, orc):
 SERVED_SCENES = [100_000, 500_000, 1_000_000, 2_000_000]
 
 
-def test_served_mix_over_full_size_scenes_equals_standalone(gpu):
+@pytest.fixture(scope="module")
+def served_scenes(gpu):
     tracers = []
     for k, T in enumerate(SERVED_SCENES):
         s = gpu.scenes.pt_scaled(T, seed=k)
         tracers.append(gpu.CudaTracer(gpu.scenes.build(s), s))
+    return tracers
+
+
+@pytest.mark.parametrize("mode", ["streams", "lockstep"])
+def test_served_mix_over_full_size_scenes_equals_standalone(gpu, served_scenes, mode):
+    tracers = served_scenes
     pt_cam, bd_cam = gpu.scenes.pt_camera(1024, 1024), gpu.scenes.pt_camera(512, 512)
     rng = np.random.default_rng(5)
     reqs = []
@@ -149,7 +157,7 @@ def test_served_mix_over_full_size_scenes_equals_standalone(gpu):
     # one large path-traced tile: half the 1024^2 frame at 16 spp
     reqs.append(("pt", 3, pt_cam, (0, 256, 1024, 512), dict(spp=16, seed=999, max_depth=8, rr_start=3, rr_prob=0.5)))
     assert sum(k == "bdpt" for k, *_ in reqs) >= 4
-    srv = gpu.RenderServer(max_inflight=8)
+    srv = gpu.RenderServer(max_inflight=8, mode=mode)
     sids = [srv.add_scene(t) for t in tracers]
     rids = [srv.submit_render(k, sids[sc], cam, tile, **p) for k, sc, cam, tile, p in reqs]
     served = []

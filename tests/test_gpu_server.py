@@ -1,6 +1,8 @@
 """GPU tests of the request server: results equal the same request rendered
-alone, bit for bit, whatever else shares the GPU; admission within the
-memory budget; rejection, injected allocation failure and busy scenes."""
+alone, bit for bit, whatever else shares the GPU and in both serving modes
+(each request on its own stream, or all in lockstep with one batched trace
+per round); admission within the memory budget; rejection, injected
+allocation failure and busy scenes."""
 import numpy as np
 import pytest
 
@@ -40,8 +42,9 @@ def _alone(gpu, tracers, kind, cam, tile, params):
     return gpu.SamplerRenderer(tracers, cam, kind, **params).render(tile)
 
 
-def test_served_results_equal_standalone_renders(gpu, scenes):
-    srv = gpu.RenderServer(max_inflight=4)
+@pytest.mark.parametrize("mode", ["streams", "lockstep"])
+def test_served_results_equal_standalone_renders(gpu, scenes, mode):
+    srv = gpu.RenderServer(max_inflight=4, mode=mode)
     ids = (srv.add_scene(scenes[0]), srv.add_scene(scenes[1]))
     tr = {ids[0]: scenes[0], ids[1]: scenes[1]}
     mix = _mix(gpu, ids) * 2  # every request twice: the second copy runs beside different neighbours
@@ -60,7 +63,8 @@ def test_served_results_equal_standalone_renders(gpu, scenes):
     srv.close()
 
 
-def test_trace_requests_match_rb_trace(gpu, scenes):
+@pytest.mark.parametrize("mode", ["streams", "lockstep"])
+def test_trace_requests_match_rb_trace(gpu, scenes, mode):
     tr = scenes[0]
     n = 50000
     rng = np.random.default_rng(3)
@@ -71,7 +75,7 @@ def test_trace_requests_match_rb_trace(gpu, scenes):
     rays[:, 4:7] = d
     rays[:, 7] = 1e30
     ids, t = tr.trace(rays)
-    srv = gpu.RenderServer(max_inflight=2)
+    srv = gpu.RenderServer(max_inflight=2, mode=mode)
     sid = srv.add_scene(tr)
     dr = gpu.DeviceBuffer.from_array(rays)
     dh = [gpu.DeviceBuffer(n * 8) for _ in range(3)]
@@ -87,10 +91,11 @@ def test_trace_requests_match_rb_trace(gpu, scenes):
     srv.close()
 
 
-def test_small_budget_defers_but_completes(gpu, scenes):
+@pytest.mark.parametrize("mode", ["streams", "lockstep"])
+def test_small_budget_defers_but_completes(gpu, scenes, mode):
     big = gpu.RenderServer.render_request("pt", 0, gpu.scenes.pt_camera(64, 48), None, spp=2, seed=1, max_depth=6)
     need = gpu.RenderServer.request_bytes(big)
-    srv = gpu.RenderServer(max_inflight=8, hbm_budget=int(need * 1.5))  # one at a time
+    srv = gpu.RenderServer(max_inflight=8, hbm_budget=int(need * 1.5), mode=mode)  # one at a time
     sid = srv.add_scene(scenes[0])
     cam = gpu.scenes.pt_camera(64, 48)
     rids = [srv.submit_render("pt", sid, cam, None, spp=2, seed=s, max_depth=6) for s in range(6)]
@@ -116,8 +121,9 @@ def test_request_larger_than_budget_is_rejected(gpu, scenes):
     srv.close()
 
 
-def test_injected_allocation_failure_fails_one_request_only(gpu, scenes):
-    srv = gpu.RenderServer(max_inflight=1)
+@pytest.mark.parametrize("mode", ["streams", "lockstep"])
+def test_injected_allocation_failure_fails_one_request_only(gpu, scenes, mode):
+    srv = gpu.RenderServer(max_inflight=1, mode=mode)
     sid = srv.add_scene(scenes[0])
     cam = gpu.scenes.pt_camera(32, 32)
     srv.inject_alloc_failures(1)
