@@ -24,6 +24,7 @@
 // stored vertices uniformly, and scale by n_j / connections (unbiased).
 #pragma once
 #include "rb_shade.h"
+#include "../../include/rb_trace_device.h"
 
 namespace rb {
 
@@ -168,78 +169,88 @@ __global__ __launch_bounds__(kQueueBlock) void k_light_emit(uint32_t npaths, Sce
     }
 }
 
-__global__ __launch_bounds__(kQueueBlock) void k_light_hit(uint32_t qi, uint32_t d, SceneDev S, RngArg rng_a,
-                                                            LightPath P, LvcBufs C) {
+// One light vertex (kernel_sk_light_trace's body, integrator_bdpt_lvc_sk.cu:
+// 47-168): light path j's ray (o, dd) hit h at depth d.  A non-emitter hit
+// is stored at slot (j, d) with the throughput a arriving there; then, below
+// the cache depth, a BSDF sample with roulette (continue probability
+// min(1, luminance)) gives the next ray, throughput and specular mask in
+// place and returns true.  Emitters end the path.
+__device__ __forceinline__ bool light_vertex(const SceneDev& S, const Rng& rng, const LvcBufs& C, uint32_t j, uint32_t d,
+                                             int2 h, float4* o, float4* dd, float4* a, uint32_t* mask) {
+    const float t = __int_as_float(h.y);
+    const V3 dv = v3(dd->x, dd->y, dd->z);
+    const V3 p = vadd(v3(o->x, o->y, o->z), vmul(dv, t));
+    V3 v0, v1, v2;
+    uint32_t mid, tf;
+    tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
+    const rb_material m = S.mats[mid];
+    if (m.kind == RB_MAT_EMITTER) return false;
+    const V3 ng = tri_normal(v0, v1, v2);
+    V3 ns;
+    const bool sn = shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
+    const V3 wo = vneg(dv);
+    // the stored vertex is specular when every layer is; the path continues
+    // with the sampled lobe's kind
+    const uint32_t dbit = mat_delta(S, m) ? 1u << d : 0u;
+    const size_t slot = (size_t)j * C.depth + d;
+    float4* rec = C.v + C.rows * slot;
+    rec[0] = make_float4(p.x, p.y, p.z, __uint_as_float(sn ? 1u : 0u));
+    rec[1] = make_float4(ng.x, ng.y, ng.z, __uint_as_float(mid));
+    rec[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(*mask | dbit));
+    rec[3] = *a;
+    if (C.rows > 4) rec[4] = make_float4(ns.x, ns.y, ns.z, 0.f);
+    C.count[j] = d + 1;
+    if (d + 1 >= C.depth) return false;
+    const U4 r = rng.draw(j, d + 1, 0, 0);
+    const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
+    V3 wi, f;
+    float pdf;
+    bool spec, weight;  // weight: f already holds f |cos| / pdf
+    mat_sample(S, m, wo, ng, ns, nf, u01(r.w), u01(r.x), u01(r.y), false, &wi, &f, &pdf, &spec, &weight);
+    if (is_black(f) || !(pdf > 0.0f)) return false;
+    const float s = weight ? 1.0f : fabsf(vdot(ns, wi));
+    V3 tmp = v3(f.x * s / pdf, f.y * s / pdf, f.z * s / pdf);
+    // importance transport: shading-normal adjoint factor
+    if (sn) tmp = vmul(tmp, adjoint_factor(wo, wi, ng, ns));
+    const float lum = luminance(tmp);
+    const float cp = lum < 1.0f ? lum : 1.0f;
+    if (u01(r.z) > cp) return false;
+    const float4 na = make_float4(a->x * (tmp.x / cp), a->y * (tmp.y / cp), a->z * (tmp.z / cp), 0.f);
+    if (na.x == 0.f && na.y == 0.f && na.z == 0.f) return false;
+    *a = na;
+    if (spec) *mask |= 1u << d;
+    *o = make_float4(p.x, p.y, p.z, 1.0e-3f * t);
+    *dd = make_float4(wi.x, wi.y, wi.z, 3.402823466e+38f);
+    return true;
+}
+
+// The light paths after their first trace, one thread per path: depth 0's
+// hit comes from the launch that traced the light rays with the camera rays
+// (queue 0, hit q of entry q); every later depth's ray is traced by the
+// thread itself with rb_trace_one, whose (id, t) equal the persistent
+// kernel's bit for bit.  One launch for all depths replaces a trace launch
+// per depth, whose ramp-up and tail dominated the thinning later depths.
+// Optional counters: rays traced (ray_counter) and rays per depth
+// (depth_counter[d], d >= 1).
+__global__ __launch_bounds__(256) void k_light_walk(SceneDev S, RngArg rng_a, LightPath P, LvcBufs C, rb_bvh_dev bv,
+                                                    unsigned long long* ray_counter,
+                                                    unsigned long long* depth_counter) {
     const Rng rng = rng_a.get();
-    const uint32_t count = P.cnt[qi];
-    const uint32_t end = uniform_end(count);
-    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < end; q += gridDim.x * blockDim.x) {
-        const bool valid = q < count;
-        bool next = false;
-        uint32_t j = 0;
-        if (valid) {
-            j = P.q[qi][q];
-            const int2 h = P.hit[q];
-            if (h.x >= 0) {
-                const float4 o = P.ray[2 * (size_t)j], dd = P.ray[2 * (size_t)j + 1];
-                const float t = __int_as_float(h.y);
-                const V3 dv = v3(dd.x, dd.y, dd.z);
-                const V3 p = vadd(v3(o.x, o.y, o.z), vmul(dv, t));
-                V3 v0, v1, v2;
-                uint32_t mid, tf;
-                tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
-                const rb_material m = S.mats[mid];
-                if (m.kind != RB_MAT_EMITTER) {
-                    const V3 ng = tri_normal(v0, v1, v2);
-                    V3 ns;
-                    const bool sn = shade_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
-                    const V3 wo = vneg(dv);
-                    const float4 a = P.alpha[j];
-                    uint32_t mask = P.mask[j];
-                    // the stored vertex is specular when every layer is; the
-                    // path continues with the sampled lobe's kind
-                    const uint32_t dbit = mat_delta(S, m) ? 1u << d : 0u;
-                    const size_t slot = (size_t)j * C.depth + d;
-                    float4* rec = C.v + C.rows * slot;
-                    rec[0] = make_float4(p.x, p.y, p.z, __uint_as_float(sn ? 1u : 0u));
-                    rec[1] = make_float4(ng.x, ng.y, ng.z, __uint_as_float(mid));
-                    rec[2] = make_float4(wo.x, wo.y, wo.z, __uint_as_float(mask | dbit));
-                    rec[3] = a;
-                    if (C.rows > 4) rec[4] = make_float4(ns.x, ns.y, ns.z, 0.f);
-                    C.count[j] = d + 1;
-                    if (d + 1 < C.depth) {
-                        const U4 r = rng.draw(j, d + 1, 0, 0);
-                        const V3 nf = vdot(ng, wo) >= 0.0f ? ns : vneg(ns);
-                        V3 wi, f;
-                        float pdf;
-                        bool spec, weight;  // weight: f already holds f |cos| / pdf
-                        mat_sample(S, m, wo, ng, ns, nf, u01(r.w), u01(r.x), u01(r.y), false, &wi, &f, &pdf, &spec,
-                                   &weight);
-                        if (spec) mask |= 1u << d;
-                        if (!is_black(f) && pdf > 0.0f) {
-                            const float s = weight ? 1.0f : fabsf(vdot(ns, wi));
-                            V3 tmp = v3(f.x * s / pdf, f.y * s / pdf, f.z * s / pdf);
-                            // importance transport: shading-normal adjoint factor
-                            if (sn) tmp = vmul(tmp, adjoint_factor(wo, wi, ng, ns));
-                            const float lum = luminance(tmp);
-                            const float cp = lum < 1.0f ? lum : 1.0f;
-                            if (!(u01(r.z) > cp)) {
-                                const float4 na = make_float4(a.x * (tmp.x / cp), a.y * (tmp.y / cp), a.z * (tmp.z / cp), 0.f);
-                                if (!(na.x == 0.f && na.y == 0.f && na.z == 0.f)) {
-                                    P.alpha[j] = na;
-                                    P.mask[j] = mask;
-                                    P.ray[2 * (size_t)j] = make_float4(p.x, p.y, p.z, 1.0e-3f * t);
-                                    P.ray[2 * (size_t)j + 1] = make_float4(wi.x, wi.y, wi.z, 3.402823466e+38f);
-                                    next = true;
-                                }
-                            }
-                        }
-                    }
-                }
-            }
+    const uint32_t count = P.cnt[0];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
+        const uint32_t j = P.q[0][q];
+        int2 h = P.hit[q];
+        float4 o = P.ray[2 * (size_t)j], dd = P.ray[2 * (size_t)j + 1];
+        float4 a = P.alpha[j];
+        uint32_t mask = P.mask[j];
+        for (uint32_t d = 0; h.x >= 0 && light_vertex(S, rng, C, j, d, h, &o, &dd, &a, &mask);) {
+            d++;
+            const rb_ray r = {o.x, o.y, o.z, o.w, dd.x, dd.y, dd.z, dd.w};
+            const rb_hit hh = rb_trace_one(bv, r, RB_CLOSEST);
+            h = make_int2(hh.id, __float_as_int(hh.t));
+            if (ray_counter) atomicAdd(ray_counter, 1ull);
+            if (depth_counter) atomicAdd(depth_counter + d, 1ull);
         }
-        const uint32_t o = block_reserve(&P.cnt[qi ^ 1], next ? 1u : 0u);
-        if (next) P.q[qi ^ 1][o] = j;
     }
 }
 

@@ -924,9 +924,9 @@ struct CameraJob : RenderJob {
     float4* lrays = nullptr;
     uint32_t* ncam = nullptr;
     uint32_t NL = 0, D = 0;
-    // progress: 0 start, 1 light depths, 2 camera bounces, 3 last contributions
+    // progress: 0 start, 1 light pass, 2 camera bounces, 3 last contributions
     int phase = 0;
-    uint32_t d = 0, b = 0;
+    uint32_t b = 0;
     explicit CameraJob(hipStream_t s) : M(s) {}
 
     TileArgs tile() const { return TileArgs{cam.width, x0, y0, w, p.spp}; }
@@ -981,20 +981,20 @@ struct CameraJob : RenderJob {
             phase = 1;
             return RB_OK;
         }
-        if (phase == 1) {  // light depth d has been traced
-            const uint32_t qi = d & 1;
-            if (p.ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p.ray_counter);
+        if (phase == 1) {  // light depth 0 has been traced: the rest of the light pass, then the camera pass
+            if (p.ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + 0, p.ray_counter);
             if (p.light_path_counter)
-                hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + qi, p.light_path_counter + d);
-            hipLaunchKernelGGL(k_light_hit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, qi, d, S, rng_arg(p.seed, RNG_LIGHT),
-                               LP, B.C);
-            if (++d < D) {
-                light_rays(s, d);
-                sets[0] = TraceSet{lrays, LP.hit, NL, LP.cnt + (d & 1), RB_CLOSEST, nullptr, dev().cfg};
-                *nsets = 1;
-                return RB_OK;
-            }
-            phase = 2;  // the camera pass starts in this step
+                hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + 0, p.light_path_counter);
+            rb_bvh_dev bv;
+            bv.nodes = bvh->nodes;
+            bv.tris = bvh->tris;
+            bv.index = bvh->index;
+            bv.n_nodes = bvh->n_nodes;
+            bv.n_refs = bvh->n_refs;
+            bv.errors = dev().errors;
+            hipLaunchKernelGGL(k_light_walk, dim3(grid_of(NL)), dim3(256), 0, s, S, rng_arg(p.seed, RNG_LIGHT), LP, B.C,
+                               bv, p.ray_counter, p.light_path_counter);
+            phase = 2;
         }
         if (phase == 2) {
             if (b > 0 && p.ray_counter)  // the previous bounce's rays
@@ -1130,8 +1130,7 @@ rb_status make_render_job(uint32_t kind, rb_bvh_t bvh, const rb_camera* cam, uin
             LP.ray = M.get<float4>(2 * (size_t)NL);
             LP.alpha = M.get<float4>(NL);
             LP.mask = M.get<uint32_t>(NL);
-            LP.q[0] = M.get<uint32_t>(NL);
-            LP.q[1] = M.get<uint32_t>(NL);
+            LP.q[0] = M.get<uint32_t>(NL);  // the walk kernel follows each path itself: one queue
             LP.cnt = M.get<uint32_t>(4);
             LP.hit = M.get<int2>(NL);
             J->lrays = M.get<float4>(2 * (size_t)NL);
