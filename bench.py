@@ -429,7 +429,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget (0 = skip)")
     ap.add_argument("--extras", type=int, default=1, help="also time coherent / any-hit / mixed sets")
-    ap.add_argument("--autotune", type=int, default=1, help="pick the traversal config on this batch first")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="1: pick the traversal config on this batch first (default 0: the library's default "
+                         "config, which the autotuner also settles on; its timing noise picked configs up to "
+                         "1%% slower)")
     ap.add_argument("--config", default="",
                     help="block,lds_stack,chunk,fetch_threshold,waves_per_cu[,speculative[,leaf_gate[,lds_nodes]]]")
     ap.add_argument("--speculative", type=int, default=0,

@@ -47,9 +47,9 @@ struct LightPath {
     float4* ray;    // 2 per path
     float4* alpha;
     uint32_t* mask;
-    uint32_t* q[2];
-    uint32_t* cnt;  // [0],[1] queue lengths
-    int2* hit;
+    uint32_t* q;    // the emitted paths, in queue order
+    uint32_t* cnt;  // [0]: their number
+    int2* hit;      // depth 0's hit of queue entry q
 };
 
 // Queue kernels (shade, connect, path and light-path init / extension) run
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(kQueueBlock) void k_light_emit(uint32_t npaths, Sce
             P.mask[j] = 0;
         }
         const uint32_t q = block_reserve(&P.cnt[0], live ? 1u : 0u);
-        if (live) P.q[0][q] = j;
+        if (live) P.q[q] = j;
     }
 }
 
@@ -238,7 +238,7 @@ __global__ __launch_bounds__(256) void k_light_walk(SceneDev S, RngArg rng_a, Li
     const Rng rng = rng_a.get();
     const uint32_t count = P.cnt[0];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
-        const uint32_t j = P.q[0][q];
+        const uint32_t j = P.q[q];
         int2 h = P.hit[q];
         float4 o = P.ray[2 * (size_t)j], dd = P.ray[2 * (size_t)j + 1];
         float4 a = P.alpha[j];
@@ -254,11 +254,11 @@ __global__ __launch_bounds__(256) void k_light_walk(SceneDev S, RngArg rng_a, Li
     }
 }
 
-// Gather the rays of the queued light paths into a dense buffer for tracing.
-__global__ void k_light_gather(uint32_t qi, LightPath P, float4* rays) {
-    const uint32_t count = P.cnt[qi];
+// Gather the rays of the emitted light paths into a dense buffer for tracing.
+__global__ void k_light_gather(LightPath P, float4* rays) {
+    const uint32_t count = P.cnt[0];
     for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < count; q += gridDim.x * blockDim.x) {
-        const uint32_t j = P.q[qi][q];
+        const uint32_t j = P.q[q];
         rays[2 * (size_t)q] = P.ray[2 * (size_t)j];
         rays[2 * (size_t)q + 1] = P.ray[2 * (size_t)j + 1];
     }

@@ -949,13 +949,6 @@ struct CameraJob : RenderJob {
         }
     }
 
-    // the light rays of depth dd (queue dd & 1) for the next trace
-    void light_rays(hipStream_t s, uint32_t dd) {
-        const uint32_t qi = dd & 1;
-        hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, qi ^ 1, 2u, 3u);
-        hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, qi, LP, lrays);
-    }
-
     rb_status step(hipStream_t s, TraceSet* sets, uint32_t* nsets, bool* done) override {
         rb_status st;
         *nsets = 0;
@@ -974,7 +967,7 @@ struct CameraJob : RenderJob {
             hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, LP.cnt, 0u, 1u, 2u);
             hipLaunchKernelGGL(k_light_emit, dim3(grid_q(NL)), dim3(kQueueBlock), 0, s, NL, S,
                                rng_arg(p.seed, RNG_LIGHT), LP, B.C);
-            light_rays(s, 0);
+            hipLaunchKernelGGL(k_light_gather, dim3(grid_of(NL)), dim3(256), 0, s, LP, lrays);
             sets[0] = TraceSet{lrays, LP.hit, NL, LP.cnt + 0, RB_CLOSEST, nullptr, dev().cfg};
             sets[1] = TraceSet{prim, phits, n, ncam, RB_CLOSEST, nullptr, dev().cfg};
             *nsets = 2;
@@ -1130,7 +1123,7 @@ rb_status make_render_job(uint32_t kind, rb_bvh_t bvh, const rb_camera* cam, uin
             LP.ray = M.get<float4>(2 * (size_t)NL);
             LP.alpha = M.get<float4>(NL);
             LP.mask = M.get<uint32_t>(NL);
-            LP.q[0] = M.get<uint32_t>(NL);  // the walk kernel follows each path itself: one queue
+            LP.q = M.get<uint32_t>(NL);
             LP.cnt = M.get<uint32_t>(4);
             LP.hit = M.get<int2>(NL);
             J->lrays = M.get<float4>(2 * (size_t)NL);
