@@ -434,7 +434,7 @@ def main():
                          "config, which the autotuner also settles on; its timing noise picked configs up to "
                          "1%% slower)")
     ap.add_argument("--config", default="",
-                    help="block,lds_stack,chunk,fetch_threshold,waves_per_cu[,speculative[,leaf_gate[,lds_nodes]]]")
+                    help="block,lds_stack,chunk,fetch_threshold,waves_per_cu[,speculative[,leaf_gate[,lds_nodes[,wide]]]]")
     ap.add_argument("--speculative", type=int, default=0,
                     help="1: the reference's speculative leaf order (wave-dependent results in rare cases)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -502,7 +502,7 @@ def main():
     rb.set_trace_config(speculative=args.speculative)
     if args.config:
         keys = ["block", "lds_stack", "chunk", "fetch_threshold", "waves_per_cu", "speculative", "leaf_gate",
-                "lds_nodes"]
+                "lds_nodes", "wide"]
         rb.set_trace_config(**dict(zip(keys, (int(x) for x in args.config.split(",")))))
     elif args.autotune:
         _, tuned_ms = rb.autotune(tracer, d_diff, n, d_hits, reps=5, stream=stream)
@@ -536,9 +536,22 @@ def main():
         log(f"mixed set, one GPU's share: {share}")
         del mixed
 
-    # device-side traversal counters (this kernel's own visit order), all rays
+    # device-side traversal counters of the binary walk (the reference's
+    # visit order, priced with the reference's formula), all rays
     ids, _, gcnt = tracer.trace_counts(d_diff.download(np.float32, n * 8).reshape(n, 8))
     gpu_bpr = float((40 + 64 * gcnt[:, 0] + 48 * gcnt[:, 1] + 16 * gcnt[:, 2] + 4 * (ids >= 0)).mean())
+    # bytes the timed kernel itself reads and writes per ray: the ray (32),
+    # the hit (8), each visited record (64 binary; 112 = seven 16-byte rows
+    # wide), each tested triangle (48) and the hit's index entry (4)
+    wide = bool(cfg["wide"]) and not cfg["speculative"]
+    if wide:
+        wids, _, wcnt = tracer.trace_counts(d_diff.download(np.float32, n * 8).reshape(n, 8), wide=True)
+        kernel_bpr = float((40 + 112 * wcnt[:, 0] + 48 * wcnt[:, 1] + 4 * (wids >= 0)).mean())
+        wide_records = float(wcnt[:, 0].mean())
+        del wids, wcnt
+    else:
+        kernel_bpr = float((40 + 64 * gcnt[:, 0] + 48 * gcnt[:, 1] + 4 * (ids >= 0)).mean())
+        wide_records = None
 
     # CPU baseline (the oracle's restatement of the reference traversal) and
     # the algorithmic bytes per ray it counts on the reference visit order.
@@ -616,7 +629,10 @@ def main():
         log("served mix (config 5)", sv)
         extra["served"] = sv
 
+    # achieved: the reference traversal's algorithmic bytes over the timed
+    # kernel's time; kernel_achieved: the kernel's own reads and writes
     achieved = bytes_per_ray * n / (ms * 1e-3) / 1e9
+    kernel_achieved = kernel_bpr * n / (ms * 1e-3) / 1e9
     traffic = read_traffic(os.path.join(ROOT, "profiles", "trace_incoherent_1m_pmc.txt"))
     l2_hit = read_profile_value(os.path.join(ROOT, "profiles", "trace_incoherent_1m_l2.txt"), "l2_hit_rate")
     out = {
@@ -642,6 +658,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "bytes_per_ray": round(bytes_per_ray, 1), "gpu_visit_bytes_per_ray": round(gpu_bpr, 1),
+                     "kernel": "k_trace_wide" if wide else "k_trace",
+                     "kernel_bytes_per_ray": round(kernel_bpr, 1), "kernel_achieved": round(kernel_achieved, 1),
+                     "kernel_frac": round(kernel_achieved / HBM_PEAK_GBS, 4),
+                     "wide_records_per_ray": None if wide_records is None else round(wide_records, 2),
+                     "binary_records_per_ray": round(float(gcnt[:, 0].mean()), 2),
                      "l2_hit_rate": l2_hit},
         "cpu_baseline": cpu,
     }

@@ -303,7 +303,10 @@ rb_status rb_bvh_device_view(rb_bvh_t b, rb_bvh_dev* out);
 enum {
     RB_CLOSEST = 0,
     RB_ANY = 1,       /* terminate on the first hit (occlusion) */
-    RB_MASKED = 2     /* mask[i] == 0 skips ray i (writes a miss) */
+    RB_MASKED = 2,    /* mask[i] == 0 skips ray i (writes a miss) */
+    RB_WIDE_COUNTS = 4 /* rb_trace_counts only: count the wide (4-child) walk a
+                          closest-hit trace takes by default instead of the
+                          binary one (no counterpart in the reference) */
 };
 
 /* CudaTracer::trace(rb, isects, scene, stencil, any_hit)
@@ -379,6 +382,13 @@ typedef struct rb_trace_config {
                                 none).  The MI355X layout stores the first 512 records
                                 breadth-first, so these are the top levels of the tree.
                                 Any value gives the same results */
+    uint32_t wide;           /* 1 (default): closest-hit traces without a visit-count
+                                buffer walk 4-child records (each binary record's
+                                children, inner ones replaced by their own two), about
+                                half the dependent fetches per ray; lds_nodes / 2 such
+                                records go to LDS.  The same (id, t) as the binary walk,
+                                whose order any-hit traces and visit counts keep; 0: the
+                                binary records for every trace */
 } rb_trace_config;
 rb_status rb_trace_get_config(rb_trace_config* c);
 rb_status rb_trace_set_config(const rb_trace_config* c);
@@ -402,7 +412,11 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
 enum { RB_TRACE_ERR_STACK = 1 };
 rb_status rb_trace_errors(uint32_t* flags);
 /* Per-ray traversal counters of the device kernel (inner nodes, triangles,
- * leaves) for traffic accounting; counts is device uint32[3*n]. */
+ * leaves) for traffic accounting; counts is device uint32[3*n].  The
+ * counts follow the binary walk (the reference's visit order, whatever
+ * rb_trace_config.wide says); with RB_WIDE_COUNTS (closest hit, not
+ * speculative) they follow the wide walk, inner = 4-child records visited
+ * (RB_ERR_UNSUPPORTED when the BVH has no wide records). */
 rb_status rb_trace_counts(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
                           rb_hit* hits, uint32_t* counts, void* stream);
 

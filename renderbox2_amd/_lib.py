@@ -17,7 +17,7 @@ I32 = C.c_int32
 SZ = C.c_size_t
 
 RB_OK, RB_ERR_INVALID, RB_ERR_OOM, RB_ERR_DEVICE, RB_ERR_BUSY, RB_ERR_UNSUPPORTED, RB_ERR_NOT_READY = range(7)
-RB_CLOSEST, RB_ANY, RB_MASKED = 0, 1, 2
+RB_CLOSEST, RB_ANY, RB_MASKED, RB_WIDE_COUNTS = 0, 1, 2, 4
 RB_LAYOUT_REF_COMPACT2, RB_LAYOUT_MI355X = 1, 2
 (RB_MAT_LAMBERT, RB_MAT_MIRROR, RB_MAT_EMITTER, RB_MAT_GLASS, RB_MAT_ORENNAYAR, RB_MAT_MICROFACET,
  RB_MAT_LAYERED) = range(7)
@@ -58,7 +58,8 @@ class rb_bvh_stats(C.Structure):
 
 class rb_trace_config(C.Structure):
     _fields_ = [("block", U32), ("lds_stack", U32), ("chunk", U32), ("fetch_threshold", U32),
-                ("waves_per_cu", U32), ("speculative", U32), ("leaf_gate", U32), ("lds_nodes", U32)]
+                ("waves_per_cu", U32), ("speculative", U32), ("leaf_gate", U32), ("lds_nodes", U32),
+                ("wide", U32)]
 
 
 class rb_camera(C.Structure):

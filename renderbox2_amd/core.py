@@ -449,13 +449,16 @@ class CudaTracer:
         check(lib().rb_bvh_device_view(self._h, C.byref(v)))
         return v
 
-    def trace_counts(self, rays, any_hit=False):
+    def trace_counts(self, rays, any_hit=False, wide=False):
+        """(ids, t, counts[n, 3]) of the binary walk, or of the wide walk
+        (closest hit) with wide=True."""
         rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
         n = len(rays)
         dr = DeviceBuffer.from_array(rays)
         dh = DeviceBuffer(n * 8)
         dc = DeviceBuffer(n * 12)
-        check(lib().rb_trace_counts(self._h, dr.ptr, n, L.RB_ANY if any_hit else L.RB_CLOSEST, dh.ptr, dc.ptr, None))
+        flags = (L.RB_ANY if any_hit else L.RB_CLOSEST) | (L.RB_WIDE_COUNTS if wide else 0)
+        check(lib().rb_trace_counts(self._h, dr.ptr, n, flags, dh.ptr, dc.ptr, None))
         h = dh.download(HIT_DTYPE, n)
         c = dc.download(np.uint32, 3 * n).reshape(n, 3)
         return h["id"].copy(), h["t"].copy(), c

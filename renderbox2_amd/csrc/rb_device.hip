@@ -58,6 +58,7 @@ static rb_status init_locked(int device) {
     g_dev.cfg.speculative = 0;
     g_dev.cfg.leaf_gate = 16;
     g_dev.cfg.lds_nodes = 384;
+    g_dev.cfg.wide = 1;
     g_dev.device = device;
     return RB_OK;
 }
@@ -331,18 +332,20 @@ rb_status rb_bvh_upload(rb_host_bvh_t hb, rb_scene_t scene, rb_bvh_t* out) {
     if (!hb || !out) return fail(RB_ERR_INVALID, "NULL argument");
     RB_ENSURE();
     Mi355Layout L;
+    std::vector<uint32_t> wide;
     try {
         rb_status st = export_mi355x(*hb->b, L);
         if (st != RB_OK) return st;
-    } catch (...) {
-        return fail(RB_ERR_OOM, "layout export failed (host allocation)");
-    }
-    {
         // every stack entry a ray can need must exist: the kernels hold kMaxStack
         const uint32_t depth = layout_depth(L.nodes);
         if (depth == UINT32_MAX || depth + 1 >= (uint32_t)kMaxStack)
             return fail(RB_ERR_UNSUPPORTED, "BVH node layout is %s (the traversal stack holds %d entries)",
                         depth == UINT32_MAX ? "not a tree" : "too deep", kMaxStack);
+        // a wide record pushes up to three children and the wide tree is half
+        // as deep: wide walks only where that fits the same stack
+        if (3 * ((depth + 2) / 2) + 1 < (uint32_t)kMaxStack && (st = export_wide(L.nodes, wide)) != RB_OK) return st;
+    } catch (...) {
+        return fail(RB_ERR_OOM, "layout export failed (host allocation)");
     }
     if (scene && scene->s.num_tris() != (uint32_t)(hb->b->idx.size() / 3))
         return fail(RB_ERR_INVALID, "scene has %u triangles, the BVH was built over %zu", scene->s.num_tris(),
@@ -363,6 +366,8 @@ rb_status rb_bvh_upload(rb_host_bvh_t hb, rb_scene_t scene, rb_bvh_t* out) {
     B->n_nodes = (uint32_t)(L.nodes.size() / 16);
     B->n_refs = (uint32_t)L.index.size();
     if ((st = up((void**)&B->nodes, L.nodes.data(), L.nodes.size() * 4)) != RB_OK) { cleanup(); return st; }
+    B->n_wnodes = (uint32_t)(wide.size() / kWideWords);
+    if (!wide.empty() && (st = up((void**)&B->wnodes, wide.data(), wide.size() * 4)) != RB_OK) { cleanup(); return st; }
     // one spare row after the triangle array keeps 16-byte loads in bounds
     L.tris.resize(L.tris.size() + 4, 0.0f);
     if ((st = up((void**)&B->tris, L.tris.data(), L.tris.size() * 4)) != RB_OK) { cleanup(); return st; }
@@ -408,6 +413,7 @@ rb_status rb_bvh_release(rb_bvh_t b) {
     if (!b) return RB_OK;
     if (dev().device >= 0) {
         dfree(b->nodes, (size_t)b->n_nodes * 64);
+        dfree(b->wnodes, (size_t)b->n_wnodes * kWideWords * 4);
         dfree(b->tris, ((size_t)b->n_refs * 12 + 4) * 4);
         dfree(b->index, ((size_t)b->n_refs + 1) * 4);
         dfree(b->tri_v, (size_t)b->n_tris * 48);

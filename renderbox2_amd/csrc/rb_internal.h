@@ -163,6 +163,16 @@ struct Mi355Layout {
     std::vector<uint32_t> index;  // 1 per triangle reference
 };
 rb_status export_mi355x(const HostBvh& b, Mi355Layout& out);
+// Wide (4-child) records of an MI355X binary node array, 32 words each:
+// rows 0..5 (4 words each) the children's lo.x, hi.x, lo.y, hi.y, lo.z,
+// hi.z; row 6 their links (>= 0: wide record; < 0: leaf, as in the binary
+// records; kWideEmpty: no child); row 7 unused.  Record 0 is the root, the
+// records are numbered breadth-first.  Closest-hit results do not depend on
+// the visit order (ties at equal t go to the lowest triangle id), so the
+// wide traversal returns the binary one's (id, t).
+constexpr uint32_t kWideWords = 32;
+constexpr int32_t kWideEmpty = 0x7FFFFFFF;
+rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& wide);
 // Depth of the deepest inner record of an MI355X node array (root = 0);
 // UINT32_MAX when a link leaves the array or the links form a cycle.  A
 // traversal holds at most depth + 1 stack entries (the sentinel, then one

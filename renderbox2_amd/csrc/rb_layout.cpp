@@ -253,6 +253,69 @@ rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
     return RB_OK;
 }
 
+// Wide records from the binary ones: the children of a binary record, every
+// inner child replaced by its own two children, so one fetch holds up to
+// four boxes and a ray reads about half as many records.  Binary records
+// that became wide records are numbered breadth-first from the root, so the
+// records the traversal keeps in LDS are the top of the tree.
+rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& wide) {
+    const uint32_t n = (uint32_t)(bin.size() / 16);
+    wide.clear();
+    if (n == 0) return RB_OK;
+    std::vector<uint32_t> order;  // binary record of each wide record
+    std::vector<int32_t> wid(n, -1);
+    order.push_back(0);
+    wid[0] = 0;
+    for (size_t h = 0; h < order.size(); h++) {
+        const uint32_t* d = &bin[(size_t)order[h] * 16];
+        uint32_t box[4][6];
+        int32_t link[4];
+        uint32_t k = 0;
+        auto add = [&](const uint32_t* rec, int c) {
+            // child c of binary record rec: box words and link
+            const uint32_t* b = rec + 4 * c;  // lo.x, hi.x, lo.y, hi.y
+            box[k][0] = b[0];
+            box[k][1] = b[1];
+            box[k][2] = b[2];
+            box[k][3] = b[3];
+            box[k][4] = rec[8 + 2 * c];
+            box[k][5] = rec[9 + 2 * c];
+            const int32_t l = (int32_t)rec[12 + c];
+            if (l >= 0) {
+                if ((uint32_t)l >= n) return false;
+                if (wid[l] < 0) {
+                    wid[l] = (int32_t)order.size();
+                    order.push_back((uint32_t)l);
+                }
+                link[k] = wid[l];
+            } else {
+                link[k] = l;
+            }
+            k++;
+            return true;
+        };
+        for (int c = 0; c < 2; c++) {
+            const int32_t l = (int32_t)d[12 + c];
+            if (l >= 0) {  // an inner child: its two children instead
+                if ((uint32_t)l >= n) return fail(RB_ERR_INVALID, "bad link in the node array");
+                const uint32_t* g = &bin[(size_t)l * 16];
+                if (!add(g, 0) || !add(g, 1)) return fail(RB_ERR_INVALID, "bad link in the node array");
+            } else if (!add(d, c)) {
+                return fail(RB_ERR_INVALID, "bad link in the node array");
+            }
+        }
+        const size_t base = wide.size();
+        wide.resize(base + kWideWords, 0);
+        uint32_t* w = &wide[base];
+        for (uint32_t i = 0; i < 4; i++) {
+            for (int r = 0; r < 6; r++) w[4 * r + i] = i < k ? box[i][r] : 0u;
+            w[24 + i] = i < k ? (uint32_t)link[i] : (uint32_t)kWideEmpty;
+        }
+        if (order.size() > 0x7FFFFFF0u) return fail(RB_ERR_UNSUPPORTED, "too many BVH nodes");
+    }
+    return RB_OK;
+}
+
 uint32_t layout_depth(const std::vector<uint32_t>& nodes) {
     const uint32_t n = (uint32_t)(nodes.size() / 16);
     if (n == 0) return 0;

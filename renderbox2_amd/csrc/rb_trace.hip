@@ -178,7 +178,9 @@ __global__ __launch_bounds__(256) void k_rcp_check(unsigned long long* out) {
 
 // PAIR: rays [0, n0) are the first set (a.rays / a.hits; any hit when
 // a.any0), rays [n0, nrays) the closest-hit set (a.rays2 / a.hits2).
-template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC, bool PAIR = false>
+// WIDE: a.nodes holds wide (4-child) records (export_wide); closest hit
+// only, where the visit order does not change the (id, t) a ray reports.
+template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC, bool PAIR = false, bool WIDE = false>
 __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays,
                                            uint32_t n0 = 0) {
     __shared__ int lstack[S * BLOCK];
@@ -220,9 +222,10 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     // top of the tree (the layout's first records, breadth-first) in LDS
     extern __shared__ V4f lnodes[];
     const uint32_t top = MULTI ? 0u : a.top_nodes;
+    constexpr uint32_t kRows = WIDE ? 8u : 4u;  // float4 rows per record
     if (top) {
         const GF4 src = (GF4)a.nodes;
-        for (uint32_t i = tid; i < 4 * top; i += BLOCK) lnodes[i] = src[i];
+        for (uint32_t i = tid; i < kRows * top; i += BLOCK) lnodes[i] = src[i];
         __syncthreads();
     }
     uint32_t pool_next = 0, pool_end = 0;
@@ -316,7 +319,65 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
 
         // ---- traversal ----
         while (nodeAddr != kSentinel) {
-            while ((uint32_t)nodeAddr < (uint32_t)kSentinel && (SPEC || leafAddr >= 0)) {
+            while (WIDE && (uint32_t)nodeAddr < (uint32_t)kSentinel && leafAddr >= 0) {
+                // a wide record: the four children's slab tests, then the hit
+                // children nearest first (the rest pushed, farthest deepest)
+                V4f lx, hx, ly, hy, lz, hz;
+                V4i lk;
+                if ((uint32_t)nodeAddr < top) {
+                    const V4f* ln = lnodes + 8 * nodeAddr;
+                    lx = ln[0]; hx = ln[1]; ly = ln[2]; hy = ln[3]; lz = ln[4]; hz = ln[5];
+                    lk = __builtin_bit_cast(V4i, ln[6]);
+                } else {
+                    const GF4 np = nodes + 8 * (size_t)nodeAddr;
+                    lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
+                    lk = *(GI4)(np + 6);
+                    asm volatile("" ::"v"(lx.x), "v"(hx.x), "v"(ly.x), "v"(hy.x), "v"(lz.x), "v"(hz.x), "v"(lk.x));
+                }
+                if (COUNT) c_inner++;
+                float key[4];
+                int lnk[4];
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    const float clox = fmaf(lx[c], idx_, -oodx), chix = fmaf(hx[c], idx_, -oodx);
+                    const float cloy = fmaf(ly[c], idy_, -oody), chiy = fmaf(hy[c], idy_, -oody);
+                    const float cloz = fmaf(lz[c], idz_, -oodz), chiz = fmaf(hz[c], idz_, -oodz);
+                    const float cmin = vmax3_(vmin_(clox, chix), vmin_(cloy, chiy), vmax_(vmin_(cloz, chiz), tmin));
+                    const float cmax = vmin3_(vmax_(clox, chix), vmax_(cloy, chiy), vmin_(vmax_(cloz, chiz), hitT));
+                    const bool h = cmax >= cmin && lk[c] != kWideEmpty;
+                    // a hit box's key stays finite (misses sort last as inf)
+                    key[c] = h ? fminf(cmin, 0x1.fffffep127f) : __builtin_inff();
+                    lnk[c] = lk[c];
+                }
+                // sort the four (entry distance, link) pairs, misses last
+#define RB_CS(i, j)                                   \
+    if (key[j] < key[i]) {                            \
+        const float tk = key[i];                      \
+        key[i] = key[j];                              \
+        key[j] = tk;                                  \
+        const int tl = lnk[i];                        \
+        lnk[i] = lnk[j];                              \
+        lnk[j] = tl;                                  \
+    }
+                RB_CS(0, 1) RB_CS(2, 3) RB_CS(0, 2) RB_CS(1, 3) RB_CS(1, 2)
+#undef RB_CS
+                const bool h1 = key[1] != __builtin_inff(), h2 = key[2] != __builtin_inff(),
+                           h3 = key[3] != __builtin_inff();
+                if (key[0] == __builtin_inff()) {
+                    RB_POP(nodeAddr);
+                } else {
+                    if (h3) RB_PUSH(lnk[3]);
+                    if (h2) RB_PUSH(lnk[2]);
+                    if (h1) RB_PUSH(lnk[1]);
+                    nodeAddr = lnk[0];
+                }
+                if (nodeAddr < 0 && leafAddr >= 0) {
+                    leafAddr = nodeAddr;
+                    RB_POP(nodeAddr);
+                }
+                if (__popcll(__ballot(1)) <= a.leaf_gate) break;
+            }
+            while (!WIDE && (uint32_t)nodeAddr < (uint32_t)kSentinel && (SPEC || leafAddr >= 0)) {
                 V4f n0, n1, nz;
                 V4i lk;
                 if ((uint32_t)nodeAddr < top) {
@@ -454,6 +515,13 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     trace_body<ANY, COUNT, S, BLOCK, false, SPEC>(a, nullptr, 0, nrays);
 }
 
+// Closest hit over the wide records (COUNT: inner = wide records visited).
+template <bool COUNT, int S, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_trace_wide(TraceArgs a) {
+    const uint32_t nrays = a.n_dev ? __builtin_nontemporal_load(a.n_dev) : a.n;
+    trace_body<false, COUNT, S, BLOCK, false, false, false, true>(a, nullptr, 0, nrays);
+}
+
 template <int S, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace_pair(TraceArgs a) {
     const uint32_t n0 = __builtin_nontemporal_load(a.n_dev);
@@ -504,6 +572,15 @@ static TraceKernel pick(const rb_trace_config& c) {
     return nullptr;
 }
 
+template <bool COUNT>
+static TraceKernel pick_wide(const rb_trace_config& c) {
+#define RB_K(s, b) \
+    if (c.lds_stack == s && c.block == b) return k_trace_wide<COUNT, s, b>;
+    RB_FOR_CONFIGS(RB_K)
+#undef RB_K
+    return nullptr;
+}
+
 static TraceKernel pick_pair(const rb_trace_config& c) {
 #define RB_K(s, b) \
     if (c.lds_stack == s && c.block == b) return k_trace_pair<s, b>;
@@ -537,7 +614,16 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     if (n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "too many rays in one call (%u)", n);
     if ((flags & RB_MASKED) && !mask) return fail(RB_ERR_INVALID, "RB_MASKED without a mask");
     const bool any = (flags & RB_ANY) != 0;
-    TraceKernel k = counts ? (any ? pick<true, true>(cfg) : pick<false, true>(cfg))
+    // Closest hit walks the wide records (the same (id, t)); visit counts
+    // follow the binary walk unless RB_WIDE_COUNTS asks for the wide one.
+    const bool wide_counts = (flags & RB_WIDE_COUNTS) != 0;
+    if (wide_counts && (!counts || any || cfg.speculative))
+        return fail(RB_ERR_INVALID, "RB_WIDE_COUNTS: visit counts of a closest-hit, non-speculative trace only");
+    if (wide_counts && !bvh->wnodes)
+        return fail(RB_ERR_UNSUPPORTED, "RB_WIDE_COUNTS: this BVH has no wide records (too deep for the stack)");
+    const bool wide = wide_counts || (cfg.wide && !any && !counts && !cfg.speculative && bvh->wnodes);
+    TraceKernel k = wide ? (counts ? pick_wide<true>(cfg) : pick_wide<false>(cfg))
+                  : counts ? (any ? pick<true, true>(cfg) : pick<false, true>(cfg))
                            : (any ? pick<true, false>(cfg) : pick<false, false>(cfg));
     if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
     TraceArgs a;
@@ -556,7 +642,13 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.threshold = cfg.fetch_threshold;
     a.leaf_gate = cfg.leaf_gate;
     a.top_nodes = cfg.lds_nodes < bvh->n_nodes ? cfg.lds_nodes : bvh->n_nodes;
-    const size_t lds = (size_t)cfg.lds_stack * cfg.block * 4 + (size_t)a.top_nodes * 64;
+    size_t top_bytes = (size_t)a.top_nodes * 64;
+    if (wide) {  // the same LDS budget: half as many records of twice the size
+        a.nodes = bvh->wnodes;
+        a.top_nodes = cfg.lds_nodes / 2 < bvh->n_wnodes ? cfg.lds_nodes / 2 : bvh->n_wnodes;
+        top_bytes = (size_t)a.top_nodes * 128;
+    }
+    const size_t lds = (size_t)cfg.lds_stack * cfg.block * 4 + top_bytes;
     if (lds > dev().lds_per_block)
         return fail(RB_ERR_INVALID, "trace config needs %zu bytes of LDS per block (stack %u x %u lanes + %u node records); "
                     "the device allows %zu", lds, cfg.lds_stack, cfg.block, a.top_nodes, dev().lds_per_block);
@@ -569,7 +661,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
         if (!L.counter) return fail(RB_ERR_INVALID, "render graph: out of work-counter slots");
     }
     a.counter = L.counter;
-    hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), (size_t)a.top_nodes * 64, stream, a);
+    hipLaunchKernelGGL(k, dim3(persistent_blocks(cfg, n)), dim3(cfg.block), top_bytes, stream, a);
     const hipError_t e = hipGetLastError();
     release_counter_slot(L, stream);
     if (e != hipSuccess) return hip_fail(e, "trace launch");
@@ -702,6 +794,7 @@ static rb_status check_cfg(const rb_trace_config& c) {
     if (c.speculative > 1) return fail(RB_ERR_INVALID, "speculative must be 0 or 1");
     if (c.leaf_gate > 63) return fail(RB_ERR_INVALID, "leaf_gate must be 0..63");
     if (c.lds_nodes > 512 || c.lds_nodes % 64) return fail(RB_ERR_INVALID, "lds_nodes must be 0..512, step 64");
+    if (c.wide > 1) return fail(RB_ERR_INVALID, "wide must be 0 or 1");
     return RB_OK;
 }
 

@@ -87,7 +87,7 @@ def test_trace_configs_without_a_kernel_are_rejected(gpu):
     saved = gpu.get_trace_config()
     try:
         for bad in (dict(block=1024, lds_stack=24), dict(block=512, lds_stack=20), dict(block=384),
-                    dict(lds_nodes=576), dict(lds_nodes=100)):
+                    dict(lds_nodes=576), dict(lds_nodes=100), dict(wide=2)):
             with pytest.raises(gpu.RbError):
                 gpu.set_trace_config(**bad)
         assert gpu.get_trace_config() == saved

@@ -63,11 +63,14 @@ def _check_any(orc, geom, rays, ids, ts, aid, at):
     assert np.array_equal(t1.view(np.uint32), at[h].view(np.uint32))
 
 
+@pytest.mark.parametrize("wide", [1, 0])
 @pytest.mark.parametrize("cfg", range(len(CONFIGS)))
-def test_trace_small_bit_exact(gpu, orc, small, cfg):
+def test_trace_small_bit_exact(gpu, orc, small, cfg, wide):
+    """Closest and any hit of every launch shape equal the oracle; wide=1
+    walks the 4-child records for closest hits, wide=0 the binary ones."""
     v, idx, b, tr, c2 = small
     saved = gpu.get_trace_config()
-    gpu.set_trace_config(**CONFIGS[cfg])
+    gpu.set_trace_config(**CONFIGS[cfg], wide=wide)
     try:
         rays = random_rays(50000, seed=60 + cfg)
         ids, ts = tr.trace(rays)
@@ -117,6 +120,24 @@ def test_visit_counts_equal_the_oracle(gpu, orc):
             assert np.array_equal(cnt, ocnt), cfg
     finally:
         gpu.set_trace_config(**saved)
+
+
+def test_wide_walk_counts(gpu, orc, small):
+    """The wide walk's own visit counts (RB_WIDE_COUNTS): the same (id, t)
+    as the oracle, the root record read by every ray, and fewer records per
+    ray than the binary walk reads; any hit has no wide counts."""
+    v, idx, b, tr, c2 = small
+    rays = random_rays(30000, seed=81)
+    oid, ot = orc.trace_compact2(*c2, rays)
+    ids, ts, wc = tr.trace_counts(rays, wide=True)
+    assert np.array_equal(ids, oid) and np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
+    bid, _, bc = tr.trace_counts(rays)
+    assert np.array_equal(bid, oid)
+    assert (wc[:, 0] >= 1).all()
+    assert wc[:, 0].sum() < 0.75 * bc[:, 0].sum()
+    assert (wc[ids >= 0, 1] >= 1).all() and (wc[ids >= 0, 2] >= 1).all()  # a hit tested its triangle
+    with pytest.raises(gpu.RbError):
+        tr.trace_counts(rays[:64], any_hit=True, wide=True)
 
 
 def test_trace_room_camera_and_diffuse_rays(gpu, orc, room):
@@ -259,10 +280,11 @@ def test_million_triangles_properties(gpu, orc):
     assert np.array_equal(ids[sel], oid) and np.array_equal(ts[sel].view(np.uint32), ot.view(np.uint32))
     oaid, _ = orc.trace_compact2(*c2, diff[sel], any_hit=True)
     assert np.array_equal(aid[sel], oaid)
-    # every launch shape gives the same answers: each ray's visit order is its own
+    # every launch shape and both record widths give the same answers: each
+    # ray's visit order is its own, and ties at equal t go to the lowest id
     saved = gpu.get_trace_config()
     try:
-        for cfg in CONFIGS[1:3]:
+        for cfg in [dict(saved, wide=1 - saved["wide"])] + CONFIGS[1:3]:
             gpu.set_trace_config(**cfg)
             i3, t3 = tr.trace(diff)
             a3, _ = tr.trace(diff, any_hit=True)
