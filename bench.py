@@ -541,12 +541,12 @@ def main():
     ids, _, gcnt = tracer.trace_counts(d_diff.download(np.float32, n * 8).reshape(n, 8))
     gpu_bpr = float((40 + 64 * gcnt[:, 0] + 48 * gcnt[:, 1] + 16 * gcnt[:, 2] + 4 * (ids >= 0)).mean())
     # bytes the timed kernel itself reads and writes per ray: the ray (32),
-    # the hit (8), each visited record (64 binary; 112 = seven 16-byte rows
-    # wide), each tested triangle (48) and the hit's index entry (4)
+    # the hit (8), each visited record (64, binary or wide), each tested
+    # triangle (48) and the hit's index entry (4)
     wide = bool(cfg["wide"]) and not cfg["speculative"]
     if wide:
         wids, _, wcnt = tracer.trace_counts(d_diff.download(np.float32, n * 8).reshape(n, 8), wide=True)
-        kernel_bpr = float((40 + 112 * wcnt[:, 0] + 48 * wcnt[:, 1] + 4 * (wids >= 0)).mean())
+        kernel_bpr = float((40 + 64 * wcnt[:, 0] + 48 * wcnt[:, 1] + 4 * (wids >= 0)).mean())
         wide_records = float(wcnt[:, 0].mean())
         del wids, wcnt
     else:
@@ -663,6 +663,7 @@ def main():
                      "kernel_frac": round(kernel_achieved / HBM_PEAK_GBS, 4),
                      "wide_records_per_ray": None if wide_records is None else round(wide_records, 2),
                      "binary_records_per_ray": round(float(gcnt[:, 0].mean()), 2),
+                     "tris_per_ray": round(float(gcnt[:, 1].mean()), 2),
                      "l2_hit_rate": l2_hit},
         "cpu_baseline": cpu,
     }

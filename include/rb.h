@@ -384,11 +384,12 @@ typedef struct rb_trace_config {
                                 Any value gives the same results */
     uint32_t wide;           /* 1 (default): closest-hit traces without a visit-count
                                 buffer walk 4-child records (each binary record's
-                                children, inner ones replaced by their own two), about
-                                half the dependent fetches per ray; lds_nodes / 2 such
-                                records go to LDS.  The same (id, t) as the binary walk,
-                                whose order any-hit traces and visit counts keep; 0: the
-                                binary records for every trace */
+                                children, inner ones replaced by their own two; 64
+                                bytes, boxes in 8-bit steps that hold the exact ones),
+                                about half the dependent fetches per ray; lds_nodes
+                                such records go to LDS.  The same (id, t) as the binary
+                                walk, whose order any-hit traces and visit counts keep;
+                                0: the binary records for every trace */
 } rb_trace_config;
 rb_status rb_trace_get_config(rb_trace_config* c);
 rb_status rb_trace_set_config(const rb_trace_config* c);

@@ -163,14 +163,18 @@ struct Mi355Layout {
     std::vector<uint32_t> index;  // 1 per triangle reference
 };
 rb_status export_mi355x(const HostBvh& b, Mi355Layout& out);
-// Wide (4-child) records of an MI355X binary node array, 32 words each:
-// rows 0..5 (4 words each) the children's lo.x, hi.x, lo.y, hi.y, lo.z,
-// hi.z; row 6 their links (>= 0: wide record; < 0: leaf, as in the binary
-// records; kWideEmpty: no child); row 7 unused.  Record 0 is the root, the
+// Wide (4-child) records of an MI355X binary node array, 16 words each:
+// words 0..2 the origin p (float), word 3 the biased exponents of the step
+// sizes s = 2^e (byte a: e + 127 of axis a), words 4..9 the children's 8-bit
+// steps (byte c of each word = child c) for lo.x, hi.x, lo.y, hi.y, lo.z,
+// hi.z (plane = fmaf(q, s, p), a box holding the exact one), words 10..13
+// their links (>= 0: wide record; < 0: leaf, as in the binary records;
+// kWideEmpty: no child), words 14..15 unused.  Record 0 is the root, the
 // records are numbered breadth-first.  Closest-hit results do not depend on
-// the visit order (ties at equal t go to the lowest triangle id), so the
-// wide traversal returns the binary one's (id, t).
-constexpr uint32_t kWideWords = 32;
+// the visit order (ties at equal t go to the lowest triangle id) and every
+// leaf the binary walk tests is tested, so the wide traversal returns the
+// binary one's (id, t).
+constexpr uint32_t kWideWords = 16;
 constexpr int32_t kWideEmpty = 0x7FFFFFFF;
 rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& wide);
 // Depth of the deepest inner record of an MI355X node array (root = 0);

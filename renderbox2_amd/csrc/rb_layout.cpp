@@ -4,6 +4,7 @@
 //    (renderbox2 src/accelerators/sbvh/cudabvh.cpp:216-296): kept so that
 //    BVH files and byte streams can be compared with the reference format.
 //  * MI355X: the traversal layout of this library (see export_mi355x).
+#include <algorithm>
 #include <cstdio>
 #include <fstream>
 
@@ -258,6 +259,60 @@ rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
 // four boxes and a ray reads about half as many records.  Binary records
 // that became wide records are numbered breadth-first from the root, so the
 // records the traversal keeps in LDS are the top of the tree.
+//
+// Child boxes are stored as 8-bit steps from the record's origin p (the
+// smallest corner of its children) in units of s = 2^e per axis; the
+// traversal decodes a plane as fmaf(q, s, p), rounded once.  Each lo step is
+// the largest q whose decoded plane is <= the box's lo and each hi step the
+// smallest q whose plane is >= its hi, checked with that same fmaf, so the
+// decoded box holds the exact one.  The slab distances fmaf(plane, 1/d, -o/d)
+// are monotone in the plane, so a ray passes a decoded box whenever it
+// passes the exact one: a ray tests every leaf the binary walk would at the
+// same hitT, and the closest hit (ties to the lowest triangle id) is the
+// same.  Empty leaves (no triangles) are left out.
+namespace {
+
+float decode_plane(uint32_t q, float s, float p) { return std::fma((float)q, s, p); }
+
+// Steps of one axis for `k` children (lo[i] <= hi[i]) from origin p: the
+// smallest exponent (biased, 1..254) whose steps all fit in 8 bits.
+bool quantize_axis(const float* lo, const float* hi, uint32_t k, float p, uint32_t* qlo, uint32_t* qhi,
+                   uint32_t* ebias) {
+    double ext = 0.0;
+    for (uint32_t i = 0; i < k; i++) ext = std::max(ext, (double)hi[i] - (double)p);
+    int e = -126;
+    if (ext > 0.0) {
+        int ex;
+        std::frexp(ext / 250.0, &ex);  // 2^(ex-1) <= ext/250 < 2^ex
+        e = std::max(-126, ex);
+    }
+    for (; e <= 127; e++) {
+        const float s = std::ldexp(1.0f, e);
+        bool fits = true;
+        for (uint32_t i = 0; i < k && fits; i++) {
+            double ql = std::floor(((double)lo[i] - (double)p) / (double)s);
+            uint32_t a = ql <= 0.0 ? 0u : (ql >= 255.0 ? 255u : (uint32_t)ql);
+            while (a > 0 && decode_plane(a, s, p) > lo[i]) a--;
+            double qh = std::ceil(((double)hi[i] - (double)p) / (double)s);
+            uint32_t b = qh <= (double)a ? a : (qh >= 256.0 ? 256u : (uint32_t)qh);
+            while (b <= 255 && decode_plane(b, s, p) < hi[i]) b++;
+            if (b > 255 || decode_plane(a, s, p) > lo[i]) {
+                fits = false;
+                break;
+            }
+            qlo[i] = a;
+            qhi[i] = b;
+        }
+        if (fits) {
+            *ebias = (uint32_t)(e + 127);
+            return true;
+        }
+    }
+    return false;
+}
+
+}  // namespace
+
 rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& wide) {
     const uint32_t n = (uint32_t)(bin.size() / 16);
     wide.clear();
@@ -268,21 +323,23 @@ rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& w
     wid[0] = 0;
     for (size_t h = 0; h < order.size(); h++) {
         const uint32_t* d = &bin[(size_t)order[h] * 16];
-        uint32_t box[4][6];
+        float box[4][6];  // lo.x, hi.x, lo.y, hi.y, lo.z, hi.z
         int32_t link[4];
         uint32_t k = 0;
+        bool bad = false;
         auto add = [&](const uint32_t* rec, int c) {
-            // child c of binary record rec: box words and link
-            const uint32_t* b = rec + 4 * c;  // lo.x, hi.x, lo.y, hi.y
-            box[k][0] = b[0];
-            box[k][1] = b[1];
-            box[k][2] = b[2];
-            box[k][3] = b[3];
-            box[k][4] = rec[8 + 2 * c];
-            box[k][5] = rec[9 + 2 * c];
             const int32_t l = (int32_t)rec[12 + c];
+            if (l < 0 && (~(uint32_t)l & 31u) == 0) return;  // empty leaf
+            const uint32_t* b = rec + 4 * c;
+            const uint32_t w[6] = {b[0], b[1], b[2], b[3], rec[8 + 2 * c], rec[9 + 2 * c]};
+            for (int r = 0; r < 6; r++) std::memcpy(&box[k][r], &w[r], 4);
+            for (int a = 0; a < 3; a++)
+                if (!(box[k][2 * a] <= box[k][2 * a + 1])) bad = true;  // inverted or NaN
             if (l >= 0) {
-                if ((uint32_t)l >= n) return false;
+                if ((uint32_t)l >= n) {
+                    bad = true;
+                    return;
+                }
                 if (wid[l] < 0) {
                     wid[l] = (int32_t)order.size();
                     order.push_back((uint32_t)l);
@@ -292,25 +349,42 @@ rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& w
                 link[k] = l;
             }
             k++;
-            return true;
         };
         for (int c = 0; c < 2; c++) {
             const int32_t l = (int32_t)d[12 + c];
             if (l >= 0) {  // an inner child: its two children instead
                 if ((uint32_t)l >= n) return fail(RB_ERR_INVALID, "bad link in the node array");
                 const uint32_t* g = &bin[(size_t)l * 16];
-                if (!add(g, 0) || !add(g, 1)) return fail(RB_ERR_INVALID, "bad link in the node array");
-            } else if (!add(d, c)) {
-                return fail(RB_ERR_INVALID, "bad link in the node array");
+                add(g, 0);
+                add(g, 1);
+            } else {
+                add(d, c);
             }
         }
+        if (bad) return fail(RB_ERR_UNSUPPORTED, "a BVH box is inverted, NaN or links out of the array");
         const size_t base = wide.size();
         wide.resize(base + kWideWords, 0);
         uint32_t* w = &wide[base];
-        for (uint32_t i = 0; i < 4; i++) {
-            for (int r = 0; r < 6; r++) w[4 * r + i] = i < k ? box[i][r] : 0u;
-            w[24 + i] = i < k ? (uint32_t)link[i] : (uint32_t)kWideEmpty;
+        uint32_t ebias[3] = {127, 127, 127};
+        for (int a = 0; a < 3 && k; a++) {
+            float lo[4], hi[4];
+            uint32_t qlo[4], qhi[4];
+            float p = box[0][2 * a];
+            for (uint32_t i = 0; i < k; i++) {
+                lo[i] = box[i][2 * a];
+                hi[i] = box[i][2 * a + 1];
+                p = std::min(p, lo[i]);
+            }
+            if (!quantize_axis(lo, hi, k, p, qlo, qhi, &ebias[a]))
+                return fail(RB_ERR_UNSUPPORTED, "a BVH box cannot be stored in 8-bit steps");
+            std::memcpy(&w[a], &p, 4);
+            for (uint32_t i = 0; i < k; i++) {
+                w[4 + 2 * a] |= qlo[i] << (8 * i);
+                w[5 + 2 * a] |= qhi[i] << (8 * i);
+            }
         }
+        w[3] = ebias[0] | ebias[1] << 8 | ebias[2] << 16;
+        for (uint32_t i = 0; i < 4; i++) w[10 + i] = i < k ? (uint32_t)link[i] : (uint32_t)kWideEmpty;
         if (order.size() > 0x7FFFFFF0u) return fail(RB_ERR_UNSUPPORTED, "too many BVH nodes");
     }
     return RB_OK;

@@ -222,10 +222,9 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     // top of the tree (the layout's first records, breadth-first) in LDS
     extern __shared__ V4f lnodes[];
     const uint32_t top = MULTI ? 0u : a.top_nodes;
-    constexpr uint32_t kRows = WIDE ? 8u : 4u;  // float4 rows per record
-    if (top) {
+    if (top) {  // 4 float4 rows per record, binary or wide
         const GF4 src = (GF4)a.nodes;
-        for (uint32_t i = tid; i < kRows * top; i += BLOCK) lnodes[i] = src[i];
+        for (uint32_t i = tid; i < 4 * top; i += BLOCK) lnodes[i] = src[i];
         __syncthreads();
     }
     uint32_t pool_next = 0, pool_end = 0;
@@ -320,28 +319,44 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
         // ---- traversal ----
         while (nodeAddr != kSentinel) {
             while (WIDE && (uint32_t)nodeAddr < (uint32_t)kSentinel && leafAddr >= 0) {
-                // a wide record: the four children's slab tests, then the hit
-                // children nearest first (the rest pushed, farthest deepest)
-                V4f lx, hx, ly, hy, lz, hz;
-                V4i lk;
+                // a wide record: the four children's slab tests on their
+                // decoded boxes, then the hit children nearest first (the
+                // rest pushed, farthest deepest)
+                V4f r0;
+                V4i r1, r2, r3;
                 if ((uint32_t)nodeAddr < top) {
-                    const V4f* ln = lnodes + 8 * nodeAddr;
-                    lx = ln[0]; hx = ln[1]; ly = ln[2]; hy = ln[3]; lz = ln[4]; hz = ln[5];
-                    lk = __builtin_bit_cast(V4i, ln[6]);
+                    const V4f* ln = lnodes + 4 * nodeAddr;
+                    r0 = ln[0];
+                    r1 = __builtin_bit_cast(V4i, ln[1]);
+                    r2 = __builtin_bit_cast(V4i, ln[2]);
+                    r3 = __builtin_bit_cast(V4i, ln[3]);
                 } else {
-                    const GF4 np = nodes + 8 * (size_t)nodeAddr;
-                    lx = np[0]; hx = np[1]; ly = np[2]; hy = np[3]; lz = np[4]; hz = np[5];
-                    lk = *(GI4)(np + 6);
-                    asm volatile("" ::"v"(lx.x), "v"(hx.x), "v"(ly.x), "v"(hy.x), "v"(lz.x), "v"(hz.x), "v"(lk.x));
+                    const GF4 np = nodes + 4 * (size_t)nodeAddr;
+                    r0 = np[0];
+                    r1 = *(GI4)(np + 1);
+                    r2 = *(GI4)(np + 2);
+                    r3 = *(GI4)(np + 3);
+                    asm volatile("" ::"v"(r0.x), "v"(r1.x), "v"(r2.x), "v"(r3.x));
                 }
                 if (COUNT) c_inner++;
+                // step sizes 2^e from the biased exponents (bytes of word 3)
+                const uint32_t eb = __float_as_uint(r0.w);
+                const float sx = __uint_as_float((eb & 0xFFu) << 23);
+                const float sy = __uint_as_float(((eb >> 8) & 0xFFu) << 23);
+                const float sz = __uint_as_float(((eb >> 16) & 0xFFu) << 23);
+                const int lk[4] = {r2.z, r2.w, r3.x, r3.y};
                 float key[4];
                 int lnk[4];
 #pragma unroll
                 for (int c = 0; c < 4; c++) {
-                    const float clox = fmaf(lx[c], idx_, -oodx), chix = fmaf(hx[c], idx_, -oodx);
-                    const float cloy = fmaf(ly[c], idy_, -oody), chiy = fmaf(hy[c], idy_, -oody);
-                    const float cloz = fmaf(lz[c], idz_, -oodz), chiz = fmaf(hz[c], idz_, -oodz);
+#define RB_Q(w, s, p) fmaf((float)(((uint32_t)(w) >> (8 * c)) & 0xFFu), s, p)
+                    const float lx = RB_Q(r1.x, sx, r0.x), hx = RB_Q(r1.y, sx, r0.x);
+                    const float ly = RB_Q(r1.z, sy, r0.y), hy = RB_Q(r1.w, sy, r0.y);
+                    const float lz = RB_Q(r2.x, sz, r0.z), hz = RB_Q(r2.y, sz, r0.z);
+#undef RB_Q
+                    const float clox = fmaf(lx, idx_, -oodx), chix = fmaf(hx, idx_, -oodx);
+                    const float cloy = fmaf(ly, idy_, -oody), chiy = fmaf(hy, idy_, -oody);
+                    const float cloz = fmaf(lz, idz_, -oodz), chiz = fmaf(hz, idz_, -oodz);
                     const float cmin = vmax3_(vmin_(clox, chix), vmin_(cloy, chiy), vmax_(vmin_(cloz, chiz), tmin));
                     const float cmax = vmin3_(vmax_(clox, chix), vmax_(cloy, chiy), vmin_(vmax_(cloz, chiz), hitT));
                     const bool h = cmax >= cmin && lk[c] != kWideEmpty;
@@ -642,12 +657,11 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     a.threshold = cfg.fetch_threshold;
     a.leaf_gate = cfg.leaf_gate;
     a.top_nodes = cfg.lds_nodes < bvh->n_nodes ? cfg.lds_nodes : bvh->n_nodes;
-    size_t top_bytes = (size_t)a.top_nodes * 64;
-    if (wide) {  // the same LDS budget: half as many records of twice the size
+    if (wide) {  // wide records are 64 bytes too
         a.nodes = bvh->wnodes;
-        a.top_nodes = cfg.lds_nodes / 2 < bvh->n_wnodes ? cfg.lds_nodes / 2 : bvh->n_wnodes;
-        top_bytes = (size_t)a.top_nodes * 128;
+        a.top_nodes = cfg.lds_nodes < bvh->n_wnodes ? cfg.lds_nodes : bvh->n_wnodes;
     }
+    const size_t top_bytes = (size_t)a.top_nodes * 64;
     const size_t lds = (size_t)cfg.lds_stack * cfg.block * 4 + top_bytes;
     if (lds > dev().lds_per_block)
         return fail(RB_ERR_INVALID, "trace config needs %zu bytes of LDS per block (stack %u x %u lanes + %u node records); "
