@@ -97,6 +97,12 @@ inline RngArg rng_arg(uint64_t seed, uint32_t tag) {
 }
 rb_status hip_fail(hipError_t e, const char* what);
 
+// Internal trace flag, beside RB_ANY: an any-hit set whose consumer reads
+// only hit / miss (hit.x < 0).  Such a set may walk the wide records, which
+// find a hit exactly when the binary walk does but not necessarily the same
+// first one.  The public entry points refuse the bit.
+constexpr uint32_t kOcclusion = 1u << 16;
+
 // Persistent traversal launch (rb_trace.hip).  n_dev, when set, holds the ray
 // count in device memory (queues sized by earlier kernels); n is then an
 // upper bound that only sizes the grid.  counter, when set, is the launch's
@@ -124,6 +130,7 @@ struct BatchSet {
     const uint32_t* n_dev;
     bool any;
     const uint8_t* mask;  // NULL: every ray is traced
+    bool occlusion = false;  // any hit read only as hit / miss (kOcclusion)
 };
 // Ray sets of any resident BVHs through batched persistent launches (up to
 // 64 sets and 2^31 - 1 rays of bound per launch).  Same per-ray results as
@@ -136,7 +143,7 @@ struct TraceSet {
     int2* hits;
     uint32_t n;             // rays, or the bound of *n_dev
     const uint32_t* n_dev;  // device count (NULL: n)
-    uint32_t flags;         // RB_CLOSEST or RB_ANY, with RB_MASKED
+    uint32_t flags;         // RB_CLOSEST or RB_ANY (with kOcclusion), with RB_MASKED
     const uint8_t* mask;
     rb_trace_config cfg;    // launch shape when the set is traced on its own
 };

@@ -53,10 +53,10 @@ static rb_status init_locked(int device) {
     g_dev.cfg.block = 512;
     g_dev.cfg.lds_stack = 12;
     g_dev.cfg.chunk = 64;
-    g_dev.cfg.fetch_threshold = 48;
+    g_dev.cfg.fetch_threshold = 60;
     g_dev.cfg.waves_per_cu = 24;
     g_dev.cfg.speculative = 0;
-    g_dev.cfg.leaf_gate = 16;
+    g_dev.cfg.leaf_gate = 24;
     g_dev.cfg.lds_nodes = 384;
     g_dev.cfg.wide = 1;
     g_dev.device = device;

@@ -850,7 +850,7 @@ struct AoJob : RenderJob {
             rb_trace_config ao_cfg = dev().cfg;
             const uint64_t resident = (uint64_t)dev().num_cus * ao_cfg.waves_per_cu;
             if (ao_cfg.chunk < 128 && na >= resident * 128 * 2) ao_cfg.chunk = 128;
-            sets[0] = TraceSet{arays, ahits, (uint32_t)na, nullptr, RB_ANY | RB_MASKED, amask, ao_cfg};
+            sets[0] = TraceSet{arays, ahits, (uint32_t)na, nullptr, RB_ANY | kOcclusion | RB_MASKED, amask, ao_cfg};
             *nsets = 1;
         } else {
             hipLaunchKernelGGL(k_ao_resolve, dim3(grid_of(n)), dim3(256), 0, s, hits, ahits, n, p.ao_samples, Ls);
@@ -1012,7 +1012,7 @@ struct CameraJob : RenderJob {
                 if (p.path_counter)
                     hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, (const uint32_t*)P.cnt + qout,
                                        p.path_counter + b);
-                sets[0] = TraceSet{P.ra, P.ha, na, P.cnt + 2, RB_ANY, nullptr, dev().cfg};
+                sets[0] = TraceSet{P.ra, P.ha, na, P.cnt + 2, RB_ANY | kOcclusion, nullptr, dev().cfg};
                 sets[1] = TraceSet{P.rbc, P.hbc, 2 * n, P.cnt + 3, RB_CLOSEST, nullptr, dev().cfg};
                 *nsets = 2;
                 b++;

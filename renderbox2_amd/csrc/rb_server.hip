@@ -371,7 +371,8 @@ bool rb_server_s::round() {
             for (uint32_t k = 0; k < ns; k++)
                 round_sets.push_back(BatchSet{q->bvh, ts[k].rays, ts[k].hits, ts[k].n, ts[k].n_dev,
                                               (ts[k].flags & RB_ANY) != 0,
-                                              (ts[k].flags & RB_MASKED) ? ts[k].mask : nullptr});
+                                              (ts[k].flags & RB_MASKED) ? ts[k].mask : nullptr,
+                                              (ts[k].flags & kOcclusion) != 0});
         }
         if (st != RB_OK) {
             sl.job.reset();

@@ -530,21 +530,22 @@ __global__ __launch_bounds__(BLOCK) void k_trace(TraceArgs a) {
     trace_body<ANY, COUNT, S, BLOCK, false, SPEC>(a, nullptr, 0, nrays);
 }
 
-// Closest hit over the wide records (COUNT: inner = wide records visited).
-template <bool COUNT, int S, int BLOCK>
+// Closest hit, or occlusion (ANY), over the wide records (COUNT: inner =
+// wide records visited).
+template <bool ANY, bool COUNT, int S, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_trace_wide(TraceArgs a) {
     const uint32_t nrays = a.n_dev ? __builtin_nontemporal_load(a.n_dev) : a.n;
-    trace_body<false, COUNT, S, BLOCK, false, false, false, true>(a, nullptr, 0, nrays);
+    trace_body<ANY, COUNT, S, BLOCK, false, false, false, true>(a, nullptr, 0, nrays);
 }
 
-template <int S, int BLOCK>
+template <int S, int BLOCK, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_trace_pair(TraceArgs a) {
     const uint32_t n0 = __builtin_nontemporal_load(a.n_dev);
     const uint32_t n1 = __builtin_nontemporal_load(a.n_dev2);
-    trace_body<true, false, S, BLOCK, false, false, true>(a, nullptr, 0, n0 + n1, n0);
+    trace_body<true, false, S, BLOCK, false, false, true, WIDE>(a, nullptr, 0, n0 + n1, n0);
 }
 
-template <int S, int BLOCK, bool SPEC>
+template <int S, int BLOCK, bool SPEC, bool WIDE>
 __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a, BatchTable t) {
     __shared__ BatchDesc tab[kMaxBatch];
     __shared__ uint32_t s_total;
@@ -564,7 +565,7 @@ __global__ __launch_bounds__(BLOCK) void k_trace_batch(TraceArgs a, BatchTable t
         s_total = sum;
     }
     __syncthreads();
-    trace_body<false, false, S, BLOCK, true, SPEC>(a, tab, t.nb, s_total);
+    trace_body<false, false, S, BLOCK, true, SPEC, false, WIDE>(a, tab, t.nb, s_total);
 }
 
 // ---------------------------------------------------------------- launch --
@@ -587,30 +588,39 @@ static TraceKernel pick(const rb_trace_config& c) {
     return nullptr;
 }
 
-template <bool COUNT>
+template <bool ANY, bool COUNT>
 static TraceKernel pick_wide(const rb_trace_config& c) {
 #define RB_K(s, b) \
-    if (c.lds_stack == s && c.block == b) return k_trace_wide<COUNT, s, b>;
+    if (c.lds_stack == s && c.block == b) return k_trace_wide<ANY, COUNT, s, b>;
     RB_FOR_CONFIGS(RB_K)
 #undef RB_K
     return nullptr;
 }
 
+template <bool WIDE>
 static TraceKernel pick_pair(const rb_trace_config& c) {
 #define RB_K(s, b) \
-    if (c.lds_stack == s && c.block == b) return k_trace_pair<s, b>;
+    if (c.lds_stack == s && c.block == b) return k_trace_pair<s, b, WIDE>;
     RB_FOR_CONFIGS(RB_K)
 #undef RB_K
     return nullptr;
 }
 
-static BatchKernel pick_batch(const rb_trace_config& c) {
-#define RB_K(s, b)                             \
-    if (c.lds_stack == s && c.block == b)      \
-        return c.speculative ? k_trace_batch<s, b, true> : k_trace_batch<s, b, false>;
+static BatchKernel pick_batch(const rb_trace_config& c, bool wide) {
+#define RB_K(s, b)                                                                    \
+    if (c.lds_stack == s && c.block == b)                                             \
+        return c.speculative ? k_trace_batch<s, b, true, false>                       \
+                             : wide ? k_trace_batch<s, b, false, true> : k_trace_batch<s, b, false, false>;
     RB_FOR_CONFIGS(RB_K)
 #undef RB_K
     return nullptr;
+}
+
+// Whether a set may walk the wide records: closest hit, or an any-hit set
+// read only as hit / miss (kOcclusion), under a non-speculative config with
+// the wide walk on, over a BVH that has wide records.
+static bool wide_walk(const rb_trace_config& cfg, rb_bvh_t bvh, uint32_t flags) {
+    return cfg.wide && !cfg.speculative && bvh->wnodes && (!(flags & RB_ANY) || (flags & kOcclusion));
 }
 
 static uint32_t persistent_blocks(const rb_trace_config& cfg, uint64_t n) {
@@ -629,15 +639,17 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     if (n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "too many rays in one call (%u)", n);
     if ((flags & RB_MASKED) && !mask) return fail(RB_ERR_INVALID, "RB_MASKED without a mask");
     const bool any = (flags & RB_ANY) != 0;
-    // Closest hit walks the wide records (the same (id, t)); visit counts
-    // follow the binary walk unless RB_WIDE_COUNTS asks for the wide one.
+    // Closest hit (the same (id, t)) and occlusion walk the wide records;
+    // visit counts follow the binary walk unless RB_WIDE_COUNTS asks for the
+    // wide one.
     const bool wide_counts = (flags & RB_WIDE_COUNTS) != 0;
     if (wide_counts && (!counts || any || cfg.speculative))
         return fail(RB_ERR_INVALID, "RB_WIDE_COUNTS: visit counts of a closest-hit, non-speculative trace only");
     if (wide_counts && !bvh->wnodes)
         return fail(RB_ERR_UNSUPPORTED, "RB_WIDE_COUNTS: this BVH has no wide records (too deep for the stack)");
-    const bool wide = wide_counts || (cfg.wide && !any && !counts && !cfg.speculative && bvh->wnodes);
-    TraceKernel k = wide ? (counts ? pick_wide<true>(cfg) : pick_wide<false>(cfg))
+    const bool wide = wide_counts || (!counts && wide_walk(cfg, bvh, flags));
+    TraceKernel k = wide ? (counts ? pick_wide<false, true>(cfg) : any ? pick_wide<true, false>(cfg)
+                                                                       : pick_wide<false, false>(cfg))
                   : counts ? (any ? pick<true, true>(cfg) : pick<false, true>(cfg))
                            : (any ? pick<true, false>(cfg) : pick<false, false>(cfg));
     if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
@@ -685,19 +697,23 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
 rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, const uint32_t* n0_dev, rb_hit* hits0,
                             uint32_t flags0, const rb_ray* rays1, uint32_t cap1, const uint32_t* n1_dev, rb_hit* hits1,
                             hipStream_t stream, const rb_trace_config& cfg) {
-    TraceKernel k = cfg.speculative ? nullptr : pick_pair(cfg);
+    if (!bvh) return fail(RB_ERR_INVALID, "paired trace: NULL bvh");
+    // both sets walk the wide records when set 0 may (set 1 is closest hit)
+    const bool wide = wide_walk(cfg, bvh, flags0);
+    TraceKernel k = cfg.speculative ? nullptr : wide ? pick_pair<true>(cfg) : pick_pair<false>(cfg);
     // the speculative order, or more rays than one counter range takes: two launches
     if (!k || (uint64_t)cap0 + cap1 > 0x7FFFFFFFull) {
-        rb_status st = launch_trace(bvh, rays0, cap0, flags0 & RB_ANY, nullptr, hits0, nullptr, stream, cfg, n0_dev);
+        rb_status st = launch_trace(bvh, rays0, cap0, flags0 & (RB_ANY | kOcclusion), nullptr, hits0, nullptr, stream,
+                                    cfg, n0_dev);
         if (st != RB_OK) return st;
         return launch_trace(bvh, rays1, cap1, RB_CLOSEST, nullptr, hits1, nullptr, stream, cfg, n1_dev);
     }
     if (cap0 + cap1 == 0) return RB_OK;
-    if (!bvh || !rays0 || !rays1 || !hits0 || !hits1 || !n0_dev || !n1_dev)
+    if (!rays0 || !rays1 || !hits0 || !hits1 || !n0_dev || !n1_dev)
         return fail(RB_ERR_INVALID, "paired trace: NULL argument");
     TraceArgs a;
     std::memset(&a, 0, sizeof(a));
-    a.nodes = bvh->nodes;
+    a.nodes = wide ? bvh->wnodes : bvh->nodes;
     a.tris = bvh->tris;
     a.index = bvh->index;
     a.rays = reinterpret_cast<const float4*>(rays0);
@@ -712,7 +728,8 @@ rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, co
     a.chunk = cfg.chunk;
     a.threshold = cfg.fetch_threshold;
     a.leaf_gate = cfg.leaf_gate;
-    a.top_nodes = cfg.lds_nodes < bvh->n_nodes ? cfg.lds_nodes : bvh->n_nodes;
+    const uint32_t nrec = wide ? bvh->n_wnodes : bvh->n_nodes;
+    a.top_nodes = cfg.lds_nodes < nrec ? cfg.lds_nodes : nrec;
     const size_t lds = (size_t)cfg.lds_stack * cfg.block * 4 + (size_t)a.top_nodes * 64;
     if (lds > dev().lds_per_block)
         return fail(RB_ERR_INVALID, "trace config needs %zu bytes of LDS per block; the device allows %zu", lds,
@@ -728,21 +745,41 @@ rb_status launch_trace_pair(rb_bvh_t bvh, const rb_ray* rays0, uint32_t cap0, co
     return RB_OK;
 }
 
+static rb_status launch_trace_sets_pass(const BatchSet* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg,
+                                        bool wide);
+
+// The sets that may walk the wide records go through launches of their own
+// (pass 1), the rest through the binary kernel (pass 0).
 rb_status launch_trace_sets(const BatchSet* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg) {
-    BatchKernel k = pick_batch(cfg);
+    for (uint32_t i = 0; i < nb; i++) {
+        if (b[i].n == 0) continue;
+        if (!b[i].bvh || !b[i].rays || !b[i].hits) return fail(RB_ERR_INVALID, "batch %u: NULL bvh, rays or hits", i);
+        if (b[i].n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "batch %u: more than 2^31 - 1 rays", i);
+    }
+    for (int pass = 0; pass < 2; pass++) {
+        const rb_status st = launch_trace_sets_pass(b, nb, stream, cfg, pass == 1);
+        if (st != RB_OK) return st;
+    }
+    return RB_OK;
+}
+
+static rb_status launch_trace_sets_pass(const BatchSet* b, uint32_t nb, hipStream_t stream, const rb_trace_config& cfg,
+                                        bool wide) {
+    BatchKernel k = pick_batch(cfg, wide);
     if (!k) return fail(RB_ERR_INVALID, "unsupported trace config (lds_stack %u, block %u)", cfg.lds_stack, cfg.block);
+    auto in_pass = [&](const BatchSet& x) {
+        return wide_walk(cfg, x.bvh, (x.any ? RB_ANY : 0u) | (x.occlusion ? kOcclusion : 0u)) == wide;
+    };
     for (uint32_t i0 = 0; i0 < nb;) {
         BatchTable t;
         std::memset(&t, 0, sizeof(t));
         uint64_t total = 0;  // bound of the launch's rays (sizes the grid)
         uint32_t i = i0;
         for (; i < nb && t.nb < kMaxBatch; i++) {
-            if (b[i].n == 0) continue;
-            if (!b[i].bvh || !b[i].rays || !b[i].hits) return fail(RB_ERR_INVALID, "batch %u: NULL bvh, rays or hits", i);
-            if (b[i].n > 0x7FFFFFFFu) return fail(RB_ERR_INVALID, "batch %u: more than 2^31 - 1 rays", i);
+            if (b[i].n == 0 || !in_pass(b[i])) continue;
             if (total + b[i].n > 0x7FFFFFFFull) break;  // the next launch takes it
             BatchDesc& d = t.d[t.nb++];
-            d.nodes = b[i].bvh->nodes;
+            d.nodes = wide ? b[i].bvh->wnodes : b[i].bvh->nodes;
             d.tris = b[i].bvh->tris;
             d.index = b[i].bvh->index;
             d.rays = b[i].rays;
@@ -786,6 +823,7 @@ rb_status launch_trace_batch(const rb_batch_desc* b, uint32_t nb, hipStream_t st
         if (b[i].n == 0) continue;
         if (!b[i].bvh || !b[i].rays || !b[i].hits) return fail(RB_ERR_INVALID, "batch %u: NULL bvh, rays or hits", i);
         if (b[i].flags & RB_MASKED) return fail(RB_ERR_INVALID, "batch %u: masks are not supported in batches", i);
+        if (b[i].flags & ~(uint32_t)RB_ANY) return fail(RB_ERR_INVALID, "batch %u: unknown trace flags 0x%x", i, b[i].flags);
         // the rays of one table of kMaxBatch batches go through one launch
         total = (sets.size() % kMaxBatch == 0 ? 0 : total) + b[i].n;
         if (total > 0x7FFFFFFFull) return fail(RB_ERR_INVALID, "more than 2^31 rays in one batched call");
@@ -839,6 +877,7 @@ rb_status rb_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
                    void* stream) {
     rb_status st = ensure_device();
     if (st != RB_OK) return st;
+    if (flags & ~(uint32_t)(RB_ANY | RB_MASKED)) return fail(RB_ERR_INVALID, "unknown trace flags 0x%x", flags);
     return launch_trace(bvh, rays, n, flags, mask, hits, nullptr, stream ? (hipStream_t)stream : dev().stream,
                         dev().cfg);
 }
@@ -856,6 +895,7 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
     if (st != RB_OK) return st;
     if (!bvh || !rays || !hits || n == 0) return fail(RB_ERR_INVALID, "autotune needs a bvh and a non-empty ray batch");
     if (flags & RB_MASKED) return fail(RB_ERR_INVALID, "autotune takes unmasked batches");
+    if (flags & ~(uint32_t)RB_ANY) return fail(RB_ERR_INVALID, "unknown trace flags 0x%x", flags);
     if (reps == 0) reps = 3;
     hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
     hipEvent_t e0, e1;
@@ -882,7 +922,7 @@ rb_status rb_trace_autotune(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32
         const uint32_t blocks[] = {64, 128, 256, 512, 1024};
         const uint32_t stacks[] = {8, 12, 16, 20, 24};
         const uint32_t chunks[] = {64, 128, 256, 1024};
-        const uint32_t thresholds[] = {0, 8, 16, 24, 32, 40, 48};
+        const uint32_t thresholds[] = {0, 16, 32, 40, 48, 56, 60};
         const uint32_t waves[] = {8, 12, 16, 24, 32};
         const uint32_t gates[] = {0, 4, 8, 16, 24, 32};
         const uint32_t tops[] = {0, 64, 128, 192, 256, 384};
@@ -972,6 +1012,8 @@ rb_status rb_trace_counts(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t
     rb_status st = ensure_device();
     if (st != RB_OK) return st;
     if (!counts) return fail(RB_ERR_INVALID, "NULL counts");
+    if (flags & ~(uint32_t)(RB_ANY | RB_MASKED | RB_WIDE_COUNTS))
+        return fail(RB_ERR_INVALID, "unknown trace flags 0x%x", flags);
     return launch_trace(bvh, rays, n, flags & ~RB_MASKED, nullptr, hits, counts,
                         stream ? (hipStream_t)stream : dev().stream, dev().cfg);
 }
