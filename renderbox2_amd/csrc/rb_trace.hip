@@ -376,15 +376,28 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     }
                 RB_CS(0, 1) RB_CS(2, 3) RB_CS(0, 2) RB_CS(1, 3) RB_CS(1, 2)
 #undef RB_CS
-                const bool h1 = key[1] != __builtin_inff(), h2 = key[2] != __builtin_inff(),
-                           h3 = key[3] != __builtin_inff();
+                // the hit children are a prefix of the sorted four
+                const int np = (int)(key[1] != __builtin_inff()) + (int)(key[2] != __builtin_inff()) +
+                               (int)(key[3] != __builtin_inff());
                 if (key[0] == __builtin_inff()) {
                     RB_POP(nodeAddr);
                 } else {
-                    if (h3) RB_PUSH(lnk[3]);
-                    if (h2) RB_PUSH(lnk[2]);
-                    if (h1) RB_PUSH(lnk[1]);
                     nodeAddr = lnk[0];
+                    if (sp + 3 < S) {
+                        // entries sp+1..sp+np take lnk[np..1] (farthest
+                        // deepest), written as three straight-line stores:
+                        // those past sp+np lie above the new top, unread
+                        const int e1 = np == 3 ? lnk[3] : np == 2 ? lnk[2] : lnk[1];
+                        const int e2 = np == 3 ? lnk[2] : lnk[1];
+                        lstack[(sp + 1) * BLOCK + tid] = e1;
+                        lstack[(sp + 2) * BLOCK + tid] = e2;
+                        lstack[(sp + 3) * BLOCK + tid] = lnk[1];
+                        sp += np;
+                    } else {
+                        if (np == 3) RB_PUSH(lnk[3]);
+                        if (np >= 2) RB_PUSH(lnk[2]);
+                        if (np >= 1) RB_PUSH(lnk[1]);
+                    }
                 }
                 if (nodeAddr < 0 && leafAddr >= 0) {
                     leafAddr = nodeAddr;
