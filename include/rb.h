@@ -295,6 +295,16 @@ typedef struct rb_bvh_dev {
     const uint32_t* index;  /* uint32[n_refs]: original triangle id per reference */
     uint32_t n_nodes, n_refs;
     uint32_t* errors;       /* device word for RB_TRACE_ERR_* flags (rb_trace_errors); may be NULL */
+    const void* wnodes;     /* float4[4 * n_wnodes]: wide (4-child) records, which rb_trace_one
+                               walks for closest hits (the same (id, t)); NULL: the binary
+                               records only.  Set when rb_trace_config.wide is on */
+    uint32_t n_wnodes;
+    uint32_t n_tris;        /* triangles of the scene the BVH was uploaded with (0: none) */
+    const void* tri_v;      /* float4[3 * n_tris]: v0.xyz|material id bits, v1.xyz|flags bits
+                               (bit 0: the triangle has vertex normals), v2.xyz|0; NULL
+                               without a scene.  Read by rb_fill_one */
+    const void* tri_n;      /* float4[3 * n_tris]: vertex normals n0, n1, n2 of the triangles
+                               that have them; NULL when no triangle has any */
 } rb_bvh_dev;
 rb_status rb_bvh_device_view(rb_bvh_t b, rb_bvh_dev* out);
 
@@ -340,6 +350,15 @@ rb_status rb_trace_fill_noalloc(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, ui
  * rather than the persistent kernel.  Same results as rb_trace, bit for
  * bit; kept for in-kernel callers' reference and comparison. */
 rb_status rb_trace_simple(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, rb_hit* hits, void* stream);
+
+/* One ray per thread through rb_trace_one_isect(), the reference's
+ * trace_ray with its Intersection argument (cuda_tracer_functions.cu:236-276):
+ * each thread fills its ray's intersection record and writes the fields
+ * `out` names.  Needs a bvh uploaded with its scene.  Same records as
+ * rb_trace_fill, bit for bit (closest hit; with RB_ANY only `intersected`
+ * and `t` are written, as the reference fills nothing else for any hit). */
+rb_status rb_trace_simple_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags,
+                               const rb_isect_soa* out, void* stream);
 
 /* Batched trace (no reference counterpart: the reference issues one
  * synchronous launch per call, cudatracer.cu:134-183).  Ray batches of

@@ -141,6 +141,7 @@ _SIGS = {
     "rb_trace_batch": (C.c_int, [P, U32, P]),
     "rb_bvh_device_view": (C.c_int, [P, P]),
     "rb_trace_simple": (C.c_int, [P, P, U32, U32, P, P]),
+    "rb_trace_simple_fill": (C.c_int, [P, P, U32, U32, P, P]),
     "rb_render_params_default": (None, [P]),
     "rb_render_ao": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
     "rb_render_pt": (C.c_int, [P, P, U32, U32, U32, U32, P, P, P]),
@@ -191,7 +192,8 @@ class rb_server_stats(C.Structure):
 
 
 class rb_bvh_dev(C.Structure):
-    _fields_ = [("nodes", P), ("tris", P), ("index", P), ("n_nodes", U32), ("n_refs", U32), ("errors", P)]
+    _fields_ = [("nodes", P), ("tris", P), ("index", P), ("n_nodes", U32), ("n_refs", U32), ("errors", P),
+                ("wnodes", P), ("n_wnodes", U32), ("n_tris", U32), ("tri_v", P), ("tri_n", P)]
 
 
 class rb_batch_desc(C.Structure):

@@ -97,6 +97,11 @@ inline RngArg rng_arg(uint64_t seed, uint32_t tag) {
 }
 rb_status hip_fail(hipError_t e, const char* what);
 
+// Device view of a resident BVH for rb_trace_one (rb_trace_one.hip): its
+// records, the wide ones when the traversal config walks them, and the
+// scene rows it was uploaded with.
+rb_bvh_dev device_view(rb_bvh_t bvh);
+
 // Internal trace flag, beside RB_ANY: an any-hit set whose consumer reads
 // only hit / miss (hit.x < 0).  Such a set may walk the wide records, which
 // find a hit exactly when the binary walk does but not necessarily the same

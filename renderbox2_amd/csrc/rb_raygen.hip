@@ -9,6 +9,7 @@
 #include "rb_device.h"
 #include "rb_sample.h"
 #include "rb_shade.h"
+#include "../../include/rb_trace_device.h"
 
 namespace rb {
 
@@ -88,47 +89,27 @@ __global__ void k_raygen_diffuse(const float4* __restrict__ prim, const int2* __
     }
 }
 
+// The intersection records of a trace's hits: rb_fill_one of the public
+// device header (the same arithmetic as a kernel that traces and fills in
+// its own code), stored into the caller's arrays.
 __global__ void k_fill(const float4* __restrict__ rays, const int2* __restrict__ hits, uint32_t n, SceneDev S,
                        rb_isect_soa o) {
+    rb_bvh_dev b;
+    memset(&b, 0, sizeof(b));  // only the scene rows are read
+    b.tri_v = S.tri_v;
+    b.tri_n = S.tri_n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const int2 h = hits[i];
-        const bool hit = h.x >= 0;
-        if (o.intersected) o.intersected[i] = hit ? 1 : 0;
-        if (o.t) o.t[i] = __int_as_float(h.y);
-        if (!hit) continue;
+        const int2 h2 = hits[i];
         const float4 ro = rays[2 * (size_t)i], rd = rays[2 * (size_t)i + 1];
-        const float t = __int_as_float(h.y);
-        V3 v0, v1, v2;
-        uint32_t mid, tf;
-        tri_fetch(S, (uint32_t)h.x, &v0, &v1, &v2, &mid, &tf);
-        const V3 d = v3(rd.x, rd.y, rd.z);
-        const V3 p = vadd(v3(ro.x, ro.y, ro.z), vmul(d, t));
-        const V3 ng = tri_normal(v0, v1, v2);
-        V3 ns;
-        interp_normal(S, (uint32_t)h.x, tf, p, v0, v1, v2, ng, &ns);
-        if (o.primitive_id) o.primitive_id[i] = (uint32_t)h.x;
-        if (o.material_id) o.material_id[i] = mid;
-        for (int k = 0; k < 3; k++) {
-            const float pv = k == 0 ? p.x : (k == 1 ? p.y : p.z);
-            const float nv = k == 0 ? ng.x : (k == 1 ? ng.y : ng.z);
-            const float sv = k == 0 ? ns.x : (k == 1 ? ns.y : ns.z);
-            const float dv = k == 0 ? d.x : (k == 1 ? d.y : d.z);
-            if (o.position[k]) o.position[k][i] = pv;
-            if (o.geometric_normal[k]) o.geometric_normal[k][i] = nv;
-            if (o.shading_normal[k]) o.shading_normal[k][i] = sv;
-            if (o.incoming_direction[k]) o.incoming_direction[k][i] = -dv;
-        }
-        if (o.front) o.front[i] = vdot(ng, d) < 0.0f ? 1 : 0;
-        if (o.epsilon) o.epsilon[i] = 1.0e-3f * t;
-        if (o.uv) {
-            // barycentrics of p: weights of v0 and v1 (the Woop u, v
-            // convention), of the hit triangle (the reference stores those of
-            // the last triangle tested)
-            float u, v;
-            bary_uv(p, v0, v1, v2, &u, &v);
-            o.uv[2 * (size_t)i] = u;
-            o.uv[2 * (size_t)i + 1] = v;
-        }
+        rb_ray r;
+        r.ox = ro.x; r.oy = ro.y; r.oz = ro.z; r.tmin = ro.w;
+        r.dx = rd.x; r.dy = rd.y; r.dz = rd.z; r.tmax = rd.w;
+        rb_hit h;
+        h.id = h2.x;
+        h.t = __int_as_float(h2.y);
+        rb_isect_one rec;
+        rb_fill_one(b, r, h, &rec);
+        rb_isect_store(o, i, rec);
     }
 }
 

@@ -978,13 +978,7 @@ struct CameraJob : RenderJob {
             if (p.ray_counter) hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + 0, p.ray_counter);
             if (p.light_path_counter)
                 hipLaunchKernelGGL(k_add_count, dim3(1), dim3(64), 0, s, LP.cnt + 0, p.light_path_counter);
-            rb_bvh_dev bv;
-            bv.nodes = bvh->nodes;
-            bv.tris = bvh->tris;
-            bv.index = bvh->index;
-            bv.n_nodes = bvh->n_nodes;
-            bv.n_refs = bvh->n_refs;
-            bv.errors = dev().errors;
+            const rb_bvh_dev bv = device_view(bvh);
             hipLaunchKernelGGL(k_light_walk, dim3(grid_of(NL)), dim3(256), 0, s, S, rng_arg(p.seed, RNG_LIGHT), LP, B.C,
                                bv, p.ray_counter, p.light_path_counter);
             phase = 2;

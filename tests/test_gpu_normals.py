@@ -31,8 +31,10 @@ def _normals(orc, s):
     return orc.tri_normals(n, nidx, s.get_num_triangles())
 
 
-def _fill(gpu, tr, rays, hits_buffer=True):
-    """rb_trace_fill into fresh buffers (pre-filled with 0xAB): dict of the fields."""
+def _fill(gpu, tr, rays, hits_buffer=True, simple=False, flags=L.RB_CLOSEST):
+    """rb_trace_fill (or, simple=True, rb_trace_simple_fill: one thread per ray
+    through the device functions rb_trace_one_isect / rb_fill_one) into fresh
+    buffers (pre-filled with 0xAB): dict of the fields."""
     n = len(rays)
     d_rays = gpu.DeviceBuffer.from_array(rays)
     d_hits = gpu.DeviceBuffer(n * 8) if hits_buffer else None
@@ -56,8 +58,11 @@ def _fill(gpu, tr, rays, hits_buffer=True):
     soa.intersected = buf("intersected", n)
     soa.epsilon = buf("epsilon", n * 4)
     soa.t = buf("t", n * 4)
-    L.check(L.lib().rb_trace_fill(tr.handle, d_rays.ptr, n, L.RB_CLOSEST, None, d_hits.ptr if d_hits else None,
-                                  C.byref(soa), None))
+    if simple:
+        L.check(L.lib().rb_trace_simple_fill(tr.handle, d_rays.ptr, n, flags, C.byref(soa), None))
+    else:
+        L.check(L.lib().rb_trace_fill(tr.handle, d_rays.ptr, n, flags, None, d_hits.ptr if d_hits else None,
+                                      C.byref(soa), None))
     L.check(L.lib().rb_stream_synchronize(None))
     out = {}
     for f in FIELDS:
@@ -69,9 +74,9 @@ def _fill(gpu, tr, rays, hits_buffer=True):
     return out
 
 
-def _check_fill(gpu, orc, s, b, rays, hits_buffer=True):
+def _check_fill(gpu, orc, s, b, rays, hits_buffer=True, simple=False):
     tr = gpu.CudaTracer(b, s)
-    got = _fill(gpu, tr, rays, hits_buffer)
+    got = _fill(gpu, tr, rays, hits_buffer, simple)
     v, idx, tm, _ = s.geometry()
     ids, ts = orc.trace_compact2(*_c2(b), rays)
     ref = orc.fill(rays, ids, ts, v, idx, tm, _normals(orc, s))
@@ -91,27 +96,68 @@ def _check_fill(gpu, orc, s, b, rays, hits_buffer=True):
     return got, ref, h
 
 
-def test_trace_fill_every_field_equals_the_oracle_flat(gpu, orc):
+@pytest.mark.parametrize("simple", [False, True])
+def test_trace_fill_every_field_equals_the_oracle_flat(gpu, orc, simple):
     s = gpu.scenes.ao_room(seed=3)
     b = gpu.scenes.build(s, max_leaf=4)
     rays = random_rays(30000, seed=91, box=4.0, tmax=3.0)
-    got, ref, h = _check_fill(gpu, orc, s, b, rays)
+    got, ref, h = _check_fill(gpu, orc, s, b, rays, simple=simple)
     # flat scene: the shading normal is the geometric one
     assert np.array_equal(got["shading_normal"][h], got["geometric_normal"][h])
     # and without a caller hits buffer (stream-ordered scratch) the same
-    _check_fill(gpu, orc, s, b, rays, hits_buffer=False)
+    if not simple:
+        _check_fill(gpu, orc, s, b, rays, hits_buffer=False)
 
 
-def test_trace_fill_every_field_equals_the_oracle_smooth(gpu, orc):
+@pytest.mark.parametrize("simple", [False, True])
+def test_trace_fill_every_field_equals_the_oracle_smooth(gpu, orc, simple):
     s = gpu.scenes.ao_room(seed=3, smooth=True)
     b = gpu.scenes.build(s, max_leaf=4)
     rays = random_rays(30000, seed=92, box=4.0, tmax=3.0)
-    got, ref, h = _check_fill(gpu, orc, s, b, rays)
+    got, ref, h = _check_fill(gpu, orc, s, b, rays, simple=simple)
     sphere = h & (got["primitive_id"] >= 2048)  # the room's 2,048 triangles come first
     assert sphere.sum() > 500
     ns, ng = got["shading_normal"][sphere], got["geometric_normal"][sphere]
     assert np.allclose(np.linalg.norm(ns, axis=1), 1.0, atol=1e-6)
     assert np.median(np.linalg.norm(ns - ng, axis=1)) > 1e-3
+
+
+def test_simple_fill_any_hit_writes_only_intersected_and_t(gpu, orc):
+    # trace_ray fills the Intersection only for closest hits
+    # (cuda_tracer_functions.cu:239-276): any hit writes intersected and t
+    s = gpu.scenes.ao_room(seed=3)
+    b = gpu.scenes.build(s, max_leaf=4)
+    rays = random_rays(20000, seed=93, box=4.0, tmax=3.0)
+    tr = gpu.CudaTracer(b, s)
+    got = _fill(gpu, tr, rays, simple=True, flags=L.RB_ANY)
+    ids, ts = orc.trace_compact2(*_c2(b), rays, any_hit=True)
+    h = ids >= 0
+    assert 0.1 < h.mean() < 0.99
+    assert np.array_equal(got["intersected"], h.astype(np.uint8))
+    assert np.array_equal(got["t"].view(np.uint32), ts.view(np.uint32))
+    for f in FIELDS + ["uv", "epsilon"]:
+        assert np.all(got[f].view(np.uint32) == 0xABABABAB), f
+    assert np.all(got["primitive_id"] == 0xABABABAB)
+    # a BVH uploaded without its scene has no rows to fill from
+    bare = gpu.CudaTracer(b)
+    soa = L.rb_isect_soa()
+    d = gpu.DeviceBuffer.from_array(rays[:64])
+    assert L.lib().rb_trace_simple_fill(bare.handle, d.ptr, 64, L.RB_CLOSEST, C.byref(soa), None) == L.RB_ERR_INVALID
+
+
+def test_device_view_holds_the_wide_records_and_scene_rows(gpu):
+    s = gpu.scenes.ao_room(seed=3)
+    b = gpu.scenes.build(s, max_leaf=4)
+    v = gpu.CudaTracer(b, s).device_view()
+    assert v.wnodes and 0 < v.n_wnodes < v.n_nodes
+    assert v.tri_v and v.n_tris == s.get_num_triangles() and not v.tri_n  # a flat scene has no normals
+    saved = gpu.get_trace_config()
+    try:
+        gpu.set_trace_config(wide=0)
+        v0 = gpu.CudaTracer(b, s).device_view()
+        assert not v0.wnodes and v0.n_wnodes == 0
+    finally:
+        gpu.set_trace_config(**saved)
 
 
 @pytest.fixture(scope="module")

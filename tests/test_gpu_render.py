@@ -33,11 +33,18 @@ def pt_setup(gpu):
     return s, b, gpu.CudaTracer(b, s)
 
 
-def test_ao_image_bit_exact(gpu, orc, ao_setup):
+@pytest.mark.parametrize("wide", [1, 0])
+def test_ao_image_bit_exact(gpu, orc, ao_setup, wide):
+    # wide=1: the AO rays (occlusion only) walk the wide records
     s, b, tr = ao_setup
     cam = gpu.scenes.ao_camera(128, 128)
-    r = gpu.SamplerRenderer(tr, cam, "ao", spp=2, seed=0, ao_samples=8, ao_radius=1.0)
-    img = r.render()
+    saved = gpu.get_trace_config()
+    try:
+        gpu.set_trace_config(wide=wide)
+        r = gpu.SamplerRenderer(tr, cam, "ao", spp=2, seed=0, ao_samples=8, ao_radius=1.0)
+        img = r.render()
+    finally:
+        gpu.set_trace_config(**saved)
     v, idx, _, _ = s.geometry()
     ref = orc.render_ao(_c2(b), v, idx, _cam_tuple(cam), (0, 0, 128, 128), 2, 0, 8, 1.0)
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
@@ -58,11 +65,12 @@ def test_pt_image_bit_exact(gpu, orc, pt_setup):
 
 @pytest.mark.parametrize("kind", ["pt", "bdpt"])
 def test_launch_shapes_give_the_same_image(gpu, pt_setup, kind):
-    # the bounce traces run as one paired launch (any-hit shadow queue +
-    # closest-hit queue) with the top of the tree in LDS; other block sizes,
-    # no LDS records and the speculative order (which falls back to two
-    # launches per bounce) give the same image -- bit for bit except the
-    # speculative order, whose rare boundary cases may move a sample
+    # the bounce traces run as one paired launch (occlusion shadow queue +
+    # closest-hit queue) over the wide records with the top of the tree in
+    # LDS; other block sizes, no LDS records, the binary records (wide=0)
+    # and the speculative order (which falls back to two launches per
+    # bounce) give the same image -- bit for bit except the speculative
+    # order, whose rare boundary cases may move a sample
     s, b, tr = pt_setup
     cam = gpu.scenes.pt_camera(48, 48)
     params = dict(spp=2, seed=9, max_depth=6, light_paths=2048, light_max_depth=3, connections=3) \
@@ -70,7 +78,8 @@ def test_launch_shapes_give_the_same_image(gpu, pt_setup, kind):
     saved = gpu.get_trace_config()
     try:
         ref = gpu.SamplerRenderer(tr, cam, kind, **params).render()
-        for cfg in (dict(block=256, lds_stack=16, lds_nodes=0), dict(block=1024, lds_stack=8, lds_nodes=512)):
+        for cfg in (dict(block=256, lds_stack=16, lds_nodes=0), dict(block=1024, lds_stack=8, lds_nodes=512),
+                    dict(saved, wide=0)):
             gpu.set_trace_config(**cfg)
             img = gpu.SamplerRenderer(tr, cam, kind, **params).render()
             assert np.array_equal(img.view(np.uint32), ref.view(np.uint32)), cfg

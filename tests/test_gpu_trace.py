@@ -140,6 +140,21 @@ def test_wide_walk_counts(gpu, orc, small):
         tr.trace_counts(rays[:64], any_hit=True, wide=True)
 
 
+def test_unknown_trace_flags_are_refused(gpu, small):
+    # bits beyond RB_ANY / RB_MASKED (and RB_WIDE_COUNTS for the counts
+    # call) are refused before any work, the library's internal ones included
+    from renderbox2_amd import _lib as L
+    v, idx, b, tr, c2 = small
+    rays = gpu.DeviceBuffer.from_array(random_rays(64, seed=3))
+    hits = gpu.DeviceBuffer(64 * 8)
+    counts = gpu.DeviceBuffer(64 * 12)
+    for bit in (L.RB_WIDE_COUNTS, 1 << 16, 1 << 31):
+        assert L.lib().rb_trace(tr.handle, rays.ptr, 64, bit, None, hits.ptr, None) == L.RB_ERR_INVALID
+    assert L.lib().rb_trace_counts(tr.handle, rays.ptr, 64, 1 << 16, hits.ptr, counts.ptr, None) == L.RB_ERR_INVALID
+    assert L.lib().rb_trace(tr.handle, rays.ptr, 64, L.RB_ANY, None, hits.ptr, None) == L.RB_OK
+    assert len(hits.download(np.int32, 128)) == 128  # waits for the trace
+
+
 def test_trace_room_camera_and_diffuse_rays(gpu, orc, room):
     s, b, tr, c2 = room
     cam = gpu.scenes.ao_camera(256, 256)
