@@ -253,7 +253,11 @@ rb_status rb_bvh_host_release(rb_host_bvh_t b);
  * with the reference's Compact2 (uint4 rows / uint4 Woop rows with
  * terminators / uint32 index); MI355X is this library's traversal layout.
  * Call with NULL buffers to get the sizes in bytes. */
-enum { RB_LAYOUT_REF_COMPACT2 = 1, RB_LAYOUT_MI355X = 2 };
+enum { RB_LAYOUT_REF_COMPACT2 = 1, RB_LAYOUT_MI355X = 2,
+       RB_LAYOUT_MI355X_WIDE = 3 /* no counterpart: the MI355X triangles and index with the
+                                    wide (4-child, 64-byte) records the upload builds as nodes;
+                                    RB_ERR_UNSUPPORTED where the upload would keep the binary
+                                    walk only */ };
 rb_status rb_bvh_export(rb_host_bvh_t b, uint32_t layout, void* nodes, size_t* nodes_bytes,
                         void* tris, size_t* tris_bytes, void* index, size_t* index_bytes);
 

@@ -18,7 +18,7 @@ SZ = C.c_size_t
 
 RB_OK, RB_ERR_INVALID, RB_ERR_OOM, RB_ERR_DEVICE, RB_ERR_BUSY, RB_ERR_UNSUPPORTED, RB_ERR_NOT_READY = range(7)
 RB_CLOSEST, RB_ANY, RB_MASKED, RB_WIDE_COUNTS = 0, 1, 2, 4
-RB_LAYOUT_REF_COMPACT2, RB_LAYOUT_MI355X = 1, 2
+RB_LAYOUT_REF_COMPACT2, RB_LAYOUT_MI355X, RB_LAYOUT_MI355X_WIDE = 1, 2, 3
 (RB_MAT_LAMBERT, RB_MAT_MIRROR, RB_MAT_EMITTER, RB_MAT_GLASS, RB_MAT_ORENNAYAR, RB_MAT_MICROFACET,
  RB_MAT_LAYERED) = range(7)
 RB_FRESNEL_NONE, RB_FRESNEL_CONDUCTOR, RB_FRESNEL_DIELECTRIC = 0, 1, 2

@@ -22,7 +22,7 @@ constexpr uint32_t kErrorWords = 64;
 
 // Traversal stack bound: the builder stops at depth 64 and leaves of more than
 // 31 references are split by the exporter, so 96 entries never overflow.
-constexpr int kMaxStack = 96;
+constexpr int kMaxStack = (int)kTraceStackEntries;
 constexpr int kSentinel = 0x76543210;
 
 struct DeviceState {

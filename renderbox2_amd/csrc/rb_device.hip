@@ -341,9 +341,17 @@ rb_status rb_bvh_upload(rb_host_bvh_t hb, rb_scene_t scene, rb_bvh_t* out) {
         if (depth == UINT32_MAX || depth + 1 >= (uint32_t)kMaxStack)
             return fail(RB_ERR_UNSUPPORTED, "BVH node layout is %s (the traversal stack holds %d entries)",
                         depth == UINT32_MAX ? "not a tree" : "too deep", kMaxStack);
-        // a wide record pushes up to three children and the wide tree is half
-        // as deep: wide walks only where that fits the same stack
-        if (3 * ((depth + 2) / 2) + 1 < (uint32_t)kMaxStack && (st = export_wide(L.nodes, wide)) != RB_OK) return st;
+        // wide records only where their walk fits the same stack and their
+        // 8-bit steps can hold the boxes (else the binary walk alone)
+        if (wide_fits_stack(depth)) {
+            st = export_wide(L.nodes, wide);
+            if (st == RB_ERR_UNSUPPORTED) {
+                wide.clear();
+                set_error("");
+            } else if (st != RB_OK) {
+                return st;
+            }
+        }
     } catch (...) {
         return fail(RB_ERR_OOM, "layout export failed (host allocation)");
     }

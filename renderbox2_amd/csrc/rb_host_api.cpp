@@ -594,10 +594,17 @@ rb_status rb_bvh_export(rb_host_bvh_t b, uint32_t layout, void* nodes, size_t* n
             if ((st = copy_out(w.data(), w.size() * 4, tris, tb)) != RB_OK) return st;
             return copy_out(x.data(), x.size() * 4, index, ib);
         }
-        if (layout == RB_LAYOUT_MI355X) {
+        if (layout == RB_LAYOUT_MI355X || layout == RB_LAYOUT_MI355X_WIDE) {
             Mi355Layout L;
-            export_mi355x(*b->b, L);
-            rb_status st;
+            rb_status st = export_mi355x(*b->b, L);
+            if (st != RB_OK) return st;
+            if (layout == RB_LAYOUT_MI355X_WIDE) {
+                if (!wide_fits_stack(layout_depth(L.nodes)))
+                    return fail(RB_ERR_UNSUPPORTED, "the tree is too deep for the wide walk's stack");
+                std::vector<uint32_t> wide;
+                if ((st = export_wide(L.nodes, wide)) != RB_OK) return st;
+                L.nodes.swap(wide);
+            }
             if ((st = copy_out(L.nodes.data(), L.nodes.size() * 4, nodes, nb)) != RB_OK) return st;
             if ((st = copy_out(L.tris.data(), L.tris.size() * 4, tris, tb)) != RB_OK) return st;
             return copy_out(L.index.data(), L.index.size() * 4, index, ib);

@@ -182,6 +182,13 @@ rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& w
 // traversal holds at most depth + 1 stack entries (the sentinel, then one
 // postponed child per inner node on the current path).
 uint32_t layout_depth(const std::vector<uint32_t>& nodes);
+// Entries of a trace's stack (the kernels' and rb_trace_one's).
+constexpr uint32_t kTraceStackEntries = 96;
+// Whether the wide walk's stack fits a tree of this binary depth: a wide
+// record pushes up to three children and the wide tree is half as deep.
+inline bool wide_fits_stack(uint32_t depth) {
+    return depth != UINT32_MAX && 3 * ((depth + 2) / 2) + 1 < kTraceStackEntries;
+}
 rb_status export_compact2(const HostBvh& b, std::vector<uint32_t>& nodes, std::vector<uint32_t>& woop,
                           std::vector<uint32_t>& index);
 
