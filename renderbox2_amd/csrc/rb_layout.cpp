@@ -269,7 +269,9 @@ rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
 // are monotone in the plane, so a ray passes a decoded box whenever it
 // passes the exact one: a ray tests every leaf the binary walk would at the
 // same hitT, and the closest hit (ties to the lowest triangle id) is the
-// same.  Empty leaves (no triangles) are left out.
+// same.  Empty leaves (no triangles) are left out; an inverted pair of
+// planes is held as the box between them, which is what the slab test of
+// the binary walk sees.
 namespace {
 
 float decode_plane(uint32_t q, float s, float p) { return std::fma((float)q, s, p); }
@@ -333,8 +335,14 @@ rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& w
             const uint32_t* b = rec + 4 * c;
             const uint32_t w[6] = {b[0], b[1], b[2], b[3], rec[8 + 2 * c], rec[9 + 2 * c]};
             for (int r = 0; r < 6; r++) std::memcpy(&box[k][r], &w[r], 4);
-            for (int a = 0; a < 3; a++)
-                if (!(box[k][2 * a] <= box[k][2 * a + 1])) bad = true;  // inverted or NaN
+            for (int a = 0; a < 3; a++) {
+                float& lo = box[k][2 * a];
+                float& hi = box[k][2 * a + 1];
+                if (lo != lo || hi != hi) bad = true;  // NaN
+                // the slab test takes the min and max of a pair of planes, so
+                // an inverted pair (lo > hi) is the box between them: hold that
+                if (lo > hi) std::swap(lo, hi);
+            }
             if (l >= 0) {
                 if ((uint32_t)l >= n) {
                     bad = true;
@@ -361,7 +369,7 @@ rb_status export_wide(const std::vector<uint32_t>& bin, std::vector<uint32_t>& w
                 add(d, c);
             }
         }
-        if (bad) return fail(RB_ERR_UNSUPPORTED, "a BVH box is inverted, NaN or links out of the array");
+        if (bad) return fail(RB_ERR_UNSUPPORTED, "a BVH box is NaN or a link leaves the array");
         const size_t base = wide.size();
         wide.resize(base + kWideWords, 0);
         uint32_t* w = &wide[base];

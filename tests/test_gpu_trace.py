@@ -220,6 +220,58 @@ def test_masked_degenerate_and_empty_batches(gpu, orc, small):
     assert one_id[0] == oid[5] and one_t[0] == ot[5]
 
 
+def _flat_quads(n, seed):
+    # axis-aligned quads: every box is flat along one axis (zero extent)
+    rng = np.random.default_rng(seed)
+    v, idx = [], []
+    for i in range(n):
+        a = i % 3
+        c = rng.uniform(-5, 5, 3)
+        u, w = [k for k in range(3) if k != a]
+        corners = []
+        for du, dw in ((0, 0), (1, 0), (1, 1), (0, 1)):
+            q = c.copy()
+            q[u] += du * 0.5
+            q[w] += dw * 0.5
+            corners.append(q)
+        base = len(v)
+        v += corners
+        idx += [[base, base + 1, base + 2], [base, base + 2, base + 3]]
+    return np.array(v, np.float32), np.array(idx, np.uint32)
+
+
+@pytest.mark.parametrize("case", ["one_triangle", "flat_quads", "tiny_coords", "large_coords", "huge_coords"])
+def test_wide_walk_edge_scenes_equal_the_oracle(gpu, orc, case):
+    """Scenes at the edges of the 8-bit record encoding, closest hit with
+    the wide records against the oracle: a single triangle (a root leaf
+    beside an empty one), zero-thickness boxes, coordinates scaled by 1e-9
+    and 1e9 (step exponents near -40 and +20), and by 1e30, where the
+    builder's SAH overflows and leaves inverted boxes (held as the box
+    between their planes; the Woop rows overflow too, so every ray misses
+    in both walks)."""
+    if case == "one_triangle":
+        v = np.array([[-1, -1, 0], [1, -1, 0], [0, 1, 0]], np.float32)
+        idx = np.array([[0, 1, 2]], np.uint32)
+        rays = random_rays(4096, seed=11, box=2.0)
+    elif case == "flat_quads":
+        v, idx = _flat_quads(1500, 12)
+        rays = random_rays(20000, seed=12)
+    else:
+        sc = np.float32({"tiny_coords": 1e-9, "large_coords": 1e9, "huge_coords": 1e30}[case])
+        v, idx = random_triangles(2000, seed=13)
+        v = (v * sc).astype(np.float32)
+        rays = random_rays(20000, seed=14)
+        rays[:, 0:3] *= sc
+        rays[:, 7] = 3.0e38
+    b = gpu.BVH((v, idx))
+    tr = gpu.CudaTracer(b)
+    assert tr.device_view().wnodes  # the wide walk is what runs
+    ids, ts = tr.trace(rays)
+    oid, ot = orc.trace_compact2(*_compact2(gpu, b), rays)
+    assert (oid >= 0).sum() > (-1 if case == "huge_coords" else 20)
+    assert np.array_equal(ids, oid) and np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
+
+
 def test_trace_is_deterministic_and_reentrant(gpu, small):
     v, idx, b, tr, c2 = small
     rays = random_rays(100000, seed=80)
