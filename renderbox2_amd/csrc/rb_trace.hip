@@ -178,8 +178,9 @@ __global__ __launch_bounds__(256) void k_rcp_check(unsigned long long* out) {
 
 // PAIR: rays [0, n0) are the first set (a.rays / a.hits; any hit when
 // a.any0), rays [n0, nrays) the closest-hit set (a.rays2 / a.hits2).
-// WIDE: a.nodes holds wide (4-child) records (export_wide); closest hit
-// only, where the visit order does not change the (id, t) a ray reports.
+// WIDE: a.nodes holds wide (4-child) records (export_wide); for closest
+// hit, whose (id, t) does not depend on the visit order, and for occlusion
+// sets (any hit read only as hit / miss).
 template <bool ANY_T, bool COUNT, int S, int BLOCK, bool MULTI, bool SPEC, bool PAIR = false, bool WIDE = false>
 __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* tab, uint32_t nb, uint32_t nrays,
                                            uint32_t n0 = 0) {
@@ -377,26 +378,26 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
                 RB_CS(0, 1) RB_CS(2, 3) RB_CS(0, 2) RB_CS(1, 3) RB_CS(1, 2)
 #undef RB_CS
                 // the hit children are a prefix of the sorted four
-                const int np = (int)(key[1] != __builtin_inff()) + (int)(key[2] != __builtin_inff()) +
-                               (int)(key[3] != __builtin_inff());
+                const int npush = (int)(key[1] != __builtin_inff()) + (int)(key[2] != __builtin_inff()) +
+                                  (int)(key[3] != __builtin_inff());
                 if (key[0] == __builtin_inff()) {
                     RB_POP(nodeAddr);
                 } else {
                     nodeAddr = lnk[0];
                     if (sp + 3 < S) {
-                        // entries sp+1..sp+np take lnk[np..1] (farthest
-                        // deepest), written as three straight-line stores:
-                        // those past sp+np lie above the new top, unread
-                        const int e1 = np == 3 ? lnk[3] : np == 2 ? lnk[2] : lnk[1];
-                        const int e2 = np == 3 ? lnk[2] : lnk[1];
+                        // entries sp+1..sp+npush take lnk[npush..1]
+                        // (farthest deepest), written as three straight-line
+                        // stores: those past sp+npush lie above the new top
+                        const int e1 = npush == 3 ? lnk[3] : npush == 2 ? lnk[2] : lnk[1];
+                        const int e2 = npush == 3 ? lnk[2] : lnk[1];
                         lstack[(sp + 1) * BLOCK + tid] = e1;
                         lstack[(sp + 2) * BLOCK + tid] = e2;
                         lstack[(sp + 3) * BLOCK + tid] = lnk[1];
-                        sp += np;
+                        sp += npush;
                     } else {
-                        if (np == 3) RB_PUSH(lnk[3]);
-                        if (np >= 2) RB_PUSH(lnk[2]);
-                        if (np >= 1) RB_PUSH(lnk[1]);
+                        if (npush == 3) RB_PUSH(lnk[3]);
+                        if (npush >= 2) RB_PUSH(lnk[2]);
+                        if (npush >= 1) RB_PUSH(lnk[1]);
                     }
                 }
                 if (nodeAddr < 0 && leafAddr >= 0) {
@@ -659,7 +660,7 @@ rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t fl
     if (wide_counts && (!counts || any || cfg.speculative))
         return fail(RB_ERR_INVALID, "RB_WIDE_COUNTS: visit counts of a closest-hit, non-speculative trace only");
     if (wide_counts && !bvh->wnodes)
-        return fail(RB_ERR_UNSUPPORTED, "RB_WIDE_COUNTS: this BVH has no wide records (too deep for the stack)");
+        return fail(RB_ERR_UNSUPPORTED, "RB_WIDE_COUNTS: this BVH has no wide records (the upload kept the binary walk)");
     const bool wide = wide_counts || (!counts && wide_walk(cfg, bvh, flags));
     TraceKernel k = wide ? (counts ? pick_wide<false, true>(cfg) : any ? pick_wide<true, false>(cfg)
                                                                        : pick_wide<false, false>(cfg))
