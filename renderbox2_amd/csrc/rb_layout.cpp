@@ -263,9 +263,10 @@ rb_status export_mi355x(const HostBvh& H, Mi355Layout& L) {
 // Child boxes are stored as 8-bit steps from the record's origin p (the
 // smallest corner of its children) in units of s = 2^e per axis; the
 // traversal decodes a plane as fmaf(q, s, p), rounded once.  Each lo step is
-// the largest q whose decoded plane is <= the box's lo and each hi step the
-// smallest q whose plane is >= its hi, checked with that same fmaf, so the
-// decoded box holds the exact one.  The slab distances fmaf(plane, 1/d, -o/d)
+// the floor of (lo - p) / s, lowered while its decoded plane lies above lo,
+// and each hi step the ceiling of (hi - p) / s, raised while its plane lies
+// below hi, both checked with that same fmaf, so the decoded box holds the
+// exact one.  The slab distances fmaf(plane, 1/d, -o/d)
 // are monotone in the plane, so a ray passes a decoded box whenever it
 // passes the exact one: a ray tests every leaf the binary walk would at the
 // same hitT, and the closest hit (ties to the lowest triangle id) is the
