@@ -194,7 +194,7 @@ void dfree(void* p, size_t bytes);
 // Device-resident BVH + optional shading data.
 struct rb_bvh_s {
     float4* nodes = nullptr;    // 4 float4 per inner node (MI355X layout)
-    float4* wnodes = nullptr;   // 8 float4 per wide (4-child) record (export_wide)
+    float4* wnodes = nullptr;   // 4 float4 per wide (4-child) record (export_wide)
     uint32_t n_wnodes = 0;
     float4* tris = nullptr;     // 3 float4 Woop rows per triangle reference
     uint32_t* index = nullptr;  // original triangle id per reference
