@@ -341,7 +341,7 @@ rb_status rb_trace_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t f
  * reference: it is for callers that cannot allocate (code under HIP graph
  * capture, serving loops, hosts that forbid hipMalloc).  The (id, t) staging
  * and the trace's work counter live in a static device array of the library
- * (8 MiB + 256 bytes per device, held until the process ends), so at most
+ * (8 MiB + 1,280 bytes per device, held until the process ends), so at most
  * 1,048,576 rays per call (more: RB_ERR_INVALID).  rb_device_init must have
  * run (RB_ERR_INVALID otherwise).  Every call on a device shares that
  * array: calls that overlap on one device (different streams or threads)

@@ -9,15 +9,26 @@
 
 namespace rb {
 
-// Work-counter slots: each launch of a persistent kernel takes one slot
-// (work counter + exit counter); the last block out resets both to zero, so
-// no memset node is needed between launches or inside captured graphs.  A
+// Work-counter slots: each launch of a persistent kernel takes one slot, a
+// block of kCounterWords words: [1] the exit counter, [kBandStride * (b + 1)]
+// the work counter of ray band b (b < kRayBands), each on a 128-byte line of
+// its own (with all eight on one line, claims from different XCDs contend
+// for it: 1.43 against 0.97 ms on a 2 Mi-ray launch); the rest unused.  The
+// last block out resets them to zero, so no memset node is needed between
+// launches or inside captured graphs.  A
 // slot is reused kCounterSlots launches later; when that launch is on
 // another stream it first waits (stream-ordered, no host wait) for the
 // event recorded after the slot's previous launch, so two launches never
 // share a counter whatever the streams and the number in flight.
 constexpr uint32_t kCounterSlots = 4096;
-// Words after the counter pairs: [0] traversal error flags (RB_TRACE_ERR_*).
+// A launch's rays are split into this many contiguous bands, one per XCD:
+// the waves of an XCD claim rays from the band of its number first, so the
+// rays that share an XCD's L2 start near each other (contiguous rays are
+// neighbouring pixels' rays), then from the other bands in turn.
+constexpr uint32_t kRayBands = 8;
+constexpr uint32_t kBandStride = 32;  // words: one 128-byte line per counter
+constexpr uint32_t kCounterWords = kBandStride * (kRayBands + 1);
+// Words after the counter blocks: [0] traversal error flags (RB_TRACE_ERR_*).
 constexpr uint32_t kErrorWords = 64;
 
 // Traversal stack bound: the builder stops at depth 64 and leaves of more than
@@ -30,8 +41,8 @@ struct DeviceState {
     uint32_t num_cus = 0;
     size_t lds_per_block = 65536;  // LDS one workgroup may use (static + dynamic)
     hipStream_t stream = nullptr;  // library default stream
-    uint32_t* counters = nullptr;  // 2 * kCounterSlots, then kErrorWords
-    uint32_t* errors = nullptr;    // counters + 2 * kCounterSlots
+    uint32_t* counters = nullptr;  // kCounterWords * kCounterSlots, then kErrorWords
+    uint32_t* errors = nullptr;    // counters + kCounterWords * kCounterSlots
     std::atomic<uint32_t> next_slot{0};
     std::atomic<size_t> library_bytes{0};
     rb_trace_config cfg{};
@@ -42,7 +53,7 @@ rb_status ensure_device();
 // scratch array for `device` (rb_raygen.hip); run once per device by the
 // init, with that device current.
 rb_status noalloc_setup(int device);
-// A launch's work-counter pair: from the ring, or from the pool of the render
+// A launch's work-counter block: from the ring, or from the pool of the render
 // graph being captured on this thread (counter == nullptr when that pool is
 // used up).  take_counter_slot makes `stream` wait for the slot's previous
 // launch when that was on another stream; release_counter_slot, called once
@@ -58,7 +69,7 @@ void release_counter_slot(const SlotLease& l, hipStream_t stream);
 
 // Set on the capturing thread while a render pass is recorded into a graph
 // (rb_render_graph_create): kernels read their stream keys from `keys`
-// (indexed by tag), trace launches take counter pairs from the graph's own
+// (indexed by tag), trace launches take counter blocks from the graph's own
 // pool, and scratch comes from persistent allocations the graph frees.
 struct CaptureCtx {
     const Rng* keys = nullptr;
@@ -111,7 +122,7 @@ constexpr uint32_t kOcclusion = 1u << 16;
 // Persistent traversal launch (rb_trace.hip).  n_dev, when set, holds the ray
 // count in device memory (queues sized by earlier kernels); n is then an
 // upper bound that only sizes the grid.  counter, when set, is the launch's
-// zeroed work-counter pair (two words the kernel leaves zero again) instead
+// zeroed work-counter block (kCounterWords words the kernel leaves zero again) instead
 // of a slot of the ring.
 rb_status launch_trace(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
                        rb_hit* hits, uint32_t* counts, hipStream_t stream, const rb_trace_config& cfg,

@@ -30,11 +30,11 @@ static rb_status init_locked(int device) {
     if ((e = hipStreamCreateWithFlags(&g_dev.stream, hipStreamNonBlocking)) != hipSuccess)
         return hip_fail(e, "hipStreamCreate");
     void* c = nullptr;
-    const size_t cbytes = sizeof(uint32_t) * (2 * kCounterSlots + kErrorWords);
+    const size_t cbytes = sizeof(uint32_t) * ((size_t)kCounterWords * kCounterSlots + kErrorWords);
     if ((e = hipMalloc(&c, cbytes)) != hipSuccess) return hip_fail(e, "hipMalloc(counters)");
     if ((e = hipMemset(c, 0, cbytes)) != hipSuccess) return hip_fail(e, "hipMemset(counters)");
     g_dev.counters = (uint32_t*)c;
-    g_dev.errors = g_dev.counters + 2 * kCounterSlots;
+    g_dev.errors = g_dev.counters + (size_t)kCounterWords * kCounterSlots;
     // the allocation-free entry point's scratch address; if the lookup
     // fails, that entry point reports it and everything else still works
     (void)noalloc_setup(device);
@@ -83,7 +83,8 @@ static std::mutex g_slot_mu[64];
 SlotLease take_counter_slot(hipStream_t stream) {
     SlotLease l;
     if (t_capture) {
-        if (t_capture->next_pair < t_capture->n_pairs) l.counter = t_capture->counters + 2 * t_capture->next_pair++;
+        if (t_capture->next_pair < t_capture->n_pairs)
+            l.counter = t_capture->counters + (size_t)kCounterWords * t_capture->next_pair++;
         return l;
     }
     const uint32_t s = g_dev.next_slot.fetch_add(1) % kCounterSlots;
@@ -105,7 +106,7 @@ SlotLease take_counter_slot(hipStream_t stream) {
             return l;
         }
     }
-    l.counter = g_dev.counters + 2 * s;
+    l.counter = g_dev.counters + (size_t)kCounterWords * s;
     l.slot = (int32_t)s;
     return l;  // the slot's lock is held until release_counter_slot
 }

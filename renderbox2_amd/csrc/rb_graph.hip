@@ -8,7 +8,7 @@
 // with a one-block kernel that writes the stream keys of the new seed into a
 // device table, and every kernel of the pass reads its key from that table.
 // The graph owns its scratch (allocated once, at capture) and its own
-// work-counter pairs, which the trace kernels reset as they finish, so a
+// work-counter blocks, which the trace kernels reset as they finish, so a
 // replay needs no memset node.  One graph replays on one stream at a time.
 #include "rb_device.h"
 
@@ -64,8 +64,8 @@ rb_status rb_render_graph_create(uint32_t kind, rb_bvh_t bvh, const rb_camera* c
     const uint32_t pairs = 2 * (p->max_depth + 1) + p->light_max_depth + 8;
     hipError_t e;
     if ((e = hipMalloc((void**)&G->keys, kKeyTags * sizeof(Rng))) != hipSuccess ||
-        (e = hipMalloc((void**)&G->counters, pairs * 2 * sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMemset(G->counters, 0, pairs * 2 * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMalloc((void**)&G->counters, pairs * kCounterWords * sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMemset(G->counters, 0, pairs * kCounterWords * sizeof(uint32_t))) != hipSuccess ||
         (e = hipMemset(G->keys, 0, kKeyTags * sizeof(Rng))) != hipSuccess) {
         graph_free(G);
         return hip_fail(e, "render graph setup");

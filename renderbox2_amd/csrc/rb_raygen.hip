@@ -123,13 +123,14 @@ static uint32_t grid_for(uint64_t n, uint32_t block) {
 // ------------------------------------------ allocation-free trace + fill --
 // rb_trace_fill_noalloc keeps what rb_trace_fill would allocate -- the
 // (id, t) staging between the trace and the fill, and the trace's
-// work-counter pair -- in this file-scope device array: bytes
-// [0, kScratchHead) hold the counter pair, then kNoAllocRays rb_hit
+// work-counter block -- in this file-scope device array: bytes
+// [0, kScratchHead) hold the counter block, then kNoAllocRays rb_hit
 // records.  Each device holds its own copy for the life of the process.  In
 // host code `scratch` names a host-side stand-in; kernels get its device
 // address, looked up once per device at init (noalloc_setup).
 constexpr uint32_t kNoAllocRays = 1u << 20;
-constexpr size_t kScratchHead = 256;
+constexpr size_t kScratchHead = 1280;  // the counter block, rounded up to 256 bytes
+static_assert(kScratchHead >= kCounterWords * sizeof(uint32_t), "the scratch head holds a counter block");
 static __device__ __align__(256) unsigned char scratch[kScratchHead + (size_t)kNoAllocRays * sizeof(rb_hit)];
 constexpr int kMaxDevices = 64;
 static void* g_scratch_dev[kMaxDevices];  // device address of `scratch`, per device
@@ -139,7 +140,7 @@ __global__ void k_zero_words(uint32_t* p, uint32_t n) {
 }
 
 // Trace into `hits`, then write the intersection records from them.
-// `counter`: the trace's work-counter pair when the caller brings one (else
+// `counter`: the trace's work-counter block when the caller brings one (else
 // a slot of the library's ring).
 static rb_status launch_fill(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, uint32_t flags, const uint8_t* mask,
                              rb_hit* hits, const rb_isect_soa* out, hipStream_t s, uint32_t* counter) {
@@ -250,9 +251,9 @@ rb_status rb_trace_fill_noalloc(rb_bvh_t bvh, const rb_ray* rays, uint32_t n, ui
     const hipStream_t s = stream ? (hipStream_t)stream : dev().stream;
     uint32_t* counter = static_cast<uint32_t*>(g_scratch_dev[d]);
     rb_hit* hits = reinterpret_cast<rb_hit*>(static_cast<char*>(g_scratch_dev[d]) + kScratchHead);
-    // the counter pair must start at zero (a kernel, not a memset: callers
+    // the counter block must start at zero (a kernel, not a memset: callers
     // may capture this into a graph)
-    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counter, 2u);
+    hipLaunchKernelGGL(k_zero_words, dim3(1), dim3(64), 0, s, counter, kCounterWords);
     if ((e = hipGetLastError()) != hipSuccess) return hip_fail(e, "scratch clear");
     return launch_fill(bvh, rays, n, flags, mask, hits, out, s, counter);
 }
