@@ -13,7 +13,8 @@ namespace rb {
 // block of kCounterWords words: [1] the exit counter, [kBandStride * (b + 1)]
 // the work counter of ray band b (b < kRayBands), each on a 128-byte line of
 // its own (with all eight on one line, claims from different XCDs contend
-// for it: 1.43 against 0.97 ms on a 2 Mi-ray launch); the rest unused.  The
+// for it: 1.43 against 0.97 ms on a 2 Mi-ray launch); [0] the one work
+// counter of a batched launch; the rest unused.  The
 // last block out resets them to zero, so no memset node is needed between
 // launches or inside captured graphs.  A
 // slot is reused kCounterSlots launches later; when that launch is on

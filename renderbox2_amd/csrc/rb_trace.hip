@@ -231,11 +231,15 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
     uint32_t pool_next = 0, pool_end = 0;
     // ray bands: kRayBands contiguous ranges of chunk-multiple length; a
     // wave claims from the band of its XCD until that runs out, then from
-    // the next band in turn (so every ray is claimed exactly once)
-    const uint32_t band_len = ((nrays + kRayBands - 1) / kRayBands + a.chunk - 1) / a.chunk * a.chunk;
-    uint32_t band;
-    asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(band));
-    band &= kRayBands - 1;
+    // the next band in turn (so every ray is claimed exactly once).  A
+    // batched launch claims in order from counter word 0, as before the
+    // bands: its batches come from several scenes
+    const uint32_t band_len = MULTI ? 0u : ((nrays + kRayBands - 1) / kRayBands + a.chunk - 1) / a.chunk * a.chunk;
+    uint32_t band = 0;
+    if (!MULTI) {
+        asm volatile("s_getreg_b32 %0, hwreg(HW_REG_XCC_ID)" : "=s"(band));
+        band &= kRayBands - 1;
+    }
     int nodeAddr = kSentinel, leafAddr = 0, sp = 0, hitIndex = -1;
     uint32_t rayidx = 0;
     float ox = 0.f, oy = 0.f, oz = 0.f, tmin = 0.f, dx = 0.f, dy = 0.f, dz = 0.f, hitT = 0.f, tmax_in = 0.f;
@@ -263,18 +267,22 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
             } else {
                 const int leader = __builtin_ctzll(m);
                 uint32_t base = 0;
-                if ((int)lane == leader) {
-                    base = nrays;  // every band claimed: the lanes finish
-                    for (uint32_t t = 0; t < kRayBands; t++) {
-                        const uint32_t o = atomicAdd(a.counter + kBandStride * (band + 1), a.chunk);
-                        if (o < band_len && band * band_len + o < nrays) {  // the last band ends at nrays
-                            base = band * band_len + o;
-                            break;
+                if (MULTI) {
+                    if ((int)lane == leader) base = atomicAdd(a.counter, a.chunk);
+                } else {
+                    if ((int)lane == leader) {
+                        base = nrays;  // every band claimed: the lanes finish
+                        for (uint32_t t = 0; t < kRayBands; t++) {
+                            const uint32_t o = atomicAdd(a.counter + kBandStride * (band + 1), a.chunk);
+                            if (o < band_len && band * band_len + o < nrays) {  // the last band ends at nrays
+                                base = band * band_len + o;
+                                break;
+                            }
+                            band = (band + 1) & (kRayBands - 1);
                         }
-                        band = (band + 1) & (kRayBands - 1);
                     }
+                    band = __shfl(band, leader);
                 }
-                band = __shfl(band, leader);
                 base = __shfl(base, leader);
                 mine = rank < avail ? pool_next + rank : base + (rank - avail);
                 pool_next = base + (nterm - avail);
@@ -551,7 +559,14 @@ __device__ __forceinline__ void trace_body(const TraceArgs& a, const BatchDesc* 
         __threadfence();
         if (atomicAdd(a.counter + 1, 1u) == gridDim.x - 1) {
             atomicExch(a.counter + 1, 0u);
-            for (uint32_t b = 0; b < kRayBands; b++) atomicExch(a.counter + kBandStride * (b + 1), 0u);
+            if (MULTI) {
+                atomicExch(a.counter + 0, 0u);
+            } else {
+                // unrolled: as a loop it took the batched kernel from 71 to
+                // 81 VGPRs (7 -> 5 waves per SIMD)
+#pragma unroll
+                for (uint32_t b = 0; b < kRayBands; b++) atomicExch(a.counter + kBandStride * (b + 1), 0u);
+            }
         }
     }
 }
