@@ -272,6 +272,21 @@ def test_wide_walk_edge_scenes_equal_the_oracle(gpu, orc, case):
     assert np.array_equal(ids, oid) and np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
 
 
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 511, 513, 4097])
+def test_ray_band_edges_equal_the_oracle(gpu, orc, small, n):
+    """Launch sizes around the ray bands' edges (a launch's rays are split
+    into eight bands of whole 64-ray chunks, the last one ending at the ray
+    count): every ray is traced once, closest and any hit."""
+    v, idx, b, tr, c2 = small
+    rays = random_rays(n, seed=200 + n)
+    ids, ts = tr.trace(rays)
+    oid, ot = orc.trace_compact2(*c2, rays)
+    assert np.array_equal(ids, oid) and np.array_equal(ts.view(np.uint32), ot.view(np.uint32))
+    aid, at = tr.trace(rays, any_hit=True)
+    oaid, oat = orc.trace_compact2(*c2, rays, any_hit=True)
+    assert np.array_equal(aid, oaid) and np.array_equal(at.view(np.uint32), oat.view(np.uint32))
+
+
 def test_trace_is_deterministic_and_reentrant(gpu, small):
     v, idx, b, tr, c2 = small
     rays = random_rays(100000, seed=80)
